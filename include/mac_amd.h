@@ -1,0 +1,159 @@
+/*
+ * mac_amd.h -- C-ABI of the MI355X-native MAC-grid projection library
+ * (libmac_amd.so).  Plain pointers and sizes only; every host array is owned
+ * by the caller and copied in or out, every device buffer is owned by the
+ * context.  One context per host thread; all work of a context is issued on
+ * its own HIP stream and serialises there.  Every entry point returns 0 on
+ * success and a negative MAC_E_* code on failure (the reason is available
+ * from mac_last_error()).
+ *
+ * Grid format (shared by every entry point that takes cells):
+ *   The grid is a set of 8x8x8 bricks ("chunks" in the reference).  Brick b
+ *   has integer chunk key keys3[3b+0..2]; its cells are stored x-fastest,
+ *   cell (x,y,z) at offset x + 8y + 64z (reference Offset.h:14-24), and a
+ *   cell with global integer key k lives in chunk floor(k/8)
+ *   (ChunkHashTable.tpp:181-196).  Per cell one flag byte:
+ *     bits 0..2  material, MaterialToInt numbering
+ *                (Data_CellType_Fluid_Air_Solid_Inflow_Interior_0_Pres.tpp:3-21):
+ *                Air=0 Fluid=1 Solid=2 Inflow=3 Interior=4 _0_Pres=5
+ *     bits 3..5  const bit of the x/y/z face on the cell's low side
+ *                (Data_Grid_Speed m_const, Data_Grid_Speed.tpp:1-26)
+ *     bit  7     cell exists (the chunk's existence bitset,
+ *                Data_Chunk_Bool_Array.h:8-28); a cell without it is an
+ *                invalid neighbour (Offset_Base_Neighbour.h:14-33)
+ *   Face velocities u,v,w are per cell, the face on the cell's low side in
+ *   that direction (staggered-left MAC layout, Policy_Divergence.h:20-29).
+ */
+#ifndef MAC_AMD_H
+#define MAC_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MAC_OK 0
+#define MAC_E_HIP -1      /* a HIP runtime call failed                        */
+#define MAC_E_ARG -2      /* bad argument / inconsistent sizes                */
+#define MAC_E_NOCONV -3   /* the solver hit max_iter before rtol              */
+#define MAC_E_STATE -4    /* call order violated (e.g. project before grid)   */
+#define MAC_E_NODEV -5    /* no GPU visible                                   */
+
+#define MAC_FLAG_MAT_MASK 0x07u
+#define MAC_FLAG_CONST_X 0x08u
+#define MAC_FLAG_CONST_Y 0x10u
+#define MAC_FLAG_CONST_Z 0x20u
+#define MAC_FLAG_EXISTS 0x80u
+
+enum mac_material { MAC_AIR = 0, MAC_FLUID = 1, MAC_SOLID = 2, MAC_INFLOW = 3, MAC_INTERIOR = 4, MAC_0_PRES = 5 };
+
+enum mac_precision {
+  MAC_FP64 = 0,  /* FP64 V-cycle inside FP64 CG                              */
+  MAC_MIXED = 1  /* FP32 V-cycle (smooth/restrict/prolong) inside FP64 CG    */
+};
+
+typedef struct mac_ctx mac_ctx;
+
+typedef struct {
+  double h[3];     /* cell spacing per direction; 1/h is formed as 1.0/h like
+                      Inversible_Value (Inversible_Value_Vect.h)                  */
+  double rtol;     /* stop when ||f - L p||_2 <= rtol * ||f||_2                    */
+  int max_iter;    /* PCG iteration cap                                            */
+  int precision;   /* enum mac_precision                                           */
+  int nu;          /* damped-Jacobi sweeps before and after each coarse correction */
+  double omega;    /* Jacobi damping                                               */
+  int check_every; /* host polls the device convergence flag every k iterations   */
+} mac_params;
+
+typedef struct {
+  int iters;             /* PCG iterations of the final (accepted) solve        */
+  int restarts;          /* true-residual restarts taken                         */
+  int fallback_used;     /* 1 when the guided solve stalled and was redone       */
+  int levels;            /* multigrid levels                                     */
+  long long n_unknowns;  /* in-domain cells                                      */
+  long long n_bricks;    /* bricks at level 0 / active bricks at level 0         */
+  long long n_active_bricks;
+  double rel_resid;      /* recursive residual ||r||/||f|| at exit               */
+  double true_rel_resid; /* recomputed ||f - L p|| / ||f||                       */
+  double max_div;        /* max |div u| over in-domain cells after the update    */
+  double ms_total;       /* wall time of the projection (device-resident)        */
+  double ms_solve;       /* PCG part only                                        */
+} mac_stats;
+
+/* Defaults: h=1, rtol=1e-8, max_iter=500, FP64, nu=2, omega=0.8, check_every=4. */
+void mac_default_params(mac_params* prm);
+
+/* Create / destroy a context on the current HIP device. */
+int mac_create(mac_ctx** out, const mac_params* prm);
+void mac_destroy(mac_ctx* ctx);
+int mac_set_params(mac_ctx* ctx, const mac_params* prm);
+const char* mac_last_error(void);
+
+/* Upload a brick grid (keys + flags) and build the device brick hash, the
+ * 27-neighbour brick table, the per-cell stencil codes and the multigrid
+ * hierarchy.  Replaces the reference's per-projection matrix assembly
+ * (Algorithms_Solve_Pressure.h:44-157: the operator is applied matrix-free
+ * from the codes instead of being stored).  u,v,w may be NULL (zeroed). */
+int mac_upload_bricks(mac_ctx* ctx, int nbricks, const int32_t* keys3, const uint8_t* flags,
+                      const double* u, const double* v, const double* w);
+
+/* Host <-> device copies of the cell fields, brick layout, nbricks*512 each. */
+int mac_set_velocity(mac_ctx* ctx, const double* u, const double* v, const double* w);
+int mac_get_velocity(mac_ctx* ctx, double* u, double* v, double* w);
+int mac_get_pressure(mac_ctx* ctx, double* p);
+int mac_set_pressure(mac_ctx* ctx, const double* p);
+/* Device-side snapshot / restore of u,v,w (to re-run a projection on the same input). */
+int mac_save_velocity(mac_ctx* ctx);
+int mac_restore_velocity(mac_ctx* ctx);
+
+/* One free-surface projection on the device-resident fields, the
+ * Divergence_Projection of Algorithms_Solve_Pressure.h:168-221:
+ *   f = div(u) per in-domain cell            (Policy_Divergence.h:20-29)
+ *   solve A p = f, A the 7-point operator of Init_Iteration (air / missing
+ *     neighbours Dirichlet p=0, const faces Neumann), by MG-preconditioned CG
+ *   p := solution in the domain, 0 elsewhere     (:184-197)
+ *   u_i -= p(c)/h_i - p(c-e_i)/h_i on correctable, non-const faces (:198-207,
+ *     Policy_Gradiant.h:32-37, Policy_Pressure_If_Correction.h:33-49)
+ *   max |div u| over the domain                  (:209-219, reported)
+ * guess: 0 = zero start, 1 = start from the pressure currently held on the
+ * device (previous step or mac_set_pressure).  stats may be NULL. */
+int mac_project(mac_ctx* ctx, int guess, mac_stats* stats);
+
+/* Pieces of the projection, for parity tests (all on device-resident data):
+ * div_out[c] = Policy_Divergence::Get_Divergence(c) for in-domain cells, else 0. */
+int mac_divergence(mac_ctx* ctx, double* div_out);
+/* y = A x for the assembled operator (reference sign: diagonal negative),
+ * x,y host arrays in brick layout; entries outside the domain read as 0. */
+int mac_apply_operator(mac_ctx* ctx, const double* x, double* y);
+/* u -= grad p using the device pressure (the write-back + update half of
+ * Divergence_Projection). */
+int mac_gradient_update(mac_ctx* ctx);
+/* max |div u| over in-domain cells of the device velocity. */
+int mac_max_divergence(mac_ctx* ctx, double* out);
+
+/* Slot A of the reference (Policy_Solve_Linear_Umfpack.h:7-71): solve
+ * A x = b for a symmetric CSR/CSC matrix (offset[n+1], indice/value[nnz]),
+ * as assembled by Algorithms_Solve_Pressure::Init_Iteration.  n == 0 is a
+ * no-op; n <= 3 is solved on the host by Gaussian elimination with partial
+ * pivoting (the 3x3 systems of the extrapolation boundary condition); larger
+ * systems by Jacobi-preconditioned CG in FP64 on the device to rtol. */
+int mac_solve_csr(mac_ctx* ctx, int n, const int* offset, const int* indice, const double* value,
+                  const double* b, double* x, double rtol);
+
+/* Kernel timing for roofline figures: when enabled, mac_project records HIP
+ * events around every launch of the named kernel classes on the context's
+ * stream and accumulates their durations. */
+int mac_enable_kernel_timing(mac_ctx* ctx, int on);
+/* Returns the number of kernel classes; fills up to cap entries. */
+int mac_kernel_times(mac_ctx* ctx, int cap, const char** names, double* total_ms, long long* launches,
+                     double* bytes);
+int mac_reset_kernel_times(mac_ctx* ctx);
+
+/* Synchronise the context's stream. */
+int mac_sync(mac_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

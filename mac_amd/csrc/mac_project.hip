@@ -1,0 +1,932 @@
+// mac_project.hip -- context, brick hierarchy and the C-ABI of libmac_amd.so
+// for the free-surface pressure projection (reference:
+// Algorithms_Solve_Pressure.h, Policy_Solve_Linear_Umfpack.h).
+//
+// Host code here only builds the brick structure (keys -> ids, neighbour,
+// child and parent tables: a few tens of thousands of entries) and drives the
+// kernels of project_kernels.h on the context's stream.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <type_traits>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/mac_amd.h"
+#include "project_kernels.h"
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIPCHK(expr)                                                                                   \
+  do {                                                                                                 \
+    hipError_t e_ = (expr);                                                                            \
+    if (e_ != hipSuccess) return fail(MAC_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+#define CHK(expr)             \
+  do {                        \
+    int rc_ = (expr);         \
+    if (rc_ != 0) return rc_; \
+  } while (0)
+
+inline uint64_t pack_key(int x, int y, int z) {
+  return ((uint64_t)(uint32_t)(x + (1 << 20)) << 42) | ((uint64_t)(uint32_t)(y + (1 << 20)) << 21) |
+         (uint64_t)(uint32_t)(z + (1 << 20));
+}
+
+enum KClass { K_DIV, K_PMATVEC, K_UPDATE, K_JACOBI0, K_RESID0, K_PROLONG0, K_RESTRICT1, K_COARSE, K_CSOLVE, K_GRAD, K_MAXDIV, K_INIT, K_N };
+const char* kNames[K_N] = {"divergence",   "cg_pupdate_matvec", "cg_update",  "jacobi_l0",  "residual_l0", "prolong_l0",
+                           "restrict_l1",  "vcycle_coarse",     "coarse_solve", "gradient_update", "max_div", "resid_init"};
+
+struct Level {
+  int nb = 0;
+  std::vector<int> keys;  // 3*nb, shifted so that every key is >= 0
+  std::vector<int> h_nbr, h_child, h_parent, h_oct;
+  int *nbr = nullptr, *child = nullptr, *parent = nullptr, *oct = nullptr, *act = nullptr;
+  int nact = 0;
+  long long ndom = 0;
+  uint8_t *code = nullptr, *lab = nullptr;
+  double ih2[3] = {1, 1, 1};
+  void *r = nullptr, *z = nullptr, *zt = nullptr, *res = nullptr;
+};
+
+}  // namespace
+
+struct mac_ctx {
+  mac_params prm;
+  hipStream_t st = nullptr;
+  int dev = 0;
+  int nb0 = 0;
+  std::vector<int> keys0;
+  std::vector<Level> L;
+  uint8_t* flags = nullptr;
+  double *u = nullptr, *v = nullptr, *w = nullptr, *u0 = nullptr, *v0 = nullptr, *w0 = nullptr;
+  double *x = nullptr, *f = nullptr, *r = nullptr, *d[2] = {nullptr, nullptr}, *q = nullptr, *tmpv = nullptr;
+  int* glist = nullptr;
+  int nglist = 0;
+  // coarsest level exact solve
+  int nc = 0;
+  int *rowof = nullptr, *cellof = nullptr;
+  double* cinv64 = nullptr;
+  float* cinv32 = nullptr;
+  double* partials = nullptr;
+  Scal* s = nullptr;
+  Scal* hs = nullptr;  // pinned
+  bool have_grid = false;
+  // timing
+  bool timing = false;
+  std::vector<hipEvent_t> evpool;
+  size_t evused = 0;
+  struct Pend {
+    int cls;
+    hipEvent_t a, b;
+    double bytes;
+    long long seq;
+  };
+  std::vector<Pend> pend;
+  std::vector<std::pair<long long, long long>> noop;  // seq ranges of launches that exited at once
+  long long seq = 0;
+  double kt_ms[K_N] = {0};
+  long long kt_n[K_N] = {0};
+  double kt_bytes[K_N] = {0};
+  std::vector<void*> allocs;
+};
+
+namespace {
+
+void free_all(mac_ctx* c) {
+  for (void* p : c->allocs) hipFree(p);
+  c->allocs.clear();
+  c->L.clear();
+  c->have_grid = false;
+}
+
+template <typename T>
+int dalloc(mac_ctx* c, T** p, size_t n) {
+  *p = nullptr;
+  if (n == 0) n = 1;
+  HIPCHK(hipMalloc((void**)p, n * sizeof(T)));
+  c->allocs.push_back((void*)*p);
+  return 0;
+}
+
+template <typename T>
+int upload(mac_ctx* c, T** p, const std::vector<T>& h) {
+  CHK(dalloc(c, p, h.size()));
+  if (!h.empty()) HIPCHK(hipMemcpyAsync(*p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, c->st));
+  return 0;
+}
+
+// ---- kernel launch bookkeeping (optional event timing) ----
+struct TimedLaunch {
+  mac_ctx* c;
+  int cls;
+  double bytes;
+  hipEvent_t a = nullptr, b = nullptr;
+  long long myseq;
+  TimedLaunch(mac_ctx* c_, int cls_, double bytes_) : c(c_), cls(cls_), bytes(bytes_), myseq(c_->seq++) {
+    if (!c->timing) return;
+    while (c->evused + 2 > c->evpool.size()) {
+      hipEvent_t e;
+      hipEventCreate(&e);
+      c->evpool.push_back(e);
+    }
+    a = c->evpool[c->evused++];
+    b = c->evpool[c->evused++];
+    hipEventRecord(a, c->st);
+  }
+  ~TimedLaunch() {
+    if (!c->timing) return;
+    hipEventRecord(b, c->st);
+    c->pend.push_back({cls, a, b, bytes, myseq});
+  }
+};
+
+void flush_timing(mac_ctx* c) {
+  if (!c->timing) {
+    c->noop.clear();
+    return;
+  }
+  for (auto& p : c->pend) {
+    bool noop = false;  // launched after convergence: the kernel exited at once
+    for (auto& rg : c->noop) noop |= (p.seq >= rg.first && p.seq <= rg.second);
+    if (noop) continue;
+    float ms = 0.f;
+    hipEventElapsedTime(&ms, p.a, p.b);
+    c->kt_ms[p.cls] += ms;
+    c->kt_n[p.cls] += 1;
+    c->kt_bytes[p.cls] += p.bytes;
+  }
+  c->pend.clear();
+  c->noop.clear();
+  c->evused = 0;
+}
+
+// ---- hierarchy construction ----
+
+void build_nbr(Level& lv) {
+  std::unordered_map<uint64_t, int> map;
+  map.reserve(lv.nb * 2);
+  for (int b = 0; b < lv.nb; ++b) map[pack_key(lv.keys[3 * b], lv.keys[3 * b + 1], lv.keys[3 * b + 2])] = b;
+  lv.h_nbr.assign((size_t)lv.nb * 27, -1);
+  for (int b = 0; b < lv.nb; ++b)
+    for (int oz = -1; oz <= 1; ++oz)
+      for (int oy = -1; oy <= 1; ++oy)
+        for (int ox = -1; ox <= 1; ++ox) {
+          auto it = map.find(pack_key(lv.keys[3 * b] + ox, lv.keys[3 * b + 1] + oy, lv.keys[3 * b + 2] + oz));
+          lv.h_nbr[(size_t)b * 27 + (ox + 1) + 3 * (oy + 1) + 9 * (oz + 1)] = (it == map.end()) ? -1 : it->second;
+        }
+}
+
+// Dense inverse of the coarsest operator (Cholesky; a tiny diagonal shift if
+// a component has no Dirichlet contact, which keeps the preconditioner SPD).
+void dense_inverse(const std::vector<uint8_t>& code, const double ih2[3], std::vector<int>& rowof, std::vector<int>& cellof,
+                   std::vector<double>& inv) {
+  rowof.assign(MAC_BC, -1);
+  cellof.clear();
+  for (int t = 0; t < MAC_BC; ++t)
+    if (code[t] & CODE_DOM) {
+      rowof[t] = (int)cellof.size();
+      cellof.push_back(t);
+    }
+  int n = (int)cellof.size();
+  inv.assign((size_t)n * n, 0.0);
+  if (n == 0) return;
+  std::vector<double> A((size_t)n * n, 0.0);
+  const int dx[6] = {-1, 1, 0, 0, 0, 0}, dy[6] = {0, 0, -1, 1, 0, 0}, dz[6] = {0, 0, 0, 0, -1, 1};
+  for (int i = 0; i < n; ++i) {
+    int t = cellof[i];
+    int x = t & 7, y = (t >> 3) & 7, z = t >> 6;
+    for (int k = 0; k < 6; ++k) {
+      if (!(code[t] & (1u << k))) continue;
+      double w = ih2[k >> 1];
+      A[(size_t)i * n + i] += w;
+      int xx = x + dx[k], yy = y + dy[k], zz = z + dz[k];
+      if (xx < 0 || yy < 0 || zz < 0 || xx > 7 || yy > 7 || zz > 7) continue;
+      int j = rowof[xx + 8 * yy + 64 * zz];
+      if (j >= 0) A[(size_t)i * n + j] -= w;
+    }
+  }
+  double maxd = 0;
+  for (int i = 0; i < n; ++i) maxd = std::max(maxd, A[(size_t)i * n + i]);
+  // Cholesky A = G G^T
+  std::vector<double> G = A;
+  for (int attempt = 0; attempt < 2; ++attempt) {
+    bool ok = true;
+    G = A;
+    if (attempt == 1)
+      for (int i = 0; i < n; ++i) G[(size_t)i * n + i] += 1e-10 * maxd;
+    for (int j = 0; j < n && ok; ++j) {
+      double s = G[(size_t)j * n + j];
+      for (int k = 0; k < j; ++k) s -= G[(size_t)j * n + k] * G[(size_t)j * n + k];
+      if (s <= 1e-13 * maxd) {
+        ok = false;
+        break;
+      }
+      double gjj = std::sqrt(s);
+      G[(size_t)j * n + j] = gjj;
+      for (int i = j + 1; i < n; ++i) {
+        double t2 = G[(size_t)i * n + j];
+        for (int k = 0; k < j; ++k) t2 -= G[(size_t)i * n + k] * G[(size_t)j * n + k];
+        G[(size_t)i * n + j] = t2 / gjj;
+      }
+    }
+    if (ok) break;
+  }
+  // inverse column by column: solve G y = e_c, G^T x = y
+  std::vector<double> y(n);
+  for (int col = 0; col < n; ++col) {
+    for (int i = 0; i < n; ++i) {
+      double s = (i == col) ? 1.0 : 0.0;
+      for (int k = 0; k < i; ++k) s -= G[(size_t)i * n + k] * y[k];
+      y[i] = s / G[(size_t)i * n + i];
+    }
+    for (int i = n - 1; i >= 0; --i) {
+      double s = y[i];
+      for (int k = i + 1; k < n; ++k) s -= G[(size_t)k * n + i] * inv[(size_t)k * n + col];
+      inv[(size_t)i * n + col] = s / G[(size_t)i * n + i];
+    }
+  }
+  // symmetrise (rounding)
+  for (int i = 0; i < n; ++i)
+    for (int j = i + 1; j < n; ++j) {
+      double a = 0.5 * (inv[(size_t)i * n + j] + inv[(size_t)j * n + i]);
+      inv[(size_t)i * n + j] = inv[(size_t)j * n + i] = a;
+    }
+}
+
+int build_grid(mac_ctx* c, int nbricks, const int32_t* keys3, const uint8_t* flags) {
+  free_all(c);
+  c->nb0 = nbricks;
+  c->keys0.assign(keys3, keys3 + 3 * (size_t)nbricks);
+  // duplicate check + shift
+  int mn[3] = {1 << 30, 1 << 30, 1 << 30};
+  {
+    std::unordered_map<uint64_t, int> seen;
+    seen.reserve(nbricks * 2);
+    for (int b = 0; b < nbricks; ++b) {
+      for (int i = 0; i < 3; ++i) {
+        if (keys3[3 * b + i] < -(1 << 19) || keys3[3 * b + i] >= (1 << 19)) return fail(MAC_E_ARG, "chunk key out of range");
+        mn[i] = std::min(mn[i], (int)keys3[3 * b + i]);
+      }
+      if (!seen.emplace(pack_key(keys3[3 * b], keys3[3 * b + 1], keys3[3 * b + 2]), b).second)
+        return fail(MAC_E_ARG, "duplicate chunk key");
+    }
+  }
+  const mac_params& P = c->prm;
+  double inv_h[3];
+  for (int i = 0; i < 3; ++i) inv_h[i] = 1.0 / P.h[i];
+
+  c->L.clear();
+  c->L.emplace_back();
+  {
+    Level& l0 = c->L[0];
+    l0.nb = nbricks;
+    l0.keys.resize(3 * (size_t)nbricks);
+    for (int b = 0; b < nbricks; ++b)
+      for (int i = 0; i < 3; ++i) l0.keys[3 * b + i] = keys3[3 * b + i] - mn[i];
+    for (int i = 0; i < 3; ++i) l0.ih2[i] = inv_h[i] * inv_h[i];
+  }
+  // coarse levels until a single brick remains
+  while (c->L.back().nb > 1) {
+    Level& f = c->L.back();
+    Level cl;
+    std::unordered_map<uint64_t, int> map;
+    map.reserve(f.nb);
+    f.h_parent.assign(f.nb, -1);
+    f.h_oct.assign(f.nb, 0);
+    for (int b = 0; b < f.nb; ++b) {
+      int kx = f.keys[3 * b] >> 1, ky = f.keys[3 * b + 1] >> 1, kz = f.keys[3 * b + 2] >> 1;
+      uint64_t k = pack_key(kx, ky, kz);
+      auto it = map.find(k);
+      int id;
+      if (it == map.end()) {
+        id = cl.nb++;
+        map.emplace(k, id);
+        cl.keys.push_back(kx);
+        cl.keys.push_back(ky);
+        cl.keys.push_back(kz);
+        cl.h_child.resize((size_t)cl.nb * 8, -1);
+      } else {
+        id = it->second;
+      }
+      int o = (f.keys[3 * b] & 1) + 2 * (f.keys[3 * b + 1] & 1) + 4 * (f.keys[3 * b + 2] & 1);
+      cl.h_child[(size_t)id * 8 + o] = b;
+      f.h_parent[b] = id;
+      f.h_oct[b] = o;
+    }
+    for (int i = 0; i < 3; ++i) cl.ih2[i] = f.ih2[i] * 0.25;
+    c->L.push_back(std::move(cl));
+  }
+  for (auto& lv : c->L) build_nbr(lv);
+
+  // device structures
+  for (size_t l = 0; l < c->L.size(); ++l) {
+    Level& lv = c->L[l];
+    CHK(upload(c, &lv.nbr, lv.h_nbr));
+    if (!lv.h_child.empty()) CHK(upload(c, &lv.child, lv.h_child));
+    if (!lv.h_parent.empty()) {
+      CHK(upload(c, &lv.parent, lv.h_parent));
+      CHK(upload(c, &lv.oct, lv.h_oct));
+    }
+    CHK(dalloc(c, &lv.code, (size_t)lv.nb * MAC_BC));
+    CHK(dalloc(c, &lv.lab, (size_t)lv.nb * MAC_BC));
+  }
+  size_t n0 = (size_t)nbricks * MAC_BC;
+  CHK(dalloc(c, &c->flags, n0));
+  HIPCHK(hipMemcpyAsync(c->flags, flags, n0, hipMemcpyHostToDevice, c->st));
+  Level& l0 = c->L[0];
+  k_code0<<<nbricks, KTHREADS, 0, c->st>>>(nbricks, l0.nbr, c->flags, l0.code, l0.lab);
+  HIPCHK(hipGetLastError());
+  for (size_t l = 1; l < c->L.size(); ++l) {
+    Level& lv = c->L[l];
+    k_coarsen_lab<<<lv.nb, KTHREADS, 0, c->st>>>(lv.child, c->L[l - 1].lab, lv.lab);
+    k_code_c<<<lv.nb, KTHREADS, 0, c->st>>>(lv.nbr, lv.lab, lv.code);
+  }
+  HIPCHK(hipGetLastError());
+  // active bricks per level
+  int* dcnt;
+  CHK(dalloc(c, &dcnt, (size_t)nbricks));
+  for (size_t l = 0; l < c->L.size(); ++l) {
+    Level& lv = c->L[l];
+    k_brick_count<<<lv.nb, KTHREADS, 0, c->st>>>(lv.nb, lv.code, dcnt);
+    std::vector<int> cnt(lv.nb);
+    HIPCHK(hipMemcpyAsync(cnt.data(), dcnt, lv.nb * sizeof(int), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    std::vector<int> act;
+    lv.ndom = 0;
+    for (int b = 0; b < lv.nb; ++b)
+      if (cnt[b] > 0) {
+        act.push_back(b);
+        lv.ndom += cnt[b];
+      }
+    lv.nact = (int)act.size();
+    CHK(upload(c, &lv.act, act));
+    if (l == 0) {
+      // bricks whose faces the gradient update may touch: active, or with an
+      // active -x/-y/-z neighbour
+      std::vector<int> gl;
+      for (int b = 0; b < lv.nb; ++b) {
+        bool use = cnt[b] > 0;
+        for (int k : {12, 10, 4}) {
+          int nbb = lv.h_nbr[(size_t)b * 27 + k];
+          if (nbb >= 0 && cnt[nbb] > 0) use = true;
+        }
+        if (use) gl.push_back(b);
+      }
+      c->nglist = (int)gl.size();
+      CHK(upload(c, &c->glist, gl));
+    }
+  }
+  // coarsest exact solve
+  {
+    Level& lc = c->L.back();
+    std::vector<uint8_t> code(MAC_BC);
+    HIPCHK(hipMemcpyAsync(code.data(), lc.code, MAC_BC, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    std::vector<int> rowof, cellof;
+    std::vector<double> inv;
+    dense_inverse(code, lc.ih2, rowof, cellof, inv);
+    c->nc = (int)cellof.size();
+    std::vector<float> inv32(inv.begin(), inv.end());
+    CHK(upload(c, &c->rowof, rowof));
+    CHK(upload(c, &c->cellof, cellof));
+    CHK(upload(c, &c->cinv64, inv));
+    CHK(upload(c, &c->cinv32, inv32));
+  }
+  // vectors
+  CHK(dalloc(c, &c->u, n0));
+  CHK(dalloc(c, &c->v, n0));
+  CHK(dalloc(c, &c->w, n0));
+  CHK(dalloc(c, &c->u0, n0));
+  CHK(dalloc(c, &c->v0, n0));
+  CHK(dalloc(c, &c->w0, n0));
+  CHK(dalloc(c, &c->x, n0));
+  CHK(dalloc(c, &c->f, n0));
+  CHK(dalloc(c, &c->r, n0));
+  CHK(dalloc(c, &c->d[0], n0));
+  CHK(dalloc(c, &c->d[1], n0));
+  CHK(dalloc(c, &c->q, n0));
+  CHK(dalloc(c, &c->tmpv, n0));
+  for (double* p : {c->u, c->v, c->w, c->x, c->f, c->r, c->d[0], c->d[1], c->q, c->tmpv})
+    HIPCHK(hipMemsetAsync(p, 0, n0 * sizeof(double), c->st));
+  for (size_t l = 0; l < c->L.size(); ++l) {
+    Level& lv = c->L[l];
+    size_t n = (size_t)lv.nb * MAC_BC;
+    double *a, *b, *cc, *dd;
+    CHK(dalloc(c, &a, n));
+    CHK(dalloc(c, &b, n));
+    CHK(dalloc(c, &cc, n));
+    CHK(dalloc(c, &dd, n));
+    for (double* p : {a, b, cc, dd}) HIPCHK(hipMemsetAsync(p, 0, n * sizeof(double), c->st));
+    lv.r = a;
+    lv.z = b;
+    lv.zt = cc;
+    lv.res = dd;
+  }
+  CHK(dalloc(c, &c->partials, std::max(nbricks, 1024)));
+  CHK(dalloc(c, &c->s, 1));
+  HIPCHK(hipMemsetAsync(c->s, 0, sizeof(Scal), c->st));
+  HIPCHK(hipStreamSynchronize(c->st));
+  c->have_grid = true;
+  return 0;
+}
+
+// ---- the V-cycle ----
+
+inline long long lvl_dom(const mac_ctx* c, int l) { return c->L[l].ndom; }
+
+// Damped-Jacobi sweep at level l (level 0 reads the CG residual in FP64).
+template <typename TZ>
+void sweep(mac_ctx* c, int l, const TZ* zin, TZ* zout, int want_dot) {
+  Level& lv = c->L[l];
+  const TZ ih[3] = {(TZ)lv.ih2[0], (TZ)lv.ih2[1], (TZ)lv.ih2[2]};
+  const TZ om = (TZ)c->prm.omega;
+  const double sz = sizeof(TZ), nd = (double)lv.ndom;
+  const double sr = (l == 0) ? 8.0 : sz;
+  TimedLaunch tl(c, l == 0 ? K_JACOBI0 : K_COARSE, nd * (sr + (zin ? 2 : 1) * sz + 1));
+  if (l == 0)
+    k_jacobi<double, TZ><<<lv.nact, KTHREADS, 0, c->st>>>(lv.act, lv.nbr, lv.code, c->r, zin, zout, ih[0], ih[1], ih[2], om,
+                                                          want_dot, c->partials, c->s);
+  else
+    k_jacobi<TZ, TZ><<<lv.nact, KTHREADS, 0, c->st>>>(lv.act, lv.nbr, lv.code, (const TZ*)lv.r, zin, zout, ih[0], ih[1], ih[2],
+                                                      om, want_dot, c->partials, c->s);
+}
+
+// z = V(r) at level 0 (r = the CG residual).  Levels below the first level
+// without unknowns are not visited: the bottom visited level then just
+// smooths (nu sweeps down, nu up), which keeps the cycle symmetric.
+template <typename TZ>
+int vcycle(mac_ctx* c, TZ** zout) {
+  const int nl = (int)c->L.size();
+  const int nu = std::max(1, c->prm.nu);
+  const double sz = sizeof(TZ);
+  int nlv = nl;
+  for (int l = 1; l < nl; ++l)
+    if (c->L[l].nact == 0) {
+      nlv = l;
+      break;
+    }
+  const int bot = nlv - 1;
+  std::vector<TZ*> zfin(nl, nullptr);
+  // down
+  for (int l = 0; l < bot; ++l) {
+    Level& lv = c->L[l];
+    TZ ih[3] = {(TZ)lv.ih2[0], (TZ)lv.ih2[1], (TZ)lv.ih2[2]};
+    TZ* z = (TZ*)lv.z;
+    TZ* zt = (TZ*)lv.zt;
+    TZ* res = (TZ*)lv.res;
+    double nd = (double)lvl_dom(c, l);
+    double sr = (l == 0) ? 8.0 : sz;
+    for (int k = 0; k < nu; ++k) {
+      sweep<TZ>(c, l, k ? z : nullptr, k ? zt : z, 0);
+      if (k) std::swap(z, zt);
+    }
+    {
+      TimedLaunch tl(c, l == 0 ? K_RESID0 : K_COARSE, nd * (sr + 2 * sz + 1));
+      if (l == 0)
+        k_residual<double, TZ><<<lv.nact, KTHREADS, 0, c->st>>>(lv.act, lv.nbr, lv.code, c->r, z, res, ih[0], ih[1], ih[2], c->s);
+      else
+        k_residual<TZ, TZ><<<lv.nact, KTHREADS, 0, c->st>>>(lv.act, lv.nbr, lv.code, (const TZ*)lv.r, z, res, ih[0], ih[1], ih[2],
+                                                            c->s);
+    }
+    Level& cl = c->L[l + 1];
+    {
+      TimedLaunch tl(c, l == 0 ? K_RESTRICT1 : K_COARSE, nd * sz + (double)cl.ndom * sz);
+      k_restrict<TZ><<<cl.nact, KTHREADS, 0, c->st>>>(cl.act, cl.nbr, cl.child, cl.code, res, (TZ*)cl.r, c->s);
+    }
+    zfin[l] = z;  // pre-smoothed iterate, finished on the way up
+    lv.z = z;
+    lv.zt = zt;
+  }
+  // bottom
+  {
+    Level& lb = c->L[bot];
+    const int want_dot = (bot == 0) ? 1 : 0;
+    if (bot == nl - 1) {
+      TimedLaunch tl(c, K_CSOLVE, 0.0);
+      const TZ* cinv = std::is_same<TZ, float>::value ? (const TZ*)c->cinv32 : (const TZ*)c->cinv64;
+      if (bot == 0)
+        k_coarse_solve<double, TZ><<<(MAC_BC + 63) / 64, 64, 0, c->st>>>(c->nc, c->rowof, c->cellof, cinv, c->r, (TZ*)lb.z,
+                                                                          want_dot, c->partials, c->s);
+      else
+        k_coarse_solve<TZ, TZ><<<(MAC_BC + 63) / 64, 64, 0, c->st>>>(c->nc, c->rowof, c->cellof, cinv, (const TZ*)lb.r, (TZ*)lb.z,
+                                                                      want_dot, c->partials, c->s);
+      zfin[bot] = (TZ*)lb.z;
+    } else {
+      TZ* z = (TZ*)lb.z;
+      TZ* zt = (TZ*)lb.zt;
+      for (int k = 0; k < 2 * nu; ++k) {
+        sweep<TZ>(c, bot, k ? z : nullptr, k ? zt : z, (k == 2 * nu - 1) ? want_dot : 0);
+        if (k) std::swap(z, zt);
+      }
+      zfin[bot] = z;
+      lb.z = z;
+      lb.zt = zt;
+    }
+  }
+  // up
+  for (int l = bot - 1; l >= 0; --l) {
+    Level& lv = c->L[l];
+    Level& cl = c->L[l + 1];
+    TZ* z = zfin[l];
+    TZ* zt = (z == (TZ*)lv.z) ? (TZ*)lv.zt : (TZ*)lv.z;
+    double nd = (double)lvl_dom(c, l);
+    {
+      TimedLaunch tl(c, l == 0 ? K_PROLONG0 : K_COARSE, nd * (2 * sz + 1) + (double)cl.ndom * sz);
+      k_prolong<TZ><<<lv.nact, KTHREADS, 0, c->st>>>(lv.act, lv.parent, lv.oct, cl.nbr, lv.code, zfin[l + 1], z, c->s);
+    }
+    for (int k = 0; k < nu; ++k) {
+      sweep<TZ>(c, l, z, zt, (l == 0 && k == nu - 1) ? 1 : 0);
+      std::swap(z, zt);
+    }
+    zfin[l] = z;
+    lv.z = z;
+    lv.zt = zt;
+  }
+  *zout = zfin[0];
+  return 0;
+}
+
+int read_scal(mac_ctx* c) {
+  HIPCHK(hipMemcpyAsync(c->hs, c->s, sizeof(Scal), hipMemcpyDeviceToHost, c->st));
+  HIPCHK(hipStreamSynchronize(c->st));
+  return 0;
+}
+
+template <typename TZ>
+int pcg(mac_ctx* c, bool use_guess, mac_stats* st) {
+  Level& l0 = c->L[0];
+  const double* ih = l0.ih2;
+  size_t n0 = (size_t)c->nb0 * MAC_BC;
+  if (!use_guess) HIPCHK(hipMemsetAsync(c->x, 0, n0 * sizeof(double), c->st));
+  const int check = std::max(1, c->prm.check_every);
+  const double nd = (double)l0.ndom;
+  int restarts = 0;
+  for (;;) {
+    HIPCHK(hipMemsetAsync(c->d[0], 0, n0 * sizeof(double), c->st));
+    int cur = 0;
+    // (re)start: r = f - L x
+    {
+      TimedLaunch tl(c, K_INIT, nd * 25);
+      k_resid0<<<l0.nact, KTHREADS, 0, c->st>>>(l0.act, l0.nbr, l0.code, c->f, c->x, c->r, ih[0], ih[1], ih[2], c->partials, c->s, 0);
+    }
+    CHK(read_scal(c));
+    TZ* z = nullptr;
+    if (!c->hs->done) CHK(vcycle<TZ>(c, &z));
+    int iter_before = c->hs->iter;
+    while (!c->hs->done) {
+      // A batch of `check` iterations; once the update kernel of one of them
+      // sets `done`, every later kernel of the batch returns at once.
+      std::vector<long long> useq;
+      for (int k = 0; k < check; ++k) {
+        {
+          TimedLaunch tl(c, K_PMATVEC, nd * (25 + sizeof(TZ)));
+          k_pmatvec<TZ><<<l0.nact, KTHREADS, 0, c->st>>>(l0.act, l0.nbr, l0.code, z, c->d[cur], c->d[cur ^ 1], c->q, ih[0], ih[1], ih[2],
+                                                         c->partials, c->s);
+        }
+        cur ^= 1;
+        {
+          TimedLaunch tl(c, K_UPDATE, nd * 49);
+          k_update<<<l0.nact, KTHREADS, 0, c->st>>>(l0.act, l0.code, c->x, c->r, c->d[cur], c->q, c->partials, c->s);
+          useq.push_back(tl.myseq);
+        }
+        CHK(vcycle<TZ>(c, &z));
+      }
+      HIPCHK(hipGetLastError());
+      CHK(read_scal(c));
+      if (c->hs->done) {
+        int eff = c->hs->iter - iter_before;
+        long long last = eff > 0 ? useq[eff - 1] : useq[0];
+        c->noop.push_back({last + 1, c->seq - 1});
+      }
+      iter_before = c->hs->iter;
+    }
+    int done = c->hs->done;
+    // true residual
+    {
+      TimedLaunch tl(c, K_INIT, nd * 17);
+      k_resid0<<<l0.nact, KTHREADS, 0, c->st>>>(l0.act, l0.nbr, l0.code, c->f, c->x, c->r, ih[0], ih[1], ih[2], c->partials, c->s, 1);
+    }
+    CHK(read_scal(c));
+    double ff = c->hs->ff;
+    double tr = ff > 0 ? std::sqrt(c->hs->tmp / ff) : 0.0;
+    if (st) {
+      st->iters = c->hs->iter;
+      st->rel_resid = ff > 0 ? std::sqrt(c->hs->rr / ff) : 0.0;
+      st->true_rel_resid = tr;
+    }
+    if (done == 1 && tr > c->prm.rtol * 1.0000001 && restarts < 3 && c->hs->iter < c->prm.max_iter) {
+      // recursive residual drifted from the true one: restart from x
+      ++restarts;
+      c->hs->done = 0;
+      HIPCHK(hipMemcpyAsync(&c->s->done, &c->hs->done, sizeof(int), hipMemcpyHostToDevice, c->st));
+      continue;
+    }
+    if (st) st->restarts = restarts;
+    if (done == 2) return fail(MAC_E_NOCONV, "PCG reached max_iter before rtol");
+    if (done == 3) return fail(MAC_E_NOCONV, "PCG breakdown (d.q <= 0)");
+    return 0;
+  }
+}
+
+int ensure(mac_ctx* c) {
+  if (!c) return fail(MAC_E_ARG, "null context");
+  if (!c->have_grid) return fail(MAC_E_STATE, "no grid uploaded");
+  return 0;
+}
+
+}  // namespace
+
+// ============================================================== C-ABI ====
+
+extern "C" {
+
+__attribute__((visibility("hidden"))) int mac_csr_set_error(const char* msg) { return fail(MAC_E_ARG, msg); }
+
+const char* mac_last_error(void) { return g_err.c_str(); }
+
+void mac_default_params(mac_params* p) {
+  p->h[0] = p->h[1] = p->h[2] = 1.0;
+  p->rtol = 1e-8;
+  p->max_iter = 500;
+  p->precision = MAC_FP64;
+  p->nu = 2;
+  p->omega = 0.8;
+  p->check_every = 4;
+}
+
+int mac_create(mac_ctx** out, const mac_params* prm) {
+  if (!out) return fail(MAC_E_ARG, "null out");
+  *out = nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(MAC_E_NODEV, "no HIP device visible");
+  mac_ctx* c = new mac_ctx();
+  if (prm) c->prm = *prm;
+  else mac_default_params(&c->prm);
+  hipGetDevice(&c->dev);
+  if (hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking) != hipSuccess) {
+    delete c;
+    return fail(MAC_E_HIP, "hipStreamCreate failed");
+  }
+  if (hipHostMalloc((void**)&c->hs, sizeof(Scal), hipHostMallocDefault) != hipSuccess) {
+    hipStreamDestroy(c->st);
+    delete c;
+    return fail(MAC_E_HIP, "hipHostMalloc failed");
+  }
+  memset(c->hs, 0, sizeof(Scal));
+  *out = c;
+  return 0;
+}
+
+void mac_destroy(mac_ctx* c) {
+  if (!c) return;
+  hipStreamSynchronize(c->st);
+  free_all(c);
+  for (auto e : c->evpool) hipEventDestroy(e);
+  if (c->hs) hipHostFree(c->hs);
+  hipStreamDestroy(c->st);
+  delete c;
+}
+
+int mac_set_params(mac_ctx* c, const mac_params* prm) {
+  if (!c || !prm) return fail(MAC_E_ARG, "null argument");
+  bool hchange = c->have_grid && (prm->h[0] != c->prm.h[0] || prm->h[1] != c->prm.h[1] || prm->h[2] != c->prm.h[2]);
+  if (hchange) return fail(MAC_E_STATE, "h cannot change after the grid is uploaded");
+  if (prm->nu < 1 || prm->max_iter < 1 || !(prm->rtol > 0)) return fail(MAC_E_ARG, "bad solver parameters");
+  c->prm = *prm;
+  return 0;
+}
+
+int mac_upload_bricks(mac_ctx* c, int nbricks, const int32_t* keys3, const uint8_t* flags, const double* u, const double* v,
+                      const double* w) {
+  if (!c || nbricks <= 0 || !keys3 || !flags) return fail(MAC_E_ARG, "bad upload arguments");
+  for (int i = 0; i < 3; ++i)
+    if (!(c->prm.h[i] > 0)) return fail(MAC_E_ARG, "h must be positive");
+  CHK(build_grid(c, nbricks, keys3, flags));
+  if (u && v && w) CHK(mac_set_velocity(c, u, v, w));
+  return 0;
+}
+
+int mac_set_velocity(mac_ctx* c, const double* u, const double* v, const double* w) {
+  CHK(ensure(c));
+  size_t n = (size_t)c->nb0 * MAC_BC * sizeof(double);
+  HIPCHK(hipMemcpyAsync(c->u, u, n, hipMemcpyHostToDevice, c->st));
+  HIPCHK(hipMemcpyAsync(c->v, v, n, hipMemcpyHostToDevice, c->st));
+  HIPCHK(hipMemcpyAsync(c->w, w, n, hipMemcpyHostToDevice, c->st));
+  HIPCHK(hipStreamSynchronize(c->st));
+  return 0;
+}
+
+int mac_get_velocity(mac_ctx* c, double* u, double* v, double* w) {
+  CHK(ensure(c));
+  size_t n = (size_t)c->nb0 * MAC_BC * sizeof(double);
+  HIPCHK(hipMemcpyAsync(u, c->u, n, hipMemcpyDeviceToHost, c->st));
+  HIPCHK(hipMemcpyAsync(v, c->v, n, hipMemcpyDeviceToHost, c->st));
+  HIPCHK(hipMemcpyAsync(w, c->w, n, hipMemcpyDeviceToHost, c->st));
+  HIPCHK(hipStreamSynchronize(c->st));
+  return 0;
+}
+
+int mac_get_pressure(mac_ctx* c, double* p) {
+  CHK(ensure(c));
+  HIPCHK(hipMemcpyAsync(p, c->x, (size_t)c->nb0 * MAC_BC * sizeof(double), hipMemcpyDeviceToHost, c->st));
+  HIPCHK(hipStreamSynchronize(c->st));
+  return 0;
+}
+
+int mac_set_pressure(mac_ctx* c, const double* p) {
+  CHK(ensure(c));
+  HIPCHK(hipMemcpyAsync(c->x, p, (size_t)c->nb0 * MAC_BC * sizeof(double), hipMemcpyHostToDevice, c->st));
+  k_mask<<<c->nb0, KTHREADS, 0, c->st>>>(c->nb0, c->L[0].code, c->x);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(c->st));
+  return 0;
+}
+
+int mac_save_velocity(mac_ctx* c) {
+  CHK(ensure(c));
+  size_t n = (size_t)c->nb0 * MAC_BC * sizeof(double);
+  HIPCHK(hipMemcpyAsync(c->u0, c->u, n, hipMemcpyDeviceToDevice, c->st));
+  HIPCHK(hipMemcpyAsync(c->v0, c->v, n, hipMemcpyDeviceToDevice, c->st));
+  HIPCHK(hipMemcpyAsync(c->w0, c->w, n, hipMemcpyDeviceToDevice, c->st));
+  HIPCHK(hipStreamSynchronize(c->st));
+  return 0;
+}
+
+int mac_restore_velocity(mac_ctx* c) {
+  CHK(ensure(c));
+  size_t n = (size_t)c->nb0 * MAC_BC * sizeof(double);
+  HIPCHK(hipMemcpyAsync(c->u, c->u0, n, hipMemcpyDeviceToDevice, c->st));
+  HIPCHK(hipMemcpyAsync(c->v, c->v0, n, hipMemcpyDeviceToDevice, c->st));
+  HIPCHK(hipMemcpyAsync(c->w, c->w0, n, hipMemcpyDeviceToDevice, c->st));
+  HIPCHK(hipStreamSynchronize(c->st));
+  return 0;
+}
+
+int mac_project(mac_ctx* c, int guess, mac_stats* st) {
+  CHK(ensure(c));
+  auto t0 = std::chrono::steady_clock::now();
+  Level& l0 = c->L[0];
+  double inv_h[3];
+  for (int i = 0; i < 3; ++i) inv_h[i] = 1.0 / c->prm.h[i];
+  if (st) {
+    memset(st, 0, sizeof(*st));
+    st->levels = (int)c->L.size();
+    st->n_unknowns = l0.ndom;
+    st->n_bricks = c->nb0;
+    st->n_active_bricks = l0.nact;
+  }
+  Scal init;
+  memset(&init, 0, sizeof(init));
+  init.max_iter = c->prm.max_iter;
+  HIPCHK(hipMemcpyAsync(c->s, &init, sizeof(Scal), hipMemcpyHostToDevice, c->st));
+  c->seq = 0;
+  const double nd = (double)l0.ndom;
+  if (l0.nact > 0) {
+    TimedLaunch tl(c, K_DIV, nd * 33);
+    k_div<<<l0.nact, KTHREADS, 0, c->st>>>(l0.act, l0.nbr, l0.code, c->u, c->v, c->w, inv_h[0], inv_h[1], inv_h[2], c->f, nullptr,
+                                           c->partials, c->s);
+  }
+  HIPCHK(hipGetLastError());
+  CHK(read_scal(c));
+  // tolerance on the device
+  c->hs->tol2 = c->prm.rtol * c->prm.rtol * c->hs->ff;
+  HIPCHK(hipMemcpyAsync(&c->s->tol2, &c->hs->tol2, sizeof(double), hipMemcpyHostToDevice, c->st));
+  auto ts0 = std::chrono::steady_clock::now();
+  int rc = 0;
+  if (l0.nact > 0 && c->hs->ff > 0) {
+    if (c->prm.precision == MAC_MIXED) rc = pcg<float>(c, guess != 0, st);
+    else rc = pcg<double>(c, guess != 0, st);
+  } else {
+    HIPCHK(hipMemsetAsync(c->x, 0, (size_t)c->nb0 * MAC_BC * sizeof(double), c->st));
+  }
+  auto ts1 = std::chrono::steady_clock::now();
+  if (rc != 0 && rc != MAC_E_NOCONV) return rc;
+  if (c->nglist > 0) {
+    TimedLaunch tl(c, K_GRAD, nd * 57);
+    k_gradient<<<c->nglist, KTHREADS, 0, c->st>>>(c->glist, l0.nbr, c->flags, l0.code, c->x, c->u, c->v, c->w, inv_h[0], inv_h[1],
+                                                  inv_h[2]);
+  }
+  if (l0.nact > 0) {
+    TimedLaunch tl(c, K_MAXDIV, nd * 25);
+    k_maxdiv<<<l0.nact, KTHREADS, 0, c->st>>>(l0.act, l0.nbr, l0.code, c->u, c->v, c->w, inv_h[0], inv_h[1], inv_h[2], c->partials,
+                                              c->s);
+  }
+  HIPCHK(hipGetLastError());
+  CHK(read_scal(c));
+  auto t1 = std::chrono::steady_clock::now();
+  flush_timing(c);
+  if (st) {
+    st->max_div = c->hs->maxdiv;
+    st->ms_total = std::chrono::duration<double, std::milli>(t1 - t0).count();
+    st->ms_solve = std::chrono::duration<double, std::milli>(ts1 - ts0).count();
+  }
+  return rc;
+}
+
+int mac_divergence(mac_ctx* c, double* div_out) {
+  CHK(ensure(c));
+  Level& l0 = c->L[0];
+  double inv_h[3];
+  for (int i = 0; i < 3; ++i) inv_h[i] = 1.0 / c->prm.h[i];
+  size_t n0 = (size_t)c->nb0 * MAC_BC;
+  HIPCHK(hipMemsetAsync(c->tmpv, 0, n0 * sizeof(double), c->st));
+  HIPCHK(hipMemsetAsync(c->s, 0, sizeof(Scal), c->st));
+  if (l0.nact > 0)
+    k_div<<<l0.nact, KTHREADS, 0, c->st>>>(l0.act, l0.nbr, l0.code, c->u, c->v, c->w, inv_h[0], inv_h[1], inv_h[2], c->f, c->tmpv,
+                                           c->partials, c->s);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(div_out, c->tmpv, n0 * sizeof(double), hipMemcpyDeviceToHost, c->st));
+  HIPCHK(hipStreamSynchronize(c->st));
+  return 0;
+}
+
+int mac_apply_operator(mac_ctx* c, const double* xh, double* yh) {
+  CHK(ensure(c));
+  Level& l0 = c->L[0];
+  size_t n0 = (size_t)c->nb0 * MAC_BC;
+  HIPCHK(hipMemcpyAsync(c->q, xh, n0 * sizeof(double), hipMemcpyHostToDevice, c->st));
+  k_mask<<<c->nb0, KTHREADS, 0, c->st>>>(c->nb0, l0.code, c->q);
+  HIPCHK(hipMemsetAsync(c->tmpv, 0, n0 * sizeof(double), c->st));
+  if (l0.nact > 0) k_apply_A<<<l0.nact, KTHREADS, 0, c->st>>>(l0.act, l0.nbr, l0.code, c->q, c->tmpv, l0.ih2[0], l0.ih2[1], l0.ih2[2]);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(yh, c->tmpv, n0 * sizeof(double), hipMemcpyDeviceToHost, c->st));
+  HIPCHK(hipMemsetAsync(c->q, 0, n0 * sizeof(double), c->st));
+  HIPCHK(hipStreamSynchronize(c->st));
+  return 0;
+}
+
+int mac_gradient_update(mac_ctx* c) {
+  CHK(ensure(c));
+  Level& l0 = c->L[0];
+  double inv_h[3];
+  for (int i = 0; i < 3; ++i) inv_h[i] = 1.0 / c->prm.h[i];
+  if (c->nglist > 0)
+    k_gradient<<<c->nglist, KTHREADS, 0, c->st>>>(c->glist, l0.nbr, c->flags, l0.code, c->x, c->u, c->v, c->w, inv_h[0], inv_h[1],
+                                                  inv_h[2]);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(c->st));
+  return 0;
+}
+
+int mac_max_divergence(mac_ctx* c, double* out) {
+  CHK(ensure(c));
+  Level& l0 = c->L[0];
+  double inv_h[3];
+  for (int i = 0; i < 3; ++i) inv_h[i] = 1.0 / c->prm.h[i];
+  HIPCHK(hipMemsetAsync(c->s, 0, sizeof(Scal), c->st));
+  if (l0.nact > 0)
+    k_maxdiv<<<l0.nact, KTHREADS, 0, c->st>>>(l0.act, l0.nbr, l0.code, c->u, c->v, c->w, inv_h[0], inv_h[1], inv_h[2], c->partials, c->s);
+  HIPCHK(hipGetLastError());
+  CHK(read_scal(c));
+  *out = c->hs->maxdiv;
+  return 0;
+}
+
+int mac_enable_kernel_timing(mac_ctx* c, int on) {
+  if (!c) return fail(MAC_E_ARG, "null context");
+  c->timing = on != 0;
+  return 0;
+}
+
+int mac_kernel_times(mac_ctx* c, int cap, const char** names, double* total_ms, long long* launches, double* bytes) {
+  if (!c) return fail(MAC_E_ARG, "null context");
+  for (int i = 0; i < K_N && i < cap; ++i) {
+    if (names) names[i] = kNames[i];
+    if (total_ms) total_ms[i] = c->kt_ms[i];
+    if (launches) launches[i] = c->kt_n[i];
+    if (bytes) bytes[i] = c->kt_bytes[i];
+  }
+  return K_N;
+}
+
+int mac_reset_kernel_times(mac_ctx* c) {
+  if (!c) return fail(MAC_E_ARG, "null context");
+  for (int i = 0; i < K_N; ++i) {
+    c->kt_ms[i] = 0;
+    c->kt_n[i] = 0;
+    c->kt_bytes[i] = 0;
+  }
+  return 0;
+}
+
+int mac_sync(mac_ctx* c) {
+  if (!c) return fail(MAC_E_ARG, "null context");
+  HIPCHK(hipStreamSynchronize(c->st));
+  return 0;
+}
+
+}  // extern "C"

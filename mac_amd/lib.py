@@ -1,0 +1,276 @@
+"""ctypes binding of libmac_amd.so (declarations in include/mac_amd.h).
+
+The Python names mirror the reference's projection slots:
+``Projection.Init_Iteration`` uploads the brick grid (the matrix-free
+counterpart of Algorithms_Solve_Pressure::Init_Iteration), and
+``Projection.Divergence_Projection`` runs the solve and the velocity update.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libmac_amd.so")
+
+MAC_FP64 = 0
+MAC_MIXED = 1
+
+AIR, FLUID, SOLID, INFLOW, INTERIOR, ZERO_PRES = 0, 1, 2, 3, 4, 5
+CONST_X, CONST_Y, CONST_Z, EXISTS = 0x08, 0x10, 0x20, 0x80
+
+
+class MacError(RuntimeError):
+    pass
+
+
+class Params(ctypes.Structure):
+    _fields_ = [
+        ("h", ctypes.c_double * 3),
+        ("rtol", ctypes.c_double),
+        ("max_iter", ctypes.c_int),
+        ("precision", ctypes.c_int),
+        ("nu", ctypes.c_int),
+        ("omega", ctypes.c_double),
+        ("check_every", ctypes.c_int),
+    ]
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [
+        ("iters", ctypes.c_int),
+        ("restarts", ctypes.c_int),
+        ("fallback_used", ctypes.c_int),
+        ("levels", ctypes.c_int),
+        ("n_unknowns", ctypes.c_longlong),
+        ("n_bricks", ctypes.c_longlong),
+        ("n_active_bricks", ctypes.c_longlong),
+        ("rel_resid", ctypes.c_double),
+        ("true_rel_resid", ctypes.c_double),
+        ("max_div", ctypes.c_double),
+        ("ms_total", ctypes.c_double),
+        ("ms_solve", ctypes.c_double),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+_P = ctypes.c_void_p
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+
+# name -> (restype, argtypes); the exported C-ABI, kept in step with include/mac_amd.h
+SIGNATURES = {
+    "mac_default_params": (None, [ctypes.POINTER(Params)]),
+    "mac_create": (ctypes.c_int, [ctypes.POINTER(_P), ctypes.POINTER(Params)]),
+    "mac_destroy": (None, [_P]),
+    "mac_set_params": (ctypes.c_int, [_P, ctypes.POINTER(Params)]),
+    "mac_last_error": (ctypes.c_char_p, []),
+    "mac_upload_bricks": (ctypes.c_int, [_P, ctypes.c_int, _P, _P, _P, _P, _P]),
+    "mac_set_velocity": (ctypes.c_int, [_P, _P, _P, _P]),
+    "mac_get_velocity": (ctypes.c_int, [_P, _P, _P, _P]),
+    "mac_get_pressure": (ctypes.c_int, [_P, _P]),
+    "mac_set_pressure": (ctypes.c_int, [_P, _P]),
+    "mac_save_velocity": (ctypes.c_int, [_P]),
+    "mac_restore_velocity": (ctypes.c_int, [_P]),
+    "mac_project": (ctypes.c_int, [_P, ctypes.c_int, ctypes.POINTER(Stats)]),
+    "mac_divergence": (ctypes.c_int, [_P, _P]),
+    "mac_apply_operator": (ctypes.c_int, [_P, _P, _P]),
+    "mac_gradient_update": (ctypes.c_int, [_P]),
+    "mac_max_divergence": (ctypes.c_int, [_P, _dp]),
+    "mac_solve_csr": (ctypes.c_int, [_P, ctypes.c_int, _P, _P, _P, _P, _P, ctypes.c_double]),
+    "mac_enable_kernel_timing": (ctypes.c_int, [_P, ctypes.c_int]),
+    "mac_kernel_times": (ctypes.c_int, [_P, ctypes.c_int, ctypes.POINTER(ctypes.c_char_p), _dp,
+                                         ctypes.POINTER(ctypes.c_longlong), _dp]),
+    "mac_reset_kernel_times": (ctypes.c_int, [_P]),
+    "mac_sync": (ctypes.c_int, [_P]),
+}
+
+_lib = None
+
+
+def load():
+    """Load libmac_amd.so; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise MacError(f"{LIB_PATH} is missing: build it with `python -m mac_amd.build`")
+    lib = ctypes.CDLL(LIB_PATH)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def _ptr(a):
+    return a.ctypes.data_as(ctypes.c_void_p) if a is not None else None
+
+
+def _check(rc):
+    if rc != 0:
+        msg = load().mac_last_error().decode(errors="replace")
+        raise MacError(f"mac_amd error {rc}: {msg}")
+
+
+def default_params(**kw):
+    p = Params()
+    load().mac_default_params(ctypes.byref(p))
+    for k, v in kw.items():
+        if k == "h":
+            if np.isscalar(v):
+                v = (v, v, v)
+            for i in range(3):
+                p.h[i] = float(v[i])
+        else:
+            setattr(p, k, v)
+    return p
+
+
+class Projection:
+    """A projection context on the current GPU (one per host thread)."""
+
+    def __init__(self, **params):
+        self.lib = load()
+        self.ctx = ctypes.c_void_p()
+        self.params = default_params(**params)
+        _check(self.lib.mac_create(ctypes.byref(self.ctx), ctypes.byref(self.params)))
+        self.nb = 0
+
+    def close(self):
+        if self.ctx:
+            self.lib.mac_destroy(self.ctx)
+            self.ctx = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_params(self, **kw):
+        for k, v in kw.items():
+            if k == "h":
+                raise MacError("h is fixed once the context exists")
+            setattr(self.params, k, v)
+        _check(self.lib.mac_set_params(self.ctx, ctypes.byref(self.params)))
+
+    # -- grid / fields -------------------------------------------------------
+    def Init_Iteration(self, keys, flags, u=None, v=None, w=None):
+        keys = np.ascontiguousarray(keys, dtype=np.int32).reshape(-1, 3)
+        flags = np.ascontiguousarray(flags, dtype=np.uint8).reshape(-1)
+        nb = keys.shape[0]
+        if flags.size != nb * 512:
+            raise MacError("flags must hold 512 bytes per brick")
+        self.nb = nb
+        uu = [None if a is None else np.ascontiguousarray(a, dtype=np.float64).reshape(-1) for a in (u, v, w)]
+        _check(self.lib.mac_upload_bricks(self.ctx, nb, _ptr(keys), _ptr(flags), *[_ptr(a) for a in uu]))
+
+    upload = Init_Iteration
+
+    def set_velocity(self, u, v, w):
+        a = [np.ascontiguousarray(x, dtype=np.float64).reshape(-1) for x in (u, v, w)]
+        for x in a:
+            if x.size != self.nb * 512:
+                raise MacError("velocity arrays must hold 512 values per brick")
+        _check(self.lib.mac_set_velocity(self.ctx, *[_ptr(x) for x in a]))
+
+    def get_velocity(self):
+        a = [np.empty(self.nb * 512, dtype=np.float64) for _ in range(3)]
+        _check(self.lib.mac_get_velocity(self.ctx, *[_ptr(x) for x in a]))
+        return a
+
+    def get_pressure(self):
+        p = np.empty(self.nb * 512, dtype=np.float64)
+        _check(self.lib.mac_get_pressure(self.ctx, _ptr(p)))
+        return p
+
+    def set_pressure(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float64).reshape(-1)
+        _check(self.lib.mac_set_pressure(self.ctx, _ptr(p)))
+
+    def save_velocity(self):
+        _check(self.lib.mac_save_velocity(self.ctx))
+
+    def restore_velocity(self):
+        _check(self.lib.mac_restore_velocity(self.ctx))
+
+    # -- projection -----------------------------------------------------------
+    def Divergence_Projection(self, guess=False, raise_on_noconv=True):
+        st = Stats()
+        rc = self.lib.mac_project(self.ctx, 1 if guess else 0, ctypes.byref(st))
+        if rc != 0 and (raise_on_noconv or rc != -3):
+            _check(rc)
+        return st.as_dict()
+
+    project = Divergence_Projection
+
+    def divergence(self):
+        out = np.empty(self.nb * 512, dtype=np.float64)
+        _check(self.lib.mac_divergence(self.ctx, _ptr(out)))
+        return out
+
+    def apply_operator(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float64).reshape(-1)
+        y = np.zeros(self.nb * 512, dtype=np.float64)
+        _check(self.lib.mac_apply_operator(self.ctx, _ptr(x), _ptr(y)))
+        return y
+
+    def gradient_update(self):
+        _check(self.lib.mac_gradient_update(self.ctx))
+
+    def max_divergence(self):
+        out = ctypes.c_double()
+        _check(self.lib.mac_max_divergence(self.ctx, ctypes.byref(out)))
+        return out.value
+
+    # -- kernel timing --------------------------------------------------------
+    def enable_timing(self, on=True):
+        _check(self.lib.mac_enable_kernel_timing(self.ctx, 1 if on else 0))
+
+    def reset_timing(self):
+        _check(self.lib.mac_reset_kernel_times(self.ctx))
+
+    def kernel_times(self):
+        cap = 32
+        names = (ctypes.c_char_p * cap)()
+        ms = (ctypes.c_double * cap)()
+        n = (ctypes.c_longlong * cap)()
+        by = (ctypes.c_double * cap)()
+        k = self.lib.mac_kernel_times(self.ctx, cap, names, ms, n, by)
+        return {names[i].decode(): {"ms": ms[i], "launches": n[i], "bytes": by[i]} for i in range(k)}
+
+    def sync(self):
+        _check(self.lib.mac_sync(self.ctx))
+
+
+class LinearSolver:
+    """Slot A: Policy_Solve_Linear_Umfpack's three calls on the GPU
+    (Policy_Solve_Linear_Umfpack.h:28-70).  The CSR arrays are the caller's,
+    as in the reference; FactorizeMatrice only records them."""
+
+    def __init__(self, rtol=1e-13):
+        self.proj = Projection()
+        self.rtol = rtol
+        self.mat = None
+
+    def Solve_Linear_FactorizeMatrice(self, n, offset, indice, value):
+        self.mat = (int(n), np.ascontiguousarray(offset, dtype=np.int32),
+                    np.ascontiguousarray(indice, dtype=np.int32), np.ascontiguousarray(value, dtype=np.float64))
+
+    def Solve_Linear(self, n, offset, indice, value, b, res=None):
+        n = int(n)
+        off = np.ascontiguousarray(offset, dtype=np.int32)
+        idx = np.ascontiguousarray(indice, dtype=np.int32)
+        val = np.ascontiguousarray(value, dtype=np.float64)
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        out = np.zeros(n, dtype=np.float64) if res is None else res
+        _check(self.proj.lib.mac_solve_csr(self.proj.ctx, n, _ptr(off), _ptr(idx), _ptr(val), _ptr(b), _ptr(out),
+                                           self.rtol))
+        return out
+
+    def Solve_Linear_Clean(self):
+        self.mat = None

@@ -1,0 +1,188 @@
+"""GPU parity of the projection (HIP library through the C-ABI) against the
+CPU oracle on identical seeded inputs.
+
+Bars: divergence and gradient update bit-exact (same operation order, no
+FMA contraction on either side); operator within 1e-14 relative; pressure
+and face velocities within 1e-8 relative of the oracle's direct-solve
+stand-in (solves at rtol 1e-12); mixed precision within 1e-6 relative at
+rtol 1e-10."""
+import numpy as np
+import pytest
+
+from mac_amd import workloads as W
+from mac_amd.lib import MAC_FP64, MAC_MIXED, Projection
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+GRIDS = {
+    "rand_2x2x2": lambda: W.random_grid(1),
+    "rand_3x2x2_neg_origin": lambda: W.random_grid(2, 3, 2, 2, origin=(-2, -1, 5)),
+    "rand_dropped_bricks": lambda: W.random_grid(3, 3, 3, 2, drop_bricks=4),
+    "rand_dense_fluid": lambda: W.random_grid(4, 2, 3, 2, fluid_p=0.9, const_p=0.02),
+    "single_brick": lambda: W.random_grid(5, 1, 1, 1),
+    "line_1d_kat": W.line_1d_kat,
+}
+
+
+def dom_mask(g):
+    f = g["flags"]
+    return ((f & 0x80) > 0) & np.isin(f & 7, [1, 3, 4])
+
+
+def rel(a, b):
+    nb = np.linalg.norm(b)
+    return np.linalg.norm(a - b) / (nb if nb > 0 else 1.0)
+
+
+def make(g, **kw):
+    pr = Projection(h=g["h"], **kw)
+    pr.Init_Iteration(g["keys"], g["flags"], g["u"], g["v"], g["w"])
+    return pr
+
+
+@pytest.mark.parametrize("name", sorted(GRIDS))
+def test_divergence_bitexact(name):
+    g = GRIDS[name]()
+    pr = make(g)
+    dg = pr.divergence()
+    do = O.divergence(g)
+    assert np.array_equal(dg, do), f"max diff {np.abs(dg - do).max()}"
+
+
+@pytest.mark.parametrize("name", sorted(GRIDS))
+def test_operator(name):
+    g = GRIDS[name]()
+    pr = make(g)
+    x = W.normal(7, 3, np.arange(g["flags"].size, dtype=np.uint64)) * dom_mask(g)
+    yg = pr.apply_operator(x)
+    yo = O.apply_A(g, x)
+    assert rel(yg, yo) < 1e-14
+
+
+@pytest.mark.parametrize("name", sorted(GRIDS))
+def test_gradient_bitexact(name):
+    g = GRIDS[name]()
+    pr = make(g)
+    p = W.normal(9, 4, np.arange(g["flags"].size, dtype=np.uint64)) * dom_mask(g)
+    pr.set_pressure(p)
+    pr.gradient_update()
+    ug, vg, wg = pr.get_velocity()
+    uo, vo, wo = O.gradient_update(g, p, g["u"], g["v"], g["w"])
+    assert np.array_equal(ug, uo) and np.array_equal(vg, vo) and np.array_equal(wg, wo)
+
+
+@pytest.mark.parametrize("name", sorted(GRIDS))
+def test_project_fp64_vs_oracle(name):
+    g = GRIDS[name]()
+    ref = O.project(g, rtol=1e-14)
+    pr = make(g, rtol=1e-12, precision=MAC_FP64)
+    st = pr.Divergence_Projection()
+    p = pr.get_pressure()
+    u, v, w = pr.get_velocity()
+    assert st["true_rel_resid"] <= 1e-12 * 1.0001
+    assert rel(p, ref["p"]) < 1e-8, (rel(p, ref["p"]), st)
+    for a, b in ((u, ref["u"]), (v, ref["v"]), (w, ref["w"])):
+        assert rel(a, b) < 1e-8
+    # max |div u| after the update equals the oracle's to the solve tolerance
+    assert abs(st["max_div"] - ref["max_div"]) <= 1e-8 * max(1.0, ref["max_div"]) + 1e-9
+
+
+def test_line_1d_known_answer():
+    """test/old/TestAlgorithms_Solve_Pressure.h:56-200: u(0)=u(1)=1 after projection."""
+    g = W.line_1d_kat()
+    pr = make(g, rtol=1e-12)
+    pr.Divergence_Projection()
+    u, _, _ = pr.get_velocity()
+    p = pr.get_pressure()
+    for gx in (0, 1):
+        b, t = g["cells"][gx]
+        assert abs(u[b * 512 + t] - 1.0) < 1e-12
+        assert abs(p[b * 512 + t] - 1.0) < 1e-12
+    b, t = g["cells"][2]
+    assert abs(u[b * 512 + t] - 1.0) < 1e-12
+
+
+def test_uniform_flow_gives_zero_pressure():
+    """test/old/TestSolvePressureUmfpack.h:271-336: a divergence-free uniform
+    flow through the domain leaves p = 0 and u unchanged."""
+    keys = W.box_keys(2, 2, 2)
+    n = 16
+    dense = np.full((n, n, n), W.FLUID | W.EXISTS, dtype=np.uint8)
+    dense[[0, -1], :, :] = W.AIR | W.EXISTS
+    dense[:, [0, -1], :] = W.AIR | W.EXISTS
+    dense[:, :, [0, -1]] = W.AIR | W.EXISTS
+    g = {"keys": keys, "flags": W.dense_to_bricks(dense, 2, 2, 2).reshape(-1), "h": (0.5, 0.25, 1.0)}
+    n = g["flags"].size
+    g["u"], g["v"], g["w"] = np.full(n, 0.3), np.full(n, -0.2), np.full(n, 1.5)
+    pr = make(g, rtol=1e-12)
+    st = pr.Divergence_Projection()
+    assert st["iters"] == 0
+    assert np.all(pr.get_pressure() == 0.0)
+    u, v, w = pr.get_velocity()
+    assert np.all(u == 0.3) and np.all(v == -0.2) and np.all(w == 1.5)
+
+
+def small_spheres(n, seed):
+    return W.sphere_box(n, seed, nspheres=6, rmin=3.0, rmax=10.0, slab=6)
+
+
+@pytest.mark.parametrize("seed", [0, 1])
+def test_sphere_box_48_fp64_vs_cpu_mg(seed):
+    g = small_spheres(48, seed)
+    cpu = O.mgpcg(g, rtol=1e-10)
+    ref = O.project(g, rtol=1e-13)
+    pr = make(g, rtol=1e-10, precision=MAC_FP64)
+    st = pr.Divergence_Projection()
+    p = pr.get_pressure()
+    # same algorithm on the CPU: same iteration count up to rounding
+    assert abs(st["iters"] - cpu["iters"]) <= 1, (st["iters"], cpu["iters"])
+    assert st["levels"] == cpu["levels"]
+    assert rel(p, ref["p"]) < 1e-7
+    assert rel(p, cpu["p"]) < 1e-7
+
+
+@pytest.mark.parametrize("seed", [0, 1])
+def test_sphere_box_48_mixed(seed):
+    g = small_spheres(48, seed)
+    ref = O.project(g, rtol=1e-13)
+    pr = make(g, rtol=1e-10, precision=MAC_MIXED)
+    st = pr.Divergence_Projection()
+    p = pr.get_pressure()
+    assert st["true_rel_resid"] <= 1e-10 * 1.0001
+    assert rel(p, ref["p"]) < 1e-6, rel(p, ref["p"])
+
+
+def test_warm_start_previous_pressure():
+    g = small_spheres(32, 3)
+    pr = make(g, rtol=1e-10)
+    pr.save_velocity()
+    st0 = pr.Divergence_Projection()
+    pr.restore_velocity()
+    st1 = pr.Divergence_Projection(guess=True)  # exact previous answer as guess
+    assert st1["iters"] < st0["iters"]
+    assert st1["true_rel_resid"] <= 1e-10 * 1.0001
+
+
+def test_solve_csr_slot_a():
+    """Slot A: mac_solve_csr on the reference-assembled CSR vs the oracle's solve."""
+    from mac_amd.lib import LinearSolver
+    g = W.random_grid(6)
+    n, off, idx, val, row = O.assemble(g)
+    div = O.divergence(g)
+    b = np.zeros(n)
+    b[row[row >= 0]] = div[row >= 0]
+    xo, _ = O.cg_csr(n, off, idx, val, b, rtol=1e-14)
+    ls = LinearSolver(rtol=1e-13)
+    ls.Solve_Linear_FactorizeMatrice(n, off, idx, val)
+    xg = ls.Solve_Linear(n, off, idx, val, b)
+    ls.Solve_Linear_Clean()
+    assert rel(xg, xo) < 1e-9
+    # n <= 3 host path (extrapolation corner systems)
+    off3 = np.array([0, 2, 5, 7], dtype=np.int32)
+    idx3 = np.array([0, 1, 0, 1, 2, 1, 2], dtype=np.int32)
+    val3 = np.array([-2.0, 1.0, 1.0, -3.0, 1.0, 1.0, -2.0])
+    b3 = np.array([1.0, 2.0, 3.0])
+    x3 = ls.Solve_Linear(3, off3, idx3, val3, b3)
+    A = np.array([[-2, 1, 0], [1, -3, 1], [0, 1, -2.0]])
+    assert np.allclose(A @ x3, b3, rtol=0, atol=1e-14)

@@ -46,9 +46,15 @@ inline uint64_t pack_key(int x, int y, int z) {
          (uint64_t)(uint32_t)(z + (1 << 20));
 }
 
-enum KClass { K_DIV, K_PMATVEC, K_UPDATE, K_JACOBI0, K_RESID0, K_PROLONG0, K_RESTRICT1, K_COARSE, K_CSOLVE, K_GRAD, K_MAXDIV, K_INIT, K_N };
+enum KClass { K_DIV, K_PMATVEC, K_UPDATE, K_JACOBI0, K_RESID0, K_PROLONG0, K_RESTRICT1, K_COARSE, K_CSOLVE, K_GRAD, K_MAXDIV, K_INIT, K_FIN, K_N };
 const char* kNames[K_N] = {"divergence",   "cg_pupdate_matvec", "cg_update",  "jacobi_l0",  "residual_l0", "prolong_l0",
-                           "restrict_l1",  "vcycle_coarse",     "coarse_solve", "gradient_update", "max_div", "resid_init"};
+                           "restrict_l1",  "vcycle_coarse",     "coarse_solve", "gradient_update", "max_div", "resid_init",
+                           "finalize"};
+
+inline int grid_of(int nact) {
+  int g = (nact + WPB - 1) / WPB;
+  return g < 1 ? 1 : (g > GRID_CAP ? GRID_CAP : g);
+}
 
 struct Level {
   int nb = 0;
@@ -153,6 +159,14 @@ struct TimedLaunch {
     c->pend.push_back({cls, a, b, bytes, myseq});
   }
 };
+
+// One-block reduction of the partials of the previous kernel plus the CG
+// scalar step (see k_finalize); returns the launch's sequence number.
+long long finalize(mac_ctx* c, int nparts, int op) {
+  TimedLaunch tl(c, K_FIN, 0.0);
+  k_finalize<<<1, 1024, 0, c->st>>>(nparts, c->partials, c->s, op, c->prm.rtol * c->prm.rtol);
+  return tl.myseq;
+}
 
 void flush_timing(mac_ctx* c) {
   if (!c->timing) {
@@ -456,13 +470,16 @@ void sweep(mac_ctx* c, int l, const TZ* zin, TZ* zout, int want_dot) {
   const TZ om = (TZ)c->prm.omega;
   const double sz = sizeof(TZ), nd = (double)lv.ndom;
   const double sr = (l == 0) ? 8.0 : sz;
+  {
   TimedLaunch tl(c, l == 0 ? K_JACOBI0 : K_COARSE, nd * (sr + (zin ? 2 : 1) * sz + 1));
   if (l == 0)
-    k_jacobi<double, TZ><<<lv.nact, KTHREADS, 0, c->st>>>(lv.act, lv.nbr, lv.code, c->r, zin, zout, ih[0], ih[1], ih[2], om,
-                                                          want_dot, c->partials, c->s);
+    k_jacobi<double, TZ><<<grid_of(lv.nact), KTHREADS, 0, c->st>>>(lv.nact, lv.act, lv.nbr, lv.code, c->r, zin, zout, ih[0], ih[1],
+                                                                   ih[2], om, want_dot, c->partials, c->s);
   else
-    k_jacobi<TZ, TZ><<<lv.nact, KTHREADS, 0, c->st>>>(lv.act, lv.nbr, lv.code, (const TZ*)lv.r, zin, zout, ih[0], ih[1], ih[2],
-                                                      om, want_dot, c->partials, c->s);
+    k_jacobi<TZ, TZ><<<grid_of(lv.nact), KTHREADS, 0, c->st>>>(lv.nact, lv.act, lv.nbr, lv.code, (const TZ*)lv.r, zin, zout, ih[0],
+                                                               ih[1], ih[2], om, want_dot, c->partials, c->s);
+  }
+  if (want_dot) finalize(c, grid_of(lv.nact), OP_RZ);
 }
 
 // z = V(r) at level 0 (r = the CG residual).  Levels below the first level
@@ -497,10 +514,11 @@ int vcycle(mac_ctx* c, TZ** zout) {
     {
       TimedLaunch tl(c, l == 0 ? K_RESID0 : K_COARSE, nd * (sr + 2 * sz + 1));
       if (l == 0)
-        k_residual<double, TZ><<<lv.nact, KTHREADS, 0, c->st>>>(lv.act, lv.nbr, lv.code, c->r, z, res, ih[0], ih[1], ih[2], c->s);
+        k_residual<double, TZ><<<grid_of(lv.nact), KTHREADS, 0, c->st>>>(lv.nact, lv.act, lv.nbr, lv.code, c->r, z, res, ih[0], ih[1],
+                                                                         ih[2], c->s);
       else
-        k_residual<TZ, TZ><<<lv.nact, KTHREADS, 0, c->st>>>(lv.act, lv.nbr, lv.code, (const TZ*)lv.r, z, res, ih[0], ih[1], ih[2],
-                                                            c->s);
+        k_residual<TZ, TZ><<<grid_of(lv.nact), KTHREADS, 0, c->st>>>(lv.nact, lv.act, lv.nbr, lv.code, (const TZ*)lv.r, z, res, ih[0],
+                                                                     ih[1], ih[2], c->s);
     }
     Level& cl = c->L[l + 1];
     {
@@ -516,14 +534,16 @@ int vcycle(mac_ctx* c, TZ** zout) {
     Level& lb = c->L[bot];
     const int want_dot = (bot == 0) ? 1 : 0;
     if (bot == nl - 1) {
-      TimedLaunch tl(c, K_CSOLVE, 0.0);
       const TZ* cinv = std::is_same<TZ, float>::value ? (const TZ*)c->cinv32 : (const TZ*)c->cinv64;
+      TimedLaunch* tl = new TimedLaunch(c, K_CSOLVE, 0.0);
       if (bot == 0)
         k_coarse_solve<double, TZ><<<(MAC_BC + 63) / 64, 64, 0, c->st>>>(c->nc, c->rowof, c->cellof, cinv, c->r, (TZ*)lb.z,
                                                                           want_dot, c->partials, c->s);
       else
         k_coarse_solve<TZ, TZ><<<(MAC_BC + 63) / 64, 64, 0, c->st>>>(c->nc, c->rowof, c->cellof, cinv, (const TZ*)lb.r, (TZ*)lb.z,
                                                                       want_dot, c->partials, c->s);
+      delete tl;
+      if (want_dot) finalize(c, (MAC_BC + 63) / 64, OP_RZ);
       zfin[bot] = (TZ*)lb.z;
     } else {
       TZ* z = (TZ*)lb.z;
@@ -581,8 +601,10 @@ int pcg(mac_ctx* c, bool use_guess, mac_stats* st) {
     // (re)start: r = f - L x
     {
       TimedLaunch tl(c, K_INIT, nd * 25);
-      k_resid0<<<l0.nact, KTHREADS, 0, c->st>>>(l0.act, l0.nbr, l0.code, c->f, c->x, c->r, ih[0], ih[1], ih[2], c->partials, c->s, 0);
+      k_resid0<<<grid_of(l0.nact), KTHREADS, 0, c->st>>>(l0.nact, l0.act, l0.nbr, l0.code, c->f, c->x, c->r, ih[0], ih[1], ih[2], 1,
+                                                         c->partials);
     }
+    finalize(c, grid_of(l0.nact), OP_RR0);
     CHK(read_scal(c));
     TZ* z = nullptr;
     if (!c->hs->done) CHK(vcycle<TZ>(c, &z));
@@ -594,15 +616,16 @@ int pcg(mac_ctx* c, bool use_guess, mac_stats* st) {
       for (int k = 0; k < check; ++k) {
         {
           TimedLaunch tl(c, K_PMATVEC, nd * (25 + sizeof(TZ)));
-          k_pmatvec<TZ><<<l0.nact, KTHREADS, 0, c->st>>>(l0.act, l0.nbr, l0.code, z, c->d[cur], c->d[cur ^ 1], c->q, ih[0], ih[1], ih[2],
-                                                         c->partials, c->s);
+          k_pmatvec<TZ><<<grid_of(l0.nact), KTHREADS, 0, c->st>>>(l0.nact, l0.act, l0.nbr, l0.code, z, c->d[cur], c->d[cur ^ 1], c->q,
+                                                                  ih[0], ih[1], ih[2], c->partials, c->s);
         }
+        finalize(c, grid_of(l0.nact), OP_DQ);
         cur ^= 1;
         {
           TimedLaunch tl(c, K_UPDATE, nd * 49);
-          k_update<<<l0.nact, KTHREADS, 0, c->st>>>(l0.act, l0.code, c->x, c->r, c->d[cur], c->q, c->partials, c->s);
-          useq.push_back(tl.myseq);
+          k_update<<<grid_of(l0.nact), KTHREADS, 0, c->st>>>(l0.nact, l0.act, l0.code, c->x, c->r, c->d[cur], c->q, c->partials, c->s);
         }
+        useq.push_back(finalize(c, grid_of(l0.nact), OP_RR));
         CHK(vcycle<TZ>(c, &z));
       }
       HIPCHK(hipGetLastError());
@@ -618,8 +641,10 @@ int pcg(mac_ctx* c, bool use_guess, mac_stats* st) {
     // true residual
     {
       TimedLaunch tl(c, K_INIT, nd * 17);
-      k_resid0<<<l0.nact, KTHREADS, 0, c->st>>>(l0.act, l0.nbr, l0.code, c->f, c->x, c->r, ih[0], ih[1], ih[2], c->partials, c->s, 1);
+      k_resid0<<<grid_of(l0.nact), KTHREADS, 0, c->st>>>(l0.nact, l0.act, l0.nbr, l0.code, c->f, c->x, c->r, ih[0], ih[1], ih[2], 0,
+                                                         c->partials);
     }
+    finalize(c, grid_of(l0.nact), OP_TRUE);
     CHK(read_scal(c));
     double ff = c->hs->ff;
     double tr = ff > 0 ? std::sqrt(c->hs->tmp / ff) : 0.0;
@@ -797,17 +822,14 @@ int mac_project(mac_ctx* c, int guess, mac_stats* st) {
   const double nd = (double)l0.ndom;
   if (l0.nact > 0) {
     TimedLaunch tl(c, K_DIV, nd * 33);
-    k_div<<<l0.nact, KTHREADS, 0, c->st>>>(l0.act, l0.nbr, l0.code, c->u, c->v, c->w, inv_h[0], inv_h[1], inv_h[2], c->f, nullptr,
-                                           c->partials, c->s);
+    k_div<<<grid_of(l0.nact), KTHREADS, 0, c->st>>>(l0.nact, l0.act, l0.nbr, l0.code, c->u, c->v, c->w, inv_h[0], inv_h[1], inv_h[2],
+                                                    c->f, nullptr, c->partials);
   }
+  if (l0.nact > 0) finalize(c, grid_of(l0.nact), OP_FF);
   HIPCHK(hipGetLastError());
-  CHK(read_scal(c));
-  // tolerance on the device
-  c->hs->tol2 = c->prm.rtol * c->prm.rtol * c->hs->ff;
-  HIPCHK(hipMemcpyAsync(&c->s->tol2, &c->hs->tol2, sizeof(double), hipMemcpyHostToDevice, c->st));
   auto ts0 = std::chrono::steady_clock::now();
   int rc = 0;
-  if (l0.nact > 0 && c->hs->ff > 0) {
+  if (l0.nact > 0) {
     if (c->prm.precision == MAC_MIXED) rc = pcg<float>(c, guess != 0, st);
     else rc = pcg<double>(c, guess != 0, st);
   } else {
@@ -822,9 +844,10 @@ int mac_project(mac_ctx* c, int guess, mac_stats* st) {
   }
   if (l0.nact > 0) {
     TimedLaunch tl(c, K_MAXDIV, nd * 25);
-    k_maxdiv<<<l0.nact, KTHREADS, 0, c->st>>>(l0.act, l0.nbr, l0.code, c->u, c->v, c->w, inv_h[0], inv_h[1], inv_h[2], c->partials,
-                                              c->s);
+    k_maxdiv<<<grid_of(l0.nact), KTHREADS, 0, c->st>>>(l0.nact, l0.act, l0.nbr, l0.code, c->u, c->v, c->w, inv_h[0], inv_h[1], inv_h[2],
+                                                       c->partials);
   }
+  if (l0.nact > 0) finalize(c, grid_of(l0.nact), OP_MAX);
   HIPCHK(hipGetLastError());
   CHK(read_scal(c));
   auto t1 = std::chrono::steady_clock::now();
@@ -846,8 +869,8 @@ int mac_divergence(mac_ctx* c, double* div_out) {
   HIPCHK(hipMemsetAsync(c->tmpv, 0, n0 * sizeof(double), c->st));
   HIPCHK(hipMemsetAsync(c->s, 0, sizeof(Scal), c->st));
   if (l0.nact > 0)
-    k_div<<<l0.nact, KTHREADS, 0, c->st>>>(l0.act, l0.nbr, l0.code, c->u, c->v, c->w, inv_h[0], inv_h[1], inv_h[2], c->f, c->tmpv,
-                                           c->partials, c->s);
+    k_div<<<grid_of(l0.nact), KTHREADS, 0, c->st>>>(l0.nact, l0.act, l0.nbr, l0.code, c->u, c->v, c->w, inv_h[0], inv_h[1], inv_h[2],
+                                                    c->f, c->tmpv, c->partials);
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpyAsync(div_out, c->tmpv, n0 * sizeof(double), hipMemcpyDeviceToHost, c->st));
   HIPCHK(hipStreamSynchronize(c->st));
@@ -889,7 +912,11 @@ int mac_max_divergence(mac_ctx* c, double* out) {
   for (int i = 0; i < 3; ++i) inv_h[i] = 1.0 / c->prm.h[i];
   HIPCHK(hipMemsetAsync(c->s, 0, sizeof(Scal), c->st));
   if (l0.nact > 0)
-    k_maxdiv<<<l0.nact, KTHREADS, 0, c->st>>>(l0.act, l0.nbr, l0.code, c->u, c->v, c->w, inv_h[0], inv_h[1], inv_h[2], c->partials, c->s);
+  {
+    k_maxdiv<<<grid_of(l0.nact), KTHREADS, 0, c->st>>>(l0.nact, l0.act, l0.nbr, l0.code, c->u, c->v, c->w, inv_h[0], inv_h[1], inv_h[2],
+                                                       c->partials);
+    finalize(c, grid_of(l0.nact), OP_MAX);
+  }
   HIPCHK(hipGetLastError());
   CHK(read_scal(c));
   *out = c->hs->maxdiv;

@@ -1,12 +1,19 @@
 // project_kernels.h -- HIP kernels of the free-surface pressure projection.
 //
-// Every kernel runs one workgroup per active brick (a brick holding at least
-// one unknown) of its level, 256 threads, two cells per thread.  All vectors
-// are zero outside the domain, so the 7-point operator needs only the cell's
-// own code byte (see brick.h).  Reductions are grid-wide and deterministic
-// (grid_reduce), and their scalar consequences (alpha, beta, convergence) are
-// computed on the device by the last block, so a CG iteration never waits on
-// the host.
+// Layout and launch shape.  One wavefront owns one brick at a time: lane
+// = x + 8y holds the brick's column of 8 cells along z in registers, so the
+// z-neighbours are register reads, the x/y neighbours inside the brick are
+// cross-lane shuffles (lane +-1, +-8), and only the lanes on a brick face
+// load halo values from the neighbour brick.  Every load of a brick's own
+// cells is a fully coalesced 64-lane row.  Waves walk the active-brick list
+// with a grid stride (WPB waves per 256-thread block, at most GRID_CAP
+// blocks), so a launch fills all 256 CUs and per-block set-up is amortised.
+//
+// All vectors are zero outside the domain, so the 7-point operator needs only
+// the cell's own code byte (brick.h).  Reductions write one partial per block;
+// a one-block finalize kernel (k_finalize) sums them in block order
+// (bitwise reproducible) and applies the CG scalar logic on the device, so a
+// CG iteration never waits on the host.
 #pragma once
 #include "brick.h"
 
@@ -28,19 +35,53 @@ struct Scal {
 };
 
 #define KTHREADS 256
+#define WPB 4          // waves per block
+#define GRID_CAP 2048  // blocks per launch at most (8 per CU)
 
-// Operator row of a cell: returns L v at the cell (L = -A, A the reference's
-// assembled matrix), dg = diagonal of L.  ih2 = inv_h^2 per direction.
-template <typename T, typename TV>
-__device__ __forceinline__ T lap_row(const TV* __restrict__ v, const int* __restrict__ nbr, int b, int x, int y, int z,
-                                     unsigned cd, T vc, T ihx2, T ihy2, T ihz2, T& dg) {
+enum FinOp { OP_FF = 0, OP_RR0 = 1, OP_TRUE = 2, OP_DQ = 3, OP_RR = 4, OP_RZ = 5, OP_MAX = 6 };
+
+__device__ __forceinline__ int lane_id() { return threadIdx.x & 63; }
+__device__ __forceinline__ int wave_in_block() { return threadIdx.x >> 6; }
+
+// Wave-uniform brick loop over the list act[0..n).
+#define FOR_EACH_BRICK(act, n, b)                                                             \
+  for (int i_ = blockIdx.x * WPB + wave_in_block(); i_ < (n); i_ += gridDim.x * WPB)         \
+    if (int b = __builtin_amdgcn_readfirstlane((act)[i_]); true)
+
+// Walk the 8 cells of a lane's column with the 6 neighbour values of each.
+// col holds the lane's own values; ld(brick, cell) fetches a value from a
+// neighbour brick (already composed, e.g. z + beta d); f(z, xm, xp, ym, yp, zm, zp).
+template <typename T, typename Ld, typename F>
+__device__ __forceinline__ void col_stencil(const int* __restrict__ nbr, int b, int lane, const T (&col)[8], Ld ld, F f) {
+  const int x = lane & 7, y = lane >> 3;
+  const int* nb = nbr + (size_t)b * 27;
+  const int bxm = nb[12], bxp = nb[14], bym = nb[10], byp = nb[16], bzm = nb[4], bzp = nb[22];
+  const T below = (bzm >= 0) ? ld(bzm, lane + 448) : T(0);
+  const T above = (bzp >= 0) ? ld(bzp, lane) : T(0);
+#pragma unroll
+  for (int z = 0; z < 8; ++z) {
+    T v = col[z];
+    T xm = __shfl_up(v, 1), xp = __shfl_down(v, 1), ym = __shfl_up(v, 8), yp = __shfl_down(v, 8);
+    if (x == 0) xm = (bxm >= 0) ? ld(bxm, 7 + 8 * y + 64 * z) : T(0);
+    if (x == 7) xp = (bxp >= 0) ? ld(bxp, 8 * y + 64 * z) : T(0);
+    if (y == 0) ym = (bym >= 0) ? ld(bym, x + 56 + 64 * z) : T(0);
+    if (y == 7) yp = (byp >= 0) ? ld(byp, x + 64 * z) : T(0);
+    T zm = (z > 0) ? col[z > 0 ? z - 1 : 0] : below;
+    T zp = (z < 7) ? col[z < 7 ? z + 1 : 7] : above;
+    f(z, xm, xp, ym, yp, zm, zp);
+  }
+}
+
+// L v at a cell from its code and neighbour values (L = -A, dg = diagonal).
+template <typename T>
+__device__ __forceinline__ T lap_cell(unsigned cd, T vc, T xm, T xp, T ym, T yp, T zm, T zp, T ihx2, T ihy2, T ihz2, T& dg) {
   T acc = T(0), d = T(0);
-  if (cd & 1u) { d += ihx2; acc += ihx2 * (T)ld_rel(v, nbr, b, x - 1, y, z); }
-  if (cd & 2u) { d += ihx2; acc += ihx2 * (T)ld_rel(v, nbr, b, x + 1, y, z); }
-  if (cd & 4u) { d += ihy2; acc += ihy2 * (T)ld_rel(v, nbr, b, x, y - 1, z); }
-  if (cd & 8u) { d += ihy2; acc += ihy2 * (T)ld_rel(v, nbr, b, x, y + 1, z); }
-  if (cd & 16u) { d += ihz2; acc += ihz2 * (T)ld_rel(v, nbr, b, x, y, z - 1); }
-  if (cd & 32u) { d += ihz2; acc += ihz2 * (T)ld_rel(v, nbr, b, x, y, z + 1); }
+  if (cd & 1u) { d += ihx2; acc += ihx2 * xm; }
+  if (cd & 2u) { d += ihx2; acc += ihx2 * xp; }
+  if (cd & 4u) { d += ihy2; acc += ihy2 * ym; }
+  if (cd & 8u) { d += ihy2; acc += ihy2 * yp; }
+  if (cd & 16u) { d += ihz2; acc += ihz2 * zm; }
+  if (cd & 32u) { d += ihz2; acc += ihz2 * zp; }
   dg = d;
   return d * vc - acc;
 }
@@ -55,6 +96,60 @@ __device__ __forceinline__ T diag_of(unsigned cd, T ihx2, T ihy2, T ihz2) {
   if (cd & 16u) d += ihz2;
   if (cd & 32u) d += ihz2;
   return d;
+}
+
+__device__ __forceinline__ void write_partial(double v, double* __restrict__ partials, double* sh, bool is_max) {
+  double t = is_max ? block_max(v, sh) : block_sum(v, sh);
+  if (threadIdx.x == 0) partials[blockIdx.x] = t;
+}
+
+// One block: total of the partials in block order, then the scalar step.
+__global__ void k_finalize(int n, const double* __restrict__ partials, Scal* s, int op, double rtol2) {
+  __shared__ double sh[16];
+  if ((op == OP_DQ || op == OP_RR || op == OP_RZ) && s->done) return;
+  const bool mx = (op == OP_MAX);
+  double acc = 0.0;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) acc = mx ? fmax(acc, partials[i]) : acc + partials[i];
+  double t = mx ? block_max(acc, sh) : block_sum(acc, sh);
+  if (threadIdx.x != 0) return;
+  switch (op) {
+    case OP_FF:
+      s->ff = t;
+      s->tol2 = rtol2 * t;
+      s->done = (t == 0.0) ? 1 : 0;
+      break;
+    case OP_RR0:
+      s->rr = t;
+      s->rz_valid = 0;
+      s->done = (t <= s->tol2) ? 1 : 0;
+      break;
+    case OP_TRUE:
+      s->tmp = t;
+      break;
+    case OP_DQ:
+      s->dq = t;
+      if (t > 0.0) {
+        s->alpha = s->rz / t;
+      } else {
+        s->alpha = 0.0;
+        s->done = 3;
+      }
+      break;
+    case OP_RR:
+      s->rr = t;
+      s->iter += 1;
+      if (t <= s->tol2) s->done = 1;
+      else if (s->iter >= s->max_iter) s->done = 2;
+      break;
+    case OP_RZ:
+      s->beta = s->rz_valid ? t / s->rz : 0.0;
+      s->rz = t;
+      s->rz_valid = 1;
+      break;
+    case OP_MAX:
+      s->maxdiv = t;
+      break;
+  }
 }
 
 // ---------------------------------------------------------------- setup ----
@@ -152,236 +247,258 @@ __global__ void k_brick_count(int nb, const uint8_t* __restrict__ code, int* __r
 
 // ------------------------------------------------------------ projection ----
 
-// f = -div(u) on in-domain cells (L p = f with L = -A), div optionally kept.
-// Divergence in the exact operation order of Policy_Divergence.h:20-29.
-__global__ void k_div(const int* __restrict__ act, const int* __restrict__ nbr, const uint8_t* __restrict__ code,
-                      const double* __restrict__ u, const double* __restrict__ v, const double* __restrict__ w, double ihx,
-                      double ihy, double ihz, double* __restrict__ f, double* __restrict__ div_out,
-                      double* __restrict__ partials, Scal* s) {
+// Divergence of a column, operation order of Policy_Divergence.h:20-29:
+// ret = 0; ret -= u(c)/h; ret += u(c+e_x)/h; ... (no contraction).
+struct DivCol {
+  double d[8];
+};
+
+__device__ __forceinline__ DivCol div_column(const int* __restrict__ nbr, int b, int lane, const double* __restrict__ u,
+                                             const double* __restrict__ v, const double* __restrict__ w, double ihx, double ihy,
+                                             double ihz) {
 #pragma clang fp contract(off)
+  const int x = lane & 7, y = lane >> 3;
+  const size_t base = (size_t)b * MAC_BC + lane;
+  const int* nb = nbr + (size_t)b * 27;
+  const int bxp = nb[14], byp = nb[16], bzp = nb[22];
+  double wc[8];
+#pragma unroll
+  for (int z = 0; z < 8; ++z) wc[z] = w[base + 64 * z];
+  const double wtop = (bzp >= 0) ? w[(size_t)bzp * MAC_BC + lane] : 0.0;
+  DivCol out;
+#pragma unroll
+  for (int z = 0; z < 8; ++z) {
+    double uc = u[base + 64 * z], vc = v[base + 64 * z];
+    double up = __shfl_down(uc, 1), vp = __shfl_down(vc, 8);
+    if (x == 7) up = (bxp >= 0) ? u[(size_t)bxp * MAC_BC + 8 * y + 64 * z] : 0.0;
+    if (y == 7) vp = (byp >= 0) ? v[(size_t)byp * MAC_BC + x + 64 * z] : 0.0;
+    double wp = (z < 7) ? wc[z < 7 ? z + 1 : 7] : wtop;
+    double ret = 0.0;
+    ret -= uc * ihx;
+    ret += up * ihx;
+    ret -= vc * ihy;
+    ret += vp * ihy;
+    ret -= wc[z] * ihz;
+    ret += wp * ihz;
+    out.d[z] = ret;
+  }
+  return out;
+}
+
+// f = -div(u) on in-domain cells (L p = f with L = -A), div optionally kept;
+// partial ||f||^2.
+__global__ void __launch_bounds__(KTHREADS) k_div(int nact, const int* __restrict__ act, const int* __restrict__ nbr,
+                                                  const uint8_t* __restrict__ code, const double* __restrict__ u,
+                                                  const double* __restrict__ v, const double* __restrict__ w, double ihx,
+                                                  double ihy, double ihz, double* __restrict__ f, double* __restrict__ div_out,
+                                                  double* __restrict__ partials) {
   __shared__ double sh[16];
-  int b = act[blockIdx.x];
+  const int lane = lane_id();
   double acc = 0.0;
-  for (int t = threadIdx.x; t < MAC_BC; t += blockDim.x) {
-    int x = t & 7, y = (t >> 3) & 7, z = t >> 6;
-    size_t c = (size_t)b * MAC_BC + t;
-    double dv = 0.0;
-    if (code[c] & CODE_DOM) {
-      double ret = 0.0;
-      ret -= u[c] * ihx;
-      ret += ld_rel(u, nbr, b, x + 1, y, z) * ihx;
-      ret -= v[c] * ihy;
-      ret += ld_rel(v, nbr, b, x, y + 1, z) * ihy;
-      ret -= w[c] * ihz;
-      ret += ld_rel(w, nbr, b, x, y, z + 1) * ihz;
-      dv = ret;
+  FOR_EACH_BRICK(act, nact, b) {
+    DivCol dc = div_column(nbr, b, lane, u, v, w, ihx, ihy, ihz);
+    const size_t base = (size_t)b * MAC_BC + lane;
+#pragma unroll
+    for (int z = 0; z < 8; ++z) {
+      bool dom = code[base + 64 * z] & CODE_DOM;
+      double dv = dom ? dc.d[z] : 0.0;
+      if (div_out) div_out[base + 64 * z] = dv;
+      f[base + 64 * z] = -dv;
+      acc += dv * dv;
     }
-    if (div_out) div_out[c] = dv;
-    f[c] = -dv;
-    acc += dv * dv;
   }
-  double part = block_sum(acc, sh);
-  double tot;
-  if (grid_reduce(part, partials, &s->counter, false, &tot, sh) && threadIdx.x == 0) {
-    s->ff = tot;
-    s->done = (tot == 0.0) ? 1 : 0;
-  }
+  write_partial(acc, partials, sh, false);
 }
 
 // max |div u| over the domain (the reference's post-check, :209-219).
-__global__ void k_maxdiv(const int* __restrict__ act, const int* __restrict__ nbr, const uint8_t* __restrict__ code,
-                         const double* __restrict__ u, const double* __restrict__ v, const double* __restrict__ w,
-                         double ihx, double ihy, double ihz, double* __restrict__ partials, Scal* s) {
-#pragma clang fp contract(off)
+__global__ void __launch_bounds__(KTHREADS) k_maxdiv(int nact, const int* __restrict__ act, const int* __restrict__ nbr,
+                                                     const uint8_t* __restrict__ code, const double* __restrict__ u,
+                                                     const double* __restrict__ v, const double* __restrict__ w, double ihx,
+                                                     double ihy, double ihz, double* __restrict__ partials) {
   __shared__ double sh[16];
-  int b = act[blockIdx.x];
+  const int lane = lane_id();
   double m = 0.0;
-  for (int t = threadIdx.x; t < MAC_BC; t += blockDim.x) {
-    int x = t & 7, y = (t >> 3) & 7, z = t >> 6;
-    size_t c = (size_t)b * MAC_BC + t;
-    if (code[c] & CODE_DOM) {
-      double ret = 0.0;
-      ret -= u[c] * ihx;
-      ret += ld_rel(u, nbr, b, x + 1, y, z) * ihx;
-      ret -= v[c] * ihy;
-      ret += ld_rel(v, nbr, b, x, y + 1, z) * ihy;
-      ret -= w[c] * ihz;
-      ret += ld_rel(w, nbr, b, x, y, z + 1) * ihz;
-      m = fmax(m, fabs(ret));
-    }
+  FOR_EACH_BRICK(act, nact, b) {
+    DivCol dc = div_column(nbr, b, lane, u, v, w, ihx, ihy, ihz);
+    const size_t base = (size_t)b * MAC_BC + lane;
+#pragma unroll
+    for (int z = 0; z < 8; ++z)
+      if (code[base + 64 * z] & CODE_DOM) m = fmax(m, fabs(dc.d[z]));
   }
-  double part = block_max(m, sh);
-  double tot;
-  if (grid_reduce(part, partials, &s->counter, true, &tot, sh) && threadIdx.x == 0) s->maxdiv = tot;
+  write_partial(m, partials, sh, true);
 }
 
-// r = f - L x and ||r||^2 (x = starting guess; zero outside the domain).
-// mode 0: start of a solve (sets rr, convergence); mode 1: true-residual
-// check only (total to s->tmp, r untouched).
-__global__ void k_resid0(const int* __restrict__ act, const int* __restrict__ nbr, const uint8_t* __restrict__ code,
-                         const double* __restrict__ f, const double* __restrict__ x, double* __restrict__ r, double ihx2,
-                         double ihy2, double ihz2, double* __restrict__ partials, Scal* s, int mode) {
+// r = f - L x and partial ||r||^2.  write_r = 0: true-residual check only.
+__global__ void __launch_bounds__(KTHREADS) k_resid0(int nact, const int* __restrict__ act, const int* __restrict__ nbr,
+                                                     const uint8_t* __restrict__ code, const double* __restrict__ f,
+                                                     const double* __restrict__ xv, double* __restrict__ r, double ihx2,
+                                                     double ihy2, double ihz2, int write_r, double* __restrict__ partials) {
   __shared__ double sh[16];
-  int b = act[blockIdx.x];
+  const int lane = lane_id();
   double acc = 0.0;
-  for (int t = threadIdx.x; t < MAC_BC; t += blockDim.x) {
-    int xx = t & 7, y = (t >> 3) & 7, z = t >> 6;
-    size_t c = (size_t)b * MAC_BC + t;
-    unsigned cd = code[c];
-    if (cd & CODE_DOM) {
-      double dg;
-      double lx = lap_row<double, double>(x, nbr, b, xx, y, z, cd, x[c], ihx2, ihy2, ihz2, dg);
-      double rv = f[c] - lx;
-      if (mode == 0) r[c] = rv;
-      acc += rv * rv;
+  FOR_EACH_BRICK(act, nact, b) {
+    const size_t base = (size_t)b * MAC_BC + lane;
+    double col[8];
+    unsigned cd[8];
+#pragma unroll
+    for (int z = 0; z < 8; ++z) {
+      col[z] = xv[base + 64 * z];
+      cd[z] = code[base + 64 * z];
     }
+    col_stencil<double>(nbr, b, lane, col, [&](int nb, int cell) { return xv[(size_t)nb * MAC_BC + cell]; },
+                        [&](int z, double xm, double xp, double ym, double yp, double zm, double zp) {
+                          if (cd[z] & CODE_DOM) {
+                            double dg;
+                            double lx = lap_cell<double>(cd[z], col[z], xm, xp, ym, yp, zm, zp, ihx2, ihy2, ihz2, dg);
+                            double rv = f[base + 64 * z] - lx;
+                            if (write_r) r[base + 64 * z] = rv;
+                            acc += rv * rv;
+                          }
+                        });
   }
-  double part = block_sum(acc, sh);
-  double tot;
-  if (grid_reduce(part, partials, &s->counter, false, &tot, sh) && threadIdx.x == 0) {
-    if (mode == 0) {
-      s->rr = tot;
-      s->rz_valid = 0;
-      s->done = (tot <= s->tol2) ? 1 : 0;
-    } else {
-      s->tmp = tot;
-    }
-  }
+  write_partial(acc, partials, sh, false);
 }
 
 // d_new = z + beta d_old (formed on the fly at the neighbours too), q = L d_new,
-// and alpha = (r.z) / (d_new . q).
+// partial d_new . q.
 template <typename TZ>
-__global__ void k_pmatvec(const int* __restrict__ act, const int* __restrict__ nbr, const uint8_t* __restrict__ code,
-                          const TZ* __restrict__ zv, const double* __restrict__ dold, double* __restrict__ dnew,
-                          double* __restrict__ q, double ihx2, double ihy2, double ihz2, double* __restrict__ partials,
-                          Scal* s) {
+__global__ void __launch_bounds__(KTHREADS) k_pmatvec(int nact, const int* __restrict__ act, const int* __restrict__ nbr,
+                                                      const uint8_t* __restrict__ code, const TZ* __restrict__ zv,
+                                                      const double* __restrict__ dold, double* __restrict__ dnew,
+                                                      double* __restrict__ q, double ihx2, double ihy2, double ihz2,
+                                                      double* __restrict__ partials, const Scal* s) {
   if (s->done) return;
   __shared__ double sh[16];
   const double beta = s->beta;
-  int b = act[blockIdx.x];
+  const int lane = lane_id();
   double acc = 0.0;
-  for (int t = threadIdx.x; t < MAC_BC; t += blockDim.x) {
-    int x = t & 7, y = (t >> 3) & 7, z = t >> 6;
-    size_t c = (size_t)b * MAC_BC + t;
-    unsigned cd = code[c];
-    if (cd & CODE_DOM) {
-      double dc = (double)zv[c] + beta * dold[c];
-      double a = 0.0, dg = 0.0;
-#define NBD(X, Y, Z) ((double)ld_rel(zv, nbr, b, X, Y, Z) + beta * ld_rel(dold, nbr, b, X, Y, Z))
-      if (cd & 1u) { dg += ihx2; a += ihx2 * NBD(x - 1, y, z); }
-      if (cd & 2u) { dg += ihx2; a += ihx2 * NBD(x + 1, y, z); }
-      if (cd & 4u) { dg += ihy2; a += ihy2 * NBD(x, y - 1, z); }
-      if (cd & 8u) { dg += ihy2; a += ihy2 * NBD(x, y + 1, z); }
-      if (cd & 16u) { dg += ihz2; a += ihz2 * NBD(x, y, z - 1); }
-      if (cd & 32u) { dg += ihz2; a += ihz2 * NBD(x, y, z + 1); }
-#undef NBD
-      double qv = dg * dc - a;
-      dnew[c] = dc;
-      q[c] = qv;
-      acc += dc * qv;
+  FOR_EACH_BRICK(act, nact, b) {
+    const size_t base = (size_t)b * MAC_BC + lane;
+    double col[8];
+    unsigned cd[8];
+#pragma unroll
+    for (int z = 0; z < 8; ++z) {
+      col[z] = (double)zv[base + 64 * z] + beta * dold[base + 64 * z];
+      cd[z] = code[base + 64 * z];
     }
+    col_stencil<double>(
+        nbr, b, lane, col,
+        [&](int nb, int cell) { return (double)zv[(size_t)nb * MAC_BC + cell] + beta * dold[(size_t)nb * MAC_BC + cell]; },
+        [&](int z, double xm, double xp, double ym, double yp, double zm, double zp) {
+          if (cd[z] & CODE_DOM) {
+            double dg;
+            double qv = lap_cell<double>(cd[z], col[z], xm, xp, ym, yp, zm, zp, ihx2, ihy2, ihz2, dg);
+            dnew[base + 64 * z] = col[z];
+            q[base + 64 * z] = qv;
+            acc += col[z] * qv;
+          }
+        });
   }
-  double part = block_sum(acc, sh);
-  double tot;
-  if (grid_reduce(part, partials, &s->counter, false, &tot, sh) && threadIdx.x == 0) {
-    s->dq = tot;
-    if (tot > 0.0) {
-      s->alpha = s->rz / tot;
-    } else {
-      s->alpha = 0.0;
-      s->done = 3;
-    }
-  }
+  write_partial(acc, partials, sh, false);
 }
 
-// x += alpha d, r -= alpha q, ||r||^2, convergence test.
-__global__ void k_update(const int* __restrict__ act, const uint8_t* __restrict__ code, double* __restrict__ x,
-                         double* __restrict__ r, const double* __restrict__ d, const double* __restrict__ q,
-                         double* __restrict__ partials, Scal* s) {
+// x += alpha d, r -= alpha q, partial ||r||^2.
+__global__ void __launch_bounds__(KTHREADS) k_update(int nact, const int* __restrict__ act, const uint8_t* __restrict__ code,
+                                                     double* __restrict__ x, double* __restrict__ r,
+                                                     const double* __restrict__ d, const double* __restrict__ q,
+                                                     double* __restrict__ partials, const Scal* s) {
   if (s->done) return;
   __shared__ double sh[16];
   const double alpha = s->alpha;
-  int b = act[blockIdx.x];
+  const int lane = lane_id();
   double acc = 0.0;
-  for (int t = threadIdx.x; t < MAC_BC; t += blockDim.x) {
-    size_t c = (size_t)b * MAC_BC + t;
-    if (code[c] & CODE_DOM) {
-      x[c] += alpha * d[c];
-      double rv = r[c] - alpha * q[c];
-      r[c] = rv;
-      acc += rv * rv;
+  FOR_EACH_BRICK(act, nact, b) {
+    const size_t base = (size_t)b * MAC_BC + lane;
+#pragma unroll
+    for (int z = 0; z < 8; ++z) {
+      const size_t c = base + 64 * z;
+      if (code[c] & CODE_DOM) {
+        x[c] += alpha * d[c];
+        double rv = r[c] - alpha * q[c];
+        r[c] = rv;
+        acc += rv * rv;
+      }
     }
   }
-  double part = block_sum(acc, sh);
-  double tot;
-  if (grid_reduce(part, partials, &s->counter, false, &tot, sh) && threadIdx.x == 0) {
-    s->rr = tot;
-    s->iter += 1;
-    if (tot <= s->tol2) s->done = 1;
-    else if (s->iter >= s->max_iter) s->done = 2;
-  }
+  write_partial(acc, partials, sh, false);
 }
 
 // Damped Jacobi sweep: zout = zin + omega D^-1 (r - L zin); zin == nullptr
-// means a zero start (zout = omega D^-1 r).  With want_dot the sweep also
-// forms r.z and the CG beta.
+// means a zero start (zout = omega D^-1 r).  want_dot: partial r . zout.
 template <typename TR, typename TZ>
-__global__ void k_jacobi(const int* __restrict__ act, const int* __restrict__ nbr, const uint8_t* __restrict__ code,
-                         const TR* __restrict__ r, const TZ* __restrict__ zin, TZ* __restrict__ zout, TZ ihx2, TZ ihy2,
-                         TZ ihz2, TZ omega, int want_dot, double* __restrict__ partials, Scal* s) {
+__global__ void __launch_bounds__(KTHREADS) k_jacobi(int nact, const int* __restrict__ act, const int* __restrict__ nbr,
+                                                     const uint8_t* __restrict__ code, const TR* __restrict__ r,
+                                                     const TZ* __restrict__ zin, TZ* __restrict__ zout, TZ ihx2, TZ ihy2,
+                                                     TZ ihz2, TZ omega, int want_dot, double* __restrict__ partials,
+                                                     const Scal* s) {
   if (s->done) return;
   __shared__ double sh[16];
-  int b = act[blockIdx.x];
+  const int lane = lane_id();
   double acc = 0.0;
-  for (int t = threadIdx.x; t < MAC_BC; t += blockDim.x) {
-    int x = t & 7, y = (t >> 3) & 7, z = t >> 6;
-    size_t c = (size_t)b * MAC_BC + t;
-    unsigned cd = code[c];
-    TZ out = TZ(0);
-    if (cd & CODE_DOM) {
-      TZ rc = (TZ)r[c];
-      if (zin) {
-        TZ dg;
-        TZ zc = zin[c];
-        TZ lz = lap_row<TZ, TZ>(zin, nbr, b, x, y, z, cd, zc, ihx2, ihy2, ihz2, dg);
-        out = dg > TZ(0) ? zc + omega * (rc - lz) / dg : TZ(0);
-      } else {
-        TZ dg = diag_of<TZ>(cd, ihx2, ihy2, ihz2);
-        out = dg > TZ(0) ? omega * rc / dg : TZ(0);
+  FOR_EACH_BRICK(act, nact, b) {
+    const size_t base = (size_t)b * MAC_BC + lane;
+    unsigned cd[8];
+#pragma unroll
+    for (int z = 0; z < 8; ++z) cd[z] = code[base + 64 * z];
+    if (zin) {
+      TZ col[8];
+#pragma unroll
+      for (int z = 0; z < 8; ++z) col[z] = zin[base + 64 * z];
+      col_stencil<TZ>(nbr, b, lane, col, [&](int nb, int cell) { return zin[(size_t)nb * MAC_BC + cell]; },
+                      [&](int z, TZ xm, TZ xp, TZ ym, TZ yp, TZ zm, TZ zp) {
+                        TZ out = TZ(0);
+                        if (cd[z] & CODE_DOM) {
+                          TZ dg;
+                          TZ lz = lap_cell<TZ>(cd[z], col[z], xm, xp, ym, yp, zm, zp, ihx2, ihy2, ihz2, dg);
+                          TR rv = r[base + 64 * z];
+                          out = dg > TZ(0) ? col[z] + omega * ((TZ)rv - lz) / dg : TZ(0);
+                          if (want_dot) acc += (double)rv * (double)out;
+                        }
+                        zout[base + 64 * z] = out;
+                      });
+    } else {
+#pragma unroll
+      for (int z = 0; z < 8; ++z) {
+        TZ out = TZ(0);
+        if (cd[z] & CODE_DOM) {
+          TR rv = r[base + 64 * z];
+          TZ dg = diag_of<TZ>(cd[z], ihx2, ihy2, ihz2);
+          out = dg > TZ(0) ? omega * (TZ)rv / dg : TZ(0);
+          if (want_dot) acc += (double)rv * (double)out;
+        }
+        zout[base + 64 * z] = out;
       }
-      if (want_dot) acc += (double)r[c] * (double)out;
     }
-    zout[c] = out;
   }
-  if (!want_dot) return;
-  double part = block_sum(acc, sh);
-  double tot;
-  if (grid_reduce(part, partials, &s->counter, false, &tot, sh) && threadIdx.x == 0) {
-    s->beta = s->rz_valid ? tot / s->rz : 0.0;
-    s->rz = tot;
-    s->rz_valid = 1;
-  }
+  if (want_dot) write_partial(acc, partials, sh, false);
 }
 
 // res = r - L z.
 template <typename TR, typename TZ>
-__global__ void k_residual(const int* __restrict__ act, const int* __restrict__ nbr, const uint8_t* __restrict__ code,
-                           const TR* __restrict__ r, const TZ* __restrict__ zv, TZ* __restrict__ res, TZ ihx2, TZ ihy2,
-                           TZ ihz2, Scal* s) {
+__global__ void __launch_bounds__(KTHREADS) k_residual(int nact, const int* __restrict__ act, const int* __restrict__ nbr,
+                                                       const uint8_t* __restrict__ code, const TR* __restrict__ r,
+                                                       const TZ* __restrict__ zv, TZ* __restrict__ res, TZ ihx2, TZ ihy2,
+                                                       TZ ihz2, const Scal* s) {
   if (s->done) return;
-  int b = act[blockIdx.x];
-  for (int t = threadIdx.x; t < MAC_BC; t += blockDim.x) {
-    int x = t & 7, y = (t >> 3) & 7, z = t >> 6;
-    size_t c = (size_t)b * MAC_BC + t;
-    unsigned cd = code[c];
-    TZ out = TZ(0);
-    if (cd & CODE_DOM) {
-      TZ dg;
-      TZ lz = lap_row<TZ, TZ>(zv, nbr, b, x, y, z, cd, zv[c], ihx2, ihy2, ihz2, dg);
-      out = (TZ)r[c] - lz;
+  const int lane = lane_id();
+  FOR_EACH_BRICK(act, nact, b) {
+    const size_t base = (size_t)b * MAC_BC + lane;
+    unsigned cd[8];
+    TZ col[8];
+#pragma unroll
+    for (int z = 0; z < 8; ++z) {
+      cd[z] = code[base + 64 * z];
+      col[z] = zv[base + 64 * z];
     }
-    res[c] = out;
+    col_stencil<TZ>(nbr, b, lane, col, [&](int nb, int cell) { return zv[(size_t)nb * MAC_BC + cell]; },
+                    [&](int z, TZ xm, TZ xp, TZ ym, TZ yp, TZ zm, TZ zp) {
+                      TZ out = TZ(0);
+                      if (cd[z] & CODE_DOM) {
+                        TZ dg;
+                        out = (TZ)r[base + 64 * z] - lap_cell<TZ>(cd[z], col[z], xm, xp, ym, yp, zm, zp, ihx2, ihy2, ihz2, dg);
+                      }
+                      res[base + 64 * z] = out;
+                    });
   }
 }
 
@@ -389,7 +506,7 @@ __global__ void k_residual(const int* __restrict__ act, const int* __restrict__ 
 // weights (1/4, 3/4, 3/4, 1/4) per direction over fine cells 2X-1 .. 2X+2.
 template <typename T>
 __global__ void k_restrict(const int* __restrict__ actc, const int* __restrict__ nbrc, const int* __restrict__ childc,
-                           const uint8_t* __restrict__ codec, const T* __restrict__ resf, T* __restrict__ rc, Scal* s) {
+                           const uint8_t* __restrict__ codec, const T* __restrict__ resf, T* __restrict__ rc, const Scal* s) {
   if (s->done) return;
   __shared__ int fb[64];
   int C = actc[blockIdx.x];
@@ -432,7 +549,7 @@ __global__ void k_restrict(const int* __restrict__ actc, const int* __restrict__
 template <typename T>
 __global__ void k_prolong(const int* __restrict__ actf, const int* __restrict__ parent, const int* __restrict__ oct,
                           const int* __restrict__ nbrc, const uint8_t* __restrict__ codef, const T* __restrict__ zc,
-                          T* __restrict__ zf, Scal* s) {
+                          T* __restrict__ zf, const Scal* s) {
   if (s->done) return;
   int B = actf[blockIdx.x];
   int P = parent[B], o = oct[B];
@@ -455,11 +572,11 @@ __global__ void k_prolong(const int* __restrict__ actf, const int* __restrict__ 
 // Exact solve on the coarsest level (a single brick): z = Linv r with the
 // dense inverse precomputed on the host (symmetric, so column j of Linv is
 // read coalesced as row j).  cellof[i] = cell of unknown i, rowof[cell] = i
-// or -1.
+// or -1.  want_dot: partial r . z.
 template <typename TR, typename TZ>
 __global__ void k_coarse_solve(int nc, const int* __restrict__ rowof, const int* __restrict__ cellof,
-                               const TZ* __restrict__ cinv, const TR* __restrict__ rc, TZ* __restrict__ zc,
-                               int want_dot, double* __restrict__ partials, Scal* s) {
+                               const TZ* __restrict__ cinv, const TR* __restrict__ rc, TZ* __restrict__ zc, int want_dot,
+                               double* __restrict__ partials, const Scal* s) {
   if (s->done) return;
   __shared__ TZ rl[MAC_BC];
   __shared__ double sh[16];
@@ -476,14 +593,7 @@ __global__ void k_coarse_solve(int nc, const int* __restrict__ rowof, const int*
     }
     zc[t] = out;
   }
-  if (!want_dot) return;
-  double part = block_sum(acc, sh);
-  double tot;
-  if (grid_reduce(part, partials, &s->counter, false, &tot, sh) && threadIdx.x == 0) {
-    s->beta = s->rz_valid ? tot / s->rz : 0.0;
-    s->rz = tot;
-    s->rz_valid = 1;
-  }
+  if (want_dot) write_partial(acc, partials, sh, false);
 }
 
 // u_i -= p(c)/h_i - p(c-e_i)/h_i on every face of an existing cell whose
@@ -539,7 +649,10 @@ __global__ void k_apply_A(const int* __restrict__ act, const int* __restrict__ n
     unsigned cd = code[c];
     if (cd & CODE_DOM) {
       double dg;
-      y[c] = -lap_row<double, double>(xv, nbr, b, x, yy, z, cd, xv[c], ihx2, ihy2, ihz2, dg);
+      double lx = lap_cell<double>(cd, xv[c], ld_rel(xv, nbr, b, x - 1, yy, z), ld_rel(xv, nbr, b, x + 1, yy, z),
+                                   ld_rel(xv, nbr, b, x, yy - 1, z), ld_rel(xv, nbr, b, x, yy + 1, z),
+                                   ld_rel(xv, nbr, b, x, yy, z - 1), ld_rel(xv, nbr, b, x, yy, z + 1), ihx2, ihy2, ihz2, dg);
+      y[c] = -lx;
     }
   }
 }
@@ -547,9 +660,8 @@ __global__ void k_apply_A(const int* __restrict__ act, const int* __restrict__ n
 // Zero the out-of-domain entries of a host-supplied vector (so that the
 // operator sees the zero-outside-domain convention).
 __global__ void k_mask(int nb, const uint8_t* __restrict__ code, double* __restrict__ v) {
-  size_t c = (size_t)blockIdx.x * MAC_BC + threadIdx.x;
   for (int t = threadIdx.x; t < MAC_BC; t += blockDim.x) {
-    c = (size_t)blockIdx.x * MAC_BC + t;
+    size_t c = (size_t)blockIdx.x * MAC_BC + t;
     if (!(code[c] & CODE_DOM)) v[c] = 0.0;
   }
 }

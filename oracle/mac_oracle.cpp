@@ -98,6 +98,28 @@ extern "C" {
 
 // ---------------------------------------------------------------- storage --
 int orc_chunk_key(int k) { return chunk_of(k); }
+// Chunk key for a general chunk size n (ChunkHashTable.tpp:181-196).
+int orc_chunk_key_n(int k, int n) { return k >= 0 ? k / n : (k + 1) / n - 1; }
+// Neighbour of offset `off` inside an n^dim chunk (x fastest) along direction
+// i (0-based) with sign dir; returns 1 when it wraps into the neighbouring
+// chunk (the table of Offset_GetNeighbour.h:12-43, restated digit-wise).
+int orc_offset_neighbour(int n, int dim, int i, int dir, int off, int* res) {
+  int dig[8], o = off;
+  for (int d = 0; d < dim; ++d) {
+    dig[d] = o % n;
+    o /= n;
+  }
+  int v = dig[i] + dir;
+  int wrap = (v < 0 || v >= n) ? 1 : 0;
+  dig[i] = (v % n + n) % n;
+  int r = 0, b = 1;
+  for (int d = 0; d < dim; ++d) {
+    r += dig[d] * b;
+    b *= n;
+  }
+  *res = r;
+  return wrap;
+}
 int orc_offset(int kx, int ky, int kz) { return mod_n(kx) + N * mod_n(ky) + N * N * mod_n(kz); }
 
 // ------------------------------------------------------------- assembly ----

@@ -30,6 +30,10 @@ def load():
     lib = ctypes.CDLL(LIB_PATH)
     lib.orc_chunk_key.restype = ctypes.c_int
     lib.orc_chunk_key.argtypes = [ctypes.c_int]
+    lib.orc_chunk_key_n.restype = ctypes.c_int
+    lib.orc_chunk_key_n.argtypes = [ctypes.c_int, ctypes.c_int]
+    lib.orc_offset_neighbour.restype = ctypes.c_int
+    lib.orc_offset_neighbour.argtypes = [ctypes.c_int] * 5 + [ctypes.POINTER(ctypes.c_int)]
     lib.orc_offset.restype = ctypes.c_int
     lib.orc_offset.argtypes = [ctypes.c_int] * 3
     lib.orc_assemble.restype = ctypes.c_int
@@ -65,6 +69,17 @@ def inv_h(h):
 
 def chunk_key(k):
     return load().orc_chunk_key(int(k))
+
+
+def chunk_key_n(k, n):
+    return load().orc_chunk_key_n(int(k), int(n))
+
+
+def offset_neighbour(n, dim, i, direction, off):
+    """(wraps, neighbour offset) -- Offset::GetNeighbour with 0-based i."""
+    r = ctypes.c_int()
+    w = load().orc_offset_neighbour(int(n), int(dim), int(i), int(direction), int(off), ctypes.byref(r))
+    return bool(w), r.value
 
 
 def offset(kx, ky, kz):

@@ -81,7 +81,7 @@ typedef struct {
   double ms_solve;       /* PCG part only                                        */
 } mac_stats;
 
-/* Defaults: h=1, rtol=1e-8, max_iter=500, FP64, nu=2, omega=0.8, check_every=4. */
+/* Defaults: h=1, rtol=1e-8, max_iter=500, FP64, nu=2, omega=0.9, check_every=4. */
 void mac_default_params(mac_params* prm);
 
 /* Create / destroy a context on the current HIP device. */

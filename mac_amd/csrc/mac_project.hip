@@ -61,6 +61,10 @@ struct Level {
   std::vector<int> keys;  // 3*nb, shifted so that every key is >= 0
   std::vector<int> h_nbr, h_child, h_parent, h_oct;
   int *nbr = nullptr, *child = nullptr, *parent = nullptr, *oct = nullptr, *act = nullptr;
+  int *meta = nullptr;   // per active brick: id + 6 face neighbours (MSTRIDE)
+  int *pmeta = nullptr;  // per active brick: prolongation sources from level+1 (PSTRIDE)
+  int *rmeta = nullptr;  // per active brick: restriction sources from level-1 (RSTRIDE)
+  std::vector<int> h_act;
   int nact = 0;
   long long ndom = 0;
   uint8_t *code = nullptr, *lab = nullptr;
@@ -387,6 +391,7 @@ int build_grid(mac_ctx* c, int nbricks, const int32_t* keys3, const uint8_t* fla
         lv.ndom += cnt[b];
       }
     lv.nact = (int)act.size();
+    lv.h_act = act;
     CHK(upload(c, &lv.act, act));
     if (l == 0) {
       // bricks whose faces the gradient update may touch: active, or with an
@@ -402,6 +407,55 @@ int build_grid(mac_ctx* c, int nbricks, const int32_t* keys3, const uint8_t* fla
       }
       c->nglist = (int)gl.size();
       CHK(upload(c, &c->glist, gl));
+    }
+  }
+  // flattened per-active-brick metadata
+  for (size_t l = 0; l < c->L.size(); ++l) {
+    Level& lv = c->L[l];
+    std::vector<int> meta((size_t)std::max(lv.nact, 1) * MSTRIDE, -1);
+    for (int i = 0; i < lv.nact; ++i) {
+      int b = lv.h_act[i];
+      const int* nb = &lv.h_nbr[(size_t)b * 27];
+      int* m = &meta[(size_t)i * MSTRIDE];
+      m[0] = b;
+      m[1] = nb[12];
+      m[2] = nb[14];
+      m[3] = nb[10];
+      m[4] = nb[16];
+      m[5] = nb[4];
+      m[6] = nb[22];
+      m[7] = 0;
+    }
+    CHK(upload(c, &lv.meta, meta));
+    if (l + 1 < c->L.size()) {
+      Level& cl = c->L[l + 1];
+      std::vector<int> pm((size_t)std::max(lv.nact, 1) * PSTRIDE, -1);
+      for (int i = 0; i < lv.nact; ++i) {
+        int B = lv.h_act[i];
+        int P = lv.h_parent[B], o = lv.h_oct[B];
+        int ex = (o & 1) ? 1 : -1, ey = (o & 2) ? 1 : -1, ez = (o & 4) ? 1 : -1;
+        int* m = &pm[(size_t)i * PSTRIDE];
+        m[0] = B;
+        m[1] = o;
+        for (int k = 0; k < 8; ++k) {
+          int sx = (k & 1) ? ex : 0, sy = (k & 2) ? ey : 0, sz = (k & 4) ? ez : 0;
+          m[2 + k] = cl.h_nbr[(size_t)P * 27 + (sx + 1) + 3 * (sy + 1) + 9 * (sz + 1)];
+        }
+      }
+      CHK(upload(c, &lv.pmeta, pm));
+      std::vector<int> rm((size_t)std::max(cl.nact, 1) * RSTRIDE, -1);
+      for (int i = 0; i < cl.nact; ++i) {
+        int C = cl.h_act[i];
+        int* m = &rm[(size_t)i * RSTRIDE];
+        m[0] = C;
+        m[1] = m[2] = m[3] = 0;
+        for (int t = 0; t < 64; ++t) {
+          int rx = (t & 3) - 1, ry = ((t >> 2) & 3) - 1, rz = (t >> 4) - 1;
+          int cn = cl.h_nbr[(size_t)C * 27 + ((rx >> 1) + 1) + 3 * ((ry >> 1) + 1) + 9 * ((rz >> 1) + 1)];
+          m[4 + t] = cn >= 0 ? cl.h_child[(size_t)cn * 8 + (rx & 1) + 2 * (ry & 1) + 4 * (rz & 1)] : -1;
+        }
+      }
+      CHK(upload(c, &cl.rmeta, rm));
     }
   }
   // coarsest exact solve
@@ -473,10 +527,10 @@ void sweep(mac_ctx* c, int l, const TZ* zin, TZ* zout, int want_dot) {
   {
   TimedLaunch tl(c, l == 0 ? K_JACOBI0 : K_COARSE, nd * (sr + (zin ? 2 : 1) * sz + 1));
   if (l == 0)
-    k_jacobi<double, TZ><<<grid_of(lv.nact), KTHREADS, 0, c->st>>>(lv.nact, lv.act, lv.nbr, lv.code, c->r, zin, zout, ih[0], ih[1],
+    k_jacobi<double, TZ><<<grid_of(lv.nact), KTHREADS, 0, c->st>>>(lv.nact, lv.meta, lv.code, c->r, zin, zout, ih[0], ih[1],
                                                                    ih[2], om, want_dot, c->partials, c->s);
   else
-    k_jacobi<TZ, TZ><<<grid_of(lv.nact), KTHREADS, 0, c->st>>>(lv.nact, lv.act, lv.nbr, lv.code, (const TZ*)lv.r, zin, zout, ih[0],
+    k_jacobi<TZ, TZ><<<grid_of(lv.nact), KTHREADS, 0, c->st>>>(lv.nact, lv.meta, lv.code, (const TZ*)lv.r, zin, zout, ih[0],
                                                                ih[1], ih[2], om, want_dot, c->partials, c->s);
   }
   if (want_dot) finalize(c, grid_of(lv.nact), OP_RZ);
@@ -507,23 +561,35 @@ int vcycle(mac_ctx* c, TZ** zout) {
     TZ* res = (TZ*)lv.res;
     double nd = (double)lvl_dom(c, l);
     double sr = (l == 0) ? 8.0 : sz;
-    for (int k = 0; k < nu; ++k) {
-      sweep<TZ>(c, l, k ? z : nullptr, k ? zt : z, 0);
-      if (k) std::swap(z, zt);
+    if (nu >= 2) {
+      // the first two sweeps from zero in one pass
+      TimedLaunch tl(c, l == 0 ? K_JACOBI0 : K_COARSE, nd * (sr + sz + 1));
+      if (l == 0)
+        k_jacobi2<double, TZ><<<grid_of(lv.nact), KTHREADS, 0, c->st>>>(lv.nact, lv.meta, lv.code, c->r, z, ih[0], ih[1],
+                                                                         ih[2], (TZ)c->prm.omega, c->s);
+      else
+        k_jacobi2<TZ, TZ><<<grid_of(lv.nact), KTHREADS, 0, c->st>>>(lv.nact, lv.meta, lv.code, (const TZ*)lv.r, z, ih[0],
+                                                                     ih[1], ih[2], (TZ)c->prm.omega, c->s);
+    } else {
+      sweep<TZ>(c, l, nullptr, z, 0);
+    }
+    for (int k = 2; k < nu; ++k) {
+      sweep<TZ>(c, l, z, zt, 0);
+      std::swap(z, zt);
     }
     {
       TimedLaunch tl(c, l == 0 ? K_RESID0 : K_COARSE, nd * (sr + 2 * sz + 1));
       if (l == 0)
-        k_residual<double, TZ><<<grid_of(lv.nact), KTHREADS, 0, c->st>>>(lv.nact, lv.act, lv.nbr, lv.code, c->r, z, res, ih[0], ih[1],
+        k_residual<double, TZ><<<grid_of(lv.nact), KTHREADS, 0, c->st>>>(lv.nact, lv.meta, lv.code, c->r, z, res, ih[0], ih[1],
                                                                          ih[2], c->s);
       else
-        k_residual<TZ, TZ><<<grid_of(lv.nact), KTHREADS, 0, c->st>>>(lv.nact, lv.act, lv.nbr, lv.code, (const TZ*)lv.r, z, res, ih[0],
+        k_residual<TZ, TZ><<<grid_of(lv.nact), KTHREADS, 0, c->st>>>(lv.nact, lv.meta, lv.code, (const TZ*)lv.r, z, res, ih[0],
                                                                      ih[1], ih[2], c->s);
     }
     Level& cl = c->L[l + 1];
     {
       TimedLaunch tl(c, l == 0 ? K_RESTRICT1 : K_COARSE, nd * sz + (double)cl.ndom * sz);
-      k_restrict<TZ><<<cl.nact, KTHREADS, 0, c->st>>>(cl.act, cl.nbr, cl.child, cl.code, res, (TZ*)cl.r, c->s);
+      k_restrict_sep<TZ><<<cl.nact, KTHREADS, 0, c->st>>>(cl.rmeta, cl.code, res, (TZ*)cl.r, c->s);
     }
     zfin[l] = z;  // pre-smoothed iterate, finished on the way up
     lv.z = z;
@@ -566,7 +632,7 @@ int vcycle(mac_ctx* c, TZ** zout) {
     double nd = (double)lvl_dom(c, l);
     {
       TimedLaunch tl(c, l == 0 ? K_PROLONG0 : K_COARSE, nd * (2 * sz + 1) + (double)cl.ndom * sz);
-      k_prolong<TZ><<<lv.nact, KTHREADS, 0, c->st>>>(lv.act, lv.parent, lv.oct, cl.nbr, lv.code, zfin[l + 1], z, c->s);
+      k_prolong_col<TZ><<<grid_of(lv.nact), KTHREADS, 0, c->st>>>(lv.nact, lv.pmeta, lv.code, zfin[l + 1], z, c->s);
     }
     for (int k = 0; k < nu; ++k) {
       sweep<TZ>(c, l, z, zt, (l == 0 && k == nu - 1) ? 1 : 0);
@@ -601,7 +667,7 @@ int pcg(mac_ctx* c, bool use_guess, mac_stats* st) {
     // (re)start: r = f - L x
     {
       TimedLaunch tl(c, K_INIT, nd * 25);
-      k_resid0<<<grid_of(l0.nact), KTHREADS, 0, c->st>>>(l0.nact, l0.act, l0.nbr, l0.code, c->f, c->x, c->r, ih[0], ih[1], ih[2], 1,
+      k_resid0<<<grid_of(l0.nact), KTHREADS, 0, c->st>>>(l0.nact, l0.meta, l0.code, c->f, c->x, c->r, ih[0], ih[1], ih[2], 1,
                                                          c->partials);
     }
     finalize(c, grid_of(l0.nact), OP_RR0);
@@ -616,14 +682,14 @@ int pcg(mac_ctx* c, bool use_guess, mac_stats* st) {
       for (int k = 0; k < check; ++k) {
         {
           TimedLaunch tl(c, K_PMATVEC, nd * (25 + sizeof(TZ)));
-          k_pmatvec<TZ><<<grid_of(l0.nact), KTHREADS, 0, c->st>>>(l0.nact, l0.act, l0.nbr, l0.code, z, c->d[cur], c->d[cur ^ 1], c->q,
+          k_pmatvec<TZ><<<grid_of(l0.nact), KTHREADS, 0, c->st>>>(l0.nact, l0.meta, l0.code, z, c->d[cur], c->d[cur ^ 1], c->q,
                                                                   ih[0], ih[1], ih[2], c->partials, c->s);
         }
         finalize(c, grid_of(l0.nact), OP_DQ);
         cur ^= 1;
         {
           TimedLaunch tl(c, K_UPDATE, nd * 49);
-          k_update<<<grid_of(l0.nact), KTHREADS, 0, c->st>>>(l0.nact, l0.act, l0.code, c->x, c->r, c->d[cur], c->q, c->partials, c->s);
+          k_update<<<grid_of(l0.nact), KTHREADS, 0, c->st>>>(l0.nact, l0.meta, l0.code, c->x, c->r, c->d[cur], c->q, c->partials, c->s);
         }
         useq.push_back(finalize(c, grid_of(l0.nact), OP_RR));
         CHK(vcycle<TZ>(c, &z));
@@ -641,7 +707,7 @@ int pcg(mac_ctx* c, bool use_guess, mac_stats* st) {
     // true residual
     {
       TimedLaunch tl(c, K_INIT, nd * 17);
-      k_resid0<<<grid_of(l0.nact), KTHREADS, 0, c->st>>>(l0.nact, l0.act, l0.nbr, l0.code, c->f, c->x, c->r, ih[0], ih[1], ih[2], 0,
+      k_resid0<<<grid_of(l0.nact), KTHREADS, 0, c->st>>>(l0.nact, l0.meta, l0.code, c->f, c->x, c->r, ih[0], ih[1], ih[2], 0,
                                                          c->partials);
     }
     finalize(c, grid_of(l0.nact), OP_TRUE);
@@ -689,7 +755,7 @@ void mac_default_params(mac_params* p) {
   p->max_iter = 500;
   p->precision = MAC_FP64;
   p->nu = 2;
-  p->omega = 0.8;
+  p->omega = 0.9;
   p->check_every = 4;
 }
 
@@ -822,7 +888,7 @@ int mac_project(mac_ctx* c, int guess, mac_stats* st) {
   const double nd = (double)l0.ndom;
   if (l0.nact > 0) {
     TimedLaunch tl(c, K_DIV, nd * 33);
-    k_div<<<grid_of(l0.nact), KTHREADS, 0, c->st>>>(l0.nact, l0.act, l0.nbr, l0.code, c->u, c->v, c->w, inv_h[0], inv_h[1], inv_h[2],
+    k_div<<<grid_of(l0.nact), KTHREADS, 0, c->st>>>(l0.nact, l0.meta, l0.code, c->u, c->v, c->w, inv_h[0], inv_h[1], inv_h[2],
                                                     c->f, nullptr, c->partials);
   }
   if (l0.nact > 0) finalize(c, grid_of(l0.nact), OP_FF);
@@ -844,7 +910,7 @@ int mac_project(mac_ctx* c, int guess, mac_stats* st) {
   }
   if (l0.nact > 0) {
     TimedLaunch tl(c, K_MAXDIV, nd * 25);
-    k_maxdiv<<<grid_of(l0.nact), KTHREADS, 0, c->st>>>(l0.nact, l0.act, l0.nbr, l0.code, c->u, c->v, c->w, inv_h[0], inv_h[1], inv_h[2],
+    k_maxdiv<<<grid_of(l0.nact), KTHREADS, 0, c->st>>>(l0.nact, l0.meta, l0.code, c->u, c->v, c->w, inv_h[0], inv_h[1], inv_h[2],
                                                        c->partials);
   }
   if (l0.nact > 0) finalize(c, grid_of(l0.nact), OP_MAX);
@@ -869,7 +935,7 @@ int mac_divergence(mac_ctx* c, double* div_out) {
   HIPCHK(hipMemsetAsync(c->tmpv, 0, n0 * sizeof(double), c->st));
   HIPCHK(hipMemsetAsync(c->s, 0, sizeof(Scal), c->st));
   if (l0.nact > 0)
-    k_div<<<grid_of(l0.nact), KTHREADS, 0, c->st>>>(l0.nact, l0.act, l0.nbr, l0.code, c->u, c->v, c->w, inv_h[0], inv_h[1], inv_h[2],
+    k_div<<<grid_of(l0.nact), KTHREADS, 0, c->st>>>(l0.nact, l0.meta, l0.code, c->u, c->v, c->w, inv_h[0], inv_h[1], inv_h[2],
                                                     c->f, c->tmpv, c->partials);
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpyAsync(div_out, c->tmpv, n0 * sizeof(double), hipMemcpyDeviceToHost, c->st));
@@ -913,7 +979,7 @@ int mac_max_divergence(mac_ctx* c, double* out) {
   HIPCHK(hipMemsetAsync(c->s, 0, sizeof(Scal), c->st));
   if (l0.nact > 0)
   {
-    k_maxdiv<<<grid_of(l0.nact), KTHREADS, 0, c->st>>>(l0.nact, l0.act, l0.nbr, l0.code, c->u, c->v, c->w, inv_h[0], inv_h[1], inv_h[2],
+    k_maxdiv<<<grid_of(l0.nact), KTHREADS, 0, c->st>>>(l0.nact, l0.meta, l0.code, c->u, c->v, c->w, inv_h[0], inv_h[1], inv_h[2],
                                                        c->partials);
     finalize(c, grid_of(l0.nact), OP_MAX);
   }

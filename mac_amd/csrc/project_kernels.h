@@ -43,29 +43,42 @@ enum FinOp { OP_FF = 0, OP_RR0 = 1, OP_TRUE = 2, OP_DQ = 3, OP_RR = 4, OP_RZ = 5
 __device__ __forceinline__ int lane_id() { return threadIdx.x & 63; }
 __device__ __forceinline__ int wave_in_block() { return threadIdx.x >> 6; }
 
-// Wave-uniform brick loop over the list act[0..n).
-#define FOR_EACH_BRICK(act, n, b)                                                             \
+// Wave-uniform loop over the active bricks of a level.  meta holds, per
+// active brick, MSTRIDE ints: the brick id and its 6 face neighbours
+// (-x,+x,-y,+y,-z,+z; -1 = none), so a wave fetches everything it needs about
+// its brick with one scalar load and no dependent table lookups.
+#define MSTRIDE 8
+#define FOR_EACH_BRICK(meta, n, M)                                                            \
   for (int i_ = blockIdx.x * WPB + wave_in_block(); i_ < (n); i_ += gridDim.x * WPB)         \
-    if (int b = __builtin_amdgcn_readfirstlane((act)[i_]); true)
+    if (const int* M = (meta) + (size_t)MSTRIDE * __builtin_amdgcn_readfirstlane(i_); true)
 
 // Walk the 8 cells of a lane's column with the 6 neighbour values of each.
 // col holds the lane's own values; ld(brick, cell) fetches a value from a
 // neighbour brick (already composed, e.g. z + beta d); f(z, xm, xp, ym, yp, zm, zp).
+// The four side faces of the brick are fetched lane-parallel, one full
+// 64-value face plane per load (x faces indexed (y, z), y faces (x, z)), and
+// handed to the edge lanes by cross-lane shuffles; the z faces are ordinary
+// rows.  The neighbour-brick tests are wave-uniform, so nothing diverges.
 template <typename T, typename Ld, typename F>
-__device__ __forceinline__ void col_stencil(const int* __restrict__ nbr, int b, int lane, const T (&col)[8], Ld ld, F f) {
+__device__ __forceinline__ void col_stencil(const int* M, int lane, const T (&col)[8], Ld ld, F f) {
   const int x = lane & 7, y = lane >> 3;
-  const int* nb = nbr + (size_t)b * 27;
-  const int bxm = nb[12], bxp = nb[14], bym = nb[10], byp = nb[16], bzm = nb[4], bzp = nb[22];
+  const int bxm = M[1], bxp = M[2], bym = M[3], byp = M[4], bzm = M[5], bzp = M[6];
+  const T hxm = (bxm >= 0) ? ld(bxm, 7 + 8 * x + 64 * y) : T(0);  // lane = y' + 8 z'
+  const T hxp = (bxp >= 0) ? ld(bxp, 8 * x + 64 * y) : T(0);
+  const T hym = (bym >= 0) ? ld(bym, x + 56 + 64 * y) : T(0);     // lane = x' + 8 z'
+  const T hyp = (byp >= 0) ? ld(byp, x + 64 * y) : T(0);
   const T below = (bzm >= 0) ? ld(bzm, lane + 448) : T(0);
   const T above = (bzp >= 0) ? ld(bzp, lane) : T(0);
 #pragma unroll
   for (int z = 0; z < 8; ++z) {
     T v = col[z];
     T xm = __shfl_up(v, 1), xp = __shfl_down(v, 1), ym = __shfl_up(v, 8), yp = __shfl_down(v, 8);
-    if (x == 0) xm = (bxm >= 0) ? ld(bxm, 7 + 8 * y + 64 * z) : T(0);
-    if (x == 7) xp = (bxp >= 0) ? ld(bxp, 8 * y + 64 * z) : T(0);
-    if (y == 0) ym = (bym >= 0) ? ld(bym, x + 56 + 64 * z) : T(0);
-    if (y == 7) yp = (byp >= 0) ? ld(byp, x + 64 * z) : T(0);
+    const T gxm = __shfl(hxm, y + 8 * z), gxp = __shfl(hxp, y + 8 * z);
+    const T gym = __shfl(hym, x + 8 * z), gyp = __shfl(hyp, x + 8 * z);
+    if (x == 0) xm = gxm;
+    if (x == 7) xp = gxp;
+    if (y == 0) ym = gym;
+    if (y == 7) yp = gyp;
     T zm = (z > 0) ? col[z > 0 ? z - 1 : 0] : below;
     T zp = (z < 7) ? col[z < 7 ? z + 1 : 7] : above;
     f(z, xm, xp, ym, yp, zm, zp);
@@ -253,25 +266,26 @@ struct DivCol {
   double d[8];
 };
 
-__device__ __forceinline__ DivCol div_column(const int* __restrict__ nbr, int b, int lane, const double* __restrict__ u,
-                                             const double* __restrict__ v, const double* __restrict__ w, double ihx, double ihy,
-                                             double ihz) {
+__device__ __forceinline__ DivCol div_column(const int* M, int lane, const double* __restrict__ u, const double* __restrict__ v,
+                                             const double* __restrict__ w, double ihx, double ihy, double ihz) {
 #pragma clang fp contract(off)
   const int x = lane & 7, y = lane >> 3;
-  const size_t base = (size_t)b * MAC_BC + lane;
-  const int* nb = nbr + (size_t)b * 27;
-  const int bxp = nb[14], byp = nb[16], bzp = nb[22];
+  const size_t base = (size_t)M[0] * MAC_BC + lane;
+  const int bxp = M[2], byp = M[4], bzp = M[6];
   double wc[8];
 #pragma unroll
   for (int z = 0; z < 8; ++z) wc[z] = w[base + 64 * z];
   const double wtop = (bzp >= 0) ? w[(size_t)bzp * MAC_BC + lane] : 0.0;
+  const double hup = (bxp >= 0) ? u[(size_t)bxp * MAC_BC + 8 * x + 64 * y] : 0.0;  // +x face, lane = y' + 8 z'
+  const double hvp = (byp >= 0) ? v[(size_t)byp * MAC_BC + x + 64 * y] : 0.0;      // +y face, lane = x' + 8 z'
   DivCol out;
 #pragma unroll
   for (int z = 0; z < 8; ++z) {
     double uc = u[base + 64 * z], vc = v[base + 64 * z];
     double up = __shfl_down(uc, 1), vp = __shfl_down(vc, 8);
-    if (x == 7) up = (bxp >= 0) ? u[(size_t)bxp * MAC_BC + 8 * y + 64 * z] : 0.0;
-    if (y == 7) vp = (byp >= 0) ? v[(size_t)byp * MAC_BC + x + 64 * z] : 0.0;
+    const double gu = __shfl(hup, y + 8 * z), gv = __shfl(hvp, x + 8 * z);
+    if (x == 7) up = gu;
+    if (y == 7) vp = gv;
     double wp = (z < 7) ? wc[z < 7 ? z + 1 : 7] : wtop;
     double ret = 0.0;
     ret -= uc * ihx;
@@ -287,7 +301,7 @@ __device__ __forceinline__ DivCol div_column(const int* __restrict__ nbr, int b,
 
 // f = -div(u) on in-domain cells (L p = f with L = -A), div optionally kept;
 // partial ||f||^2.
-__global__ void __launch_bounds__(KTHREADS) k_div(int nact, const int* __restrict__ act, const int* __restrict__ nbr,
+__global__ void __launch_bounds__(KTHREADS) k_div(int nact, const int* __restrict__ meta,
                                                   const uint8_t* __restrict__ code, const double* __restrict__ u,
                                                   const double* __restrict__ v, const double* __restrict__ w, double ihx,
                                                   double ihy, double ihz, double* __restrict__ f, double* __restrict__ div_out,
@@ -295,8 +309,9 @@ __global__ void __launch_bounds__(KTHREADS) k_div(int nact, const int* __restric
   __shared__ double sh[16];
   const int lane = lane_id();
   double acc = 0.0;
-  FOR_EACH_BRICK(act, nact, b) {
-    DivCol dc = div_column(nbr, b, lane, u, v, w, ihx, ihy, ihz);
+  FOR_EACH_BRICK(meta, nact, M) {
+    const int b = M[0];
+    DivCol dc = div_column(M, lane, u, v, w, ihx, ihy, ihz);
     const size_t base = (size_t)b * MAC_BC + lane;
 #pragma unroll
     for (int z = 0; z < 8; ++z) {
@@ -311,15 +326,16 @@ __global__ void __launch_bounds__(KTHREADS) k_div(int nact, const int* __restric
 }
 
 // max |div u| over the domain (the reference's post-check, :209-219).
-__global__ void __launch_bounds__(KTHREADS) k_maxdiv(int nact, const int* __restrict__ act, const int* __restrict__ nbr,
+__global__ void __launch_bounds__(KTHREADS) k_maxdiv(int nact, const int* __restrict__ meta,
                                                      const uint8_t* __restrict__ code, const double* __restrict__ u,
                                                      const double* __restrict__ v, const double* __restrict__ w, double ihx,
                                                      double ihy, double ihz, double* __restrict__ partials) {
   __shared__ double sh[16];
   const int lane = lane_id();
   double m = 0.0;
-  FOR_EACH_BRICK(act, nact, b) {
-    DivCol dc = div_column(nbr, b, lane, u, v, w, ihx, ihy, ihz);
+  FOR_EACH_BRICK(meta, nact, M) {
+    const int b = M[0];
+    DivCol dc = div_column(M, lane, u, v, w, ihx, ihy, ihz);
     const size_t base = (size_t)b * MAC_BC + lane;
 #pragma unroll
     for (int z = 0; z < 8; ++z)
@@ -329,14 +345,15 @@ __global__ void __launch_bounds__(KTHREADS) k_maxdiv(int nact, const int* __rest
 }
 
 // r = f - L x and partial ||r||^2.  write_r = 0: true-residual check only.
-__global__ void __launch_bounds__(KTHREADS) k_resid0(int nact, const int* __restrict__ act, const int* __restrict__ nbr,
+__global__ void __launch_bounds__(KTHREADS) k_resid0(int nact, const int* __restrict__ meta,
                                                      const uint8_t* __restrict__ code, const double* __restrict__ f,
                                                      const double* __restrict__ xv, double* __restrict__ r, double ihx2,
                                                      double ihy2, double ihz2, int write_r, double* __restrict__ partials) {
   __shared__ double sh[16];
   const int lane = lane_id();
   double acc = 0.0;
-  FOR_EACH_BRICK(act, nact, b) {
+  FOR_EACH_BRICK(meta, nact, M) {
+    const int b = M[0];
     const size_t base = (size_t)b * MAC_BC + lane;
     double col[8];
     unsigned cd[8];
@@ -345,7 +362,7 @@ __global__ void __launch_bounds__(KTHREADS) k_resid0(int nact, const int* __rest
       col[z] = xv[base + 64 * z];
       cd[z] = code[base + 64 * z];
     }
-    col_stencil<double>(nbr, b, lane, col, [&](int nb, int cell) { return xv[(size_t)nb * MAC_BC + cell]; },
+    col_stencil<double>(M, lane, col, [&](int nb, int cell) { return xv[(size_t)nb * MAC_BC + cell]; },
                         [&](int z, double xm, double xp, double ym, double yp, double zm, double zp) {
                           if (cd[z] & CODE_DOM) {
                             double dg;
@@ -362,7 +379,7 @@ __global__ void __launch_bounds__(KTHREADS) k_resid0(int nact, const int* __rest
 // d_new = z + beta d_old (formed on the fly at the neighbours too), q = L d_new,
 // partial d_new . q.
 template <typename TZ>
-__global__ void __launch_bounds__(KTHREADS) k_pmatvec(int nact, const int* __restrict__ act, const int* __restrict__ nbr,
+__global__ void __launch_bounds__(KTHREADS) k_pmatvec(int nact, const int* __restrict__ meta,
                                                       const uint8_t* __restrict__ code, const TZ* __restrict__ zv,
                                                       const double* __restrict__ dold, double* __restrict__ dnew,
                                                       double* __restrict__ q, double ihx2, double ihy2, double ihz2,
@@ -372,7 +389,8 @@ __global__ void __launch_bounds__(KTHREADS) k_pmatvec(int nact, const int* __res
   const double beta = s->beta;
   const int lane = lane_id();
   double acc = 0.0;
-  FOR_EACH_BRICK(act, nact, b) {
+  FOR_EACH_BRICK(meta, nact, M) {
+    const int b = M[0];
     const size_t base = (size_t)b * MAC_BC + lane;
     double col[8];
     unsigned cd[8];
@@ -382,7 +400,7 @@ __global__ void __launch_bounds__(KTHREADS) k_pmatvec(int nact, const int* __res
       cd[z] = code[base + 64 * z];
     }
     col_stencil<double>(
-        nbr, b, lane, col,
+        M, lane, col,
         [&](int nb, int cell) { return (double)zv[(size_t)nb * MAC_BC + cell] + beta * dold[(size_t)nb * MAC_BC + cell]; },
         [&](int z, double xm, double xp, double ym, double yp, double zm, double zp) {
           if (cd[z] & CODE_DOM) {
@@ -398,7 +416,7 @@ __global__ void __launch_bounds__(KTHREADS) k_pmatvec(int nact, const int* __res
 }
 
 // x += alpha d, r -= alpha q, partial ||r||^2.
-__global__ void __launch_bounds__(KTHREADS) k_update(int nact, const int* __restrict__ act, const uint8_t* __restrict__ code,
+__global__ void __launch_bounds__(KTHREADS) k_update(int nact, const int* __restrict__ meta, const uint8_t* __restrict__ code,
                                                      double* __restrict__ x, double* __restrict__ r,
                                                      const double* __restrict__ d, const double* __restrict__ q,
                                                      double* __restrict__ partials, const Scal* s) {
@@ -407,7 +425,8 @@ __global__ void __launch_bounds__(KTHREADS) k_update(int nact, const int* __rest
   const double alpha = s->alpha;
   const int lane = lane_id();
   double acc = 0.0;
-  FOR_EACH_BRICK(act, nact, b) {
+  FOR_EACH_BRICK(meta, nact, M) {
+    const int b = M[0];
     const size_t base = (size_t)b * MAC_BC + lane;
 #pragma unroll
     for (int z = 0; z < 8; ++z) {
@@ -426,7 +445,7 @@ __global__ void __launch_bounds__(KTHREADS) k_update(int nact, const int* __rest
 // Damped Jacobi sweep: zout = zin + omega D^-1 (r - L zin); zin == nullptr
 // means a zero start (zout = omega D^-1 r).  want_dot: partial r . zout.
 template <typename TR, typename TZ>
-__global__ void __launch_bounds__(KTHREADS) k_jacobi(int nact, const int* __restrict__ act, const int* __restrict__ nbr,
+__global__ void __launch_bounds__(KTHREADS) k_jacobi(int nact, const int* __restrict__ meta,
                                                      const uint8_t* __restrict__ code, const TR* __restrict__ r,
                                                      const TZ* __restrict__ zin, TZ* __restrict__ zout, TZ ihx2, TZ ihy2,
                                                      TZ ihz2, TZ omega, int want_dot, double* __restrict__ partials,
@@ -435,7 +454,8 @@ __global__ void __launch_bounds__(KTHREADS) k_jacobi(int nact, const int* __rest
   __shared__ double sh[16];
   const int lane = lane_id();
   double acc = 0.0;
-  FOR_EACH_BRICK(act, nact, b) {
+  FOR_EACH_BRICK(meta, nact, M) {
+    const int b = M[0];
     const size_t base = (size_t)b * MAC_BC + lane;
     unsigned cd[8];
 #pragma unroll
@@ -444,7 +464,7 @@ __global__ void __launch_bounds__(KTHREADS) k_jacobi(int nact, const int* __rest
       TZ col[8];
 #pragma unroll
       for (int z = 0; z < 8; ++z) col[z] = zin[base + 64 * z];
-      col_stencil<TZ>(nbr, b, lane, col, [&](int nb, int cell) { return zin[(size_t)nb * MAC_BC + cell]; },
+      col_stencil<TZ>(M, lane, col, [&](int nb, int cell) { return zin[(size_t)nb * MAC_BC + cell]; },
                       [&](int z, TZ xm, TZ xp, TZ ym, TZ yp, TZ zm, TZ zp) {
                         TZ out = TZ(0);
                         if (cd[z] & CODE_DOM) {
@@ -473,15 +493,183 @@ __global__ void __launch_bounds__(KTHREADS) k_jacobi(int nact, const int* __rest
   if (want_dot) write_partial(acc, partials, sh, false);
 }
 
+// Two damped-Jacobi sweeps from a zero start in one pass:
+//   z1 = omega D^-1 r (formed on the fly, also at the halo), zout = z1 + omega D^-1 (r - L z1).
+// Bitwise equal to two k_jacobi launches (same expressions, same order).
+template <typename TR, typename TZ>
+__global__ void __launch_bounds__(KTHREADS) k_jacobi2(int nact, const int* __restrict__ meta,
+                                                      const uint8_t* __restrict__ code, const TR* __restrict__ r,
+                                                      TZ* __restrict__ zout, TZ ihx2, TZ ihy2, TZ ihz2, TZ omega,
+                                                      const Scal* s) {
+  if (s->done) return;
+  const int lane = lane_id();
+  FOR_EACH_BRICK(meta, nact, M) {
+    const int b = M[0];
+    const size_t base = (size_t)b * MAC_BC + lane;
+    unsigned cd[8];
+    TZ z1[8], rv[8];
+#pragma unroll
+    for (int z = 0; z < 8; ++z) {
+      cd[z] = code[base + 64 * z];
+      rv[z] = (TZ)r[base + 64 * z];
+      TZ dg = diag_of<TZ>(cd[z], ihx2, ihy2, ihz2);
+      z1[z] = ((cd[z] & CODE_DOM) && dg > TZ(0)) ? omega * rv[z] / dg : TZ(0);
+    }
+    col_stencil<TZ>(
+        M, lane, z1,
+        [&](int nb, int cell) {
+          const size_t j = (size_t)nb * MAC_BC + cell;
+          unsigned c2 = code[j];
+          TZ d2 = diag_of<TZ>(c2, ihx2, ihy2, ihz2);
+          return ((c2 & CODE_DOM) && d2 > TZ(0)) ? omega * (TZ)r[j] / d2 : TZ(0);
+        },
+        [&](int z, TZ xm, TZ xp, TZ ym, TZ yp, TZ zm, TZ zp) {
+          TZ out = TZ(0);
+          if (cd[z] & CODE_DOM) {
+            TZ dg;
+            TZ lz = lap_cell<TZ>(cd[z], z1[z], xm, xp, ym, yp, zm, zp, ihx2, ihy2, ihz2, dg);
+            out = dg > TZ(0) ? z1[z] + omega * (rv[z] - lz) / dg : TZ(0);
+          }
+          zout[base + 64 * z] = out;
+        });
+  }
+}
+
+// Restriction, separable: r_c = (1/8) W^T res over the coarse domain with the
+// 1-D weights (1/4, 3/4, 3/4, 1/4) on fine cells 2X-1 .. 2X+2.  One block per
+// active coarse brick, in two halves of 4 coarse z-planes: the 18x18x10 fine
+// residual block a half needs is staged in LDS with all loads in flight at
+// once, then reduced along x, y and z in LDS.
+// rmeta per active coarse brick (RSTRIDE ints): brick id, then at +4 the
+// 64 fine bricks covering fine cells -1..16 relative to the coarse brick's
+// first child, fb[bx + 4 by + 16 bz] with b* = 0..3 for fine brick offsets
+// -1..2 (-1 = none).
+#define RSTRIDE 68
+template <typename T>
+__global__ void __launch_bounds__(KTHREADS) k_restrict_sep(const int* __restrict__ rmeta, const uint8_t* __restrict__ codec,
+                                                           const T* __restrict__ resf, T* __restrict__ rc, const Scal* s) {
+  if (s->done) return;
+  constexpr int NF = 10 * 18 * 18;  // [fz][fy][fx], fine x,y in -1..16, 10 fine z per half
+  constexpr int NT1 = 10 * 18 * 8;  // [fz][fy][X]
+  __shared__ int fb[64];
+  __shared__ T F[NF];
+  __shared__ T t1[NT1];
+  T* t2 = F;  // [fz][Y][X], 10*64, after F is consumed
+  const int* M = rmeta + (size_t)RSTRIDE * blockIdx.x;
+  const int C = M[0];
+  if (threadIdx.x < 64) fb[threadIdx.x] = M[4 + threadIdx.x];
+  for (int half = 0; half < 2; ++half) {
+    __syncthreads();
+    T v[(NF + KTHREADS - 1) / KTHREADS];
+#pragma unroll
+    for (int k = 0; k < (NF + KTHREADS - 1) / KTHREADS; ++k) {
+      const int i = threadIdx.x + KTHREADS * k;
+      v[k] = T(0);
+      if (i < NF) {
+        const int gx = i % 18 - 1, gy = (i / 18) % 18 - 1, gz = 8 * half + i / 324 - 1;
+        const int f = fb[((gx + 8) >> 3) + 4 * ((gy + 8) >> 3) + 16 * ((gz + 8) >> 3)];
+        if (f >= 0) v[k] = resf[(size_t)f * MAC_BC + (gx & 7) + 8 * (gy & 7) + 64 * (gz & 7)];
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < (NF + KTHREADS - 1) / KTHREADS; ++k) {
+      const int i = threadIdx.x + KTHREADS * k;
+      if (i < NF) F[i] = v[k];
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < NT1; i += blockDim.x) {
+      const int X = i & 7, fy = (i >> 3) % 18, fz = (i >> 3) / 18;
+      const T* row = &F[(fz * 18 + fy) * 18 + 2 * X];
+      t1[i] = T(0.25) * row[0] + T(0.75) * row[1] + T(0.75) * row[2] + T(0.25) * row[3];
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < 10 * 64; i += blockDim.x) {
+      const int X = i & 7, Y = (i >> 3) & 7, fz = i >> 6;
+      const T* col = &t1[(fz * 18 + 2 * Y) * 8 + X];
+      t2[i] = T(0.25) * col[0] + T(0.75) * col[8] + T(0.75) * col[16] + T(0.25) * col[24];
+    }
+    __syncthreads();
+    {
+      const int t = threadIdx.x;  // 256 = 4 coarse z-planes of 64
+      const int X = t & 7, Y = (t >> 3) & 7, zz = t >> 6;
+      const size_t c = (size_t)C * MAC_BC + X + 8 * Y + 64 * (4 * half + zz);
+      T acc = T(0);
+      if (codec[c] & CODE_DOM) {
+        const T* col = &t2[(2 * zz) * 64 + Y * 8 + X];
+        acc = (T(0.25) * col[0] + T(0.75) * col[64] + T(0.75) * col[128] + T(0.25) * col[192]) * T(0.125);
+      }
+      rc[c] = acc;
+    }
+  }
+}
+
+// Prolongation, column form: z_f += W z_c on the fine domain.  Each lane
+// gathers the 4 coarse columns (X0|X1, Y0|Y1) over the 6 coarse z-levels its
+// 8 fine cells need, then forms the trilinear weights 3/4 (parent side) and
+// 1/4 per direction.  The coarse cells a fine brick touches lie in the parent
+// brick and its neighbours toward the brick's octant (2x2x2 bricks), whose
+// ids are wave-uniform, so every gather is a plain indexed load.
+// pmeta per active fine brick (PSTRIDE ints): brick id, octant in its
+// parent, then the 8 coarse bricks bb[sx + 2 sy + 4 sz] (sx = 1: one step
+// from the parent toward the octant's side in x, etc.; -1 = none).
+#define PSTRIDE 12
+template <typename T>
+__global__ void __launch_bounds__(KTHREADS) k_prolong_col(int nact, const int* __restrict__ pmeta,
+                                                          const uint8_t* __restrict__ codef, const T* __restrict__ zc,
+                                                          T* __restrict__ zf, const Scal* s) {
+  if (s->done) return;
+  const int lane = lane_id();
+  const int x = lane & 7, y = lane >> 3;
+  for (int i_ = blockIdx.x * WPB + wave_in_block(); i_ < nact; i_ += gridDim.x * WPB) {
+    const int* M = pmeta + (size_t)PSTRIDE * __builtin_amdgcn_readfirstlane(i_);
+    const int B = M[0], o = M[1];
+    const int ox = o & 1, oy = (o >> 1) & 1, oz = o >> 2;
+    const int bb0 = M[2], bb1 = M[3], bb2 = M[4], bb3 = M[5], bb4 = M[6], bb5 = M[7], bb6 = M[8], bb7 = M[9];
+    const int bx = 4 * ox, by = 4 * oy, bz = 4 * oz;
+    const int X0 = bx + (x >> 1), Y0 = by + (y >> 1);
+    const int X1 = X0 + ((x & 1) ? 1 : -1), Y1 = Y0 + ((y & 1) ? 1 : -1);
+    const bool sx = (X1 < 0 || X1 > 7), sy = (Y1 < 0 || Y1 > 7);
+    // brick of each of the 4 coarse columns, for z inside / outside the parent
+    const int k00 = bb0, k10 = sx ? bb1 : bb0, k01 = sy ? bb2 : bb0, k11 = sx ? (sy ? bb3 : bb1) : (sy ? bb2 : bb0);
+    const int z00 = bb4, z10 = sx ? bb5 : bb4, z01 = sy ? bb6 : bb4, z11 = sx ? (sy ? bb7 : bb5) : (sy ? bb6 : bb4);
+    auto gather = [&](int brk, int X, int Y, int Z) -> T {
+      return brk >= 0 ? zc[(size_t)brk * MAC_BC + (X & 7) + 8 * (Y & 7) + 64 * (Z & 7)] : T(0);
+    };
+    T c00[6], c10[6], c01[6], c11[6];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+      const int Z = bz - 1 + k;
+      const bool out = (Z < 0 || Z > 7);
+      c00[k] = gather(out ? z00 : k00, X0, Y0, Z);
+      c10[k] = gather(out ? z10 : k10, X1, Y0, Z);
+      c01[k] = gather(out ? z01 : k01, X0, Y1, Z);
+      c11[k] = gather(out ? z11 : k11, X1, Y1, Z);
+    }
+    const size_t base = (size_t)B * MAC_BC + lane;
+#pragma unroll
+    for (int z = 0; z < 8; ++z) {
+      const size_t c = base + 64 * z;
+      if (!(codef[c] & CODE_DOM)) continue;
+      const int k0 = (z >> 1) + 1, k1 = (z & 1) ? k0 + 1 : k0 - 1;
+      const T a00 = T(0.75) * c00[k0] + T(0.25) * c00[k1];
+      const T a10 = T(0.75) * c10[k0] + T(0.25) * c10[k1];
+      const T a01 = T(0.75) * c01[k0] + T(0.25) * c01[k1];
+      const T a11 = T(0.75) * c11[k0] + T(0.25) * c11[k1];
+      zf[c] += T(0.75) * (T(0.75) * a00 + T(0.25) * a01) + T(0.25) * (T(0.75) * a10 + T(0.25) * a11);
+    }
+  }
+}
+
 // res = r - L z.
 template <typename TR, typename TZ>
-__global__ void __launch_bounds__(KTHREADS) k_residual(int nact, const int* __restrict__ act, const int* __restrict__ nbr,
+__global__ void __launch_bounds__(KTHREADS) k_residual(int nact, const int* __restrict__ meta,
                                                        const uint8_t* __restrict__ code, const TR* __restrict__ r,
                                                        const TZ* __restrict__ zv, TZ* __restrict__ res, TZ ihx2, TZ ihy2,
                                                        TZ ihz2, const Scal* s) {
   if (s->done) return;
   const int lane = lane_id();
-  FOR_EACH_BRICK(act, nact, b) {
+  FOR_EACH_BRICK(meta, nact, M) {
+    const int b = M[0];
     const size_t base = (size_t)b * MAC_BC + lane;
     unsigned cd[8];
     TZ col[8];
@@ -490,7 +678,7 @@ __global__ void __launch_bounds__(KTHREADS) k_residual(int nact, const int* __re
       cd[z] = code[base + 64 * z];
       col[z] = zv[base + 64 * z];
     }
-    col_stencil<TZ>(nbr, b, lane, col, [&](int nb, int cell) { return zv[(size_t)nb * MAC_BC + cell]; },
+    col_stencil<TZ>(M, lane, col, [&](int nb, int cell) { return zv[(size_t)nb * MAC_BC + cell]; },
                     [&](int z, TZ xm, TZ xp, TZ ym, TZ yp, TZ zm, TZ zp) {
                       TZ out = TZ(0);
                       if (cd[z] & CODE_DOM) {
@@ -499,73 +687,6 @@ __global__ void __launch_bounds__(KTHREADS) k_residual(int nact, const int* __re
                       }
                       res[base + 64 * z] = out;
                     });
-  }
-}
-
-// r_c = (1/8) W^T res_f over the coarse domain: cell-centred full weighting,
-// weights (1/4, 3/4, 3/4, 1/4) per direction over fine cells 2X-1 .. 2X+2.
-template <typename T>
-__global__ void k_restrict(const int* __restrict__ actc, const int* __restrict__ nbrc, const int* __restrict__ childc,
-                           const uint8_t* __restrict__ codec, const T* __restrict__ resf, T* __restrict__ rc, const Scal* s) {
-  if (s->done) return;
-  __shared__ int fb[64];
-  int C = actc[blockIdx.x];
-  if (threadIdx.x < 64) {
-    int rx = (threadIdx.x & 3) - 1, ry = ((threadIdx.x >> 2) & 3) - 1, rz = (threadIdx.x >> 4) - 1;
-    int cn = nbrc[C * 27 + nbr_slot(rx >> 1, ry >> 1, rz >> 1)];
-    fb[threadIdx.x] = cn >= 0 ? childc[cn * 8 + (rx & 1) + 2 * (ry & 1) + 4 * (rz & 1)] : -1;
-  }
-  __syncthreads();
-  const T wt[4] = {T(0.25), T(0.75), T(0.75), T(0.25)};
-  for (int t = threadIdx.x; t < MAC_BC; t += blockDim.x) {
-    int X = t & 7, Y = (t >> 3) & 7, Z = t >> 6;
-    size_t c = (size_t)C * MAC_BC + t;
-    T acc = T(0);
-    if (codec[c] & CODE_DOM) {
-      for (int dz = 0; dz < 4; ++dz) {
-        int gz = 2 * Z + dz - 1;
-        int bz = ((gz + 8) >> 3);
-        for (int dy = 0; dy < 4; ++dy) {
-          int gy = 2 * Y + dy - 1;
-          int by = ((gy + 8) >> 3);
-          T wyz = wt[dy] * wt[dz];
-          for (int dx = 0; dx < 4; ++dx) {
-            int gx = 2 * X + dx - 1;
-            int bx = ((gx + 8) >> 3);
-            int f = fb[bx + 4 * by + 16 * bz];
-            if (f >= 0) acc += wt[dx] * wyz * resf[(size_t)f * MAC_BC + (gx & 7) + 8 * (gy & 7) + 64 * (gz & 7)];
-          }
-        }
-      }
-      acc *= T(0.125);
-    }
-    rc[c] = acc;
-  }
-}
-
-// z_f += W z_c on the fine domain: cell-centred trilinear interpolation,
-// weight 3/4 from the parent cell and 1/4 from its neighbour toward the
-// fine cell, per direction.
-template <typename T>
-__global__ void k_prolong(const int* __restrict__ actf, const int* __restrict__ parent, const int* __restrict__ oct,
-                          const int* __restrict__ nbrc, const uint8_t* __restrict__ codef, const T* __restrict__ zc,
-                          T* __restrict__ zf, const Scal* s) {
-  if (s->done) return;
-  int B = actf[blockIdx.x];
-  int P = parent[B], o = oct[B];
-  int bx = 4 * (o & 1), by = 4 * ((o >> 1) & 1), bz = 4 * (o >> 2);
-  for (int t = threadIdx.x; t < MAC_BC; t += blockDim.x) {
-    int x = t & 7, y = (t >> 3) & 7, z = t >> 6;
-    size_t c = (size_t)B * MAC_BC + t;
-    if (!(codef[c] & CODE_DOM)) continue;
-    int X0 = bx + (x >> 1), Y0 = by + (y >> 1), Z0 = bz + (z >> 1);
-    int X1 = X0 + ((x & 1) ? 1 : -1), Y1 = Y0 + ((y & 1) ? 1 : -1), Z1 = Z0 + ((z & 1) ? 1 : -1);
-    T acc = T(0);
-    acc += T(27.0 / 64.0) * ld_rel(zc, nbrc, P, X0, Y0, Z0);
-    acc += T(9.0 / 64.0) * (ld_rel(zc, nbrc, P, X1, Y0, Z0) + ld_rel(zc, nbrc, P, X0, Y1, Z0) + ld_rel(zc, nbrc, P, X0, Y0, Z1));
-    acc += T(3.0 / 64.0) * (ld_rel(zc, nbrc, P, X1, Y1, Z0) + ld_rel(zc, nbrc, P, X1, Y0, Z1) + ld_rel(zc, nbrc, P, X0, Y1, Z1));
-    acc += T(1.0 / 64.0) * ld_rel(zc, nbrc, P, X1, Y1, Z1);
-    zf[c] += acc;
   }
 }
 

@@ -154,7 +154,7 @@ def project(grid, rtol=1e-14):
     return {"p": p, "u": u, "v": v, "w": w, "max_div": md.value, "iters": it}
 
 
-def mgpcg(grid, rtol=1e-8, maxit=500, nu=2, omega=0.8, threads=0):
+def mgpcg(grid, rtol=1e-8, maxit=500, nu=2, omega=0.9, threads=0):
     """CPU MG-PCG pressure (the same algorithm as the device solver).
     Returns dict(p, iters, ms_solve, ms_setup, true_rel_resid, levels, ok)."""
     keys = _c(grid["keys"], np.int32)

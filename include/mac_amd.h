@@ -153,6 +153,87 @@ int mac_reset_kernel_times(mac_ctx* ctx);
 /* Synchronise the context's stream. */
 int mac_sync(mac_ctx* ctx);
 
+/* ===================================================================== */
+/* Per-timestep MAC-grid loop on a sparse device brick grid.              */
+/* ===================================================================== */
+/* The grid is a device hash of chunk keys -> 8^3 bricks (ChunkHashTable,
+ * ChunkHashTable.tpp:63-161) with a 27-neighbour brick table; bricks are
+ * created when particles or air layers reach them and freed when their last
+ * cell is deleted.  Flag byte as above plus bit 6 = NoDelete
+ * (Data_CellType_Fluid_Air_Solid_Inflow_Interior_0_Pres.h:354-361).  Per
+ * cell: layer (int8, -1 = empty, Data_Grid_Layer_Empty), two speed slots and
+ * one acceleration slot (Data_Chunk_Speed.h:5-96), pressure.  Particles are
+ * positions (x,y,z), FP64. */
+#define MAC_FLAG_NODELETE 0x40u
+
+typedef struct mac_sim mac_sim;
+
+typedef struct {
+  double h[3];
+  double gravity[3];    /* Policy_Gravity(g, i) for every axis i with g != 0          */
+  double viscosity;     /* nu of Algorithms_Viscosity; 0 disables the term             */
+  double cfl;           /* m_factor of Algorithms_Calculate_Time_Step.h:35-66          */
+  double dt_min, dt_max;/* Policy_CheckDT.h:16-29                                      */
+  int layer_max;        /* Policy_Layer_Max: air layers built around the fluid         */
+  int integrator;       /* 0 = RK2 midpoint (Algorithms_RungeKutta_RK2.h:20-64),
+                           1 = RK2-TVD (Algorithms_RungeKutta_RK2_TVD.h:20-66)        */
+  int particle_rk;      /* 2 = RK2 midpoint, 4 = classical RK4
+                           (Policy_Advance_ODE_RungeKutta.h:15-27)                    */
+  int particles_per_inflow_cell; /* seeded-jitter fill of empty inflow cells           */
+  unsigned long long seed;
+  double box_lo[3], box_hi[3];   /* particles leaving [lo, hi) are deleted (lo == hi: none) */
+  int max_particles;
+  int max_bricks;
+  mac_params proj;      /* projection solver settings                                  */
+} mac_sim_params;
+
+typedef struct {
+  double dt;            /* time step used                                              */
+  int nbricks;          /* bricks after the topology update                            */
+  int nparticles;       /* particles after the move                                    */
+  long long nfluid;     /* in-domain cells                                             */
+  int pcg_iters[2];     /* PCG iterations of the two projections                       */
+  double max_div;       /* max |div u| after the last projection                       */
+  double ms_topology, ms_forces, ms_project, ms_particles, ms_total;
+} mac_sim_stats;
+
+void mac_sim_default_params(mac_sim_params* p);
+int mac_sim_create(mac_sim** out, const mac_sim_params* p);
+void mac_sim_destroy(mac_sim* sim);
+/* Upload a state: bricks (keys, flags, optional layers, speed slot 0) and particles
+ * (pxyz interleaved x,y,z per particle). */
+int mac_sim_upload(mac_sim* sim, int nbricks, const int32_t* keys3, const uint8_t* flags, const int8_t* layer,
+                   const double* u, const double* v, const double* w, int nparticles, const double* pxyz);
+/* Bricks and particles currently held. */
+int mac_sim_counts(mac_sim* sim, int* nbricks, int* nparticles);
+/* Download the state; bricks come in ascending (z, y, x) chunk-key order.  Any
+ * pointer may be NULL.  a* = acceleration slot, p = pressure. */
+int mac_sim_download(mac_sim* sim, int32_t* keys3, uint8_t* flags, int8_t* layer, double* u, double* v, double* w,
+                     double* ax, double* ay, double* az, double* p, double* pxyz);
+/* The cell-type/layer update of one step, in the order of the reference's
+ * init chain (TestJetDEau_Basic.cpp:324-326): Algorithms_Initialize_MacCell,
+ * Algorithms_Layer_Initial_With_Particle (particle -> key = lround(x/h),
+ * Policy_Particle_To_Key.h:24-35), Algorithms_Create_Fluid_Particle (empty
+ * inflow cells get particles), Algorithms_Update_CellType_Layer (air layers
+ * 1..layer_max by BFS, then the +1-direction extra layer),
+ * Algorithms_Delete_MacCell (layer-empty cells erased, empty bricks freed). */
+int mac_sim_topology(mac_sim* sim);
+/* Algorithms_Calculate_Time_Step + Policy_CheckDT. */
+int mac_sim_compute_dt(mac_sim* sim, double* dt);
+/* acceleration := 0, then gravity, viscosity and first-order upwind
+ * convection of speed slot `slot` (Algorithms_Gravity/Viscosity/Convection). */
+int mac_sim_forces(mac_sim* sim, int slot);
+/* One RK2 / RK2-TVD velocity step with both projections. */
+int mac_sim_advance(mac_sim* sim, double dt, mac_sim_stats* stats);
+/* Algorithms_Move_Particles with the interpolated velocity of slot 0, then
+ * the box deletion. */
+int mac_sim_move_particles(mac_sim* sim, double dt);
+/* Policy_Speed_Interpolation_Linear_Symmetric::Get_Speed of slot 0 at n
+ * positions (xyz interleaved in and out). */
+int mac_sim_interpolate(mac_sim* sim, int n, const double* pos, double* vel);
+/* One whole time step: topology, dt, advance, move particles. */
+int mac_sim_step(mac_sim* sim, mac_sim_stats* stats);
+
 #ifdef __cplusplus
 }
 #endif

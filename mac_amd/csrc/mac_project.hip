@@ -18,6 +18,7 @@
 #include <vector>
 
 #include "../../include/mac_amd.h"
+#include "mac_internal.h"
 #include "project_kernels.h"
 
 namespace {
@@ -364,7 +365,7 @@ int build_grid(mac_ctx* c, int nbricks, const int32_t* keys3, const uint8_t* fla
   }
   size_t n0 = (size_t)nbricks * MAC_BC;
   CHK(dalloc(c, &c->flags, n0));
-  HIPCHK(hipMemcpyAsync(c->flags, flags, n0, hipMemcpyHostToDevice, c->st));
+  HIPCHK(hipMemcpyAsync(c->flags, flags, n0, hipMemcpyDefault, c->st));
   Level& l0 = c->L[0];
   k_code0<<<nbricks, KTHREADS, 0, c->st>>>(nbricks, l0.nbr, c->flags, l0.code, l0.lab);
   HIPCHK(hipGetLastError());
@@ -987,6 +988,33 @@ int mac_max_divergence(mac_ctx* c, double* out) {
   CHK(read_scal(c));
   *out = c->hs->maxdiv;
   return 0;
+}
+
+MAC_HIDDEN int mac_internal_set_error(int code, const char* msg) { return fail(code, msg); }
+
+MAC_HIDDEN int mac_internal_build(mac_ctx* c, int nbricks, const int32_t* keys3, const uint8_t* flags) {
+  if (!c || nbricks <= 0) return fail(MAC_E_ARG, "bad build arguments");
+  return build_grid(c, nbricks, keys3, flags);
+}
+
+MAC_HIDDEN int mac_internal_project(mac_ctx* c, double* u, double* v, double* w, double* p, int guess, mac_stats* st) {
+  CHK(ensure(c));
+  size_t n = (size_t)c->nb0 * MAC_BC * sizeof(double);
+  HIPCHK(hipMemcpyAsync(c->u, u, n, hipMemcpyDeviceToDevice, c->st));
+  HIPCHK(hipMemcpyAsync(c->v, v, n, hipMemcpyDeviceToDevice, c->st));
+  HIPCHK(hipMemcpyAsync(c->w, w, n, hipMemcpyDeviceToDevice, c->st));
+  if (guess) {
+    HIPCHK(hipMemcpyAsync(c->x, p, n, hipMemcpyDeviceToDevice, c->st));
+    k_mask<<<c->nb0, KTHREADS, 0, c->st>>>(c->nb0, c->L[0].code, c->x);
+  }
+  int rc = mac_project(c, guess, st);
+  if (rc != 0 && rc != MAC_E_NOCONV) return rc;
+  HIPCHK(hipMemcpyAsync(u, c->u, n, hipMemcpyDeviceToDevice, c->st));
+  HIPCHK(hipMemcpyAsync(v, c->v, n, hipMemcpyDeviceToDevice, c->st));
+  HIPCHK(hipMemcpyAsync(w, c->w, n, hipMemcpyDeviceToDevice, c->st));
+  HIPCHK(hipMemcpyAsync(p, c->x, n, hipMemcpyDeviceToDevice, c->st));
+  HIPCHK(hipStreamSynchronize(c->st));
+  return rc;
 }
 
 int mac_enable_kernel_timing(mac_ctx* c, int on) {

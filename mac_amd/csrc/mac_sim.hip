@@ -1,0 +1,602 @@
+// mac_sim.hip -- the per-timestep MAC-grid loop on a sparse device brick
+// grid: host-side brick bookkeeping and the C-ABI (mac_sim_*).
+//
+// The host owns the authoritative list of chunk keys, kept in ascending
+// (z, y, x) order so that brick ids -- and with them every reduction order --
+// are reproducible.  All cell and particle data live on the device; the only
+// host traffic per step is the list of bricks to create (keys of particle
+// bricks that are missing, plus per-brick flags) and the per-brick cell
+// counts after the delete pass.  Lookups from particles go through a device
+// hash of chunk keys (sim_kernels.h), stencils through the 27-neighbour table.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <set>
+#include <string>
+#include <tuple>
+#include <unordered_map>
+#include <vector>
+
+#include "mac_internal.h"
+#include "sim_kernels.h"
+
+namespace {
+
+#define SCHK(expr)                                                                               \
+  do {                                                                                           \
+    hipError_t e_ = (expr);                                                                      \
+    if (e_ != hipSuccess)                                                                        \
+      return mac_internal_set_error(MAC_E_HIP, (std::string(#expr) + ": " + hipGetErrorString(e_)).c_str()); \
+  } while (0)
+#define SRC(expr)             \
+  do {                        \
+    int rc_ = (expr);         \
+    if (rc_ != 0) return rc_; \
+  } while (0)
+
+struct Key3 {
+  int x, y, z;
+  bool operator<(const Key3& o) const { return std::tie(z, y, x) < std::tie(o.z, o.y, o.x); }
+  bool operator==(const Key3& o) const { return x == o.x && y == o.y && z == o.z; }
+};
+
+inline int ceil_pow2(int v) {
+  int c = 1;
+  while (c < v) c <<= 1;
+  return c;
+}
+
+}  // namespace
+
+struct mac_sim {
+  mac_sim_params prm;
+  hipStream_t st = nullptr;
+  mac_ctx* proj = nullptr;
+  std::vector<Key3> keys;
+  int nb = 0, capb = 0;
+  // device brick arrays (capacity capb)
+  int* d_keys = nullptr;
+  uint8_t* flags = nullptr;
+  int8_t* layer = nullptr;
+  double* S[2][3] = {{nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr}};
+  double* A[3] = {nullptr, nullptr, nullptr};
+  double* P = nullptr;
+  int* nbr = nullptr;
+  uint64_t* hk = nullptr;
+  int* hv = nullptr;
+  int hcap = 0;
+  // particles (SoA, capacity capp) and the second buffer for compaction
+  double *px = nullptr, *py = nullptr, *pz = nullptr, *qx = nullptr, *qy = nullptr, *qz = nullptr;
+  int np = 0, capp = 0;
+  // scratch
+  int* d_int = nullptr;  // [0] particle count, [1] missing count, [2] overflow, [3] kept
+  int* d_missing = nullptr;
+  int maxmissing = 0;
+  int* d_bflag = nullptr;
+  int* d_from = nullptr;
+  double* d_part = nullptr;
+  unsigned long long tick = 0;
+  bool grid_ready = false;
+};
+
+namespace {
+
+void sim_free_bricks(mac_sim* s) {
+  hipFree(s->d_keys);
+  hipFree(s->flags);
+  hipFree(s->layer);
+  for (int a = 0; a < 2; ++a)
+    for (int i = 0; i < 3; ++i) hipFree(s->S[a][i]);
+  for (int i = 0; i < 3; ++i) hipFree(s->A[i]);
+  hipFree(s->P);
+  hipFree(s->nbr);
+  hipFree(s->hk);
+  hipFree(s->hv);
+  hipFree(s->d_bflag);
+  hipFree(s->d_from);
+  hipFree(s->d_part);
+}
+
+// Replace the brick set by `nk` (sorted, unique).  Bricks present before keep
+// their data; new ones start as the prototype.  Rebuilds the hash and the
+// neighbour table.
+int sim_set_bricks(mac_sim* s, const std::vector<Key3>& nk) {
+  if ((int)nk.size() > s->prm.max_bricks) return mac_internal_set_error(MAC_E_ARG, "brick pool exhausted (max_bricks)");
+  const int nbn = (int)nk.size();
+  std::vector<int> from(std::max(nbn, 1), -1);
+  {
+    size_t i = 0, j = 0;
+    while (j < nk.size()) {
+      while (i < s->keys.size() && s->keys[i] < nk[j]) ++i;
+      from[j] = (i < s->keys.size() && s->keys[i] == nk[j]) ? (int)i : -1;
+      ++j;
+    }
+  }
+  const size_t cells = (size_t)std::max(nbn, 1) * MAC_BC;
+  int* d_keys;
+  uint8_t* flags;
+  int8_t* layer;
+  double* S[2][3];
+  double* A[3];
+  double* P;
+  SCHK(hipMalloc(&d_keys, sizeof(int) * 3 * std::max(nbn, 1)));
+  SCHK(hipMalloc(&flags, cells));
+  SCHK(hipMalloc(&layer, cells));
+  for (int a = 0; a < 2; ++a)
+    for (int i = 0; i < 3; ++i) SCHK(hipMalloc(&S[a][i], cells * sizeof(double)));
+  for (int i = 0; i < 3; ++i) SCHK(hipMalloc(&A[i], cells * sizeof(double)));
+  SCHK(hipMalloc(&P, cells * sizeof(double)));
+  std::vector<int> hk3(3 * std::max(nbn, 1), 0);
+  for (int b = 0; b < nbn; ++b) {
+    hk3[3 * b] = nk[b].x;
+    hk3[3 * b + 1] = nk[b].y;
+    hk3[3 * b + 2] = nk[b].z;
+  }
+  SCHK(hipMemcpyAsync(d_keys, hk3.data(), sizeof(int) * hk3.size(), hipMemcpyHostToDevice, s->st));
+  int* d_from;
+  SCHK(hipMalloc(&d_from, sizeof(int) * from.size()));
+  SCHK(hipMemcpyAsync(d_from, from.data(), sizeof(int) * from.size(), hipMemcpyHostToDevice, s->st));
+  if (nbn > 0) {
+    k_relayout_u8<<<nbn, 256, 0, s->st>>>(nbn, d_from, s->flags, flags, (uint8_t)0);
+    k_relayout_u8<<<nbn, 256, 0, s->st>>>(nbn, d_from, (const uint8_t*)s->layer, (uint8_t*)layer, (uint8_t)0xFF);
+    for (int a = 0; a < 2; ++a)
+      for (int i = 0; i < 3; ++i) k_relayout_f64<<<nbn, 256, 0, s->st>>>(nbn, d_from, s->S[a][i], S[a][i]);
+    for (int i = 0; i < 3; ++i) k_relayout_f64<<<nbn, 256, 0, s->st>>>(nbn, d_from, s->A[i], A[i]);
+    k_relayout_f64<<<nbn, 256, 0, s->st>>>(nbn, d_from, s->P, P);
+  }
+  SCHK(hipGetLastError());
+  SCHK(hipStreamSynchronize(s->st));
+  sim_free_bricks(s);
+  s->d_keys = d_keys;
+  s->flags = flags;
+  s->layer = layer;
+  for (int a = 0; a < 2; ++a)
+    for (int i = 0; i < 3; ++i) s->S[a][i] = S[a][i];
+  for (int i = 0; i < 3; ++i) s->A[i] = A[i];
+  s->P = P;
+  s->d_from = d_from;
+  s->keys = nk;
+  s->nb = nbn;
+  // hash (load factor <= 1/2) and neighbour table
+  s->hcap = ceil_pow2(std::max(2 * nbn, 64));
+  SCHK(hipMalloc(&s->hk, sizeof(uint64_t) * s->hcap));
+  SCHK(hipMalloc(&s->hv, sizeof(int) * s->hcap));
+  SCHK(hipMalloc(&s->nbr, sizeof(int) * 27 * std::max(nbn, 1)));
+  SCHK(hipMalloc(&s->d_bflag, sizeof(int) * std::max(nbn, 1)));
+  SCHK(hipMalloc(&s->d_part, sizeof(double) * std::max(nbn, 1)));
+  k_hash_clear<<<(s->hcap + 255) / 256, 256, 0, s->st>>>(s->hcap, s->hk, s->hv);
+  if (nbn > 0) {
+    k_hash_insert<<<(nbn + 255) / 256, 256, 0, s->st>>>(nbn, s->d_keys, s->hk, s->hv, s->hcap);
+    k_build_nbr<<<(nbn * 27 + 255) / 256, 256, 0, s->st>>>(nbn, s->d_keys, s->hk, s->hv, s->hcap, s->nbr);
+  }
+  SCHK(hipGetLastError());
+  SCHK(hipStreamSynchronize(s->st));
+  s->grid_ready = false;  // the projection hierarchy must be rebuilt
+  return 0;
+}
+
+int sim_alloc_particles(mac_sim* s, int cap) {
+  s->capp = std::max(cap, 1);
+  for (double** p : {&s->px, &s->py, &s->pz, &s->qx, &s->qy, &s->qz}) SCHK(hipMalloc(p, sizeof(double) * s->capp));
+  return 0;
+}
+
+int sim_topology(mac_sim* s) {
+  const mac_sim_params& P = s->prm;
+  SimIH ih{{1.0 / P.h[0], 1.0 / P.h[1], 1.0 / P.h[2]}};
+  // Algorithms_Initialize_MacCell
+  if (s->nb > 0) k_reset_layer<<<1024, 256, 0, s->st>>>((size_t)s->nb * MAC_BC, s->layer);
+  // bricks needed this step: missing particle bricks, and the 27-neighbourhood
+  // of every brick that will hold a layer-0 cell (particles, inflow); the air
+  // layers reach at most layer_max + 1 <= 8 cells, so they stay inside it
+  SCHK(hipMemsetAsync(s->d_int, 0, 4 * sizeof(int), s->st));
+  if (s->nb > 0) {
+    SCHK(hipMemsetAsync(s->d_bflag, 0, sizeof(int) * s->nb, s->st));
+    k_inflow_bricks<<<s->nb, 256, 0, s->st>>>(s->nb, s->flags, s->d_bflag);
+  }
+  if (s->np > 0)
+    k_particle_bricks<<<(s->np + 255) / 256, 256, 0, s->st>>>(s->np, s->px, s->py, s->pz, ih, s->hk, s->hv, s->hcap, s->d_bflag,
+                                                             s->d_missing, s->d_int + 1, s->maxmissing);
+  SCHK(hipGetLastError());
+  int cnt[4];
+  SCHK(hipMemcpyAsync(cnt, s->d_int, sizeof(cnt), hipMemcpyDeviceToHost, s->st));
+  std::vector<int> bflag(s->nb);
+  if (s->nb > 0) SCHK(hipMemcpyAsync(bflag.data(), s->d_bflag, sizeof(int) * s->nb, hipMemcpyDeviceToHost, s->st));
+  SCHK(hipStreamSynchronize(s->st));
+  int nmiss = std::min(cnt[1], s->maxmissing);
+  std::vector<int> miss(3 * std::max(nmiss, 1));
+  if (nmiss > 0) {
+    SCHK(hipMemcpyAsync(miss.data(), s->d_missing, sizeof(int) * 3 * nmiss, hipMemcpyDeviceToHost, s->st));
+    SCHK(hipStreamSynchronize(s->st));
+  }
+  if (cnt[1] > s->maxmissing) return mac_internal_set_error(MAC_E_ARG, "too many particles outside the grid in one step");
+  std::set<Key3> centres;
+  for (int b = 0; b < s->nb; ++b)
+    if (bflag[b]) centres.insert(s->keys[b]);
+  for (int i = 0; i < nmiss; ++i) centres.insert({miss[3 * i], miss[3 * i + 1], miss[3 * i + 2]});
+  std::set<Key3> have(s->keys.begin(), s->keys.end());
+  std::vector<Key3> add;
+  for (const Key3& c : centres)
+    for (int oz = -1; oz <= 1; ++oz)
+      for (int oy = -1; oy <= 1; ++oy)
+        for (int ox = -1; ox <= 1; ++ox) {
+          Key3 k{c.x + ox, c.y + oy, c.z + oz};
+          if (!have.count(k)) {
+            have.insert(k);
+            add.push_back(k);
+          }
+        }
+  if (!add.empty()) {
+    std::vector<Key3> nk(have.begin(), have.end());
+    SRC(sim_set_bricks(s, nk));
+  }
+  // Algorithms_Layer_Initial_With_Particle
+  if (s->np > 0)
+    k_mark_particles<<<(s->np + 255) / 256, 256, 0, s->st>>>(s->np, s->px, s->py, s->pz, ih, s->hk, s->hv, s->hcap, s->flags,
+                                                            s->layer);
+  // Algorithms_Create_Fluid_Particle
+  SCHK(hipMemcpyAsync(s->d_int, &s->np, sizeof(int), hipMemcpyHostToDevice, s->st));
+  if (s->nb > 0 && P.particles_per_inflow_cell > 0)
+    k_inflow_fill<<<s->nb, 256, 0, s->st>>>(s->nb, s->d_keys, s->flags, s->layer, P.h[0], P.h[1], P.h[2],
+                                             P.particles_per_inflow_cell, P.seed, s->tick, s->px, s->py, s->pz, s->d_int,
+                                             s->capp, s->d_int + 2);
+  // Algorithms_Update_CellType_Layer
+  if (s->nb > 0)
+    for (int lay = 1; lay <= P.layer_max; ++lay) k_layer_round<<<s->nb, 256, 0, s->st>>>(s->nb, s->nbr, s->flags, s->layer, lay);
+  if (s->nb > 0) k_layer_extra<<<s->nb, 256, 0, s->st>>>(s->nb, s->nbr, s->flags, s->layer, P.layer_max);
+  // Algorithms_Delete_MacCell
+  if (s->nb > 0) k_delete_cells<<<s->nb, 256, 0, s->st>>>(s->nb, s->flags, s->layer, s->d_bflag);
+  SCHK(hipGetLastError());
+  SCHK(hipMemcpyAsync(cnt, s->d_int, sizeof(cnt), hipMemcpyDeviceToHost, s->st));
+  std::vector<int> alive(s->nb);
+  if (s->nb > 0) SCHK(hipMemcpyAsync(alive.data(), s->d_bflag, sizeof(int) * s->nb, hipMemcpyDeviceToHost, s->st));
+  SCHK(hipStreamSynchronize(s->st));
+  if (cnt[2]) return mac_internal_set_error(MAC_E_ARG, "particle pool exhausted (max_particles)");
+  s->np = cnt[0];
+  s->tick++;
+  // free the bricks whose last cell was erased (ChunkHashTable.tpp:133-161)
+  std::vector<Key3> keep;
+  keep.reserve(s->nb);
+  for (int b = 0; b < s->nb; ++b)
+    if (alive[b] > 0) keep.push_back(s->keys[b]);
+  if ((int)keep.size() != s->nb) SRC(sim_set_bricks(s, keep));
+  s->grid_ready = false;
+  return 0;
+}
+
+int sim_ensure_projection(mac_sim* s) {
+  if (s->grid_ready) return 0;
+  if (s->nb == 0) return mac_internal_set_error(MAC_E_STATE, "empty grid");
+  SCHK(hipStreamSynchronize(s->st));
+  std::vector<int32_t> k3(3 * s->nb);
+  for (int b = 0; b < s->nb; ++b) {
+    k3[3 * b] = s->keys[b].x;
+    k3[3 * b + 1] = s->keys[b].y;
+    k3[3 * b + 2] = s->keys[b].z;
+  }
+  SRC(mac_internal_build(s->proj, s->nb, k3.data(), s->flags));
+  s->grid_ready = true;
+  return 0;
+}
+
+int sim_forces(mac_sim* s, int slot) {
+  if (s->nb == 0) return 0;
+  const mac_sim_params& P = s->prm;
+  SimForce F;
+  for (int i = 0; i < 3; ++i) {
+    F.ih[i] = 1.0 / P.h[i];
+    F.g[i] = P.gravity[i];
+  }
+  F.nu = P.viscosity;
+  k_forces<<<s->nb, 256, 0, s->st>>>(s->nb, s->nbr, s->flags, s->layer, s->S[slot][0], s->S[slot][1], s->S[slot][2], s->A[0],
+                                      s->A[1], s->A[2], F);
+  SCHK(hipGetLastError());
+  return 0;
+}
+
+int sim_project(mac_sim* s, int slot, mac_stats* st) {
+  // the projection context works on its own stream: the velocity written by
+  // this stream's kernels must be complete before it is copied over
+  SCHK(hipStreamSynchronize(s->st));
+  SRC(sim_ensure_projection(s));
+  int rc = mac_internal_project(s->proj, s->S[slot][0], s->S[slot][1], s->S[slot][2], s->P, 0, st);
+  return rc;
+}
+
+SimGrid sim_grid(mac_sim* s, int slot) {
+  SimGrid G;
+  G.hk = s->hk;
+  G.hv = s->hv;
+  G.cap = s->hcap;
+  for (int i = 0; i < 3; ++i) {
+    G.U[i] = s->S[slot][i];
+    G.ih[i] = 1.0 / s->prm.h[i];
+  }
+  return G;
+}
+
+}  // namespace
+
+extern "C" {
+
+void mac_sim_default_params(mac_sim_params* p) {
+  memset(p, 0, sizeof(*p));
+  p->h[0] = p->h[1] = p->h[2] = 0.01;
+  p->gravity[2] = -9.81;
+  p->viscosity = 0.0;
+  p->cfl = 0.5;
+  p->dt_min = 0.0;
+  p->dt_max = 1e30;
+  p->layer_max = 2;
+  p->integrator = 0;
+  p->particle_rk = 2;
+  p->particles_per_inflow_cell = 8;
+  p->seed = 0;
+  p->max_particles = 1 << 22;
+  p->max_bricks = 1 << 16;
+  mac_default_params(&p->proj);
+}
+
+int mac_sim_create(mac_sim** out, const mac_sim_params* p) {
+  if (!out || !p) return mac_internal_set_error(MAC_E_ARG, "null argument");
+  *out = nullptr;
+  if (p->layer_max < 1 || p->layer_max > 7) return mac_internal_set_error(MAC_E_ARG, "layer_max must be in 1..7");
+  for (int i = 0; i < 3; ++i)
+    if (!(p->h[i] > 0)) return mac_internal_set_error(MAC_E_ARG, "h must be positive");
+  mac_sim* s = new mac_sim();
+  s->prm = *p;
+  mac_params pp = p->proj;
+  for (int i = 0; i < 3; ++i) pp.h[i] = p->h[i];
+  int rc = mac_create(&s->proj, &pp);
+  if (rc) {
+    delete s;
+    return rc;
+  }
+  if (hipStreamCreateWithFlags(&s->st, hipStreamNonBlocking) != hipSuccess) {
+    mac_destroy(s->proj);
+    delete s;
+    return mac_internal_set_error(MAC_E_HIP, "hipStreamCreate failed");
+  }
+  s->maxmissing = 1 << 16;
+  if (hipMalloc(&s->d_int, 8 * sizeof(int)) != hipSuccess || hipMalloc(&s->d_missing, sizeof(int) * 3 * s->maxmissing) != hipSuccess ||
+      sim_alloc_particles(s, p->max_particles) != 0) {
+    mac_sim_destroy(s);
+    return mac_internal_set_error(MAC_E_HIP, "allocation failed");
+  }
+  *out = s;
+  return 0;
+}
+
+void mac_sim_destroy(mac_sim* s) {
+  if (!s) return;
+  if (s->st) hipStreamSynchronize(s->st);
+  sim_free_bricks(s);
+  for (double* p : {s->px, s->py, s->pz, s->qx, s->qy, s->qz}) hipFree(p);
+  hipFree(s->d_int);
+  hipFree(s->d_missing);
+  if (s->proj) mac_destroy(s->proj);
+  if (s->st) hipStreamDestroy(s->st);
+  delete s;
+}
+
+int mac_sim_upload(mac_sim* s, int nbricks, const int32_t* keys3, const uint8_t* flags, const int8_t* layer, const double* u,
+                   const double* v, const double* w, int nparticles, const double* pxyz) {
+  if (!s || nbricks < 0 || (nbricks > 0 && (!keys3 || !flags)) || nparticles < 0 || (nparticles > 0 && !pxyz))
+    return mac_internal_set_error(MAC_E_ARG, "bad upload arguments");
+  if (nparticles > s->capp) return mac_internal_set_error(MAC_E_ARG, "more particles than max_particles");
+  // sort the input bricks; keys must be unique
+  std::vector<int> order(nbricks);
+  for (int b = 0; b < nbricks; ++b) order[b] = b;
+  auto kof = [&](int b) { return Key3{keys3[3 * b], keys3[3 * b + 1], keys3[3 * b + 2]}; };
+  std::sort(order.begin(), order.end(), [&](int a, int b) { return kof(a) < kof(b); });
+  std::vector<Key3> nk(nbricks);
+  for (int j = 0; j < nbricks; ++j) {
+    nk[j] = kof(order[j]);
+    if (j > 0 && nk[j] == nk[j - 1]) return mac_internal_set_error(MAC_E_ARG, "duplicate chunk key");
+  }
+  s->keys.clear();
+  s->nb = 0;
+  SRC(sim_set_bricks(s, nk));
+  // scatter host data into sorted brick order
+  const size_t cells = (size_t)nbricks * MAC_BC;
+  std::vector<uint8_t> f8(cells);
+  std::vector<int8_t> l8(cells, -1);
+  std::vector<double> buf(cells);
+  for (int j = 0; j < nbricks; ++j) {
+    memcpy(&f8[(size_t)j * MAC_BC], &flags[(size_t)order[j] * MAC_BC], MAC_BC);
+    if (layer) memcpy(&l8[(size_t)j * MAC_BC], &layer[(size_t)order[j] * MAC_BC], MAC_BC);
+  }
+  if (nbricks > 0) {
+    SCHK(hipMemcpy(s->flags, f8.data(), cells, hipMemcpyHostToDevice));
+    SCHK(hipMemcpy(s->layer, l8.data(), cells, hipMemcpyHostToDevice));
+    const double* src[3] = {u, v, w};
+    for (int i = 0; i < 3; ++i) {
+      for (int j = 0; j < nbricks; ++j) {
+        if (src[i]) memcpy(&buf[(size_t)j * MAC_BC], &src[i][(size_t)order[j] * MAC_BC], MAC_BC * sizeof(double));
+        else memset(&buf[(size_t)j * MAC_BC], 0, MAC_BC * sizeof(double));
+      }
+      SCHK(hipMemcpy(s->S[0][i], buf.data(), cells * sizeof(double), hipMemcpyHostToDevice));
+      SCHK(hipMemcpy(s->S[1][i], buf.data(), cells * sizeof(double), hipMemcpyHostToDevice));
+      SCHK(hipMemset(s->A[i], 0, cells * sizeof(double)));
+    }
+    SCHK(hipMemset(s->P, 0, cells * sizeof(double)));
+  }
+  std::vector<double> c(std::max(nparticles, 1));
+  double* dst[3] = {s->px, s->py, s->pz};
+  for (int i = 0; i < 3; ++i) {
+    for (int k = 0; k < nparticles; ++k) c[k] = pxyz[3 * k + i];
+    if (nparticles > 0) SCHK(hipMemcpy(dst[i], c.data(), sizeof(double) * nparticles, hipMemcpyHostToDevice));
+  }
+  s->np = nparticles;
+  s->grid_ready = false;
+  return 0;
+}
+
+int mac_sim_counts(mac_sim* s, int* nbricks, int* nparticles) {
+  if (!s) return mac_internal_set_error(MAC_E_ARG, "null sim");
+  if (nbricks) *nbricks = s->nb;
+  if (nparticles) *nparticles = s->np;
+  return 0;
+}
+
+int mac_sim_download(mac_sim* s, int32_t* keys3, uint8_t* flags, int8_t* layer, double* u, double* v, double* w, double* ax,
+                     double* ay, double* az, double* p, double* pxyz) {
+  if (!s) return mac_internal_set_error(MAC_E_ARG, "null sim");
+  SCHK(hipStreamSynchronize(s->st));
+  const size_t cells = (size_t)s->nb * MAC_BC;
+  if (keys3)
+    for (int b = 0; b < s->nb; ++b) {
+      keys3[3 * b] = s->keys[b].x;
+      keys3[3 * b + 1] = s->keys[b].y;
+      keys3[3 * b + 2] = s->keys[b].z;
+    }
+  if (cells > 0) {
+    if (flags) SCHK(hipMemcpy(flags, s->flags, cells, hipMemcpyDeviceToHost));
+    if (layer) SCHK(hipMemcpy(layer, s->layer, cells, hipMemcpyDeviceToHost));
+    double* dst[7] = {u, v, w, ax, ay, az, p};
+    double* src[7] = {s->S[0][0], s->S[0][1], s->S[0][2], s->A[0], s->A[1], s->A[2], s->P};
+    for (int i = 0; i < 7; ++i)
+      if (dst[i]) SCHK(hipMemcpy(dst[i], src[i], cells * sizeof(double), hipMemcpyDeviceToHost));
+  }
+  if (pxyz && s->np > 0) {
+    std::vector<double> c(s->np);
+    double* srcp[3] = {s->px, s->py, s->pz};
+    for (int i = 0; i < 3; ++i) {
+      SCHK(hipMemcpy(c.data(), srcp[i], sizeof(double) * s->np, hipMemcpyDeviceToHost));
+      for (int k = 0; k < s->np; ++k) pxyz[3 * k + i] = c[k];
+    }
+  }
+  return 0;
+}
+
+int mac_sim_topology(mac_sim* s) {
+  if (!s) return mac_internal_set_error(MAC_E_ARG, "null sim");
+  return sim_topology(s);
+}
+
+int mac_sim_compute_dt(mac_sim* s, double* dt) {
+  if (!s || !dt) return mac_internal_set_error(MAC_E_ARG, "null argument");
+  const mac_sim_params& P = s->prm;
+  double mx = 0.0;
+  if (s->nb > 0) {
+    SimIH ih{{1.0 / P.h[0], 1.0 / P.h[1], 1.0 / P.h[2]}};
+    k_dt_max<<<s->nb, 256, 0, s->st>>>(s->nb, s->flags, s->S[0][0], s->S[0][1], s->S[0][2], ih, s->d_part);
+    SCHK(hipGetLastError());
+    std::vector<double> part(s->nb);
+    SCHK(hipMemcpyAsync(part.data(), s->d_part, sizeof(double) * s->nb, hipMemcpyDeviceToHost, s->st));
+    SCHK(hipStreamSynchronize(s->st));
+    for (double x : part) mx = std::max(mx, x);
+  }
+  // m_dt = 1/sqrt(max) * m_factor with FP exceptions held (division by 0 -> inf),
+  // then Policy_CheckDT
+  double d = (mx > 0.0) ? 1.0 / std::sqrt(mx) * P.cfl : HUGE_VAL;
+  if (d < P.dt_min) d = P.dt_min;
+  if (d > P.dt_max) d = P.dt_max;
+  *dt = d;
+  return 0;
+}
+
+int mac_sim_forces(mac_sim* s, int slot) {
+  if (!s || slot < 0 || slot > 1) return mac_internal_set_error(MAC_E_ARG, "bad arguments");
+  SRC(sim_forces(s, slot));
+  SCHK(hipStreamSynchronize(s->st));
+  return 0;
+}
+
+int mac_sim_advance(mac_sim* s, double dt, mac_sim_stats* stats) {
+  if (!s) return mac_internal_set_error(MAC_E_ARG, "null sim");
+  if (s->nb == 0) return 0;
+  const bool tvd = s->prm.integrator == 1;
+  const size_t cells = (size_t)s->nb * MAC_BC;
+  auto t0 = std::chrono::steady_clock::now();
+  SRC(sim_ensure_projection(s));
+  mac_stats st;
+  // stage 1: forces at u0, u1 = u0 + a dt (/2), acceleration reset, project u1
+  SRC(sim_forces(s, 0));
+  k_rk_combine<<<s->nb, 256, 0, s->st>>>(s->nb, s->flags, s->S[0][0], s->S[0][1], s->S[0][2], s->S[1][0], s->S[1][1], s->S[1][2],
+                                         s->A[0], s->A[1], s->A[2], dt, tvd ? 1 : 0);
+  k_zero_accel<<<1024, 256, 0, s->st>>>(cells, s->A[0], s->A[1], s->A[2]);
+  SCHK(hipGetLastError());
+  int rc = sim_project(s, 1, &st);
+  if (rc) return rc;
+  if (stats) stats->pcg_iters[0] = st.iters;
+  // stage 2: forces at u1, combine into u0, project u0
+  SRC(sim_forces(s, 1));
+  k_rk_combine<<<s->nb, 256, 0, s->st>>>(s->nb, s->flags, s->S[0][0], s->S[0][1], s->S[0][2], s->S[1][0], s->S[1][1], s->S[1][2],
+                                         s->A[0], s->A[1], s->A[2], dt, tvd ? 3 : 2);
+  SCHK(hipGetLastError());
+  rc = sim_project(s, 0, &st);
+  if (rc) return rc;
+  if (stats) {
+    stats->pcg_iters[1] = st.iters;
+    stats->max_div = st.max_div;
+    stats->nfluid = st.n_unknowns;
+    stats->ms_project = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  }
+  return 0;
+}
+
+int mac_sim_move_particles(mac_sim* s, double dt) {
+  if (!s) return mac_internal_set_error(MAC_E_ARG, "null sim");
+  if (s->np == 0) return 0;
+  const mac_sim_params& P = s->prm;
+  bool box = !(P.box_lo[0] == P.box_hi[0] && P.box_lo[1] == P.box_hi[1] && P.box_lo[2] == P.box_hi[2]);
+  SCHK(hipMemsetAsync(s->d_int + 3, 0, sizeof(int), s->st));
+  k_move_particles<<<(s->np + 255) / 256, 256, 0, s->st>>>(s->np, s->px, s->py, s->pz, s->qx, s->qy, s->qz, s->d_int + 3,
+                                                            sim_grid(s, 0), dt, P.particle_rk == 4 ? 4 : 2, box ? 1 : 0, P.box_lo[0],
+                                                            P.box_lo[1], P.box_lo[2], P.box_hi[0], P.box_hi[1], P.box_hi[2]);
+  SCHK(hipGetLastError());
+  int kept = 0;
+  SCHK(hipMemcpyAsync(&kept, s->d_int + 3, sizeof(int), hipMemcpyDeviceToHost, s->st));
+  SCHK(hipStreamSynchronize(s->st));
+  std::swap(s->px, s->qx);
+  std::swap(s->py, s->qy);
+  std::swap(s->pz, s->qz);
+  s->np = kept;
+  return 0;
+}
+
+int mac_sim_interpolate(mac_sim* s, int n, const double* pos, double* vel) {
+  if (!s || n < 0 || (n > 0 && (!pos || !vel))) return mac_internal_set_error(MAC_E_ARG, "bad arguments");
+  if (n == 0) return 0;
+  double *dp, *dv;
+  SCHK(hipMalloc(&dp, sizeof(double) * 3 * n));
+  SCHK(hipMalloc(&dv, sizeof(double) * 3 * n));
+  SCHK(hipMemcpy(dp, pos, sizeof(double) * 3 * n, hipMemcpyHostToDevice));
+  k_interpolate<<<(n + 255) / 256, 256, 0, s->st>>>(n, dp, dv, sim_grid(s, 0));
+  SCHK(hipGetLastError());
+  SCHK(hipStreamSynchronize(s->st));
+  SCHK(hipMemcpy(vel, dv, sizeof(double) * 3 * n, hipMemcpyDeviceToHost));
+  hipFree(dp);
+  hipFree(dv);
+  return 0;
+}
+
+int mac_sim_step(mac_sim* s, mac_sim_stats* stats) {
+  if (!s) return mac_internal_set_error(MAC_E_ARG, "null sim");
+  mac_sim_stats local;
+  memset(&local, 0, sizeof(local));
+  auto t0 = std::chrono::steady_clock::now();
+  SRC(sim_topology(s));
+  auto t1 = std::chrono::steady_clock::now();
+  double dt;
+  SRC(mac_sim_compute_dt(s, &dt));
+  local.dt = dt;
+  SRC(mac_sim_advance(s, dt, &local));
+  auto t2 = std::chrono::steady_clock::now();
+  SRC(mac_sim_move_particles(s, dt));
+  auto t3 = std::chrono::steady_clock::now();
+  local.nbricks = s->nb;
+  local.nparticles = s->np;
+  local.ms_topology = std::chrono::duration<double, std::milli>(t1 - t0).count();
+  local.ms_forces = std::chrono::duration<double, std::milli>(t2 - t1).count() - local.ms_project;
+  local.ms_particles = std::chrono::duration<double, std::milli>(t3 - t2).count();
+  local.ms_total = std::chrono::duration<double, std::milli>(t3 - t0).count();
+  if (stats) *stats = local;
+  return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,595 @@
+// sim_kernels.h -- HIP kernels of the per-timestep MAC-grid loop on the
+// sparse brick grid (topology, time step, forces, RK2 combination,
+// staggered interpolation, particle advection).
+//
+// Cell addressing: brick b, cell t = x + 8y + 64z (x fastest); global key of
+// the cell = 8 * key(b) + (x, y, z).  Missing bricks read as zero; a cell
+// whose existence bit is off keeps the data it had (as a chunk keeps a
+// deleted cell's data until the chunk itself is freed,
+// ChunkHashTable.tpp:133-161).
+#pragma once
+#include <stdint.h>
+
+#include "brick.h"
+
+#define SIM_HEMPTY 0xFFFFFFFFFFFFFFFFull
+
+__host__ __device__ inline uint64_t sim_pack(int x, int y, int z) {
+  return ((uint64_t)(uint32_t)(x + (1 << 20)) << 42) | ((uint64_t)(uint32_t)(y + (1 << 20)) << 21) |
+         (uint64_t)(uint32_t)(z + (1 << 20));
+}
+
+__host__ __device__ inline uint64_t sim_mix(uint64_t k) {
+  k ^= k >> 33;
+  k *= 0xff51afd7ed558ccdull;
+  k ^= k >> 33;
+  k *= 0xc4ceb9fe1a85ec53ull;
+  k ^= k >> 33;
+  return k;
+}
+
+// Counter-based uniform in (0,1) of (seed, stream, a, b) -- SplitMix64 finaliser.
+__host__ __device__ inline double sim_uniform(uint64_t seed, uint64_t stream, uint64_t a, uint64_t b) {
+  uint64_t z = seed * 0xD1B54A32D192ED03ull + stream * 0x8CB92BA72F3D8DD7ull + a * 0x9E3779B97F4A7C15ull +
+               (b + 1) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  z ^= z >> 31;
+  return ((double)(z >> 11) + 0.5) * (1.0 / 9007199254740992.0);
+}
+
+// Chunk key of a cell key (ChunkHashTable.tpp:181-196).
+__host__ __device__ inline int sim_chunk(int k) { return k >= 0 ? k / 8 : (k + 1) / 8 - 1; }
+
+// Materials (MaterialToInt numbering) and predicates of
+// Data_CellType_Fluid_Air_Solid_Inflow_Interior_0_Pres.tpp:25-233.
+__host__ __device__ inline bool mat_is_fluid(unsigned m) { return m == 1u || m == 3u || m == 4u || m == 5u; }
+__host__ __device__ inline unsigned mat_set_fluid(unsigned m) { return (m == 0u || m == 4u) ? 1u : m; }
+__host__ __device__ inline unsigned mat_set_air(unsigned m) { return (m == 1u || m == 4u) ? 0u : m; }
+
+#define F_EXISTS 0x80u
+#define F_NODEL 0x40u
+
+__device__ inline int dh_lookup(const uint64_t* __restrict__ hk, const int* __restrict__ hv, int cap, uint64_t k) {
+  const uint32_t m = (uint32_t)cap - 1u;
+  uint32_t i = (uint32_t)sim_mix(k) & m;
+  for (int n = 0; n < cap; ++n) {
+    uint64_t kk = hk[i];
+    if (kk == k) return hv[i];
+    if (kk == SIM_HEMPTY) return -1;
+    i = (i + 1u) & m;
+  }
+  return -1;
+}
+
+__global__ void k_hash_clear(int cap, uint64_t* __restrict__ hk, int* __restrict__ hv) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < cap; i += gridDim.x * blockDim.x) {
+    hk[i] = SIM_HEMPTY;
+    hv[i] = -1;
+  }
+}
+
+// Insert every brick key (open addressing, linear probing; keys are unique).
+__global__ void k_hash_insert(int nb, const int* __restrict__ keys3, uint64_t* __restrict__ hk, int* __restrict__ hv, int cap) {
+  int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nb) return;
+  const uint64_t k = sim_pack(keys3[3 * b], keys3[3 * b + 1], keys3[3 * b + 2]);
+  const uint32_t m = (uint32_t)cap - 1u;
+  uint32_t i = (uint32_t)sim_mix(k) & m;
+  for (int n = 0; n < cap; ++n) {
+    unsigned long long prev = atomicCAS((unsigned long long*)&hk[i], (unsigned long long)SIM_HEMPTY, (unsigned long long)k);
+    if (prev == SIM_HEMPTY || prev == k) {
+      hv[i] = b;
+      return;
+    }
+    i = (i + 1u) & m;
+  }
+}
+
+// 27-neighbour brick table from the hash (Hook_Neighbour_List_key.h:97-120).
+__global__ void k_build_nbr(int nb, const int* __restrict__ keys3, const uint64_t* __restrict__ hk, const int* __restrict__ hv,
+                            int cap, int* __restrict__ nbr27) {
+  int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= nb * 27) return;
+  int b = idx / 27, s = idx % 27;
+  int ox = s % 3 - 1, oy = (s / 3) % 3 - 1, oz = s / 9 - 1;
+  nbr27[idx] = dh_lookup(hk, hv, cap, sim_pack(keys3[3 * b] + ox, keys3[3 * b + 1] + oy, keys3[3 * b + 2] + oz));
+}
+
+// Cell of a global cell key through the hash: returns brick*512+t or -1.
+__device__ inline long long cell_of_key(const uint64_t* hk, const int* hv, int cap, int kx, int ky, int kz) {
+  int b = dh_lookup(hk, hv, cap, sim_pack(sim_chunk(kx), sim_chunk(ky), sim_chunk(kz)));
+  if (b < 0) return -1;
+  return (long long)b * MAC_BC + (kx & 7) + 8 * (ky & 7) + 64 * (kz & 7);
+}
+
+// Particle -> cell key: key_i = lround(x_i * (1/h_i)) (Policy_Particle_To_Key.h:24-35).
+__device__ inline void part_key(double x, double y, double z, const double* ih, int& kx, int& ky, int& kz) {
+#pragma clang fp contract(off)
+  kx = (int)lround(x * ih[0]);
+  ky = (int)lround(y * ih[1]);
+  kz = (int)lround(z * ih[2]);
+}
+
+struct SimIH {
+  double ih[3];
+};
+
+// ---------------------------------------------------------------- topology --
+
+// Algorithms_Initialize_MacCell::Do: every layer := empty.
+__global__ void k_reset_layer(size_t ncell, int8_t* __restrict__ layer) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < ncell; i += (size_t)gridDim.x * blockDim.x) layer[i] = -1;
+}
+
+// Bricks the particles land in: missing ones are appended to `missing`
+// (duplicates allowed), existing ones flagged.
+__global__ void k_particle_bricks(int np, const double* __restrict__ px, const double* __restrict__ py,
+                                  const double* __restrict__ pz, SimIH ih, const uint64_t* __restrict__ hk,
+                                  const int* __restrict__ hv, int cap, int* __restrict__ bflag, int* __restrict__ missing,
+                                  int* __restrict__ nmissing, int maxmissing) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= np) return;
+  int kx, ky, kz;
+  part_key(px[i], py[i], pz[i], ih.ih, kx, ky, kz);
+  int cx = sim_chunk(kx), cy = sim_chunk(ky), cz = sim_chunk(kz);
+  int b = dh_lookup(hk, hv, cap, sim_pack(cx, cy, cz));
+  if (b >= 0) {
+    bflag[b] = 1;
+  } else {
+    int slot = atomicAdd(nmissing, 1);
+    if (slot < maxmissing) {
+      missing[3 * slot] = cx;
+      missing[3 * slot + 1] = cy;
+      missing[3 * slot + 2] = cz;
+    }
+  }
+}
+
+// Flag bricks holding an existing inflow cell.
+__global__ void k_inflow_bricks(int nb, const uint8_t* __restrict__ flags, int* __restrict__ bflag) {
+  int b = blockIdx.x;
+  for (int t = threadIdx.x; t < MAC_BC; t += blockDim.x) {
+    unsigned f = flags[(size_t)b * MAC_BC + t];
+    if ((f & F_EXISTS) && (f & 7u) == 3u) bflag[b] = 1;
+  }
+}
+
+// Algorithms_Layer_Initial_With_Particle::Do (:20-35 of the non-depth
+// variant): the cell of each particle gets layer 0 and SetFluid.  Every
+// writer of a cell writes the same values, so concurrent updates agree.
+__global__ void k_mark_particles(int np, const double* __restrict__ px, const double* __restrict__ py,
+                                 const double* __restrict__ pz, SimIH ih, const uint64_t* __restrict__ hk,
+                                 const int* __restrict__ hv, int cap, uint8_t* __restrict__ flags, int8_t* __restrict__ layer) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= np) return;
+  int kx, ky, kz;
+  part_key(px[i], py[i], pz[i], ih.ih, kx, ky, kz);
+  long long c = cell_of_key(hk, hv, cap, kx, ky, kz);
+  if (c < 0) return;
+  unsigned f = flags[c];
+  unsigned m = mat_set_fluid(f & 7u);
+  flags[c] = (uint8_t)((f & ~7u) | m | F_EXISTS);
+  layer[c] = 0;
+}
+
+// Algorithms_Create_Fluid_Particle::Do with Policy_Is_Inbound_Filling_Layer_CellType
+// (:16-19): an existing Inflow cell with an empty layer gets `ppc` particles
+// at seeded jitter inside the cell ((key + U - 1/2) h per axis), layer 0 and
+// SetFluid.
+__global__ void k_inflow_fill(int nb, const int* __restrict__ keys3, uint8_t* __restrict__ flags, int8_t* __restrict__ layer,
+                              double h0, double h1, double h2, int ppc, uint64_t seed, uint64_t tick, double* __restrict__ px,
+                              double* __restrict__ py, double* __restrict__ pz, int* __restrict__ np, int maxp,
+                              int* __restrict__ overflow) {
+#pragma clang fp contract(off)
+  int b = blockIdx.x;
+  for (int t = threadIdx.x; t < MAC_BC; t += blockDim.x) {
+    size_t c = (size_t)b * MAC_BC + t;
+    unsigned f = flags[c];
+    if (!(f & F_EXISTS) || (f & 7u) != 3u || layer[c] != -1) continue;
+    int kx = keys3[3 * b] * 8 + (t & 7), ky = keys3[3 * b + 1] * 8 + ((t >> 3) & 7), kz = keys3[3 * b + 2] * 8 + (t >> 6);
+    uint64_t ck = sim_pack(kx, ky, kz);
+    int base = atomicAdd(np, ppc);
+    for (int k = 0; k < ppc; ++k) {
+      if (base + k >= maxp) {
+        *overflow = 1;
+        break;
+      }
+      double ux = sim_uniform(seed, tick, ck, 3 * k), uy = sim_uniform(seed, tick, ck, 3 * k + 1),
+             uz = sim_uniform(seed, tick, ck, 3 * k + 2);
+      px[base + k] = ((double)kx + (ux - 0.5)) * h0;
+      py[base + k] = ((double)ky + (uy - 0.5)) * h1;
+      pz[base + k] = ((double)kz + (uz - 0.5)) * h2;
+    }
+    layer[c] = 0;
+    flags[c] = (uint8_t)((f & ~7u) | mat_set_fluid(f & 7u));
+  }
+}
+
+// One level of Algorithms_Update_CellType_Layer::Do (:22-76): every missing
+// or layer-empty face neighbour of a layer-(lay-1) cell gets layer `lay` and
+// SetAir.  Level-synchronous: a cell set in this pass has layer lay != lay-1,
+// so the outcome does not depend on thread order (the reference's stack
+// order does not matter either, each level being a set).
+__global__ void k_layer_round(int nb, const int* __restrict__ nbr27, uint8_t* __restrict__ flags, int8_t* __restrict__ layer,
+                              int lay) {
+  int b = blockIdx.x;
+  for (int t = threadIdx.x; t < MAC_BC; t += blockDim.x) {
+    size_t c = (size_t)b * MAC_BC + t;
+    unsigned f = flags[c];
+    if ((f & F_EXISTS) && layer[c] != -1) continue;
+    int x = t & 7, y = (t >> 3) & 7, z = t >> 6;
+    const int dx[6] = {-1, 1, 0, 0, 0, 0}, dy[6] = {0, 0, -1, 1, 0, 0}, dz[6] = {0, 0, 0, 0, -1, 1};
+    bool hit = false;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+      int cell;
+      int nbk = locate(nbr27, b, x + dx[k], y + dy[k], z + dz[k], cell);
+      if (nbk < 0) continue;
+      size_t nc = (size_t)nbk * MAC_BC + cell;
+      if ((flags[nc] & F_EXISTS) && layer[nc] == lay - 1) hit = true;
+    }
+    if (hit) {
+      layer[c] = (int8_t)lay;
+      flags[c] = (uint8_t)((f & ~7u) | mat_set_air(f & 7u) | F_EXISTS);
+    }
+  }
+}
+
+// The closing pass (:77-95): from each cell of the last layer, only the +1
+// neighbours that are missing or empty get layer_max + 1 and SetAir.
+__global__ void k_layer_extra(int nb, const int* __restrict__ nbr27, uint8_t* __restrict__ flags, int8_t* __restrict__ layer,
+                              int lmax) {
+  int b = blockIdx.x;
+  for (int t = threadIdx.x; t < MAC_BC; t += blockDim.x) {
+    size_t c = (size_t)b * MAC_BC + t;
+    unsigned f = flags[c];
+    if ((f & F_EXISTS) && layer[c] != -1) continue;
+    int x = t & 7, y = (t >> 3) & 7, z = t >> 6;
+    bool hit = false;
+    for (int k = 0; k < 3; ++k) {
+      int cell;
+      int nbk = locate(nbr27, b, x - (k == 0), y - (k == 1), z - (k == 2), cell);
+      if (nbk < 0) continue;
+      size_t nc = (size_t)nbk * MAC_BC + cell;
+      if ((flags[nc] & F_EXISTS) && layer[nc] == lmax) hit = true;
+    }
+    if (hit) {
+      layer[c] = (int8_t)(lmax + 1);
+      flags[c] = (uint8_t)((f & ~7u) | mat_set_air(f & 7u) | F_EXISTS);
+    }
+  }
+}
+
+// Algorithms_Delete_MacCell::Do (:22-44): erase layer-empty cells unless
+// NoDelete; count the cells left per brick.
+__global__ void k_delete_cells(int nb, uint8_t* __restrict__ flags, const int8_t* __restrict__ layer, int* __restrict__ alive) {
+  int b = blockIdx.x;
+  __shared__ int cnt;
+  if (threadIdx.x == 0) cnt = 0;
+  __syncthreads();
+  int my = 0;
+  for (int t = threadIdx.x; t < MAC_BC; t += blockDim.x) {
+    size_t c = (size_t)b * MAC_BC + t;
+    unsigned f = flags[c];
+    if ((f & F_EXISTS) && layer[c] == -1 && !(f & F_NODEL)) {
+      f &= ~F_EXISTS;
+      flags[c] = (uint8_t)f;
+    }
+    my += (f & F_EXISTS) ? 1 : 0;
+  }
+  atomicAdd(&cnt, my);
+  __syncthreads();
+  if (threadIdx.x == 0) alive[b] = cnt;
+}
+
+// Re-layout after the brick set changed: new brick j takes old brick
+// from[j], or the prototype cell (Air, no const, not existing, layer empty,
+// zero fields) when from[j] < 0 -- a chunk allocated from the prototype.
+__global__ void k_relayout_u8(int nb, const int* __restrict__ from, const uint8_t* __restrict__ src, uint8_t* __restrict__ dst,
+                              uint8_t proto) {
+  int j = blockIdx.x;
+  int o = from[j];
+  for (int t = threadIdx.x; t < MAC_BC; t += blockDim.x)
+    dst[(size_t)j * MAC_BC + t] = o >= 0 ? src[(size_t)o * MAC_BC + t] : proto;
+}
+__global__ void k_relayout_f64(int nb, const int* __restrict__ from, const double* __restrict__ src, double* __restrict__ dst) {
+  int j = blockIdx.x;
+  int o = from[j];
+  for (int t = threadIdx.x; t < MAC_BC; t += blockDim.x)
+    dst[(size_t)j * MAC_BC + t] = o >= 0 ? src[(size_t)o * MAC_BC + t] : 0.0;
+}
+
+// ----------------------------------------------------------------- dt -------
+
+// max over existing fluid cells of sum_i (u_i inv_h_i)^2 (Algorithms_Calculate_Time_Step.h:41-54).
+__global__ void k_dt_max(int nb, const uint8_t* __restrict__ flags, const double* __restrict__ u, const double* __restrict__ v,
+                         const double* __restrict__ w, SimIH ih, double* __restrict__ partials) {
+#pragma clang fp contract(off)
+  __shared__ double sh[16];
+  int b = blockIdx.x;
+  double m = 0.0;
+  for (int t = threadIdx.x; t < MAC_BC; t += blockDim.x) {
+    size_t c = (size_t)b * MAC_BC + t;
+    unsigned f = flags[c];
+    if (!(f & F_EXISTS) || !mat_is_fluid(f & 7u)) continue;
+    double a = u[c] * ih.ih[0], bb = v[c] * ih.ih[1], cc = w[c] * ih.ih[2];
+    double temp = 0.0;
+    temp += a * a;
+    temp += bb * bb;
+    temp += cc * cc;
+    if (temp > m) m = temp;
+  }
+  double t = block_max(m, sh);
+  if (threadIdx.x == 0) partials[b] = t;
+}
+
+// --------------------------------------------------------------- forces -----
+
+struct SimForce {
+  double ih[3];
+  double g[3];
+  double nu;
+};
+
+__device__ inline bool cell_fluid(const uint8_t* flags, const int* nbr27, int b, int x, int y, int z, bool& valid) {
+  int cell;
+  int nb = locate(nbr27, b, x, y, z, cell);
+  if (nb < 0) {
+    valid = false;
+    return false;
+  }
+  unsigned f = flags[(size_t)nb * MAC_BC + cell];
+  valid = f & F_EXISTS;
+  return valid && mat_is_fluid(f & 7u);
+}
+
+// acceleration := 0 (forced), then in the reference's order
+// Algorithms<Gravity, Viscosity, Convection> (TestJetDEau_Basic.cpp:363):
+//   gravity    Policy_Gravity.h:14-45 (fluid cell, or non-empty cell whose -e_i neighbour is fluid)
+//   viscosity  Algorithms_Viscosity.h:19-35 + Policy_Laplacian_Speed.h:22-37 + Policy_Viscosity_Apply_If.h:16-39
+//   convection Algorithms_Convection.h:18-37 + Policy_Upwind_1_Order.h:18-66 + Policy_Convection_Apply_If.h:41-61
+// every accumulation a non-forced Set (const components untouched,
+// Data_Grid_Speed.tpp:20-26).  Same operation order as the reference.
+__global__ void k_forces(int nb, const int* __restrict__ nbr27, const uint8_t* __restrict__ flags,
+                         const int8_t* __restrict__ layer, const double* __restrict__ u, const double* __restrict__ v,
+                         const double* __restrict__ w, double* __restrict__ ax, double* __restrict__ ay, double* __restrict__ az,
+                         SimForce P) {
+#pragma clang fp contract(off)
+  int b = blockIdx.x;
+  const double* U[3] = {u, v, w};
+  double* A[3] = {ax, ay, az};
+  for (int t = threadIdx.x; t < MAC_BC; t += blockDim.x) {
+    size_t c = (size_t)b * MAC_BC + t;
+    unsigned f = flags[c];
+    if (!(f & F_EXISTS)) continue;
+    int x = t & 7, y = (t >> 3) & 7, z = t >> 6;
+    double a[3] = {0.0, 0.0, 0.0};
+    const bool fl = mat_is_fluid(f & 7u);
+    const bool lay = layer[c] != -1;
+    bool minus_fluid[3];
+    for (int i = 0; i < 3; ++i) {
+      bool valid;
+      minus_fluid[i] = cell_fluid(flags, nbr27, b, x - (i == 0), y - (i == 1), z - (i == 2), valid);
+    }
+    // gravity
+    for (int i = 0; i < 3; ++i) {
+      if (P.g[i] == 0.0) continue;
+      bool apply = fl || (lay && minus_fluid[i]);
+      if (apply && !(f & (0x08u << i))) a[i] = a[i] + P.g[i];
+    }
+    // viscosity
+    if (P.nu != 0.0) {
+      bool apply = fl || (lay && (minus_fluid[0] || minus_fluid[1] || minus_fluid[2]));
+      if (apply) {
+        for (int i = 0; i < 3; ++i) {
+          double temp = 0.0;
+          double uc = U[i][c];
+          for (int j = 0; j < 3; ++j) {
+            double ih2 = P.ih[j] * P.ih[j];
+            temp -= 2 * uc * ih2;
+            temp += ld_rel(U[i], nbr27, b, x + (j == 0), y + (j == 1), z + (j == 2)) * ih2;
+            temp += ld_rel(U[i], nbr27, b, x - (j == 0), y - (j == 1), z - (j == 2)) * ih2;
+          }
+          if (!(f & (0x08u << i))) a[i] = a[i] + temp * P.nu;
+        }
+      }
+    }
+    // convection
+    for (int i = 0; i < 3; ++i) {
+      bool apply = fl || (lay && minus_fluid[i]);
+      if (!apply) continue;
+      double temp = 0.0;
+      for (int j = 0; j < 3; ++j) {
+        const int ex = (j == 0), ey = (j == 1), ez = (j == 2);
+        const int mx = (i == 0), my = (i == 1), mz = (i == 2);
+        double Ub;
+        if (i == j) {
+          Ub = U[j][c];
+        } else {
+          Ub = 0.25 * (U[j][c] + ld_rel(U[j], nbr27, b, x + ex, y + ey, z + ez) + ld_rel(U[j], nbr27, b, x - mx, y - my, z - mz) +
+                       ld_rel(U[j], nbr27, b, x - mx + ex, y - my + ey, z - mz + ez));
+        }
+        double U1 = U[i][c], U2;
+        if (Ub > 0) {
+          U2 = ld_rel(U[i], nbr27, b, x - ex, y - ey, z - ez);
+          Ub *= (U1 - U2) * P.ih[j];
+        } else {
+          U2 = ld_rel(U[i], nbr27, b, x + ex, y + ey, z + ez);
+          Ub *= (U2 - U1) * P.ih[j];
+        }
+        temp -= Ub;
+      }
+      if (!(f & (0x08u << i))) a[i] = a[i] + temp;
+    }
+    for (int i = 0; i < 3; ++i) A[i][c] = a[i];
+  }
+}
+
+// ------------------------------------------------------------ RK2 stages ----
+// mode 0: RK2 midpoint stage 1  u1 = u0 + a dt 0.5           (forced)   RK2.h:41
+// mode 1: RK2-TVD stage 1       u1 = u0 + a dt               (forced)   RK2_TVD.h:40-43
+// mode 2: RK2 midpoint stage 2  u0 = u0 + a dt               (non-forced) RK2.h:56
+// mode 3: RK2-TVD stage 2       u0 = 0.5 u0 + 0.5 u1 + 0.5 a dt (non-forced) RK2_TVD.h:55-58
+__global__ void k_rk_combine(int nb, const uint8_t* __restrict__ flags, double* __restrict__ u0, double* __restrict__ v0,
+                             double* __restrict__ w0, double* __restrict__ u1, double* __restrict__ v1, double* __restrict__ w1,
+                             const double* __restrict__ ax, const double* __restrict__ ay, const double* __restrict__ az,
+                             double dt, int mode) {
+#pragma clang fp contract(off)
+  int b = blockIdx.x;
+  double* S0[3] = {u0, v0, w0};
+  double* S1[3] = {u1, v1, w1};
+  const double* A[3] = {ax, ay, az};
+  for (int t = threadIdx.x; t < MAC_BC; t += blockDim.x) {
+    size_t c = (size_t)b * MAC_BC + t;
+    unsigned f = flags[c];
+    if (!(f & F_EXISTS)) continue;
+    for (int i = 0; i < 3; ++i) {
+      const double a = A[i][c];
+      const bool cst = f & (0x08u << i);
+      switch (mode) {
+        case 0: S1[i][c] = S0[i][c] + a * dt * 0.5; break;
+        case 1: S1[i][c] = S0[i][c] + a * dt; break;
+        case 2:
+          if (!cst) S0[i][c] = S0[i][c] + a * dt;
+          break;
+        default:
+          if (!cst) S0[i][c] = 0.5 * S0[i][c] + 0.5 * S1[i][c] + 0.5 * a * dt;
+          break;
+      }
+    }
+  }
+}
+
+__global__ void k_zero_accel(size_t n, double* __restrict__ ax, double* __restrict__ ay, double* __restrict__ az) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) ax[i] = ay[i] = az[i] = 0.0;
+}
+
+// ------------------------------------------------------- interpolation ------
+
+struct SimGrid {
+  const uint64_t* hk;
+  const int* hv;
+  int cap;
+  const double* U[3];
+  double ih[3];
+};
+
+__device__ inline double speed_at(const SimGrid& G, int k, int kx, int ky, int kz) {
+  long long c = cell_of_key(G.hk, G.hv, G.cap, kx, ky, kz);
+  return c >= 0 ? G.U[k][c] : 0.0;
+}
+
+// Policy_Speed_Interpolation_Linear_Symmetric::Get_Speed (:46-119) with the
+// recursive Get_Speed_Impl (:13-44) unrolled in the same nesting (x outer,
+// z inner) and operation order.
+__device__ inline void interp_speed(const SimGrid& G, const double pos[3], double out[3]) {
+#pragma clang fp contract(off)
+  double d[3];
+  int key0[3];
+  for (int i = 0; i < 3; ++i) {
+    d[i] = pos[i] * G.ih[i];
+    key0[i] = (int)lround(d[i]);
+    d[i] -= key0[i];
+  }
+  for (int k = 0; k < 3; ++k) {
+    int key[3];
+    double d2[3];
+    for (int j = 0; j < 3; ++j) {
+      if (j != k && d[j] < 0) {
+        key[j] = key0[j] - 1;
+        d2[j] = 1 + d[j];
+      } else {
+        key[j] = key0[j];
+        d2[j] = d[j];
+      }
+    }
+    double w0[3], w1[3];
+    for (int i = 0; i < 3; ++i) {
+      if (i != k) {
+        w0[i] = 1 - d2[i];
+        w1[i] = d2[i];
+      } else {
+        w0[i] = 0.5 - d2[i];
+        w1[i] = 0.5 + d2[i];
+      }
+    }
+    double vx[2];
+    for (int a = 0; a < 2; ++a) {
+      double vy[2];
+      for (int bb = 0; bb < 2; ++bb) {
+        double s0 = speed_at(G, k, key[0] + a, key[1] + bb, key[2]);
+        double s1 = speed_at(G, k, key[0] + a, key[1] + bb, key[2] + 1);
+        vy[bb] = w0[2] * s0 + w1[2] * s1;
+      }
+      vx[a] = w0[1] * vy[0] + w1[1] * vy[1];
+    }
+    out[k] = w0[0] * vx[0] + w1[0] * vx[1];
+  }
+}
+
+__global__ void k_interpolate(int n, const double* __restrict__ pos, double* __restrict__ vel, SimGrid G) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double p[3] = {pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]}, o[3];
+  interp_speed(G, p, o);
+  vel[3 * i] = o[0];
+  vel[3 * i + 1] = o[1];
+  vel[3 * i + 2] = o[2];
+}
+
+// Algorithms_Move_Particles::Do (:22-36): rk == 4 is
+// Policy_Advance_Ode_RungeKutta::AdvanceSolveODE (:15-27), rk == 2 the
+// midpoint rule x += dt v(x + dt v(x) / 2).  Particles leaving [lo, hi) are
+// dropped; the kept ones are appended (order is not significant).
+__global__ void k_move_particles(int np, const double* __restrict__ px, const double* __restrict__ py,
+                                 const double* __restrict__ pz, double* __restrict__ qx, double* __restrict__ qy,
+                                 double* __restrict__ qz, int* __restrict__ nkept, SimGrid G, double dt, int rk, int use_box,
+                                 double lo0, double lo1, double lo2, double hi0, double hi1, double hi2) {
+#pragma clang fp contract(off)
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= np) return;
+  double pos[3] = {px[i], py[i], pz[i]};
+  double res[3] = {pos[0], pos[1], pos[2]};
+  double v[3], k[3], q[3];
+  if (rk == 4) {
+    interp_speed(G, pos, v);
+    for (int j = 0; j < 3; ++j) {
+      k[j] = dt * v[j];
+      res[j] += (1. / 6.) * k[j];
+      q[j] = pos[j] + k[j] * 0.5;
+    }
+    interp_speed(G, q, v);
+    for (int j = 0; j < 3; ++j) {
+      k[j] = dt * v[j];
+      res[j] += (1. / 3.) * k[j];
+      q[j] = pos[j] + k[j] * 0.5;
+    }
+    interp_speed(G, q, v);
+    for (int j = 0; j < 3; ++j) {
+      k[j] = dt * v[j];
+      res[j] += (1. / 3.) * k[j];
+      q[j] = pos[j] + k[j];
+    }
+    interp_speed(G, q, v);
+    for (int j = 0; j < 3; ++j) {
+      k[j] = dt * v[j];
+      res[j] += (1. / 6.) * k[j];
+    }
+  } else {
+    interp_speed(G, pos, v);
+    for (int j = 0; j < 3; ++j) {
+      k[j] = dt * v[j];
+      q[j] = pos[j] + k[j] * 0.5;
+    }
+    interp_speed(G, q, v);
+    for (int j = 0; j < 3; ++j) res[j] = pos[j] + dt * v[j];
+  }
+  if (use_box) {
+    if (!(res[0] >= lo0 && res[0] < hi0 && res[1] >= lo1 && res[1] < hi1 && res[2] >= lo2 && res[2] < hi2)) return;
+  }
+  int o = atomicAdd(nkept, 1);
+  qx[o] = res[0];
+  qy[o] = res[1];
+  qz[o] = res[2];
+}

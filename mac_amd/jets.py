@@ -1,0 +1,51 @@
+"""Seeded jet set-ups for the time-step loop (brick format of include/mac_amd.h).
+
+vertical_jet(): a box of nx x ny x nz cells (h per axis), open sides, solid
+floor and a square inflow nozzle centred in the top layer.  Only the floor
+and the nozzle exist at the start; fluid enters through the nozzle, whose
+empty cells are refilled with jittered particles every step.
+"""
+import numpy as np
+
+from .workloads import AIR, CONST, EXISTS, INFLOW, SOLID
+
+NODELETE = 0x40
+
+
+def _bricks_of(cells):
+    """cells: dict (x,y,z) -> (flag, w).  Returns keys, flags, w arrays."""
+    bkeys = sorted({(x >> 3, y >> 3, z >> 3) for (x, y, z) in cells}, key=lambda k: (k[2], k[1], k[0]))
+    index = {k: i for i, k in enumerate(bkeys)}
+    nb = len(bkeys)
+    flags = np.zeros((nb, 512), np.uint8)
+    w = np.zeros((nb, 512))
+    for (x, y, z), (f, wv) in cells.items():
+        b = index[(x >> 3, y >> 3, z >> 3)]
+        t = (x & 7) + 8 * (y & 7) + 64 * (z & 7)
+        flags[b, t] = f
+        w[b, t] = wv
+    return np.array(bkeys, np.int32), flags.reshape(-1), w.reshape(-1)
+
+
+def vertical_jet(nx=32, ny=32, nz=64, h=0.01, nozzle=4, inflow=1.0, floor=True):
+    """Config-1 style jet: returns dict(keys, flags, u, v, w, params) where
+    params are the keyword arguments of mac_amd.lib.Sim / oracle.Sim for the
+    box (particle deletion outside it)."""
+    cells = {}
+    if floor:
+        for x in range(nx):
+            for y in range(ny):
+                # the floor: solid, never deleted; the layer above keeps the
+                # floor face const (no flow through it, Neumann for p)
+                cells[(x, y, 0)] = (SOLID | EXISTS | NODELETE, 0.0)
+                cells[(x, y, 1)] = (AIR | EXISTS | NODELETE | CONST[2], 0.0)
+    x0, y0 = (nx - nozzle) // 2, (ny - nozzle) // 2
+    for x in range(x0, x0 + nozzle):
+        for y in range(y0, y0 + nozzle):
+            # inflow cell: its low z face carries the prescribed downward speed
+            cells[(x, y, nz - 1)] = (INFLOW | EXISTS | NODELETE | CONST[2], -float(inflow))
+    keys, flags, w = _bricks_of(cells)
+    n = flags.size
+    params = dict(h=(h, h, h), gravity=(0.0, 0.0, -9.81), box_lo=(-0.5 * h, -0.5 * h, -0.5 * h),
+                  box_hi=((nx - 0.5) * h, (ny - 0.5) * h, (nz - 0.5) * h))
+    return {"keys": keys, "flags": flags, "u": np.zeros(n), "v": np.zeros(n), "w": w, "params": params}

@@ -274,3 +274,160 @@ class LinearSolver:
 
     def Solve_Linear_Clean(self):
         self.mat = None
+
+
+# ---------------------------------------------------------------- time step --
+class SimParams(ctypes.Structure):
+    _fields_ = [
+        ("h", ctypes.c_double * 3),
+        ("gravity", ctypes.c_double * 3),
+        ("viscosity", ctypes.c_double),
+        ("cfl", ctypes.c_double),
+        ("dt_min", ctypes.c_double),
+        ("dt_max", ctypes.c_double),
+        ("layer_max", ctypes.c_int),
+        ("integrator", ctypes.c_int),
+        ("particle_rk", ctypes.c_int),
+        ("particles_per_inflow_cell", ctypes.c_int),
+        ("seed", ctypes.c_ulonglong),
+        ("box_lo", ctypes.c_double * 3),
+        ("box_hi", ctypes.c_double * 3),
+        ("max_particles", ctypes.c_int),
+        ("max_bricks", ctypes.c_int),
+        ("proj", Params),
+    ]
+
+
+class SimStats(ctypes.Structure):
+    _fields_ = [
+        ("dt", ctypes.c_double),
+        ("nbricks", ctypes.c_int),
+        ("nparticles", ctypes.c_int),
+        ("nfluid", ctypes.c_longlong),
+        ("pcg_iters", ctypes.c_int * 2),
+        ("max_div", ctypes.c_double),
+        ("ms_topology", ctypes.c_double),
+        ("ms_forces", ctypes.c_double),
+        ("ms_project", ctypes.c_double),
+        ("ms_particles", ctypes.c_double),
+        ("ms_total", ctypes.c_double),
+    ]
+
+    def as_dict(self):
+        d = {k: getattr(self, k) for k, _ in self._fields_}
+        d["pcg_iters"] = list(self.pcg_iters)
+        return d
+
+
+SIGNATURES.update({
+    "mac_sim_default_params": (None, [ctypes.POINTER(SimParams)]),
+    "mac_sim_create": (ctypes.c_int, [ctypes.POINTER(_P), ctypes.POINTER(SimParams)]),
+    "mac_sim_destroy": (None, [_P]),
+    "mac_sim_upload": (ctypes.c_int, [_P, ctypes.c_int, _P, _P, _P, _P, _P, _P, ctypes.c_int, _P]),
+    "mac_sim_counts": (ctypes.c_int, [_P, _ip, _ip]),
+    "mac_sim_download": (ctypes.c_int, [_P] + [_P] * 11),
+    "mac_sim_topology": (ctypes.c_int, [_P]),
+    "mac_sim_compute_dt": (ctypes.c_int, [_P, _dp]),
+    "mac_sim_forces": (ctypes.c_int, [_P, ctypes.c_int]),
+    "mac_sim_advance": (ctypes.c_int, [_P, ctypes.c_double, ctypes.POINTER(SimStats)]),
+    "mac_sim_move_particles": (ctypes.c_int, [_P, ctypes.c_double]),
+    "mac_sim_interpolate": (ctypes.c_int, [_P, ctypes.c_int, _P, _P]),
+    "mac_sim_step": (ctypes.c_int, [_P, ctypes.POINTER(SimStats)]),
+})
+
+
+def sim_params(**kw):
+    p = SimParams()
+    load().mac_sim_default_params(ctypes.byref(p))
+    for k, v in kw.items():
+        if k in ("h", "gravity", "box_lo", "box_hi"):
+            if np.isscalar(v):
+                v = (v, v, v)
+            arr = getattr(p, k)
+            for i in range(3):
+                arr[i] = float(v[i])
+        elif k == "proj":
+            for kk, vv in v.items():
+                setattr(p.proj, kk, vv)
+        else:
+            setattr(p, k, v)
+    for i in range(3):
+        p.proj.h[i] = p.h[i]
+    return p
+
+
+class Sim:
+    """Device-resident time stepper (mac_sim_* of include/mac_amd.h)."""
+
+    def __init__(self, **params):
+        self.lib = load()
+        self.params = sim_params(**params)
+        self.ptr = ctypes.c_void_p()
+        _check(self.lib.mac_sim_create(ctypes.byref(self.ptr), ctypes.byref(self.params)))
+
+    def close(self):
+        if self.ptr:
+            self.lib.mac_sim_destroy(self.ptr)
+            self.ptr = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def upload(self, keys, flags, layer=None, u=None, v=None, w=None, particles=None):
+        keys = np.ascontiguousarray(keys, dtype=np.int32).reshape(-1, 3)
+        nb = keys.shape[0]
+        flags = np.ascontiguousarray(flags, dtype=np.uint8).reshape(-1)
+        lay = None if layer is None else np.ascontiguousarray(layer, dtype=np.int8).reshape(-1)
+        uu = [None if a is None else np.ascontiguousarray(a, dtype=np.float64).reshape(-1) for a in (u, v, w)]
+        parts = np.zeros((0, 3)) if particles is None else np.ascontiguousarray(particles, dtype=np.float64).reshape(-1, 3)
+        _check(self.lib.mac_sim_upload(self.ptr, nb, _ptr(keys), _ptr(flags), _ptr(lay), *[_ptr(a) for a in uu],
+                                       parts.shape[0], _ptr(parts)))
+
+    def counts(self):
+        nb, npart = ctypes.c_int(), ctypes.c_int()
+        _check(self.lib.mac_sim_counts(self.ptr, ctypes.byref(nb), ctypes.byref(npart)))
+        return nb.value, npart.value
+
+    def download(self):
+        nb, npart = self.counts()
+        out = {"keys": np.zeros((nb, 3), np.int32), "flags": np.zeros(nb * 512, np.uint8),
+               "layer": np.zeros(nb * 512, np.int8)}
+        for k in ("u", "v", "w", "ax", "ay", "az", "p"):
+            out[k] = np.zeros(nb * 512)
+        out["particles"] = np.zeros((npart, 3))
+        _check(self.lib.mac_sim_download(self.ptr, *[_ptr(out[k]) for k in (
+            "keys", "flags", "layer", "u", "v", "w", "ax", "ay", "az", "p", "particles")]))
+        return out
+
+    def topology(self):
+        _check(self.lib.mac_sim_topology(self.ptr))
+
+    def compute_dt(self):
+        d = ctypes.c_double()
+        _check(self.lib.mac_sim_compute_dt(self.ptr, ctypes.byref(d)))
+        return d.value
+
+    def forces(self, slot=0):
+        _check(self.lib.mac_sim_forces(self.ptr, slot))
+
+    def advance(self, dt):
+        st = SimStats()
+        _check(self.lib.mac_sim_advance(self.ptr, dt, ctypes.byref(st)))
+        return st.as_dict()
+
+    def move_particles(self, dt):
+        _check(self.lib.mac_sim_move_particles(self.ptr, dt))
+
+    def interpolate(self, pos):
+        pos = np.ascontiguousarray(pos, dtype=np.float64).reshape(-1, 3)
+        vel = np.zeros_like(pos)
+        _check(self.lib.mac_sim_interpolate(self.ptr, pos.shape[0], _ptr(pos), _ptr(vel)))
+        return vel
+
+    def step(self):
+        st = SimStats()
+        _check(self.lib.mac_sim_step(self.ptr, ctypes.byref(st)))
+        return st.as_dict()

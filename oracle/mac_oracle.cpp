@@ -104,7 +104,7 @@ int orc_chunk_key_n(int k, int n) { return k >= 0 ? k / n : (k + 1) / n - 1; }
 // i (0-based) with sign dir; returns 1 when it wraps into the neighbouring
 // chunk (the table of Offset_GetNeighbour.h:12-43, restated digit-wise).
 int orc_offset_neighbour(int n, int dim, int i, int dir, int off, int* res) {
-  int dig[8], o = off;
+  int dig[8] = {0, 0, 0, 0, 0, 0, 0, 0}, o = off;
   for (int d = 0; d < dim; ++d) {
     dig[d] = o % n;
     o /= n;

@@ -168,3 +168,94 @@ def mgpcg(grid, rtol=1e-8, maxit=500, nu=2, omega=0.9, threads=0):
                           _p(out))
     return {"p": p, "iters": int(out[0]), "ms_solve": out[1], "ms_setup": out[2], "true_rel_resid": out[3],
             "levels": int(out[4]), "ok": rc == 0}
+
+
+# ---------------------------------------------------------------- time step --
+class Sim:
+    """CPU restatement of the time-step loop (sim_oracle.cpp)."""
+
+    def __init__(self, h=(0.01, 0.01, 0.01), gravity=(0, 0, -9.81), viscosity=0.0, cfl=0.5, dt_min=0.0, dt_max=1e30,
+                 layer_max=2, integrator=0, particle_rk=2, particles_per_inflow_cell=8, seed=0, box_lo=(0, 0, 0),
+                 box_hi=(0, 0, 0), **_ignored):
+        lib = load()
+        if not hasattr(lib, "_sim_bound"):
+            lib.orc_sim_create.restype = ctypes.c_void_p
+            lib.orc_sim_create.argtypes = [_P, _P, ctypes.c_ulonglong]
+            lib.orc_sim_destroy.argtypes = [_P]
+            lib.orc_sim_upload.argtypes = [_P, ctypes.c_int, _P, _P, _P, _P, _P, _P, ctypes.c_int, _P]
+            lib.orc_sim_counts.argtypes = [_P, _P, _P]
+            lib.orc_sim_download.argtypes = [_P] * 12
+            lib.orc_sim_topology.argtypes = [_P]
+            lib.orc_sim_compute_dt.restype = ctypes.c_double
+            lib.orc_sim_compute_dt.argtypes = [_P]
+            lib.orc_sim_forces.argtypes = [_P, ctypes.c_int]
+            lib.orc_sim_advance.argtypes = [_P, ctypes.c_double, _P]
+            lib.orc_sim_move.argtypes = [_P, ctypes.c_double]
+            lib.orc_sim_interpolate.argtypes = [_P, ctypes.c_int, _P, _P]
+            lib.orc_sim_step.restype = ctypes.c_double
+            lib.orc_sim_step.argtypes = [_P, _P]
+            lib._sim_bound = True
+        self.lib = lib
+        prm = np.array(list(h) + list(gravity) + [viscosity, cfl, dt_min, dt_max] + list(box_lo) + list(box_hi),
+                       dtype=np.float64)
+        iprm = np.array([layer_max, integrator, particle_rk, particles_per_inflow_cell], dtype=np.int32)
+        self.ptr = ctypes.c_void_p(lib.orc_sim_create(_p(prm), _p(iprm), seed))
+
+    def __del__(self):
+        try:
+            self.lib.orc_sim_destroy(self.ptr)
+        except Exception:
+            pass
+
+    def upload(self, keys, flags, layer=None, u=None, v=None, w=None, particles=None):
+        keys = _c(keys, np.int32).reshape(-1, 3)
+        flags = _c(flags, np.uint8)
+        lay = None if layer is None else _c(layer, np.int8)
+        uu = [None if a is None else _c(a, np.float64) for a in (u, v, w)]
+        parts = np.zeros((0, 3)) if particles is None else _c(particles, np.float64).reshape(-1, 3)
+        self.lib.orc_sim_upload(self.ptr, keys.shape[0], _p(keys), _p(flags), None if lay is None else _p(lay),
+                                *[None if a is None else _p(a) for a in uu], parts.shape[0], _p(parts))
+
+    def counts(self):
+        nb, npart = ctypes.c_int(), ctypes.c_int()
+        self.lib.orc_sim_counts(self.ptr, ctypes.byref(nb), ctypes.byref(npart))
+        return nb.value, npart.value
+
+    def download(self):
+        nb, npart = self.counts()
+        out = {"keys": np.zeros((nb, 3), np.int32), "flags": np.zeros(nb * 512, np.uint8),
+               "layer": np.zeros(nb * 512, np.int8)}
+        for k in ("u", "v", "w", "ax", "ay", "az", "p"):
+            out[k] = np.zeros(nb * 512)
+        out["particles"] = np.zeros((npart, 3))
+        self.lib.orc_sim_download(self.ptr, *[_p(out[k]) for k in (
+            "keys", "flags", "layer", "u", "v", "w", "ax", "ay", "az", "p", "particles")])
+        return out
+
+    def topology(self):
+        self.lib.orc_sim_topology(self.ptr)
+
+    def compute_dt(self):
+        return self.lib.orc_sim_compute_dt(self.ptr)
+
+    def forces(self, slot=0):
+        self.lib.orc_sim_forces(self.ptr, slot)
+
+    def advance(self, dt):
+        md = ctypes.c_double()
+        self.lib.orc_sim_advance(self.ptr, dt, ctypes.byref(md))
+        return {"max_div": md.value}
+
+    def move_particles(self, dt):
+        self.lib.orc_sim_move(self.ptr, dt)
+
+    def interpolate(self, pos):
+        pos = _c(pos, np.float64).reshape(-1, 3)
+        vel = np.zeros_like(pos)
+        self.lib.orc_sim_interpolate(self.ptr, pos.shape[0], _p(pos), _p(vel))
+        return vel
+
+    def step(self):
+        md = ctypes.c_double()
+        dt = self.lib.orc_sim_step(self.ptr, ctypes.byref(md))
+        return {"dt": dt, "max_div": md.value}

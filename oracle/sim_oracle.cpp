@@ -1,0 +1,532 @@
+// sim_oracle.cpp -- CPU oracle for the per-timestep loop (topology, dt,
+// forces, RK2, interpolation, particles).  TEST INFRASTRUCTURE ONLY (see
+// mac_oracle.cpp's header): loaded by tests/ as the checker, never by the
+// product.
+//
+// A sequential restatement of the reference algorithms on a sparse chunk map
+// (std::map of chunk key -> 8^3 chunk), following the reference's own control
+// flow: stack-based layer BFS (Algorithms_Update_CellType_Layer.h:22-95),
+// grid[key] inserting cells (ChunkHashTable.tpp:63-87), cell erase and chunk
+// free (:133-161), per-particle sequential loops.
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <stack>
+#include <tuple>
+#include <vector>
+
+extern "C" int orc_project(int nb, const int32_t* keys, const uint8_t* flags, const double* h, double* u, double* v, double* w,
+                           double* p, double rtol, double* maxdiv);
+
+namespace {
+
+struct K3 {
+  int x, y, z;
+  bool operator<(const K3& o) const { return std::tie(z, y, x) < std::tie(o.z, o.y, o.x); }
+};
+
+struct Chunk {
+  uint8_t flags[512];
+  int8_t layer[512];
+  double S[2][3][512];
+  double A[3][512];
+  double P[512];
+  Chunk() {
+    // the prototype cell: Air, no const, not existing, layer empty, zero fields
+    memset(flags, 0, sizeof(flags));
+    memset(layer, -1, sizeof(layer));
+    memset(S, 0, sizeof(S));
+    memset(A, 0, sizeof(A));
+    memset(P, 0, sizeof(P));
+  }
+  bool any() const {
+    for (int t = 0; t < 512; ++t)
+      if (flags[t] & 0x80) return true;
+    return false;
+  }
+};
+
+inline int chunk_of(int k) { return k >= 0 ? k / 8 : (k + 1) / 8 - 1; }
+inline int off_of(int kx, int ky, int kz) { return (kx & 7) + 8 * (ky & 7) + 64 * (kz & 7); }
+
+inline bool is_fluid(unsigned m) { return m == 1 || m == 3 || m == 4 || m == 5; }
+inline unsigned set_fluid(unsigned m) { return (m == 0 || m == 4) ? 1 : m; }
+inline unsigned set_air(unsigned m) { return (m == 1 || m == 4) ? 0 : m; }
+
+// same counter-based generator as the library's jittered inflow fill
+inline double uni(uint64_t seed, uint64_t stream, uint64_t a, uint64_t b) {
+  uint64_t z = seed * 0xD1B54A32D192ED03ull + stream * 0x8CB92BA72F3D8DD7ull + a * 0x9E3779B97F4A7C15ull + (b + 1) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  z ^= z >> 31;
+  return ((double)(z >> 11) + 0.5) * (1.0 / 9007199254740992.0);
+}
+inline uint64_t pack(int x, int y, int z) {
+  return ((uint64_t)(uint32_t)(x + (1 << 20)) << 42) | ((uint64_t)(uint32_t)(y + (1 << 20)) << 21) | (uint64_t)(uint32_t)(z + (1 << 20));
+}
+
+struct Params {
+  double h[3], g[3], nu, cfl, dt_min, dt_max;
+  int layer_max, integrator, particle_rk, ppc;
+  uint64_t seed;
+  double lo[3], hi[3];
+};
+
+struct Sim {
+  Params P;
+  double ih[3];
+  std::map<K3, std::unique_ptr<Chunk>> grid;
+  std::vector<std::array<double, 3>> parts;
+  uint64_t tick = 0;
+
+  Chunk* chunk(int kx, int ky, int kz) {
+    auto it = grid.find({chunk_of(kx), chunk_of(ky), chunk_of(kz)});
+    return it == grid.end() ? nullptr : it->second.get();
+  }
+  // grid[key]: creates the chunk from the prototype if needed, sets the bit
+  Chunk* insert(int kx, int ky, int kz) {
+    K3 c{chunk_of(kx), chunk_of(ky), chunk_of(kz)};
+    auto it = grid.find(c);
+    if (it == grid.end()) it = grid.emplace(c, std::make_unique<Chunk>()).first;
+    it->second->flags[off_of(kx, ky, kz)] |= 0x80;
+    return it->second.get();
+  }
+  bool valid(int kx, int ky, int kz) {
+    Chunk* ch = chunk(kx, ky, kz);
+    return ch && (ch->flags[off_of(kx, ky, kz)] & 0x80);
+  }
+  // a speed read through GetNeighbour: the chunk's memory if the chunk exists
+  double speed(int slot, int comp, int kx, int ky, int kz) {
+    Chunk* ch = chunk(kx, ky, kz);
+    return ch ? ch->S[slot][comp][off_of(kx, ky, kz)] : 0.0;
+  }
+  bool fluid(int kx, int ky, int kz) {
+    Chunk* ch = chunk(kx, ky, kz);
+    if (!ch) return false;
+    unsigned f = ch->flags[off_of(kx, ky, kz)];
+    return (f & 0x80) && is_fluid(f & 7);
+  }
+  int8_t layer_of(int kx, int ky, int kz) {
+    Chunk* ch = chunk(kx, ky, kz);
+    return ch ? ch->layer[off_of(kx, ky, kz)] : -1;
+  }
+  template <typename F>
+  void for_cells(F f) {
+    for (auto& kv : grid)
+      for (int t = 0; t < 512; ++t)
+        if (kv.second->flags[t] & 0x80)
+          f(kv.second.get(), t, kv.first.x * 8 + (t & 7), kv.first.y * 8 + ((t >> 3) & 7), kv.first.z * 8 + (t >> 6));
+  }
+  void key_of(const std::array<double, 3>& p, int k[3]) {
+    for (int i = 0; i < 3; ++i) k[i] = (int)lround(p[i] * ih[i]);
+  }
+
+  void topology() {
+    // Algorithms_Initialize_MacCell
+    for_cells([&](Chunk* c, int t, int, int, int) { c->layer[t] = -1; });
+    // Algorithms_Layer_Initial_With_Particle
+    for (auto& p : parts) {
+      int k[3];
+      key_of(p, k);
+      Chunk* c = insert(k[0], k[1], k[2]);
+      int o = off_of(k[0], k[1], k[2]);
+      c->layer[o] = 0;
+      c->flags[o] = (uint8_t)((c->flags[o] & ~7u) | set_fluid(c->flags[o] & 7u));
+    }
+    // Algorithms_Create_Fluid_Particle with the jitter policy
+    for_cells([&](Chunk* c, int t, int kx, int ky, int kz) {
+      unsigned f = c->flags[t];
+      if ((f & 7u) == 3u && c->layer[t] == -1) {
+        uint64_t ck = pack(kx, ky, kz);
+        for (int k = 0; k < P.ppc; ++k) {
+          double ux = uni(P.seed, tick, ck, 3 * k), uy = uni(P.seed, tick, ck, 3 * k + 1), uz = uni(P.seed, tick, ck, 3 * k + 2);
+          parts.push_back({((double)kx + (ux - 0.5)) * P.h[0], ((double)ky + (uy - 0.5)) * P.h[1], ((double)kz + (uz - 0.5)) * P.h[2]});
+        }
+        c->layer[t] = 0;
+        c->flags[t] = (uint8_t)((f & ~7u) | set_fluid(f & 7u));
+      }
+    });
+    // Algorithms_Update_CellType_Layer (stack BFS)
+    std::stack<std::array<int, 3>> s, s2;
+    for_cells([&](Chunk* c, int t, int kx, int ky, int kz) {
+      if (c->layer[t] != 0) return;
+      bool b = false;
+      for (int i = 0; i < 3 && !b; ++i)
+        for (int si = -1; si <= 1; si += 2) {
+          int n[3] = {kx, ky, kz};
+          n[i] += si;
+          if (!valid(n[0], n[1], n[2]) || layer_of(n[0], n[1], n[2]) == -1) {
+            s.push({kx, ky, kz});
+            b = true;
+            break;
+          }
+        }
+    });
+    for (int lay = 1; lay <= P.layer_max; ++lay) {
+      while (!s.empty()) {
+        auto k = s.top();
+        s.pop();
+        for (int i = 0; i < 3; ++i)
+          for (int si = -1; si <= 1; si += 2) {
+            int n[3] = {k[0], k[1], k[2]};
+            n[i] += si;
+            if (!valid(n[0], n[1], n[2]) || layer_of(n[0], n[1], n[2]) == -1) {
+              Chunk* c = insert(n[0], n[1], n[2]);
+              int o = off_of(n[0], n[1], n[2]);
+              c->layer[o] = (int8_t)lay;
+              c->flags[o] = (uint8_t)((c->flags[o] & ~7u) | set_air(c->flags[o] & 7u));
+              s2.push({n[0], n[1], n[2]});
+            }
+          }
+      }
+      std::swap(s, s2);
+    }
+    while (!s.empty()) {
+      auto k = s.top();
+      s.pop();
+      for (int i = 0; i < 3; ++i) {
+        int n[3] = {k[0], k[1], k[2]};
+        n[i] += 1;
+        if (!valid(n[0], n[1], n[2]) || layer_of(n[0], n[1], n[2]) == -1) {
+          Chunk* c = insert(n[0], n[1], n[2]);
+          int o = off_of(n[0], n[1], n[2]);
+          c->layer[o] = (int8_t)(P.layer_max + 1);
+          c->flags[o] = (uint8_t)((c->flags[o] & ~7u) | set_air(c->flags[o] & 7u));
+        }
+      }
+    }
+    // Algorithms_Delete_MacCell: erase layer-empty cells unless NoDelete, free empty chunks
+    for (auto it = grid.begin(); it != grid.end();) {
+      Chunk* c = it->second.get();
+      for (int t = 0; t < 512; ++t)
+        if ((c->flags[t] & 0x80) && c->layer[t] == -1 && !(c->flags[t] & 0x40)) c->flags[t] &= (uint8_t)~0x80u;
+      if (!c->any()) it = grid.erase(it);
+      else ++it;
+    }
+    tick++;
+  }
+
+  double compute_dt() {
+    double mx = 0;
+    for_cells([&](Chunk* c, int t, int, int, int) {
+      if (!is_fluid(c->flags[t] & 7u)) return;
+      double temp = 0;
+      for (int i = 0; i < 3; ++i) temp += std::pow(c->S[0][i][t] * ih[i], 2);
+      if (temp > mx) mx = temp;
+    });
+    double dt = mx > 0 ? 1 / std::sqrt(mx) * P.cfl : HUGE_VAL;
+    if (dt < P.dt_min) dt = P.dt_min;
+    if (dt > P.dt_max) dt = P.dt_max;
+    return dt;
+  }
+
+  void forces(int slot) {
+    // Acceleration SetZero(true), then Algorithms<Gravity, Viscosity, Convection>
+    for_cells([&](Chunk* c, int t, int, int, int) {
+      for (int i = 0; i < 3; ++i) c->A[i][t] = 0.0;
+    });
+    // Policy_Gravity::ApplyGravity
+    for (int i = 0; i < 3; ++i) {
+      if (P.g[i] == 0.0) continue;
+      for_cells([&](Chunk* c, int t, int kx, int ky, int kz) {
+        bool need = is_fluid(c->flags[t] & 7u);
+        if (!need && c->layer[t] != -1) {
+          int n[3] = {kx, ky, kz};
+          n[i] -= 1;
+          need = valid(n[0], n[1], n[2]) && fluid(n[0], n[1], n[2]);
+        }
+        if (need && !(c->flags[t] & (0x08u << i))) c->A[i][t] = c->A[i][t] + P.g[i];
+      });
+    }
+    // Algorithms_Viscosity
+    if (P.nu != 0.0) {
+      for_cells([&](Chunk* c, int t, int kx, int ky, int kz) {
+        bool app = is_fluid(c->flags[t] & 7u);
+        if (!app && c->layer[t] != -1)
+          for (int i = 0; i < 3; ++i) {
+            int n[3] = {kx, ky, kz};
+            n[i] -= 1;
+            if (valid(n[0], n[1], n[2]) && fluid(n[0], n[1], n[2])) app = true;
+          }
+        if (!app) return;
+        double lap[3];
+        for (int i = 0; i < 3; ++i) {
+          double temp = 0;
+          for (int j = 0; j < 3; ++j) {
+            int p1[3] = {kx, ky, kz}, m1[3] = {kx, ky, kz};
+            p1[j] += 1;
+            m1[j] -= 1;
+            temp -= 2 * c->S[slot][i][t] * std::pow(ih[j], 2);
+            temp += speed(slot, i, p1[0], p1[1], p1[2]) * std::pow(ih[j], 2);
+            temp += speed(slot, i, m1[0], m1[1], m1[2]) * std::pow(ih[j], 2);
+          }
+          lap[i] = temp;
+        }
+        for (int i = 0; i < 3; ++i)
+          if (!(c->flags[t] & (0x08u << i))) c->A[i][t] = c->A[i][t] + lap[i] * P.nu;
+      });
+    }
+    // Algorithms_Convection with Policy_Upwind_1_Order
+    for_cells([&](Chunk* c, int t, int kx, int ky, int kz) {
+      int K[3] = {kx, ky, kz};
+      for (int i = 0; i < 3; ++i) {
+        bool app = is_fluid(c->flags[t] & 7u);
+        if (!app && c->layer[t] != -1) {
+          int n[3] = {kx, ky, kz};
+          n[i] -= 1;
+          app = valid(n[0], n[1], n[2]) && fluid(n[0], n[1], n[2]);
+        }
+        if (!app) continue;
+        double temp = 0;
+        for (int j = 0; j < 3; ++j) {
+          double U;
+          if (i == j) {
+            U = c->S[slot][j][t];
+          } else {
+            int a[3] = {K[0], K[1], K[2]}, b[3] = {K[0], K[1], K[2]}, d[3] = {K[0], K[1], K[2]};
+            a[j] += 1;
+            b[i] -= 1;
+            d[i] -= 1;
+            d[j] += 1;
+            U = 0.25 * (c->S[slot][j][t] + speed(slot, j, a[0], a[1], a[2]) + speed(slot, j, b[0], b[1], b[2]) +
+                        speed(slot, j, d[0], d[1], d[2]));
+          }
+          double U1 = c->S[slot][i][t], U2;
+          int n[3] = {K[0], K[1], K[2]};
+          if (U > 0) {
+            n[j] -= 1;
+            U2 = speed(slot, i, n[0], n[1], n[2]);
+            U *= (U1 - U2) * ih[j];
+          } else {
+            n[j] += 1;
+            U2 = speed(slot, i, n[0], n[1], n[2]);
+            U *= (U2 - U1) * ih[j];
+          }
+          temp -= U;
+        }
+        if (!(c->flags[t] & (0x08u << i))) c->A[i][t] = c->A[i][t] + temp;
+      }
+    });
+  }
+
+  void project(int slot, double* maxdiv) {
+    std::vector<int32_t> keys;
+    std::vector<uint8_t> fl;
+    std::vector<double> u, v, w, p;
+    std::vector<Chunk*> chs;
+    for (auto& kv : grid) {
+      keys.push_back(kv.first.x);
+      keys.push_back(kv.first.y);
+      keys.push_back(kv.first.z);
+      chs.push_back(kv.second.get());
+    }
+    int nb = (int)chs.size();
+    fl.resize((size_t)nb * 512);
+    u.resize((size_t)nb * 512);
+    v.resize(u.size());
+    w.resize(u.size());
+    p.resize(u.size());
+    for (int b = 0; b < nb; ++b) {
+      memcpy(&fl[(size_t)b * 512], chs[b]->flags, 512);
+      memcpy(&u[(size_t)b * 512], chs[b]->S[slot][0], 512 * 8);
+      memcpy(&v[(size_t)b * 512], chs[b]->S[slot][1], 512 * 8);
+      memcpy(&w[(size_t)b * 512], chs[b]->S[slot][2], 512 * 8);
+    }
+    if (nb > 0) orc_project(nb, keys.data(), fl.data(), P.h, u.data(), v.data(), w.data(), p.data(), 1e-14, maxdiv);
+    for (int b = 0; b < nb; ++b) {
+      memcpy(chs[b]->S[slot][0], &u[(size_t)b * 512], 512 * 8);
+      memcpy(chs[b]->S[slot][1], &v[(size_t)b * 512], 512 * 8);
+      memcpy(chs[b]->S[slot][2], &w[(size_t)b * 512], 512 * 8);
+      memcpy(chs[b]->P, &p[(size_t)b * 512], 512 * 8);
+    }
+  }
+
+  void advance(double dt, double* maxdiv) {
+    bool tvd = P.integrator == 1;
+    forces(0);
+    for_cells([&](Chunk* c, int t, int, int, int) {
+      for (int i = 0; i < 3; ++i)
+        c->S[1][i][t] = tvd ? c->S[0][i][t] + c->A[i][t] * dt : c->S[0][i][t] + c->A[i][t] * dt * 0.5;
+      for (int i = 0; i < 3; ++i) c->A[i][t] = 0.0;
+    });
+    project(1, maxdiv);
+    forces(1);
+    for_cells([&](Chunk* c, int t, int, int, int) {
+      for (int i = 0; i < 3; ++i) {
+        if (c->flags[t] & (0x08u << i)) continue;
+        c->S[0][i][t] = tvd ? 0.5 * c->S[0][i][t] + 0.5 * c->S[1][i][t] + 0.5 * c->A[i][t] * dt : c->S[0][i][t] + c->A[i][t] * dt;
+      }
+    });
+    project(0, maxdiv);
+  }
+
+  // Policy_Speed_Interpolation_Linear_Symmetric::Get_Speed, recursive as in the reference
+  double impl(const double d2[3], int i, int kx, int ky, int kz, int k) {
+    int n[3] = {kx, ky, kz};
+    n[i] += 1;
+    double w0 = (i != k) ? (1 - d2[i]) : (0.5 - d2[i]);
+    double w1 = (i != k) ? d2[i] : (0.5 + d2[i]);
+    if (i < 2) return w0 * impl(d2, i + 1, kx, ky, kz, k) + w1 * impl(d2, i + 1, n[0], n[1], n[2], k);
+    return w0 * speed(0, k, kx, ky, kz) + w1 * speed(0, k, n[0], n[1], n[2]);
+  }
+  std::array<double, 3> get_speed(const std::array<double, 3>& pos) {
+    double d[3];
+    int key0[3];
+    for (int i = 0; i < 3; ++i) {
+      d[i] = pos[i] * ih[i];
+      key0[i] = (int)lround(d[i]);
+      d[i] -= key0[i];
+    }
+    std::array<double, 3> ret;
+    for (int k = 0; k < 3; ++k) {
+      int key[3];
+      double d2[3];
+      for (int j = 0; j < 3; ++j) {
+        if (j != k && d[j] < 0) {
+          key[j] = key0[j] - 1;
+          d2[j] = 1 + d[j];
+        } else {
+          key[j] = key0[j];
+          d2[j] = d[j];
+        }
+      }
+      ret[k] = impl(d2, 0, key[0], key[1], key[2], k);
+    }
+    return ret;
+  }
+
+  void move(double dt) {
+    std::vector<std::array<double, 3>> out;
+    bool box = !(P.lo[0] == P.hi[0] && P.lo[1] == P.hi[1] && P.lo[2] == P.hi[2]);
+    for (auto& pos : parts) {
+      std::array<double, 3> res = pos, k, q, v;
+      if (P.particle_rk == 4) {
+        v = get_speed(pos);
+        for (int j = 0; j < 3; ++j) { k[j] = dt * v[j]; res[j] += (1. / 6.) * k[j]; q[j] = pos[j] + k[j] * 0.5; }
+        v = get_speed(q);
+        for (int j = 0; j < 3; ++j) { k[j] = dt * v[j]; res[j] += (1. / 3.) * k[j]; q[j] = pos[j] + k[j] * 0.5; }
+        v = get_speed(q);
+        for (int j = 0; j < 3; ++j) { k[j] = dt * v[j]; res[j] += (1. / 3.) * k[j]; q[j] = pos[j] + k[j]; }
+        v = get_speed(q);
+        for (int j = 0; j < 3; ++j) { k[j] = dt * v[j]; res[j] += (1. / 6.) * k[j]; }
+      } else {
+        v = get_speed(pos);
+        for (int j = 0; j < 3; ++j) { k[j] = dt * v[j]; q[j] = pos[j] + k[j] * 0.5; }
+        v = get_speed(q);
+        for (int j = 0; j < 3; ++j) res[j] = pos[j] + dt * v[j];
+      }
+      if (box && !(res[0] >= P.lo[0] && res[0] < P.hi[0] && res[1] >= P.lo[1] && res[1] < P.hi[1] && res[2] >= P.lo[2] && res[2] < P.hi[2]))
+        continue;
+      out.push_back(res);
+    }
+    parts.swap(out);
+  }
+};
+
+}  // namespace
+
+extern "C" {
+
+// prm: h[3], g[3], nu, cfl, dt_min, dt_max, lo[3], hi[3] (16 doubles);
+// iprm: layer_max, integrator, particle_rk, ppc; seed.
+void* orc_sim_create(const double* prm, const int* iprm, unsigned long long seed) {
+  Sim* s = new Sim();
+  for (int i = 0; i < 3; ++i) {
+    s->P.h[i] = prm[i];
+    s->P.g[i] = prm[3 + i];
+    s->P.lo[i] = prm[10 + i];
+    s->P.hi[i] = prm[13 + i];
+    s->ih[i] = 1.0 / prm[i];
+  }
+  s->P.nu = prm[6];
+  s->P.cfl = prm[7];
+  s->P.dt_min = prm[8];
+  s->P.dt_max = prm[9];
+  s->P.layer_max = iprm[0];
+  s->P.integrator = iprm[1];
+  s->P.particle_rk = iprm[2];
+  s->P.ppc = iprm[3];
+  s->P.seed = seed;
+  return s;
+}
+
+void orc_sim_destroy(void* p) { delete (Sim*)p; }
+
+void orc_sim_upload(void* p, int nb, const int32_t* keys, const uint8_t* flags, const int8_t* layer, const double* u, const double* v,
+                    const double* w, int np, const double* pxyz) {
+  Sim* s = (Sim*)p;
+  s->grid.clear();
+  for (int b = 0; b < nb; ++b) {
+    auto c = std::make_unique<Chunk>();
+    memcpy(c->flags, &flags[(size_t)b * 512], 512);
+    if (layer) memcpy(c->layer, &layer[(size_t)b * 512], 512);
+    const double* src[3] = {u, v, w};
+    for (int i = 0; i < 3; ++i)
+      if (src[i]) {
+        memcpy(c->S[0][i], &src[i][(size_t)b * 512], 512 * 8);
+        memcpy(c->S[1][i], &src[i][(size_t)b * 512], 512 * 8);
+      }
+    s->grid.emplace(K3{keys[3 * b], keys[3 * b + 1], keys[3 * b + 2]}, std::move(c));
+  }
+  s->parts.clear();
+  for (int k = 0; k < np; ++k) s->parts.push_back({pxyz[3 * k], pxyz[3 * k + 1], pxyz[3 * k + 2]});
+}
+
+void orc_sim_counts(void* p, int* nb, int* np) {
+  Sim* s = (Sim*)p;
+  *nb = (int)s->grid.size();
+  *np = (int)s->parts.size();
+}
+
+// Bricks in ascending (z, y, x) chunk-key order, like the library.
+void orc_sim_download(void* p, int32_t* keys, uint8_t* flags, int8_t* layer, double* u, double* v, double* w, double* ax, double* ay,
+                      double* az, double* pr, double* pxyz) {
+  Sim* s = (Sim*)p;
+  int b = 0;
+  for (auto& kv : s->grid) {
+    Chunk* c = kv.second.get();
+    if (keys) {
+      keys[3 * b] = kv.first.x;
+      keys[3 * b + 1] = kv.first.y;
+      keys[3 * b + 2] = kv.first.z;
+    }
+    if (flags) memcpy(&flags[(size_t)b * 512], c->flags, 512);
+    if (layer) memcpy(&layer[(size_t)b * 512], c->layer, 512);
+    double* dst[7] = {u, v, w, ax, ay, az, pr};
+    const double* src[7] = {c->S[0][0], c->S[0][1], c->S[0][2], c->A[0], c->A[1], c->A[2], c->P};
+    for (int i = 0; i < 7; ++i)
+      if (dst[i]) memcpy(&dst[i][(size_t)b * 512], src[i], 512 * 8);
+    ++b;
+  }
+  if (pxyz)
+    for (size_t k = 0; k < s->parts.size(); ++k)
+      for (int i = 0; i < 3; ++i) pxyz[3 * k + i] = s->parts[k][i];
+}
+
+void orc_sim_topology(void* p) { ((Sim*)p)->topology(); }
+double orc_sim_compute_dt(void* p) { return ((Sim*)p)->compute_dt(); }
+void orc_sim_forces(void* p, int slot) { ((Sim*)p)->forces(slot); }
+void orc_sim_advance(void* p, double dt, double* maxdiv) { ((Sim*)p)->advance(dt, maxdiv); }
+void orc_sim_move(void* p, double dt) { ((Sim*)p)->move(dt); }
+void orc_sim_interpolate(void* p, int n, const double* pos, double* vel) {
+  Sim* s = (Sim*)p;
+  for (int k = 0; k < n; ++k) {
+    auto v = s->get_speed({pos[3 * k], pos[3 * k + 1], pos[3 * k + 2]});
+    for (int i = 0; i < 3; ++i) vel[3 * k + i] = v[i];
+  }
+}
+// One step in the library's order: topology, dt, advance, move.
+double orc_sim_step(void* p, double* maxdiv) {
+  Sim* s = (Sim*)p;
+  s->topology();
+  double dt = s->compute_dt();
+  s->advance(dt, maxdiv);
+  s->move(dt);
+  return dt;
+}
+
+}  // extern "C"

@@ -1,0 +1,133 @@
+"""CPU checks pinning the time-step oracle (sim_oracle.cpp) with the
+reference's known answers and with properties the mathematics fixes."""
+import numpy as np
+import pytest
+
+from mac_amd import jets as J
+from mac_amd import workloads as W
+from oracle import oracle as O
+
+
+def single_cell(layer_max=1):
+    """One fluid particle in cell (0,0,0); nothing else exists."""
+    s = O.Sim(h=(1.0, 1.0, 1.0), layer_max=layer_max, gravity=(0, 0, 0), particles_per_inflow_cell=0)
+    s.upload(np.zeros((0, 3), np.int32), np.zeros(0, np.uint8), particles=np.array([[0.1, -0.2, 0.3]]))
+    s.topology()
+    return s.download()
+
+
+def cell_layers(d):
+    out = {}
+    for b, k in enumerate(d["keys"]):
+        for t in range(512):
+            if d["flags"][b * 512 + t] & W.EXISTS:
+                out[(8 * k[0] + (t & 7), 8 * k[1] + ((t >> 3) & 7), 8 * k[2] + (t >> 6))] = (
+                    int(d["layer"][b * 512 + t]), int(d["flags"][b * 512 + t] & 7))
+    return out
+
+
+def test_layer_kat_2d_slice():
+    """TestAlgorithms_Update_CellType_Layer.h:43-150 (LayerMax = 1, one fluid
+    cell): the 4 in-plane neighbours get layer 1 and (2,0), (1,1), (1,-1),
+    (0,2), (-1,1) get layer 2 (the +1-only extra layer).  In 3-D the z = 0
+    slice reproduces the 2-D answer."""
+    cl = cell_layers(single_cell(1))
+    sl = {(x, y): v for (x, y, z), v in cl.items() if z == 0}
+    assert sl[(0, 0)] == (0, W.FLUID)
+    for c in [(1, 0), (-1, 0), (0, 1), (0, -1)]:
+        assert sl[c] == (1, W.AIR)
+    for c in [(2, 0), (1, 1), (1, -1), (0, 2), (-1, 1)]:
+        assert sl[c] == (2, W.AIR)
+    assert len(sl) == 10
+
+
+def test_layer_bfs_matches_distance():
+    """Layers are the L1 distance to the fluid up to layer_max, and the extra
+    layer is exactly the empty +e_i neighbours of the last layer."""
+    for lmax in (1, 2, 3):
+        cl = cell_layers(single_cell(lmax))
+        for (x, y, z), (lay, mat) in cl.items():
+            dist = abs(x) + abs(y) + abs(z)
+            if dist <= lmax:
+                assert lay == dist
+            else:
+                assert lay == lmax + 1
+                assert any(abs(x - (i == 0)) + abs(y - (i == 1)) + abs(z - (i == 2)) == lmax for i in range(3))
+        brute = {(x, y, z) for x in range(-5, 6) for y in range(-5, 6) for z in range(-5, 6)
+                 if abs(x) + abs(y) + abs(z) <= lmax}
+        extra = {(x + (i == 0), y + (i == 1), z + (i == 2)) for (x, y, z) in brute
+                 if abs(x) + abs(y) + abs(z) == lmax for i in range(3)} - brute
+        assert set(cl) == brute | extra
+
+
+def test_delete_frees_cells_and_chunks():
+    """Algorithms_Delete_MacCell: when the particle moves away, the old cells
+    are erased and empty chunks freed."""
+    s = O.Sim(h=(1.0, 1.0, 1.0), layer_max=1, gravity=(0, 0, 0), particles_per_inflow_cell=0)
+    s.upload(np.zeros((0, 3), np.int32), np.zeros(0, np.uint8), particles=np.array([[0.0, 0.0, 0.0]]))
+    s.topology()
+    d0 = s.download()
+    # the origin chunk and the three chunks reached by the -x, -y, -z neighbours
+    assert {tuple(k) for k in d0["keys"]} == {(0, 0, 0), (-1, 0, 0), (0, -1, 0), (0, 0, -1)}
+    s.upload(d0["keys"], d0["flags"], d0["layer"], particles=np.array([[40.0, 40.0, 40.0]]))
+    s.topology()
+    d1 = s.download()
+    assert {tuple(int(c) for c in k) for k in d1["keys"]} == {(5, 5, 5), (4, 5, 5), (5, 4, 5), (5, 5, 4)}
+
+
+def test_inflow_fill_jitter_inside_cell():
+    jet = J.vertical_jet(8, 8, 16, h=0.5, nozzle=2)
+    s = O.Sim(**jet["params"], layer_max=2)
+    s.upload(jet["keys"], jet["flags"], u=jet["u"], v=jet["v"], w=jet["w"])
+    s.topology()
+    d = s.download()
+    parts = d["particles"]
+    assert parts.shape == (4 * 8, 3)
+    keys = np.rint(parts / 0.5).astype(int)
+    assert set(map(tuple, keys)) == {(x, y, 15) for x in (3, 4) for y in (3, 4)}
+
+
+def test_interpolation_reproduces_linear_fields():
+    """Staggered trilinear interpolation is exact for any linear field
+    (Policy_Speed_Interpolation_Linear_Symmetric)."""
+    g = W.random_grid(3, 3, 3, 3, fluid_p=1.0, solid_p=0.0, const_p=0.0)
+    g["flags"][:] = W.FLUID | W.EXISTS
+    keys = g["keys"]
+    n = keys.shape[0]
+    t = np.arange(512)
+    gx = (keys[:, 0:1] * 8 + (t & 7)[None, :]).reshape(-1).astype(float)
+    gy = (keys[:, 1:2] * 8 + ((t >> 3) & 7)[None, :]).reshape(-1).astype(float)
+    gz = (keys[:, 2:3] * 8 + (t >> 6)[None, :]).reshape(-1).astype(float)
+    h = 0.5
+    A = np.array([[0.3, -1.2, 0.7, 0.1], [2.0, 0.5, -0.4, -0.3], [-0.6, 0.2, 1.1, 0.9]])
+    # component k lives on the low face of cell (x,y,z): position (x,y,z) - 0.5 e_k, times h
+    comps = []
+    for k in range(3):
+        X = np.stack([gx - 0.5 * (k == 0), gy - 0.5 * (k == 1), gz - 0.5 * (k == 2)]) * h
+        comps.append(A[k, 0] * X[0] + A[k, 1] * X[1] + A[k, 2] * X[2] + A[k, 3])
+    s = O.Sim(h=(h, h, h))
+    s.upload(keys, g["flags"], u=comps[0], v=comps[1], w=comps[2])
+    rng = np.random.default_rng(0)
+    pos = rng.uniform(2.0, 20.0, size=(200, 3)) * h
+    vel = s.interpolate(pos)
+    exact = pos @ A[:, :3].T + A[:, 3]
+    assert np.allclose(vel, exact, rtol=0, atol=1e-12)
+
+
+def test_upwind_of_uniform_field_is_zero_and_gravity_on_fluid():
+    """First-order upwind of a uniform velocity is 0; gravity lands on the
+    faces of fluid cells and of non-empty cells above fluid."""
+    jet = J.vertical_jet(16, 16, 16, h=0.1, nozzle=2, floor=False)
+    g = W.random_grid(5, 2, 2, 2, fluid_p=1.0, solid_p=0.0, const_p=0.0)
+    flags = np.full(g["flags"].size, W.FLUID | W.EXISTS, np.uint8)
+    n = flags.size
+    s = O.Sim(h=(0.1, 0.1, 0.1), gravity=(0, 0, -9.81))
+    s.upload(g["keys"], flags, np.zeros(n, np.int8), u=np.full(n, 0.7), v=np.full(n, -0.2), w=np.full(n, 0.4))
+    s.forces(0)
+    d = s.download()
+    # interior cells (away from the box boundary, where missing neighbours read 0)
+    dense = {k: W.bricks_to_dense(d[k].reshape(-1, 512), 2, 2, 2) for k in ("ax", "ay", "az")}
+    inner = (slice(2, -2),) * 3
+    assert np.all(dense["ax"][inner] == 0.0) and np.all(dense["ay"][inner] == 0.0)
+    assert np.allclose(dense["az"][inner], -9.81, rtol=0, atol=1e-12)
+    del jet

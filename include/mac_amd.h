@@ -64,10 +64,13 @@ typedef struct {
   int nu;          /* damped-Jacobi sweeps before and after each coarse correction */
   double omega;    /* Jacobi damping                                               */
   int check_every; /* host polls the device convergence flag every k iterations   */
+  int stall_iters; /* guided solve (guess = 1): restart from zero if not converged
+                      after this many iterations (0: never); a guess whose first
+                      residual is no smaller than ||f|| is dropped at once        */
 } mac_params;
 
 typedef struct {
-  int iters;             /* PCG iterations of the final (accepted) solve        */
+  int iters;             /* PCG iterations in total (a dropped guided start included) */
   int restarts;          /* true-residual restarts taken                         */
   int fallback_used;     /* 1 when the guided solve stalled and was redone       */
   int levels;            /* multigrid levels                                     */
@@ -81,7 +84,7 @@ typedef struct {
   double ms_solve;       /* PCG part only                                        */
 } mac_stats;
 
-/* Defaults: h=1, rtol=1e-8, max_iter=500, FP64, nu=2, omega=0.9, check_every=4. */
+/* Defaults: h=1, rtol=1e-8, max_iter=500, FP64, nu=2, omega=0.9, check_every=4, stall_iters=0. */
 void mac_default_params(mac_params* prm);
 
 /* Create / destroy a context on the current HIP device. */
@@ -233,6 +236,33 @@ int mac_sim_move_particles(mac_sim* sim, double dt);
 int mac_sim_interpolate(mac_sim* sim, int n, const double* pos, double* vel);
 /* One whole time step: topology, dt, advance, move particles. */
 int mac_sim_step(mac_sim* sim, mac_sim_stats* stats);
+
+/* ===================================================================== */
+/* Learned initial guess: 3-level 3-D U-Net (no counterpart in the        */
+/* reference; it only seeds the PCG of mac_project, which then solves to   */
+/* the same tolerance).  16/32/64 channels, 3x3x3 convolutions on the BF16 */
+/* matrix cores with FP32 accumulation, ReLU, 2x2x2 average pooling,       */
+/* 2x2x2 transposed-convolution upsampling, skip concatenation, 1x1x1 head.*/
+/* Volumes are D x H x W voxels, z slowest, x fastest; D and H multiples   */
+/* of 4, W a multiple of 64.  Input: 5 FP32 channels per voxel             */
+/* (divergence, fluid, air, solid masks, previous pressure), [V][5].       */
+/* ===================================================================== */
+typedef struct mac_unet mac_unet;
+int mac_unet_create(mac_unet** out, int D, int H, int W, unsigned long long seed);
+void mac_unet_destroy(mac_unet* net);
+long long mac_unet_num_params(mac_unet* net);
+/* FP32 master parameters, layer by layer (weights [Cout][27 Cin] with K
+ * ordered (tap, cin), tap = 9(dz+1)+3(dy+1)+(dx+1), then biases). */
+int mac_unet_get_params(mac_unet* net, float* out);
+int mac_unet_set_params(mac_unet* net, const float* in);
+int mac_unet_forward(mac_unet* net, const float* in5, float* out);
+/* Parameter gradients of sum(out * gout) for the last forward. */
+int mac_unet_backward(mac_unet* net, const float* gout, float* grads);
+/* One Adam step on the masked mean-squared error to target. */
+int mac_unet_train_step(mac_unet* net, const float* in5, const float* target, const float* mask, float lr, float* loss);
+/* Event-timed forward convolutions: returns the time and FLOPs accumulated
+ * since the previous call, then switches timing on or off. */
+int mac_unet_conv_timing(mac_unet* net, int on, double* ms, double* flop);
 
 #ifdef __cplusplus
 }

@@ -662,6 +662,22 @@ int pcg(mac_ctx* c, bool use_guess, mac_stats* st) {
   const int check = std::max(1, c->prm.check_every);
   const double nd = (double)l0.ndom;
   int restarts = 0;
+  bool fellback = false;
+  int wasted = 0;  // iterations of a dropped guided start
+  // A guided solve (use_guess) falls back to the zero start when its first
+  // residual is no better than the zero start's (or not finite), when it has
+  // not converged after stall_iters iterations, or when it ends without
+  // converging (max_iter, breakdown).  The zero start gets the full max_iter.
+  auto fall_back = [&]() -> int {
+    fellback = true;
+    if (st) st->fallback_used = 1;
+    wasted = c->hs->iter;
+    c->hs->iter = 0;
+    c->hs->done = 0;
+    HIPCHK(hipMemcpyAsync(&c->s->iter, &c->hs->iter, 2 * sizeof(int), hipMemcpyHostToDevice, c->st));
+    HIPCHK(hipMemsetAsync(c->x, 0, n0 * sizeof(double), c->st));
+    return 0;
+  };
   for (;;) {
     HIPCHK(hipMemsetAsync(c->d[0], 0, n0 * sizeof(double), c->st));
     int cur = 0;
@@ -673,6 +689,10 @@ int pcg(mac_ctx* c, bool use_guess, mac_stats* st) {
     }
     finalize(c, grid_of(l0.nact), OP_RR0);
     CHK(read_scal(c));
+    if (use_guess && !fellback && !(c->hs->rr <= c->hs->ff)) {
+      CHK(fall_back());
+      continue;
+    }
     TZ* z = nullptr;
     if (!c->hs->done) CHK(vcycle<TZ>(c, &z));
     int iter_before = c->hs->iter;
@@ -703,6 +723,11 @@ int pcg(mac_ctx* c, bool use_guess, mac_stats* st) {
         c->noop.push_back({last + 1, c->seq - 1});
       }
       iter_before = c->hs->iter;
+      if (use_guess && !fellback && c->prm.stall_iters > 0 && !c->hs->done && c->hs->iter >= c->prm.stall_iters) break;
+    }
+    if (use_guess && !fellback && c->hs->done != 1) {
+      CHK(fall_back());
+      continue;
     }
     int done = c->hs->done;
     // true residual
@@ -716,7 +741,7 @@ int pcg(mac_ctx* c, bool use_guess, mac_stats* st) {
     double ff = c->hs->ff;
     double tr = ff > 0 ? std::sqrt(c->hs->tmp / ff) : 0.0;
     if (st) {
-      st->iters = c->hs->iter;
+      st->iters = c->hs->iter + wasted;
       st->rel_resid = ff > 0 ? std::sqrt(c->hs->rr / ff) : 0.0;
       st->true_rel_resid = tr;
     }
@@ -758,6 +783,7 @@ void mac_default_params(mac_params* p) {
   p->nu = 2;
   p->omega = 0.9;
   p->check_every = 4;
+  p->stall_iters = 0;
 }
 
 int mac_create(mac_ctx** out, const mac_params* prm) {

@@ -1,0 +1,377 @@
+// unet_kernels.h -- kernels of the 3-D U-Net that proposes the initial
+// pressure guess.  Activations are BF16, channels-last ([z][y][x][c]),
+// accumulation FP32; master weights FP32.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+typedef __bf16 bf16;
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+
+struct Vol {
+  int D, H, W;
+};
+
+__device__ inline float bf2f(bf16 v) { return (float)v; }
+__device__ inline bf16 f2bf(float v) { return (bf16)v; }
+
+// 3x3x3 convolution, zero padding 1, stride 1, as an implicit GEMM on the
+// BF16 matrix cores (v_mfma_f32_16x16x32_bf16, FP32 accumulate):
+//   M = output voxels (one wave: 16 consecutive x), N = Cout (NB blocks of 16),
+//   K = 27 * Cin ordered (tap, cin), tap = 9 (dz+1) + 3 (dy+1) + (dx+1).
+// Lane l holds A row l % 16 and the 8 consecutive K of group l / 16 (one tap,
+// 8 channels: one 16-byte load of the channels-last input), and B column
+// l % 16 with the same 8 K (one 16-byte load of the [Cout][Kpad] weights).
+// Epilogue: + bias, optional ReLU, optional accumulation into the output.
+// Cin, the channel strides and offsets are multiples of 8; W a multiple of 16.
+template <int NB>
+__global__ void __launch_bounds__(256) k_conv3(const bf16* __restrict__ in, int in_cs, int in_co, int Cin, Vol v,
+                                               const bf16* __restrict__ wt, int Kpad, const float* __restrict__ bias,
+                                               bf16* __restrict__ out, int out_cs, int out_co, int relu, int accum) {
+  const int lane = threadIdx.x & 63;
+  const int wave = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  const int nxw = v.W >> 4;
+  if (wave >= v.D * v.H * nxw) return;
+  const int xw = wave % nxw, y = (wave / nxw) % v.H, z = wave / (nxw * v.H);
+  const int x0 = xw << 4;
+  const int i = lane & 15, g = lane >> 4;
+  floatx4 acc[NB];
+#pragma unroll
+  for (int nb = 0; nb < NB; ++nb) acc[nb] = floatx4{0.f, 0.f, 0.f, 0.f};
+  const int K = 27 * Cin;
+  for (int kc = 0; kc < Kpad; kc += 32) {
+    const int k = kc + 8 * g;
+    bf16x8 a;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) a[j] = (bf16)0.0f;
+    if (k < K) {
+      const int tap = k / Cin, ci = k - tap * Cin;
+      const int dz = tap / 9 - 1, dy = (tap / 3) % 3 - 1, dx = tap % 3 - 1;
+      const int xx = x0 + i + dx, yy = y + dy, zz = z + dz;
+      if (xx >= 0 && xx < v.W && yy >= 0 && yy < v.H && zz >= 0 && zz < v.D)
+        a = *(const bf16x8*)(in + ((size_t)(zz * v.H + yy) * v.W + xx) * in_cs + in_co + ci);
+    }
+#pragma unroll
+    for (int nb = 0; nb < NB; ++nb) {
+      const bf16x8 b = *(const bf16x8*)(wt + (size_t)(16 * nb + i) * Kpad + k);
+      acc[nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[nb], 0, 0, 0);
+    }
+  }
+#pragma unroll
+  for (int nb = 0; nb < NB; ++nb) {
+    const int co = 16 * nb + i;
+    const float bv = bias ? bias[co] : 0.f;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int row = 4 * g + j;
+      bf16* o = out + ((size_t)(z * v.H + y) * v.W + x0 + row) * out_cs + out_co + co;
+      float val = acc[nb][j] + bv;
+      if (relu) val = val > 0.f ? val : 0.f;
+      if (accum) val += bf2f(*o);
+      *o = f2bf(val);
+    }
+  }
+}
+
+// Weight gradient of a 3x3x3 convolution, FP32:
+//   dW[co][tap*Cin + ci] += sum_v G[v][co] * X[v + off(tap)][ci]
+// Block = (tap, chunk of x-rows); thread = one 4x4 (co, ci) tile and a
+// residue class of x; rows are staged in LDS.  The R threads sharing a tile
+// are reduced with a fixed butterfly, so the result is reproducible; each
+// block writes its partial, summed in chunk order by k_wgrad_reduce.
+__global__ void __launch_bounds__(256) k_conv3_wgrad(const bf16* __restrict__ G, int g_cs, int g_co, int Cout,
+                                                     const bf16* __restrict__ X, int x_cs, int x_co, int Cin, Vol v,
+                                                     int rows_per_chunk, float* __restrict__ partial) {
+  __shared__ float gs[64 * 64];  // [x][co]
+  __shared__ float xs[64 * 64];  // [x][ci]
+  const int tap = blockIdx.x % 27, chunk = blockIdx.x / 27;
+  const int dz = tap / 9 - 1, dy = (tap / 3) % 3 - 1, dx = tap % 3 - 1;
+  const int ntc = Cin >> 2, ntiles = (Cout >> 2) * ntc;
+  const int R = 256 / ntiles;  // threads per tile (power of 2, <= 32)
+  const int tile = threadIdx.x / R, r = threadIdx.x % R;
+  const int co0 = (tile / ntc) * 4, ci0 = (tile % ntc) * 4;
+  float acc[4][4];
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) acc[a][b] = 0.f;
+  const int nrows = v.D * v.H;
+  const int r0 = chunk * rows_per_chunk, r1 = min(nrows, r0 + rows_per_chunk);
+  for (int row = r0; row < r1; ++row) {
+    const int y = row % v.H, z = row / v.H;
+    const int yy = y + dy, zz = z + dz;
+    __syncthreads();
+    for (int e = threadIdx.x; e < v.W * Cout; e += blockDim.x) {
+      const int x = e / Cout, c = e % Cout;
+      gs[x * Cout + c] = bf2f(G[((size_t)(z * v.H + y) * v.W + x) * g_cs + g_co + c]);
+    }
+    const bool rowok = (yy >= 0 && yy < v.H && zz >= 0 && zz < v.D);
+    for (int e = threadIdx.x; e < v.W * Cin; e += blockDim.x) {
+      const int x = e / Cin, c = e % Cin;
+      const int xx = x + dx;
+      float val = 0.f;
+      if (rowok && xx >= 0 && xx < v.W) val = bf2f(X[((size_t)(zz * v.H + yy) * v.W + xx) * x_cs + x_co + c]);
+      xs[x * Cin + c] = val;
+    }
+    __syncthreads();
+    if (tile < ntiles) {
+      for (int x = r; x < v.W; x += R) {
+        float gv[4], xv[4];
+#pragma unroll
+        for (int a = 0; a < 4; ++a) gv[a] = gs[x * Cout + co0 + a];
+#pragma unroll
+        for (int b = 0; b < 4; ++b) xv[b] = xs[x * Cin + ci0 + b];
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+          for (int b = 0; b < 4; ++b) acc[a][b] += gv[a] * xv[b];
+      }
+    }
+  }
+  // butterfly over the R lanes of a tile (they are consecutive lanes of one wave)
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b)
+      for (int o = 1; o < R; o <<= 1) acc[a][b] += __shfl_xor(acc[a][b], o);
+  if (tile < ntiles && r == 0) {
+    float* P = partial + ((size_t)chunk * 27 + tap) * Cout * Cin;
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+      for (int b = 0; b < 4; ++b) P[(co0 + a) * Cin + ci0 + b] = acc[a][b];
+  }
+}
+
+// dW[co][tap*Cin+ci] = sum over chunks (in order) of partial[chunk][tap][co][ci].
+__global__ void k_wgrad_reduce(int nchunk, int Cout, int Cin, const float* __restrict__ partial, float* __restrict__ dw) {
+  const int n = 27 * Cout * Cin;
+  for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x) {
+    const int tap = e / (Cout * Cin), co = (e / Cin) % Cout, ci = e % Cin;
+    float s = 0.f;
+    for (int c = 0; c < nchunk; ++c) s += partial[(size_t)c * n + e];
+    dw[(size_t)co * 27 * Cin + tap * Cin + ci] = s;
+  }
+}
+
+// Bias gradient: db[c] = sum_v G[v][c]; one block per channel, fixed order.
+__global__ void k_bias_grad(const bf16* __restrict__ G, int cs, int co, size_t nvox, float* __restrict__ db) {
+  __shared__ float sh[256];
+  const int c = blockIdx.x;
+  float s = 0.f;
+  for (size_t v = threadIdx.x; v < nvox; v += blockDim.x) s += bf2f(G[v * cs + co + c]);
+  sh[threadIdx.x] = s;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) db[c] = sh[0];
+}
+
+// Forward/backward weight layouts for the matrix-core convolution:
+//   fwd[co][tap*Cin + ci]            (K padded to Kpad with zeros)
+//   bwd[ci][tap'*Cout + co] = w[co][tap*Cin+ci], tap' = 26 - tap (mirror),
+// so that the data gradient is the same convolution kernel on the bwd weights.
+__global__ void k_pack_weights(int Cout, int Cin, const float* __restrict__ w, bf16* __restrict__ fwd, int Kpad,
+                               bf16* __restrict__ bwd, int Kpad2) {
+  const int nf = Cout * Kpad, nbw = Cin * Kpad2;
+  for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < nf + nbw; e += gridDim.x * blockDim.x) {
+    if (e < nf) {
+      const int co = e / Kpad, k = e % Kpad;
+      fwd[e] = f2bf(k < 27 * Cin ? w[(size_t)co * 27 * Cin + k] : 0.f);
+    } else if (bwd) {
+      const int e2 = e - nf;
+      const int ci = e2 / Kpad2, k = e2 % Kpad2;
+      float val = 0.f;
+      if (k < 27 * Cout) {
+        const int tp = k / Cout, co = k % Cout, tap = 26 - tp;
+        val = w[(size_t)co * 27 * Cin + tap * Cin + ci];
+      }
+      bwd[e2] = f2bf(val);
+    }
+  }
+}
+
+// G = dY * (Y > 0), in place on dY (ReLU backward).
+__global__ void k_relu_mask(size_t nvox, int C, bf16* __restrict__ dy, int dy_cs, int dy_co, const bf16* __restrict__ y, int y_cs,
+                            int y_co) {
+  const size_t n = nvox * C;
+  for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x) {
+    const size_t v = e / C;
+    const int c = (int)(e % C);
+    if (!(bf2f(y[v * y_cs + y_co + c]) > 0.f)) dy[v * dy_cs + dy_co + c] = f2bf(0.f);
+  }
+}
+
+// 2x2x2 average pooling (fine volume v -> coarse v/2).
+__global__ void k_pool_fwd(const bf16* __restrict__ in, int in_cs, int in_co, int C, Vol vf, bf16* __restrict__ out) {
+  const int Dc = vf.D / 2, Hc = vf.H / 2, Wc = vf.W / 2;
+  const size_t n = (size_t)Dc * Hc * Wc * C;
+  for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x) {
+    const int c = (int)(e % C);
+    const size_t vc = e / C;
+    const int x = (int)(vc % Wc), y = (int)((vc / Wc) % Hc), z = (int)(vc / ((size_t)Wc * Hc));
+    float s = 0.f;
+    for (int o = 0; o < 8; ++o) {
+      const int xx = 2 * x + (o & 1), yy = 2 * y + ((o >> 1) & 1), zz = 2 * z + (o >> 2);
+      s += bf2f(in[((size_t)(zz * vf.H + yy) * vf.W + xx) * in_cs + in_co + c]);
+    }
+    out[e] = f2bf(s * 0.125f);
+  }
+}
+
+// Pool backward, accumulated into the fine gradient: dIn[child] += dOut[parent]/8.
+__global__ void k_pool_bwd(const bf16* __restrict__ dout, int C, Vol vf, bf16* __restrict__ din, int din_cs, int din_co) {
+  const size_t n = (size_t)vf.D * vf.H * vf.W * C;
+  const int Hc = vf.H / 2, Wc = vf.W / 2;
+  for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x) {
+    const int c = (int)(e % C);
+    const size_t vv = e / C;
+    const int x = (int)(vv % vf.W), y = (int)((vv / vf.W) % vf.H), z = (int)(vv / ((size_t)vf.W * vf.H));
+    const float g = bf2f(dout[((size_t)((z / 2) * Hc + y / 2) * Wc + x / 2) * C + c]) * 0.125f;
+    bf16* p = din + vv * din_cs + din_co + c;
+    *p = f2bf(bf2f(*p) + g);
+  }
+}
+
+// Transposed 2x2x2 convolution, stride 2 (coarse vc -> fine vf), + bias, ReLU:
+//   out[2v + o][co] = relu(b[co] + sum_ci W[o][co][ci] in[v][ci]).
+__global__ void k_up_fwd(const bf16* __restrict__ in, int Cin, Vol vc, const float* __restrict__ w, const float* __restrict__ b,
+                         int Cout, bf16* __restrict__ out, int out_cs, int out_co) {
+  const int Df = 2 * vc.D, Hf = 2 * vc.H, Wf = 2 * vc.W;
+  const size_t n = (size_t)Df * Hf * Wf * Cout;
+  for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x) {
+    const int co = (int)(e % Cout);
+    const size_t vv = e / Cout;
+    const int x = (int)(vv % Wf), y = (int)((vv / Wf) % Hf), z = (int)(vv / ((size_t)Wf * Hf));
+    const int o = (x & 1) + 2 * (y & 1) + 4 * (z & 1);
+    const bf16* ip = in + ((size_t)((z / 2) * vc.H + y / 2) * vc.W + x / 2) * Cin;
+    const float* wp = w + ((size_t)o * Cout + co) * Cin;
+    float s = b[co];
+    for (int ci = 0; ci < Cin; ++ci) s += wp[ci] * bf2f(ip[ci]);
+    out[vv * out_cs + out_co + co] = f2bf(s > 0.f ? s : 0.f);
+  }
+}
+
+// Transposed conv backward-data: dIn[v][ci] = sum_{o,co} W[o][co][ci] G[2v+o][co].
+__global__ void k_up_bwd_data(const bf16* __restrict__ g, int g_cs, int g_co, int Cout, Vol vc, const float* __restrict__ w,
+                              int Cin, bf16* __restrict__ din) {
+  const size_t n = (size_t)vc.D * vc.H * vc.W * Cin;
+  const int Hf = 2 * vc.H, Wf = 2 * vc.W;
+  for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x) {
+    const int ci = (int)(e % Cin);
+    const size_t v = e / Cin;
+    const int x = (int)(v % vc.W), y = (int)((v / vc.W) % vc.H), z = (int)(v / ((size_t)vc.W * vc.H));
+    float s = 0.f;
+    for (int o = 0; o < 8; ++o) {
+      const int xx = 2 * x + (o & 1), yy = 2 * y + ((o >> 1) & 1), zz = 2 * z + (o >> 2);
+      const bf16* gp = g + ((size_t)(zz * Hf + yy) * Wf + xx) * g_cs + g_co;
+      for (int co = 0; co < Cout; ++co) s += w[((size_t)o * Cout + co) * Cin + ci] * bf2f(gp[co]);
+    }
+    din[e] = f2bf(s);
+  }
+}
+
+// Transposed conv weight/bias gradient: one thread per (o, co, ci) (+ bias),
+// summing over coarse voxels in order.
+__global__ void k_up_wgrad(const bf16* __restrict__ g, int g_cs, int g_co, int Cout, Vol vc, const bf16* __restrict__ in, int Cin,
+                           float* __restrict__ dw, float* __restrict__ db) {
+  const int n = 8 * Cout * Cin + Cout;
+  const int Hf = 2 * vc.H, Wf = 2 * vc.W;
+  const size_t nv = (size_t)vc.D * vc.H * vc.W;
+  for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x) {
+    float s = 0.f;
+    if (e < 8 * Cout * Cin) {
+      const int o = e / (Cout * Cin), co = (e / Cin) % Cout, ci = e % Cin;
+      for (size_t v = 0; v < nv; ++v) {
+        const int x = (int)(v % vc.W), y = (int)((v / vc.W) % vc.H), z = (int)(v / ((size_t)vc.W * vc.H));
+        const int xx = 2 * x + (o & 1), yy = 2 * y + ((o >> 1) & 1), zz = 2 * z + (o >> 2);
+        s += bf2f(g[((size_t)(zz * Hf + yy) * Wf + xx) * g_cs + g_co + co]) * bf2f(in[v * Cin + ci]);
+      }
+      dw[e] = s;
+    } else {
+      const int co = e - 8 * Cout * Cin;
+      for (size_t v = 0; v < nv * 8; ++v) s += bf2f(g[v * g_cs + g_co + co]);
+      db[co] = s;
+    }
+  }
+}
+
+// Head: 1x1x1 convolution C -> 1, FP32 output.
+__global__ void k_head_fwd(const bf16* __restrict__ in, int C, size_t nvox, const float* __restrict__ w, const float* __restrict__ b,
+                           float* __restrict__ out) {
+  for (size_t v = blockIdx.x * (size_t)blockDim.x + threadIdx.x; v < nvox; v += (size_t)gridDim.x * blockDim.x) {
+    float s = b[0];
+    for (int c = 0; c < C; ++c) s += w[c] * bf2f(in[v * C + c]);
+    out[v] = s;
+  }
+}
+
+__global__ void k_head_bwd_data(const float* __restrict__ gout, int C, size_t nvox, const float* __restrict__ w,
+                                bf16* __restrict__ din) {
+  const size_t n = nvox * C;
+  for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x)
+    din[e] = f2bf(w[e % C] * gout[e / C]);
+}
+
+// Head weight/bias gradient with per-block partials (reduced in order on the host side of the launch).
+__global__ void k_head_wgrad(const float* __restrict__ gout, const bf16* __restrict__ in, int C, size_t nvox,
+                             float* __restrict__ partial) {
+  __shared__ float sh[256];
+  for (int c = 0; c <= C; ++c) {
+    float s = 0.f;
+    for (size_t v = blockIdx.x * (size_t)blockDim.x + threadIdx.x; v < nvox; v += (size_t)gridDim.x * blockDim.x)
+      s += gout[v] * (c < C ? bf2f(in[v * C + c]) : 1.f);
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+      if (threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[(size_t)blockIdx.x * (C + 1) + c] = sh[0];
+    __syncthreads();
+  }
+}
+
+// Masked MSE: loss partials and d loss / d out = 2 m (p - t) / sum(m).
+__global__ void k_mse(size_t n, const float* __restrict__ p, const float* __restrict__ t, const float* __restrict__ m, float inv_msum,
+                      float* __restrict__ gout, float* __restrict__ partial) {
+  __shared__ float sh[256];
+  float s = 0.f;
+  for (size_t v = blockIdx.x * (size_t)blockDim.x + threadIdx.x; v < n; v += (size_t)gridDim.x * blockDim.x) {
+    const float d = (p[v] - t[v]) * m[v];
+    s += d * d;
+    gout[v] = 2.f * d * inv_msum;
+  }
+  sh[threadIdx.x] = s;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[blockIdx.x] = sh[0];
+}
+
+// Adam (bias-corrected step sizes passed in).
+__global__ void k_adam(size_t n, float* __restrict__ w, const float* __restrict__ g, float* __restrict__ m, float* __restrict__ v2,
+                       float lr_t, float b1, float b2, float eps) {
+  for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x) {
+    const float gg = g[e];
+    const float mm = b1 * m[e] + (1.f - b1) * gg;
+    const float vv = b2 * v2[e] + (1.f - b2) * gg * gg;
+    m[e] = mm;
+    v2[e] = vv;
+    w[e] -= lr_t * mm / (sqrtf(vv) + eps);
+  }
+}
+
+// Host FP32 input [V][5] -> BF16 [V][8] (channels 5..7 zero).
+__global__ void k_pack_input(size_t nvox, const float* __restrict__ in5, bf16* __restrict__ out8) {
+  for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < nvox * 8; e += (size_t)gridDim.x * blockDim.x) {
+    const size_t v = e >> 3;
+    const int c = (int)(e & 7);
+    out8[e] = f2bf(c < 5 ? in5[v * 5 + c] : 0.f);
+  }
+}

@@ -33,6 +33,7 @@ class Params(ctypes.Structure):
         ("nu", ctypes.c_int),
         ("omega", ctypes.c_double),
         ("check_every", ctypes.c_int),
+        ("stall_iters", ctypes.c_int),
     ]
 
 
@@ -431,3 +432,74 @@ class Sim:
         st = SimStats()
         _check(self.lib.mac_sim_step(self.ptr, ctypes.byref(st)))
         return st.as_dict()
+
+
+# ----------------------------------------------------------------- U-Net ----
+SIGNATURES.update({
+    "mac_unet_create": (ctypes.c_int, [ctypes.POINTER(_P), ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_ulonglong]),
+    "mac_unet_destroy": (None, [_P]),
+    "mac_unet_num_params": (ctypes.c_longlong, [_P]),
+    "mac_unet_get_params": (ctypes.c_int, [_P, _P]),
+    "mac_unet_set_params": (ctypes.c_int, [_P, _P]),
+    "mac_unet_forward": (ctypes.c_int, [_P, _P, _P]),
+    "mac_unet_backward": (ctypes.c_int, [_P, _P, _P]),
+    "mac_unet_train_step": (ctypes.c_int, [_P, _P, _P, _P, ctypes.c_float, ctypes.POINTER(ctypes.c_float)]),
+    "mac_unet_conv_timing": (ctypes.c_int, [_P, ctypes.c_int, _dp, _dp]),
+})
+
+
+class UNet:
+    """The learned-guess network (mac_unet_* of include/mac_amd.h)."""
+
+    def __init__(self, D, H, W, seed=0):
+        self.lib = load()
+        self.ptr = ctypes.c_void_p()
+        self.shape = (D, H, W)
+        _check(self.lib.mac_unet_create(ctypes.byref(self.ptr), D, H, W, seed))
+        self.nparams = int(self.lib.mac_unet_num_params(self.ptr))
+
+    def close(self):
+        if self.ptr:
+            self.lib.mac_unet_destroy(self.ptr)
+            self.ptr = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def get_params(self):
+        p = np.zeros(self.nparams, np.float32)
+        _check(self.lib.mac_unet_get_params(self.ptr, _ptr(p)))
+        return p
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, np.float32)
+        assert p.size == self.nparams
+        _check(self.lib.mac_unet_set_params(self.ptr, _ptr(p)))
+
+    def forward(self, x5):
+        x5 = np.ascontiguousarray(x5, np.float32).reshape(-1, 5)
+        out = np.zeros(x5.shape[0], np.float32)
+        _check(self.lib.mac_unet_forward(self.ptr, _ptr(x5), _ptr(out)))
+        return out
+
+    def backward(self, gout):
+        gout = np.ascontiguousarray(gout, np.float32).reshape(-1)
+        g = np.zeros(self.nparams, np.float32)
+        _check(self.lib.mac_unet_backward(self.ptr, _ptr(gout), _ptr(g)))
+        return g
+
+    def train_step(self, x5, target, mask, lr):
+        x5 = np.ascontiguousarray(x5, np.float32).reshape(-1, 5)
+        t = np.ascontiguousarray(target, np.float32).reshape(-1)
+        m = np.ascontiguousarray(mask, np.float32).reshape(-1)
+        loss = ctypes.c_float()
+        _check(self.lib.mac_unet_train_step(self.ptr, _ptr(x5), _ptr(t), _ptr(m), float(lr), ctypes.byref(loss)))
+        return loss.value
+
+    def conv_timing(self, on):
+        ms, fl = ctypes.c_double(), ctypes.c_double()
+        _check(self.lib.mac_unet_conv_timing(self.ptr, 1 if on else 0, ctypes.byref(ms), ctypes.byref(fl)))
+        return ms.value, fl.value

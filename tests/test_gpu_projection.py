@@ -186,3 +186,37 @@ def test_solve_csr_slot_a():
     x3 = ls.Solve_Linear(3, off3, idx3, val3, b3)
     A = np.array([[-2, 1, 0], [1, -3, 1], [0, 1, -2.0]])
     assert np.allclose(A @ x3, b3, rtol=0, atol=1e-14)
+
+
+@pytest.mark.parametrize("poison", ["nan", "huge"])
+def test_guided_solve_falls_back_on_bad_guess(poison):
+    """A guess whose first residual is worse than the zero start's (or not
+    finite) is dropped: the solve restarts from zero and still converges."""
+    g = small_spheres(32, 4)
+    ref = O.project(g, rtol=1e-13)
+    pr = make(g, rtol=1e-10)
+    n = g["flags"].size
+    bad = np.full(n, np.nan) if poison == "nan" else W.normal(11, 0, np.arange(n, dtype=np.uint64)) * 1e12
+    pr.set_pressure(bad * dom_mask(g) if poison == "huge" else bad)
+    st = pr.Divergence_Projection(guess=True)
+    assert st["fallback_used"] == 1
+    assert st["true_rel_resid"] <= 1e-10 * 1.0001
+    assert rel(pr.get_pressure(), ref["p"]) < 1e-7
+
+
+def test_guided_solve_falls_back_after_stall_iters():
+    """stall_iters forces the fallback once a guided solve has not converged
+    after that many iterations; the result equals the unguided solve's."""
+    g = small_spheres(32, 5)
+    pr = make(g, rtol=1e-10, check_every=1)
+    pr.save_velocity()
+    st0 = pr.Divergence_Projection()
+    p0 = pr.get_pressure()
+    pr.restore_velocity()
+    pr.set_params(stall_iters=3)
+    pr.set_pressure(0.5 * p0)  # a fair guess that still needs many iterations
+    st1 = pr.Divergence_Projection(guess=True)
+    assert st1["fallback_used"] == 1
+    assert st1["iters"] == st0["iters"] + 3, (st0["iters"], st1["iters"])
+    assert st1["true_rel_resid"] <= 1e-10 * 1.0001
+    assert rel(pr.get_pressure(), p0) < 1e-8

@@ -264,6 +264,29 @@ int mac_unet_train_step(mac_unet* net, const float* in5, const float* target, co
  * since the previous call, then switches timing on or off. */
 int mac_unet_conv_timing(mac_unet* net, int on, double* ms, double* flop);
 
+/* The network on a projection context.  The grid's bounding box (chunk keys
+ * shifted to a minimum of 0 per axis) is mapped to the network volume, cell
+ * (8kx+x, 8ky+y, 8kz+z); it must fit.  Cells outside the grid are solid.
+ * Scaling: s = rms of the right-hand side over the unknowns, hm = mean of
+ * 1/h^2 over the axes; inputs f/s and prev*hm/s, output p*hm/(s*out_scale).
+ *
+ * mac_unet_guess: the learned starting iterate for mac_project(ctx, 1, ...)
+ * from the context's current face velocities, written into its pressure.
+ * use_prev = 1 takes the context's pressure as it stands as the previous
+ * pressure input, 0 a zero one. */
+int mac_unet_guess(mac_unet* net, mac_ctx* ctx, int use_prev, float out_scale);
+/* Training pair of the context's last mac_project: in5 [V][5] for its
+ * right-hand side with prev (host, brick layout; NULL = zero) as the previous
+ * pressure, target [V] its pressure in output units, mask [V] 1 on unknowns. */
+int mac_unet_sample(mac_unet* net, mac_ctx* ctx, const double* prev, float out_scale, float* in5, float* target, float* mask);
+/* Starting iterate of the simulation's projections: mode 0 zero, 1 the last
+ * projection's pressure, 2 the learned guess of net with that pressure as its
+ * previous-pressure input (the network stays owned by the caller). */
+int mac_sim_set_guess(mac_sim* sim, int mode, mac_unet* net, float out_scale);
+/* Training pair of the simulation's last projection (previous pressure = the
+ * pressure that projection started from); outputs as for mac_unet_sample. */
+int mac_sim_capture(mac_sim* sim, mac_unet* net, float out_scale, float* in5, float* target, float* mask);
+
 #ifdef __cplusplus
 }
 #endif

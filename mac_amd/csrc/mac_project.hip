@@ -47,10 +47,16 @@ inline uint64_t pack_key(int x, int y, int z) {
          (uint64_t)(uint32_t)(z + (1 << 20));
 }
 
-enum KClass { K_DIV, K_PMATVEC, K_UPDATE, K_JACOBI0, K_RESID0, K_PROLONG0, K_RESTRICT1, K_COARSE, K_CSOLVE, K_GRAD, K_MAXDIV, K_INIT, K_FIN, K_N };
-const char* kNames[K_N] = {"divergence",   "cg_pupdate_matvec", "cg_update",  "jacobi_l0",  "residual_l0", "prolong_l0",
-                           "restrict_l1",  "vcycle_coarse",     "coarse_solve", "gradient_update", "max_div", "resid_init",
-                           "finalize"};
+// Event-timing classes: each is one kernel at one level range, so that its
+// average launch time can be matched against a profiler's per-kernel figure.
+enum KClass {
+  K_DIV, K_PMATVEC, K_UPDATE, K_JACOBI2_0, K_JACOBI_0, K_RESID0, K_PROLONG0, K_RESTRICT1,
+  K_C_JACOBI2, K_C_JACOBI, K_C_RESID, K_C_RESTRICT, K_C_PROLONG, K_CSOLVE, K_GRAD, K_MAXDIV, K_INIT, K_FIN, K_N
+};
+const char* kNames[K_N] = {"divergence",    "cg_pupdate_matvec", "cg_update",       "jacobi2_l0",     "jacobi_l0",
+                           "residual_l0",   "prolong_l0",        "restrict_l1",     "coarse_jacobi2", "coarse_jacobi",
+                           "coarse_residual", "coarse_restrict", "coarse_prolong",  "coarse_solve",   "gradient_update",
+                           "max_div",       "resid_init",        "finalize"};
 
 inline int grid_of(int nact) {
   int g = (nact + WPB - 1) / WPB;
@@ -87,6 +93,8 @@ struct mac_ctx {
   double *x = nullptr, *f = nullptr, *r = nullptr, *d[2] = {nullptr, nullptr}, *q = nullptr, *tmpv = nullptr;
   int* glist = nullptr;
   int nglist = 0;
+  int* dkeys0 = nullptr;  // level-0 shifted chunk keys on the device
+  int ext0[3] = {0, 0, 0};
   // coarsest level exact solve
   int nc = 0;
   int *rowof = nullptr, *cellof = nullptr;
@@ -408,6 +416,10 @@ int build_grid(mac_ctx* c, int nbricks, const int32_t* keys3, const uint8_t* fla
       }
       c->nglist = (int)gl.size();
       CHK(upload(c, &c->glist, gl));
+      CHK(upload(c, &c->dkeys0, c->L[0].keys));
+      for (int i = 0; i < 3; ++i) c->ext0[i] = 0;
+      for (int b = 0; b < c->nb0; ++b)
+        for (int i = 0; i < 3; ++i) c->ext0[i] = std::max(c->ext0[i], c->L[0].keys[3 * b + i] + 1);
     }
   }
   // flattened per-active-brick metadata
@@ -526,7 +538,7 @@ void sweep(mac_ctx* c, int l, const TZ* zin, TZ* zout, int want_dot) {
   const double sz = sizeof(TZ), nd = (double)lv.ndom;
   const double sr = (l == 0) ? 8.0 : sz;
   {
-  TimedLaunch tl(c, l == 0 ? K_JACOBI0 : K_COARSE, nd * (sr + (zin ? 2 : 1) * sz + 1));
+  TimedLaunch tl(c, l == 0 ? K_JACOBI_0 : K_C_JACOBI, nd * (sr + (zin ? 2 : 1) * sz + 1));
   if (l == 0)
     k_jacobi<double, TZ><<<grid_of(lv.nact), KTHREADS, 0, c->st>>>(lv.nact, lv.meta, lv.code, c->r, zin, zout, ih[0], ih[1],
                                                                    ih[2], om, want_dot, c->partials, c->s);
@@ -564,7 +576,7 @@ int vcycle(mac_ctx* c, TZ** zout) {
     double sr = (l == 0) ? 8.0 : sz;
     if (nu >= 2) {
       // the first two sweeps from zero in one pass
-      TimedLaunch tl(c, l == 0 ? K_JACOBI0 : K_COARSE, nd * (sr + sz + 1));
+      TimedLaunch tl(c, l == 0 ? K_JACOBI2_0 : K_C_JACOBI2, nd * (sr + sz + 1));
       if (l == 0)
         k_jacobi2<double, TZ><<<grid_of(lv.nact), KTHREADS, 0, c->st>>>(lv.nact, lv.meta, lv.code, c->r, z, ih[0], ih[1],
                                                                          ih[2], (TZ)c->prm.omega, c->s);
@@ -579,7 +591,7 @@ int vcycle(mac_ctx* c, TZ** zout) {
       std::swap(z, zt);
     }
     {
-      TimedLaunch tl(c, l == 0 ? K_RESID0 : K_COARSE, nd * (sr + 2 * sz + 1));
+      TimedLaunch tl(c, l == 0 ? K_RESID0 : K_C_RESID, nd * (sr + 2 * sz + 1));
       if (l == 0)
         k_residual<double, TZ><<<grid_of(lv.nact), KTHREADS, 0, c->st>>>(lv.nact, lv.meta, lv.code, c->r, z, res, ih[0], ih[1],
                                                                          ih[2], c->s);
@@ -589,7 +601,7 @@ int vcycle(mac_ctx* c, TZ** zout) {
     }
     Level& cl = c->L[l + 1];
     {
-      TimedLaunch tl(c, l == 0 ? K_RESTRICT1 : K_COARSE, nd * sz + (double)cl.ndom * sz);
+      TimedLaunch tl(c, l == 0 ? K_RESTRICT1 : K_C_RESTRICT, nd * sz + (double)cl.ndom * sz);
       k_restrict_sep<TZ><<<cl.nact, KTHREADS, 0, c->st>>>(cl.rmeta, cl.code, res, (TZ*)cl.r, c->s);
     }
     zfin[l] = z;  // pre-smoothed iterate, finished on the way up
@@ -632,7 +644,7 @@ int vcycle(mac_ctx* c, TZ** zout) {
     TZ* zt = (z == (TZ*)lv.z) ? (TZ*)lv.zt : (TZ*)lv.z;
     double nd = (double)lvl_dom(c, l);
     {
-      TimedLaunch tl(c, l == 0 ? K_PROLONG0 : K_COARSE, nd * (2 * sz + 1) + (double)cl.ndom * sz);
+      TimedLaunch tl(c, l == 0 ? K_PROLONG0 : K_C_PROLONG, nd * (2 * sz + 1) + (double)cl.ndom * sz);
       k_prolong_col<TZ><<<grid_of(lv.nact), KTHREADS, 0, c->st>>>(lv.nact, lv.pmeta, lv.code, zfin[l + 1], z, c->s);
     }
     for (int k = 0; k < nu; ++k) {
@@ -1023,13 +1035,51 @@ MAC_HIDDEN int mac_internal_build(mac_ctx* c, int nbricks, const int32_t* keys3,
   return build_grid(c, nbricks, keys3, flags);
 }
 
-MAC_HIDDEN int mac_internal_project(mac_ctx* c, double* u, double* v, double* w, double* p, int guess, mac_stats* st) {
+MAC_HIDDEN int mac_internal_view(mac_ctx* c, mac_grid_view* v) {
+  CHK(ensure(c));
+  v->st = c->st;
+  v->nb = c->nb0;
+  for (int i = 0; i < 3; ++i) v->ext[i] = c->ext0[i];
+  v->dkeys = c->dkeys0;
+  v->flags = c->flags;
+  v->code = c->L[0].code;
+  v->x = c->x;
+  v->f = c->f;
+  v->ff = &c->s->ff;
+  v->ndom = c->L[0].ndom;
+  v->ih2m = (c->L[0].ih2[0] + c->L[0].ih2[1] + c->L[0].ih2[2]) / 3.0;
+  return 0;
+}
+
+MAC_HIDDEN int mac_internal_rhs(mac_ctx* c) {
+  CHK(ensure(c));
+  Level& l0 = c->L[0];
+  double inv_h[3];
+  for (int i = 0; i < 3; ++i) inv_h[i] = 1.0 / c->prm.h[i];
+  HIPCHK(hipMemsetAsync(c->s, 0, sizeof(Scal), c->st));
+  if (l0.nact > 0) {
+    k_div<<<grid_of(l0.nact), KTHREADS, 0, c->st>>>(l0.nact, l0.meta, l0.code, c->u, c->v, c->w, inv_h[0], inv_h[1], inv_h[2],
+                                                    c->f, nullptr, c->partials);
+    finalize(c, grid_of(l0.nact), OP_FF);
+  }
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+MAC_HIDDEN int mac_internal_load(mac_ctx* c, const double* u, const double* v, const double* w) {
   CHK(ensure(c));
   size_t n = (size_t)c->nb0 * MAC_BC * sizeof(double);
   HIPCHK(hipMemcpyAsync(c->u, u, n, hipMemcpyDeviceToDevice, c->st));
   HIPCHK(hipMemcpyAsync(c->v, v, n, hipMemcpyDeviceToDevice, c->st));
   HIPCHK(hipMemcpyAsync(c->w, w, n, hipMemcpyDeviceToDevice, c->st));
-  if (guess) {
+  return 0;
+}
+
+MAC_HIDDEN int mac_internal_project(mac_ctx* c, double* u, double* v, double* w, double* p, int guess, mac_stats* st) {
+  CHK(ensure(c));
+  size_t n = (size_t)c->nb0 * MAC_BC * sizeof(double);
+  CHK(mac_internal_load(c, u, v, w));
+  if (guess == 1) {
     HIPCHK(hipMemcpyAsync(c->x, p, n, hipMemcpyDeviceToDevice, c->st));
     k_mask<<<c->nb0, KTHREADS, 0, c->st>>>(c->nb0, c->L[0].code, c->x);
   }

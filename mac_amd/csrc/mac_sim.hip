@@ -64,6 +64,7 @@ struct mac_sim {
   double* S[2][3] = {{nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr}};
   double* A[3] = {nullptr, nullptr, nullptr};
   double* P = nullptr;
+  double* Pprev = nullptr;  // the pressure the last projection started from
   int* nbr = nullptr;
   uint64_t* hk = nullptr;
   int* hv = nullptr;
@@ -80,6 +81,10 @@ struct mac_sim {
   double* d_part = nullptr;
   unsigned long long tick = 0;
   bool grid_ready = false;
+  // starting iterate of the projections: 0 zero, 1 last pressure, 2 learned
+  int guess_mode = 0;
+  mac_unet* net = nullptr;
+  float out_scale = 1.f;
 };
 
 namespace {
@@ -92,6 +97,7 @@ void sim_free_bricks(mac_sim* s) {
     for (int i = 0; i < 3; ++i) hipFree(s->S[a][i]);
   for (int i = 0; i < 3; ++i) hipFree(s->A[i]);
   hipFree(s->P);
+  hipFree(s->Pprev);
   hipFree(s->nbr);
   hipFree(s->hk);
   hipFree(s->hv);
@@ -121,7 +127,7 @@ int sim_set_bricks(mac_sim* s, const std::vector<Key3>& nk) {
   int8_t* layer;
   double* S[2][3];
   double* A[3];
-  double* P;
+  double *P, *Pprev;
   SCHK(hipMalloc(&d_keys, sizeof(int) * 3 * std::max(nbn, 1)));
   SCHK(hipMalloc(&flags, cells));
   SCHK(hipMalloc(&layer, cells));
@@ -129,6 +135,7 @@ int sim_set_bricks(mac_sim* s, const std::vector<Key3>& nk) {
     for (int i = 0; i < 3; ++i) SCHK(hipMalloc(&S[a][i], cells * sizeof(double)));
   for (int i = 0; i < 3; ++i) SCHK(hipMalloc(&A[i], cells * sizeof(double)));
   SCHK(hipMalloc(&P, cells * sizeof(double)));
+  SCHK(hipMalloc(&Pprev, cells * sizeof(double)));
   std::vector<int> hk3(3 * std::max(nbn, 1), 0);
   for (int b = 0; b < nbn; ++b) {
     hk3[3 * b] = nk[b].x;
@@ -157,6 +164,7 @@ int sim_set_bricks(mac_sim* s, const std::vector<Key3>& nk) {
     for (int i = 0; i < 3; ++i) s->S[a][i] = S[a][i];
   for (int i = 0; i < 3; ++i) s->A[i] = A[i];
   s->P = P;
+  s->Pprev = Pprev;
   s->d_from = d_from;
   s->keys = nk;
   s->nb = nbn;
@@ -302,8 +310,17 @@ int sim_project(mac_sim* s, int slot, mac_stats* st) {
   // this stream's kernels must be complete before it is copied over
   SCHK(hipStreamSynchronize(s->st));
   SRC(sim_ensure_projection(s));
-  int rc = mac_internal_project(s->proj, s->S[slot][0], s->S[slot][1], s->S[slot][2], s->P, 0, st);
-  return rc;
+  SCHK(hipMemcpyAsync(s->Pprev, s->P, (size_t)s->nb * MAC_BC * sizeof(double), hipMemcpyDeviceToDevice, s->st));
+  SCHK(hipStreamSynchronize(s->st));
+  int guess = 0;
+  if (s->guess_mode == 1) {
+    guess = 1;
+  } else if (s->guess_mode == 2) {
+    SRC(mac_internal_load(s->proj, s->S[slot][0], s->S[slot][1], s->S[slot][2]));
+    SRC(mac_internal_unet_guess(s->net, s->proj, s->Pprev, s->out_scale));
+    guess = 2;
+  }
+  return mac_internal_project(s->proj, s->S[slot][0], s->S[slot][1], s->S[slot][2], s->P, guess, st);
 }
 
 SimGrid sim_grid(mac_sim* s, int slot) {
@@ -573,6 +590,21 @@ int mac_sim_interpolate(mac_sim* s, int n, const double* pos, double* vel) {
   hipFree(dp);
   hipFree(dv);
   return 0;
+}
+
+int mac_sim_set_guess(mac_sim* s, int mode, mac_unet* net, float out_scale) {
+  if (!s || mode < 0 || mode > 2) return mac_internal_set_error(MAC_E_ARG, "bad arguments");
+  if (mode == 2 && (!net || !(out_scale > 0.f))) return mac_internal_set_error(MAC_E_ARG, "learned guess needs a network and out_scale > 0");
+  s->guess_mode = mode;
+  s->net = mode == 2 ? net : nullptr;
+  s->out_scale = out_scale;
+  return 0;
+}
+
+int mac_sim_capture(mac_sim* s, mac_unet* net, float out_scale, float* in5, float* target, float* mask) {
+  if (!s || !net) return mac_internal_set_error(MAC_E_ARG, "null argument");
+  if (!s->grid_ready) return mac_internal_set_error(MAC_E_STATE, "no projection since the grid last changed");
+  return mac_internal_unet_sample(net, s->proj, s->Pprev, out_scale, in5, target, mask);
 }
 
 int mac_sim_step(mac_sim* s, mac_sim_stats* stats) {

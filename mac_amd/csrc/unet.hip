@@ -71,6 +71,7 @@ struct mac_unet {
   std::vector<void*> allocs;
   // conv timing
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  hipEvent_t evx[2] = {nullptr, nullptr};  // hand-over to and from a projection stream
   double conv_ms = 0.0, conv_flop = 0.0;
   bool time_convs = false;
 };
@@ -241,9 +242,105 @@ int upload_input(mac_unet* u, const float* in5) {
   return 0;
 }
 
+// The grid of a projection context as the network's volume; fails when the
+// grid's bounding box does not fit.
+int guess_grid(mac_unet* u, mac_ctx* c, GuessGrid* g, mac_grid_view* v) {
+  URC(mac_internal_view(c, v));
+  if (8 * v->ext[0] > u->v1.W || 8 * v->ext[1] > u->v1.H || 8 * v->ext[2] > u->v1.D)
+    return mac_internal_set_error(MAC_E_ARG, "grid bounding box larger than the network volume");
+  g->nb = v->nb;
+  g->keys = v->dkeys;
+  g->flags = v->flags;
+  g->code = v->code;
+  g->ff = v->ff;
+  g->ndom = (double)v->ndom;
+  g->ih2m = v->ih2m;
+  g->D = u->v1.D;
+  g->H = u->v1.H;
+  g->W = u->v1.W;
+  return 0;
+}
+
+// Network input from the context's right-hand side (already computed, on its
+// stream) and `prev`, on the network's stream after the context's work.
+int guess_input(mac_unet* u, const GuessGrid& g, const mac_grid_view& v, const double* prev) {
+  UCHK(hipEventRecord(u->evx[0], v.st));
+  UCHK(hipStreamWaitEvent(u->st, u->evx[0], 0));
+  k_guess_fill<<<nblk(u->n1 * 5), 256, 0, u->st>>>(u->n1, u->in5);
+  k_guess_pack<<<nblk((size_t)g.nb * 512), 256, 0, u->st>>>(g, v.f, prev, u->in5);
+  k_pack_input<<<nblk(u->n1 * 8), 256, 0, u->st>>>(u->n1, u->in5, u->X0);
+  UCHK(hipGetLastError());
+  return 0;
+}
+
 }  // namespace
 
 extern "C" {
+
+MAC_HIDDEN int mac_internal_unet_guess(mac_unet* u, mac_ctx* c, const double* prev, float out_scale) {
+  if (!u || !c) return mac_internal_set_error(MAC_E_ARG, "null argument");
+  if (!(out_scale > 0.f)) return mac_internal_set_error(MAC_E_ARG, "out_scale must be positive");
+  GuessGrid g;
+  mac_grid_view v;
+  URC(guess_grid(u, c, &g, &v));
+  URC(mac_internal_rhs(c));
+  URC(guess_input(u, g, v, prev));
+  URC(forward(u));
+  k_guess_unpack<<<nblk((size_t)g.nb * 512), 256, 0, u->st>>>(g, u->out, out_scale, v.x);
+  UCHK(hipGetLastError());
+  UCHK(hipEventRecord(u->evx[1], u->st));
+  UCHK(hipStreamWaitEvent(v.st, u->evx[1], 0));
+  return 0;
+}
+
+int mac_unet_guess(mac_unet* u, mac_ctx* c, int use_prev, float out_scale) {
+  if (!u || !c) return mac_internal_set_error(MAC_E_ARG, "null argument");
+  mac_grid_view v;
+  URC(mac_internal_view(c, &v));
+  URC(mac_internal_unet_guess(u, c, use_prev ? v.x : nullptr, out_scale));
+  UCHK(hipStreamSynchronize(v.st));
+  return 0;
+}
+
+MAC_HIDDEN int mac_internal_unet_sample(mac_unet* u, mac_ctx* c, const double* prev, float out_scale, float* in5, float* target,
+                                        float* mask) {
+  if (!u || !c || !in5 || !target || !mask) return mac_internal_set_error(MAC_E_ARG, "null argument");
+  if (!(out_scale > 0.f)) return mac_internal_set_error(MAC_E_ARG, "out_scale must be positive");
+  GuessGrid g;
+  mac_grid_view v;
+  URC(guess_grid(u, c, &g, &v));
+  URC(guess_input(u, g, v, prev));
+  UCHK(hipMemsetAsync(u->tgt, 0, sizeof(float) * u->n1, u->st));
+  UCHK(hipMemsetAsync(u->msk, 0, sizeof(float) * u->n1, u->st));
+  k_guess_target<<<nblk((size_t)g.nb * 512), 256, 0, u->st>>>(g, v.x, out_scale, u->tgt, u->msk);
+  UCHK(hipGetLastError());
+  UCHK(hipMemcpyAsync(in5, u->in5, sizeof(float) * u->n1 * 5, hipMemcpyDeviceToHost, u->st));
+  UCHK(hipMemcpyAsync(target, u->tgt, sizeof(float) * u->n1, hipMemcpyDeviceToHost, u->st));
+  UCHK(hipMemcpyAsync(mask, u->msk, sizeof(float) * u->n1, hipMemcpyDeviceToHost, u->st));
+  UCHK(hipStreamSynchronize(u->st));
+  return 0;
+}
+
+int mac_unet_sample(mac_unet* u, mac_ctx* c, const double* prev, float out_scale, float* in5, float* target, float* mask) {
+  if (!u || !c) return mac_internal_set_error(MAC_E_ARG, "null argument");
+  mac_grid_view v;
+  URC(mac_internal_view(c, &v));
+  double* dprev = nullptr;
+  const size_t n = (size_t)v.nb * 512;
+  if (prev) {
+    UCHK(hipMalloc(&dprev, n * sizeof(double)));
+    if (hipMemcpyAsync(dprev, prev, n * sizeof(double), hipMemcpyHostToDevice, v.st) != hipSuccess) {
+      hipFree(dprev);
+      return mac_internal_set_error(MAC_E_HIP, "hipMemcpyAsync of prev failed");
+    }
+  }
+  int rc = mac_internal_unet_sample(u, c, dprev, out_scale, in5, target, mask);
+  if (dprev) {
+    hipStreamSynchronize(v.st);
+    hipFree(dprev);
+  }
+  return rc;
+}
 
 int mac_unet_create(mac_unet** out, int D, int H, int W, unsigned long long seed) {
   if (!out) return mac_internal_set_error(MAC_E_ARG, "null out");
@@ -341,6 +438,8 @@ int mac_unet_create(mac_unet** out, int D, int H, int W, unsigned long long seed
   }
   hipEventCreate(&u->ev0);
   hipEventCreate(&u->ev1);
+  hipEventCreateWithFlags(&u->evx[0], hipEventDisableTiming);
+  hipEventCreateWithFlags(&u->evx[1], hipEventDisableTiming);
   // He-normal initial weights (std sqrt(2 / fan_in)), zero biases
   std::vector<float> p(off, 0.f);
   std::mt19937_64 gen(seed);
@@ -370,6 +469,8 @@ void mac_unet_destroy(mac_unet* u) {
   for (void* p : u->allocs) hipFree(p);
   if (u->ev0) hipEventDestroy(u->ev0);
   if (u->ev1) hipEventDestroy(u->ev1);
+  for (hipEvent_t e : u->evx)
+    if (e) hipEventDestroy(e);
   if (u->st) hipStreamDestroy(u->st);
   delete u;
 }

@@ -375,3 +375,81 @@ __global__ void k_pack_input(size_t nvox, const float* __restrict__ in5, bf16* _
     out8[e] = f2bf(c < 5 ? in5[v * 5 + c] : 0.f);
   }
 }
+
+// ------------------------------------------------ brick grid <-> volume ----
+// The network sees the grid's bounding box as a dense D x H x W volume (x
+// fastest); a cell of brick b with chunk key (kx, ky, kz) (shifted so that the
+// minimum of each axis is 0) and in-brick index t = x + 8y + 64z sits at voxel
+// (8kx + x) + W (8ky + y + H (8kz + z)).  Scales: s = rms of the right-hand side
+// over the unknowns; inputs are f / s and p ih2 / s, the output is p ih2 /
+// (s out_scale).  A zero right-hand side gives a zero guess.
+
+struct GuessGrid {
+  int nb;
+  const int* keys;        // 3 per brick
+  const uint8_t* flags;   // bit 7 exists, bits 0-2 material (2 = Solid)
+  const uint8_t* code;    // bit 6: unknown of the pressure system
+  const double* ff;       // ||f||^2 over the unknowns
+  double ndom, ih2m;
+  int D, H, W;
+};
+
+__device__ inline double guess_inv_scale(const GuessGrid& g) {
+  const double ff = *g.ff;
+  return ff > 0.0 ? sqrt(g.ndom / ff) : 0.0;
+}
+
+__device__ inline size_t guess_voxel(const GuessGrid& g, int b, int t) {
+  const int X = 8 * g.keys[3 * b] + (t & 7), Y = 8 * g.keys[3 * b + 1] + ((t >> 3) & 7), Z = 8 * g.keys[3 * b + 2] + (t >> 6);
+  return (size_t)X + (size_t)g.W * ((size_t)Y + (size_t)g.H * Z);
+}
+
+// every voxel: no right-hand side, no pressure, solid (cells outside the grid)
+__global__ void k_guess_fill(size_t nvox, float* __restrict__ in5) {
+  for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < nvox * 5; e += (size_t)gridDim.x * blockDim.x)
+    in5[e] = (e % 5 == 3) ? 1.f : 0.f;
+}
+
+// in5 from the grid: f / s, fluid (unknown), air (existing, not solid, not an
+// unknown: p = 0), solid (the rest), prev ih2 / s.
+__global__ void k_guess_pack(GuessGrid g, const double* __restrict__ f, const double* __restrict__ prev, float* __restrict__ in5) {
+  const double is = guess_inv_scale(g);
+  for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < (size_t)g.nb * 512; e += (size_t)gridDim.x * blockDim.x) {
+    const int b = (int)(e >> 9), t = (int)(e & 511);
+    const size_t v = guess_voxel(g, b, t);
+    const bool dom = (g.code[e] & 0x40) != 0;
+    const uint8_t fl = g.flags[e];
+    const bool air = !dom && (fl & 0x80) && (fl & 7) != 2;
+    float* o = in5 + 5 * v;
+    o[0] = dom ? (float)(f[e] * is) : 0.f;
+    o[1] = dom ? 1.f : 0.f;
+    o[2] = air ? 1.f : 0.f;
+    o[3] = (dom || air) ? 0.f : 1.f;
+    o[4] = (dom && prev) ? (float)(prev[e] * g.ih2m * is) : 0.f;
+  }
+}
+
+// guess x from the network output (zero off the unknowns)
+__global__ void k_guess_unpack(GuessGrid g, const float* __restrict__ out, float out_scale, double* __restrict__ x) {
+  const double is = guess_inv_scale(g);
+  const double sc = is > 0.0 ? (double)out_scale / (is * g.ih2m) : 0.0;
+  for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < (size_t)g.nb * 512; e += (size_t)gridDim.x * blockDim.x) {
+    const int b = (int)(e >> 9), t = (int)(e & 511);
+    const bool dom = (g.code[e] & 0x40) != 0;
+    x[e] = dom ? (double)out[guess_voxel(g, b, t)] * sc : 0.0;
+  }
+}
+
+// training target p ih2 / (s out_scale) and mask on the unknowns (target and
+// mask zero-filled beforehand)
+__global__ void k_guess_target(GuessGrid g, const double* __restrict__ p, float out_scale, float* __restrict__ tgt,
+                               float* __restrict__ mask) {
+  const double is = guess_inv_scale(g);
+  for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < (size_t)g.nb * 512; e += (size_t)gridDim.x * blockDim.x) {
+    const int b = (int)(e >> 9), t = (int)(e & 511);
+    if (!(g.code[e] & 0x40)) continue;
+    const size_t v = guess_voxel(g, b, t);
+    tgt[v] = (float)(p[e] * g.ih2m * is / out_scale);
+    mask[v] = 1.f;
+  }
+}

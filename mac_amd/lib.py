@@ -433,6 +433,11 @@ class Sim:
         _check(self.lib.mac_sim_step(self.ptr, ctypes.byref(st)))
         return st.as_dict()
 
+    def set_guess(self, mode, net=None, out_scale=1.0):
+        """Projection starting iterate: GUESS_ZERO, GUESS_PREVIOUS or GUESS_LEARNED (with net)."""
+        self._net = net  # the library does not own the network
+        _check(self.lib.mac_sim_set_guess(self.ptr, int(mode), net.ptr if net is not None else None, float(out_scale)))
+
 
 # ----------------------------------------------------------------- U-Net ----
 SIGNATURES.update({
@@ -445,7 +450,13 @@ SIGNATURES.update({
     "mac_unet_backward": (ctypes.c_int, [_P, _P, _P]),
     "mac_unet_train_step": (ctypes.c_int, [_P, _P, _P, _P, ctypes.c_float, ctypes.POINTER(ctypes.c_float)]),
     "mac_unet_conv_timing": (ctypes.c_int, [_P, ctypes.c_int, _dp, _dp]),
+    "mac_unet_guess": (ctypes.c_int, [_P, _P, ctypes.c_int, ctypes.c_float]),
+    "mac_unet_sample": (ctypes.c_int, [_P, _P, _P, ctypes.c_float, _P, _P, _P]),
+    "mac_sim_set_guess": (ctypes.c_int, [_P, ctypes.c_int, _P, ctypes.c_float]),
+    "mac_sim_capture": (ctypes.c_int, [_P, _P, ctypes.c_float, _P, _P, _P]),
 })
+
+GUESS_ZERO, GUESS_PREVIOUS, GUESS_LEARNED = 0, 1, 2
 
 
 class UNet:
@@ -498,6 +509,27 @@ class UNet:
         loss = ctypes.c_float()
         _check(self.lib.mac_unet_train_step(self.ptr, _ptr(x5), _ptr(t), _ptr(m), float(lr), ctypes.byref(loss)))
         return loss.value
+
+    def guess(self, proj, use_prev=False, out_scale=1.0):
+        """Write the learned starting iterate into proj's pressure."""
+        _check(self.lib.mac_unet_guess(self.ptr, proj.ctx, 1 if use_prev else 0, float(out_scale)))
+
+    def _pair_buffers(self):
+        n = int(np.prod(self.shape))
+        return np.zeros((n, 5), np.float32), np.zeros(n, np.float32), np.zeros(n, np.float32)
+
+    def sample(self, proj, prev=None, out_scale=1.0):
+        """(in5, target, mask) of proj's last projection; prev in brick layout or None."""
+        x5, t, m = self._pair_buffers()
+        pv = None if prev is None else np.ascontiguousarray(prev, np.float64).reshape(-1)
+        _check(self.lib.mac_unet_sample(self.ptr, proj.ctx, _ptr(pv), float(out_scale), _ptr(x5), _ptr(t), _ptr(m)))
+        return x5, t, m
+
+    def capture(self, sim, out_scale=1.0):
+        """(in5, target, mask) of the simulation's last projection."""
+        x5, t, m = self._pair_buffers()
+        _check(self.lib.mac_sim_capture(sim.ptr, self.ptr, float(out_scale), _ptr(x5), _ptr(t), _ptr(m)))
+        return x5, t, m
 
     def conv_timing(self, on):
         ms, fl = ctypes.c_double(), ctypes.c_double()

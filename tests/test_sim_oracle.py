@@ -131,3 +131,65 @@ def test_upwind_of_uniform_field_is_zero_and_gravity_on_fluid():
     assert np.all(dense["ax"][inner] == 0.0) and np.all(dense["ay"][inner] == 0.0)
     assert np.allclose(dense["az"][inner], -9.81, rtol=0, atol=1e-12)
     del jet
+
+
+def _all_fluid(seed=7, nb=2, h=0.1):
+    g = W.random_grid(seed, nb, nb, nb, fluid_p=1.0, solid_p=0.0, const_p=0.0)
+    flags = np.full(g["flags"].size, W.FLUID | W.EXISTS, np.uint8)
+    keys = g["keys"]
+    t = np.arange(512)
+    X = (keys[:, 0:1] * 8 + (t & 7)[None, :]).reshape(-1) * h
+    Y = (keys[:, 1:2] * 8 + ((t >> 3) & 7)[None, :]).reshape(-1) * h
+    Z = (keys[:, 2:3] * 8 + (t >> 6)[None, :]).reshape(-1) * h
+    return keys, flags, X, Y, Z
+
+
+def test_dt_closed_form_uniform_flow():
+    """Algorithms_Calculate_Time_Step.h:35-66: dt = cfl / sqrt(max_c sum_i
+    (u_i ih_i)^2); for a uniform field that is cfl / |u / h|, clamped."""
+    h = (0.1, 0.2, 0.05)
+    keys, flags, X, _, _ = _all_fluid(h=1.0)
+    n = flags.size
+    a, b, c = 0.7, -1.3, 0.4
+    for cfl, lo, hi in ((0.5, 0.0, 1e30), (0.9, 0.0, 1e-3), (0.5, 0.5, 1e30)):
+        s = O.Sim(h=h, cfl=cfl, dt_min=lo, dt_max=hi, gravity=(0, 0, 0))
+        s.upload(keys, flags, np.zeros(n, np.int8), u=np.full(n, a), v=np.full(n, b), w=np.full(n, c))
+        dt = s.compute_dt()
+        exact = cfl / np.sqrt((a / h[0]) ** 2 + (b / h[1]) ** 2 + (c / h[2]) ** 2)
+        assert abs(dt - min(max(exact, lo), hi)) <= 1e-15 * exact, (dt, exact)
+
+
+def test_viscosity_of_quadratic_field_is_exact():
+    """Algorithms_Viscosity / Policy_Laplacian_Speed (7-point): nu * lap of
+    u = Y^2 + 2 Z^2 is 6 nu in the interior; with v = w = 0 and u independent
+    of x the upwind convection term vanishes."""
+    h, nu = 0.1, 0.37
+    keys, flags, X, Y, Z = _all_fluid(h=h)
+    n = flags.size
+    s = O.Sim(h=(h, h, h), viscosity=nu, gravity=(0, 0, 0))
+    s.upload(keys, flags, np.zeros(n, np.int8), u=Y ** 2 + 2 * Z ** 2, v=np.zeros(n), w=np.zeros(n))
+    s.forces(0)
+    d = s.download()
+    ax = W.bricks_to_dense(d["ax"].reshape(-1, 512), 2, 2, 2)
+    inner = (slice(2, -2),) * 3
+    assert np.allclose(ax[inner], 6 * nu, rtol=0, atol=1e-10)
+    assert np.all(W.bricks_to_dense(d["ay"].reshape(-1, 512), 2, 2, 2)[inner] == 0.0)
+
+
+@pytest.mark.parametrize("prk", [2, 4])
+def test_particles_in_uniform_flow_move_exactly(prk):
+    """Algorithms_Move_Particles + Policy_Advance_ODE_RungeKutta: RK2 and RK4
+    of a uniform velocity field move every particle by dt * u."""
+    h = 0.1
+    keys, flags, *_ = _all_fluid(h=h)
+    n = flags.size
+    vel = (0.3, -0.25, 0.15)
+    s = O.Sim(h=(h, h, h), gravity=(0, 0, 0), particle_rk=prk, particles_per_inflow_cell=0)
+    rng = np.random.default_rng(1)
+    pos = rng.uniform(4 * h, 12 * h, size=(50, 3))
+    s.upload(keys, flags, np.zeros(n, np.int8), u=np.full(n, vel[0]), v=np.full(n, vel[1]), w=np.full(n, vel[2]),
+             particles=pos)
+    dt = 0.01
+    s.move_particles(dt)
+    out = s.download()["particles"]
+    assert np.allclose(out, pos + dt * np.array(vel), rtol=0, atol=1e-14)

@@ -44,6 +44,7 @@ struct Conv {
 struct Up {
   int Cin, Cout;
   size_t w_off, b_off;
+  bf16* wb = nullptr;  // BF16 copy of the weights for the forward pass
 };
 
 enum { ENC1A, ENC1B, ENC2A, ENC2B, ENC3A, ENC3B, DEC2A, DEC2B, DEC1A, DEC1B, NCONV };
@@ -90,6 +91,10 @@ int repack(mac_unet* u) {
     Conv& c = u->conv[l];
     size_t n = (size_t)c.Cout * c.Kpad + (size_t)c.Cin * c.Kpad2;
     k_pack_weights<<<nblk(n), 256, 0, u->st>>>(c.Cout, c.Cin, u->params + c.w_off, c.wf, c.Kpad, c.wb, c.Kpad2);
+  }
+  for (Up* up : {&u->up2, &u->up1}) {
+    const size_t n = (size_t)8 * up->Cout * up->Cin;
+    k_to_bf16<<<nblk(n), 256, 0, u->st>>>(n, u->params + up->w_off, up->wb);
   }
   UCHK(hipGetLastError());
   return 0;
@@ -156,10 +161,12 @@ int forward(mac_unet* u) {
   k_pool_fwd<<<nblk(u->n3 * 32), 256, 0, u->st>>>(u->C2, 64, 32, 32, v2, u->P2);
   conv_fwd(u, ENC3A, u->P2, 32, 0, v3, u->E3a, 64, 0, 1);
   conv_fwd(u, ENC3B, u->E3a, 64, 0, v3, u->E3, 64, 0, 1);
-  k_up_fwd<<<nblk(u->n2 * 32), 256, 0, u->st>>>(u->E3, 64, v3, u->params + u->up2.w_off, u->params + u->up2.b_off, 32, u->C2, 64, 0);
+  k_up_fwd_mfma<2><<<(v3.D * v3.H * (v3.W / 16) + 3) / 4, 256, 0, u->st>>>(u->E3, 64, v3, u->up2.wb, u->params + u->up2.b_off, u->C2,
+                                                                           64, 0);
   conv_fwd(u, DEC2A, u->C2, 64, 0, v2, u->D2a, 32, 0, 1);
   conv_fwd(u, DEC2B, u->D2a, 32, 0, v2, u->D2, 32, 0, 1);
-  k_up_fwd<<<nblk(u->n1 * 16), 256, 0, u->st>>>(u->D2, 32, v2, u->params + u->up1.w_off, u->params + u->up1.b_off, 16, u->C1, 32, 0);
+  k_up_fwd_mfma<1><<<(v2.D * v2.H * (v2.W / 16) + 3) / 4, 256, 0, u->st>>>(u->D2, 32, v2, u->up1.wb, u->params + u->up1.b_off, u->C1,
+                                                                           32, 0);
   conv_fwd(u, DEC1A, u->C1, 32, 0, v1, u->D1a, 16, 0, 1);
   conv_fwd(u, DEC1B, u->D1a, 16, 0, v1, u->D1, 16, 0, 1);
   k_head_fwd<<<nblk(u->n1), 256, 0, u->st>>>(u->D1, 16, u->n1, u->params + u->head_w, u->params + u->head_b, u->out);
@@ -397,6 +404,8 @@ int mac_unet_create(mac_unet** out, int D, int H, int W, unsigned long long seed
     A(c.wf, (size_t)c.Cout * c.Kpad);
     A(c.wb, (size_t)c.Cin * c.Kpad2);
   }
+  A(u->up2.wb, (size_t)8 * 32 * 64);
+  A(u->up1.wb, (size_t)8 * 16 * 32);
   const size_t n1 = u->n1, n2 = u->n2, n3 = u->n3;
   A(u->X0, n1 * 8);
   A(u->E1a, n1 * 16);

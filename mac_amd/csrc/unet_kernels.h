@@ -255,6 +255,58 @@ __global__ void k_up_fwd(const bf16* __restrict__ in, int Cin, Vol vc, const flo
   }
 }
 
+// FP32 -> BF16 copy (the transposed-convolution weights for the matrix cores).
+__global__ void k_to_bf16(size_t n, const float* __restrict__ w, bf16* __restrict__ out) {
+  for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x) out[e] = f2bf(w[e]);
+}
+
+// Transposed convolution forward on the matrix cores: for each octant o,
+// out[2v+o][co] = relu(b[co] + sum_ci W[o][co][ci] in[v][ci]), a GEMM of the
+// coarse voxels (rows) by Cin (K) by Cout (columns).  One wave: 16 voxels of
+// a coarse x row, all 8 octants, Cout = 16 NB; Cin a multiple of 32.  Weights
+// wb [8][Cout][Cin] in BF16.  MFMA 16x16x32 operand layout: lane l holds A
+// row l%16 and B column l%16 at k = 8(l/16) + j; D rows 4(l/16) + j.
+template <int NB>
+__global__ void __launch_bounds__(256) k_up_fwd_mfma(const bf16* __restrict__ in, int Cin, Vol vc, const bf16* __restrict__ wb,
+                                                     const float* __restrict__ b, bf16* __restrict__ out, int out_cs, int out_co) {
+  const int lane = threadIdx.x & 63;
+  const int wave = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  const int nxw = vc.W >> 4;
+  if (wave >= vc.D * vc.H * nxw) return;
+  const int xw = wave % nxw, y = (wave / nxw) % vc.H, z = wave / (nxw * vc.H);
+  const int i = lane & 15, g = lane >> 4;
+  const int Cout = 16 * NB;
+  const size_t vrow = (size_t)(z * vc.H + y) * vc.W + (xw << 4);
+  const int Hf = 2 * vc.H, Wf = 2 * vc.W;
+  for (int o = 0; o < 8; ++o) {
+    floatx4 acc[NB];
+#pragma unroll
+    for (int nb = 0; nb < NB; ++nb) acc[nb] = floatx4{0.f, 0.f, 0.f, 0.f};
+    for (int kc = 0; kc < Cin; kc += 32) {
+      const int k = kc + 8 * g;
+      const bf16x8 a = *(const bf16x8*)(in + (vrow + i) * Cin + k);
+#pragma unroll
+      for (int nb = 0; nb < NB; ++nb) {
+        const bf16x8 bb = *(const bf16x8*)(wb + ((size_t)o * Cout + 16 * nb + i) * Cin + k);
+        acc[nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bb, acc[nb], 0, 0, 0);
+      }
+    }
+    const int ox = o & 1, oy = (o >> 1) & 1, oz = o >> 2;
+#pragma unroll
+    for (int nb = 0; nb < NB; ++nb) {
+      const int co = 16 * nb + i;
+      const float bv = b[co];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int xc = (xw << 4) + 4 * g + j;
+        const size_t fv = ((size_t)(2 * z + oz) * Hf + 2 * y + oy) * Wf + 2 * xc + ox;
+        const float val = acc[nb][j] + bv;
+        out[fv * out_cs + out_co + co] = f2bf(val > 0.f ? val : 0.f);
+      }
+    }
+  }
+}
+
 // Transposed conv backward-data: dIn[v][ci] = sum_{o,co} W[o][co][ci] G[2v+o][co].
 __global__ void k_up_bwd_data(const bf16* __restrict__ g, int g_cs, int g_co, int Cout, Vol vc, const float* __restrict__ w,
                               int Cin, bf16* __restrict__ din) {

@@ -9,8 +9,9 @@ dec1b; weights [Cout][27 Cin] with K ordered (tap, cin), tap = 9(dz+1) +
 3(dy+1) + (dx+1), then biases), the two 2x2x2 transposed convolutions up2,
 up1 (weights [8][Cout][Cin], biases), and the 1x1x1 head (16 weights, 1 bias).
 Arrays are channels-last (D, H, W, C).  With bf16=True every stored
-activation and every convolution weight is rounded to bfloat16 (round to
-nearest even) as the device stores them.
+activation and every convolution and transposed-convolution weight of the
+forward pass is rounded to bfloat16 (round to nearest even) as the device
+stores them.
 """
 import numpy as np
 
@@ -148,11 +149,11 @@ def forward(p, x5, shape, bf=False):
     c["p2"] = r(pool(c["e2"]))
     c["e3a"] = cv("enc3a", c["p2"])
     c["e3"] = cv("enc3b", c["e3a"])
-    c["u2"] = r(np.maximum(up(c["e3"], P["up2.w"], P["up2.b"]), 0.0))
+    c["u2"] = r(np.maximum(up(c["e3"], wr(P["up2.w"]), P["up2.b"]), 0.0))
     c["c2"] = np.concatenate([c["u2"], c["e2"]], axis=-1)
     c["d2a"] = cv("dec2a", c["c2"])
     c["d2"] = cv("dec2b", c["d2a"])
-    c["u1"] = r(np.maximum(up(c["d2"], P["up1.w"], P["up1.b"]), 0.0))
+    c["u1"] = r(np.maximum(up(c["d2"], wr(P["up1.w"]), P["up1.b"]), 0.0))
     c["c1"] = np.concatenate([c["u1"], c["e1"]], axis=-1)
     c["d1a"] = cv("dec1a", c["c1"])
     c["d1"] = cv("dec1b", c["d1a"])

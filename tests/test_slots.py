@@ -22,6 +22,10 @@ def _binary():
 
 
 def _case(tmp_path, g):
+    # a reference grid stores speeds only for cells that exist
+    ex = (g["flags"] & 0x80) > 0
+    for k in ("u", "v", "w"):
+        g[k] = np.where(ex, g[k], 0.0)
     n, off, idx, val, row = O.assemble(g)
     div = O.divergence(g)
     b = np.zeros(n)

@@ -53,6 +53,13 @@ enum mac_precision {
   MAC_MIXED = 1  /* FP32 V-cycle (smooth/restrict/prolong) inside FP64 CG    */
 };
 
+enum mac_smoother {
+  MAC_SMOOTH_JACOBI = 0,    /* nu damped-Jacobi sweeps with weight omega                  */
+  MAC_SMOOTH_CHEBYSHEV = 1  /* nu Jacobi sweeps with the Chebyshev weights of [a, 2]:
+                               omega_k = 1 / (1 + a/2 + (1 - a/2) cos((2k-1) pi / (2 nu))),
+                               k = 1..nu before the coarse correction, reversed after      */
+};
+
 typedef struct mac_ctx mac_ctx;
 
 typedef struct {
@@ -67,6 +74,9 @@ typedef struct {
   int stall_iters; /* guided solve (guess = 1): restart from zero if not converged
                       after this many iterations (0: never); a guess whose first
                       residual is no smaller than ||f|| is dropped at once        */
+  int smoother;    /* enum mac_smoother                                            */
+  double cheb_lower; /* MAC_SMOOTH_CHEBYSHEV: lower end a of the eigenvalue range
+                      [a, 2] of D^-1 A that the nu sweeps damp                    */
 } mac_params;
 
 typedef struct {
@@ -84,7 +94,8 @@ typedef struct {
   double ms_solve;       /* PCG part only                                        */
 } mac_stats;
 
-/* Defaults: h=1, rtol=1e-8, max_iter=500, FP64, nu=2, omega=0.9, check_every=4, stall_iters=0. */
+/* Defaults: h=1, rtol=1e-8, max_iter=500, FP64, nu=2, omega=0.9, check_every=4, stall_iters=0,
+ * Chebyshev smoothing with a = 0.25. */
 void mac_default_params(mac_params* prm);
 
 /* Create / destroy a context on the current HIP device. */

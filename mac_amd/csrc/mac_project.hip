@@ -83,6 +83,7 @@ struct Level {
 
 struct mac_ctx {
   mac_params prm;
+  double om_pre[8], om_post[8];  // per-sweep Jacobi weights before / after the coarse correction
   hipStream_t st = nullptr;
   int dev = 0;
   int nb0 = 0;
@@ -531,10 +532,10 @@ inline long long lvl_dom(const mac_ctx* c, int l) { return c->L[l].ndom; }
 
 // Damped-Jacobi sweep at level l (level 0 reads the CG residual in FP64).
 template <typename TZ>
-void sweep(mac_ctx* c, int l, const TZ* zin, TZ* zout, int want_dot) {
+void sweep(mac_ctx* c, int l, const TZ* zin, TZ* zout, int want_dot, double omega) {
   Level& lv = c->L[l];
   const TZ ih[3] = {(TZ)lv.ih2[0], (TZ)lv.ih2[1], (TZ)lv.ih2[2]};
-  const TZ om = (TZ)c->prm.omega;
+  const TZ om = (TZ)omega;
   const double sz = sizeof(TZ), nd = (double)lv.ndom;
   const double sr = (l == 0) ? 8.0 : sz;
   {
@@ -555,7 +556,7 @@ void sweep(mac_ctx* c, int l, const TZ* zin, TZ* zout, int want_dot) {
 template <typename TZ>
 int vcycle(mac_ctx* c, TZ** zout) {
   const int nl = (int)c->L.size();
-  const int nu = std::max(1, c->prm.nu);
+  const int nu = std::min(8, std::max(1, c->prm.nu));
   const double sz = sizeof(TZ);
   int nlv = nl;
   for (int l = 1; l < nl; ++l)
@@ -579,15 +580,15 @@ int vcycle(mac_ctx* c, TZ** zout) {
       TimedLaunch tl(c, l == 0 ? K_JACOBI2_0 : K_C_JACOBI2, nd * (sr + sz + 1));
       if (l == 0)
         k_jacobi2<double, TZ><<<grid_of(lv.nact), KTHREADS, 0, c->st>>>(lv.nact, lv.meta, lv.code, c->r, z, ih[0], ih[1],
-                                                                         ih[2], (TZ)c->prm.omega, c->s);
+                                                                         ih[2], (TZ)c->om_pre[0], (TZ)c->om_pre[1], c->s);
       else
         k_jacobi2<TZ, TZ><<<grid_of(lv.nact), KTHREADS, 0, c->st>>>(lv.nact, lv.meta, lv.code, (const TZ*)lv.r, z, ih[0],
-                                                                     ih[1], ih[2], (TZ)c->prm.omega, c->s);
+                                                                     ih[1], ih[2], (TZ)c->om_pre[0], (TZ)c->om_pre[1], c->s);
     } else {
-      sweep<TZ>(c, l, nullptr, z, 0);
+      sweep<TZ>(c, l, nullptr, z, 0, c->om_pre[0]);
     }
     for (int k = 2; k < nu; ++k) {
-      sweep<TZ>(c, l, z, zt, 0);
+      sweep<TZ>(c, l, z, zt, 0, c->om_pre[k]);
       std::swap(z, zt);
     }
     {
@@ -628,7 +629,8 @@ int vcycle(mac_ctx* c, TZ** zout) {
       TZ* z = (TZ*)lb.z;
       TZ* zt = (TZ*)lb.zt;
       for (int k = 0; k < 2 * nu; ++k) {
-        sweep<TZ>(c, bot, k ? z : nullptr, k ? zt : z, (k == 2 * nu - 1) ? want_dot : 0);
+        sweep<TZ>(c, bot, k ? z : nullptr, k ? zt : z, (k == 2 * nu - 1) ? want_dot : 0,
+                  k < nu ? c->om_pre[k] : c->om_post[k - nu]);
         if (k) std::swap(z, zt);
       }
       zfin[bot] = z;
@@ -648,7 +650,7 @@ int vcycle(mac_ctx* c, TZ** zout) {
       k_prolong_col<TZ><<<grid_of(lv.nact), KTHREADS, 0, c->st>>>(lv.nact, lv.pmeta, lv.code, zfin[l + 1], z, c->s);
     }
     for (int k = 0; k < nu; ++k) {
-      sweep<TZ>(c, l, z, zt, (l == 0 && k == nu - 1) ? 1 : 0);
+      sweep<TZ>(c, l, z, zt, (l == 0 && k == nu - 1) ? 1 : 0, c->om_post[k]);
       std::swap(z, zt);
     }
     zfin[l] = z;
@@ -771,6 +773,31 @@ int pcg(mac_ctx* c, bool use_guess, mac_stats* st) {
   }
 }
 
+int check_params(const mac_params* p) {
+  if (p->nu < 1 || p->nu > 8 || p->max_iter < 1 || !(p->rtol > 0)) return fail(MAC_E_ARG, "bad solver parameters");
+  if (p->smoother != MAC_SMOOTH_JACOBI && p->smoother != MAC_SMOOTH_CHEBYSHEV) return fail(MAC_E_ARG, "unknown smoother");
+  if (p->smoother == MAC_SMOOTH_CHEBYSHEV && !(p->cheb_lower > 0 && p->cheb_lower < 2))
+    return fail(MAC_E_ARG, "cheb_lower must lie in (0, 2)");
+  if (p->smoother == MAC_SMOOTH_JACOBI && !(p->omega > 0 && p->omega < 2)) return fail(MAC_E_ARG, "omega must lie in (0, 2)");
+  return 0;
+}
+
+// Sweep weights from the parameters: nu copies of omega, or the reciprocals
+// of the Chebyshev nodes of [a, 2] (the polynomial of degree nu with value 1
+// at 0 that is smallest on [a, 2]); the sweeps after the coarse correction
+// take them in reverse order.
+void set_schedule(mac_ctx* c) {
+  const int nu = std::min(8, std::max(1, c->prm.nu));
+  const double a = c->prm.cheb_lower, b = 2.0;
+  for (int k = 0; k < nu; ++k) {
+    double om = c->prm.omega;
+    if (c->prm.smoother == MAC_SMOOTH_CHEBYSHEV)
+      om = 1.0 / (0.5 * (b + a) + 0.5 * (b - a) * std::cos((2.0 * k + 1.0) * M_PI / (2.0 * nu)));
+    c->om_pre[k] = om;
+    c->om_post[nu - 1 - k] = om;
+  }
+}
+
 int ensure(mac_ctx* c) {
   if (!c) return fail(MAC_E_ARG, "null context");
   if (!c->have_grid) return fail(MAC_E_STATE, "no grid uploaded");
@@ -796,6 +823,8 @@ void mac_default_params(mac_params* p) {
   p->omega = 0.9;
   p->check_every = 4;
   p->stall_iters = 0;
+  p->smoother = MAC_SMOOTH_CHEBYSHEV;
+  p->cheb_lower = 0.25;
 }
 
 int mac_create(mac_ctx** out, const mac_params* prm) {
@@ -804,8 +833,10 @@ int mac_create(mac_ctx** out, const mac_params* prm) {
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(MAC_E_NODEV, "no HIP device visible");
   mac_ctx* c = new mac_ctx();
+  if (prm && check_params(prm) != 0) return MAC_E_ARG;
   if (prm) c->prm = *prm;
   else mac_default_params(&c->prm);
+  set_schedule(c);
   hipGetDevice(&c->dev);
   if (hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking) != hipSuccess) {
     delete c;
@@ -835,8 +866,9 @@ int mac_set_params(mac_ctx* c, const mac_params* prm) {
   if (!c || !prm) return fail(MAC_E_ARG, "null argument");
   bool hchange = c->have_grid && (prm->h[0] != c->prm.h[0] || prm->h[1] != c->prm.h[1] || prm->h[2] != c->prm.h[2]);
   if (hchange) return fail(MAC_E_STATE, "h cannot change after the grid is uploaded");
-  if (prm->nu < 1 || prm->max_iter < 1 || !(prm->rtol > 0)) return fail(MAC_E_ARG, "bad solver parameters");
+  CHK(check_params(prm));
   c->prm = *prm;
+  set_schedule(c);
   return 0;
 }
 

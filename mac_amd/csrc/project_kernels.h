@@ -494,13 +494,13 @@ __global__ void __launch_bounds__(KTHREADS) k_jacobi(int nact, const int* __rest
 }
 
 // Two damped-Jacobi sweeps from a zero start in one pass:
-//   z1 = omega D^-1 r (formed on the fly, also at the halo), zout = z1 + omega D^-1 (r - L z1).
+//   z1 = omega D^-1 r (formed on the fly, also at the halo), zout = z1 + omega2 D^-1 (r - L z1).
 // Bitwise equal to two k_jacobi launches (same expressions, same order).
 template <typename TR, typename TZ>
 __global__ void __launch_bounds__(KTHREADS) k_jacobi2(int nact, const int* __restrict__ meta,
                                                       const uint8_t* __restrict__ code, const TR* __restrict__ r,
                                                       TZ* __restrict__ zout, TZ ihx2, TZ ihy2, TZ ihz2, TZ omega,
-                                                      const Scal* s) {
+                                                      TZ omega2, const Scal* s) {
   if (s->done) return;
   const int lane = lane_id();
   FOR_EACH_BRICK(meta, nact, M) {
@@ -528,7 +528,7 @@ __global__ void __launch_bounds__(KTHREADS) k_jacobi2(int nact, const int* __res
           if (cd[z] & CODE_DOM) {
             TZ dg;
             TZ lz = lap_cell<TZ>(cd[z], z1[z], xm, xp, ym, yp, zm, zp, ihx2, ihy2, ihz2, dg);
-            out = dg > TZ(0) ? z1[z] + omega * (rv[z] - lz) / dg : TZ(0);
+            out = dg > TZ(0) ? z1[z] + omega2 * (rv[z] - lz) / dg : TZ(0);
           }
           zout[base + 64 * z] = out;
         });

@@ -34,6 +34,8 @@ class Params(ctypes.Structure):
         ("omega", ctypes.c_double),
         ("check_every", ctypes.c_int),
         ("stall_iters", ctypes.c_int),
+        ("smoother", ctypes.c_int),
+        ("cheb_lower", ctypes.c_double),
     ]
 
 

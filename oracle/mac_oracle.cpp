@@ -379,7 +379,8 @@ struct MG {
   int nc = 0;
   std::vector<long long> cellof;
   std::vector<double> inv;
-  double omega = 0.8;
+  double om_pre[8] = {0}, om_post[8] = {0};  // per-sweep weights (the device's schedule)
+  double omega = 0.8;                        // weight of the sweep being run
   int nu = 2;
 
   double lap(const DLevel& l, const std::vector<double>& v, int x, int y, int z, double& dg) const {
@@ -496,8 +497,10 @@ struct MG {
     }
     for (int l = 0; l < nl - 1; ++l) {
       DLevel& lv = L[l];
+      omega = om_pre[0];
       jacobi(lv, lv.r, nullptr, lv.z);
       for (int k = 1; k < nu; ++k) {
+        omega = om_pre[k];
         jacobi(lv, lv.r, &lv.z, lv.zt);
         std::swap(lv.z, lv.zt);
       }
@@ -509,6 +512,7 @@ struct MG {
       DLevel& lv = L[l];
       prolong(lv, L[l + 1], lv.z);
       for (int k = 0; k < nu; ++k) {
+        omega = om_post[k];
         jacobi(lv, lv.r, &lv.z, lv.zt);
         std::swap(lv.z, lv.zt);
       }
@@ -526,7 +530,8 @@ extern "C" {
 // out[1] = solve ms (excluding set-up), out[2] = set-up ms, out[3] = true
 // relative residual, out[4] = levels.  Returns 0, or -1 on non-convergence.
 int orc_mgpcg(int nb, const int32_t* keys, const uint8_t* flags, const double* h, const double* u, const double* v,
-              const double* w, double* p, double rtol, int maxit, int nu, double omega, int nthreads, double* out) {
+              const double* w, double* p, double rtol, int maxit, int nu, double omega, int smoother, double cheb_lower,
+              int nthreads, double* out) {
   if (nthreads > 0) omp_set_num_threads(nthreads);
   auto t0 = std::chrono::steady_clock::now();
   double inv_h[3] = {1.0 / h[0], 1.0 / h[1], 1.0 / h[2]};
@@ -538,8 +543,15 @@ int orc_mgpcg(int nb, const int32_t* keys, const uint8_t* flags, const double* h
       mx[i] = std::max(mx[i], (int)keys[3 * b + i]);
     }
   MG mg;
-  mg.omega = omega;
-  mg.nu = nu;
+  mg.nu = nu = std::min(8, std::max(1, nu));
+  // weights: omega for every sweep, or the reciprocal Chebyshev nodes of
+  // [cheb_lower, 2] (mac_amd.h, enum mac_smoother), reversed after the coarse solve
+  for (int k = 0; k < nu; ++k) {
+    double om = omega;
+    if (smoother == 1) om = 1.0 / (0.5 * (2.0 + cheb_lower) + 0.5 * (2.0 - cheb_lower) * std::cos((2.0 * k + 1.0) * M_PI / (2.0 * nu)));
+    mg.om_pre[k] = om;
+    mg.om_post[nu - 1 - k] = om;
+  }
   // level 0 dense over the brick bounding box
   {
     DLevel l0;

@@ -16,7 +16,8 @@ from tests import slots_io as S
 
 
 def _binary():
-    if not os.path.exists(S.BIN) or os.path.getmtime(S.BIN) < os.path.getmtime(S.SRC):
+    deps = [S.SRC, os.path.join(S.ROOT, "include", "mac_amd.h"), os.path.join(S.ROOT, "include", "mac_amd_slots.hpp")]
+    if not os.path.exists(S.BIN) or os.path.getmtime(S.BIN) < max(os.path.getmtime(d) for d in deps):
         S.build()
     return S.BIN
 

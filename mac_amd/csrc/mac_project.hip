@@ -84,6 +84,9 @@ struct Level {
 struct mac_ctx {
   mac_params prm;
   double om_pre[8], om_post[8];  // per-sweep Jacobi weights before / after the coarse correction
+  // two CG iterations captured as one graph, per V-cycle precision (0 FP64, 1 FP32)
+  hipGraphExec_t gexec[2] = {nullptr, nullptr};
+  const void* gz[2] = {nullptr, nullptr};  // the V-cycle output the captured pair starts from
   hipStream_t st = nullptr;
   int dev = 0;
   int nb0 = 0;
@@ -126,7 +129,16 @@ struct mac_ctx {
 
 namespace {
 
+void drop_graphs(mac_ctx* c) {
+  for (int i = 0; i < 2; ++i) {
+    if (c->gexec[i]) hipGraphExecDestroy(c->gexec[i]);
+    c->gexec[i] = nullptr;
+    c->gz[i] = nullptr;
+  }
+}
+
 void free_all(mac_ctx* c) {
+  drop_graphs(c);
   for (void* p : c->allocs) hipFree(p);
   c->allocs.clear();
   c->L.clear();
@@ -552,7 +564,10 @@ void sweep(mac_ctx* c, int l, const TZ* zin, TZ* zout, int want_dot, double omeg
 
 // z = V(r) at level 0 (r = the CG residual).  Levels below the first level
 // without unknowns are not visited: the bottom visited level then just
-// smooths (nu sweeps down, nu up), which keeps the cycle symmetric.
+// smooths (nu sweeps down, nu up), which keeps the cycle symmetric.  Every
+// cycle uses the same buffers in the same roles (each level starts from its
+// z / zt pair and ends where the sweep count puts it), so a captured cycle
+// replays correctly.
 template <typename TZ>
 int vcycle(mac_ctx* c, TZ** zout) {
   const int nl = (int)c->L.size();
@@ -606,8 +621,6 @@ int vcycle(mac_ctx* c, TZ** zout) {
       k_restrict_sep<TZ><<<cl.nact, KTHREADS, 0, c->st>>>(cl.rmeta, cl.code, res, (TZ*)cl.r, c->s);
     }
     zfin[l] = z;  // pre-smoothed iterate, finished on the way up
-    lv.z = z;
-    lv.zt = zt;
   }
   // bottom
   {
@@ -634,8 +647,6 @@ int vcycle(mac_ctx* c, TZ** zout) {
         if (k) std::swap(z, zt);
       }
       zfin[bot] = z;
-      lb.z = z;
-      lb.zt = zt;
     }
   }
   // up
@@ -654,8 +665,6 @@ int vcycle(mac_ctx* c, TZ** zout) {
       std::swap(z, zt);
     }
     zfin[l] = z;
-    lv.z = z;
-    lv.zt = zt;
   }
   *zout = zfin[0];
   return 0;
@@ -710,7 +719,65 @@ int pcg(mac_ctx* c, bool use_guess, mac_stats* st) {
     TZ* z = nullptr;
     if (!c->hs->done) CHK(vcycle<TZ>(c, &z));
     int iter_before = c->hs->iter;
-    while (!c->hs->done) {
+    // Without event timing the iterations run as a captured graph of two
+    // (the search directions alternate between d[0] and d[1]), and the host
+    // polls when the residual decay so far predicts convergence.
+    const int gi = std::is_same<TZ, float>::value ? 1 : 0;
+    if (!c->timing && !c->hs->done) {
+      if (!c->gexec[gi] || c->gz[gi] != (const void*)z) {
+        if (c->gexec[gi]) hipGraphExecDestroy(c->gexec[gi]);
+        c->gexec[gi] = nullptr;
+        hipGraph_t g = nullptr;
+        HIPCHK(hipStreamBeginCapture(c->st, hipStreamCaptureModeRelaxed));
+        TZ* zc = z;
+        int cc = 0;
+        for (int k = 0; k < 2; ++k) {
+          k_pmatvec<TZ><<<grid_of(l0.nact), KTHREADS, 0, c->st>>>(l0.nact, l0.meta, l0.code, zc, c->d[cc], c->d[cc ^ 1], c->q,
+                                                                  ih[0], ih[1], ih[2], c->partials, c->s);
+          finalize(c, grid_of(l0.nact), OP_DQ);
+          cc ^= 1;
+          k_update<<<grid_of(l0.nact), KTHREADS, 0, c->st>>>(l0.nact, l0.meta, l0.code, c->x, c->r, c->d[cc], c->q, c->partials, c->s);
+          finalize(c, grid_of(l0.nact), OP_RR);
+          int rc = vcycle<TZ>(c, &zc);
+          if (rc || zc != z) {
+            hipStreamEndCapture(c->st, &g);
+            if (g) hipGraphDestroy(g);
+            return rc ? rc : fail(MAC_E_STATE, "V-cycle output buffer moved between cycles");
+          }
+        }
+        HIPCHK(hipStreamEndCapture(c->st, &g));
+        hipError_t e = hipGraphInstantiate(&c->gexec[gi], g, nullptr, nullptr, 0);
+        hipGraphDestroy(g);
+        if (e != hipSuccess) {
+          c->gexec[gi] = nullptr;
+          return fail(MAC_E_HIP, (std::string("hipGraphInstantiate: ") + hipGetErrorString(e)).c_str());
+        }
+        c->gz[gi] = (const void*)z;
+      }
+      int pairs = 1;
+      double rr_prev = c->hs->rr;
+      while (!c->hs->done) {
+        for (int k = 0; k < pairs; ++k) HIPCHK(hipGraphLaunch(c->gexec[gi], c->st));
+        CHK(read_scal(c));
+        if (use_guess && !fellback && c->prm.stall_iters > 0 && !c->hs->done && c->hs->iter >= c->prm.stall_iters) break;
+        if (c->hs->done) break;
+        // iterations still needed at the mean decay rate so far; polling
+        // early costs one host round trip, polling late a pair of iterations
+        // whose kernels return at once
+        const int di = c->hs->iter - iter_before;
+        pairs = 1;
+        if (di > 0 && c->hs->rr > 0 && c->hs->rr < rr_prev && c->hs->tol2 > 0) {
+          const double rate = std::log(c->hs->rr / rr_prev) / di;  // < 0
+          const double need = std::log(c->hs->tol2 / c->hs->rr) / rate;
+          pairs = std::max(1, (int)std::floor(0.75 * need / 2.0));
+        }
+        if (use_guess && !fellback && c->prm.stall_iters > 0)  // stop at the first pair past stall_iters
+          pairs = std::min(pairs, std::max(1, (c->prm.stall_iters - c->hs->iter + 1) / 2));
+        rr_prev = c->hs->rr;
+        iter_before = c->hs->iter;
+      }
+    }
+    while (c->timing && !c->hs->done) {
       // A batch of `check` iterations; once the update kernel of one of them
       // sets `done`, every later kernel of the batch returns at once.
       std::vector<long long> useq;
@@ -869,6 +936,7 @@ int mac_set_params(mac_ctx* c, const mac_params* prm) {
   CHK(check_params(prm));
   c->prm = *prm;
   set_schedule(c);
+  drop_graphs(c);  // weights, tolerance and sweep counts are baked into them
   return 0;
 }
 

@@ -217,6 +217,35 @@ def test_guided_solve_falls_back_after_stall_iters():
     pr.set_pressure(0.5 * p0)  # a fair guess that still needs many iterations
     st1 = pr.Divergence_Projection(guess=True)
     assert st1["fallback_used"] == 1
-    assert st1["iters"] == st0["iters"] + 3, (st0["iters"], st1["iters"])
+    # the stall check runs after each captured pair of iterations
+    assert st0["iters"] + 3 <= st1["iters"] <= st0["iters"] + 4, (st0["iters"], st1["iters"])
     assert st1["true_rel_resid"] <= 1e-10 * 1.0001
     assert rel(pr.get_pressure(), p0) < 1e-8
+
+
+@pytest.mark.parametrize("prec", [MAC_FP64, MAC_MIXED])
+def test_graph_replays_match_eager_launches(prec):
+    """The captured two-iteration graphs give bit-identical results to the
+    same kernels launched one by one (event-timing mode), on every replay,
+    with other solves and host round trips in between."""
+    gs = [small_spheres(32, 7), small_spheres(32, 8)]
+    ctx = {}
+    for mode in ("eager", "graph"):
+        for i, g in enumerate(gs):
+            pr = make(g, rtol=1e-10, precision=prec)
+            pr.enable_timing(mode == "eager")
+            pr.save_velocity()
+            ctx[(mode, i)] = pr
+    outs = {}
+    for rep in range(3):
+        for i in range(len(gs)):
+            for mode in ("eager", "graph"):
+                pr = ctx[(mode, i)]
+                pr.restore_velocity()
+                st = pr.Divergence_Projection()
+                outs[(mode, i, rep)] = (st["iters"], pr.get_pressure(), np.concatenate(pr.get_velocity()))
+    for i in range(len(gs)):
+        for rep in range(3):
+            ie, pe, ue = outs[("eager", i, rep)]
+            ig, pg, ug = outs[("graph", i, rep)]
+            assert ie == ig and np.array_equal(pe, pg) and np.array_equal(ue, ug), (i, rep)

@@ -81,6 +81,10 @@ typedef struct {
   int smoother;    /* enum mac_smoother                                            */
   double cheb_lower; /* MAC_SMOOTH_CHEBYSHEV: lower end a of the eigenvalue range
                       [a, 2] of D^-1 A that the nu sweeps damp                    */
+  int nu_coarse;   /* sweeps before and after the correction on the coarse levels
+                      (0: nu); they are short, latency-bound launches, and more
+                      of them buy fewer outer iterations                          */
+  double cheb_lower_coarse; /* a of the coarse levels' Chebyshev weights          */
 } mac_params;
 
 typedef struct {
@@ -99,7 +103,7 @@ typedef struct {
 } mac_stats;
 
 /* Defaults: h=1, rtol=1e-8, max_iter=500, FP64, nu=2, omega=0.9, check_every=4, stall_iters=0,
- * Chebyshev smoothing with a = 0.25. */
+ * Chebyshev smoothing with a = 0.25 on level 0 and nu_coarse=4, a = 0.1 below. */
 void mac_default_params(mac_params* prm);
 
 /* Create / destroy a context on the current HIP device. */

@@ -83,7 +83,8 @@ struct Level {
 
 struct mac_ctx {
   mac_params prm;
-  double om_pre[8], om_post[8];  // per-sweep Jacobi weights before / after the coarse correction
+  double om_pre[8], om_post[8];  // per-sweep Jacobi weights before / after the coarse correction, level 0
+  double omc_pre[8], omc_post[8];  // the same on the coarse levels
   // two CG iterations captured as one graph, per V-cycle precision (0 FP64, 1 FP32)
   hipGraphExec_t gexec[2] = {nullptr, nullptr};
   const void* gz[2] = {nullptr, nullptr};  // the V-cycle output the captured pair starts from
@@ -538,6 +539,41 @@ int build_grid(mac_ctx* c, int nbricks, const int32_t* keys3, const uint8_t* fla
   return 0;
 }
 
+int check_params(const mac_params* p) {
+  if (p->nu < 1 || p->nu > 8 || p->max_iter < 1 || !(p->rtol > 0)) return fail(MAC_E_ARG, "bad solver parameters");
+  if (p->smoother != MAC_SMOOTH_JACOBI && p->smoother != MAC_SMOOTH_CHEBYSHEV) return fail(MAC_E_ARG, "unknown smoother");
+  if (p->smoother == MAC_SMOOTH_CHEBYSHEV && !(p->cheb_lower > 0 && p->cheb_lower < 2 && p->cheb_lower_coarse > 0 &&
+                                                p->cheb_lower_coarse < 2))
+    return fail(MAC_E_ARG, "cheb_lower and cheb_lower_coarse must lie in (0, 2)");
+  if (p->nu_coarse < 0 || p->nu_coarse > 8) return fail(MAC_E_ARG, "nu_coarse must lie in 0..8");
+  if (p->smoother == MAC_SMOOTH_JACOBI && !(p->omega > 0 && p->omega < 2)) return fail(MAC_E_ARG, "omega must lie in (0, 2)");
+  return 0;
+}
+
+// Sweep weights from the parameters: nu copies of omega, or the reciprocals
+// of the Chebyshev nodes of [a, 2] (the polynomial of degree nu with value 1
+// at 0 that is smallest on [a, 2]); the sweeps after the coarse correction
+// take them in reverse order.
+void schedule(const mac_params& p, int nu, double a, double* pre, double* post) {
+  const double b = 2.0;
+  for (int k = 0; k < nu; ++k) {
+    double om = p.omega;
+    if (p.smoother == MAC_SMOOTH_CHEBYSHEV) om = 1.0 / (0.5 * (b + a) + 0.5 * (b - a) * std::cos((2.0 * k + 1.0) * M_PI / (2.0 * nu)));
+    pre[k] = om;
+    post[nu - 1 - k] = om;
+  }
+}
+
+inline int nu_of(const mac_params& p, int level) {
+  const int nu = (level > 0 && p.nu_coarse > 0) ? p.nu_coarse : p.nu;
+  return std::min(8, std::max(1, nu));
+}
+
+void set_schedule(mac_ctx* c) {
+  schedule(c->prm, nu_of(c->prm, 0), c->prm.cheb_lower, c->om_pre, c->om_post);
+  schedule(c->prm, nu_of(c->prm, 1), c->prm.cheb_lower_coarse, c->omc_pre, c->omc_post);
+}
+
 // ---- the V-cycle ----
 
 inline long long lvl_dom(const mac_ctx* c, int l) { return c->L[l].ndom; }
@@ -571,7 +607,6 @@ void sweep(mac_ctx* c, int l, const TZ* zin, TZ* zout, int want_dot, double omeg
 template <typename TZ>
 int vcycle(mac_ctx* c, TZ** zout) {
   const int nl = (int)c->L.size();
-  const int nu = std::min(8, std::max(1, c->prm.nu));
   const double sz = sizeof(TZ);
   int nlv = nl;
   for (int l = 1; l < nl; ++l)
@@ -584,6 +619,8 @@ int vcycle(mac_ctx* c, TZ** zout) {
   // down
   for (int l = 0; l < bot; ++l) {
     Level& lv = c->L[l];
+    const int nu = nu_of(c->prm, l);
+    const double* opre = l ? c->omc_pre : c->om_pre;
     TZ ih[3] = {(TZ)lv.ih2[0], (TZ)lv.ih2[1], (TZ)lv.ih2[2]};
     TZ* z = (TZ*)lv.z;
     TZ* zt = (TZ*)lv.zt;
@@ -595,15 +632,15 @@ int vcycle(mac_ctx* c, TZ** zout) {
       TimedLaunch tl(c, l == 0 ? K_JACOBI2_0 : K_C_JACOBI2, nd * (sr + sz + 1));
       if (l == 0)
         k_jacobi2<double, TZ><<<grid_of(lv.nact), KTHREADS, 0, c->st>>>(lv.nact, lv.meta, lv.code, c->r, z, ih[0], ih[1],
-                                                                         ih[2], (TZ)c->om_pre[0], (TZ)c->om_pre[1], c->s);
+                                                                         ih[2], (TZ)opre[0], (TZ)opre[1], c->s);
       else
         k_jacobi2<TZ, TZ><<<grid_of(lv.nact), KTHREADS, 0, c->st>>>(lv.nact, lv.meta, lv.code, (const TZ*)lv.r, z, ih[0],
-                                                                     ih[1], ih[2], (TZ)c->om_pre[0], (TZ)c->om_pre[1], c->s);
+                                                                     ih[1], ih[2], (TZ)opre[0], (TZ)opre[1], c->s);
     } else {
-      sweep<TZ>(c, l, nullptr, z, 0, c->om_pre[0]);
+      sweep<TZ>(c, l, nullptr, z, 0, opre[0]);
     }
     for (int k = 2; k < nu; ++k) {
-      sweep<TZ>(c, l, z, zt, 0, c->om_pre[k]);
+      sweep<TZ>(c, l, z, zt, 0, opre[k]);
       std::swap(z, zt);
     }
     {
@@ -641,9 +678,11 @@ int vcycle(mac_ctx* c, TZ** zout) {
     } else {
       TZ* z = (TZ*)lb.z;
       TZ* zt = (TZ*)lb.zt;
+      const int nu = nu_of(c->prm, bot);
+      const double* opre = bot ? c->omc_pre : c->om_pre;
+      const double* opost = bot ? c->omc_post : c->om_post;
       for (int k = 0; k < 2 * nu; ++k) {
-        sweep<TZ>(c, bot, k ? z : nullptr, k ? zt : z, (k == 2 * nu - 1) ? want_dot : 0,
-                  k < nu ? c->om_pre[k] : c->om_post[k - nu]);
+        sweep<TZ>(c, bot, k ? z : nullptr, k ? zt : z, (k == 2 * nu - 1) ? want_dot : 0, k < nu ? opre[k] : opost[k - nu]);
         if (k) std::swap(z, zt);
       }
       zfin[bot] = z;
@@ -660,8 +699,10 @@ int vcycle(mac_ctx* c, TZ** zout) {
       TimedLaunch tl(c, l == 0 ? K_PROLONG0 : K_C_PROLONG, nd * (2 * sz + 1) + (double)cl.ndom * sz);
       k_prolong_col<TZ><<<grid_of(lv.nact), KTHREADS, 0, c->st>>>(lv.nact, lv.pmeta, lv.code, zfin[l + 1], z, c->s);
     }
+    const int nu = nu_of(c->prm, l);
+    const double* opost = l ? c->omc_post : c->om_post;
     for (int k = 0; k < nu; ++k) {
-      sweep<TZ>(c, l, z, zt, (l == 0 && k == nu - 1) ? 1 : 0, c->om_post[k]);
+      sweep<TZ>(c, l, z, zt, (l == 0 && k == nu - 1) ? 1 : 0, opost[k]);
       std::swap(z, zt);
     }
     zfin[l] = z;
@@ -840,31 +881,6 @@ int pcg(mac_ctx* c, bool use_guess, mac_stats* st) {
   }
 }
 
-int check_params(const mac_params* p) {
-  if (p->nu < 1 || p->nu > 8 || p->max_iter < 1 || !(p->rtol > 0)) return fail(MAC_E_ARG, "bad solver parameters");
-  if (p->smoother != MAC_SMOOTH_JACOBI && p->smoother != MAC_SMOOTH_CHEBYSHEV) return fail(MAC_E_ARG, "unknown smoother");
-  if (p->smoother == MAC_SMOOTH_CHEBYSHEV && !(p->cheb_lower > 0 && p->cheb_lower < 2))
-    return fail(MAC_E_ARG, "cheb_lower must lie in (0, 2)");
-  if (p->smoother == MAC_SMOOTH_JACOBI && !(p->omega > 0 && p->omega < 2)) return fail(MAC_E_ARG, "omega must lie in (0, 2)");
-  return 0;
-}
-
-// Sweep weights from the parameters: nu copies of omega, or the reciprocals
-// of the Chebyshev nodes of [a, 2] (the polynomial of degree nu with value 1
-// at 0 that is smallest on [a, 2]); the sweeps after the coarse correction
-// take them in reverse order.
-void set_schedule(mac_ctx* c) {
-  const int nu = std::min(8, std::max(1, c->prm.nu));
-  const double a = c->prm.cheb_lower, b = 2.0;
-  for (int k = 0; k < nu; ++k) {
-    double om = c->prm.omega;
-    if (c->prm.smoother == MAC_SMOOTH_CHEBYSHEV)
-      om = 1.0 / (0.5 * (b + a) + 0.5 * (b - a) * std::cos((2.0 * k + 1.0) * M_PI / (2.0 * nu)));
-    c->om_pre[k] = om;
-    c->om_post[nu - 1 - k] = om;
-  }
-}
-
 int ensure(mac_ctx* c) {
   if (!c) return fail(MAC_E_ARG, "null context");
   if (!c->have_grid) return fail(MAC_E_STATE, "no grid uploaded");
@@ -892,6 +908,8 @@ void mac_default_params(mac_params* p) {
   p->stall_iters = 0;
   p->smoother = MAC_SMOOTH_CHEBYSHEV;
   p->cheb_lower = 0.25;
+  p->nu_coarse = 4;
+  p->cheb_lower_coarse = 0.1;
 }
 
 int mac_create(mac_ctx** out, const mac_params* prm) {

@@ -36,6 +36,8 @@ class Params(ctypes.Structure):
         ("stall_iters", ctypes.c_int),
         ("smoother", ctypes.c_int),
         ("cheb_lower", ctypes.c_double),
+        ("nu_coarse", ctypes.c_int),
+        ("cheb_lower_coarse", ctypes.c_double),
     ]
 
 

@@ -379,7 +379,9 @@ struct MG {
   int nc = 0;
   std::vector<long long> cellof;
   std::vector<double> inv;
-  double om_pre[8] = {0}, om_post[8] = {0};  // per-sweep weights (the device's schedule)
+  double om_pre[8] = {0}, om_post[8] = {0};    // per-sweep weights on level 0 (the device's schedule)
+  double omc_pre[8] = {0}, omc_post[8] = {0};  // on the coarse levels
+  int nuc = 2;
   double omega = 0.8;                        // weight of the sweep being run
   int nu = 2;
 
@@ -497,10 +499,12 @@ struct MG {
     }
     for (int l = 0; l < nl - 1; ++l) {
       DLevel& lv = L[l];
-      omega = om_pre[0];
+      const double* pre = l ? omc_pre : om_pre;
+      const int n = l ? nuc : nu;
+      omega = pre[0];
       jacobi(lv, lv.r, nullptr, lv.z);
-      for (int k = 1; k < nu; ++k) {
-        omega = om_pre[k];
+      for (int k = 1; k < n; ++k) {
+        omega = pre[k];
         jacobi(lv, lv.r, &lv.z, lv.zt);
         std::swap(lv.z, lv.zt);
       }
@@ -511,8 +515,10 @@ struct MG {
     for (int l = nl - 2; l >= 0; --l) {
       DLevel& lv = L[l];
       prolong(lv, L[l + 1], lv.z);
-      for (int k = 0; k < nu; ++k) {
-        omega = om_post[k];
+      const double* post = l ? omc_post : om_post;
+      const int n = l ? nuc : nu;
+      for (int k = 0; k < n; ++k) {
+        omega = post[k];
         jacobi(lv, lv.r, &lv.z, lv.zt);
         std::swap(lv.z, lv.zt);
       }
@@ -531,7 +537,7 @@ extern "C" {
 // relative residual, out[4] = levels.  Returns 0, or -1 on non-convergence.
 int orc_mgpcg(int nb, const int32_t* keys, const uint8_t* flags, const double* h, const double* u, const double* v,
               const double* w, double* p, double rtol, int maxit, int nu, double omega, int smoother, double cheb_lower,
-              int nthreads, double* out) {
+              int nu_coarse, double cheb_lower_coarse, int nthreads, double* out) {
   if (nthreads > 0) omp_set_num_threads(nthreads);
   auto t0 = std::chrono::steady_clock::now();
   double inv_h[3] = {1.0 / h[0], 1.0 / h[1], 1.0 / h[2]};
@@ -544,14 +550,19 @@ int orc_mgpcg(int nb, const int32_t* keys, const uint8_t* flags, const double* h
     }
   MG mg;
   mg.nu = nu = std::min(8, std::max(1, nu));
+  mg.nuc = nu_coarse > 0 ? std::min(8, nu_coarse) : nu;
   // weights: omega for every sweep, or the reciprocal Chebyshev nodes of
-  // [cheb_lower, 2] (mac_amd.h, enum mac_smoother), reversed after the coarse solve
-  for (int k = 0; k < nu; ++k) {
-    double om = omega;
-    if (smoother == 1) om = 1.0 / (0.5 * (2.0 + cheb_lower) + 0.5 * (2.0 - cheb_lower) * std::cos((2.0 * k + 1.0) * M_PI / (2.0 * nu)));
-    mg.om_pre[k] = om;
-    mg.om_post[nu - 1 - k] = om;
-  }
+  // [a, 2] (mac_amd.h, enum mac_smoother), reversed after the coarse solve
+  auto sched = [&](int n, double a, double* pre, double* post) {
+    for (int k = 0; k < n; ++k) {
+      double om = omega;
+      if (smoother == 1) om = 1.0 / (0.5 * (2.0 + a) + 0.5 * (2.0 - a) * std::cos((2.0 * k + 1.0) * M_PI / (2.0 * n)));
+      pre[k] = om;
+      post[n - 1 - k] = om;
+    }
+  };
+  sched(mg.nu, cheb_lower, mg.om_pre, mg.om_post);
+  sched(mg.nuc, cheb_lower_coarse, mg.omc_pre, mg.omc_post);
   // level 0 dense over the brick bounding box
   {
     DLevel l0;

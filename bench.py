@@ -34,17 +34,21 @@ WEIGHTS = os.path.join(ROOT, "mac_amd", "weights", "unet_guess.npz")
 KERNEL_NAMES = {
     "divergence": "k_div", "cg_pupdate_matvec": "k_pmatvec<{z}>", "cg_update": "k_update",
     "jacobi2_l0": "k_jacobi2<double, {z}>", "jacobi_l0": "k_jacobi<double, {z}>", "residual_l0": "k_residual<double, {z}>",
-    "prolong_l0": "k_prolong_col<{z}> (level 0)", "restrict_l1": "k_restrict_sep<{z}> (level 1)",
-    "gradient_update": "k_gradient", "max_div": "k_maxdiv", "resid_init": "k_resid0", "finalize": "k_finalize",
+    "prolong_l0": "k_prolong_col<{z}> (level 0)", "restrict_l1": "k_restrict_sep<{z}> (into level 1)",
+    "coarse_jacobi2": "k_jacobi2<{z}, {z}> (levels 1-4)", "coarse_jacobi": "k_jacobi<{z}, {z}> (levels 1-4)",
+    "coarse_residual": "k_residual<{z}, {z}> (levels 1-4)", "coarse_restrict": "k_restrict_sep<{z}> (into levels 2-5)",
+    "coarse_prolong": "k_prolong_col<{z}> (levels 1-4)",
+    "gradient_update": "k_gradient", "max_div": "k_maxdiv", "resid_init": "k_resid0",
 }
-
 
 # HBM bytes per launch measured by two counter runs (FETCH_SIZE x 2.0, WRITE_SIZE
 # x 1.0, scaled by a 256 MiB copy) of this benchmark's default input (256^3,
-# seed 0): profiles/projection_256_mixed_hbm_traffic.md
-MEASURED_TRAFFIC = {("k_jacobi<double, float>", 256, 0): 125.7e6, ("k_pmatvec<float>", 256, 0): 249.7e6,
-                    ("k_update", 256, 0): 253.7e6, ("k_jacobi2<double, float>", 256, 0): 134.8e6,
-                    ("k_residual<double, float>", 256, 0): 125.6e6}
+# seed 0, mixed method): profiles/projection_256_mixed_hbm_traffic.md.  The
+# coarse classes are the launch-weighted mean over levels 1-4.
+MEASURED_TRAFFIC = {("jacobi_l0", 256, 0): 121.04e6, ("cg_pupdate_matvec", 256, 0): 249.72e6, ("cg_update", 256, 0): 253.66e6,
+                    ("jacobi2_l0", 256, 0): 129.79e6, ("residual_l0", 256, 0): 120.93e6, ("prolong_l0", 256, 0): 60.06e6,
+                    ("restrict_l1", 256, 0): 78.41e6, ("coarse_jacobi", 256, 0): 4.89e6, ("coarse_jacobi2", 256, 0): 4.05e6,
+                    ("coarse_residual", 256, 0): 4.87e6}
 
 
 def log(*a):
@@ -98,6 +102,65 @@ def copy_calibration(nbytes=1 << 28):
     log(f"copy calibration: {nbytes} bytes read, {nbytes} bytes written")
 
 
+def jet_run(args):
+    """The whole time step on the 32x32x64 vertical jet (h 0.01 m, 4x4 nozzle at
+    1 m/s downward, solid floor, open sides, 8 jittered particles per inflow
+    cell, RK2, CFL 0.5, FP64 projection to 1e-12): device ms per step over
+    `jet_steps` steps, and the oracle's state after `jet_cpu_steps` steps
+    against the device's (velocities relative, cell flags bit-exact)."""
+    from mac_amd import jets as J
+    from mac_amd.lib import Sim
+    jet = J.vertical_jet(32, 32, 64, h=0.01, nozzle=4, inflow=1.0)
+    kw = dict(jet["params"], layer_max=2, integrator=0, particle_rk=2, seed=1)
+
+    def gpu_sim():
+        s = Sim(**kw, proj={"rtol": 1e-12})
+        s.upload(jet["keys"], jet["flags"], u=jet["u"], v=jet["v"], w=jet["w"])
+        return s
+
+    res = {"workload": "vertical jet 32x32x64, h=0.01, nozzle 4x4 at 1.0 m/s, RK2, CFL 0.5, rtol 1e-12"}
+    s = gpu_sim()
+    iters, parts, t0 = [], {k: 0.0 for k in ("ms_topology", "ms_forces", "ms_project", "ms_particles")}, time.perf_counter()
+    for _ in range(args.jet_steps):
+        st = s.step()
+        iters.append(sum(st["pcg_iters"]))
+        for k in parts:
+            parts[k] += st[k] / args.jet_steps
+    wall = time.perf_counter() - t0
+    res.update({"steps": args.jet_steps, "gpu_ms_per_step": wall * 1e3 / args.jet_steps, "final_bricks": st["nbricks"],
+                "final_particles": st["nparticles"], "final_fluid_cells": st["nfluid"],
+                "pcg_iters_per_step": float(np.mean(iters)), "max_div_after": st["max_div"], "gpu_ms_breakdown": parts})
+    s.close()
+    log(f"jet: {res['gpu_ms_per_step']:.2f} ms/step over {args.jet_steps} steps, {res['final_particles']} particles")
+    if args.no_cpu or args.jet_cpu_steps <= 0:
+        return res
+    from oracle import oracle as O
+    n = min(args.jet_cpu_steps, args.jet_steps)
+    s = gpu_sim()
+    t0 = time.perf_counter()
+    for _ in range(n):
+        s.step()
+    res["gpu_ms_per_step_first_cpu_steps"] = (time.perf_counter() - t0) * 1e3 / n
+    dg = s.download()
+    s.close()
+    o = O.Sim(**kw)
+    o.upload(jet["keys"], jet["flags"], u=jet["u"], v=jet["v"], w=jet["w"])
+    t0 = time.perf_counter()
+    for _ in range(n):
+        o.step()
+    cpu = time.perf_counter() - t0
+    do = o.download()
+    same = dg["keys"].shape == do["keys"].shape and np.array_equal(dg["keys"], do["keys"]) and np.array_equal(dg["flags"], do["flags"])
+    res.update({"cpu_steps": n, "cpu_ms_per_step": cpu * 1e3 / n, "cpu_cores": 1, "cells_bit_exact_vs_oracle": bool(same)})
+    if same:
+        ex = (dg["flags"] & 0x80) > 0
+        num = sum(np.linalg.norm(dg[k][ex] - do[k][ex]) ** 2 for k in ("u", "v", "w")) ** 0.5
+        den = sum(np.linalg.norm(do[k][ex]) ** 2 for k in ("u", "v", "w")) ** 0.5
+        res["u_rel_diff_vs_oracle"] = float(num / max(den, 1e-300))
+    log(f"jet oracle: {res['cpu_ms_per_step']:.1f} ms/step over {n} steps, cells bit-exact {same}")
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -112,6 +175,8 @@ def main():
     ap.add_argument("--only", choices=["plain", "mixed", "learned"], default=None, help="time one method only (profiling)")
     ap.add_argument("--weights", default=WEIGHTS)
     ap.add_argument("--copy-calibration", action="store_true", help="run one 256 MiB device copy first")
+    ap.add_argument("--jet-steps", type=int, default=100, help="time steps of the 32x32x64 jet (0: skip)")
+    ap.add_argument("--jet-cpu-steps", type=int, default=20, help="steps of the jet run on the oracle for parity/baseline")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only: --gpus {args.gpus} is not supported (the projection runs on a "
@@ -229,23 +294,36 @@ def main():
         for name in ("mixed", "learned"):
             if name in result:
                 out[f"speedup_{name}_vs_plain"] = result["plain"]["ms_per_solve"] / result[name]["ms_per_solve"]
-    # roofline of the solve's dominant stencil kernel (largest total event time
-    # among the single-kernel level-0 classes) for the mixed solve
+    # roofline of the kernel with the largest share of the mixed solve's
+    # event-timed kernel time, of every kernel, and of the whole solve
     src = result.get("mixed") or chosen
     if "_kt" in src:
         kt = src["_kt"]
         z = "double" if src is result.get("plain") else "float"
-        cands = [k for k in kt if kt[k]["launches"] > 0 and k in KERNEL_NAMES and k != "finalize"]
+        cands = [k for k in kt if kt[k]["launches"] > 0 and k in KERNEL_NAMES]
+        tot_ms = sum(kt[k]["ms"] for k in kt)
         dom = max(cands, key=lambda k: kt[k]["ms"])
         v = kt[dom]
         per_launch_bytes = v["bytes"] / v["launches"]
         avg_s = v["ms"] / v["launches"] * 1e-3
         ach = per_launch_bytes / avg_s / 1e9
-        kname = KERNEL_NAMES[dom].format(z=z)
         out["roofline"] = {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                           "frac": ach / HBM_PEAK_GBS, "traffic": MEASURED_TRAFFIC.get((kname, n, args.seed)),
-                           "kernel": kname, "timing_class": dom,
+                           "frac": ach / HBM_PEAK_GBS, "traffic": MEASURED_TRAFFIC.get((dom, n, args.seed)),
+                           "kernel": KERNEL_NAMES[dom].format(z=z), "timing_class": dom,
+                           "share_of_kernel_time": v["ms"] / tot_ms,
                            "bytes_per_launch": per_launch_bytes, "avg_launch_us": avg_s * 1e6}
+        out["roofline_by_kernel"] = {
+            KERNEL_NAMES[k].format(z=z): {"GBps": kt[k]["bytes"] / (kt[k]["ms"] * 1e-3) / 1e9,
+                                          "frac": kt[k]["bytes"] / (kt[k]["ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                                          "share": kt[k]["ms"] / tot_ms,
+                                          "traffic_per_launch": MEASURED_TRAFFIC.get((k, n, args.seed))}
+            for k in cands}
+        solve_bytes = sum(kt[k]["bytes"] for k in kt) / 2  # two profiled solves
+        out["roofline_solve"] = {"algorithmic_bytes_per_solve": solve_bytes,
+                                 "GBps": solve_bytes / (src["ms_per_solve"] * 1e-3) / 1e9,
+                                 "frac": solve_bytes / (src["ms_per_solve"] * 1e-3) / 1e9 / HBM_PEAK_GBS}
+    if args.jet_steps > 0:
+        out["jet_32x32x64"] = jet_run(args)
     if not args.no_cpu:
         from oracle import oracle as O
         threads = args.cpu_threads or min(os.cpu_count() or 1, 16)

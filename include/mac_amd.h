@@ -70,10 +70,11 @@ typedef struct {
   int precision;   /* enum mac_precision                                           */
   int nu;          /* damped-Jacobi sweeps before and after each coarse correction */
   double omega;    /* Jacobi damping                                               */
-  int check_every; /* with kernel timing on: host polls the device convergence flag
-                      every k iterations; otherwise iterations run as captured
-                      HIP graphs of two and the host polls when the residual
-                      decay predicts convergence                                  */
+  int check_every; /* host polls the device convergence flag every k iterations;
+                      from the third solve on the same grid (kernel timing off)
+                      the iterations run as captured HIP graphs of two instead
+                      and the host polls when the residual decay predicts
+                      convergence                                                 */
   int stall_iters; /* guided solve (guess = 1): restart from zero if not converged
                       after this many iterations (0: never; checked after every
                       pair of iterations when they run as graphs); a guess whose

@@ -87,6 +87,7 @@ struct mac_ctx {
   double omc_pre[8], omc_post[8];  // the same on the coarse levels
   // two CG iterations captured as one graph, per V-cycle precision (0 FP64, 1 FP32)
   hipGraphExec_t gexec[2] = {nullptr, nullptr};
+  int solves_on_grid = 0;  // projections since the last build (graphs pay off from the third)
   const void* gz[2] = {nullptr, nullptr};  // the V-cycle output the captured pair starts from
   hipStream_t st = nullptr;
   int dev = 0;
@@ -140,6 +141,7 @@ void drop_graphs(mac_ctx* c) {
 
 void free_all(mac_ctx* c) {
   drop_graphs(c);
+  c->solves_on_grid = 0;
   for (void* p : c->allocs) hipFree(p);
   c->allocs.clear();
   c->L.clear();
@@ -764,7 +766,10 @@ int pcg(mac_ctx* c, bool use_guess, mac_stats* st) {
     // (the search directions alternate between d[0] and d[1]), and the host
     // polls when the residual decay so far predicts convergence.
     const int gi = std::is_same<TZ, float>::value ? 1 : 0;
-    if (!c->timing && !c->hs->done) {
+    // A grid that is rebuilt every step or two (the time-step loop) would
+    // pay a capture per solve: graphs start with the third solve on a grid.
+    const bool graphed = !c->timing && !c->hs->done && c->solves_on_grid >= 2;
+    if (graphed) {
       if (!c->gexec[gi] || c->gz[gi] != (const void*)z) {
         if (c->gexec[gi]) hipGraphExecDestroy(c->gexec[gi]);
         c->gexec[gi] = nullptr;
@@ -818,7 +823,7 @@ int pcg(mac_ctx* c, bool use_guess, mac_stats* st) {
         iter_before = c->hs->iter;
       }
     }
-    while (c->timing && !c->hs->done) {
+    while (!graphed && !c->hs->done) {
       // A batch of `check` iterations; once the update kernel of one of them
       // sets `done`, every later kernel of the batch returns at once.
       std::vector<long long> useq;
@@ -1055,6 +1060,7 @@ int mac_project(mac_ctx* c, int guess, mac_stats* st) {
   if (l0.nact > 0) {
     if (c->prm.precision == MAC_MIXED) rc = pcg<float>(c, guess != 0, st);
     else rc = pcg<double>(c, guess != 0, st);
+    c->solves_on_grid += 1;
   } else {
     HIPCHK(hipMemsetAsync(c->x, 0, (size_t)c->nb0 * MAC_BC * sizeof(double), c->st));
   }

@@ -100,17 +100,42 @@ int repack(mac_unet* u) {
   return 0;
 }
 
+inline int ilog2(int v) {
+  int l = 0;
+  while ((1 << l) < v) ++l;
+  return l;
+}
+
+// k_conv3 over a volume: NB = C_out / 16 column blocks, ROWS output rows per
+// wave (4 when the height allows it).
+void launch_conv(mac_unet* u, const bf16* in, int in_cs, int in_co, int cin, Vol vol, const bf16* wt, int kpad, const float* bias,
+                 bf16* out, int out_cs, int out_co, int cout, int relu) {
+  const int rows = (vol.H % 4 == 0) ? 4 : 1;
+  const int waves = vol.D * (vol.H / rows) * (vol.W / 16);
+  const int blocks = (waves + 3) / 4;
+  const int lc = ilog2(cin);
+#define CONV(NB, R) \
+  k_conv3<NB, R><<<blocks, 256, 0, u->st>>>(in, in_cs, in_co, lc, vol, wt, kpad, bias, out, out_cs, out_co, relu, 0)
+  if (rows == 4) {
+    switch (cout / 16) {
+      case 1: CONV(1, 4); break;
+      case 2: CONV(2, 4); break;
+      default: CONV(4, 4); break;
+    }
+  } else {
+    switch (cout / 16) {
+      case 1: CONV(1, 1); break;
+      case 2: CONV(2, 1); break;
+      default: CONV(4, 1); break;
+    }
+  }
+#undef CONV
+}
+
 void conv_fwd(mac_unet* u, int l, const bf16* in, int in_cs, int in_co, Vol vol, bf16* out, int out_cs, int out_co, int relu) {
   Conv& c = u->conv[l];
-  const int waves = vol.D * vol.H * (vol.W / 16);
-  const int blocks = (waves + 3) / 4;
-  const float* bias = u->params + c.b_off;
   if (u->time_convs) hipEventRecord(u->ev0, u->st);
-  switch (c.Cout / 16) {
-    case 1: k_conv3<1><<<blocks, 256, 0, u->st>>>(in, in_cs, in_co, c.Cin, vol, c.wf, c.Kpad, bias, out, out_cs, out_co, relu, 0); break;
-    case 2: k_conv3<2><<<blocks, 256, 0, u->st>>>(in, in_cs, in_co, c.Cin, vol, c.wf, c.Kpad, bias, out, out_cs, out_co, relu, 0); break;
-    default: k_conv3<4><<<blocks, 256, 0, u->st>>>(in, in_cs, in_co, c.Cin, vol, c.wf, c.Kpad, bias, out, out_cs, out_co, relu, 0); break;
-  }
+  launch_conv(u, in, in_cs, in_co, c.Cin, vol, c.wf, c.Kpad, u->params + c.b_off, out, out_cs, out_co, c.Cout, relu);
   if (u->time_convs) {
     hipEventRecord(u->ev1, u->st);
     hipEventSynchronize(u->ev1);
@@ -124,13 +149,7 @@ void conv_fwd(mac_unet* u, int l, const bf16* in, int in_cs, int in_co, Vol vol,
 // data gradient of conv layer l: dX = conv(G, mirrored transposed weights)
 void conv_bwd_data(mac_unet* u, int l, const bf16* g, int g_cs, int g_co, Vol vol, bf16* dx, int dx_cs, int dx_co) {
   Conv& c = u->conv[l];
-  const int waves = vol.D * vol.H * (vol.W / 16);
-  const int blocks = (waves + 3) / 4;
-  switch (c.Cin / 16) {
-    case 1: k_conv3<1><<<blocks, 256, 0, u->st>>>(g, g_cs, g_co, c.Cout, vol, c.wb, c.Kpad2, nullptr, dx, dx_cs, dx_co, 0, 0); break;
-    case 2: k_conv3<2><<<blocks, 256, 0, u->st>>>(g, g_cs, g_co, c.Cout, vol, c.wb, c.Kpad2, nullptr, dx, dx_cs, dx_co, 0, 0); break;
-    default: k_conv3<4><<<blocks, 256, 0, u->st>>>(g, g_cs, g_co, c.Cout, vol, c.wb, c.Kpad2, nullptr, dx, dx_cs, dx_co, 0, 0); break;
-  }
+  launch_conv(u, g, g_cs, g_co, c.Cout, vol, c.wb, c.Kpad2, nullptr, dx, dx_cs, dx_co, c.Cin, 0);
 }
 
 int conv_wgrad(mac_unet* u, int l, const bf16* g, int g_cs, int g_co, const bf16* x, int x_cs, int x_co, Vol vol) {

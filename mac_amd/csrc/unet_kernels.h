@@ -18,60 +18,72 @@ __device__ inline bf16 f2bf(float v) { return (bf16)v; }
 
 // 3x3x3 convolution, zero padding 1, stride 1, as an implicit GEMM on the
 // BF16 matrix cores (v_mfma_f32_16x16x32_bf16, FP32 accumulate):
-//   M = output voxels (one wave: 16 consecutive x), N = Cout (NB blocks of 16),
-//   K = 27 * Cin ordered (tap, cin), tap = 9 (dz+1) + 3 (dy+1) + (dx+1).
+//   M = output voxels (one wave: 16 consecutive x in each of ROWS consecutive
+//   y rows), N = Cout (NB blocks of 16), K = 27 * Cin ordered (tap, cin),
+//   tap = 9 (dz+1) + 3 (dy+1) + (dx+1).
 // Lane l holds A row l % 16 and the 8 consecutive K of group l / 16 (one tap,
 // 8 channels: one 16-byte load of the channels-last input), and B column
 // l % 16 with the same 8 K (one 16-byte load of the [Cout][Kpad] weights).
+// Each B fragment feeds ROWS matrix-core operations and each A fragment NB.
 // Epilogue: + bias, optional ReLU, optional accumulation into the output.
-// Cin, the channel strides and offsets are multiples of 8; W a multiple of 16.
-template <int NB>
-__global__ void __launch_bounds__(256) k_conv3(const bf16* __restrict__ in, int in_cs, int in_co, int Cin, Vol v,
+// Cin = 1 << lcin with Cin >= 8; channel strides and offsets multiples of 8;
+// W a multiple of 16; H a multiple of ROWS.
+template <int NB, int ROWS>
+__global__ void __launch_bounds__(256) k_conv3(const bf16* __restrict__ in, int in_cs, int in_co, int lcin, Vol v,
                                                const bf16* __restrict__ wt, int Kpad, const float* __restrict__ bias,
                                                bf16* __restrict__ out, int out_cs, int out_co, int relu, int accum) {
   const int lane = threadIdx.x & 63;
   const int wave = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-  const int nxw = v.W >> 4;
-  if (wave >= v.D * v.H * nxw) return;
-  const int xw = wave % nxw, y = (wave / nxw) % v.H, z = wave / (nxw * v.H);
+  const int nxw = v.W >> 4, nyw = v.H / ROWS;
+  if (wave >= v.D * nyw * nxw) return;
+  const int xw = wave % nxw, y0 = ((wave / nxw) % nyw) * ROWS, z = wave / (nxw * nyw);
   const int x0 = xw << 4;
   const int i = lane & 15, g = lane >> 4;
-  floatx4 acc[NB];
+  const int Cin = 1 << lcin;
+  floatx4 acc[ROWS][NB];
 #pragma unroll
-  for (int nb = 0; nb < NB; ++nb) acc[nb] = floatx4{0.f, 0.f, 0.f, 0.f};
+  for (int r = 0; r < ROWS; ++r)
+#pragma unroll
+    for (int nb = 0; nb < NB; ++nb) acc[r][nb] = floatx4{0.f, 0.f, 0.f, 0.f};
   const int K = 27 * Cin;
+  const bf16* wrow = wt + (size_t)i * Kpad;
   for (int kc = 0; kc < Kpad; kc += 32) {
     const int k = kc + 8 * g;
-    bf16x8 a;
+    bf16x8 bfr[NB];
 #pragma unroll
-    for (int j = 0; j < 8; ++j) a[j] = (bf16)0.0f;
-    if (k < K) {
-      const int tap = k / Cin, ci = k - tap * Cin;
-      const int dz = tap / 9 - 1, dy = (tap / 3) % 3 - 1, dx = tap % 3 - 1;
-      const int xx = x0 + i + dx, yy = y + dy, zz = z + dz;
-      if (xx >= 0 && xx < v.W && yy >= 0 && yy < v.H && zz >= 0 && zz < v.D)
-        a = *(const bf16x8*)(in + ((size_t)(zz * v.H + yy) * v.W + xx) * in_cs + in_co + ci);
+    for (int nb = 0; nb < NB; ++nb) bfr[nb] = *(const bf16x8*)(wrow + (size_t)16 * nb * Kpad + k);
+    const int tap = k >> lcin, ci = k & (Cin - 1);
+    const int dz = tap / 9 - 1, dy = (tap / 3) % 3 - 1, dx = tap % 3 - 1;
+    const int xx = x0 + i + dx, zz = z + dz;
+    const bool okxz = (k < K) && xx >= 0 && xx < v.W && zz >= 0 && zz < v.D;
+    const bf16* base = in + ((size_t)zz * v.H * v.W + xx) * in_cs + in_co + ci;
+#pragma unroll
+    for (int r = 0; r < ROWS; ++r) {
+      const int yy = y0 + r + dy;
+      bf16x8 a;
+#pragma unroll
+      for (int j = 0; j < 8; ++j) a[j] = (bf16)0.0f;
+      if (okxz && yy >= 0 && yy < v.H) a = *(const bf16x8*)(base + (size_t)yy * v.W * in_cs);
+#pragma unroll
+      for (int nb = 0; nb < NB; ++nb) acc[r][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bfr[nb], acc[r][nb], 0, 0, 0);
     }
+  }
+#pragma unroll
+  for (int r = 0; r < ROWS; ++r)
 #pragma unroll
     for (int nb = 0; nb < NB; ++nb) {
-      const bf16x8 b = *(const bf16x8*)(wt + (size_t)(16 * nb + i) * Kpad + k);
-      acc[nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[nb], 0, 0, 0);
-    }
-  }
+      const int co = 16 * nb + i;
+      const float bv = bias ? bias[co] : 0.f;
 #pragma unroll
-  for (int nb = 0; nb < NB; ++nb) {
-    const int co = 16 * nb + i;
-    const float bv = bias ? bias[co] : 0.f;
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const int row = 4 * g + j;
-      bf16* o = out + ((size_t)(z * v.H + y) * v.W + x0 + row) * out_cs + out_co + co;
-      float val = acc[nb][j] + bv;
-      if (relu) val = val > 0.f ? val : 0.f;
-      if (accum) val += bf2f(*o);
-      *o = f2bf(val);
+      for (int j = 0; j < 4; ++j) {
+        const int row = 4 * g + j;
+        bf16* o = out + ((size_t)(z * v.H + y0 + r) * v.W + x0 + row) * out_cs + out_co + co;
+        float val = acc[r][nb][j] + bv;
+        if (relu) val = val > 0.f ? val : 0.f;
+        if (accum) val += bf2f(*o);
+        *o = f2bf(val);
+      }
     }
-  }
 }
 
 // Weight gradient of a 3x3x3 convolution, FP32:

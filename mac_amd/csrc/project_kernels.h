@@ -52,6 +52,19 @@ __device__ __forceinline__ int wave_in_block() { return threadIdx.x >> 6; }
   for (int i_ = blockIdx.x * WPB + wave_in_block(); i_ < (n); i_ += gridDim.x * WPB)         \
     if (const int* M = (meta) + (size_t)MSTRIDE * __builtin_amdgcn_readfirstlane(i_); true)
 
+// The CG convergence flag read in the same round trip as the wave's first
+// brick id (both scalar loads issue before one wait), so the early exit after
+// convergence adds no latency to a live launch.  The flag is uniform, so every
+// wave of a block takes the same branch.  Brick ids are >= 0; the test on b
+// only keeps its load ahead of the branch.
+// Launches always have n >= 1.
+__device__ __forceinline__ bool converged(const int* __restrict__ meta, int stride, int n, int i, const Scal* s) {
+  const int j = __builtin_amdgcn_readfirstlane(max(0, min(i, n - 1)));
+  const int b = meta[(size_t)stride * j];
+  const int d = s->done;
+  return (b < -1) | (d != 0);
+}
+
 // Walk the 8 cells of a lane's column with the 6 neighbour values of each.
 // col holds the lane's own values; ld(brick, cell) fetches a value from a
 // neighbour brick (already composed, e.g. z + beta d); f(z, xm, xp, ym, yp, zm, zp).
@@ -384,7 +397,7 @@ __global__ void __launch_bounds__(KTHREADS) k_pmatvec(int nact, const int* __res
                                                       const double* __restrict__ dold, double* __restrict__ dnew,
                                                       double* __restrict__ q, double ihx2, double ihy2, double ihz2,
                                                       double* __restrict__ partials, const Scal* s) {
-  if (s->done) return;
+  if (converged(meta, MSTRIDE, nact, blockIdx.x * WPB + wave_in_block(), s)) return;
   __shared__ double sh[16];
   const double beta = s->beta;
   const int lane = lane_id();
@@ -420,7 +433,7 @@ __global__ void __launch_bounds__(KTHREADS) k_update(int nact, const int* __rest
                                                      double* __restrict__ x, double* __restrict__ r,
                                                      const double* __restrict__ d, const double* __restrict__ q,
                                                      double* __restrict__ partials, const Scal* s) {
-  if (s->done) return;
+  if (converged(meta, MSTRIDE, nact, blockIdx.x * WPB + wave_in_block(), s)) return;
   __shared__ double sh[16];
   const double alpha = s->alpha;
   const int lane = lane_id();
@@ -450,7 +463,7 @@ __global__ void __launch_bounds__(KTHREADS) k_jacobi(int nact, const int* __rest
                                                      const TZ* __restrict__ zin, TZ* __restrict__ zout, TZ ihx2, TZ ihy2,
                                                      TZ ihz2, TZ omega, int want_dot, double* __restrict__ partials,
                                                      const Scal* s) {
-  if (s->done) return;
+  if (converged(meta, MSTRIDE, nact, blockIdx.x * WPB + wave_in_block(), s)) return;
   __shared__ double sh[16];
   const int lane = lane_id();
   double acc = 0.0;
@@ -501,7 +514,7 @@ __global__ void __launch_bounds__(KTHREADS) k_jacobi2(int nact, const int* __res
                                                       const uint8_t* __restrict__ code, const TR* __restrict__ r,
                                                       TZ* __restrict__ zout, TZ ihx2, TZ ihy2, TZ ihz2, TZ omega,
                                                       TZ omega2, const Scal* s) {
-  if (s->done) return;
+  if (converged(meta, MSTRIDE, nact, blockIdx.x * WPB + wave_in_block(), s)) return;
   const int lane = lane_id();
   FOR_EACH_BRICK(meta, nact, M) {
     const int b = M[0];
@@ -548,7 +561,7 @@ __global__ void __launch_bounds__(KTHREADS) k_jacobi2(int nact, const int* __res
 template <typename T>
 __global__ void __launch_bounds__(KTHREADS) k_restrict_sep(const int* __restrict__ rmeta, const uint8_t* __restrict__ codec,
                                                            const T* __restrict__ resf, T* __restrict__ rc, const Scal* s) {
-  if (s->done) return;
+  if (converged(rmeta, RSTRIDE, (int)gridDim.x, (int)blockIdx.x, s)) return;
   constexpr int NF = 10 * 18 * 18;  // [fz][fy][fx], fine x,y in -1..16, 10 fine z per half
   constexpr int NT1 = 10 * 18 * 8;  // [fz][fy][X]
   __shared__ int fb[64];
@@ -617,7 +630,7 @@ template <typename T>
 __global__ void __launch_bounds__(KTHREADS) k_prolong_col(int nact, const int* __restrict__ pmeta,
                                                           const uint8_t* __restrict__ codef, const T* __restrict__ zc,
                                                           T* __restrict__ zf, const Scal* s) {
-  if (s->done) return;
+  if (converged(pmeta, PSTRIDE, nact, blockIdx.x * WPB + wave_in_block(), s)) return;
   const int lane = lane_id();
   const int x = lane & 7, y = lane >> 3;
   for (int i_ = blockIdx.x * WPB + wave_in_block(); i_ < nact; i_ += gridDim.x * WPB) {
@@ -666,7 +679,7 @@ __global__ void __launch_bounds__(KTHREADS) k_residual(int nact, const int* __re
                                                        const uint8_t* __restrict__ code, const TR* __restrict__ r,
                                                        const TZ* __restrict__ zv, TZ* __restrict__ res, TZ ihx2, TZ ihy2,
                                                        TZ ihz2, const Scal* s) {
-  if (s->done) return;
+  if (converged(meta, MSTRIDE, nact, blockIdx.x * WPB + wave_in_block(), s)) return;
   const int lane = lane_id();
   FOR_EACH_BRICK(meta, nact, M) {
     const int b = M[0];

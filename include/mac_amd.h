@@ -256,6 +256,12 @@ int mac_sim_move_particles(mac_sim* sim, double dt);
 int mac_sim_interpolate(mac_sim* sim, int n, const double* pos, double* vel);
 /* One whole time step: topology, dt, advance, move particles. */
 int mac_sim_step(mac_sim* sim, mac_sim_stats* stats);
+/* Algorithms_Extrapolate::Do (Algorithms_Extrapolate.h:25-235) on speed slot
+ * `slot`: starting from the cells of layer 1 and moving out layer by layer,
+ * each face between two cells of layer >= L gets the mean of the parallel
+ * faces next to it that touch a cell of layer < L (non-forced: const faces
+ * keep their value; a mean over no face is 0/0 = NaN, as in the reference). */
+int mac_sim_extrapolate(mac_sim* sim, int slot);
 
 /* ===================================================================== */
 /* Learned initial guess: 3-level 3-D U-Net (no counterpart in the        */

@@ -592,6 +592,44 @@ int mac_sim_interpolate(mac_sim* s, int n, const double* pos, double* vel) {
   return 0;
 }
 
+int mac_sim_extrapolate(mac_sim* s, int slot) {
+  if (!s || slot < 0 || slot > 1) return mac_internal_set_error(MAC_E_ARG, "bad arguments");
+  if (s->nb == 0) return 0;
+  const size_t cells = (size_t)s->nb * MAC_BC;
+  uint8_t* fr = nullptr;
+  int* dany = nullptr;
+  SCHK(hipMalloc(&fr, 2 * cells));
+  if (hipMalloc(&dany, sizeof(int)) != hipSuccess) {
+    hipFree(fr);
+    return mac_internal_set_error(MAC_E_HIP, "hipMalloc failed");
+  }
+  uint8_t* front = fr;
+  uint8_t* next = fr + cells;
+  int rc = 0;
+  k_extrap_seed<<<(int)std::min<size_t>((cells + 255) / 256, 4096), 256, 0, s->st>>>(s->nb, s->flags, s->layer, front);
+  // one frontier step per layer; layers are int8, so at most 127 steps
+  for (int lay = 1; lay < 128 && rc == 0; ++lay) {
+    int any = 0;
+    if (hipMemsetAsync(next, 0, cells, s->st) != hipSuccess || hipMemsetAsync(dany, 0, sizeof(int), s->st) != hipSuccess) {
+      rc = mac_internal_set_error(MAC_E_HIP, "hipMemsetAsync failed");
+      break;
+    }
+    k_extrapolate<<<s->nb, 256, 0, s->st>>>(s->nb, s->nbr, s->flags, s->layer, s->S[slot][0], s->S[slot][1], s->S[slot][2], front, next,
+                                            dany, lay);
+    if (hipGetLastError() != hipSuccess || hipMemcpyAsync(&any, dany, sizeof(int), hipMemcpyDeviceToHost, s->st) != hipSuccess ||
+        hipStreamSynchronize(s->st) != hipSuccess) {
+      rc = mac_internal_set_error(MAC_E_HIP, "extrapolation launch failed");
+      break;
+    }
+    if (!any) break;
+    std::swap(front, next);
+  }
+  hipStreamSynchronize(s->st);
+  hipFree(fr);
+  hipFree(dany);
+  return rc;
+}
+
 int mac_sim_set_guess(mac_sim* s, int mode, mac_unet* net, float out_scale) {
   if (!s || mode < 0 || mode > 2) return mac_internal_set_error(MAC_E_ARG, "bad arguments");
   if (mode == 2 && (!net || !(out_scale > 0.f))) return mac_internal_set_error(MAC_E_ARG, "learned guess needs a network and out_scale > 0");

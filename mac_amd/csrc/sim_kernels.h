@@ -593,3 +593,153 @@ __global__ void k_move_particles(int np, const double* __restrict__ px, const do
   qy[o] = res[1];
   qz[o] = res[2];
 }
+
+// ------------------------------------------------------ extrapolation ----
+// Algorithms_Extrapolate::Do (Algorithms_Extrapolate.h:25-235), one frontier
+// step per launch.  A cell is "ok" when it exists and its layer is not empty.
+// For every frontier cell A (layer pass `lay`) and direction i:
+//   +1: B = A+e_i ok with layer >= lay: face A|B (stored at B) := mean of the
+//       faces parallel to it that touch a cell of layer < lay; B joins the
+//       next frontier when its layer > lay;
+//   -1: C = A-e_i ok with layer >= lay: face C|A (stored at A), likewise;
+//       C missing or empty: A's -e_i face from its +e_i and transverse
+//       neighbours.
+// Non-forced Set (const faces keep their value); a mean over no face is 0/0.
+// Within one pass the faces written join two cells of layer >= lay and the
+// faces read touch a cell of layer < lay, so threads never read what others
+// write; a face reached from both of its cells gets the same sum in the same
+// order from each, so the duplicate writes agree bit for bit.
+struct ExCell {
+  long long c;  // slot, -1 when the brick is missing
+  int lay;      // layer; -2 when not ok
+};
+
+__device__ inline ExCell ex_cell(const uint8_t* flags, const int8_t* layer, const int* nbr27, int b, int x, int y, int z) {
+  int cell;
+  const int nb = locate(nbr27, b, x, y, z, cell);
+  ExCell e{-1, -2};
+  if (nb < 0) return e;
+  e.c = (long long)nb * MAC_BC + cell;
+  if ((flags[e.c] & F_EXISTS) && layer[e.c] != -1) e.lay = layer[e.c];
+  return e;
+}
+
+__global__ void k_extrap_seed(int nb, const uint8_t* __restrict__ flags, const int8_t* __restrict__ layer, uint8_t* __restrict__ front) {
+  for (long long c = blockIdx.x * (long long)blockDim.x + threadIdx.x; c < (long long)nb * MAC_BC; c += (long long)gridDim.x * blockDim.x)
+    front[c] = ((flags[c] & F_EXISTS) && layer[c] == 1) ? 1 : 0;
+}
+
+__global__ void k_extrapolate(int nb, const int* __restrict__ nbr27, const uint8_t* __restrict__ flags, const int8_t* __restrict__ layer,
+                              double* u, double* v, double* w, const uint8_t* __restrict__ front, uint8_t* __restrict__ next,
+                              int* __restrict__ any_next, int lay) {
+#pragma clang fp contract(off)
+  const int b = blockIdx.x;
+  double* U[3] = {u, v, w};
+  for (int t = threadIdx.x; t < MAC_BC; t += blockDim.x) {
+    const size_t ca = (size_t)b * MAC_BC + t;
+    if (!front[ca]) continue;
+    const int x = t & 7, y = (t >> 3) & 7, z = t >> 6;
+    auto at = [&](int dx, int dy, int dz) { return ex_cell(flags, layer, nbr27, b, x + dx, y + dy, z + dz); };
+    auto set = [&](size_t c, int i, double val) {
+      if (!(flags[c] & (0x08u << i))) U[i][c] = val;
+    };
+    for (int i = 0; i < 3; ++i) {
+      const int ex = (i == 0), ey = (i == 1), ez = (i == 2);
+      // +1 direction
+      const ExCell B = at(ex, ey, ez);
+      if (B.lay >= lay) {
+        if (B.lay > lay) {
+          next[B.c] = 1;
+          *any_next = 1;
+        }
+        int n = 0;
+        double val = 0.0;
+        for (int j = 0; j < 3; ++j)
+          for (int s2 = -1; s2 <= 1; s2 += 2) {
+            const int jx = s2 * (j == 0), jy = s2 * (j == 1), jz = s2 * (j == 2);
+            if (s2 == -1 && j == i) {
+              const ExCell C = at(-ex, -ey, -ez);
+              if (C.lay != -2 && C.lay < lay) {
+                ++n;
+                val += U[i][ca];
+              }
+            } else if (s2 == 1 && j == i) {
+              const ExCell D = at(2 * ex, 2 * ey, 2 * ez);
+              if (D.lay != -2 && D.lay < lay) {
+                ++n;
+                val += U[i][D.c];
+              }
+            } else {
+              const ExCell N3 = at(ex + jx, ey + jy, ez + jz), N4 = at(jx, jy, jz);
+              if (N3.lay != -2 && N4.lay != -2 && (N3.lay < lay || N4.lay < lay)) {
+                ++n;
+                val += U[i][N3.c];
+              }
+            }
+          }
+        set(B.c, i, val / n);
+      }
+      // -1 direction
+      const ExCell C = at(-ex, -ey, -ez);
+      if (C.lay != -2) {
+        if (C.lay >= lay) {
+          if (C.lay > lay) {
+            next[C.c] = 1;
+            *any_next = 1;
+          }
+          int n = 0;
+          double val = 0.0;
+          for (int j = 0; j < 3; ++j)
+            for (int s2 = -1; s2 <= 1; s2 += 2) {
+              const int jx = s2 * (j == 0), jy = s2 * (j == 1), jz = s2 * (j == 2);
+              if (s2 == 1 && j == i) {
+                const ExCell N3 = at(ex, ey, ez);
+                if (N3.lay != -2 && N3.lay < lay) {
+                  ++n;
+                  val += U[j][N3.c];
+                }
+              } else if (s2 == -1 && j == i) {
+                const ExCell N3 = at(-2 * ex, -2 * ey, -2 * ez);
+                if (N3.lay != -2 && N3.lay < lay) {
+                  ++n;
+                  val += U[j][C.c];
+                }
+              } else {
+                const ExCell N3 = at(-ex + jx, -ey + jy, -ez + jz), N4 = at(jx, jy, jz);
+                if (N3.lay != -2 && N4.lay != -2 && (N3.lay < lay || N4.lay < lay)) {
+                  ++n;
+                  val += U[i][N4.c];
+                }
+              }
+            }
+          set(ca, i, val / n);
+        }
+      } else {
+        int n = 0;
+        double val = 0.0;
+        for (int j = 0; j < 3; ++j)
+          for (int s2 = -1; s2 <= 1; s2 += 2) {
+            const int jx = s2 * (j == 0), jy = s2 * (j == 1), jz = s2 * (j == 2);
+            if (s2 == 1 && j == i) {
+              const ExCell N3 = at(ex, ey, ez);
+              if (N3.lay != -2 && N3.lay < lay) {
+                ++n;
+                val += U[j][N3.c];
+              }
+            } else if (s2 == -1 && j == i) {
+            } else {
+              const ExCell N4 = at(jx, jy, jz);
+              if (N4.lay != -2) {
+                const ExCell N3 = at(jx - ex, jy - ey, jz - ez);
+                if (N3.lay != -2 && (N3.lay < lay || N4.lay < lay)) {
+                  ++n;
+                  val += U[i][N4.c];
+                }
+              }
+            }
+          }
+        set(ca, i, val / n);
+      }
+    }
+  }
+}

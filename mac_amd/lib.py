@@ -338,6 +338,7 @@ SIGNATURES.update({
     "mac_sim_move_particles": (ctypes.c_int, [_P, ctypes.c_double]),
     "mac_sim_interpolate": (ctypes.c_int, [_P, ctypes.c_int, _P, _P]),
     "mac_sim_step": (ctypes.c_int, [_P, ctypes.POINTER(SimStats)]),
+    "mac_sim_extrapolate": (ctypes.c_int, [_P, ctypes.c_int]),
 })
 
 
@@ -436,6 +437,10 @@ class Sim:
         st = SimStats()
         _check(self.lib.mac_sim_step(self.ptr, ctypes.byref(st)))
         return st.as_dict()
+
+    def extrapolate(self, slot=0):
+        """Algorithms_Extrapolate::Do on speed slot `slot`."""
+        _check(self.lib.mac_sim_extrapolate(self.ptr, int(slot)))
 
     def set_guess(self, mode, net=None, out_scale=1.0):
         """Projection starting iterate: GUESS_ZERO, GUESS_PREVIOUS or GUESS_LEARNED (with net)."""

@@ -193,6 +193,7 @@ class Sim:
             lib.orc_sim_advance.argtypes = [_P, ctypes.c_double, _P]
             lib.orc_sim_move.argtypes = [_P, ctypes.c_double]
             lib.orc_sim_interpolate.argtypes = [_P, ctypes.c_int, _P, _P]
+            lib.orc_sim_extrapolate.argtypes = [_P, ctypes.c_int]
             lib.orc_sim_step.restype = ctypes.c_double
             lib.orc_sim_step.argtypes = [_P, _P]
             lib._sim_bound = True
@@ -255,6 +256,9 @@ class Sim:
         vel = np.zeros_like(pos)
         self.lib.orc_sim_interpolate(self.ptr, pos.shape[0], _p(pos), _p(vel))
         return vel
+
+    def extrapolate(self, slot=0):
+        self.lib.orc_sim_extrapolate(self.ptr, slot)
 
     def step(self):
         md = ctypes.c_double()

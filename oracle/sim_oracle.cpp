@@ -224,6 +224,127 @@ struct Sim {
     return dt;
   }
 
+  // Algorithms_Extrapolate::Do (Algorithms_Extrapolate.h:25-235), in its own
+  // control flow: a frontier that starts at every cell of layer 1; for each
+  // frontier cell A and direction i, the face between A and A+e_i (and A-e_i
+  // and A) is set, when both cells have layer >= lay, to the mean of the
+  // neighbouring parallel faces that touch a cell of layer < lay (and, when
+  // A-e_i is missing or empty, A's own -e_i face from its +e_i and transverse
+  // neighbours); cells with layer > lay reached that way form the next
+  // frontier.  Non-forced Set: const faces keep their value.  A mean over no
+  // face is 0/0, as in the reference.
+  void extrapolate(int slot) {
+    auto ok = [&](int x, int y, int z) { return valid(x, y, z) && layer_of(x, y, z) != -1; };
+    auto U = [&](int comp, int x, int y, int z) { return chunk(x, y, z)->S[slot][comp][off_of(x, y, z)]; };
+    auto set = [&](int x, int y, int z, int comp, double val) {
+      Chunk* ch = chunk(x, y, z);
+      const int o = off_of(x, y, z);
+      if (!(ch->flags[o] & (0x08u << comp))) ch->S[slot][comp][o] = val;
+    };
+    std::vector<K3> cur;
+    for_cells([&](Chunk* c, int t, int x, int y, int z) {
+      if (c->layer[t] == 1) cur.push_back({x, y, z});
+    });
+    int lay = 1;
+    while (true) {
+      std::map<K3, bool> next;
+      for (const K3& A : cur) {
+        const int ax = A.x, ay = A.y, az = A.z;
+        for (int i = 0; i < 3; ++i) {
+          const int ex = (i == 0), ey = (i == 1), ez = (i == 2);
+          // +1 direction
+          const int bx = ax + ex, by = ay + ey, bz = az + ez;
+          if (ok(bx, by, bz) && layer_of(bx, by, bz) >= lay) {
+            if (layer_of(bx, by, bz) > lay) next[{bx, by, bz}] = true;
+            int n = 0;
+            double val = 0;
+            for (int j = 0; j < 3; ++j)
+              for (int s2 = -1; s2 <= 1; s2 += 2) {
+                const int jx = s2 * (j == 0), jy = s2 * (j == 1), jz = s2 * (j == 2);
+                if (s2 == -1 && j == i) {
+                  if (ok(ax - ex, ay - ey, az - ez) && layer_of(ax - ex, ay - ey, az - ez) < lay) {
+                    ++n;
+                    val += U(i, ax, ay, az);
+                  }
+                } else if (s2 == 1 && j == i) {
+                  if (ok(bx + ex, by + ey, bz + ez) && layer_of(bx + ex, by + ey, bz + ez) < lay) {
+                    ++n;
+                    val += U(i, bx + ex, by + ey, bz + ez);
+                  }
+                } else {
+                  const int x3 = bx + jx, y3 = by + jy, z3 = bz + jz, x4 = ax + jx, y4 = ay + jy, z4 = az + jz;
+                  if (ok(x3, y3, z3) && ok(x4, y4, z4) && (layer_of(x3, y3, z3) < lay || layer_of(x4, y4, z4) < lay)) {
+                    ++n;
+                    val += U(i, x3, y3, z3);
+                  }
+                }
+              }
+            set(bx, by, bz, i, val / n);
+          }
+          // -1 direction
+          const int cx = ax - ex, cy = ay - ey, cz = az - ez;
+          if (ok(cx, cy, cz)) {
+            if (layer_of(cx, cy, cz) >= lay) {
+              if (layer_of(cx, cy, cz) > lay) next[{cx, cy, cz}] = true;
+              int n = 0;
+              double val = 0;
+              for (int j = 0; j < 3; ++j)
+                for (int s2 = -1; s2 <= 1; s2 += 2) {
+                  const int jx = s2 * (j == 0), jy = s2 * (j == 1), jz = s2 * (j == 2);
+                  if (s2 == 1 && j == i) {
+                    if (ok(ax + ex, ay + ey, az + ez) && layer_of(ax + ex, ay + ey, az + ez) < lay) {
+                      ++n;
+                      val += U(j, ax + ex, ay + ey, az + ez);
+                    }
+                  } else if (s2 == -1 && j == i) {
+                    if (ok(cx - ex, cy - ey, cz - ez) && layer_of(cx - ex, cy - ey, cz - ez) < lay) {
+                      ++n;
+                      val += U(j, cx, cy, cz);
+                    }
+                  } else {
+                    const int x3 = cx + jx, y3 = cy + jy, z3 = cz + jz, x4 = ax + jx, y4 = ay + jy, z4 = az + jz;
+                    if (ok(x3, y3, z3) && ok(x4, y4, z4) && (layer_of(x3, y3, z3) < lay || layer_of(x4, y4, z4) < lay)) {
+                      ++n;
+                      val += U(i, x4, y4, z4);
+                    }
+                  }
+                }
+              set(ax, ay, az, i, val / n);
+            }
+          } else {
+            int n = 0;
+            double val = 0;
+            for (int j = 0; j < 3; ++j)
+              for (int s2 = -1; s2 <= 1; s2 += 2) {
+                const int jx = s2 * (j == 0), jy = s2 * (j == 1), jz = s2 * (j == 2);
+                if (s2 == 1 && j == i) {
+                  if (ok(ax + ex, ay + ey, az + ez) && layer_of(ax + ex, ay + ey, az + ez) < lay) {
+                    ++n;
+                    val += U(j, ax + ex, ay + ey, az + ez);
+                  }
+                } else if (s2 == -1 && j == i) {
+                } else {
+                  const int x4 = ax + jx, y4 = ay + jy, z4 = az + jz;
+                  if (ok(x4, y4, z4)) {
+                    const int x3 = x4 - ex, y3 = y4 - ey, z3 = z4 - ez;
+                    if (ok(x3, y3, z3) && (layer_of(x3, y3, z3) < lay || layer_of(x4, y4, z4) < lay)) {
+                      ++n;
+                      val += U(i, x4, y4, z4);
+                    }
+                  }
+                }
+              }
+            set(ax, ay, az, i, val / n);
+          }
+        }
+      }
+      cur.clear();
+      for (auto& kv : next) cur.push_back(kv.first);
+      ++lay;
+      if (cur.empty()) break;
+    }
+  }
+
   void forces(int slot) {
     // Acceleration SetZero(true), then Algorithms<Gravity, Viscosity, Convection>
     for_cells([&](Chunk* c, int t, int, int, int) {
@@ -512,6 +633,7 @@ double orc_sim_compute_dt(void* p) { return ((Sim*)p)->compute_dt(); }
 void orc_sim_forces(void* p, int slot) { ((Sim*)p)->forces(slot); }
 void orc_sim_advance(void* p, double dt, double* maxdiv) { ((Sim*)p)->advance(dt, maxdiv); }
 void orc_sim_move(void* p, double dt) { ((Sim*)p)->move(dt); }
+void orc_sim_extrapolate(void* p, int slot) { ((Sim*)p)->extrapolate(slot); }
 void orc_sim_interpolate(void* p, int n, const double* pos, double* vel) {
   Sim* s = (Sim*)p;
   for (int k = 0; k < n; ++k) {

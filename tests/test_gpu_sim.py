@@ -120,3 +120,45 @@ def test_jet_steps_vs_oracle(integrator, prk):
             assert np.linalg.norm(dg[k][ex] - ref) <= 1e-8 * max(np.linalg.norm(ref), 1.0), (step, k)
         same_particles(dg["particles"], do["particles"], 1e-9)
         assert sg["max_div"] < 1e-7
+
+
+def _same_speeds(dg, do):
+    for k in ("u", "v", "w"):
+        assert np.array_equal(dg[k], do[k], equal_nan=True), (k, np.nanmax(np.abs(dg[k] - do[k])))
+
+
+@pytest.mark.parametrize("seed", [5, 6])
+def test_extrapolate_bitexact_random_layers(seed):
+    """Algorithms_Extrapolate on an arbitrary state (layers not from a BFS):
+    the device's frontier steps reproduce the oracle's sets bit for bit."""
+    g, layer = random_state(seed, (3, 3, 3))
+    pair = both()
+    upload_both(g, pair, layer=layer)
+    for s in pair:
+        s.extrapolate()
+    dg, do = pair[0].download(), pair[1].download()
+    same_cells(dg, do)
+    _same_speeds(dg, do)
+
+
+def test_extrapolate_bitexact_after_jet_steps():
+    jet = J.vertical_jet(16, 16, 24, h=0.01, nozzle=4, inflow=1.0)
+    kw = dict(jet["params"], layer_max=3, seed=9)
+    pair = both(**kw)
+    upload_both(jet, pair)
+    for _ in range(5):
+        for s in pair:
+            s.step()
+    for s in pair:
+        s.topology()
+        s.extrapolate()
+    dg, do = pair[0].download(), pair[1].download()
+    same_cells(dg, do)
+    ex = (dg["flags"] & W.EXISTS) > 0
+    for k in ("u", "v", "w"):
+        # the velocities before extrapolation agree to the solver tolerance, and
+        # extrapolation only averages them
+        a, b = dg[k][ex], do[k][ex]
+        fin = np.isfinite(b)
+        assert np.array_equal(np.isfinite(a), fin), k
+        assert np.linalg.norm(a[fin] - b[fin]) <= 1e-8 * max(1.0, np.linalg.norm(b[fin])), k

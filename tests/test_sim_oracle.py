@@ -193,3 +193,59 @@ def test_particles_in_uniform_flow_move_exactly(prk):
     s.move_particles(dt)
     out = s.download()["particles"]
     assert np.allclose(out, pos + dt * np.array(vel), rtol=0, atol=1e-14)
+
+
+def _jet_state(steps=4):
+    jet = J.vertical_jet(16, 16, 24, h=0.01, nozzle=4, inflow=1.0)
+    s = O.Sim(**jet["params"], layer_max=3, seed=5)
+    s.upload(jet["keys"], jet["flags"], u=jet["u"], v=jet["v"], w=jet["w"])
+    for _ in range(steps):
+        s.step()
+    s.topology()
+    return s
+
+
+def test_extrapolation_of_uniform_field_is_uniform():
+    """Algorithms_Extrapolate: every extrapolated face is a mean of faces that
+    already hold the value, so a uniform field stays uniform (non-const faces
+    of every cell with a layer), and fluid-fluid faces are never written."""
+    s = _jet_state()
+    d = s.download()
+    ex = (d["flags"] & W.EXISTS) > 0
+    vals = (0.25, -1.5, 0.75)
+    n = d["flags"].size
+    s.upload(d["keys"], d["flags"], d["layer"], *[np.where(ex, vals[i], 0.0) for i in range(3)])
+    s.extrapolate()
+    e = s.download()
+    for i, k in enumerate(("u", "v", "w")):
+        touched = ex & (d["layer"] >= 0)
+        got = e[k][touched]
+        assert np.all((got == vals[i]) | np.isnan(got)), k
+        assert np.isfinite(got).mean() > 0.99
+    del n
+
+
+def test_extrapolation_leaves_lower_layers_and_is_idempotent():
+    """Faces between two cells of layer 0 are read, never written; running the
+    extrapolation twice gives the same field (it only reads faces that touch a
+    lower layer, which it does not write)."""
+    s = _jet_state()
+    d = s.download()
+    rng = np.random.default_rng(3)
+    ex = (d["flags"] & W.EXISTS) > 0
+    comps = [np.where(ex, rng.normal(size=d["flags"].size), 0.0) for _ in range(3)]
+    s.upload(d["keys"], d["flags"], d["layer"], *comps)
+    s.extrapolate()
+    e1 = s.download()
+    s.extrapolate()
+    e2 = s.download()
+    for k in ("u", "v", "w"):
+        assert np.array_equal(e1[k], e2[k], equal_nan=True), k
+    # the -x face of a layer-0 cell whose -x neighbour is layer 0 is untouched
+    keys = d["keys"]
+    lay = d["layer"].reshape(-1, 512)
+    t = np.arange(512)
+    for b in range(keys.shape[0]):
+        for tt in t[(lay[b] == 0) & ((t & 7) > 0)]:
+            if lay[b, tt - 1] == 0:
+                assert e1["u"][b * 512 + tt] == comps[0][b * 512 + tt]

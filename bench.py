@@ -102,24 +102,44 @@ def copy_calibration(nbytes=1 << 28):
     log(f"copy calibration: {nbytes} bytes read, {nbytes} bytes written")
 
 
-def jet_run(args):
-    """The whole time step on the 32x32x64 vertical jet (h 0.01 m, 4x4 nozzle at
-    1 m/s downward, solid floor, open sides, 8 jittered particles per inflow
-    cell, RK2, CFL 0.5, FP64 projection to 1e-12): device ms per step over
-    `jet_steps` steps, and the oracle's state after `jet_cpu_steps` steps
-    against the device's (velocities relative, cell flags bit-exact)."""
-    from mac_amd import jets as J
+def _sim(jet, kw, proj):
     from mac_amd.lib import Sim
+    s = Sim(**kw, proj=proj)
+    s.upload(jet["keys"], jet["flags"], u=jet["u"], v=jet["v"], w=jet["w"])
+    return s
+
+
+def _compare(dg, do):
+    """Cells bit-exact; relative velocity / pressure differences on cells."""
+    same = dg["keys"].shape == do["keys"].shape and np.array_equal(dg["keys"], do["keys"]) and np.array_equal(dg["flags"], do["flags"])
+    out = {"cells_bit_exact": bool(same)}
+    if same:
+        ex = (dg["flags"] & 0x80) > 0
+        num = sum(np.linalg.norm(dg[k][ex] - do[k][ex]) ** 2 for k in ("u", "v", "w")) ** 0.5
+        den = sum(np.linalg.norm(do[k][ex]) ** 2 for k in ("u", "v", "w")) ** 0.5
+        out["u_rel_diff"] = float(num / max(den, 1e-300))
+        out["p_rel_diff"] = float(np.linalg.norm(dg["p"][ex] - do["p"][ex]) / max(np.linalg.norm(do["p"][ex]), 1e-300))
+    return out
+
+
+def jet_runs(args):
+    """The whole time step on the two jet configurations, device-resident.
+
+    vertical 32x32x64: h 0.01 m, 4x4 nozzle at 1 m/s downward, solid floor,
+      open sides, 8 jittered particles per inflow cell, RK2, CFL 0.5, FP64
+      projection to 1e-12; device ms per step, and the oracle's state after
+      `jet_cpu_steps` steps against the device's.
+    lateral 192x96x96: 6x6 nozzle on the x = 0 wall at mid-height, inflow
+      U(0.5, 2.0) m/s from the seed, gravity -z, RK2-TVD, CFL 0.5; stepped
+      with the FP64 and with the mixed projection side by side, compared at
+      the end (cells bit-exact, velocity, pressure, max |div u|)."""
+    from mac_amd import jets as J
+    res = {}
+    # vertical
     jet = J.vertical_jet(32, 32, 64, h=0.01, nozzle=4, inflow=1.0)
     kw = dict(jet["params"], layer_max=2, integrator=0, particle_rk=2, seed=1)
-
-    def gpu_sim():
-        s = Sim(**kw, proj={"rtol": 1e-12})
-        s.upload(jet["keys"], jet["flags"], u=jet["u"], v=jet["v"], w=jet["w"])
-        return s
-
-    res = {"workload": "vertical jet 32x32x64, h=0.01, nozzle 4x4 at 1.0 m/s, RK2, CFL 0.5, rtol 1e-12"}
-    s = gpu_sim()
+    r = {"workload": "vertical jet 32x32x64, h=0.01, nozzle 4x4 at 1.0 m/s, RK2, CFL 0.5, FP64 rtol 1e-12"}
+    s = _sim(jet, kw, {"rtol": 1e-12})
     iters, parts, t0 = [], {k: 0.0 for k in ("ms_topology", "ms_forces", "ms_project", "ms_particles")}, time.perf_counter()
     for _ in range(args.jet_steps):
         st = s.step()
@@ -127,37 +147,53 @@ def jet_run(args):
         for k in parts:
             parts[k] += st[k] / args.jet_steps
     wall = time.perf_counter() - t0
-    res.update({"steps": args.jet_steps, "gpu_ms_per_step": wall * 1e3 / args.jet_steps, "final_bricks": st["nbricks"],
-                "final_particles": st["nparticles"], "final_fluid_cells": st["nfluid"],
-                "pcg_iters_per_step": float(np.mean(iters)), "max_div_after": st["max_div"], "gpu_ms_breakdown": parts})
+    r.update({"steps": args.jet_steps, "gpu_ms_per_step": wall * 1e3 / args.jet_steps, "final_bricks": st["nbricks"],
+              "final_particles": st["nparticles"], "final_fluid_cells": st["nfluid"],
+              "pcg_iters_per_step": float(np.mean(iters)), "max_div_after": st["max_div"], "gpu_ms_breakdown": parts})
     s.close()
-    log(f"jet: {res['gpu_ms_per_step']:.2f} ms/step over {args.jet_steps} steps, {res['final_particles']} particles")
-    if args.no_cpu or args.jet_cpu_steps <= 0:
-        return res
-    from oracle import oracle as O
-    n = min(args.jet_cpu_steps, args.jet_steps)
-    s = gpu_sim()
-    t0 = time.perf_counter()
-    for _ in range(n):
-        s.step()
-    res["gpu_ms_per_step_first_cpu_steps"] = (time.perf_counter() - t0) * 1e3 / n
-    dg = s.download()
-    s.close()
-    o = O.Sim(**kw)
-    o.upload(jet["keys"], jet["flags"], u=jet["u"], v=jet["v"], w=jet["w"])
-    t0 = time.perf_counter()
-    for _ in range(n):
-        o.step()
-    cpu = time.perf_counter() - t0
-    do = o.download()
-    same = dg["keys"].shape == do["keys"].shape and np.array_equal(dg["keys"], do["keys"]) and np.array_equal(dg["flags"], do["flags"])
-    res.update({"cpu_steps": n, "cpu_ms_per_step": cpu * 1e3 / n, "cpu_cores": 1, "cells_bit_exact_vs_oracle": bool(same)})
-    if same:
-        ex = (dg["flags"] & 0x80) > 0
-        num = sum(np.linalg.norm(dg[k][ex] - do[k][ex]) ** 2 for k in ("u", "v", "w")) ** 0.5
-        den = sum(np.linalg.norm(do[k][ex]) ** 2 for k in ("u", "v", "w")) ** 0.5
-        res["u_rel_diff_vs_oracle"] = float(num / max(den, 1e-300))
-    log(f"jet oracle: {res['cpu_ms_per_step']:.1f} ms/step over {n} steps, cells bit-exact {same}")
+    log(f"vertical jet: {r['gpu_ms_per_step']:.2f} ms/step over {args.jet_steps} steps, {r['final_particles']} particles")
+    if not args.no_cpu and args.jet_cpu_steps > 0:
+        from oracle import oracle as O
+        n = min(args.jet_cpu_steps, args.jet_steps)
+        s = _sim(jet, kw, {"rtol": 1e-12})
+        t0 = time.perf_counter()
+        for _ in range(n):
+            s.step()
+        r["gpu_ms_per_step_first_cpu_steps"] = (time.perf_counter() - t0) * 1e3 / n
+        dg = s.download()
+        s.close()
+        o = O.Sim(**kw)
+        o.upload(jet["keys"], jet["flags"], u=jet["u"], v=jet["v"], w=jet["w"])
+        t0 = time.perf_counter()
+        for _ in range(n):
+            o.step()
+        r.update({"cpu_steps": n, "cpu_ms_per_step": (time.perf_counter() - t0) * 1e3 / n, "cpu_cores": 1})
+        r["vs_oracle"] = _compare(dg, o.download())
+        log(f"vertical jet oracle: {r['cpu_ms_per_step']:.1f} ms/step over {n} steps, {r['vs_oracle']}")
+    res["jet_vertical_32x32x64"] = r
+    # lateral
+    if args.lateral_steps > 0:
+        from mac_amd.lib import MAC_FP64, MAC_MIXED
+        jet = J.lateral_jet(192, 96, 96, h=0.01, nozzle=6, seed=args.seed)
+        kw = dict(jet["params"], layer_max=2, integrator=1, particle_rk=2, seed=args.seed)
+        r = {"workload": f"lateral jet 192x96x96, h=0.01, nozzle 6x6 at {jet['inflow']:.3f} m/s, RK2-TVD, CFL 0.5, rtol 1e-8"}
+        sims = {"fp64": _sim(jet, kw, {"rtol": 1e-8, "precision": MAC_FP64}),
+                "mixed": _sim(jet, kw, {"rtol": 1e-8, "precision": MAC_MIXED})}
+        for name, s in sims.items():
+            t0, iters = time.perf_counter(), []
+            for _ in range(args.lateral_steps):
+                st = s.step()
+                iters.append(sum(st["pcg_iters"]))
+            r[f"{name}_ms_per_step"] = (time.perf_counter() - t0) * 1e3 / args.lateral_steps
+            r[f"{name}_pcg_iters_per_step"] = float(np.mean(iters))
+            r[f"{name}_max_div_after"] = st["max_div"]
+            r[f"{name}_final"] = {"bricks": st["nbricks"], "particles": st["nparticles"], "fluid_cells": st["nfluid"]}
+        r["steps"] = args.lateral_steps
+        r["mixed_vs_fp64"] = _compare(sims["mixed"].download(), sims["fp64"].download())
+        for s in sims.values():
+            s.close()
+        log(f"lateral jet: fp64 {r['fp64_ms_per_step']:.2f} / mixed {r['mixed_ms_per_step']:.2f} ms/step, {r['mixed_vs_fp64']}")
+        res["jet_lateral_192x96x96"] = r
     return res
 
 
@@ -177,6 +213,7 @@ def main():
     ap.add_argument("--copy-calibration", action="store_true", help="run one 256 MiB device copy first")
     ap.add_argument("--jet-steps", type=int, default=100, help="time steps of the 32x32x64 jet (0: skip)")
     ap.add_argument("--jet-cpu-steps", type=int, default=20, help="steps of the jet run on the oracle for parity/baseline")
+    ap.add_argument("--lateral-steps", type=int, default=500, help="time steps of the 192x96x96 lateral jet (0: skip)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only: --gpus {args.gpus} is not supported (the projection runs on a "
@@ -323,7 +360,7 @@ def main():
                                  "GBps": solve_bytes / (src["ms_per_solve"] * 1e-3) / 1e9,
                                  "frac": solve_bytes / (src["ms_per_solve"] * 1e-3) / 1e9 / HBM_PEAK_GBS}
     if args.jet_steps > 0:
-        out["jet_32x32x64"] = jet_run(args)
+        out.update(jet_runs(args))
     if not args.no_cpu:
         from oracle import oracle as O
         threads = args.cpu_threads or min(os.cpu_count() or 1, 16)

@@ -4,16 +4,18 @@ vertical_jet(): a box of nx x ny x nz cells (h per axis), open sides, solid
 floor and a square inflow nozzle centred in the top layer.  Only the floor
 and the nozzle exist at the start; fluid enters through the nozzle, whose
 empty cells are refilled with jittered particles every step.
+lateral_jet(): a horizontal nozzle on the x = 0 wall at mid-height, no floor.
 """
 import numpy as np
 
-from .workloads import AIR, CONST, EXISTS, INFLOW, SOLID
+from .workloads import AIR, CONST, EXISTS, INFLOW, SOLID, uniform
 
 NODELETE = 0x40
 
 
 def _bricks_of(cells):
-    """cells: dict (x,y,z) -> (flag, w).  Returns keys, flags, w arrays."""
+    """cells: dict (x,y,z) -> (flag, speed).  Returns keys, flags and the speed
+    component array."""
     bkeys = sorted({(x >> 3, y >> 3, z >> 3) for (x, y, z) in cells}, key=lambda k: (k[2], k[1], k[0]))
     index = {k: i for i, k in enumerate(bkeys)}
     nb = len(bkeys)
@@ -49,3 +51,25 @@ def vertical_jet(nx=32, ny=32, nz=64, h=0.01, nozzle=4, inflow=1.0, floor=True):
     params = dict(h=(h, h, h), gravity=(0.0, 0.0, -9.81), box_lo=(-0.5 * h, -0.5 * h, -0.5 * h),
                   box_hi=((nx - 0.5) * h, (ny - 0.5) * h, (nz - 0.5) * h))
     return {"keys": keys, "flags": flags, "u": np.zeros(n), "v": np.zeros(n), "w": w, "params": params}
+
+
+def lateral_jet(nx=192, ny=96, nz=96, h=0.01, nozzle=6, inflow=None, seed=0):
+    """Lateral jet: a horizontal square nozzle of `nozzle` x `nozzle` cells on
+    the x = 0 wall at mid-height, flowing along +x; no floor, open sides, so
+    particles leaving the box are deleted.  The inflow speed is uniform in
+    [0.5, 2.0] m/s from the seed unless given.  Returns the dict of
+    vertical_jet (params for a box that deletes particles leaving it)."""
+    if inflow is None:
+        inflow = 0.5 + 1.5 * float(uniform(seed, 7, np.zeros(1, dtype=np.uint64))[0])
+    cells = {}
+    y0, z0 = (ny - nozzle) // 2, (nz - nozzle) // 2
+    for y in range(y0, y0 + nozzle):
+        for z in range(z0, z0 + nozzle):
+            # inflow cell: its low x face carries the prescribed speed into the box
+            cells[(0, y, z)] = (INFLOW | EXISTS | NODELETE | CONST[0], float(inflow))
+    keys, flags, u = _bricks_of(cells)
+    n = flags.size
+    params = dict(h=(h, h, h), gravity=(0.0, 0.0, -9.81), box_lo=(-0.5 * h, -0.5 * h, -0.5 * h),
+                  box_hi=((nx - 0.5) * h, (ny - 0.5) * h, (nz - 0.5) * h))
+    return {"keys": keys, "flags": flags, "u": u, "v": np.zeros(n), "w": np.zeros(n), "params": params,
+            "inflow": float(inflow)}

@@ -207,6 +207,14 @@ typedef struct {
   double box_lo[3], box_hi[3];   /* particles leaving [lo, hi) are deleted (lo == hi: none) */
   int max_particles;
   int max_bricks;
+  int depth_interior;   /* 0: the cell-type update without depth (TestJetDEau_Basic.cpp:324-326);
+                           >= 1: the depth chain of test/Jet3.cpp:523-547 with this
+                           DepthInterior (Policy_Depth.h, Jet3: 3)                    */
+  int depth_remove;     /* DepthRemoveParticle of Policy_Depth (Jet3: 5)               */
+  int floor_axis;       /* Algorithm_DelParticle (test/Jet3.cpp:259-289): particles with
+                           position[floor_axis] < floor_value are erased at the start of
+                           the cell-type update; -1 = none (Jet3: axis y, value 0)    */
+  double floor_value;
   mac_params proj;      /* projection solver settings                                  */
 } mac_sim_params;
 
@@ -233,13 +241,26 @@ int mac_sim_counts(mac_sim* sim, int* nbricks, int* nparticles);
  * pointer may be NULL.  a* = acceleration slot, p = pressure. */
 int mac_sim_download(mac_sim* sim, int32_t* keys3, uint8_t* flags, int8_t* layer, double* u, double* v, double* w,
                      double* ax, double* ay, double* az, double* p, double* pxyz);
+/* Per-cell depth (int16, -1 = empty, Data_Grid_Depth_Empty.h), bricks in the
+ * order mac_sim_download gives them.  Only the depth chain reads and writes it;
+ * set it after mac_sim_upload to start from Interior cells. */
+int mac_sim_get_depth(mac_sim* sim, int16_t* depth);
+int mac_sim_set_depth(mac_sim* sim, const int16_t* depth);
 /* The cell-type/layer update of one step, in the order of the reference's
  * init chain (TestJetDEau_Basic.cpp:324-326): Algorithms_Initialize_MacCell,
  * Algorithms_Layer_Initial_With_Particle (particle -> key = lround(x/h),
  * Policy_Particle_To_Key.h:24-35), Algorithms_Create_Fluid_Particle (empty
  * inflow cells get particles), Algorithms_Update_CellType_Layer (air layers
  * 1..layer_max by BFS, then the +1-direction extra layer),
- * Algorithms_Delete_MacCell (layer-empty cells erased, empty bricks freed). */
+ * Algorithms_Delete_MacCell (layer-empty cells erased, empty bricks freed).
+ * With depth_interior >= 1 it is the depth chain of test/Jet3.cpp:545-547:
+ * Algorithms_Initialize_Layer_And_Depth, Algorithm_DelParticle (floor_axis),
+ * Algorithms_Layer_Initial_With_Particle_Depth (particles in Interior cells
+ * deeper than depth_remove are erased), Algorithms_Create_Fluid_Particle_Depth
+ * (empty inflow cells, and empty Interior cells no deeper than depth_remove,
+ * get particles), Algorithms_Update_CellType_Layer_Depth (surface depth 0, air
+ * layers, then the depth search inside the fluid: depth < depth_interior gives
+ * Fluid, deeper cells become Interior), Algorithms_Delete_MacCell. */
 int mac_sim_topology(mac_sim* sim);
 /* Algorithms_Calculate_Time_Step + Policy_CheckDT. */
 int mac_sim_compute_dt(mac_sim* sim, double* dt);

@@ -61,6 +61,7 @@ struct mac_sim {
   int* d_keys = nullptr;
   uint8_t* flags = nullptr;
   int8_t* layer = nullptr;
+  int16_t* depth = nullptr;  // Data_Grid_Depth_Empty: -1 = empty
   double* S[2][3] = {{nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr}};
   double* A[3] = {nullptr, nullptr, nullptr};
   double* P = nullptr;
@@ -74,6 +75,7 @@ struct mac_sim {
   int np = 0, capp = 0;
   // scratch
   int* d_int = nullptr;  // [0] particle count, [1] missing count, [2] overflow, [3] kept
+  int* d_any = nullptr;  // per-round change flags of the depth search
   int* d_missing = nullptr;
   int maxmissing = 0;
   int* d_bflag = nullptr;
@@ -93,6 +95,7 @@ void sim_free_bricks(mac_sim* s) {
   hipFree(s->d_keys);
   hipFree(s->flags);
   hipFree(s->layer);
+  hipFree(s->depth);
   for (int a = 0; a < 2; ++a)
     for (int i = 0; i < 3; ++i) hipFree(s->S[a][i]);
   for (int i = 0; i < 3; ++i) hipFree(s->A[i]);
@@ -125,12 +128,14 @@ int sim_set_bricks(mac_sim* s, const std::vector<Key3>& nk) {
   int* d_keys;
   uint8_t* flags;
   int8_t* layer;
+  int16_t* depth;
   double* S[2][3];
   double* A[3];
   double *P, *Pprev;
   SCHK(hipMalloc(&d_keys, sizeof(int) * 3 * std::max(nbn, 1)));
   SCHK(hipMalloc(&flags, cells));
   SCHK(hipMalloc(&layer, cells));
+  SCHK(hipMalloc(&depth, cells * sizeof(int16_t)));
   for (int a = 0; a < 2; ++a)
     for (int i = 0; i < 3; ++i) SCHK(hipMalloc(&S[a][i], cells * sizeof(double)));
   for (int i = 0; i < 3; ++i) SCHK(hipMalloc(&A[i], cells * sizeof(double)));
@@ -149,6 +154,7 @@ int sim_set_bricks(mac_sim* s, const std::vector<Key3>& nk) {
   if (nbn > 0) {
     k_relayout_u8<<<nbn, 256, 0, s->st>>>(nbn, d_from, s->flags, flags, (uint8_t)0);
     k_relayout_u8<<<nbn, 256, 0, s->st>>>(nbn, d_from, (const uint8_t*)s->layer, (uint8_t*)layer, (uint8_t)0xFF);
+    k_relayout_i16<<<nbn, 256, 0, s->st>>>(nbn, d_from, s->depth, depth, (int16_t)-1);
     for (int a = 0; a < 2; ++a)
       for (int i = 0; i < 3; ++i) k_relayout_f64<<<nbn, 256, 0, s->st>>>(nbn, d_from, s->S[a][i], S[a][i]);
     for (int i = 0; i < 3; ++i) k_relayout_f64<<<nbn, 256, 0, s->st>>>(nbn, d_from, s->A[i], A[i]);
@@ -160,6 +166,7 @@ int sim_set_bricks(mac_sim* s, const std::vector<Key3>& nk) {
   s->d_keys = d_keys;
   s->flags = flags;
   s->layer = layer;
+  s->depth = depth;
   for (int a = 0; a < 2; ++a)
     for (int i = 0; i < 3; ++i) s->S[a][i] = S[a][i];
   for (int i = 0; i < 3; ++i) s->A[i] = A[i];
@@ -192,22 +199,48 @@ int sim_alloc_particles(mac_sim* s, int cap) {
   return 0;
 }
 
+// Depth search of Algorithms_Update_CellType_Layer_Depth.h:96-133, one
+// level per launch.  Levels are launched four at a time before the host
+// reads their change flags: a level that changes nothing ends the search,
+// and the levels launched after it change nothing either.
+int sim_depth_search(mac_sim* s) {
+  const mac_sim_params& P = s->prm;
+  const int batch = 4;
+  for (int d0 = 1;; d0 += batch) {
+    if (d0 + batch > 32767) return mac_internal_set_error(MAC_E_ARG, "fluid deeper than 32766 cells");
+    SCHK(hipMemsetAsync(s->d_any, 0, batch * sizeof(int), s->st));
+    for (int k = 0; k < batch; ++k)
+      k_depth_round<<<s->nb, 256, 0, s->st>>>(s->nb, s->nbr, s->flags, s->layer, s->depth, d0 + k, P.depth_interior,
+                                               s->d_any + k);
+    SCHK(hipGetLastError());
+    int any[batch];
+    SCHK(hipMemcpyAsync(any, s->d_any, sizeof(any), hipMemcpyDeviceToHost, s->st));
+    SCHK(hipStreamSynchronize(s->st));
+    if (!any[batch - 1]) return 0;
+  }
+}
+
 int sim_topology(mac_sim* s) {
   const mac_sim_params& P = s->prm;
   SimIH ih{{1.0 / P.h[0], 1.0 / P.h[1], 1.0 / P.h[2]}};
-  // Algorithms_Initialize_MacCell
-  if (s->nb > 0) k_reset_layer<<<1024, 256, 0, s->st>>>((size_t)s->nb * MAC_BC, s->layer);
+  const bool dmode = P.depth_interior > 0;
+  // Algorithms_Initialize_MacCell / Algorithms_Initialize_Layer_And_Depth
+  if (s->nb > 0 && dmode)
+    k_reset_layer_depth<<<1024, 256, 0, s->st>>>((size_t)s->nb * MAC_BC, s->flags, s->layer, s->depth);
+  else if (s->nb > 0)
+    k_reset_layer<<<1024, 256, 0, s->st>>>((size_t)s->nb * MAC_BC, s->layer);
   // bricks needed this step: missing particle bricks, and the 27-neighbourhood
   // of every brick that will hold a layer-0 cell (particles, inflow); the air
   // layers reach at most layer_max + 1 <= 8 cells, so they stay inside it
   SCHK(hipMemsetAsync(s->d_int, 0, 4 * sizeof(int), s->st));
   if (s->nb > 0) {
     SCHK(hipMemsetAsync(s->d_bflag, 0, sizeof(int) * s->nb, s->st));
-    k_inflow_bricks<<<s->nb, 256, 0, s->st>>>(s->nb, s->flags, s->d_bflag);
+    k_inflow_bricks<<<s->nb, 256, 0, s->st>>>(s->nb, s->flags, s->d_bflag, dmode ? 1 : 0);
   }
   if (s->np > 0)
     k_particle_bricks<<<(s->np + 255) / 256, 256, 0, s->st>>>(s->np, s->px, s->py, s->pz, ih, s->hk, s->hv, s->hcap, s->d_bflag,
-                                                             s->d_missing, s->d_int + 1, s->maxmissing);
+                                                             s->d_missing, s->d_int + 1, s->maxmissing, P.floor_axis,
+                                                             P.floor_value);
   SCHK(hipGetLastError());
   int cnt[4];
   SCHK(hipMemcpyAsync(cnt, s->d_int, sizeof(cnt), hipMemcpyDeviceToHost, s->st));
@@ -241,20 +274,35 @@ int sim_topology(mac_sim* s) {
     std::vector<Key3> nk(have.begin(), have.end());
     SRC(sim_set_bricks(s, nk));
   }
-  // Algorithms_Layer_Initial_With_Particle
-  if (s->np > 0)
-    k_mark_particles<<<(s->np + 255) / 256, 256, 0, s->st>>>(s->np, s->px, s->py, s->pz, ih, s->hk, s->hv, s->hcap, s->flags,
-                                                            s->layer);
-  // Algorithms_Create_Fluid_Particle
-  SCHK(hipMemcpyAsync(s->d_int, &s->np, sizeof(int), hipMemcpyHostToDevice, s->st));
-  if (s->nb > 0 && P.particles_per_inflow_cell > 0)
+  // Algorithms_Layer_Initial_With_Particle, or Algorithm_DelParticle +
+  // Algorithms_Layer_Initial_With_Particle_Depth (particles erased: the kept
+  // ones are appended to the second buffer, whose count starts the fill)
+  const bool erase = dmode || P.floor_axis >= 0;
+  if (s->np > 0 && erase) {
+    SCHK(hipMemsetAsync(s->d_int, 0, sizeof(int), s->st));
+    k_mark_particles_keep<<<(s->np + 255) / 256, 256, 0, s->st>>>(s->np, s->px, s->py, s->pz, ih, s->hk, s->hv, s->hcap, s->flags,
+                                                                 s->layer, dmode ? s->depth : nullptr, P.depth_remove,
+                                                                 P.floor_axis, P.floor_value, s->qx, s->qy, s->qz, s->d_int);
+    std::swap(s->px, s->qx);
+    std::swap(s->py, s->qy);
+    std::swap(s->pz, s->qz);
+  } else {
+    if (s->np > 0)
+      k_mark_particles<<<(s->np + 255) / 256, 256, 0, s->st>>>(s->np, s->px, s->py, s->pz, ih, s->hk, s->hv, s->hcap, s->flags,
+                                                              s->layer);
+    SCHK(hipMemcpyAsync(s->d_int, &s->np, sizeof(int), hipMemcpyHostToDevice, s->st));
+  }
+  // Algorithms_Create_Fluid_Particle(_Depth)
+  if (s->nb > 0 && (P.particles_per_inflow_cell > 0 || dmode))
     k_inflow_fill<<<s->nb, 256, 0, s->st>>>(s->nb, s->d_keys, s->flags, s->layer, P.h[0], P.h[1], P.h[2],
                                              P.particles_per_inflow_cell, P.seed, s->tick, s->px, s->py, s->pz, s->d_int,
-                                             s->capp, s->d_int + 2);
-  // Algorithms_Update_CellType_Layer
+                                             s->capp, s->d_int + 2, dmode ? s->depth : nullptr, P.depth_remove);
+  // Algorithms_Update_CellType_Layer(_Depth): surface depth before the layers
+  if (s->nb > 0 && dmode) k_depth_surface<<<s->nb, 256, 0, s->st>>>(s->nb, s->nbr, s->flags, s->layer, s->depth);
   if (s->nb > 0)
     for (int lay = 1; lay <= P.layer_max; ++lay) k_layer_round<<<s->nb, 256, 0, s->st>>>(s->nb, s->nbr, s->flags, s->layer, lay);
   if (s->nb > 0) k_layer_extra<<<s->nb, 256, 0, s->st>>>(s->nb, s->nbr, s->flags, s->layer, P.layer_max);
+  if (s->nb > 0 && dmode) SRC(sim_depth_search(s));
   // Algorithms_Delete_MacCell
   if (s->nb > 0) k_delete_cells<<<s->nb, 256, 0, s->st>>>(s->nb, s->flags, s->layer, s->d_bflag);
   SCHK(hipGetLastError());
@@ -354,6 +402,10 @@ void mac_sim_default_params(mac_sim_params* p) {
   p->seed = 0;
   p->max_particles = 1 << 22;
   p->max_bricks = 1 << 16;
+  p->depth_interior = 0;
+  p->depth_remove = 0;
+  p->floor_axis = -1;
+  p->floor_value = 0.0;
   mac_default_params(&p->proj);
 }
 
@@ -361,6 +413,8 @@ int mac_sim_create(mac_sim** out, const mac_sim_params* p) {
   if (!out || !p) return mac_internal_set_error(MAC_E_ARG, "null argument");
   *out = nullptr;
   if (p->layer_max < 1 || p->layer_max > 7) return mac_internal_set_error(MAC_E_ARG, "layer_max must be in 1..7");
+  if (p->depth_interior < 0 || p->depth_interior > 32767) return mac_internal_set_error(MAC_E_ARG, "depth_interior must be in 0..32767");
+  if (p->floor_axis < -1 || p->floor_axis > 2) return mac_internal_set_error(MAC_E_ARG, "floor_axis must be -1, 0, 1 or 2");
   for (int i = 0; i < 3; ++i)
     if (!(p->h[i] > 0)) return mac_internal_set_error(MAC_E_ARG, "h must be positive");
   mac_sim* s = new mac_sim();
@@ -378,7 +432,7 @@ int mac_sim_create(mac_sim** out, const mac_sim_params* p) {
     return mac_internal_set_error(MAC_E_HIP, "hipStreamCreate failed");
   }
   s->maxmissing = 1 << 16;
-  if (hipMalloc(&s->d_int, 8 * sizeof(int)) != hipSuccess || hipMalloc(&s->d_missing, sizeof(int) * 3 * s->maxmissing) != hipSuccess ||
+  if (hipMalloc(&s->d_int, 8 * sizeof(int)) != hipSuccess || hipMalloc(&s->d_any, 64 * sizeof(int)) != hipSuccess || hipMalloc(&s->d_missing, sizeof(int) * 3 * s->maxmissing) != hipSuccess ||
       sim_alloc_particles(s, p->max_particles) != 0) {
     mac_sim_destroy(s);
     return mac_internal_set_error(MAC_E_HIP, "allocation failed");
@@ -393,6 +447,7 @@ void mac_sim_destroy(mac_sim* s) {
   sim_free_bricks(s);
   for (double* p : {s->px, s->py, s->pz, s->qx, s->qy, s->qz}) hipFree(p);
   hipFree(s->d_int);
+  hipFree(s->d_any);
   hipFree(s->d_missing);
   if (s->proj) mac_destroy(s->proj);
   if (s->st) hipStreamDestroy(s->st);
@@ -452,6 +507,18 @@ int mac_sim_upload(mac_sim* s, int nbricks, const int32_t* keys3, const uint8_t*
   return 0;
 }
 
+int mac_sim_get_depth(mac_sim* s, int16_t* depth) {
+  if (!s || (!depth && s->nb > 0)) return mac_internal_set_error(MAC_E_ARG, "bad arguments");
+  if (s->nb > 0) SCHK(hipMemcpyAsync(depth, s->depth, sizeof(int16_t) * (size_t)s->nb * MAC_BC, hipMemcpyDeviceToHost, s->st));
+  SCHK(hipStreamSynchronize(s->st));
+  return 0;
+}
+int mac_sim_set_depth(mac_sim* s, const int16_t* depth) {
+  if (!s || (!depth && s->nb > 0)) return mac_internal_set_error(MAC_E_ARG, "bad arguments");
+  if (s->nb > 0) SCHK(hipMemcpyAsync(s->depth, depth, sizeof(int16_t) * (size_t)s->nb * MAC_BC, hipMemcpyHostToDevice, s->st));
+  SCHK(hipStreamSynchronize(s->st));
+  return 0;
+}
 int mac_sim_counts(mac_sim* s, int* nbricks, int* nparticles) {
   if (!s) return mac_internal_set_error(MAC_E_ARG, "null sim");
   if (nbricks) *nbricks = s->nb;

@@ -50,6 +50,20 @@ __host__ __device__ inline unsigned mat_set_air(unsigned m) { return (m == 1u ||
 #define F_EXISTS 0x80u
 #define F_NODEL 0x40u
 
+// Slot of an appended element: one atomic per wave on *counter, lanes ranked
+// by lane id among the wave's keepers.  Every active lane of the wave must
+// call it; returns -1 where keep is false.
+__device__ inline int wave_append(bool keep, int* counter) {
+  const unsigned long long m = __ballot(keep);
+  if (m == 0ull) return -1;
+  const int lane = (int)__lane_id();
+  const int leader = __ffsll((unsigned long long)m) - 1;
+  int base = 0;
+  if (lane == leader) base = atomicAdd(counter, (int)__popcll(m));
+  base = __shfl(base, leader);
+  return keep ? base + (int)__popcll(m & ((1ull << lane) - 1ull)) : -1;
+}
+
 __device__ inline int dh_lookup(const uint64_t* __restrict__ hk, const int* __restrict__ hv, int cap, uint64_t k) {
   const uint32_t m = (uint32_t)cap - 1u;
   uint32_t i = (uint32_t)sim_mix(k) & m;
@@ -127,9 +141,10 @@ __global__ void k_reset_layer(size_t ncell, int8_t* __restrict__ layer) {
 __global__ void k_particle_bricks(int np, const double* __restrict__ px, const double* __restrict__ py,
                                   const double* __restrict__ pz, SimIH ih, const uint64_t* __restrict__ hk,
                                   const int* __restrict__ hv, int cap, int* __restrict__ bflag, int* __restrict__ missing,
-                                  int* __restrict__ nmissing, int maxmissing) {
+                                  int* __restrict__ nmissing, int maxmissing, int floor_axis, double floor_value) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= np) return;
+  if (floor_axis >= 0 && (floor_axis == 0 ? px[i] : floor_axis == 1 ? py[i] : pz[i]) < floor_value) return;
   int kx, ky, kz;
   part_key(px[i], py[i], pz[i], ih.ih, kx, ky, kz);
   int cx = sim_chunk(kx), cy = sim_chunk(ky), cz = sim_chunk(kz);
@@ -147,11 +162,13 @@ __global__ void k_particle_bricks(int np, const double* __restrict__ px, const d
 }
 
 // Flag bricks holding an existing inflow cell.
-__global__ void k_inflow_bricks(int nb, const uint8_t* __restrict__ flags, int* __restrict__ bflag) {
+// Bricks holding a cell that the fill step can give layer 0: Inflow cells,
+// and Interior cells when `interior` is set.
+__global__ void k_inflow_bricks(int nb, const uint8_t* __restrict__ flags, int* __restrict__ bflag, int interior) {
   int b = blockIdx.x;
   for (int t = threadIdx.x; t < MAC_BC; t += blockDim.x) {
     unsigned f = flags[(size_t)b * MAC_BC + t];
-    if ((f & F_EXISTS) && (f & 7u) == 3u) bflag[b] = 1;
+    if ((f & F_EXISTS) && ((f & 7u) == 3u || (interior && (f & 7u) == 4u))) bflag[b] = 1;
   }
 }
 
@@ -173,20 +190,145 @@ __global__ void k_mark_particles(int np, const double* __restrict__ px, const do
   layer[c] = 0;
 }
 
+// ------------------------------------------------------- depth chain ----
+// Algorithms_Initialize_Layer_And_Depth::Do (:16-33): layer := empty for every
+// cell, depth := empty for the fluid cells that are not Interior.
+__global__ void k_reset_layer_depth(size_t ncell, const uint8_t* __restrict__ flags, int8_t* __restrict__ layer,
+                                    int16_t* __restrict__ depth) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < ncell; i += (size_t)gridDim.x * blockDim.x) {
+    layer[i] = -1;
+    const unsigned f = flags[i];
+    if ((f & F_EXISTS) && mat_is_fluid(f & 7u) && (f & 7u) != 4u) depth[i] = -1;
+  }
+}
+
+// Algorithm_DelParticle::Do (test/Jet3.cpp:259-289) then
+// Algorithms_Layer_Initial_With_Particle_Depth::Do (:20-51).  A particle with
+// position[floor_axis] < floor_value is erased first (floor_axis < 0: none).
+// A particle in an Interior cell gives the cell layer 0 and is erased when the
+// cell's depth exceeds depth_remove; any other particle's cell gets layer 0
+// and SetFluid.  depth == nullptr: no cell is treated as Interior (the
+// non-depth Algorithms_Layer_Initial_With_Particle with the floor).  The kept
+// particles are appended to q (*nkept counts them; order across waves is not
+// kept, particles being independent).
+__global__ void k_mark_particles_keep(int np, const double* __restrict__ px, const double* __restrict__ py,
+                                      const double* __restrict__ pz, SimIH ih, const uint64_t* __restrict__ hk,
+                                      const int* __restrict__ hv, int cap, uint8_t* __restrict__ flags, int8_t* __restrict__ layer,
+                                      const int16_t* __restrict__ depth, int depth_remove, int floor_axis, double floor_value,
+                                      double* __restrict__ qx, double* __restrict__ qy, double* __restrict__ qz,
+                                      int* __restrict__ nkept) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= np) return;
+  const double x = px[i], y = py[i], z = pz[i];
+  bool keep = true;
+  if (floor_axis >= 0) keep = !((floor_axis == 0 ? x : floor_axis == 1 ? y : z) < floor_value);
+  if (keep) {
+    int kx, ky, kz;
+    part_key(x, y, z, ih.ih, kx, ky, kz);
+    long long c = cell_of_key(hk, hv, cap, kx, ky, kz);
+    if (c >= 0) {
+      const unsigned f = flags[c];
+      if (depth && (f & 7u) == 4u) {
+        // grid[key] sets the cell's bit and keeps its data (ChunkHashTable.tpp:63-87)
+        flags[c] = (uint8_t)(f | F_EXISTS);
+        layer[c] = 0;
+        keep = depth[c] <= depth_remove;
+      } else {
+        flags[c] = (uint8_t)((f & ~7u) | mat_set_fluid(f & 7u) | F_EXISTS);
+        layer[c] = 0;
+      }
+    }
+  }
+  const int slot = wave_append(keep, nkept);
+  if (slot >= 0) {
+    qx[slot] = x;
+    qy[slot] = y;
+    qz[slot] = z;
+  }
+}
+
+// Start of Algorithms_Update_CellType_Layer_Depth::Do (:31-56): depth := empty;
+// a layer-0 cell with a face neighbour that is missing or layer-empty is on
+// the surface: depth 0.  Runs before the air layers are laid.
+__global__ void k_depth_surface(int nb, const int* __restrict__ nbr27, const uint8_t* __restrict__ flags,
+                                const int8_t* __restrict__ layer, int16_t* __restrict__ depth) {
+  int b = blockIdx.x;
+  for (int t = threadIdx.x; t < MAC_BC; t += blockDim.x) {
+    size_t c = (size_t)b * MAC_BC + t;
+    if (!(flags[c] & F_EXISTS)) continue;
+    int16_t d = -1;
+    if (layer[c] == 0) {
+      const int x = t & 7, y = (t >> 3) & 7, z = t >> 6;
+      const int dx[6] = {-1, 1, 0, 0, 0, 0}, dy[6] = {0, 0, -1, 1, 0, 0}, dz[6] = {0, 0, 0, 0, -1, 1};
+      for (int k = 0; k < 6; ++k) {
+        int cell;
+        const int nbk = locate(nbr27, b, x + dx[k], y + dy[k], z + dz[k], cell);
+        if (nbk < 0) {
+          d = 0;
+          break;
+        }
+        const size_t nc = (size_t)nbk * MAC_BC + cell;
+        if (!(flags[nc] & F_EXISTS) || layer[nc] == -1) {
+          d = 0;
+          break;
+        }
+      }
+    }
+    depth[c] = d;
+  }
+}
+
+// One level of the depth search (:96-133): a layer-0 cell of empty depth next
+// to a cell of depth d-1 gets depth d, and becomes Fluid (SetFluid) when
+// d < depth_interior, Interior otherwise.  *any is set when a cell changed.
+__global__ void k_depth_round(int nb, const int* __restrict__ nbr27, uint8_t* __restrict__ flags, const int8_t* __restrict__ layer,
+                              int16_t* __restrict__ depth, int d, int depth_interior, int* __restrict__ any) {
+  int b = blockIdx.x;
+  for (int t = threadIdx.x; t < MAC_BC; t += blockDim.x) {
+    size_t c = (size_t)b * MAC_BC + t;
+    const unsigned f = flags[c];
+    if (!(f & F_EXISTS) || layer[c] != 0 || depth[c] != -1) continue;
+    const int x = t & 7, y = (t >> 3) & 7, z = t >> 6;
+    const int dx[6] = {-1, 1, 0, 0, 0, 0}, dy[6] = {0, 0, -1, 1, 0, 0}, dz[6] = {0, 0, 0, 0, -1, 1};
+    bool hit = false;
+    for (int k = 0; k < 6 && !hit; ++k) {
+      int cell;
+      const int nbk = locate(nbr27, b, x + dx[k], y + dy[k], z + dz[k], cell);
+      if (nbk < 0) continue;
+      const size_t nc = (size_t)nbk * MAC_BC + cell;
+      hit = (flags[nc] & F_EXISTS) && depth[nc] == d - 1;
+    }
+    if (!hit) continue;
+    depth[c] = (int16_t)d;
+    flags[c] = (uint8_t)((f & ~7u) | (d < depth_interior ? mat_set_fluid(f & 7u) : 4u));
+    *any = 1;
+  }
+}
+
 // Algorithms_Create_Fluid_Particle::Do with Policy_Is_Inbound_Filling_Layer_CellType
 // (:16-19): an existing Inflow cell with an empty layer gets `ppc` particles
 // at seeded jitter inside the cell ((key + U - 1/2) h per axis), layer 0 and
 // SetFluid.
+// With depth (Algorithms_Create_Fluid_Particle_Depth::Do, :20-47): an Interior
+// cell with an empty layer and depth <= depth_remove also gets particles and
+// layer 0 (no SetFluid); any other Interior cell gets layer 0.
 __global__ void k_inflow_fill(int nb, const int* __restrict__ keys3, uint8_t* __restrict__ flags, int8_t* __restrict__ layer,
                               double h0, double h1, double h2, int ppc, uint64_t seed, uint64_t tick, double* __restrict__ px,
                               double* __restrict__ py, double* __restrict__ pz, int* __restrict__ np, int maxp,
-                              int* __restrict__ overflow) {
+                              int* __restrict__ overflow, const int16_t* __restrict__ depth, int depth_remove) {
 #pragma clang fp contract(off)
   int b = blockIdx.x;
   for (int t = threadIdx.x; t < MAC_BC; t += blockDim.x) {
     size_t c = (size_t)b * MAC_BC + t;
     unsigned f = flags[c];
-    if (!(f & F_EXISTS) || (f & 7u) != 3u || layer[c] != -1) continue;
+    if (!(f & F_EXISTS)) continue;
+    const bool inflow = ppc > 0 && (f & 7u) == 3u && layer[c] == -1;
+    if (!inflow) {
+      if (!depth || (f & 7u) != 4u) continue;
+      const bool refill = layer[c] == -1 && depth[c] <= depth_remove;
+      layer[c] = 0;
+      if (!refill) continue;
+    }
     int kx = keys3[3 * b] * 8 + (t & 7), ky = keys3[3 * b + 1] * 8 + ((t >> 3) & 7), kz = keys3[3 * b + 2] * 8 + (t >> 6);
     uint64_t ck = sim_pack(kx, ky, kz);
     int base = atomicAdd(np, ppc);
@@ -202,7 +344,7 @@ __global__ void k_inflow_fill(int nb, const int* __restrict__ keys3, uint8_t* __
       pz[base + k] = ((double)kz + (uz - 0.5)) * h2;
     }
     layer[c] = 0;
-    flags[c] = (uint8_t)((f & ~7u) | mat_set_fluid(f & 7u));
+    if (inflow) flags[c] = (uint8_t)((f & ~7u) | mat_set_fluid(f & 7u));
   }
 }
 
@@ -288,6 +430,13 @@ __global__ void k_delete_cells(int nb, uint8_t* __restrict__ flags, const int8_t
 // zero fields) when from[j] < 0 -- a chunk allocated from the prototype.
 __global__ void k_relayout_u8(int nb, const int* __restrict__ from, const uint8_t* __restrict__ src, uint8_t* __restrict__ dst,
                               uint8_t proto) {
+  int j = blockIdx.x;
+  int o = from[j];
+  for (int t = threadIdx.x; t < MAC_BC; t += blockDim.x)
+    dst[(size_t)j * MAC_BC + t] = o >= 0 ? src[(size_t)o * MAC_BC + t] : proto;
+}
+__global__ void k_relayout_i16(int nb, const int* __restrict__ from, const int16_t* __restrict__ src, int16_t* __restrict__ dst,
+                               int16_t proto) {
   int j = blockIdx.x;
   int o = from[j];
   for (int t = threadIdx.x; t < MAC_BC; t += blockDim.x)
@@ -585,10 +734,10 @@ __global__ void k_move_particles(int np, const double* __restrict__ px, const do
     interp_speed(G, q, v);
     for (int j = 0; j < 3; ++j) res[j] = pos[j] + dt * v[j];
   }
-  if (use_box) {
-    if (!(res[0] >= lo0 && res[0] < hi0 && res[1] >= lo1 && res[1] < hi1 && res[2] >= lo2 && res[2] < hi2)) return;
-  }
-  int o = atomicAdd(nkept, 1);
+  const bool keep =
+      !use_box || (res[0] >= lo0 && res[0] < hi0 && res[1] >= lo1 && res[1] < hi1 && res[2] >= lo2 && res[2] < hi2);
+  const int o = wave_append(keep, nkept);
+  if (o < 0) return;
   qx[o] = res[0];
   qy[o] = res[1];
   qz[o] = res[2];

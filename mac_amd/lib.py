@@ -299,6 +299,10 @@ class SimParams(ctypes.Structure):
         ("box_hi", ctypes.c_double * 3),
         ("max_particles", ctypes.c_int),
         ("max_bricks", ctypes.c_int),
+        ("depth_interior", ctypes.c_int),
+        ("depth_remove", ctypes.c_int),
+        ("floor_axis", ctypes.c_int),
+        ("floor_value", ctypes.c_double),
         ("proj", Params),
     ]
 
@@ -339,6 +343,8 @@ SIGNATURES.update({
     "mac_sim_interpolate": (ctypes.c_int, [_P, ctypes.c_int, _P, _P]),
     "mac_sim_step": (ctypes.c_int, [_P, ctypes.POINTER(SimStats)]),
     "mac_sim_extrapolate": (ctypes.c_int, [_P, ctypes.c_int]),
+    "mac_sim_get_depth": (ctypes.c_int, [_P, _P]),
+    "mac_sim_set_depth": (ctypes.c_int, [_P, _P]),
 })
 
 
@@ -407,6 +413,20 @@ class Sim:
         _check(self.lib.mac_sim_download(self.ptr, *[_ptr(out[k]) for k in (
             "keys", "flags", "layer", "u", "v", "w", "ax", "ay", "az", "p", "particles")]))
         return out
+
+    def get_depth(self):
+        """Per-cell depth (int16, -1 = empty) in download order."""
+        nb, _ = self.counts()
+        d = np.zeros(nb * 512, np.int16)
+        _check(self.lib.mac_sim_get_depth(self.ptr, _ptr(d)))
+        return d
+
+    def set_depth(self, depth):
+        nb, _ = self.counts()
+        d = np.ascontiguousarray(depth, dtype=np.int16).reshape(-1)
+        if d.size != nb * 512:
+            raise ValueError("depth needs one value per cell of the current bricks")
+        _check(self.lib.mac_sim_set_depth(self.ptr, _ptr(d)))
 
     def topology(self):
         _check(self.lib.mac_sim_topology(self.ptr))

@@ -177,7 +177,7 @@ class Sim:
 
     def __init__(self, h=(0.01, 0.01, 0.01), gravity=(0, 0, -9.81), viscosity=0.0, cfl=0.5, dt_min=0.0, dt_max=1e30,
                  layer_max=2, integrator=0, particle_rk=2, particles_per_inflow_cell=8, seed=0, box_lo=(0, 0, 0),
-                 box_hi=(0, 0, 0), **_ignored):
+                 box_hi=(0, 0, 0), depth_interior=0, depth_remove=0, floor_axis=-1, floor_value=0.0, **_ignored):
         lib = load()
         if not hasattr(lib, "_sim_bound"):
             lib.orc_sim_create.restype = ctypes.c_void_p
@@ -194,13 +194,16 @@ class Sim:
             lib.orc_sim_move.argtypes = [_P, ctypes.c_double]
             lib.orc_sim_interpolate.argtypes = [_P, ctypes.c_int, _P, _P]
             lib.orc_sim_extrapolate.argtypes = [_P, ctypes.c_int]
+            lib.orc_sim_get_depth.argtypes = [_P, _P]
+            lib.orc_sim_set_depth.argtypes = [_P, _P]
             lib.orc_sim_step.restype = ctypes.c_double
             lib.orc_sim_step.argtypes = [_P, _P]
             lib._sim_bound = True
         self.lib = lib
-        prm = np.array(list(h) + list(gravity) + [viscosity, cfl, dt_min, dt_max] + list(box_lo) + list(box_hi),
-                       dtype=np.float64)
-        iprm = np.array([layer_max, integrator, particle_rk, particles_per_inflow_cell], dtype=np.int32)
+        prm = np.array(list(h) + list(gravity) + [viscosity, cfl, dt_min, dt_max] + list(box_lo) + list(box_hi)
+                       + [floor_value], dtype=np.float64)
+        iprm = np.array([layer_max, integrator, particle_rk, particles_per_inflow_cell, depth_interior, depth_remove,
+                         floor_axis], dtype=np.int32)
         self.ptr = ctypes.c_void_p(lib.orc_sim_create(_p(prm), _p(iprm), seed))
 
     def __del__(self):
@@ -233,6 +236,17 @@ class Sim:
         self.lib.orc_sim_download(self.ptr, *[_p(out[k]) for k in (
             "keys", "flags", "layer", "u", "v", "w", "ax", "ay", "az", "p", "particles")])
         return out
+
+    def get_depth(self):
+        nb, _ = self.counts()
+        d = np.zeros(nb * 512, np.int16)
+        self.lib.orc_sim_get_depth(self.ptr, _p(d))
+        return d
+
+    def set_depth(self, depth):
+        d = _c(depth, np.int16)
+        assert d.size == self.counts()[0] * 512
+        self.lib.orc_sim_set_depth(self.ptr, _p(d))
 
     def topology(self):
         self.lib.orc_sim_topology(self.ptr)

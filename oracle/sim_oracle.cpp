@@ -5,7 +5,8 @@
 //
 // A sequential restatement of the reference algorithms on a sparse chunk map
 // (std::map of chunk key -> 8^3 chunk), following the reference's own control
-// flow: stack-based layer BFS (Algorithms_Update_CellType_Layer.h:22-95),
+// flow: stack-based layer BFS (Algorithms_Update_CellType_Layer.h:22-95,
+// Algorithms_Update_CellType_Layer_Depth.h:24-138 with its depth stacks),
 // grid[key] inserting cells (ChunkHashTable.tpp:63-87), cell erase and chunk
 // free (:133-161), per-particle sequential loops.
 #include <algorithm>
@@ -32,6 +33,7 @@ struct K3 {
 struct Chunk {
   uint8_t flags[512];
   int8_t layer[512];
+  int16_t depth[512];
   double S[2][3][512];
   double A[3][512];
   double P[512];
@@ -39,6 +41,9 @@ struct Chunk {
     // the prototype cell: Air, no const, not existing, layer empty, zero fields
     memset(flags, 0, sizeof(flags));
     memset(layer, -1, sizeof(layer));
+    // the reference's depth prototype is never initialised (test/Jet3.cpp:333-336);
+    // a new cell's depth is overwritten before it is read, -1 is taken here
+    for (int t = 0; t < 512; ++t) depth[t] = -1;
     memset(S, 0, sizeof(S));
     memset(A, 0, sizeof(A));
     memset(P, 0, sizeof(P));
@@ -72,6 +77,8 @@ inline uint64_t pack(int x, int y, int z) {
 struct Params {
   double h[3], g[3], nu, cfl, dt_min, dt_max;
   int layer_max, integrator, particle_rk, ppc;
+  int depth_interior, depth_remove, floor_axis;  // depth_interior 0: the chain without depth
+  double floor_value;
   uint64_t seed;
   double lo[3], hi[3];
 };
@@ -114,6 +121,10 @@ struct Sim {
     Chunk* ch = chunk(kx, ky, kz);
     return ch ? ch->layer[off_of(kx, ky, kz)] : -1;
   }
+  int16_t depth_of(int kx, int ky, int kz) {
+    Chunk* ch = chunk(kx, ky, kz);
+    return ch ? ch->depth[off_of(kx, ky, kz)] : -1;
+  }
   template <typename F>
   void for_cells(F f) {
     for (auto& kv : grid)
@@ -126,33 +137,76 @@ struct Sim {
   }
 
   void topology() {
-    // Algorithms_Initialize_MacCell
-    for_cells([&](Chunk* c, int t, int, int, int) { c->layer[t] = -1; });
-    // Algorithms_Layer_Initial_With_Particle
-    for (auto& p : parts) {
-      int k[3];
-      key_of(p, k);
-      Chunk* c = insert(k[0], k[1], k[2]);
-      int o = off_of(k[0], k[1], k[2]);
-      c->layer[o] = 0;
-      c->flags[o] = (uint8_t)((c->flags[o] & ~7u) | set_fluid(c->flags[o] & 7u));
+    const bool dm = P.depth_interior > 0;
+    // Algorithms_Initialize_MacCell; with depth Algorithms_Initialize_Layer_And_Depth.h:16-33
+    for_cells([&](Chunk* c, int t, int, int, int) {
+      c->layer[t] = -1;
+      const unsigned m = c->flags[t] & 7u;
+      if (dm && is_fluid(m) && m != 4u) c->depth[t] = -1;
+    });
+    // Algorithm_DelParticle (test/Jet3.cpp:259-289): list erase, order kept
+    if (P.floor_axis >= 0) {
+      std::vector<std::array<double, 3>> kept;
+      for (auto& p : parts)
+        if (!(p[P.floor_axis] < P.floor_value)) kept.push_back(p);
+      parts.swap(kept);
     }
-    // Algorithms_Create_Fluid_Particle with the jitter policy
+    if (dm) {
+      // Algorithms_Layer_Initial_With_Particle_Depth.h:29-46
+      std::vector<std::array<double, 3>> kept;
+      for (auto& p : parts) {
+        int k[3];
+        key_of(p, k);
+        Chunk* c = insert(k[0], k[1], k[2]);
+        int o = off_of(k[0], k[1], k[2]);
+        if ((c->flags[o] & 7u) == 4u) {
+          c->layer[o] = 0;
+          if (c->depth[o] > P.depth_remove) continue;
+        } else {
+          c->layer[o] = 0;
+          c->flags[o] = (uint8_t)((c->flags[o] & ~7u) | set_fluid(c->flags[o] & 7u));
+        }
+        kept.push_back(p);
+      }
+      parts.swap(kept);
+    } else {
+      // Algorithms_Layer_Initial_With_Particle
+      for (auto& p : parts) {
+        int k[3];
+        key_of(p, k);
+        Chunk* c = insert(k[0], k[1], k[2]);
+        int o = off_of(k[0], k[1], k[2]);
+        c->layer[o] = 0;
+        c->flags[o] = (uint8_t)((c->flags[o] & ~7u) | set_fluid(c->flags[o] & 7u));
+      }
+    }
+    // Algorithms_Create_Fluid_Particle with the jitter policy; with depth
+    // Algorithms_Create_Fluid_Particle_Depth.h:29-44
+    auto fill = [&](int kx, int ky, int kz) {
+      uint64_t ck = pack(kx, ky, kz);
+      for (int k = 0; k < P.ppc; ++k) {
+        double ux = uni(P.seed, tick, ck, 3 * k), uy = uni(P.seed, tick, ck, 3 * k + 1), uz = uni(P.seed, tick, ck, 3 * k + 2);
+        parts.push_back({((double)kx + (ux - 0.5)) * P.h[0], ((double)ky + (uy - 0.5)) * P.h[1], ((double)kz + (uz - 0.5)) * P.h[2]});
+      }
+    };
     for_cells([&](Chunk* c, int t, int kx, int ky, int kz) {
       unsigned f = c->flags[t];
-      if ((f & 7u) == 3u && c->layer[t] == -1) {
-        uint64_t ck = pack(kx, ky, kz);
-        for (int k = 0; k < P.ppc; ++k) {
-          double ux = uni(P.seed, tick, ck, 3 * k), uy = uni(P.seed, tick, ck, 3 * k + 1), uz = uni(P.seed, tick, ck, 3 * k + 2);
-          parts.push_back({((double)kx + (ux - 0.5)) * P.h[0], ((double)ky + (uy - 0.5)) * P.h[1], ((double)kz + (uz - 0.5)) * P.h[2]});
-        }
+      if (P.ppc > 0 && (f & 7u) == 3u && c->layer[t] == -1) {
+        fill(kx, ky, kz);
         c->layer[t] = 0;
         c->flags[t] = (uint8_t)((f & ~7u) | set_fluid(f & 7u));
+      } else if (dm && (f & 7u) == 4u && c->layer[t] == -1 && c->depth[t] <= P.depth_remove) {
+        fill(kx, ky, kz);
+        c->layer[t] = 0;
+      } else if (dm && (f & 7u) == 4u) {
+        c->layer[t] = 0;
       }
     });
-    // Algorithms_Update_CellType_Layer (stack BFS)
+    // Algorithms_Update_CellType_Layer(_Depth) (stack BFS); with depth every
+    // cell's depth is emptied and the surface cells get depth 0 (:31-56)
     std::stack<std::array<int, 3>> s, s2;
     for_cells([&](Chunk* c, int t, int kx, int ky, int kz) {
+      if (dm) c->depth[t] = -1;
       if (c->layer[t] != 0) return;
       bool b = false;
       for (int i = 0; i < 3 && !b; ++i)
@@ -160,12 +214,14 @@ struct Sim {
           int n[3] = {kx, ky, kz};
           n[i] += si;
           if (!valid(n[0], n[1], n[2]) || layer_of(n[0], n[1], n[2]) == -1) {
+            if (dm) c->depth[t] = 0;
             s.push({kx, ky, kz});
             b = true;
             break;
           }
         }
     });
+    std::stack<std::array<int, 3>> sback = s;
     for (int lay = 1; lay <= P.layer_max; ++lay) {
       while (!s.empty()) {
         auto k = s.top();
@@ -198,6 +254,28 @@ struct Sim {
           c->flags[o] = (uint8_t)((c->flags[o] & ~7u) | set_air(c->flags[o] & 7u));
         }
       }
+    }
+    // depth search inside the fluid (:96-133): Fluid below depth_interior,
+    // Interior (SetInterior, unconditional) from there on
+    for (int depth = 1; dm; ++depth) {
+      while (!sback.empty()) {
+        auto k = sback.top();
+        sback.pop();
+        for (int i = 0; i < 3; ++i)
+          for (int si = -1; si <= 1; si += 2) {
+            int n[3] = {k[0], k[1], k[2]};
+            n[i] += si;
+            if (!valid(n[0], n[1], n[2]) || layer_of(n[0], n[1], n[2]) != 0 || depth_of(n[0], n[1], n[2]) != -1) continue;
+            Chunk* c = chunk(n[0], n[1], n[2]);
+            int o = off_of(n[0], n[1], n[2]);
+            c->depth[o] = (int16_t)depth;
+            if (depth < P.depth_interior) c->flags[o] = (uint8_t)((c->flags[o] & ~7u) | set_fluid(c->flags[o] & 7u));
+            else c->flags[o] = (uint8_t)((c->flags[o] & ~7u) | 4u);
+            s2.push({n[0], n[1], n[2]});
+          }
+      }
+      std::swap(s2, sback);
+      if (sback.empty()) break;
     }
     // Algorithms_Delete_MacCell: erase layer-empty cells unless NoDelete, free empty chunks
     for (auto it = grid.begin(); it != grid.end();) {
@@ -553,7 +631,8 @@ struct Sim {
 extern "C" {
 
 // prm: h[3], g[3], nu, cfl, dt_min, dt_max, lo[3], hi[3] (16 doubles);
-// iprm: layer_max, integrator, particle_rk, ppc; seed.
+// iprm: layer_max, integrator, particle_rk, ppc, depth_interior, depth_remove,
+// floor_axis; prm[16] = floor_value; seed.
 void* orc_sim_create(const double* prm, const int* iprm, unsigned long long seed) {
   Sim* s = new Sim();
   for (int i = 0; i < 3; ++i) {
@@ -571,6 +650,10 @@ void* orc_sim_create(const double* prm, const int* iprm, unsigned long long seed
   s->P.integrator = iprm[1];
   s->P.particle_rk = iprm[2];
   s->P.ppc = iprm[3];
+  s->P.depth_interior = iprm[4];
+  s->P.depth_remove = iprm[5];
+  s->P.floor_axis = iprm[6];
+  s->P.floor_value = prm[16];
   s->P.seed = seed;
   return s;
 }
@@ -628,6 +711,15 @@ void orc_sim_download(void* p, int32_t* keys, uint8_t* flags, int8_t* layer, dou
       for (int i = 0; i < 3; ++i) pxyz[3 * k + i] = s->parts[k][i];
 }
 
+// per-cell depth in download order
+void orc_sim_get_depth(void* p, int16_t* depth) {
+  size_t b = 0;
+  for (auto& kv : ((Sim*)p)->grid) memcpy(&depth[512 * b++], kv.second->depth, sizeof(kv.second->depth));
+}
+void orc_sim_set_depth(void* p, const int16_t* depth) {
+  size_t b = 0;
+  for (auto& kv : ((Sim*)p)->grid) memcpy(kv.second->depth, &depth[512 * b++], sizeof(kv.second->depth));
+}
 void orc_sim_topology(void* p) { ((Sim*)p)->topology(); }
 double orc_sim_compute_dt(void* p) { return ((Sim*)p)->compute_dt(); }
 void orc_sim_forces(void* p, int slot) { ((Sim*)p)->forces(slot); }

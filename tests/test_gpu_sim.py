@@ -162,3 +162,93 @@ def test_extrapolate_bitexact_after_jet_steps():
         fin = np.isfinite(b)
         assert np.array_equal(np.isfinite(a), fin), k
         assert np.linalg.norm(a[fin] - b[fin]) <= 1e-8 * max(1.0, np.linalg.norm(b[fin])), k
+
+
+# ------------------------------------------------------------ depth chain ----
+def same_depth(pair, dg):
+    ex = (dg["flags"] & W.EXISTS) > 0
+    assert np.array_equal(pair[0].get_depth()[ex], pair[1].get_depth()[ex])
+
+
+@pytest.mark.parametrize("shape", ["ball", "random"])
+def test_depth_topology_bitexact(shape):
+    """Algorithms_Initialize_Layer_And_Depth -> ..._Layer_Initial_With_Particle_Depth
+    -> Algorithms_Create_Fluid_Particle_Depth -> Algorithms_Update_CellType_Layer_Depth
+    on a blob of particles: first call makes Interior cells, then holes are
+    punched in the particle list (deep particles go, shallow Interior cells
+    are refilled) and two more calls follow.  Flags, layers, depth and the
+    particle multiset bit-exact."""
+    n = 20
+    g = np.indices((n, n, n))
+    if shape == "ball":
+        mask = ((g - (n - 1) / 2.0) ** 2).sum(0) <= (n / 2.0) ** 2
+    else:
+        mask = np.random.default_rng(9).random((n, n, n)) < 0.98
+    pts = np.argwhere(mask).astype(np.float64) * 0.5 + 0.01
+    pair = both(h=(0.5, 0.5, 0.5), layer_max=3, gravity=(0, 0, 0), particles_per_inflow_cell=3, depth_interior=3,
+                depth_remove=4, seed=5)
+    for s in pair:
+        s.upload(np.zeros((0, 3), np.int32), np.zeros(0, np.uint8), particles=pts)
+        s.topology()
+    dg, do = pair[0].download(), pair[1].download()
+    same_cells(dg, do)
+    same_depth(pair, dg)
+    assert ((dg["flags"][(dg["flags"] & W.EXISTS) > 0] & 7) == W.INTERIOR).sum() > 50
+    keep = np.random.default_rng(1).random(len(pts)) > 0.3
+    dep = pair[0].get_depth()
+    for s in pair:
+        s.upload(dg["keys"], dg["flags"], dg["layer"], particles=pts[keep])
+        s.set_depth(dep)
+    for it in range(2):
+        for s in pair:
+            s.topology()
+        dg, do = pair[0].download(), pair[1].download()
+        same_cells(dg, do)
+        same_depth(pair, dg)
+        same_particles(dg["particles"], do["particles"], 0.0)
+
+
+def test_depth_floor_bitexact():
+    """Algorithm_DelParticle: particles below the floor never mark a cell."""
+    rng = np.random.default_rng(4)
+    pts = rng.uniform(-4.0, 12.0, size=(3000, 3))
+    pair = both(h=(1.0, 1.0, 1.0), layer_max=2, gravity=(0, 0, 0), particles_per_inflow_cell=0, depth_interior=2,
+                depth_remove=2, floor_axis=1, floor_value=0.25)
+    for s in pair:
+        s.upload(np.zeros((0, 3), np.int32), np.zeros(0, np.uint8), particles=pts)
+        s.topology()
+    dg, do = pair[0].download(), pair[1].download()
+    same_cells(dg, do)
+    same_depth(pair, dg)
+    same_particles(dg["particles"], do["particles"], 0.0)
+    assert len(dg["particles"]) == int((pts[:, 1] >= 0.25).sum())
+
+
+def test_depth_jet_steps_vs_oracle():
+    """A thick lateral jet stepped with the depth chain (Interior cells,
+    deep-particle removal, Interior refill, floor) on both sides."""
+    jet = J.lateral_jet(32, 24, 24, h=0.01, nozzle=8, inflow=1.5, seed=3)
+    kw = dict(jet["params"], layer_max=2, integrator=1, particle_rk=2, seed=7, particles_per_inflow_cell=4,
+              depth_interior=2, depth_remove=3, floor_axis=2, floor_value=5.5 * 0.01)
+    # a slab of fluid in front of the nozzle, two jittered particles per cell
+    cells = np.argwhere(np.ones((14, 12, 12), bool)) + np.array([1, 6, 6])
+    rng = np.random.default_rng(2)
+    pts = np.repeat(cells, 2, axis=0) * 0.01 + rng.uniform(-0.004, 0.004, size=(2 * len(cells), 3))
+    pair = both(**kw)
+    upload_both(jet, pair, particles=pts)
+    n_interior = 0
+    for step in range(8):
+        sg = pair[0].step()
+        so = pair[1].step()
+        assert sg["dt"] == pytest.approx(so["dt"], rel=1e-9)
+        dg, do = pair[0].download(), pair[1].download()
+        same_cells(dg, do)
+        same_depth(pair, dg)
+        ex = (dg["flags"] & W.EXISTS) > 0
+        n_interior = max(n_interior, int(((dg["flags"][ex] & 7) == W.INTERIOR).sum()))
+        for k in ("u", "v", "w"):
+            ref = do[k][ex]
+            assert np.linalg.norm(dg[k][ex] - ref) <= 1e-8 * max(np.linalg.norm(ref), 1.0), (step, k)
+        same_particles(dg["particles"], do["particles"], 1e-9)
+        assert sg["max_div"] < 1e-7
+    assert n_interior > 0

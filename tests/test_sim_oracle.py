@@ -249,3 +249,134 @@ def test_extrapolation_leaves_lower_layers_and_is_idempotent():
         for tt in t[(lay[b] == 0) & ((t & 7) > 0)]:
             if lay[b, tt - 1] == 0:
                 assert e1["u"][b * 512 + tt] == comps[0][b * 512 + tt]
+
+
+# ------------------------------------------------------------ depth chain ----
+def _blob_state(mask, h=1.0):
+    """One particle at the centre of every cell of `mask` (x, y, z indexing),
+    nothing uploaded but the particles."""
+    pts = np.argwhere(mask).astype(np.float64) * h
+    return pts
+
+
+def _depth_by_key(s):
+    d = s.download()
+    dep = s.get_depth()
+    out = {}
+    for b, k in enumerate(d["keys"]):
+        for t in range(512):
+            f = d["flags"][b * 512 + t]
+            if f & W.EXISTS:
+                key = (8 * k[0] + (t & 7), 8 * k[1] + ((t >> 3) & 7), 8 * k[2] + (t >> 6))
+                out[key] = (int(d["layer"][b * 512 + t]), int(dep[b * 512 + t]), int(f & 7))
+    return out, d
+
+
+def _taxicab_depth(mask):
+    """Depth closed form: a cell's 6-connected distance, inside the mask, to
+    the cells that touch a non-mask cell is its taxicab distance to the
+    nearest non-mask cell minus one (the cells of a monotone path to the
+    nearest outside cell are all inside)."""
+    from scipy import ndimage
+    pad = np.pad(mask, 1)
+    return ndimage.distance_transform_cdt(pad, metric="taxicab")[1:-1, 1:-1, 1:-1] - 1
+
+
+@pytest.mark.parametrize("shape", ["box", "ball", "random"])
+def test_depth_is_taxicab_distance_to_the_surface(shape):
+    """Algorithms_Update_CellType_Layer_Depth.h:31-133: depth 0 on the
+    surface, then the BFS level; depth < DepthInterior gives Fluid, the rest
+    Interior; air cells keep depth -1."""
+    n = 14
+    g = np.indices((n, n, n))
+    if shape == "box":
+        mask = np.ones((n, n, n), bool)
+    elif shape == "ball":
+        mask = ((g - (n - 1) / 2.0) ** 2).sum(0) <= (n / 2.0) ** 2
+    else:
+        rng = np.random.default_rng(3)
+        mask = rng.random((n, n, n)) < 0.8
+    want = _taxicab_depth(mask)
+    s = O.Sim(h=(1.0, 1.0, 1.0), layer_max=2, gravity=(0, 0, 0), particles_per_inflow_cell=0, depth_interior=3,
+              depth_remove=5)
+    s.upload(np.zeros((0, 3), np.int32), np.zeros(0, np.uint8), particles=_blob_state(mask))
+    s.topology()
+    cells, _ = _depth_by_key(s)
+    for (x, y, z), (lay, dep, mat) in cells.items():
+        inside = 0 <= x < n and 0 <= y < n and 0 <= z < n and mask[x, y, z]
+        if inside:
+            assert lay == 0
+            assert dep == want[x, y, z]
+            assert mat == (W.FLUID if dep < 3 else W.INTERIOR)
+        else:
+            assert lay > 0 and dep == -1 and mat == W.AIR
+    if shape == "box":  # closed form: min distance to a face
+        x, y, z = g
+        box = np.minimum.reduce([x, n - 1 - x, y, n - 1 - y, z, n - 1 - z])
+        assert np.array_equal(box, want)
+
+
+def test_depth_interior_particles_removed_and_refilled():
+    """Algorithms_Layer_Initial_With_Particle_Depth.h:29-46: a particle in an
+    Interior cell deeper than DepthRemove is erased; the cell keeps layer 0.
+    Algorithms_Create_Fluid_Particle_Depth.h:29-44: an Interior cell left
+    without particles and no deeper than DepthRemove is refilled."""
+    n, ppc = 16, 2
+    mask = np.ones((n, n, n), bool)
+    depth = _taxicab_depth(mask)
+    s = O.Sim(h=(1.0, 1.0, 1.0), layer_max=2, gravity=(0, 0, 0), particles_per_inflow_cell=ppc, depth_interior=3,
+              depth_remove=5, seed=11)
+    pts = _blob_state(mask)
+    s.upload(np.zeros((0, 3), np.int32), np.zeros(0, np.uint8), particles=pts)
+    s.topology()
+    d = s.download()
+    # empty one Interior cell of depth 4 (refilled) and one of depth 7 (not)
+    hole4 = (4, 8, 8)
+    hole7 = (7, 7, 8)
+    assert depth[hole4] == 4 and depth[hole7] == 7
+    keep = ~np.all(pts == np.array(hole4, float), axis=1) & ~np.all(pts == np.array(hole7, float), axis=1)
+    s.upload(d["keys"], d["flags"], d["layer"], particles=pts[keep])
+    s.set_depth(_depth_by_key_array(d, depth))
+    s.topology()
+    cells, d2 = _depth_by_key(s)
+    left = d2["particles"]
+    keys = np.rint(left).astype(int)
+    deep = depth[keys[:, 0], keys[:, 1], keys[:, 2]]
+    # no particle is left in a cell deeper than 5, except the refill of hole4
+    assert (deep <= 5).all()
+    n_expected = int(((depth <= 5) & mask).sum()) - 1 + ppc
+    assert len(left) == n_expected
+    at4 = np.all(keys == np.array(hole4), axis=1)
+    assert at4.sum() == ppc
+    for c in (hole4, hole7):
+        assert cells[c][0] == 0 and cells[c][2] == W.INTERIOR
+    # the geometry did not change, so neither did the depth
+    for (x, y, z), (lay, dep, mat) in cells.items():
+        if 0 <= min(x, y, z) and max(x, y, z) < n:
+            assert dep == depth[x, y, z]
+
+
+def _depth_by_key_array(d, depth):
+    """The depth field in download order for cells of `depth` (others -1)."""
+    out = np.full(len(d["keys"]) * 512, -1, np.int16)
+    n = depth.shape[0]
+    for b, k in enumerate(d["keys"]):
+        for t in range(512):
+            x, y, z = 8 * k[0] + (t & 7), 8 * k[1] + ((t >> 3) & 7), 8 * k[2] + (t >> 6)
+            if 0 <= min(x, y, z) and max(x, y, z) < n:
+                out[b * 512 + t] = depth[x, y, z]
+    return out
+
+
+def test_floor_erases_particles_before_marking():
+    """Algorithm_DelParticle (test/Jet3.cpp:259-289): particles below the
+    floor are gone before any cell is marked, so they create no cell."""
+    pts = np.array([[0.0, 0.0, 0.0], [0.0, -3.0, 0.0], [5.0, -0.4, 0.0], [9.0, -0.6, 0.0]])
+    s = O.Sim(h=(1.0, 1.0, 1.0), layer_max=1, gravity=(0, 0, 0), particles_per_inflow_cell=0, floor_axis=1,
+              floor_value=-0.5)
+    s.upload(np.zeros((0, 3), np.int32), np.zeros(0, np.uint8), particles=pts)
+    s.topology()
+    cells, d = _depth_by_key(s)
+    assert np.array_equal(d["particles"], pts[[0, 2]])
+    fluid = {k for k, v in cells.items() if v[0] == 0}
+    assert fluid == {(0, 0, 0), (5, 0, 0)}

@@ -86,6 +86,9 @@ typedef struct {
                       (0: nu); they are short, latency-bound launches, and more
                       of them buy fewer outer iterations                          */
   double cheb_lower_coarse; /* a of the coarse levels' Chebyshev weights          */
+  int coarse_tail; /* 1 (default): the last two levels of each V-cycle (the top brick
+                      and its at most 8 children) run as one single-workgroup launch
+                      with the level in LDS; 0: one launch per step                */
 } mac_params;
 
 typedef struct {

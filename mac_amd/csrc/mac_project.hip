@@ -51,12 +51,12 @@ inline uint64_t pack_key(int x, int y, int z) {
 // average launch time can be matched against a profiler's per-kernel figure.
 enum KClass {
   K_DIV, K_PMATVEC, K_UPDATE, K_JACOBI2_0, K_JACOBI_0, K_RESID0, K_PROLONG0, K_RESTRICT1,
-  K_C_JACOBI2, K_C_JACOBI, K_C_RESID, K_C_RESTRICT, K_C_PROLONG, K_CSOLVE, K_GRAD, K_MAXDIV, K_INIT, K_FIN, K_N
+  K_C_JACOBI2, K_C_JACOBI, K_C_RESID, K_C_RESTRICT, K_C_PROLONG, K_CSOLVE, K_GRAD, K_MAXDIV, K_INIT, K_FIN, K_TAIL, K_N
 };
 const char* kNames[K_N] = {"divergence",    "cg_pupdate_matvec", "cg_update",       "jacobi2_l0",     "jacobi_l0",
                            "residual_l0",   "prolong_l0",        "restrict_l1",     "coarse_jacobi2", "coarse_jacobi",
                            "coarse_residual", "coarse_restrict", "coarse_prolong",  "coarse_solve",   "gradient_update",
-                           "max_div",       "resid_init",        "finalize"};
+                           "max_div",       "resid_init",        "finalize",        "coarse_tail"};
 
 inline int grid_of(int nact) {
   int g = (nact + WPB - 1) / WPB;
@@ -617,9 +617,12 @@ int vcycle(mac_ctx* c, TZ** zout) {
       break;
     }
   const int bot = nlv - 1;
+  // the last two levels in one launch: the top brick and its children level
+  const bool tail = c->prm.coarse_tail && bot == nl - 1 && nl >= 3 && c->L[nl - 1].nb == 1;
+  const int tlev = tail ? nl - 2 : bot;
   std::vector<TZ*> zfin(nl, nullptr);
   // down
-  for (int l = 0; l < bot; ++l) {
+  for (int l = 0; l < tlev; ++l) {
     Level& lv = c->L[l];
     const int nu = nu_of(c->prm, l);
     const double* opre = l ? c->omc_pre : c->om_pre;
@@ -661,8 +664,26 @@ int vcycle(mac_ctx* c, TZ** zout) {
     }
     zfin[l] = z;  // pre-smoothed iterate, finished on the way up
   }
-  // bottom
-  {
+  if (tail) {
+    Level& lt = c->L[tlev];
+    const Level& top = c->L[nl - 1];
+    TailArgs a;
+    for (int o = 0; o < 8; ++o) a.child[o] = top.h_child[o];
+    a.nu = nu_of(c->prm, tlev);
+    a.nc = c->nc;
+    for (int k = 0; k < 8; ++k) {
+      a.om_pre[k] = c->omc_pre[k];
+      a.om_post[k] = c->omc_post[k];
+    }
+    for (int i = 0; i < 3; ++i) a.ih2[i] = lt.ih2[i];
+    const TZ* cinv = std::is_same<TZ, float>::value ? (const TZ*)c->cinv32 : (const TZ*)c->cinv64;
+    {
+      TimedLaunch tl(c, K_TAIL, 0.0);
+      k_vcycle_tail<TZ><<<1, TAIL_THREADS, 0, c->st>>>(a, lt.code, (const TZ*)lt.r, (TZ*)lt.z, top.code, c->rowof, c->cellof,
+                                                       cinv, c->s);
+    }
+    zfin[tlev] = (TZ*)lt.z;
+  } else {
     Level& lb = c->L[bot];
     const int want_dot = (bot == 0) ? 1 : 0;
     if (bot == nl - 1) {
@@ -691,7 +712,7 @@ int vcycle(mac_ctx* c, TZ** zout) {
     }
   }
   // up
-  for (int l = bot - 1; l >= 0; --l) {
+  for (int l = tlev - 1; l >= 0; --l) {
     Level& lv = c->L[l];
     Level& cl = c->L[l + 1];
     TZ* z = zfin[l];
@@ -915,6 +936,7 @@ void mac_default_params(mac_params* p) {
   p->cheb_lower = 0.25;
   p->nu_coarse = 4;
   p->cheb_lower_coarse = 0.1;
+  p->coarse_tail = 1;
 }
 
 int mac_create(mac_ctx** out, const mac_params* prm) {

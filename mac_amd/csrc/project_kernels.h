@@ -730,6 +730,155 @@ __global__ void k_coarse_solve(int nc, const int* __restrict__ rowof, const int*
   if (want_dot) write_partial(acc, partials, sh, false);
 }
 
+// The last two levels of the V-cycle in one launch of one 1024-thread block.
+// The level above the top brick holds at most its 8 children, a 16^3 box of
+// cells; the box lives in LDS (z, its sweep partner, the restriction rows),
+// r and the codes in registers (4 cells per thread), so every sweep is one
+// pass over LDS between two barriers instead of a launch with its chain of
+// dependent global loads.  The steps and their expressions are those of
+// k_jacobi2 / k_jacobi (pre-sweeps), k_residual, k_restrict_sep,
+// k_coarse_solve, k_prolong_col and k_jacobi (post-sweeps); cells outside the
+// box and in missing bricks read as 0, as missing bricks do there.
+struct TailArgs {
+  int child[8];           // level-t bricks of the box, octant order (-1 = none)
+  int nu;                 // sweeps before and after the correction
+  int nc;                 // unknowns of the top brick
+  double om_pre[8], om_post[8];
+  double ih2[3];          // level t
+};
+#define TAIL_THREADS 1024
+template <typename TZ>
+__global__ void __launch_bounds__(TAIL_THREADS) k_vcycle_tail(TailArgs a, const uint8_t* __restrict__ code_t,
+                                                              const TZ* __restrict__ r_t, TZ* __restrict__ z_t,
+                                                              const uint8_t* __restrict__ code_c, const int* __restrict__ rowof,
+                                                              const int* __restrict__ cellof, const TZ* __restrict__ cinv,
+                                                              const Scal* s) {
+  if (s->done) return;
+  __shared__ TZ buf[2][4096];
+  __shared__ TZ t1[2048];
+  __shared__ TZ rc[MAC_BC], zc[MAC_BC];
+  const int tid = threadIdx.x;
+  const TZ ihx2 = (TZ)a.ih2[0], ihy2 = (TZ)a.ih2[1], ihz2 = (TZ)a.ih2[2];
+  long long g[4];
+  unsigned cd[4];
+  TZ rv[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const int i = tid + TAIL_THREADS * k;
+    const int X = i & 15, Y = (i >> 4) & 15, Z = i >> 8;
+    const int b = a.child[(X >> 3) + 2 * (Y >> 3) + 4 * (Z >> 3)];
+    g[k] = b >= 0 ? (long long)b * MAC_BC + (X & 7) + 8 * (Y & 7) + 64 * (Z & 7) : -1;
+    cd[k] = g[k] >= 0 ? code_t[g[k]] : 0u;
+    rv[k] = g[k] >= 0 ? r_t[g[k]] : TZ(0);
+  }
+  auto at = [](const TZ* v, int X, int Y, int Z) -> TZ {
+    return ((unsigned)X < 16u && (unsigned)Y < 16u && (unsigned)Z < 16u) ? v[X + 16 * Y + 256 * Z] : TZ(0);
+  };
+  // one damped-Jacobi sweep zin -> zout (zin == nullptr: from zero)
+  auto sweep = [&](const TZ* zin, TZ* zout, TZ om) {
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const int i = tid + TAIL_THREADS * k;
+      const int X = i & 15, Y = (i >> 4) & 15, Z = i >> 8;
+      TZ out = TZ(0);
+      if (cd[k] & CODE_DOM) {
+        if (zin) {
+          TZ dg;
+          const TZ lz = lap_cell<TZ>(cd[k], zin[i], at(zin, X - 1, Y, Z), at(zin, X + 1, Y, Z), at(zin, X, Y - 1, Z),
+                                     at(zin, X, Y + 1, Z), at(zin, X, Y, Z - 1), at(zin, X, Y, Z + 1), ihx2, ihy2, ihz2, dg);
+          out = dg > TZ(0) ? zin[i] + om * (rv[k] - lz) / dg : TZ(0);
+        } else {
+          const TZ dg = diag_of<TZ>(cd[k], ihx2, ihy2, ihz2);
+          out = dg > TZ(0) ? om * rv[k] / dg : TZ(0);
+        }
+      }
+      zout[i] = out;
+    }
+    __syncthreads();
+  };
+  int cur = 0;
+  sweep(nullptr, buf[0], (TZ)a.om_pre[0]);
+  for (int k = 1; k < a.nu; ++k) {
+    sweep(buf[cur], buf[cur ^ 1], (TZ)a.om_pre[k]);
+    cur ^= 1;
+  }
+  TZ* z = buf[cur];
+  TZ* res = buf[cur ^ 1];
+  // res = r - L z
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const int i = tid + TAIL_THREADS * k;
+    const int X = i & 15, Y = (i >> 4) & 15, Z = i >> 8;
+    TZ out = TZ(0);
+    if (cd[k] & CODE_DOM) {
+      TZ dg;
+      out = rv[k] - lap_cell<TZ>(cd[k], z[i], at(z, X - 1, Y, Z), at(z, X + 1, Y, Z), at(z, X, Y - 1, Z), at(z, X, Y + 1, Z),
+                                 at(z, X, Y, Z - 1), at(z, X, Y, Z + 1), ihx2, ihy2, ihz2, dg);
+    }
+    res[i] = out;
+  }
+  __syncthreads();
+  // restriction with the 1-D weights (1/4, 3/4, 3/4, 1/4): x, then y, then z
+  for (int i = tid; i < 2048; i += TAIL_THREADS) {  // t1[fz][fy][X]
+    const int X = i & 7, fy = (i >> 3) & 15, fz = i >> 7;
+    t1[i] = TZ(0.25) * at(res, 2 * X - 1, fy, fz) + TZ(0.75) * at(res, 2 * X, fy, fz) + TZ(0.75) * at(res, 2 * X + 1, fy, fz) +
+            TZ(0.25) * at(res, 2 * X + 2, fy, fz);
+  }
+  __syncthreads();
+  TZ* t2 = res;  // [fz][Y][X]; res is consumed
+  if (tid < 1024) {
+    const int X = tid & 7, Y = (tid >> 3) & 7, fz = tid >> 6;
+    auto r1 = [&](int fy) -> TZ { return (unsigned)fy < 16u ? t1[(fz * 16 + fy) * 8 + X] : TZ(0); };
+    t2[tid] = TZ(0.25) * r1(2 * Y - 1) + TZ(0.75) * r1(2 * Y) + TZ(0.75) * r1(2 * Y + 1) + TZ(0.25) * r1(2 * Y + 2);
+  }
+  __syncthreads();
+  if (tid < MAC_BC) {
+    const int X = tid & 7, Y = (tid >> 3) & 7, Zc = tid >> 6;
+    auto r2 = [&](int fz) -> TZ { return (unsigned)fz < 16u ? t2[fz * 64 + Y * 8 + X] : TZ(0); };
+    TZ acc = TZ(0);
+    if (code_c[tid] & CODE_DOM)
+      acc = (TZ(0.25) * r2(2 * Zc - 1) + TZ(0.75) * r2(2 * Zc) + TZ(0.75) * r2(2 * Zc + 1) + TZ(0.25) * r2(2 * Zc + 2)) * TZ(0.125);
+    rc[tid] = acc;
+  }
+  __syncthreads();
+  // exact solve on the top brick
+  if (tid < a.nc) t1[tid] = rc[cellof[tid]];
+  __syncthreads();
+  if (tid < MAC_BC) {
+    const int row = rowof[tid];
+    TZ out = TZ(0);
+    if (row >= 0)
+      for (int j = 0; j < a.nc; ++j) out += cinv[(size_t)j * a.nc + row] * t1[j];
+    zc[tid] = out;
+  }
+  __syncthreads();
+  // z += P zc (trilinear, 3/4 on the parent side)
+  auto czv = [&](int X, int Y, int Z) -> TZ {
+    return ((unsigned)X < 8u && (unsigned)Y < 8u && (unsigned)Z < 8u) ? zc[X + 8 * Y + 64 * Z] : TZ(0);
+  };
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const int i = tid + TAIL_THREADS * k;
+    if (!(cd[k] & CODE_DOM)) continue;
+    const int X = i & 15, Y = (i >> 4) & 15, Z = i >> 8;
+    const int X0 = X >> 1, Y0 = Y >> 1, Z0 = Z >> 1;
+    const int X1 = X0 + ((X & 1) ? 1 : -1), Y1 = Y0 + ((Y & 1) ? 1 : -1), Z1 = Z0 + ((Z & 1) ? 1 : -1);
+    const TZ a00 = TZ(0.75) * czv(X0, Y0, Z0) + TZ(0.25) * czv(X0, Y0, Z1);
+    const TZ a10 = TZ(0.75) * czv(X1, Y0, Z0) + TZ(0.25) * czv(X1, Y0, Z1);
+    const TZ a01 = TZ(0.75) * czv(X0, Y1, Z0) + TZ(0.25) * czv(X0, Y1, Z1);
+    const TZ a11 = TZ(0.75) * czv(X1, Y1, Z0) + TZ(0.25) * czv(X1, Y1, Z1);
+    z[i] += TZ(0.75) * (TZ(0.75) * a00 + TZ(0.25) * a01) + TZ(0.25) * (TZ(0.75) * a10 + TZ(0.25) * a11);
+  }
+  __syncthreads();
+  for (int k = 0; k < a.nu; ++k) {
+    sweep(buf[cur], buf[cur ^ 1], (TZ)a.om_post[k]);
+    cur ^= 1;
+  }
+#pragma unroll
+  for (int k = 0; k < 4; ++k)
+    if (g[k] >= 0) z_t[g[k]] = buf[cur][tid + TAIL_THREADS * k];
+}
+
 // u_i -= p(c)/h_i - p(c-e_i)/h_i on every face of an existing cell whose
 // cell or whose -e_i neighbour is in the domain, unless the face is const
 // (Algorithms_Solve_Pressure.h:198-207; Policy_Pressure_If_Correction.h:33-49;

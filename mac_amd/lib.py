@@ -38,6 +38,7 @@ class Params(ctypes.Structure):
         ("cheb_lower", ctypes.c_double),
         ("nu_coarse", ctypes.c_int),
         ("cheb_lower_coarse", ctypes.c_double),
+        ("coarse_tail", ctypes.c_int),
     ]
 
 

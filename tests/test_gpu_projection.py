@@ -249,3 +249,26 @@ def test_graph_replays_match_eager_launches(prec):
             ie, pe, ue = outs[("eager", i, rep)]
             ig, pg, ug = outs[("graph", i, rep)]
             assert ie == ig and np.array_equal(pe, pg) and np.array_equal(ue, ug), (i, rep)
+
+
+@pytest.mark.parametrize("prec", [MAC_FP64, MAC_MIXED])
+def test_coarse_tail_matches_launch_per_step(prec):
+    """The last two V-cycle levels in one launch (k_vcycle_tail) against one
+    launch per sweep, restriction, solve and prolongation: same iteration
+    counts and the same answer to rounding (the steps are the same
+    expressions; only the compiler's contraction choices may differ)."""
+    for g, nl in ((small_spheres(32, 1), 3), (small_spheres(48, 0), 4)):
+        res = {}
+        for tail in (0, 1):
+            pr = make(g, rtol=1e-10, precision=prec, coarse_tail=tail)
+            pr.save_velocity()
+            out = []
+            for rep in range(3):  # the third solve runs as graphs
+                pr.restore_velocity()
+                st = pr.Divergence_Projection()
+                assert st["levels"] == nl
+                out.append((st["iters"], pr.get_pressure()))
+            res[tail] = out
+        for (i0, p0), (i1, p1) in zip(res[0], res[1]):
+            assert i0 == i1
+            assert rel(p1, p0) < (1e-12 if prec == MAC_FP64 else 1e-7), rel(p1, p0)

@@ -89,6 +89,11 @@ typedef struct {
   int coarse_tail; /* 1 (default): the last two levels of each V-cycle (the top brick
                       and its at most 8 children) run as one single-workgroup launch
                       with the level in LDS; 0: one launch per step                */
+  double viscosity; /* > 0: the viscous free-surface pressure of Policy_Dirichlet_Boundary
+                      (Algorithms_Solve_Pressure_Dirichlet): every air or solid cell
+                      next to a non-air cell holds p = -sum 2 nu (u_i(c+e_i) - u_i(c))
+                      / h_i * s over its non-air neighbours, which enters the right
+                      side, the written pressure and the velocity update; 0: p = 0 */
 } mac_params;
 
 typedef struct {

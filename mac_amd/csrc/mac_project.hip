@@ -99,6 +99,10 @@ struct mac_ctx {
   double *x = nullptr, *f = nullptr, *r = nullptr, *d[2] = {nullptr, nullptr}, *q = nullptr, *tmpv = nullptr;
   int* glist = nullptr;
   int nglist = 0;
+  // viscous air pressure (viscosity > 0): p_air per cell, and the velocity it
+  // is formed from (nullptr: the field being projected)
+  double* pD = nullptr;
+  const double *ur = nullptr, *vr = nullptr, *wr = nullptr;
   int* dkeys0 = nullptr;  // level-0 shifted chunk keys on the device
   int ext0[3] = {0, 0, 0};
   // coarsest level exact solve
@@ -144,6 +148,7 @@ void free_all(mac_ctx* c) {
   c->solves_on_grid = 0;
   for (void* p : c->allocs) hipFree(p);
   c->allocs.clear();
+  c->pD = nullptr;
   c->L.clear();
   c->have_grid = false;
 }
@@ -544,6 +549,7 @@ int build_grid(mac_ctx* c, int nbricks, const int32_t* keys3, const uint8_t* fla
 int check_params(const mac_params* p) {
   if (p->nu < 1 || p->nu > 8 || p->max_iter < 1 || !(p->rtol > 0)) return fail(MAC_E_ARG, "bad solver parameters");
   if (p->smoother != MAC_SMOOTH_JACOBI && p->smoother != MAC_SMOOTH_CHEBYSHEV) return fail(MAC_E_ARG, "unknown smoother");
+  if (!(p->viscosity >= 0)) return fail(MAC_E_ARG, "viscosity must be >= 0");
   if (p->smoother == MAC_SMOOTH_CHEBYSHEV && !(p->cheb_lower > 0 && p->cheb_lower < 2 && p->cheb_lower_coarse > 0 &&
                                                 p->cheb_lower_coarse < 2))
     return fail(MAC_E_ARG, "cheb_lower and cheb_lower_coarse must lie in (0, 2)");
@@ -937,6 +943,7 @@ void mac_default_params(mac_params* p) {
   p->nu_coarse = 4;
   p->cheb_lower_coarse = 0.1;
   p->coarse_tail = 1;
+  p->viscosity = 0.0;
 }
 
 int mac_create(mac_ctx** out, const mac_params* prm) {
@@ -1076,6 +1083,19 @@ int mac_project(mac_ctx* c, int guess, mac_stats* st) {
                                                     c->f, nullptr, c->partials);
   }
   if (l0.nact > 0) finalize(c, grid_of(l0.nact), OP_FF);
+  const bool visc = c->prm.viscosity > 0;
+  if (visc) {
+    if (!c->pD) CHK(dalloc(c, &c->pD, (size_t)c->nb0 * MAC_BC));
+    k_air_pressure<<<c->nb0, KTHREADS, 0, c->st>>>(l0.nbr, c->flags, c->ur ? c->ur : c->u, c->vr ? c->vr : c->v,
+                                                   c->wr ? c->wr : c->w, inv_h[0], inv_h[1], inv_h[2], c->prm.viscosity, c->pD);
+    if (l0.nact > 0) {
+      k_dirichlet_rhs<<<grid_of(l0.nact), KTHREADS, 0, c->st>>>(l0.nact, l0.meta, l0.nbr, c->flags, l0.code, c->pD, inv_h[0],
+                                                                inv_h[1], inv_h[2], c->f, c->partials);
+      finalize(c, grid_of(l0.nact), OP_FF);
+    }
+    // a guess must not carry the last solve's air pressures into the operator
+    if (guess) k_mask<<<c->nb0, KTHREADS, 0, c->st>>>(c->nb0, l0.code, c->x);
+  }
   HIPCHK(hipGetLastError());
   auto ts0 = std::chrono::steady_clock::now();
   int rc = 0;
@@ -1088,6 +1108,7 @@ int mac_project(mac_ctx* c, int guess, mac_stats* st) {
   }
   auto ts1 = std::chrono::steady_clock::now();
   if (rc != 0 && rc != MAC_E_NOCONV) return rc;
+  if (visc) k_dirichlet_store<<<c->nb0, KTHREADS, 0, c->st>>>(c->nb0, l0.code, c->pD, c->x);
   if (c->nglist > 0) {
     TimedLaunch tl(c, K_GRAD, nd * 57);
     k_gradient<<<c->nglist, KTHREADS, 0, c->st>>>(c->glist, l0.nbr, c->flags, l0.code, c->x, c->u, c->v, c->w, inv_h[0], inv_h[1],

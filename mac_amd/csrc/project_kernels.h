@@ -942,6 +942,93 @@ __global__ void k_apply_A(const int* __restrict__ act, const int* __restrict__ n
 
 // Zero the out-of-domain entries of a host-supplied vector (so that the
 // operator sees the zero-outside-domain convention).
+// ------------------------------------------------ viscous air pressure ----
+// Policy_Dirichlet_Boundary.h with Algorithms_Solve_Pressure_Dirichlet.h.
+// "Air" is Air or Solid (GetIsAir); a Dirichlet boundary cell is an existing
+// air cell with an existing non-air face neighbour.  Its pressure
+//   p_air = - sum over those neighbours (i, s) of 2 nu (u_i(c+e_i) - u_i(c)) / h_i * s
+// (same operation order; the velocity is the real one, and a face of a cell
+// that does not exist reads what its brick holds, 0 for a missing brick).
+// pD gets p_air on Dirichlet boundary cells, 0 elsewhere.
+__device__ __forceinline__ bool cell_valid(const uint8_t* flags, const int* nbr27, int b, int x, int y, int z, size_t& idx) {
+  int cell;
+  const int nb = locate(nbr27, b, x, y, z, cell);
+  if (nb < 0) return false;
+  idx = (size_t)nb * MAC_BC + cell;
+  return (flags[idx] & 0x80u) != 0;
+}
+__device__ __forceinline__ bool mat_air(unsigned f) { return (f & 7u) == 0u || (f & 7u) == 2u; }
+
+__global__ void k_air_pressure(const int* __restrict__ nbr27, const uint8_t* __restrict__ flags, const double* __restrict__ u,
+                               const double* __restrict__ v, const double* __restrict__ w, double ihx, double ihy, double ihz,
+                               double nu, double* __restrict__ pD) {
+#pragma clang fp contract(off)
+  const int b = blockIdx.x;
+  const double* U[3] = {u, v, w};
+  const double ih[3] = {ihx, ihy, ihz};
+  for (int t = threadIdx.x; t < MAC_BC; t += blockDim.x) {
+    const size_t c = (size_t)b * MAC_BC + t;
+    const unsigned f = flags[c];
+    double ret = 0.0;
+    if ((f & 0x80u) && mat_air(f)) {
+      const int x = t & 7, y = (t >> 3) & 7, z = t >> 6;
+      for (int i = 0; i < 3; ++i)
+        for (int s = -1; s <= 1; s += 2) {
+          size_t n;
+          if (!cell_valid(flags, nbr27, b, x + (i == 0) * s, y + (i == 1) * s, z + (i == 2) * s, n) || mat_air(flags[n]))
+            continue;
+          const double v1 = U[i][c];
+          const double v2 = ld_rel(U[i], nbr27, b, x + (i == 0), y + (i == 1), z + (i == 2));
+          ret -= 2.0 * nu * (v2 - v1) * ih[i] * (double)s;
+        }
+    }
+    pD[c] = ret;
+  }
+}
+
+// f -= the Dirichlet term of b (b -= sum of p_air h_i^-2 over the Dirichlet
+// boundary neighbours, f = -b), partial ||f||^2.
+__global__ void __launch_bounds__(KTHREADS) k_dirichlet_rhs(int nact, const int* __restrict__ meta, const int* __restrict__ nbr27,
+                                                            const uint8_t* __restrict__ flags, const uint8_t* __restrict__ code,
+                                                            const double* __restrict__ pD, double ihx, double ihy, double ihz,
+                                                            double* __restrict__ f, double* __restrict__ partials) {
+#pragma clang fp contract(off)
+  __shared__ double sh[16];
+  const int lane = lane_id();
+  const double ih[3] = {ihx, ihy, ihz};
+  double acc = 0.0;
+  FOR_EACH_BRICK(meta, nact, M) {
+    const int b = M[0];
+    for (int z = 0; z < 8; ++z) {
+      const int t = lane + 64 * z;
+      const size_t c = (size_t)b * MAC_BC + t;
+      if (!(code[c] & CODE_DOM)) continue;
+      const int x = t & 7, y = (t >> 3) & 7;
+      double ret = 0.0;
+      for (int i = 0; i < 3; ++i)
+        for (int s = -1; s <= 1; s += 2) {
+          size_t n;
+          if (!cell_valid(flags, nbr27, b, x + (i == 0) * s, y + (i == 1) * s, z + (i == 2) * s, n) || !mat_air(flags[n]))
+            continue;
+          ret += pD[n] * ih[i] * ih[i];
+        }
+      const double fv = f[c] + ret;
+      f[c] = fv;
+      acc += fv * fv;
+    }
+  }
+  write_partial(acc, partials, sh, false);
+}
+
+// Pressure write-back of Divergence_Projection: p_air on the Dirichlet
+// boundary cells (0 on the other cells outside the domain).
+__global__ void k_dirichlet_store(int nb, const uint8_t* __restrict__ code, const double* __restrict__ pD, double* __restrict__ x) {
+  for (int t = threadIdx.x; t < MAC_BC; t += blockDim.x) {
+    const size_t c = (size_t)blockIdx.x * MAC_BC + t;
+    if (!(code[c] & CODE_DOM)) x[c] = pD[c];
+  }
+}
+
 __global__ void k_mask(int nb, const uint8_t* __restrict__ code, double* __restrict__ v) {
   for (int t = threadIdx.x; t < MAC_BC; t += blockDim.x) {
     size_t c = (size_t)blockIdx.x * MAC_BC + t;

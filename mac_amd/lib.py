@@ -39,6 +39,7 @@ class Params(ctypes.Structure):
         ("nu_coarse", ctypes.c_int),
         ("cheb_lower_coarse", ctypes.c_double),
         ("coarse_tail", ctypes.c_int),
+        ("viscosity", ctypes.c_double),
     ]
 
 

@@ -326,6 +326,80 @@ int orc_project(int nb, const int32_t* keys, const uint8_t* flags, const double*
   return it;
 }
 
+// Policy_Dirichlet_Boundary.h: GetIsAir is Air or Solid; a Dirichlet boundary
+// cell is an air cell with a valid non-air neighbour.  p_air follows
+// GetDirichletBoundaryPressure (:78-104) term by term, with the real
+// velocity ur (u(c+e_i) read from the chunk's memory, 0 for an absent chunk).
+namespace {
+bool is_air(const Grid& G, long long c) { return (G.flags[c] & 7) == 0 || (G.flags[c] & 7) == 2; }
+double air_pressure(const Grid& G, long long c, const double* const U[3], const double* inv_h, double nu) {
+  if (!G.exists(c) || !is_air(G, c)) return 0.0;
+  double ret = 0.0;
+  for (int i = 0; i < 3; ++i)
+    for (int s = -1; s <= 1; s += 2) {
+      long long n = G.nbr(c, i, s);
+      if (n < 0 || is_air(G, n)) continue;
+      double v1 = U[i][c];
+      int g[3];
+      G.cell_key((int)(c / NC), (int)(c % NC), g);
+      g[i] += 1;
+      long long np = G.index(g);
+      double v2 = np >= 0 ? U[i][np] : 0.0;
+      ret -= 2 * nu * (v2 - v1) * inv_h[i] * s;
+    }
+  return ret;
+}
+}  // namespace
+
+// Algorithms_Solve_Pressure_Dirichlet::Divergence_Projection (:167-218) with
+// the CSR stand-in solve: b = div - sum of p_air h_i^-2 over the Dirichlet
+// boundary neighbours (:170-180), p = solution in the domain, p_air on the
+// Dirichlet boundary cells, 0 elsewhere (:183-197), then the update.  ur/vr/wr:
+// the real velocity for p_air (NULL: u, v, w as given).
+int orc_project_dirichlet(int nb, const int32_t* keys, const uint8_t* flags, const double* h, double* u, double* v, double* w,
+                          const double* ur, const double* vr, const double* wr, double nu, double* p, double rtol,
+                          double* maxdiv) {
+  double inv_h[3] = {1.0 / h[0], 1.0 / h[1], 1.0 / h[2]};
+  long long ncell = (long long)nb * NC;
+  Grid G(nb, keys, flags);
+  std::vector<double> real[3];
+  const double* U[3] = {ur ? ur : u, vr ? vr : v, wr ? wr : w};
+  for (int i = 0; i < 3; ++i) {
+    real[i].assign(U[i], U[i] + ncell);
+    U[i] = real[i].data();
+  }
+  std::vector<double> pair(ncell);
+  for (long long c = 0; c < ncell; ++c) pair[c] = air_pressure(G, c, U, inv_h, nu);
+  std::vector<int> off(ncell + 1), idx(7 * ncell), row(ncell);
+  std::vector<double> val(7 * ncell), div(ncell);
+  int n = orc_assemble(nb, keys, flags, inv_h, off.data(), idx.data(), val.data(), row.data());
+  orc_divergence(nb, keys, flags, inv_h, u, v, w, div.data());
+  std::vector<double> b(n), x(n, 0.0);
+  for (long long c = 0; c < ncell; ++c) {
+    if (row[c] < 0) continue;
+    double bv = div[c];
+    double ret = 0.0;
+    bool any = false;
+    for (int i = 0; i < 3; ++i)
+      for (int s = -1; s <= 1; s += 2) {
+        long long nn = G.nbr(c, i, s);
+        if (nn < 0 || !is_air(G, nn)) continue;
+        any = true;
+        ret += pair[nn] * inv_h[i] * inv_h[i];
+      }
+    if (any) bv -= ret;
+    b[row[c]] = bv;
+  }
+  int it = n > 0 ? orc_cg_csr(n, off.data(), idx.data(), val.data(), b.data(), x.data(), rtol, 100000) : 0;
+  for (long long c = 0; c < ncell; ++c) p[c] = row[c] >= 0 ? x[row[c]] : pair[c];
+  orc_gradient_update(nb, keys, flags, inv_h, p, u, v, w);
+  orc_divergence(nb, keys, flags, inv_h, u, v, w, div.data());
+  double m = 0;
+  for (long long c = 0; c < ncell; ++c) m = std::max(m, std::fabs(div[c]));
+  if (maxdiv) *maxdiv = m;
+  return it;
+}
+
 // y = A x, matrix-free, with the assembled operator (for parity of the GPU
 // operator); x is read as 0 outside the domain.
 void orc_apply_A(int nb, const int32_t* keys, const uint8_t* flags, const double* inv_h, const double* x, double* y) {

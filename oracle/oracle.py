@@ -154,6 +154,26 @@ def project(grid, rtol=1e-14):
     return {"p": p, "u": u, "v": v, "w": w, "max_div": md.value, "iters": it}
 
 
+def project_dirichlet(grid, viscosity, rtol=1e-14, real=None):
+    """Viscous free-surface projection (Algorithms_Solve_Pressure_Dirichlet
+    with Policy_Dirichlet_Boundary); real = (ur, vr, wr) for the air
+    pressures, default the projected field.  Returns dict(p, u, v, w, max_div, iters)."""
+    keys = _c(grid["keys"], np.int32)
+    flags = _c(grid["flags"], np.uint8)
+    nb = keys.shape[0]
+    u, v, w = (np.array(grid[k], dtype=np.float64, copy=True) for k in ("u", "v", "w"))
+    rr = [None, None, None] if real is None else [_c(a, np.float64) for a in real]
+    p = np.zeros(nb * 512)
+    h = _c(grid["h"], np.float64)
+    md = ctypes.c_double()
+    lib = load()
+    lib.orc_project_dirichlet.argtypes = [ctypes.c_int] + [_P] * 9 + [ctypes.c_double, _P, ctypes.c_double, _P]
+    it = lib.orc_project_dirichlet(nb, _p(keys), _p(flags), _p(h), _p(u), _p(v), _p(w),
+                                   *[None if a is None else _p(a) for a in rr], float(viscosity), _p(p), rtol,
+                                   ctypes.byref(md))
+    return {"p": p, "u": u, "v": v, "w": w, "max_div": md.value, "iters": it}
+
+
 def mgpcg(grid, rtol=1e-8, maxit=500, nu=2, omega=0.9, threads=0, smoother=1, cheb_lower=0.25, nu_coarse=4,
           cheb_lower_coarse=0.1):
     """CPU MG-PCG pressure (the same algorithm as the device solver).

@@ -272,3 +272,28 @@ def test_coarse_tail_matches_launch_per_step(prec):
         for (i0, p0), (i1, p1) in zip(res[0], res[1]):
             assert i0 == i1
             assert rel(p1, p0) < (1e-12 if prec == MAC_FP64 else 1e-7), rel(p1, p0)
+
+
+@pytest.mark.parametrize("name", ["rand_2x2x2", "rand_dropped_bricks", "rand_dense_fluid", "spheres_32"])
+@pytest.mark.parametrize("prec", [MAC_FP64, MAC_MIXED])
+def test_viscous_air_pressure_vs_oracle(name, prec):
+    """Algorithms_Solve_Pressure_Dirichlet with Policy_Dirichlet_Boundary on
+    the device: the air pressures, the right side, the written pressure
+    (air pressures included) and the updated velocities against the oracle;
+    two solves in a row (the second reuses the context)."""
+    g = small_spheres(32, 2) if name == "spheres_32" else GRIDS[name]()
+    nu = 0.013
+    ref = O.project_dirichlet(g, nu, rtol=1e-13)
+    tol = 1e-8 if prec == MAC_FP64 else 1e-6
+    pr = make(g, rtol=1e-12 if prec == MAC_FP64 else 1e-10, precision=prec, viscosity=nu)
+    pr.save_velocity()
+    for rep in range(2):
+        pr.restore_velocity()
+        pr.Divergence_Projection(guess=rep == 1)
+        p = pr.get_pressure()
+        u = np.concatenate(pr.get_velocity())
+        assert rel(p, ref["p"]) < tol, (rep, rel(p, ref["p"]))
+        assert rel(u, np.concatenate([ref[k] for k in ("u", "v", "w")])) < tol
+        air = ((g["flags"] & 0x80) > 0) & ~dom_mask(g) & (ref["p"] != 0)
+        assert air.any()
+        assert np.allclose(p[air], ref["p"][air], rtol=1e-13, atol=0)

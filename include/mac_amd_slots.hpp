@@ -122,12 +122,16 @@ class Algorithms_Solve_Pressure_HIP : public Policy {
  public:
   mac_stats last_stats{};
 
-  Algorithms_Solve_Pressure_HIP(DataType data, const Policy& pol, int precision = MAC_FP64, double rtol = 1e-8)
+  // viscosity > 0: the viscous variant, Algorithms_Solve_Pressure_Dirichlet
+  // (pass the grid's m_viscosity)
+  Algorithms_Solve_Pressure_HIP(DataType data, const Policy& pol, int precision = MAC_FP64, double rtol = 1e-8,
+                                double viscosity = 0.0)
       : Policy(pol), m_grid(data.m_data.GetGridData()) {
     mac_default_params(&m_prm);
     for (int i = 0; i < 3; ++i) m_prm.h[i] = 1.0 / m_grid.m_h.GetRef_Inv().Get(i + 1);
     m_prm.rtol = rtol;
     m_prm.precision = precision;
+    m_prm.viscosity = viscosity;
   }
   Algorithms_Solve_Pressure_HIP(const Algorithms_Solve_Pressure_HIP&) = delete;
   Algorithms_Solve_Pressure_HIP& operator=(const Algorithms_Solve_Pressure_HIP&) = delete;

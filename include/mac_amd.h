@@ -207,7 +207,12 @@ typedef struct {
   double dt_min, dt_max;/* Policy_CheckDT.h:16-29                                      */
   int layer_max;        /* Policy_Layer_Max: air layers built around the fluid         */
   int integrator;       /* 0 = RK2 midpoint (Algorithms_RungeKutta_RK2.h:20-64),
-                           1 = RK2-TVD (Algorithms_RungeKutta_RK2_TVD.h:20-66)        */
+                           1 = RK2-TVD (Algorithms_RungeKutta_RK2_TVD.h:20-66),
+                           2 = RK4 "all" (Algorithms_RungeKutta_all.h:35-156): four
+                           stages, each with the cell-type update from the stage's
+                           particle positions, velocity extrapolation, forces, the
+                           projection of the acceleration and its extrapolation;
+                           particles advance inside the stages                        */
   int particle_rk;      /* 2 = RK2 midpoint, 4 = classical RK4
                            (Policy_Advance_ODE_RungeKutta.h:15-27)                    */
   int particles_per_inflow_cell; /* seeded-jitter fill of empty inflow cells           */
@@ -223,6 +228,9 @@ typedef struct {
                            position[floor_axis] < floor_value are erased at the start of
                            the cell-type update; -1 = none (Jet3: axis y, value 0)    */
   double floor_value;
+  int fill_cube_n;      /* > 0: new particles on the (2n+1)^3 lattice of
+                           Policy_Add_Particle_Cube (Jet3: n = 1) instead of the
+                           particles_per_inflow_cell jitter                           */
   mac_params proj;      /* projection solver settings                                  */
 } mac_sim_params;
 

@@ -18,6 +18,10 @@ MAC_HIDDEN int mac_internal_build(mac_ctx* c, int nbricks, const int32_t* keys3_
 // guess = 2 from the context's pressure as it stands (a learned guess).
 MAC_HIDDEN int mac_internal_project(mac_ctx* c, double* u, double* v, double* w, double* p, int guess, mac_stats* st);
 MAC_HIDDEN int mac_internal_set_error(int code, const char* msg);
+// Device velocity the viscous air pressures are formed from (the real speed
+// of Speed_GetRef(-1, false) when an acceleration is projected); NULL: the
+// projected field itself.
+MAC_HIDDEN void mac_internal_set_real_velocity(mac_ctx* c, const double* u, const double* v, const double* w);
 
 // Device view of a projection context's level-0 grid (valid until the next
 // build).

@@ -1196,6 +1196,11 @@ int mac_max_divergence(mac_ctx* c, double* out) {
 }
 
 MAC_HIDDEN int mac_internal_set_error(int code, const char* msg) { return fail(code, msg); }
+MAC_HIDDEN void mac_internal_set_real_velocity(mac_ctx* c, const double* u, const double* v, const double* w) {
+  c->ur = u;
+  c->vr = v;
+  c->wr = w;
+}
 
 MAC_HIDDEN int mac_internal_build(mac_ctx* c, int nbricks, const int32_t* keys3, const uint8_t* flags) {
   if (!c || nbricks <= 0) return fail(MAC_E_ARG, "bad build arguments");

@@ -62,7 +62,9 @@ struct mac_sim {
   uint8_t* flags = nullptr;
   int8_t* layer = nullptr;
   int16_t* depth = nullptr;  // Data_Grid_Depth_Empty: -1 = empty
-  double* S[2][3] = {{nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr}};
+  // speed slots 0..2 (Speed_GetRef(i) of the reference's 3-slot chunks; slot 2
+  // is used by the RK4 "all" integrator only)
+  double* S[3][3] = {{nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr}};
   double* A[3] = {nullptr, nullptr, nullptr};
   double* P = nullptr;
   double* Pprev = nullptr;  // the pressure the last projection started from
@@ -72,6 +74,10 @@ struct mac_sim {
   int hcap = 0;
   // particles (SoA, capacity capp) and the second buffer for compaction
   double *px = nullptr, *py = nullptr, *pz = nullptr, *qx = nullptr, *qy = nullptr, *qz = nullptr;
+  // position slots 1 and 2 of the RK4 "all" integrator (Data_Particle m_pos[1..2])
+  double* P1[3] = {nullptr, nullptr, nullptr};
+  double* P2[3] = {nullptr, nullptr, nullptr};
+  int* d_dest = nullptr;  // per particle: its place after an erasing pass (-1: erased)
   int np = 0, capp = 0;
   // scratch
   int* d_int = nullptr;  // [0] particle count, [1] missing count, [2] overflow, [3] kept
@@ -96,7 +102,7 @@ void sim_free_bricks(mac_sim* s) {
   hipFree(s->flags);
   hipFree(s->layer);
   hipFree(s->depth);
-  for (int a = 0; a < 2; ++a)
+  for (int a = 0; a < 3; ++a)
     for (int i = 0; i < 3; ++i) hipFree(s->S[a][i]);
   for (int i = 0; i < 3; ++i) hipFree(s->A[i]);
   hipFree(s->P);
@@ -129,14 +135,14 @@ int sim_set_bricks(mac_sim* s, const std::vector<Key3>& nk) {
   uint8_t* flags;
   int8_t* layer;
   int16_t* depth;
-  double* S[2][3];
+  double* S[3][3];
   double* A[3];
   double *P, *Pprev;
   SCHK(hipMalloc(&d_keys, sizeof(int) * 3 * std::max(nbn, 1)));
   SCHK(hipMalloc(&flags, cells));
   SCHK(hipMalloc(&layer, cells));
   SCHK(hipMalloc(&depth, cells * sizeof(int16_t)));
-  for (int a = 0; a < 2; ++a)
+  for (int a = 0; a < 3; ++a)
     for (int i = 0; i < 3; ++i) SCHK(hipMalloc(&S[a][i], cells * sizeof(double)));
   for (int i = 0; i < 3; ++i) SCHK(hipMalloc(&A[i], cells * sizeof(double)));
   SCHK(hipMalloc(&P, cells * sizeof(double)));
@@ -155,7 +161,7 @@ int sim_set_bricks(mac_sim* s, const std::vector<Key3>& nk) {
     k_relayout_u8<<<nbn, 256, 0, s->st>>>(nbn, d_from, s->flags, flags, (uint8_t)0);
     k_relayout_u8<<<nbn, 256, 0, s->st>>>(nbn, d_from, (const uint8_t*)s->layer, (uint8_t*)layer, (uint8_t)0xFF);
     k_relayout_i16<<<nbn, 256, 0, s->st>>>(nbn, d_from, s->depth, depth, (int16_t)-1);
-    for (int a = 0; a < 2; ++a)
+    for (int a = 0; a < 3; ++a)
       for (int i = 0; i < 3; ++i) k_relayout_f64<<<nbn, 256, 0, s->st>>>(nbn, d_from, s->S[a][i], S[a][i]);
     for (int i = 0; i < 3; ++i) k_relayout_f64<<<nbn, 256, 0, s->st>>>(nbn, d_from, s->A[i], A[i]);
     k_relayout_f64<<<nbn, 256, 0, s->st>>>(nbn, d_from, s->P, P);
@@ -167,7 +173,7 @@ int sim_set_bricks(mac_sim* s, const std::vector<Key3>& nk) {
   s->flags = flags;
   s->layer = layer;
   s->depth = depth;
-  for (int a = 0; a < 2; ++a)
+  for (int a = 0; a < 3; ++a)
     for (int i = 0; i < 3; ++i) s->S[a][i] = S[a][i];
   for (int i = 0; i < 3; ++i) s->A[i] = A[i];
   s->P = P;
@@ -195,7 +201,10 @@ int sim_set_bricks(mac_sim* s, const std::vector<Key3>& nk) {
 
 int sim_alloc_particles(mac_sim* s, int cap) {
   s->capp = std::max(cap, 1);
-  for (double** p : {&s->px, &s->py, &s->pz, &s->qx, &s->qy, &s->qz}) SCHK(hipMalloc(p, sizeof(double) * s->capp));
+  for (double** p : {&s->px, &s->py, &s->pz, &s->qx, &s->qy, &s->qz, &s->P1[0], &s->P1[1], &s->P1[2], &s->P2[0], &s->P2[1],
+                     &s->P2[2]})
+    SCHK(hipMalloc(p, sizeof(double) * s->capp));
+  SCHK(hipMalloc(&s->d_dest, sizeof(int) * s->capp));
   return 0;
 }
 
@@ -220,10 +229,16 @@ int sim_depth_search(mac_sim* s) {
   }
 }
 
-int sim_topology(mac_sim* s) {
+// full: the step's cell-type chain (particles created, floor applied);
+// otherwise the chain inside an RK4 stage (Algorithms_Create_Interior_Depth:
+// no particles created, no floor), with the particle keys taken from
+// position slot ks (0 or 2) and all three slots compacted together.
+int sim_topology(mac_sim* s, bool full = true, int ks = 0) {
   const mac_sim_params& P = s->prm;
   SimIH ih{{1.0 / P.h[0], 1.0 / P.h[1], 1.0 / P.h[2]}};
   const bool dmode = P.depth_interior > 0;
+  double** kp[3] = {ks == 2 ? &s->P2[0] : &s->px, ks == 2 ? &s->P2[1] : &s->py, ks == 2 ? &s->P2[2] : &s->pz};
+  const int floor_axis = full ? P.floor_axis : -1;
   // Algorithms_Initialize_MacCell / Algorithms_Initialize_Layer_And_Depth
   if (s->nb > 0 && dmode)
     k_reset_layer_depth<<<1024, 256, 0, s->st>>>((size_t)s->nb * MAC_BC, s->flags, s->layer, s->depth);
@@ -238,8 +253,8 @@ int sim_topology(mac_sim* s) {
     k_inflow_bricks<<<s->nb, 256, 0, s->st>>>(s->nb, s->flags, s->d_bflag, dmode ? 1 : 0);
   }
   if (s->np > 0)
-    k_particle_bricks<<<(s->np + 255) / 256, 256, 0, s->st>>>(s->np, s->px, s->py, s->pz, ih, s->hk, s->hv, s->hcap, s->d_bflag,
-                                                             s->d_missing, s->d_int + 1, s->maxmissing, P.floor_axis,
+    k_particle_bricks<<<(s->np + 255) / 256, 256, 0, s->st>>>(s->np, *kp[0], *kp[1], *kp[2], ih, s->hk, s->hv, s->hcap, s->d_bflag,
+                                                             s->d_missing, s->d_int + 1, s->maxmissing, floor_axis,
                                                              P.floor_value);
   SCHK(hipGetLastError());
   int cnt[4];
@@ -277,26 +292,43 @@ int sim_topology(mac_sim* s) {
   // Algorithms_Layer_Initial_With_Particle, or Algorithm_DelParticle +
   // Algorithms_Layer_Initial_With_Particle_Depth (particles erased: the kept
   // ones are appended to the second buffer, whose count starts the fill)
-  const bool erase = dmode || P.floor_axis >= 0;
+  const bool erase = dmode || floor_axis >= 0;
   if (s->np > 0 && erase) {
     SCHK(hipMemsetAsync(s->d_int, 0, sizeof(int), s->st));
-    k_mark_particles_keep<<<(s->np + 255) / 256, 256, 0, s->st>>>(s->np, s->px, s->py, s->pz, ih, s->hk, s->hv, s->hcap, s->flags,
+    k_mark_particles_keep<<<(s->np + 255) / 256, 256, 0, s->st>>>(s->np, *kp[0], *kp[1], *kp[2], ih, s->hk, s->hv, s->hcap, s->flags,
                                                                  s->layer, dmode ? s->depth : nullptr, P.depth_remove,
-                                                                 P.floor_axis, P.floor_value, s->qx, s->qy, s->qz, s->d_int);
-    std::swap(s->px, s->qx);
-    std::swap(s->py, s->qy);
-    std::swap(s->pz, s->qz);
+                                                                 floor_axis, P.floor_value, s->qx, s->qy, s->qz, s->d_int,
+                                                                 full ? nullptr : s->d_dest);
+    std::swap(*kp[0], s->qx);
+    std::swap(*kp[1], s->qy);
+    std::swap(*kp[2], s->qz);
+    if (!full) {
+      // the other two position slots follow the same erasures
+      double** other[2][3] = {{&s->px, &s->py, &s->pz}, {&s->P1[0], &s->P1[1], &s->P1[2]}};
+      if (ks == 0) {
+        other[0][0] = &s->P1[0], other[0][1] = &s->P1[1], other[0][2] = &s->P1[2];
+        other[1][0] = &s->P2[0], other[1][1] = &s->P2[1], other[1][2] = &s->P2[2];
+      }
+      for (auto& o : other) {
+        k_scatter_particles<<<(s->np + 255) / 256, 256, 0, s->st>>>(s->np, s->d_dest, *o[0], *o[1], *o[2], s->qx, s->qy, s->qz);
+        std::swap(*o[0], s->qx);
+        std::swap(*o[1], s->qy);
+        std::swap(*o[2], s->qz);
+      }
+    }
   } else {
     if (s->np > 0)
-      k_mark_particles<<<(s->np + 255) / 256, 256, 0, s->st>>>(s->np, s->px, s->py, s->pz, ih, s->hk, s->hv, s->hcap, s->flags,
+      k_mark_particles<<<(s->np + 255) / 256, 256, 0, s->st>>>(s->np, *kp[0], *kp[1], *kp[2], ih, s->hk, s->hv, s->hcap, s->flags,
                                                               s->layer);
     SCHK(hipMemcpyAsync(s->d_int, &s->np, sizeof(int), hipMemcpyHostToDevice, s->st));
   }
-  // Algorithms_Create_Fluid_Particle(_Depth)
-  if (s->nb > 0 && (P.particles_per_inflow_cell > 0 || dmode))
+  // Algorithms_Create_Fluid_Particle(_Depth), or inside a stage
+  // Algorithms_Create_Interior_Depth (no particles)
+  if (s->nb > 0 && (P.particles_per_inflow_cell > 0 || P.fill_cube_n > 0 || dmode))
     k_inflow_fill<<<s->nb, 256, 0, s->st>>>(s->nb, s->d_keys, s->flags, s->layer, P.h[0], P.h[1], P.h[2],
                                              P.particles_per_inflow_cell, P.seed, s->tick, s->px, s->py, s->pz, s->d_int,
-                                             s->capp, s->d_int + 2, dmode ? s->depth : nullptr, P.depth_remove);
+                                             s->capp, s->d_int + 2, dmode ? s->depth : nullptr, P.depth_remove, full ? 1 : 0,
+                                             P.fill_cube_n);
   // Algorithms_Update_CellType_Layer(_Depth): surface depth before the layers
   if (s->nb > 0 && dmode) k_depth_surface<<<s->nb, 256, 0, s->st>>>(s->nb, s->nbr, s->flags, s->layer, s->depth);
   if (s->nb > 0)
@@ -312,7 +344,7 @@ int sim_topology(mac_sim* s) {
   SCHK(hipStreamSynchronize(s->st));
   if (cnt[2]) return mac_internal_set_error(MAC_E_ARG, "particle pool exhausted (max_particles)");
   s->np = cnt[0];
-  s->tick++;
+  if (full) s->tick++;
   // free the bricks whose last cell was erased (ChunkHashTable.tpp:133-161)
   std::vector<Key3> keep;
   keep.reserve(s->nb);
@@ -338,7 +370,9 @@ int sim_ensure_projection(mac_sim* s) {
   return 0;
 }
 
-int sim_forces(mac_sim* s, int slot) {
+// rk4: the RK4 "all" stages -- forces added to the acceleration held, in
+// Jet3's order (gravity, convection, viscosity)
+int sim_forces(mac_sim* s, int slot, bool rk4 = false) {
   if (s->nb == 0) return 0;
   const mac_sim_params& P = s->prm;
   SimForce F;
@@ -347,6 +381,8 @@ int sim_forces(mac_sim* s, int slot) {
     F.g[i] = P.gravity[i];
   }
   F.nu = P.viscosity;
+  F.accumulate = rk4 ? 1 : 0;
+  F.conv_first = rk4 ? 1 : 0;
   k_forces<<<s->nb, 256, 0, s->st>>>(s->nb, s->nbr, s->flags, s->layer, s->S[slot][0], s->S[slot][1], s->S[slot][2], s->A[0],
                                       s->A[1], s->A[2], F);
   SCHK(hipGetLastError());
@@ -413,6 +449,8 @@ int mac_sim_create(mac_sim** out, const mac_sim_params* p) {
   if (!out || !p) return mac_internal_set_error(MAC_E_ARG, "null argument");
   *out = nullptr;
   if (p->layer_max < 1 || p->layer_max > 7) return mac_internal_set_error(MAC_E_ARG, "layer_max must be in 1..7");
+  if (p->integrator < 0 || p->integrator > 2) return mac_internal_set_error(MAC_E_ARG, "integrator must be 0, 1 or 2");
+  if (p->fill_cube_n < 0 || p->fill_cube_n > 3) return mac_internal_set_error(MAC_E_ARG, "fill_cube_n must be in 0..3");
   if (p->depth_interior < 0 || p->depth_interior > 32767) return mac_internal_set_error(MAC_E_ARG, "depth_interior must be in 0..32767");
   if (p->floor_axis < -1 || p->floor_axis > 2) return mac_internal_set_error(MAC_E_ARG, "floor_axis must be -1, 0, 1 or 2");
   for (int i = 0; i < 3; ++i)
@@ -445,7 +483,8 @@ void mac_sim_destroy(mac_sim* s) {
   if (!s) return;
   if (s->st) hipStreamSynchronize(s->st);
   sim_free_bricks(s);
-  for (double* p : {s->px, s->py, s->pz, s->qx, s->qy, s->qz}) hipFree(p);
+  for (double* p : {s->px, s->py, s->pz, s->qx, s->qy, s->qz, s->P1[0], s->P1[1], s->P1[2], s->P2[0], s->P2[1], s->P2[2]}) hipFree(p);
+  hipFree(s->d_dest);
   hipFree(s->d_int);
   hipFree(s->d_any);
   hipFree(s->d_missing);
@@ -492,6 +531,7 @@ int mac_sim_upload(mac_sim* s, int nbricks, const int32_t* keys3, const uint8_t*
       }
       SCHK(hipMemcpy(s->S[0][i], buf.data(), cells * sizeof(double), hipMemcpyHostToDevice));
       SCHK(hipMemcpy(s->S[1][i], buf.data(), cells * sizeof(double), hipMemcpyHostToDevice));
+      SCHK(hipMemcpy(s->S[2][i], buf.data(), cells * sizeof(double), hipMemcpyHostToDevice));
       SCHK(hipMemset(s->A[i], 0, cells * sizeof(double)));
     }
     SCHK(hipMemset(s->P, 0, cells * sizeof(double)));
@@ -659,8 +699,9 @@ int mac_sim_interpolate(mac_sim* s, int n, const double* pos, double* vel) {
   return 0;
 }
 
-int mac_sim_extrapolate(mac_sim* s, int slot) {
-  if (!s || slot < 0 || slot > 1) return mac_internal_set_error(MAC_E_ARG, "bad arguments");
+namespace {
+// Algorithms_Extrapolate::Do on the face field (u, v, w) of the current bricks.
+int sim_extrapolate_fields(mac_sim* s, double* u, double* v, double* w) {
   if (s->nb == 0) return 0;
   const size_t cells = (size_t)s->nb * MAC_BC;
   uint8_t* fr = nullptr;
@@ -681,8 +722,7 @@ int mac_sim_extrapolate(mac_sim* s, int slot) {
       rc = mac_internal_set_error(MAC_E_HIP, "hipMemsetAsync failed");
       break;
     }
-    k_extrapolate<<<s->nb, 256, 0, s->st>>>(s->nb, s->nbr, s->flags, s->layer, s->S[slot][0], s->S[slot][1], s->S[slot][2], front, next,
-                                            dany, lay);
+    k_extrapolate<<<s->nb, 256, 0, s->st>>>(s->nb, s->nbr, s->flags, s->layer, u, v, w, front, next, dany, lay);
     if (hipGetLastError() != hipSuccess || hipMemcpyAsync(&any, dany, sizeof(int), hipMemcpyDeviceToHost, s->st) != hipSuccess ||
         hipStreamSynchronize(s->st) != hipSuccess) {
       rc = mac_internal_set_error(MAC_E_HIP, "extrapolation launch failed");
@@ -695,6 +735,83 @@ int mac_sim_extrapolate(mac_sim* s, int slot) {
   hipFree(fr);
   hipFree(dany);
   return rc;
+}
+
+// Algorithms_RungeKutta_all::Do (:35-156) for one step of size dt; the cell
+// types are those of the step's own cell-type update.  Each stage runs the
+// reference's Policy::Do of test/Jet3.cpp:585-650 -- the cell-type chain of
+// m_alg_init2 from the stage's particle positions (slot 0 in stage 1, slot 2
+// after) with the extrapolation of the stage's speed slot, gravity,
+// convection and viscosity, the projection of the acceleration
+// (Algorithms_Solve_Pressure_Force; the viscous air pressures read the stage's
+// real speed) and the extrapolation of the acceleration
+// (Algorithms_Accel_To_Speed) -- then the grid and particle combinations.
+int sim_rk4all(mac_sim* s, double dt, mac_sim_stats* stats) {
+  mac_stats st;
+  int iters[2] = {0, 0};
+  double maxdiv = 0.0;
+  long long nfluid = 0;
+  // Acceleration_GetRef().SetZero(true) on the cells that exist (before the
+  // first stage and after each of the first three)
+  auto zero_accel = [&]() {
+    if (s->nb > 0) k_zero_accel_existing<<<s->nb, 256, 0, s->st>>>(s->nb, s->flags, s->A[0], s->A[1], s->A[2]);
+  };
+  zero_accel();
+  for (int stage = 1; stage <= 4; ++stage) {
+    const int ks = stage == 1 ? 0 : 2;
+    SRC(sim_topology(s, false, ks));
+    if (s->nb == 0) return 0;
+    SRC(sim_extrapolate_fields(s, s->S[ks][0], s->S[ks][1], s->S[ks][2]));
+    SRC(sim_forces(s, ks, true));
+    SCHK(hipStreamSynchronize(s->st));
+    SRC(sim_ensure_projection(s));
+    mac_internal_set_real_velocity(s->proj, s->S[ks][0], s->S[ks][1], s->S[ks][2]);
+    int rc = mac_internal_project(s->proj, s->A[0], s->A[1], s->A[2], s->P, 0, &st);
+    mac_internal_set_real_velocity(s->proj, nullptr, nullptr, nullptr);
+    if (rc) return rc;
+    iters[stage > 2] += st.iters;
+    maxdiv = st.max_div;
+    nfluid = st.n_unknowns;
+    SRC(sim_extrapolate_fields(s, s->A[0], s->A[1], s->A[2]));
+    k_rk4_combine<<<s->nb, 256, 0, s->st>>>(s->nb, s->flags, stage, dt, s->S[0][0], s->S[0][1], s->S[0][2], s->S[1][0], s->S[1][1],
+                                            s->S[1][2], s->S[2][0], s->S[2][1], s->S[2][2], s->A[0], s->A[1], s->A[2]);
+    if (s->np > 0)
+      k_rk4_particles<<<(s->np + 255) / 256, 256, 0, s->st>>>(s->np, stage, dt, s->px, s->py, s->pz, s->P1[0], s->P1[1], s->P1[2],
+                                                             s->P2[0], s->P2[1], s->P2[2], sim_grid(s, ks));
+    if (stage < 4) zero_accel();
+    SCHK(hipGetLastError());
+  }
+  if (stats) {
+    stats->pcg_iters[0] = iters[0];
+    stats->pcg_iters[1] = iters[1];
+    stats->max_div = maxdiv;
+    stats->nfluid = nfluid;
+  }
+  return 0;
+}
+
+// Particles leaving [box_lo, box_hi) are deleted (slot 0).
+int sim_box_delete(mac_sim* s) {
+  const mac_sim_params& P = s->prm;
+  if (s->np == 0 || (P.box_lo[0] == P.box_hi[0] && P.box_lo[1] == P.box_hi[1] && P.box_lo[2] == P.box_hi[2])) return 0;
+  SCHK(hipMemsetAsync(s->d_int + 3, 0, sizeof(int), s->st));
+  k_box_delete<<<(s->np + 255) / 256, 256, 0, s->st>>>(s->np, s->px, s->py, s->pz, s->qx, s->qy, s->qz, s->d_int + 3, P.box_lo[0],
+                                                        P.box_lo[1], P.box_lo[2], P.box_hi[0], P.box_hi[1], P.box_hi[2]);
+  SCHK(hipGetLastError());
+  int kept = 0;
+  SCHK(hipMemcpyAsync(&kept, s->d_int + 3, sizeof(int), hipMemcpyDeviceToHost, s->st));
+  SCHK(hipStreamSynchronize(s->st));
+  std::swap(s->px, s->qx);
+  std::swap(s->py, s->qy);
+  std::swap(s->pz, s->qz);
+  s->np = kept;
+  return 0;
+}
+}  // namespace
+
+int mac_sim_extrapolate(mac_sim* s, int slot) {
+  if (!s || slot < 0 || slot > 2) return mac_internal_set_error(MAC_E_ARG, "bad arguments");
+  return sim_extrapolate_fields(s, s->S[slot][0], s->S[slot][1], s->S[slot][2]);
 }
 
 int mac_sim_set_guess(mac_sim* s, int mode, mac_unet* net, float out_scale) {
@@ -720,12 +837,25 @@ int mac_sim_step(mac_sim* s, mac_sim_stats* stats) {
   SRC(sim_topology(s));
   auto t1 = std::chrono::steady_clock::now();
   double dt;
-  SRC(mac_sim_compute_dt(s, &dt));
-  local.dt = dt;
-  SRC(mac_sim_advance(s, dt, &local));
-  auto t2 = std::chrono::steady_clock::now();
-  SRC(mac_sim_move_particles(s, dt));
-  auto t3 = std::chrono::steady_clock::now();
+  std::chrono::steady_clock::time_point t2, t3;
+  if (s->prm.integrator == 2) {
+    // the init chain of test/Jet3.cpp:545-547 ends with the time step and the
+    // extrapolation of speed slot 0; the particles move inside the stages
+    SRC(mac_sim_compute_dt(s, &dt));
+    local.dt = dt;
+    SRC(sim_extrapolate_fields(s, s->S[0][0], s->S[0][1], s->S[0][2]));
+    SRC(sim_rk4all(s, dt, &local));
+    t2 = std::chrono::steady_clock::now();
+    SRC(sim_box_delete(s));
+    t3 = std::chrono::steady_clock::now();
+  } else {
+    SRC(mac_sim_compute_dt(s, &dt));
+    local.dt = dt;
+    SRC(mac_sim_advance(s, dt, &local));
+    t2 = std::chrono::steady_clock::now();
+    SRC(mac_sim_move_particles(s, dt));
+    t3 = std::chrono::steady_clock::now();
+  }
   local.nbricks = s->nb;
   local.nparticles = s->np;
   local.ms_topology = std::chrono::duration<double, std::milli>(t1 - t0).count();

@@ -216,7 +216,7 @@ __global__ void k_mark_particles_keep(int np, const double* __restrict__ px, con
                                       const int* __restrict__ hv, int cap, uint8_t* __restrict__ flags, int8_t* __restrict__ layer,
                                       const int16_t* __restrict__ depth, int depth_remove, int floor_axis, double floor_value,
                                       double* __restrict__ qx, double* __restrict__ qy, double* __restrict__ qz,
-                                      int* __restrict__ nkept) {
+                                      int* __restrict__ nkept, int* __restrict__ dest) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= np) return;
   const double x = px[i], y = py[i], z = pz[i];
@@ -240,11 +240,26 @@ __global__ void k_mark_particles_keep(int np, const double* __restrict__ px, con
     }
   }
   const int slot = wave_append(keep, nkept);
+  if (dest) dest[i] = slot;
   if (slot >= 0) {
     qx[slot] = x;
     qy[slot] = y;
     qz[slot] = z;
   }
+}
+
+// The other position slots of the particles k_mark_particles_keep kept, to
+// the places it gave them.
+__global__ void k_scatter_particles(int np, const int* __restrict__ dest, const double* __restrict__ sx,
+                                    const double* __restrict__ sy, const double* __restrict__ sz, double* __restrict__ qx,
+                                    double* __restrict__ qy, double* __restrict__ qz) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= np) return;
+  const int d = dest[i];
+  if (d < 0) return;
+  qx[d] = sx[i];
+  qy[d] = sy[i];
+  qz[d] = sz[i];
 }
 
 // Start of Algorithms_Update_CellType_Layer_Depth::Do (:31-56): depth := empty;
@@ -306,23 +321,27 @@ __global__ void k_depth_round(int nb, const int* __restrict__ nbr27, uint8_t* __
 }
 
 // Algorithms_Create_Fluid_Particle::Do with Policy_Is_Inbound_Filling_Layer_CellType
-// (:16-19): an existing Inflow cell with an empty layer gets `ppc` particles
-// at seeded jitter inside the cell ((key + U - 1/2) h per axis), layer 0 and
-// SetFluid.
+// (:16-19): an existing Inflow cell with an empty layer gets particles, layer
+// 0 and SetFluid.  The particles are `ppc` at seeded jitter inside the cell
+// ((key + U - 1/2) h per axis), or with cube_n > 0 the (2n+1)^3 lattice of
+// Policy_Add_Particle_Cube.
 // With depth (Algorithms_Create_Fluid_Particle_Depth::Do, :20-47): an Interior
 // cell with an empty layer and depth <= depth_remove also gets particles and
 // layer 0 (no SetFluid); any other Interior cell gets layer 0.
+// create == 0 is Algorithms_Create_Interior_Depth (the same cell updates,
+// no particles), run inside the RK4 stages.
 __global__ void k_inflow_fill(int nb, const int* __restrict__ keys3, uint8_t* __restrict__ flags, int8_t* __restrict__ layer,
                               double h0, double h1, double h2, int ppc, uint64_t seed, uint64_t tick, double* __restrict__ px,
                               double* __restrict__ py, double* __restrict__ pz, int* __restrict__ np, int maxp,
-                              int* __restrict__ overflow, const int16_t* __restrict__ depth, int depth_remove) {
+                              int* __restrict__ overflow, const int16_t* __restrict__ depth, int depth_remove, int create,
+                              int cube_n) {
 #pragma clang fp contract(off)
   int b = blockIdx.x;
   for (int t = threadIdx.x; t < MAC_BC; t += blockDim.x) {
     size_t c = (size_t)b * MAC_BC + t;
     unsigned f = flags[c];
     if (!(f & F_EXISTS)) continue;
-    const bool inflow = ppc > 0 && (f & 7u) == 3u && layer[c] == -1;
+    const bool inflow = (ppc > 0 || cube_n > 0) && (f & 7u) == 3u && layer[c] == -1;
     if (!inflow) {
       if (!depth || (f & 7u) != 4u) continue;
       const bool refill = layer[c] == -1 && depth[c] <= depth_remove;
@@ -331,17 +350,35 @@ __global__ void k_inflow_fill(int nb, const int* __restrict__ keys3, uint8_t* __
     }
     int kx = keys3[3 * b] * 8 + (t & 7), ky = keys3[3 * b + 1] * 8 + ((t >> 3) & 7), kz = keys3[3 * b + 2] * 8 + (t >> 6);
     uint64_t ck = sim_pack(kx, ky, kz);
-    int base = atomicAdd(np, ppc);
-    for (int k = 0; k < ppc; ++k) {
-      if (base + k >= maxp) {
-        *overflow = 1;
-        break;
+    if (cube_n > 0 && create) {
+      // Policy_Add_Particle_Cube::AddParticle (:29-59): (2n+1)^3 particles at
+      // (k + j dh) h, j = -n..n, dh = 1/(2n+1), x fastest
+      const int m = 2 * cube_n + 1, cnt = m * m * m;
+      const double dh = 1. / (2 * cube_n + 1);
+      const int base = atomicAdd(np, cnt);
+      for (int k = 0; k < cnt; ++k) {
+        if (base + k >= maxp) {
+          *overflow = 1;
+          break;
+        }
+        const int jx = k % m - cube_n, jy = (k / m) % m - cube_n, jz = k / (m * m) - cube_n;
+        px[base + k] = ((double)kx + jx * dh) * h0;
+        py[base + k] = ((double)ky + jy * dh) * h1;
+        pz[base + k] = ((double)kz + jz * dh) * h2;
       }
-      double ux = sim_uniform(seed, tick, ck, 3 * k), uy = sim_uniform(seed, tick, ck, 3 * k + 1),
-             uz = sim_uniform(seed, tick, ck, 3 * k + 2);
-      px[base + k] = ((double)kx + (ux - 0.5)) * h0;
-      py[base + k] = ((double)ky + (uy - 0.5)) * h1;
-      pz[base + k] = ((double)kz + (uz - 0.5)) * h2;
+    } else if (create) {
+      int base = atomicAdd(np, ppc);
+      for (int k = 0; k < ppc; ++k) {
+        if (base + k >= maxp) {
+          *overflow = 1;
+          break;
+        }
+        double ux = sim_uniform(seed, tick, ck, 3 * k), uy = sim_uniform(seed, tick, ck, 3 * k + 1),
+               uz = sim_uniform(seed, tick, ck, 3 * k + 2);
+        px[base + k] = ((double)kx + (ux - 0.5)) * h0;
+        py[base + k] = ((double)ky + (uy - 0.5)) * h1;
+        pz[base + k] = ((double)kz + (uz - 0.5)) * h2;
+      }
     }
     layer[c] = 0;
     if (inflow) flags[c] = (uint8_t)((f & ~7u) | mat_set_fluid(f & 7u));
@@ -479,6 +516,8 @@ struct SimForce {
   double ih[3];
   double g[3];
   double nu;
+  int accumulate;  // add to the acceleration held (the RK4 "all" stages) instead of to 0
+  int conv_first;  // convection before viscosity (test/Jet3.cpp:596-599) instead of after
 };
 
 __device__ inline bool cell_fluid(const uint8_t* flags, const int* nbr27, int b, int x, int y, int z, bool& valid) {
@@ -514,6 +553,8 @@ __global__ void k_forces(int nb, const int* __restrict__ nbr27, const uint8_t* _
     if (!(f & F_EXISTS)) continue;
     int x = t & 7, y = (t >> 3) & 7, z = t >> 6;
     double a[3] = {0.0, 0.0, 0.0};
+    if (P.accumulate)
+      for (int i = 0; i < 3; ++i) a[i] = A[i][c];
     const bool fl = mat_is_fluid(f & 7u);
     const bool lay = layer[c] != -1;
     bool minus_fluid[3];
@@ -527,49 +568,58 @@ __global__ void k_forces(int nb, const int* __restrict__ nbr27, const uint8_t* _
       bool apply = fl || (lay && minus_fluid[i]);
       if (apply && !(f & (0x08u << i))) a[i] = a[i] + P.g[i];
     }
-    // viscosity
-    if (P.nu != 0.0) {
-      bool apply = fl || (lay && (minus_fluid[0] || minus_fluid[1] || minus_fluid[2]));
-      if (apply) {
-        for (int i = 0; i < 3; ++i) {
-          double temp = 0.0;
-          double uc = U[i][c];
-          for (int j = 0; j < 3; ++j) {
-            double ih2 = P.ih[j] * P.ih[j];
-            temp -= 2 * uc * ih2;
-            temp += ld_rel(U[i], nbr27, b, x + (j == 0), y + (j == 1), z + (j == 2)) * ih2;
-            temp += ld_rel(U[i], nbr27, b, x - (j == 0), y - (j == 1), z - (j == 2)) * ih2;
+    auto viscosity = [&]() {
+      if (P.nu != 0.0) {
+        bool apply = fl || (lay && (minus_fluid[0] || minus_fluid[1] || minus_fluid[2]));
+        if (apply) {
+          for (int i = 0; i < 3; ++i) {
+            double temp = 0.0;
+            double uc = U[i][c];
+            for (int j = 0; j < 3; ++j) {
+              double ih2 = P.ih[j] * P.ih[j];
+              temp -= 2 * uc * ih2;
+              temp += ld_rel(U[i], nbr27, b, x + (j == 0), y + (j == 1), z + (j == 2)) * ih2;
+              temp += ld_rel(U[i], nbr27, b, x - (j == 0), y - (j == 1), z - (j == 2)) * ih2;
+            }
+            if (!(f & (0x08u << i))) a[i] = a[i] + temp * P.nu;
           }
-          if (!(f & (0x08u << i))) a[i] = a[i] + temp * P.nu;
         }
       }
-    }
-    // convection
-    for (int i = 0; i < 3; ++i) {
-      bool apply = fl || (lay && minus_fluid[i]);
-      if (!apply) continue;
-      double temp = 0.0;
-      for (int j = 0; j < 3; ++j) {
-        const int ex = (j == 0), ey = (j == 1), ez = (j == 2);
-        const int mx = (i == 0), my = (i == 1), mz = (i == 2);
-        double Ub;
-        if (i == j) {
-          Ub = U[j][c];
-        } else {
-          Ub = 0.25 * (U[j][c] + ld_rel(U[j], nbr27, b, x + ex, y + ey, z + ez) + ld_rel(U[j], nbr27, b, x - mx, y - my, z - mz) +
-                       ld_rel(U[j], nbr27, b, x - mx + ex, y - my + ey, z - mz + ez));
+    };
+    auto convection = [&]() {
+      for (int i = 0; i < 3; ++i) {
+        bool apply = fl || (lay && minus_fluid[i]);
+        if (!apply) continue;
+        double temp = 0.0;
+        for (int j = 0; j < 3; ++j) {
+          const int ex = (j == 0), ey = (j == 1), ez = (j == 2);
+          const int mx = (i == 0), my = (i == 1), mz = (i == 2);
+          double Ub;
+          if (i == j) {
+            Ub = U[j][c];
+          } else {
+            Ub = 0.25 * (U[j][c] + ld_rel(U[j], nbr27, b, x + ex, y + ey, z + ez) + ld_rel(U[j], nbr27, b, x - mx, y - my, z - mz) +
+                         ld_rel(U[j], nbr27, b, x - mx + ex, y - my + ey, z - mz + ez));
+          }
+          double U1 = U[i][c], U2;
+          if (Ub > 0) {
+            U2 = ld_rel(U[i], nbr27, b, x - ex, y - ey, z - ez);
+            Ub *= (U1 - U2) * P.ih[j];
+          } else {
+            U2 = ld_rel(U[i], nbr27, b, x + ex, y + ey, z + ez);
+            Ub *= (U2 - U1) * P.ih[j];
+          }
+          temp -= Ub;
         }
-        double U1 = U[i][c], U2;
-        if (Ub > 0) {
-          U2 = ld_rel(U[i], nbr27, b, x - ex, y - ey, z - ez);
-          Ub *= (U1 - U2) * P.ih[j];
-        } else {
-          U2 = ld_rel(U[i], nbr27, b, x + ex, y + ey, z + ez);
-          Ub *= (U2 - U1) * P.ih[j];
-        }
-        temp -= Ub;
+        if (!(f & (0x08u << i))) a[i] = a[i] + temp;
       }
-      if (!(f & (0x08u << i))) a[i] = a[i] + temp;
+    };
+    if (P.conv_first) {
+      convection();
+      viscosity();
+    } else {
+      viscosity();
+      convection();
     }
     for (int i = 0; i < 3; ++i) A[i][c] = a[i];
   }
@@ -610,6 +660,61 @@ __global__ void k_rk_combine(int nb, const uint8_t* __restrict__ flags, double* 
   }
 }
 
+// Algorithms_RungeKutta_all::Do (:35-156), grid part after stage `stage`'s
+// forces and projection (a = the projected acceleration):
+//   1: S2 = S0 + a dt 0.5, S1 = S0 + a dt 1./6.   (forced: const faces too)
+//   2: S2 = S0 + a dt 0.5, S1 = S1 + a dt 1./3.   (forced)
+//   3: S2 = S0 + a dt,     S1 = S1 + a dt 1./3.   (forced)
+//   4: S0 = S1 + a dt 1./6.                       (const faces keep S0)
+// on every existing cell, with the reference's left-to-right products.
+__global__ void k_rk4_combine(int nb, const uint8_t* __restrict__ flags, int stage, double dt, double* __restrict__ s0x,
+                              double* __restrict__ s0y, double* __restrict__ s0z, double* __restrict__ s1x,
+                              double* __restrict__ s1y, double* __restrict__ s1z, double* __restrict__ s2x,
+                              double* __restrict__ s2y, double* __restrict__ s2z, const double* __restrict__ ax,
+                              const double* __restrict__ ay, const double* __restrict__ az) {
+#pragma clang fp contract(off)
+  const int b = blockIdx.x;
+  double* S0[3] = {s0x, s0y, s0z};
+  double* S1[3] = {s1x, s1y, s1z};
+  double* S2[3] = {s2x, s2y, s2z};
+  const double* A[3] = {ax, ay, az};
+  for (int t = threadIdx.x; t < MAC_BC; t += blockDim.x) {
+    const size_t c = (size_t)b * MAC_BC + t;
+    const unsigned f = flags[c];
+    if (!(f & F_EXISTS)) continue;
+    for (int i = 0; i < 3; ++i) {
+      const double a = A[i][c];
+      switch (stage) {
+        case 1:
+          S2[i][c] = S0[i][c] + a * dt * 0.5;
+          S1[i][c] = S0[i][c] + a * dt * 1. / 6.;
+          break;
+        case 2:
+          S2[i][c] = S0[i][c] + a * dt * 0.5;
+          S1[i][c] = S1[i][c] + a * dt * 1. / 3.;
+          break;
+        case 3:
+          S2[i][c] = S0[i][c] + a * dt;
+          S1[i][c] = S1[i][c] + a * dt * 1. / 3.;
+          break;
+        default:
+          if (!(f & (0x08u << i))) S0[i][c] = S1[i][c] + a * dt * 1. / 6.;
+          break;
+      }
+    }
+  }
+}
+
+__global__ void k_zero_accel_existing(int nb, const uint8_t* __restrict__ flags, double* __restrict__ ax, double* __restrict__ ay,
+                                      double* __restrict__ az) {
+  for (int t = threadIdx.x; t < MAC_BC; t += blockDim.x) {
+    const size_t c = (size_t)blockIdx.x * MAC_BC + t;
+    if (!(flags[c] & F_EXISTS)) continue;
+    ax[c] = 0.0;
+    ay[c] = 0.0;
+    az[c] = 0.0;
+  }
+}
 __global__ void k_zero_accel(size_t n, double* __restrict__ ax, double* __restrict__ ay, double* __restrict__ az) {
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) ax[i] = ay[i] = az[i] = 0.0;
 }
@@ -742,6 +847,62 @@ __global__ void k_move_particles(int np, const double* __restrict__ px, const do
   qy[o] = res[1];
   qz[o] = res[2];
 }
+
+// Particle part of the same stages (Policy_Move_Particle: the velocity is the
+// interpolated grid speed of the stage's slot at the given position; in
+// stages 2 and 3 the second update reads it at the position just written):
+//   1: v = V(p0);  p2 = p0 + dt 0.5 v;  p1 = p0 + dt 1.0/6.0 v
+//   2: p2 = p0 + dt 0.5 V(p2);  p1 = p1 + dt 1.0/3.0 V(p2)
+//   3: p2 = p0 + dt V(p2);      p1 = p1 + dt 1.0/3.0 V(p2)
+//   4: p0 = p1 + dt 1.0/6.0 V(p2)
+__global__ void k_rk4_particles(int np, int stage, double dt, double* __restrict__ p0x, double* __restrict__ p0y,
+                                double* __restrict__ p0z, double* __restrict__ p1x, double* __restrict__ p1y,
+                                double* __restrict__ p1z, double* __restrict__ p2x, double* __restrict__ p2y,
+                                double* __restrict__ p2z, SimGrid G) {
+#pragma clang fp contract(off)
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= np) return;
+  double* P0[3] = {p0x, p0y, p0z};
+  double* P1[3] = {p1x, p1y, p1z};
+  double* P2[3] = {p2x, p2y, p2z};
+  double q[3], v[3];
+  if (stage == 1) {
+    for (int k = 0; k < 3; ++k) q[k] = P0[k][i];
+    interp_speed(G, q, v);
+    for (int k = 0; k < 3; ++k) {
+      P2[k][i] = q[k] + dt * 0.5 * v[k];
+      P1[k][i] = q[k] + dt * 1.0 / 6.0 * v[k];
+    }
+  } else if (stage == 4) {
+    for (int k = 0; k < 3; ++k) q[k] = P2[k][i];
+    interp_speed(G, q, v);
+    for (int k = 0; k < 3; ++k) P0[k][i] = P1[k][i] + dt * 1.0 / 6.0 * v[k];
+  } else {
+    for (int k = 0; k < 3; ++k) q[k] = P2[k][i];
+    interp_speed(G, q, v);
+    for (int k = 0; k < 3; ++k) q[k] = P0[k][i] + (stage == 2 ? dt * 0.5 : dt) * v[k];
+    for (int k = 0; k < 3; ++k) P2[k][i] = q[k];
+    interp_speed(G, q, v);
+    for (int k = 0; k < 3; ++k) P1[k][i] = P1[k][i] + dt * 1.0 / 3.0 * v[k];
+  }
+}
+
+// Particles outside [lo, hi) are deleted (slot 0 only; the other slots are
+// rewritten from slot 0 at the next step's first stage).
+__global__ void k_box_delete(int np, const double* __restrict__ px, const double* __restrict__ py, const double* __restrict__ pz,
+                             double* __restrict__ qx, double* __restrict__ qy, double* __restrict__ qz, int* __restrict__ nkept,
+                             double lo0, double lo1, double lo2, double hi0, double hi1, double hi2) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= np) return;
+  const double x = px[i], y = py[i], z = pz[i];
+  const bool keep = x >= lo0 && x < hi0 && y >= lo1 && y < hi1 && z >= lo2 && z < hi2;
+  const int o = wave_append(keep, nkept);
+  if (o < 0) return;
+  qx[o] = x;
+  qy[o] = y;
+  qz[o] = z;
+}
+
 
 // ------------------------------------------------------ extrapolation ----
 // Algorithms_Extrapolate::Do (Algorithms_Extrapolate.h:25-235), one frontier

@@ -305,6 +305,7 @@ class SimParams(ctypes.Structure):
         ("depth_remove", ctypes.c_int),
         ("floor_axis", ctypes.c_int),
         ("floor_value", ctypes.c_double),
+        ("fill_cube_n", ctypes.c_int),
         ("proj", Params),
     ]
 

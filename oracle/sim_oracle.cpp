@@ -22,6 +22,9 @@
 
 extern "C" int orc_project(int nb, const int32_t* keys, const uint8_t* flags, const double* h, double* u, double* v, double* w,
                            double* p, double rtol, double* maxdiv);
+extern "C" int orc_project_dirichlet(int nb, const int32_t* keys, const uint8_t* flags, const double* h, double* u, double* v,
+                                     double* w, const double* ur, const double* vr, const double* wr, double nu, double* p,
+                                     double rtol, double* maxdiv);
 
 namespace {
 
@@ -34,7 +37,7 @@ struct Chunk {
   uint8_t flags[512];
   int8_t layer[512];
   int16_t depth[512];
-  double S[2][3][512];
+  double S[3][3][512];
   double A[3][512];
   double P[512];
   Chunk() {
@@ -79,6 +82,8 @@ struct Params {
   int layer_max, integrator, particle_rk, ppc;
   int depth_interior, depth_remove, floor_axis;  // depth_interior 0: the chain without depth
   double floor_value;
+  int cube_n;      // > 0: Policy_Add_Particle_Cube lattice instead of the jitter
+  double proj_nu;  // > 0: the viscous air pressures of the projection
   uint64_t seed;
   double lo[3], hi[3];
 };
@@ -87,7 +92,8 @@ struct Sim {
   Params P;
   double ih[3];
   std::map<K3, std::unique_ptr<Chunk>> grid;
-  std::vector<std::array<double, 3>> parts;
+  std::vector<std::array<double, 3>> parts;  // position slot 0
+  std::vector<std::array<double, 3>> parts1, parts2;  // slots 1 and 2 (RK4 "all")
   uint64_t tick = 0;
 
   Chunk* chunk(int kx, int ky, int kz) {
@@ -109,8 +115,10 @@ struct Sim {
   // a speed read through GetNeighbour: the chunk's memory if the chunk exists
   double speed(int slot, int comp, int kx, int ky, int kz) {
     Chunk* ch = chunk(kx, ky, kz);
-    return ch ? ch->S[slot][comp][off_of(kx, ky, kz)] : 0.0;
+    return ch ? field(ch, slot, comp)[off_of(kx, ky, kz)] : 0.0;
   }
+  // speed slot 0..2, or 3 = the acceleration
+  static double* field(Chunk* ch, int slot, int comp) { return slot < 3 ? ch->S[slot][comp] : ch->A[comp]; }
   bool fluid(int kx, int ky, int kz) {
     Chunk* ch = chunk(kx, ky, kz);
     if (!ch) return false;
@@ -136,8 +144,23 @@ struct Sim {
     for (int i = 0; i < 3; ++i) k[i] = (int)lround(p[i] * ih[i]);
   }
 
-  void topology() {
+  // full: the step's chain; otherwise the chain inside an RK4 stage
+  // (Algorithms_Create_Interior_Depth: no particles, no floor) with the keys
+  // of position slot ks, every slot erased together
+  void topology(bool full = true, int ks = 0) {
     const bool dm = P.depth_interior > 0;
+    std::vector<std::array<double, 3>>& kparts = ks == 2 ? parts2 : parts;
+    const bool multi = !full && parts1.size() == parts.size() && parts2.size() == parts.size();
+    auto erase_where = [&](const std::vector<char>& keep) {
+      std::vector<std::array<double, 3>>* all[3] = {&parts, &parts1, &parts2};
+      for (int sl = 0; sl < (multi ? 3 : 1); ++sl) {
+        std::vector<std::array<double, 3>>& v = *(sl == 0 ? &kparts : (ks == 2 ? all[sl - 1] : all[sl]));
+        std::vector<std::array<double, 3>> out;
+        for (size_t k = 0; k < v.size(); ++k)
+          if (keep[k]) out.push_back(v[k]);
+        v.swap(out);
+      }
+    };
     // Algorithms_Initialize_MacCell; with depth Algorithms_Initialize_Layer_And_Depth.h:16-33
     for_cells([&](Chunk* c, int t, int, int, int) {
       c->layer[t] = -1;
@@ -145,33 +168,31 @@ struct Sim {
       if (dm && is_fluid(m) && m != 4u) c->depth[t] = -1;
     });
     // Algorithm_DelParticle (test/Jet3.cpp:259-289): list erase, order kept
-    if (P.floor_axis >= 0) {
-      std::vector<std::array<double, 3>> kept;
-      for (auto& p : parts)
-        if (!(p[P.floor_axis] < P.floor_value)) kept.push_back(p);
-      parts.swap(kept);
+    if (full && P.floor_axis >= 0) {
+      std::vector<char> keep(kparts.size());
+      for (size_t k = 0; k < kparts.size(); ++k) keep[k] = !(kparts[k][P.floor_axis] < P.floor_value);
+      erase_where(keep);
     }
     if (dm) {
       // Algorithms_Layer_Initial_With_Particle_Depth.h:29-46
-      std::vector<std::array<double, 3>> kept;
-      for (auto& p : parts) {
+      std::vector<char> keep(kparts.size(), 1);
+      for (size_t q = 0; q < kparts.size(); ++q) {
         int k[3];
-        key_of(p, k);
+        key_of(kparts[q], k);
         Chunk* c = insert(k[0], k[1], k[2]);
         int o = off_of(k[0], k[1], k[2]);
         if ((c->flags[o] & 7u) == 4u) {
           c->layer[o] = 0;
-          if (c->depth[o] > P.depth_remove) continue;
+          if (c->depth[o] > P.depth_remove) keep[q] = 0;
         } else {
           c->layer[o] = 0;
           c->flags[o] = (uint8_t)((c->flags[o] & ~7u) | set_fluid(c->flags[o] & 7u));
         }
-        kept.push_back(p);
       }
-      parts.swap(kept);
+      erase_where(keep);
     } else {
       // Algorithms_Layer_Initial_With_Particle
-      for (auto& p : parts) {
+      for (auto& p : kparts) {
         int k[3];
         key_of(p, k);
         Chunk* c = insert(k[0], k[1], k[2]);
@@ -183,6 +204,26 @@ struct Sim {
     // Algorithms_Create_Fluid_Particle with the jitter policy; with depth
     // Algorithms_Create_Fluid_Particle_Depth.h:29-44
     auto fill = [&](int kx, int ky, int kz) {
+      if (!full) return;  // Algorithms_Create_Interior_Depth adds no particle
+      if (P.cube_n > 0) {
+        // Policy_Add_Particle_Cube::AddParticle (:29-59), x fastest
+        const double dh = 1. / (2 * P.cube_n + 1);
+        int tab[3] = {-P.cube_n, -P.cube_n, -P.cube_n};
+        const int K[3] = {kx, ky, kz};
+        while (true) {
+          std::array<double, 3> r;
+          for (int i = 0; i < 3; ++i) r[i] = (K[i] + tab[i] * dh) * P.h[i];
+          parts.push_back(r);
+          int j = 0;
+          while (j < 3 && tab[j] >= P.cube_n) {
+            tab[j] = -P.cube_n;
+            ++j;
+          }
+          if (j < 3) ++tab[j];
+          else break;
+        }
+        return;
+      }
       uint64_t ck = pack(kx, ky, kz);
       for (int k = 0; k < P.ppc; ++k) {
         double ux = uni(P.seed, tick, ck, 3 * k), uy = uni(P.seed, tick, ck, 3 * k + 1), uz = uni(P.seed, tick, ck, 3 * k + 2);
@@ -191,7 +232,7 @@ struct Sim {
     };
     for_cells([&](Chunk* c, int t, int kx, int ky, int kz) {
       unsigned f = c->flags[t];
-      if (P.ppc > 0 && (f & 7u) == 3u && c->layer[t] == -1) {
+      if ((P.ppc > 0 || P.cube_n > 0) && (f & 7u) == 3u && c->layer[t] == -1) {
         fill(kx, ky, kz);
         c->layer[t] = 0;
         c->flags[t] = (uint8_t)((f & ~7u) | set_fluid(f & 7u));
@@ -285,7 +326,7 @@ struct Sim {
       if (!c->any()) it = grid.erase(it);
       else ++it;
     }
-    tick++;
+    if (full) tick++;
   }
 
   double compute_dt() {
@@ -313,11 +354,11 @@ struct Sim {
   // face is 0/0, as in the reference.
   void extrapolate(int slot) {
     auto ok = [&](int x, int y, int z) { return valid(x, y, z) && layer_of(x, y, z) != -1; };
-    auto U = [&](int comp, int x, int y, int z) { return chunk(x, y, z)->S[slot][comp][off_of(x, y, z)]; };
+    auto U = [&](int comp, int x, int y, int z) { return field(chunk(x, y, z), slot, comp)[off_of(x, y, z)]; };
     auto set = [&](int x, int y, int z, int comp, double val) {
       Chunk* ch = chunk(x, y, z);
       const int o = off_of(x, y, z);
-      if (!(ch->flags[o] & (0x08u << comp))) ch->S[slot][comp][o] = val;
+      if (!(ch->flags[o] & (0x08u << comp))) field(ch, slot, comp)[o] = val;
     };
     std::vector<K3> cur;
     for_cells([&](Chunk* c, int t, int x, int y, int z) {
@@ -423,11 +464,14 @@ struct Sim {
     }
   }
 
-  void forces(int slot) {
-    // Acceleration SetZero(true), then Algorithms<Gravity, Viscosity, Convection>
-    for_cells([&](Chunk* c, int t, int, int, int) {
-      for (int i = 0; i < 3; ++i) c->A[i][t] = 0.0;
-    });
+  // Acceleration SetZero(true), then Algorithms<Gravity, Viscosity, Convection>
+  // (TestJetDEau_Basic.cpp:365); rk4: no zeroing (the RK4 "all" driver
+  // zeroes) and Jet3's order Gravity, Convection, Viscosity (:596-599)
+  void forces(int slot, bool rk4 = false) {
+    if (!rk4)
+      for_cells([&](Chunk* c, int t, int, int, int) {
+        for (int i = 0; i < 3; ++i) c->A[i][t] = 0.0;
+      });
     // Policy_Gravity::ApplyGravity
     for (int i = 0; i < 3; ++i) {
       if (P.g[i] == 0.0) continue;
@@ -441,6 +485,7 @@ struct Sim {
         if (need && !(c->flags[t] & (0x08u << i))) c->A[i][t] = c->A[i][t] + P.g[i];
       });
     }
+    auto viscosity = [&]() {
     // Algorithms_Viscosity
     if (P.nu != 0.0) {
       for_cells([&](Chunk* c, int t, int kx, int ky, int kz) {
@@ -469,6 +514,8 @@ struct Sim {
           if (!(c->flags[t] & (0x08u << i))) c->A[i][t] = c->A[i][t] + lap[i] * P.nu;
       });
     }
+    };
+    auto convection = [&]() {
     // Algorithms_Convection with Policy_Upwind_1_Order
     for_cells([&](Chunk* c, int t, int kx, int ky, int kz) {
       int K[3] = {kx, ky, kz};
@@ -510,13 +557,24 @@ struct Sim {
         if (!(c->flags[t] & (0x08u << i))) c->A[i][t] = c->A[i][t] + temp;
       }
     });
+    };
+    if (rk4) {
+      convection();
+      viscosity();
+    } else {
+      viscosity();
+      convection();
+    }
   }
 
-  void project(int slot, double* maxdiv) {
+  // projection of field `slot` (3: the acceleration, Algorithms_Solve_Pressure_Force);
+  // with proj_nu > 0 the viscous air pressures read speed slot `real`
+  void project(int slot, double* maxdiv, int real = -1) {
     std::vector<int32_t> keys;
     std::vector<uint8_t> fl;
-    std::vector<double> u, v, w, p;
+    std::vector<double> u, v, w, p, ur, vr, wr;
     std::vector<Chunk*> chs;
+    const int rs = real < 0 ? slot : real;
     for (auto& kv : grid) {
       keys.push_back(kv.first.x);
       keys.push_back(kv.first.y);
@@ -531,15 +589,28 @@ struct Sim {
     p.resize(u.size());
     for (int b = 0; b < nb; ++b) {
       memcpy(&fl[(size_t)b * 512], chs[b]->flags, 512);
-      memcpy(&u[(size_t)b * 512], chs[b]->S[slot][0], 512 * 8);
-      memcpy(&v[(size_t)b * 512], chs[b]->S[slot][1], 512 * 8);
-      memcpy(&w[(size_t)b * 512], chs[b]->S[slot][2], 512 * 8);
+      memcpy(&u[(size_t)b * 512], field(chs[b], slot, 0), 512 * 8);
+      memcpy(&v[(size_t)b * 512], field(chs[b], slot, 1), 512 * 8);
+      memcpy(&w[(size_t)b * 512], field(chs[b], slot, 2), 512 * 8);
     }
-    if (nb > 0) orc_project(nb, keys.data(), fl.data(), P.h, u.data(), v.data(), w.data(), p.data(), 1e-14, maxdiv);
+    if (nb > 0 && P.proj_nu > 0) {
+      ur.resize(u.size());
+      vr.resize(u.size());
+      wr.resize(u.size());
+      for (int b = 0; b < nb; ++b) {
+        memcpy(&ur[(size_t)b * 512], field(chs[b], rs, 0), 512 * 8);
+        memcpy(&vr[(size_t)b * 512], field(chs[b], rs, 1), 512 * 8);
+        memcpy(&wr[(size_t)b * 512], field(chs[b], rs, 2), 512 * 8);
+      }
+      orc_project_dirichlet(nb, keys.data(), fl.data(), P.h, u.data(), v.data(), w.data(), ur.data(), vr.data(), wr.data(), P.proj_nu,
+                            p.data(), 1e-14, maxdiv);
+    } else if (nb > 0) {
+      orc_project(nb, keys.data(), fl.data(), P.h, u.data(), v.data(), w.data(), p.data(), 1e-14, maxdiv);
+    }
     for (int b = 0; b < nb; ++b) {
-      memcpy(chs[b]->S[slot][0], &u[(size_t)b * 512], 512 * 8);
-      memcpy(chs[b]->S[slot][1], &v[(size_t)b * 512], 512 * 8);
-      memcpy(chs[b]->S[slot][2], &w[(size_t)b * 512], 512 * 8);
+      memcpy(field(chs[b], slot, 0), &u[(size_t)b * 512], 512 * 8);
+      memcpy(field(chs[b], slot, 1), &v[(size_t)b * 512], 512 * 8);
+      memcpy(field(chs[b], slot, 2), &w[(size_t)b * 512], 512 * 8);
       memcpy(chs[b]->P, &p[(size_t)b * 512], 512 * 8);
     }
   }
@@ -564,15 +635,15 @@ struct Sim {
   }
 
   // Policy_Speed_Interpolation_Linear_Symmetric::Get_Speed, recursive as in the reference
-  double impl(const double d2[3], int i, int kx, int ky, int kz, int k) {
+  double impl(const double d2[3], int i, int kx, int ky, int kz, int k, int slot) {
     int n[3] = {kx, ky, kz};
     n[i] += 1;
     double w0 = (i != k) ? (1 - d2[i]) : (0.5 - d2[i]);
     double w1 = (i != k) ? d2[i] : (0.5 + d2[i]);
-    if (i < 2) return w0 * impl(d2, i + 1, kx, ky, kz, k) + w1 * impl(d2, i + 1, n[0], n[1], n[2], k);
-    return w0 * speed(0, k, kx, ky, kz) + w1 * speed(0, k, n[0], n[1], n[2]);
+    if (i < 2) return w0 * impl(d2, i + 1, kx, ky, kz, k, slot) + w1 * impl(d2, i + 1, n[0], n[1], n[2], k, slot);
+    return w0 * speed(slot, k, kx, ky, kz) + w1 * speed(slot, k, n[0], n[1], n[2]);
   }
-  std::array<double, 3> get_speed(const std::array<double, 3>& pos) {
+  std::array<double, 3> get_speed(const std::array<double, 3>& pos, int slot = 0) {
     double d[3];
     int key0[3];
     for (int i = 0; i < 3; ++i) {
@@ -593,9 +664,84 @@ struct Sim {
           d2[j] = d[j];
         }
       }
-      ret[k] = impl(d2, 0, key[0], key[1], key[2], k);
+      ret[k] = impl(d2, 0, key[0], key[1], key[2], k, slot);
     }
     return ret;
+  }
+
+  // Algorithms_RungeKutta_all::Do (:35-156) with Policy::Do of test/Jet3.cpp:
+  // per stage the cell-type chain of m_alg_init2 (keys of slot 0, then 2),
+  // extrapolation of the stage's speed slot, gravity / convection /
+  // viscosity added to the acceleration, its projection (air pressures from
+  // the stage's speed) and extrapolation, then the grid and particle loops in
+  // the reference's order and operand order.
+  void rk4all(double dt, double* maxdiv) {
+    auto zero = [&]() {
+      for_cells([&](Chunk* c, int t, int, int, int) {
+        for (int i = 0; i < 3; ++i) c->A[i][t] = 0.0;
+      });
+    };
+    zero();
+    parts1 = parts;
+    parts2 = parts;
+    for (int stage = 1; stage <= 4; ++stage) {
+      const int ks = stage == 1 ? 0 : 2;
+      topology(false, ks);
+      extrapolate(ks);
+      forces(ks, true);
+      project(3, maxdiv, ks);
+      extrapolate(3);
+      for_cells([&](Chunk* c, int t, int, int, int) {
+        for (int i = 0; i < 3; ++i) {
+          const double a = c->A[i][t];
+          switch (stage) {
+            case 1:
+              c->S[2][i][t] = c->S[0][i][t] + a * dt * 0.5;
+              c->S[1][i][t] = c->S[0][i][t] + a * dt * 1. / 6.;
+              break;
+            case 2:
+              c->S[2][i][t] = c->S[0][i][t] + a * dt * 0.5;
+              c->S[1][i][t] = c->S[1][i][t] + a * dt * 1. / 3.;
+              break;
+            case 3:
+              c->S[2][i][t] = c->S[0][i][t] + a * dt;
+              c->S[1][i][t] = c->S[1][i][t] + a * dt * 1. / 3.;
+              break;
+            default:
+              if (!(c->flags[t] & (0x08u << i))) c->S[0][i][t] = c->S[1][i][t] + a * dt * 1. / 6.;
+              break;
+          }
+        }
+      });
+      for (size_t q = 0; q < parts.size(); ++q) {
+        std::array<double, 3>&p0 = parts[q], &p1 = parts1[q], &p2 = parts2[q];
+        if (stage == 1) {
+          const std::array<double, 3> v = get_speed(p0, ks);
+          for (int k = 0; k < 3; ++k) {
+            p2[k] = p0[k] + dt * 0.5 * v[k];
+            p1[k] = p0[k] + dt * 1.0 / 6.0 * v[k];
+          }
+        } else if (stage == 4) {
+          const std::array<double, 3> v = get_speed(p2, ks);
+          for (int k = 0; k < 3; ++k) p0[k] = p1[k] + dt * 1.0 / 6.0 * v[k];
+        } else {
+          std::array<double, 3> v = get_speed(p2, ks);
+          for (int k = 0; k < 3; ++k) p2[k] = p0[k] + (stage == 2 ? dt * 0.5 : dt) * v[k];
+          v = get_speed(p2, ks);
+          for (int k = 0; k < 3; ++k) p1[k] = p1[k] + dt * 1.0 / 3.0 * v[k];
+        }
+      }
+      if (stage < 4) zero();
+    }
+  }
+
+  // particles leaving [lo, hi) are deleted (slot 0)
+  void box_delete() {
+    if (P.lo[0] == P.hi[0] && P.lo[1] == P.hi[1] && P.lo[2] == P.hi[2]) return;
+    std::vector<std::array<double, 3>> out;
+    for (auto& r : parts)
+      if (r[0] >= P.lo[0] && r[0] < P.hi[0] && r[1] >= P.lo[1] && r[1] < P.hi[1] && r[2] >= P.lo[2] && r[2] < P.hi[2]) out.push_back(r);
+    parts.swap(out);
   }
 
   void move(double dt) {
@@ -632,7 +778,8 @@ extern "C" {
 
 // prm: h[3], g[3], nu, cfl, dt_min, dt_max, lo[3], hi[3] (16 doubles);
 // iprm: layer_max, integrator, particle_rk, ppc, depth_interior, depth_remove,
-// floor_axis; prm[16] = floor_value; seed.
+// floor_axis, fill_cube_n; prm[16] = floor_value, prm[17] = projection
+// viscosity; seed.
 void* orc_sim_create(const double* prm, const int* iprm, unsigned long long seed) {
   Sim* s = new Sim();
   for (int i = 0; i < 3; ++i) {
@@ -654,6 +801,8 @@ void* orc_sim_create(const double* prm, const int* iprm, unsigned long long seed
   s->P.depth_remove = iprm[5];
   s->P.floor_axis = iprm[6];
   s->P.floor_value = prm[16];
+  s->P.proj_nu = prm[17];
+  s->P.cube_n = iprm[7];
   s->P.seed = seed;
   return s;
 }
@@ -673,6 +822,7 @@ void orc_sim_upload(void* p, int nb, const int32_t* keys, const uint8_t* flags, 
       if (src[i]) {
         memcpy(c->S[0][i], &src[i][(size_t)b * 512], 512 * 8);
         memcpy(c->S[1][i], &src[i][(size_t)b * 512], 512 * 8);
+        memcpy(c->S[2][i], &src[i][(size_t)b * 512], 512 * 8);
       }
     s->grid.emplace(K3{keys[3 * b], keys[3 * b + 1], keys[3 * b + 2]}, std::move(c));
   }
@@ -738,8 +888,15 @@ double orc_sim_step(void* p, double* maxdiv) {
   Sim* s = (Sim*)p;
   s->topology();
   double dt = s->compute_dt();
-  s->advance(dt, maxdiv);
-  s->move(dt);
+  if (s->P.integrator == 2) {
+    // test/Jet3.cpp's init chain ends with the extrapolation of speed slot 0
+    s->extrapolate(0);
+    s->rk4all(dt, maxdiv);
+    s->box_delete();
+  } else {
+    s->advance(dt, maxdiv);
+    s->move(dt);
+  }
   return dt;
 }
 

@@ -252,3 +252,67 @@ def test_depth_jet_steps_vs_oracle():
         same_particles(dg["particles"], do["particles"], 1e-9)
         assert sg["max_div"] < 1e-7
     assert n_interior > 0
+
+
+# ---------------------------------------------------- RK4 "all" integrator ----
+def _blob(h):
+    cells = np.argwhere(np.ones((8, 8, 8), bool)) + 4
+    pts = cells * h + np.array([1, 3, -5]) * (h / 64)
+    keys = np.array(sorted({tuple(k) for k in (cells // 8).tolist()}, key=lambda k: (k[2], k[1], k[0])), np.int32)
+    return keys, pts
+
+
+@pytest.mark.parametrize("case", ["uniform", "free_fall"])
+def test_rk4all_closed_forms_vs_oracle(case):
+    """The two closed forms the oracle is pinned by, stepped on the device:
+    cells bit-exact, grid speeds and particles equal to the oracle's."""
+    h, dt = 0.125, 0.0625
+    keys, pts = _blob(h)
+    n = len(keys) * 512
+    g = (0, 0, -8.0) if case == "free_fall" else (0, 0, 0)
+    vel = (0.0, 0.0, 0.0) if case == "free_fall" else (0.25, -0.5, 0.125)
+    pair = both(h=(h, h, h), gravity=g, integrator=2, particles_per_inflow_cell=0, layer_max=2, cfl=1e9, dt_min=dt,
+                dt_max=dt)
+    for s in pair:
+        s.upload(keys, np.zeros(n, np.uint8), u=np.full(n, vel[0]), v=np.full(n, vel[1]), w=np.full(n, vel[2]),
+                 particles=pts)
+    for step in range(2):
+        for s in pair:
+            s.step()
+        dg, do = pair[0].download(), pair[1].download()
+        same_cells(dg, do)
+        ex = (dg["flags"] & W.EXISTS) > 0
+        for k in ("u", "v", "w"):
+            assert np.allclose(dg[k][ex], do[k][ex], rtol=0, atol=1e-13), (step, k)
+        same_particles(dg["particles"], do["particles"], 1e-13)
+
+
+def test_rk4all_depth_jet_vs_oracle():
+    """The Jet3 configuration on a small lateral jet: RK4 "all" with the
+    depth chain (DepthInterior 2, DepthRemove 3), the particle floor, the
+    cube-lattice fill (n = 1), viscosity in the forces and the viscous air
+    pressures in every projection, against the oracle step by step."""
+    jet = J.lateral_jet(32, 24, 24, h=0.01, nozzle=8, inflow=1.5, seed=3)
+    kw = dict(jet["params"], layer_max=2, integrator=2, seed=7, particles_per_inflow_cell=0, fill_cube_n=1,
+              depth_interior=2, depth_remove=3, floor_axis=2, floor_value=5.5 * 0.01, viscosity=1.307e-3)
+    cells = np.argwhere(np.ones((10, 10, 10), bool)) + np.array([1, 7, 7])
+    rng = np.random.default_rng(2)
+    pts = np.repeat(cells, 2, axis=0) * 0.01 + rng.uniform(-0.004, 0.004, size=(2 * len(cells), 3))
+    pair = (Sim(**kw, proj={"rtol": 1e-13, "viscosity": 1.307e-3}), O.Sim(**kw, proj={"viscosity": 1.307e-3}))
+    upload_both(jet, pair, particles=pts)
+    n_interior = 0
+    for step in range(4):
+        sg = pair[0].step()
+        so = pair[1].step()
+        assert sg["dt"] == pytest.approx(so["dt"], rel=1e-9)
+        dg, do = pair[0].download(), pair[1].download()
+        same_cells(dg, do)
+        same_depth(pair, dg)
+        ex = (dg["flags"] & W.EXISTS) > 0
+        n_interior = max(n_interior, int(((dg["flags"][ex] & 7) == W.INTERIOR).sum()))
+        for k in ("u", "v", "w"):
+            ref = do[k][ex]
+            assert np.linalg.norm(dg[k][ex] - ref) <= 1e-8 * max(np.linalg.norm(ref), 1.0), (step, k)
+        same_particles(dg["particles"], do["particles"], 1e-9)
+        assert sg["max_div"] < 1e-7
+    assert n_interior > 0

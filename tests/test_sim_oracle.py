@@ -380,3 +380,54 @@ def test_floor_erases_particles_before_marking():
     assert np.array_equal(d["particles"], pts[[0, 2]])
     fluid = {k for k, v in cells.items() if v[0] == 0}
     assert fluid == {(0, 0, 0), (5, 0, 0)}
+
+
+# ---------------------------------------------------- RK4 "all" integrator ----
+def test_rk4all_uniform_flow_is_kept_and_particles_move_by_dt_u():
+    """Algorithms_RungeKutta_all on a uniform flow with no gravity and no
+    viscosity: every stage's acceleration is 0 (upwind convection of a
+    constant field, extrapolated into the air layers with dyadic values so
+    that the means are exact), its projection is 0, so the grid speed is
+    unchanged and the particle stages add dt (1/6 + 1/3 + 1/3 + 1/6) u."""
+    h = 0.125
+    vel = np.array([0.25, -0.5, 0.125])
+    cells = np.argwhere(np.ones((8, 8, 8), bool)) + 4
+    pts = cells * h + np.array([1, 3, -5]) * (h / 64)
+    s = O.Sim(h=(h, h, h), gravity=(0, 0, 0), integrator=2, particles_per_inflow_cell=0, layer_max=2, cfl=1e9,
+              dt_min=0.0625, dt_max=0.0625)
+    nb_keys = np.array(sorted({tuple(k) for k in (cells // 8).tolist()}, key=lambda k: (k[2], k[1], k[0])), np.int32)
+    n = len(nb_keys) * 512
+    s.upload(nb_keys, np.zeros(n, np.uint8), u=np.full(n, vel[0]), v=np.full(n, vel[1]), w=np.full(n, vel[2]), particles=pts)
+    st = s.step()
+    assert st["dt"] == 0.0625
+    d = s.download()
+    ex = (d["flags"] & W.EXISTS) > 0
+    fluid = ex & np.isin(d["flags"] & 7, [W.FLUID])
+    for k, comp in zip(("u", "v", "w"), vel):
+        assert np.all(d[k][fluid] == comp)
+    assert np.allclose(d["particles"], pts + 0.0625 * vel, rtol=0, atol=1e-15)
+
+
+def test_rk4all_free_fall_follows_the_reference_staging():
+    """A blob at rest under gravity g: every stage's acceleration is g on
+    the z faces (its divergence vanishes, so the projection keeps it), the
+    grid speed ends at g dt, and the particles, which read the stage speed
+    after the grid loop has written the next stage's value (stage 2 reads
+    g dt/2, stages 3 and 4 read g dt), move by dt^2 g (1/2 (1/3) + 1 (1/3) +
+    1 (1/6)) = (2/3) g dt^2 -- not the g dt^2 / 2 of free fall, a property
+    of Algorithms_RungeKutta_all.h:84-137."""
+    h, g, dt = 0.125, -8.0, 0.0625
+    cells = np.argwhere(np.ones((8, 8, 8), bool)) + 4
+    pts = cells * h + np.array([1, 3, -5]) * (h / 64)
+    s = O.Sim(h=(h, h, h), gravity=(0, 0, g), integrator=2, particles_per_inflow_cell=0, layer_max=2, cfl=1e9,
+              dt_min=dt, dt_max=dt)
+    keys = np.array(sorted({tuple(k) for k in (cells // 8).tolist()}, key=lambda k: (k[2], k[1], k[0])), np.int32)
+    s.upload(keys, np.zeros(len(keys) * 512, np.uint8), particles=pts)
+    s.step()
+    d = s.download()
+    fluid = ((d["flags"] & W.EXISTS) > 0) & ((d["flags"] & 7) == W.FLUID)
+    assert fluid.sum() == len(cells)
+    assert np.allclose(d["w"][fluid], g * dt, rtol=0, atol=1e-15)  # the 1/6 and 1/3 terms round
+    assert np.all(d["u"][fluid] == 0.0) and np.all(d["v"][fluid] == 0.0)
+    want = pts + np.array([0.0, 0.0, 2.0 / 3.0 * g * dt * dt])
+    assert np.allclose(d["particles"], want, rtol=0, atol=1e-15)

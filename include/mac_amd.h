@@ -82,9 +82,9 @@ typedef struct {
   int smoother;    /* enum mac_smoother                                            */
   double cheb_lower; /* MAC_SMOOTH_CHEBYSHEV: lower end a of the eigenvalue range
                       [a, 2] of D^-1 A that the nu sweeps damp                    */
-  int nu_coarse;   /* sweeps before and after the correction on the coarse levels
-                      (0: nu); they are short, latency-bound launches, and more
-                      of them buy fewer outer iterations                          */
+  int nu_coarse;   /* smoothing launches before and after the correction on the
+                      coarse levels (0: nu); they are short, latency-bound
+                      launches                                                    */
   double cheb_lower_coarse; /* a of the coarse levels' Chebyshev weights          */
   int coarse_tail; /* 1 (default): the last two levels of each V-cycle (the top brick
                       and its at most 8 children) run as one single-workgroup launch
@@ -94,6 +94,12 @@ typedef struct {
                       next to a non-air cell holds p = -sum 2 nu (u_i(c+e_i) - u_i(c))
                       / h_i * s over its non-air neighbours, which enters the right
                       side, the written pressure and the velocity update; 0: p = 0 */
+  int coarse_inner; /* > 1 (default 4): each smoothing launch on the coarse levels
+                      other than the last two is a block-Jacobi step, the brick's unknowns taking this many
+                      Chebyshev sweeps ([cheb_lower_coarse, 2]) among themselves
+                      with the neighbour bricks frozen (nu_coarse counts launches);
+                      1: point Jacobi                                             */
+  double coarse_damp; /* the block step's correction is scaled by this (0, 1]     */
 } mac_params;
 
 typedef struct {
@@ -112,7 +118,9 @@ typedef struct {
 } mac_stats;
 
 /* Defaults: h=1, rtol=1e-8, max_iter=500, FP64, nu=2, omega=0.9, check_every=4, stall_iters=0,
- * Chebyshev smoothing with a = 0.25 on level 0 and nu_coarse=4, a = 0.1 below. */
+ * Chebyshev smoothing with a = 0.25 on level 0; below it nu_coarse=2 block-Jacobi
+ * launches of coarse_inner=4 Chebyshev sweeps on [0.05, 2], damped by 0.8; the
+ * last two levels fused (coarse_tail=1); viscosity 0. */
 void mac_default_params(mac_params* prm);
 
 /* Create / destroy a context on the current HIP device. */

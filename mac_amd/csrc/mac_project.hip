@@ -85,6 +85,7 @@ struct mac_ctx {
   mac_params prm;
   double om_pre[8], om_post[8];  // per-sweep Jacobi weights before / after the coarse correction, level 0
   double omc_pre[8], omc_post[8];  // the same on the coarse levels
+  BlockW blockw;                   // inner sweeps of the coarse block smoother
   // two CG iterations captured as one graph, per V-cycle precision (0 FP64, 1 FP32)
   hipGraphExec_t gexec[2] = {nullptr, nullptr};
   int solves_on_grid = 0;  // projections since the last build (graphs pay off from the third)
@@ -550,6 +551,8 @@ int check_params(const mac_params* p) {
   if (p->nu < 1 || p->nu > 8 || p->max_iter < 1 || !(p->rtol > 0)) return fail(MAC_E_ARG, "bad solver parameters");
   if (p->smoother != MAC_SMOOTH_JACOBI && p->smoother != MAC_SMOOTH_CHEBYSHEV) return fail(MAC_E_ARG, "unknown smoother");
   if (!(p->viscosity >= 0)) return fail(MAC_E_ARG, "viscosity must be >= 0");
+  if (p->coarse_inner < 1 || p->coarse_inner > 8) return fail(MAC_E_ARG, "coarse_inner must lie in 1..8");
+  if (!(p->coarse_damp > 0 && p->coarse_damp <= 1)) return fail(MAC_E_ARG, "coarse_damp must lie in (0, 1]");
   if (p->smoother == MAC_SMOOTH_CHEBYSHEV && !(p->cheb_lower > 0 && p->cheb_lower < 2 && p->cheb_lower_coarse > 0 &&
                                                 p->cheb_lower_coarse < 2))
     return fail(MAC_E_ARG, "cheb_lower and cheb_lower_coarse must lie in (0, 2)");
@@ -580,6 +583,15 @@ inline int nu_of(const mac_params& p, int level) {
 void set_schedule(mac_ctx* c) {
   schedule(c->prm, nu_of(c->prm, 0), c->prm.cheb_lower, c->om_pre, c->om_post);
   schedule(c->prm, nu_of(c->prm, 1), c->prm.cheb_lower_coarse, c->omc_pre, c->omc_post);
+  double pre[8], post[8];
+  const int m = std::min(8, std::max(1, c->prm.coarse_inner));
+  schedule(c->prm, m, c->prm.cheb_lower_coarse, pre, post);
+  c->blockw.m = m;
+  c->blockw.damp = c->prm.coarse_damp;
+  for (int k = 0; k < 8; ++k) {
+    c->blockw.om64[k] = k < m ? pre[k] : 0.0;
+    c->blockw.om[k] = (float)c->blockw.om64[k];
+  }
 }
 
 // ---- the V-cycle ----
@@ -638,7 +650,17 @@ int vcycle(mac_ctx* c, TZ** zout) {
     TZ* res = (TZ*)lv.res;
     double nd = (double)lvl_dom(c, l);
     double sr = (l == 0) ? 8.0 : sz;
-    if (nu >= 2) {
+    // the block smoother runs on the coarse levels above the last two (those
+    // smooth point-wise, fused or not)
+    const bool block = l > 0 && l < nl - 2 && c->prm.coarse_inner > 1;
+    if (block) {
+      for (int k = 0; k < nu; ++k) {
+        TimedLaunch tl(c, K_C_JACOBI, nd * (2 * sz + (k ? 2 : 1) * sz + 1));
+        k_jacobi_block<TZ, TZ><<<grid_of(lv.nact), KTHREADS, 0, c->st>>>(lv.nact, lv.meta, lv.code, (const TZ*)lv.r, k ? z : nullptr,
+                                                                          k ? zt : z, ih[0], ih[1], ih[2], c->blockw, c->s);
+        if (k) std::swap(z, zt);
+      }
+    } else if (nu >= 2) {
       // the first two sweeps from zero in one pass
       TimedLaunch tl(c, l == 0 ? K_JACOBI2_0 : K_C_JACOBI2, nd * (sr + sz + 1));
       if (l == 0)
@@ -650,7 +672,7 @@ int vcycle(mac_ctx* c, TZ** zout) {
     } else {
       sweep<TZ>(c, l, nullptr, z, 0, opre[0]);
     }
-    for (int k = 2; k < nu; ++k) {
+    for (int k = 2; k < nu && !block; ++k) {
       sweep<TZ>(c, l, z, zt, 0, opre[k]);
       std::swap(z, zt);
     }
@@ -731,7 +753,14 @@ int vcycle(mac_ctx* c, TZ** zout) {
     const int nu = nu_of(c->prm, l);
     const double* opost = l ? c->omc_post : c->om_post;
     for (int k = 0; k < nu; ++k) {
-      sweep<TZ>(c, l, z, zt, (l == 0 && k == nu - 1) ? 1 : 0, opost[k]);
+      if (l > 0 && l < nl - 2 && c->prm.coarse_inner > 1) {
+        const TZ ih[3] = {(TZ)lv.ih2[0], (TZ)lv.ih2[1], (TZ)lv.ih2[2]};
+        TimedLaunch tl(c, K_C_JACOBI, nd * 4 * sz + nd);
+        k_jacobi_block<TZ, TZ><<<grid_of(lv.nact), KTHREADS, 0, c->st>>>(lv.nact, lv.meta, lv.code, (const TZ*)lv.r, z, zt, ih[0],
+                                                                          ih[1], ih[2], c->blockw, c->s);
+      } else {
+        sweep<TZ>(c, l, z, zt, (l == 0 && k == nu - 1) ? 1 : 0, opost[k]);
+      }
       std::swap(z, zt);
     }
     zfin[l] = z;
@@ -940,10 +969,12 @@ void mac_default_params(mac_params* p) {
   p->stall_iters = 0;
   p->smoother = MAC_SMOOTH_CHEBYSHEV;
   p->cheb_lower = 0.25;
-  p->nu_coarse = 4;
-  p->cheb_lower_coarse = 0.1;
+  p->nu_coarse = 2;
+  p->cheb_lower_coarse = 0.05;
   p->coarse_tail = 1;
   p->viscosity = 0.0;
+  p->coarse_inner = 4;
+  p->coarse_damp = 0.8;
 }
 
 int mac_create(mac_ctx** out, const mac_params* prm) {

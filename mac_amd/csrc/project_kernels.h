@@ -72,6 +72,44 @@ __device__ __forceinline__ bool converged(const int* __restrict__ meta, int stri
 // 64-value face plane per load (x faces indexed (y, z), y faces (x, z)), and
 // handed to the edge lanes by cross-lane shuffles; the z faces are ordinary
 // rows.  The neighbour-brick tests are wave-uniform, so nothing diverges.
+// The six face planes a column stencil needs from the neighbour bricks.
+template <typename T>
+struct Halo {
+  T xm, xp, ym, yp, below, above;
+};
+template <typename T, typename Ld>
+__device__ __forceinline__ Halo<T> load_halo(const int* M, int lane, Ld ld) {
+  const int x = lane & 7, y = lane >> 3;
+  const int bxm = M[1], bxp = M[2], bym = M[3], byp = M[4], bzm = M[5], bzp = M[6];
+  Halo<T> h;
+  h.xm = (bxm >= 0) ? ld(bxm, 7 + 8 * x + 64 * y) : T(0);  // lane = y' + 8 z'
+  h.xp = (bxp >= 0) ? ld(bxp, 8 * x + 64 * y) : T(0);
+  h.ym = (bym >= 0) ? ld(bym, x + 56 + 64 * y) : T(0);     // lane = x' + 8 z'
+  h.yp = (byp >= 0) ? ld(byp, x + 64 * y) : T(0);
+  h.below = (bzm >= 0) ? ld(bzm, lane + 448) : T(0);
+  h.above = (bzp >= 0) ? ld(bzp, lane) : T(0);
+  return h;
+}
+// col_stencil with the face planes already loaded.
+template <typename T, typename F>
+__device__ __forceinline__ void col_apply(int lane, const T (&col)[8], const Halo<T>& h, F f) {
+  const int x = lane & 7, y = lane >> 3;
+#pragma unroll
+  for (int z = 0; z < 8; ++z) {
+    T v = col[z];
+    T xm = __shfl_up(v, 1), xp = __shfl_down(v, 1), ym = __shfl_up(v, 8), yp = __shfl_down(v, 8);
+    const T gxm = __shfl(h.xm, y + 8 * z), gxp = __shfl(h.xp, y + 8 * z);
+    const T gym = __shfl(h.ym, x + 8 * z), gyp = __shfl(h.yp, x + 8 * z);
+    if (x == 0) xm = gxm;
+    if (x == 7) xp = gxp;
+    if (y == 0) ym = gym;
+    if (y == 7) yp = gyp;
+    T zm = (z > 0) ? col[z > 0 ? z - 1 : 0] : h.below;
+    T zp = (z < 7) ? col[z < 7 ? z + 1 : 7] : h.above;
+    f(z, xm, xp, ym, yp, zm, zp);
+  }
+}
+
 template <typename T, typename Ld, typename F>
 __device__ __forceinline__ void col_stencil(const int* M, int lane, const T (&col)[8], Ld ld, F f) {
   const int x = lane & 7, y = lane >> 3;
@@ -504,6 +542,63 @@ __global__ void __launch_bounds__(KTHREADS) k_jacobi(int nact, const int* __rest
     }
   }
   if (want_dot) write_partial(acc, partials, sh, false);
+}
+
+// Block-Jacobi smoothing step: the brick's unknowns take m damped-Jacobi
+// sweeps (weights om[0..m-1]) among themselves with the neighbour bricks'
+// face planes frozen at zin, i.e. zout = zin + q(D^-1 A_BB) D^-1 (r - A zin)
+// on each brick (q the sweeps' polynomial) scaled by w.damp, a symmetric
+// block smoother.  The
+// sweeps after the first run in registers and cross-lane shuffles.
+// zin == nullptr: from zero.
+struct BlockW {
+  int m;
+  float om[8];
+  double om64[8];
+  double damp;  // zout = zin + damp (block correction)
+};
+template <typename TR, typename TZ>
+__global__ void __launch_bounds__(KTHREADS) k_jacobi_block(int nact, const int* __restrict__ meta,
+                                                           const uint8_t* __restrict__ code, const TR* __restrict__ r,
+                                                           const TZ* __restrict__ zin, TZ* __restrict__ zout, TZ ihx2, TZ ihy2,
+                                                           TZ ihz2, BlockW w, const Scal* s) {
+  if (converged(meta, MSTRIDE, nact, blockIdx.x * WPB + wave_in_block(), s)) return;
+  const int lane = lane_id();
+  FOR_EACH_BRICK(meta, nact, M) {
+    const int b = M[0];
+    const size_t base = (size_t)b * MAC_BC + lane;
+    unsigned cd[8];
+    TZ col[8], rv[8];
+#pragma unroll
+    for (int z = 0; z < 8; ++z) {
+      cd[z] = code[base + 64 * z];
+      rv[z] = (TZ)r[base + 64 * z];
+      col[z] = zin ? zin[base + 64 * z] : TZ(0);
+    }
+    Halo<TZ> h{TZ(0), TZ(0), TZ(0), TZ(0), TZ(0), TZ(0)};
+    if (zin) h = load_halo<TZ>(M, lane, [&](int nb, int cell) { return zin[(size_t)nb * MAC_BC + cell]; });
+    TZ col0[8];
+#pragma unroll
+    for (int z = 0; z < 8; ++z) col0[z] = col[z];
+    for (int k = 0; k < w.m; ++k) {
+      const TZ om = sizeof(TZ) == 4 ? (TZ)w.om[k] : (TZ)w.om64[k];
+      TZ nw[8];
+      col_apply<TZ>(lane, col, h, [&](int z, TZ xm, TZ xp, TZ ym, TZ yp, TZ zm, TZ zp) {
+        TZ out = TZ(0);
+        if (cd[z] & CODE_DOM) {
+          TZ dg;
+          const TZ lz = lap_cell<TZ>(cd[z], col[z], xm, xp, ym, yp, zm, zp, ihx2, ihy2, ihz2, dg);
+          out = dg > TZ(0) ? col[z] + om * (rv[z] - lz) / dg : TZ(0);
+        }
+        nw[z] = out;
+      });
+#pragma unroll
+      for (int z = 0; z < 8; ++z) col[z] = nw[z];
+    }
+    const TZ dmp = (TZ)w.damp;
+#pragma unroll
+    for (int z = 0; z < 8; ++z) zout[base + 64 * z] = col0[z] + dmp * (col[z] - col0[z]);
+  }
 }
 
 // Two damped-Jacobi sweeps from a zero start in one pass:

@@ -40,6 +40,8 @@ class Params(ctypes.Structure):
         ("cheb_lower_coarse", ctypes.c_double),
         ("coarse_tail", ctypes.c_int),
         ("viscosity", ctypes.c_double),
+        ("coarse_inner", ctypes.c_int),
+        ("coarse_damp", ctypes.c_double),
     ]
 
 

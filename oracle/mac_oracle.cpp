@@ -458,6 +458,52 @@ struct MG {
   int nuc = 2;
   double omega = 0.8;                        // weight of the sweep being run
   int nu = 2;
+  // coarse block smoother (mac_params.coarse_inner > 1): inner sweeps of each
+  // 8^3 block with its neighbours frozen, scaled by damp
+  int inner = 1;
+  double omi[8] = {0};
+  double damp = 1.0;
+
+  void jacobi_block(DLevel& l, const std::vector<double>& r, const std::vector<double>* zin, std::vector<double>& zout) {
+    const int bx = l.n[0] / 8, by = l.n[1] / 8, bz = l.n[2] / 8;
+#pragma omp parallel for schedule(static)
+    for (long long B = 0; B < (long long)bx * by * bz; ++B) {
+      const int X0 = 8 * (int)(B % bx), Y0 = 8 * (int)((B / bx) % by), Z0 = 8 * (int)(B / ((long long)bx * by));
+      double z0[512], zc[512], zn[512];
+      for (int t = 0; t < 512; ++t) z0[t] = zin ? (*zin)[l.id(X0 + (t & 7), Y0 + ((t >> 3) & 7), Z0 + (t >> 6))] : 0.0;
+      std::copy(z0, z0 + 512, zc);
+      for (int k = 0; k < inner; ++k) {
+        for (int t = 0; t < 512; ++t) {
+          const int x = t & 7, y = (t >> 3) & 7, z = t >> 6;
+          const long long c = l.id(X0 + x, Y0 + y, Z0 + z);
+          const unsigned cd = l.code[c];
+          double out = 0;
+          if (cd & 0x40) {
+            double acc = 0, d = 0;
+            for (int q = 0; q < 6; ++q) {
+              if (!(cd & (1u << q))) continue;
+              const double w = l.ih2[q >> 1];
+              d += w;
+              const int xx = x + DX[q], yy = y + DY[q], zz = z + DZ[q];
+              double v;
+              if (xx >= 0 && yy >= 0 && zz >= 0 && xx < 8 && yy < 8 && zz < 8) {
+                v = zc[xx + 8 * yy + 64 * zz];
+              } else {
+                const int gx = X0 + xx, gy = Y0 + yy, gz = Z0 + zz;
+                v = (zin && gx >= 0 && gy >= 0 && gz >= 0 && gx < l.n[0] && gy < l.n[1] && gz < l.n[2]) ? (*zin)[l.id(gx, gy, gz)] : 0.0;
+              }
+              acc += w * v;
+            }
+            const double lz = d * zc[t] - acc;
+            out = d > 0 ? zc[t] + omi[k] * (r[c] - lz) / d : 0;
+          }
+          zn[t] = out;
+        }
+        std::copy(zn, zn + 512, zc);
+      }
+      for (int t = 0; t < 512; ++t) zout[l.id(X0 + (t & 7), Y0 + ((t >> 3) & 7), Z0 + (t >> 6))] = z0[t] + damp * (zc[t] - z0[t]);
+    }
+  }
 
   double lap(const DLevel& l, const std::vector<double>& v, int x, int y, int z, double& dg) const {
     long long c = l.id(x, y, z);
@@ -575,12 +621,20 @@ struct MG {
       DLevel& lv = L[l];
       const double* pre = l ? omc_pre : om_pre;
       const int n = l ? nuc : nu;
-      omega = pre[0];
-      jacobi(lv, lv.r, nullptr, lv.z);
-      for (int k = 1; k < n; ++k) {
-        omega = pre[k];
-        jacobi(lv, lv.r, &lv.z, lv.zt);
-        std::swap(lv.z, lv.zt);
+      if (l > 0 && l < nl - 2 && inner > 1) {
+        jacobi_block(lv, lv.r, nullptr, lv.z);
+        for (int k = 1; k < n; ++k) {
+          jacobi_block(lv, lv.r, &lv.z, lv.zt);
+          std::swap(lv.z, lv.zt);
+        }
+      } else {
+        omega = pre[0];
+        jacobi(lv, lv.r, nullptr, lv.z);
+        for (int k = 1; k < n; ++k) {
+          omega = pre[k];
+          jacobi(lv, lv.r, &lv.z, lv.zt);
+          std::swap(lv.z, lv.zt);
+        }
       }
       residual(lv, lv.r, lv.z, lv.res);
       restrict_(lv, L[l + 1]);
@@ -592,8 +646,12 @@ struct MG {
       const double* post = l ? omc_post : om_post;
       const int n = l ? nuc : nu;
       for (int k = 0; k < n; ++k) {
-        omega = post[k];
-        jacobi(lv, lv.r, &lv.z, lv.zt);
+        if (l > 0 && l < nl - 2 && inner > 1) {
+          jacobi_block(lv, lv.r, &lv.z, lv.zt);
+        } else {
+          omega = post[k];
+          jacobi(lv, lv.r, &lv.z, lv.zt);
+        }
         std::swap(lv.z, lv.zt);
       }
     }
@@ -611,7 +669,7 @@ extern "C" {
 // relative residual, out[4] = levels.  Returns 0, or -1 on non-convergence.
 int orc_mgpcg(int nb, const int32_t* keys, const uint8_t* flags, const double* h, const double* u, const double* v,
               const double* w, double* p, double rtol, int maxit, int nu, double omega, int smoother, double cheb_lower,
-              int nu_coarse, double cheb_lower_coarse, int nthreads, double* out) {
+              int nu_coarse, double cheb_lower_coarse, int coarse_inner, double coarse_damp, int nthreads, double* out) {
   if (nthreads > 0) omp_set_num_threads(nthreads);
   auto t0 = std::chrono::steady_clock::now();
   double inv_h[3] = {1.0 / h[0], 1.0 / h[1], 1.0 / h[2]};
@@ -637,6 +695,12 @@ int orc_mgpcg(int nb, const int32_t* keys, const uint8_t* flags, const double* h
   };
   sched(mg.nu, cheb_lower, mg.om_pre, mg.om_post);
   sched(mg.nuc, cheb_lower_coarse, mg.omc_pre, mg.omc_post);
+  mg.inner = std::min(8, std::max(1, coarse_inner));
+  mg.damp = coarse_damp;
+  {
+    double post[8];
+    sched(mg.inner, cheb_lower_coarse, mg.omi, post);
+  }
   // level 0 dense over the brick bounding box
   {
     DLevel l0;

@@ -50,7 +50,8 @@ def load():
     lib.orc_apply_A.argtypes = [ctypes.c_int, _P, _P, _P, _P, _P]
     lib.orc_mgpcg.restype = ctypes.c_int
     lib.orc_mgpcg.argtypes = [ctypes.c_int, _P, _P, _P, _P, _P, _P, _P, ctypes.c_double, ctypes.c_int, ctypes.c_int,
-                              ctypes.c_double, ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_double, ctypes.c_int, _P]
+                              ctypes.c_double, ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_double, ctypes.c_int,
+                              ctypes.c_double, ctypes.c_int, _P]
     _lib = lib
     return lib
 
@@ -174,8 +175,8 @@ def project_dirichlet(grid, viscosity, rtol=1e-14, real=None):
     return {"p": p, "u": u, "v": v, "w": w, "max_div": md.value, "iters": it}
 
 
-def mgpcg(grid, rtol=1e-8, maxit=500, nu=2, omega=0.9, threads=0, smoother=1, cheb_lower=0.25, nu_coarse=4,
-          cheb_lower_coarse=0.1):
+def mgpcg(grid, rtol=1e-8, maxit=500, nu=2, omega=0.9, threads=0, smoother=1, cheb_lower=0.25, nu_coarse=2,
+          cheb_lower_coarse=0.05, coarse_inner=4, coarse_damp=0.8):
     """CPU MG-PCG pressure (the same algorithm as the device solver).
     Returns dict(p, iters, ms_solve, ms_setup, true_rel_resid, levels, ok)."""
     keys = _c(grid["keys"], np.int32)
@@ -186,7 +187,7 @@ def mgpcg(grid, rtol=1e-8, maxit=500, nu=2, omega=0.9, threads=0, smoother=1, ch
     out = np.zeros(5)
     h = _c(grid["h"], np.float64)
     rc = load().orc_mgpcg(nb, _p(keys), _p(flags), _p(h), _p(u), _p(v), _p(w), _p(p), rtol, maxit, nu, omega, smoother,
-                          cheb_lower, nu_coarse, cheb_lower_coarse, threads, _p(out))
+                          cheb_lower, nu_coarse, cheb_lower_coarse, coarse_inner, coarse_damp, threads, _p(out))
     return {"p": p, "iters": int(out[0]), "ms_solve": out[1], "ms_setup": out[2], "true_rel_resid": out[3],
             "levels": int(out[4]), "ok": rc == 0}
 

@@ -35,7 +35,7 @@ KERNEL_NAMES = {
     "divergence": "k_div", "cg_pupdate_matvec": "k_pmatvec<{z}>", "cg_update": "k_update",
     "jacobi2_l0": "k_jacobi2<double, {z}>", "jacobi_l0": "k_jacobi<double, {z}>", "residual_l0": "k_residual<double, {z}>",
     "prolong_l0": "k_prolong_col<{z}> (level 0)", "restrict_l1": "k_restrict_sep<{z}> (into level 1)",
-    "coarse_jacobi2": "k_jacobi2<{z}, {z}> (levels 1-3)", "coarse_jacobi": "k_jacobi<{z}, {z}> (levels 1-3)",
+    "coarse_jacobi2": "k_jacobi2<{z}, {z}> (levels 1-3)", "coarse_jacobi": "k_jacobi_block<{z}, {z}> (levels 1-3)",
     "coarse_residual": "k_residual<{z}, {z}> (levels 1-3)", "coarse_restrict": "k_restrict_sep<{z}> (into levels 2-4)",
     "coarse_prolong": "k_prolong_col<{z}> (levels 1-3)", "coarse_tail": "k_vcycle_tail<{z}> (levels 4-5)",
     "gradient_update": "k_gradient", "max_div": "k_maxdiv", "resid_init": "k_resid0",

@@ -577,21 +577,40 @@ __global__ void __launch_bounds__(KTHREADS) k_jacobi_block(int nact, const int* 
     }
     Halo<TZ> h{TZ(0), TZ(0), TZ(0), TZ(0), TZ(0), TZ(0)};
     if (zin) h = load_halo<TZ>(M, lane, [&](int nb, int cell) { return zin[(size_t)nb * MAC_BC + cell]; });
+    // the frozen face values each edge lane reads, redistributed once
+    const int x = lane & 7, y = lane >> 3;
+    TZ gx[8], gy[8];
+#pragma unroll
+    for (int z = 0; z < 8; ++z) {
+      const TZ gxm = __shfl(h.xm, y + 8 * z), gxp = __shfl(h.xp, y + 8 * z);
+      const TZ gym = __shfl(h.ym, x + 8 * z), gyp = __shfl(h.yp, x + 8 * z);
+      gx[z] = x == 0 ? gxm : gxp;  // only the edge lanes use them
+      gy[z] = y == 0 ? gym : gyp;
+    }
     TZ col0[8];
 #pragma unroll
     for (int z = 0; z < 8; ++z) col0[z] = col[z];
     for (int k = 0; k < w.m; ++k) {
       const TZ om = sizeof(TZ) == 4 ? (TZ)w.om[k] : (TZ)w.om64[k];
       TZ nw[8];
-      col_apply<TZ>(lane, col, h, [&](int z, TZ xm, TZ xp, TZ ym, TZ yp, TZ zm, TZ zp) {
+#pragma unroll
+      for (int z = 0; z < 8; ++z) {
+        const TZ v = col[z];
+        TZ xm = __shfl_up(v, 1), xp = __shfl_down(v, 1), ym = __shfl_up(v, 8), yp = __shfl_down(v, 8);
+        if (x == 0) xm = gx[z];
+        if (x == 7) xp = gx[z];
+        if (y == 0) ym = gy[z];
+        if (y == 7) yp = gy[z];
+        const TZ zm = (z > 0) ? col[z > 0 ? z - 1 : 0] : h.below;
+        const TZ zp = (z < 7) ? col[z < 7 ? z + 1 : 7] : h.above;
         TZ out = TZ(0);
         if (cd[z] & CODE_DOM) {
           TZ dg;
-          const TZ lz = lap_cell<TZ>(cd[z], col[z], xm, xp, ym, yp, zm, zp, ihx2, ihy2, ihz2, dg);
-          out = dg > TZ(0) ? col[z] + om * (rv[z] - lz) / dg : TZ(0);
+          const TZ lz = lap_cell<TZ>(cd[z], v, xm, xp, ym, yp, zm, zp, ihx2, ihy2, ihz2, dg);
+          out = dg > TZ(0) ? v + om * (rv[z] - lz) / dg : TZ(0);
         }
         nw[z] = out;
-      });
+      }
 #pragma unroll
       for (int z = 0; z < 8; ++z) col[z] = nw[z];
     }

@@ -44,11 +44,11 @@ KERNEL_NAMES = {
 # HBM bytes per launch measured by two counter runs (FETCH_SIZE x 2.0, WRITE_SIZE
 # x 1.0, scaled by a 256 MiB copy) of this benchmark's default input (256^3,
 # seed 0, mixed method): profiles/projection_256_mixed_hbm_traffic.md.  The
-# coarse classes are the launch-weighted mean over levels 1-4.
-MEASURED_TRAFFIC = {("jacobi_l0", 256, 0): 121.04e6, ("cg_pupdate_matvec", 256, 0): 249.72e6, ("cg_update", 256, 0): 253.66e6,
-                    ("jacobi2_l0", 256, 0): 129.79e6, ("residual_l0", 256, 0): 120.93e6, ("prolong_l0", 256, 0): 60.06e6,
-                    ("restrict_l1", 256, 0): 78.41e6, ("coarse_jacobi", 256, 0): 4.89e6, ("coarse_jacobi2", 256, 0): 4.05e6,
-                    ("coarse_residual", 256, 0): 4.87e6}
+# coarse classes are the launch-weighted mean over their levels.
+MEASURED_TRAFFIC = {("jacobi_l0", 256, 0): 119.89e6, ("cg_pupdate_matvec", 256, 0): 249.77e6, ("cg_update", 256, 0): 253.65e6,
+                    ("jacobi2_l0", 256, 0): 128.41e6, ("residual_l0", 256, 0): 119.76e6, ("prolong_l0", 256, 0): 59.54e6,
+                    ("restrict_l1", 256, 0): 77.76e6, ("coarse_jacobi", 256, 0): 5.88e6, ("coarse_residual", 256, 0): 6.41e6,
+                    ("coarse_prolong", 256, 0): 3.47e6, ("coarse_restrict", 256, 0): 5.33e6, ("coarse_tail", 256, 0): 0.06e6}
 
 
 def log(*a):
@@ -194,6 +194,44 @@ def jet_runs(args):
             s.close()
         log(f"lateral jet: fp64 {r['fp64_ms_per_step']:.2f} / mixed {r['mixed_ms_per_step']:.2f} ms/step, {r['mixed_vs_fp64']}")
         res["jet_lateral_192x96x96"] = r
+    # the reference jets' own step (test/Jet3.cpp): RK4 "all" with the depth
+    # chain, the floor, the cube fill and the viscous air pressures
+    if args.jet3_steps > 0:
+        from mac_amd.lib import MAC_FP64
+        jet = J.lateral_jet(192, 96, 96, h=0.01, nozzle=6, seed=args.seed)
+        nu = 1.307e-6
+        kw = dict(jet["params"], layer_max=4, integrator=2, seed=args.seed, particles_per_inflow_cell=0, fill_cube_n=1,
+                  depth_interior=3, depth_remove=5, floor_axis=2, floor_value=0.0, viscosity=nu)
+        proj = {"rtol": 1e-8, "precision": MAC_FP64, "viscosity": nu}
+        r = {"workload": f"lateral jet 192x96x96, h=0.01, nozzle 6x6 at {jet['inflow']:.3f} m/s, the Jet3 step: RK4 all "
+                         "(4 cell-type updates and 4 acceleration projections per step), depth (3, 5), floor z=0, "
+                         f"27-particle cube fill, nu={nu}, FP64 rtol 1e-8"}
+        s = _sim(jet, kw, proj)
+        t0, iters = time.perf_counter(), []
+        for _ in range(args.jet3_steps):
+            st = s.step()
+            iters.append(sum(st["pcg_iters"]))
+        r.update({"steps": args.jet3_steps, "gpu_ms_per_step": (time.perf_counter() - t0) * 1e3 / args.jet3_steps,
+                  "pcg_iters_per_step": float(np.mean(iters)), "final": {"bricks": st["nbricks"], "particles": st["nparticles"],
+                                                                         "fluid_cells": st["nfluid"]}})
+        s.close()
+        if not args.no_cpu and args.jet_cpu_steps > 0:
+            from oracle import oracle as O
+            n = min(5, args.jet3_steps)
+            s = _sim(jet, kw, dict(proj, rtol=1e-12))
+            for _ in range(n):
+                s.step()
+            dg = s.download()
+            s.close()
+            o = O.Sim(**kw, proj={"viscosity": nu})
+            o.upload(jet["keys"], jet["flags"], u=jet["u"], v=jet["v"], w=jet["w"])
+            t0 = time.perf_counter()
+            for _ in range(n):
+                o.step()
+            r.update({"cpu_steps": n, "cpu_ms_per_step": (time.perf_counter() - t0) * 1e3 / n, "cpu_cores": 1,
+                      "vs_oracle": _compare(dg, o.download())})
+        log(f"jet3 step: {r['gpu_ms_per_step']:.2f} ms/step over {args.jet3_steps} steps, {r.get('vs_oracle')}")
+        res["jet3_lateral_192x96x96"] = r
     return res
 
 
@@ -214,6 +252,7 @@ def main():
     ap.add_argument("--jet-steps", type=int, default=100, help="time steps of the 32x32x64 jet (0: skip)")
     ap.add_argument("--jet-cpu-steps", type=int, default=20, help="steps of the jet run on the oracle for parity/baseline")
     ap.add_argument("--lateral-steps", type=int, default=500, help="time steps of the 192x96x96 lateral jet (0: skip)")
+    ap.add_argument("--jet3-steps", type=int, default=100, help="RK4-all (Jet3) steps of the 192x96x96 lateral jet (0: skip)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only: --gpus {args.gpus} is not supported (the projection runs on a "

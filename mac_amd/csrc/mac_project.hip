@@ -655,7 +655,7 @@ int vcycle(mac_ctx* c, TZ** zout) {
     const bool block = l > 0 && l < nl - 2 && c->prm.coarse_inner > 1;
     if (block) {
       for (int k = 0; k < nu; ++k) {
-        TimedLaunch tl(c, K_C_JACOBI, nd * (2 * sz + (k ? 2 : 1) * sz + 1));
+        TimedLaunch tl(c, K_C_JACOBI, nd * ((k ? 3 : 2) * sz + 1));  // r, zin, zout, code
         k_jacobi_block<TZ, TZ><<<grid_of(lv.nact), KTHREADS, 0, c->st>>>(lv.nact, lv.meta, lv.code, (const TZ*)lv.r, k ? z : nullptr,
                                                                           k ? zt : z, ih[0], ih[1], ih[2], c->blockw, c->s);
         if (k) std::swap(z, zt);
@@ -755,7 +755,7 @@ int vcycle(mac_ctx* c, TZ** zout) {
     for (int k = 0; k < nu; ++k) {
       if (l > 0 && l < nl - 2 && c->prm.coarse_inner > 1) {
         const TZ ih[3] = {(TZ)lv.ih2[0], (TZ)lv.ih2[1], (TZ)lv.ih2[2]};
-        TimedLaunch tl(c, K_C_JACOBI, nd * 4 * sz + nd);
+        TimedLaunch tl(c, K_C_JACOBI, nd * (3 * sz + 1));
         k_jacobi_block<TZ, TZ><<<grid_of(lv.nact), KTHREADS, 0, c->st>>>(lv.nact, lv.meta, lv.code, (const TZ*)lv.r, z, zt, ih[0],
                                                                           ih[1], ih[2], c->blockw, c->s);
       } else {

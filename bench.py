@@ -232,6 +232,36 @@ def jet_runs(args):
                       "vs_oracle": _compare(dg, o.download())})
         log(f"jet3 step: {r['gpu_ms_per_step']:.2f} ms/step over {args.jet3_steps} steps, {r.get('vs_oracle')}")
         res["jet3_lateral_192x96x96"] = r
+    # the time step at scale: a 96x96x192-cell block of fluid moving at 1 m/s
+    # in a 256^3 box, 8 particles per cell (14.2M particles)
+    if args.big_steps > 0:
+        from mac_amd.lib import MAC_MIXED, Sim
+        b = J.fluid_block(seed=args.seed)
+        kw = dict(b["params"], layer_max=2, integrator=0, particle_rk=2, seed=args.seed, particles_per_inflow_cell=0)
+        s = Sim(**kw, proj={"rtol": 1e-8, "precision": MAC_MIXED})
+        s.upload(b["keys"], b["flags"], u=b["u"], v=b["v"], w=b["w"], particles=b["particles"])
+        n0 = len(b["particles"])
+        del b
+        s.step()  # builds the air layers and the first hierarchy
+        parts = {k: 0.0 for k in ("ms_topology", "ms_forces", "ms_project", "ms_particles")}
+        iters, t0 = [], time.perf_counter()
+        for _ in range(args.big_steps):
+            st = s.step()
+            iters.append(sum(st["pcg_iters"]))
+            for k in parts:
+                parts[k] += st[k] / args.big_steps
+        wall = (time.perf_counter() - t0) * 1e3 / args.big_steps
+        npart = st["nparticles"]
+        r = {"workload": "fluid block 96x96x192 cells at 1 m/s in a 256^3 box (h=0.01), 8 particles per cell, RK2, "
+                         "particle RK2, mixed projection rtol 1e-8",
+             "particles_start": n0, "particles_end": npart, "fluid_cells": st["nfluid"], "bricks": st["nbricks"],
+             "steps": args.big_steps, "gpu_ms_per_step": wall, "gpu_ms_breakdown": parts,
+             "pcg_iters_per_step": float(np.mean(iters)),
+             "particle_move_Mparticles_per_s": npart / (parts["ms_particles"] * 1e-3) / 1e6,
+             "particle_move_GBps_48B": npart * 48 / (parts["ms_particles"] * 1e-3) / 1e9}
+        s.close()
+        log(f"fluid block: {wall:.1f} ms/step, {parts}, {npart} particles")
+        res["block_256_14M_particles"] = r
     return res
 
 
@@ -253,6 +283,7 @@ def main():
     ap.add_argument("--jet-cpu-steps", type=int, default=20, help="steps of the jet run on the oracle for parity/baseline")
     ap.add_argument("--lateral-steps", type=int, default=500, help="time steps of the 192x96x96 lateral jet (0: skip)")
     ap.add_argument("--jet3-steps", type=int, default=100, help="RK4-all (Jet3) steps of the 192x96x96 lateral jet (0: skip)")
+    ap.add_argument("--big-steps", type=int, default=10, help="steps of the 14M-particle fluid block (0: skip)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only: --gpus {args.gpus} is not supported (the projection runs on a "

@@ -14,6 +14,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <map>
 #include <unordered_map>
 #include <vector>
 
@@ -131,7 +132,10 @@ struct mac_ctx {
   double kt_ms[K_N] = {0};
   long long kt_n[K_N] = {0};
   double kt_bytes[K_N] = {0};
-  std::vector<void*> allocs;
+  // device allocations of the current grid, and those of the grid before,
+  // kept for reuse by the next build (a time step rebuilds every step)
+  std::vector<std::pair<void*, size_t>> allocs;
+  std::multimap<size_t, void*> pool;
 };
 
 namespace {
@@ -144,23 +148,45 @@ void drop_graphs(mac_ctx* c) {
   }
 }
 
-void free_all(mac_ctx* c) {
+void free_all(mac_ctx* c, bool keep_for_reuse = false) {
   drop_graphs(c);
   c->solves_on_grid = 0;
-  for (void* p : c->allocs) hipFree(p);
+  for (auto& a : c->allocs) {
+    if (keep_for_reuse) c->pool.emplace(a.second, a.first);
+    else hipFree(a.first);
+  }
   c->allocs.clear();
+  if (!keep_for_reuse) {
+    for (auto& kv : c->pool) hipFree(kv.second);
+    c->pool.clear();
+  }
   c->pD = nullptr;
   c->L.clear();
   c->have_grid = false;
 }
 
+// Device array of n elements: a block of the last grid's of at least that
+// size and at most twice it, else a new one.  Contents are undefined.
 template <typename T>
 int dalloc(mac_ctx* c, T** p, size_t n) {
   *p = nullptr;
   if (n == 0) n = 1;
-  HIPCHK(hipMalloc((void**)p, n * sizeof(T)));
-  c->allocs.push_back((void*)*p);
+  const size_t bytes = (n * sizeof(T) + 255) & ~(size_t)255;
+  auto it = c->pool.lower_bound(bytes);
+  if (it != c->pool.end() && it->first <= 2 * bytes) {
+    *p = (T*)it->second;
+    c->allocs.emplace_back(it->second, it->first);
+    c->pool.erase(it);
+    return 0;
+  }
+  HIPCHK(hipMalloc((void**)p, bytes));
+  c->allocs.emplace_back((void*)*p, bytes);
   return 0;
+}
+// Blocks of the last grid that this one did not take go back to the device.
+void drain_pool(mac_ctx* c) {
+  for (auto& kv : c->pool) hipFree(kv.second);
+  c->pool.clear();
 }
 
 template <typename T>
@@ -317,7 +343,7 @@ void dense_inverse(const std::vector<uint8_t>& code, const double ih2[3], std::v
 }
 
 int build_grid(mac_ctx* c, int nbricks, const int32_t* keys3, const uint8_t* flags) {
-  free_all(c);
+  free_all(c, true);
   c->nb0 = nbricks;
   c->keys0.assign(keys3, keys3 + 3 * (size_t)nbricks);
   // duplicate check + shift
@@ -543,6 +569,7 @@ int build_grid(mac_ctx* c, int nbricks, const int32_t* keys3, const uint8_t* fla
   CHK(dalloc(c, &c->s, 1));
   HIPCHK(hipMemsetAsync(c->s, 0, sizeof(Scal), c->st));
   HIPCHK(hipStreamSynchronize(c->st));
+  drain_pool(c);
   c->have_grid = true;
   return 0;
 }

@@ -56,7 +56,7 @@ struct mac_sim {
   hipStream_t st = nullptr;
   mac_ctx* proj = nullptr;
   std::vector<Key3> keys;
-  int nb = 0, capb = 0;
+  int nb = 0;
   // device brick arrays (capacity capb)
   int* d_keys = nullptr;
   uint8_t* flags = nullptr;
@@ -89,6 +89,23 @@ struct mac_sim {
   double* d_part = nullptr;
   unsigned long long tick = 0;
   bool grid_ready = false;
+  // allocation sizes: the per-cell arrays above hold cur_cap bricks; a second
+  // set of the same arrays (spare, spare_cap bricks) is the target of the
+  // next relayout, after which the two swap, so a changing brick set costs
+  // no allocation until it outgrows them
+  int cur_cap = 0, spare_cap = 0;
+  struct Cells {
+    uint8_t* flags = nullptr;
+    int8_t* layer = nullptr;
+    int16_t* depth = nullptr;
+    double* S[3][3] = {{nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr}};
+    double* A[3] = {nullptr, nullptr, nullptr};
+    double* P = nullptr;
+    double* Pprev = nullptr;
+  } spare;
+  int keys_cap = 0, hash_cap = 0;
+  uint8_t* d_front = nullptr;  // the extrapolation's two frontier masks
+  size_t front_cap = 0;
   // starting iterate of the projections: 0 zero, 1 last pressure, 2 learned
   int guess_mode = 0;
   mac_unet* net = nullptr;
@@ -97,27 +114,77 @@ struct mac_sim {
 
 namespace {
 
-void sim_free_bricks(mac_sim* s) {
-  hipFree(s->d_keys);
-  hipFree(s->flags);
-  hipFree(s->layer);
-  hipFree(s->depth);
+mac_sim::Cells cur_cells(const mac_sim* s) {
+  mac_sim::Cells c;
+  c.flags = s->flags;
+  c.layer = s->layer;
+  c.depth = s->depth;
   for (int a = 0; a < 3; ++a)
-    for (int i = 0; i < 3; ++i) hipFree(s->S[a][i]);
-  for (int i = 0; i < 3; ++i) hipFree(s->A[i]);
-  hipFree(s->P);
-  hipFree(s->Pprev);
+    for (int i = 0; i < 3; ++i) c.S[a][i] = s->S[a][i];
+  for (int i = 0; i < 3; ++i) c.A[i] = s->A[i];
+  c.P = s->P;
+  c.Pprev = s->Pprev;
+  return c;
+}
+void set_cur_cells(mac_sim* s, const mac_sim::Cells& c) {
+  s->flags = c.flags;
+  s->layer = c.layer;
+  s->depth = c.depth;
+  for (int a = 0; a < 3; ++a)
+    for (int i = 0; i < 3; ++i) s->S[a][i] = c.S[a][i];
+  for (int i = 0; i < 3; ++i) s->A[i] = c.A[i];
+  s->P = c.P;
+  s->Pprev = c.Pprev;
+}
+void free_cells(mac_sim::Cells& c) {
+  hipFree(c.flags);
+  hipFree(c.layer);
+  hipFree(c.depth);
+  for (int a = 0; a < 3; ++a)
+    for (int i = 0; i < 3; ++i) hipFree(c.S[a][i]);
+  for (int i = 0; i < 3; ++i) hipFree(c.A[i]);
+  hipFree(c.P);
+  hipFree(c.Pprev);
+  c = mac_sim::Cells();
+}
+int alloc_cells(mac_sim::Cells& c, int cap) {
+  const size_t cells = (size_t)std::max(cap, 1) * MAC_BC;
+  SCHK(hipMalloc(&c.flags, cells));
+  SCHK(hipMalloc(&c.layer, cells));
+  SCHK(hipMalloc(&c.depth, cells * sizeof(int16_t)));
+  for (int a = 0; a < 3; ++a)
+    for (int i = 0; i < 3; ++i) SCHK(hipMalloc(&c.S[a][i], cells * sizeof(double)));
+  for (int i = 0; i < 3; ++i) SCHK(hipMalloc(&c.A[i], cells * sizeof(double)));
+  SCHK(hipMalloc(&c.P, cells * sizeof(double)));
+  SCHK(hipMalloc(&c.Pprev, cells * sizeof(double)));
+  return 0;
+}
+
+void sim_free_bricks(mac_sim* s) {
+  mac_sim::Cells c = cur_cells(s);
+  free_cells(c);
+  set_cur_cells(s, c);
+  free_cells(s->spare);
+  s->cur_cap = s->spare_cap = 0;
+  hipFree(s->d_keys);
   hipFree(s->nbr);
-  hipFree(s->hk);
-  hipFree(s->hv);
   hipFree(s->d_bflag);
   hipFree(s->d_from);
   hipFree(s->d_part);
+  s->d_keys = s->nbr = s->d_bflag = s->d_from = nullptr;
+  s->d_part = nullptr;
+  s->keys_cap = 0;
+  hipFree(s->hk);
+  hipFree(s->hv);
+  s->hk = nullptr;
+  s->hv = nullptr;
+  s->hash_cap = 0;
 }
 
 // Replace the brick set by `nk` (sorted, unique).  Bricks present before keep
 // their data; new ones start as the prototype.  Rebuilds the hash and the
-// neighbour table.
+// neighbour table.  The cells are relaid into the spare arrays, which then
+// become the current ones.
 int sim_set_bricks(mac_sim* s, const std::vector<Key3>& nk) {
   if ((int)nk.size() > s->prm.max_bricks) return mac_internal_set_error(MAC_E_ARG, "brick pool exhausted (max_bricks)");
   const int nbn = (int)nk.size();
@@ -130,64 +197,62 @@ int sim_set_bricks(mac_sim* s, const std::vector<Key3>& nk) {
       ++j;
     }
   }
-  const size_t cells = (size_t)std::max(nbn, 1) * MAC_BC;
-  int* d_keys;
-  uint8_t* flags;
-  int8_t* layer;
-  int16_t* depth;
-  double* S[3][3];
-  double* A[3];
-  double *P, *Pprev;
-  SCHK(hipMalloc(&d_keys, sizeof(int) * 3 * std::max(nbn, 1)));
-  SCHK(hipMalloc(&flags, cells));
-  SCHK(hipMalloc(&layer, cells));
-  SCHK(hipMalloc(&depth, cells * sizeof(int16_t)));
-  for (int a = 0; a < 3; ++a)
-    for (int i = 0; i < 3; ++i) SCHK(hipMalloc(&S[a][i], cells * sizeof(double)));
-  for (int i = 0; i < 3; ++i) SCHK(hipMalloc(&A[i], cells * sizeof(double)));
-  SCHK(hipMalloc(&P, cells * sizeof(double)));
-  SCHK(hipMalloc(&Pprev, cells * sizeof(double)));
+  const int grow = std::min(s->prm.max_bricks, std::max(nbn + nbn / 4 + 8, 16));
+  if (s->spare_cap < nbn) {
+    free_cells(s->spare);
+    SRC(alloc_cells(s->spare, grow));
+    s->spare_cap = grow;
+  }
+  if (s->keys_cap < nbn || !s->d_keys) {
+    hipFree(s->d_keys);
+    hipFree(s->nbr);
+    hipFree(s->d_bflag);
+    hipFree(s->d_from);
+    hipFree(s->d_part);
+    SCHK(hipMalloc(&s->d_keys, sizeof(int) * 3 * grow));
+    SCHK(hipMalloc(&s->nbr, sizeof(int) * 27 * grow));
+    SCHK(hipMalloc(&s->d_bflag, sizeof(int) * grow));
+    SCHK(hipMalloc(&s->d_from, sizeof(int) * grow));
+    SCHK(hipMalloc(&s->d_part, sizeof(double) * grow));
+    s->keys_cap = grow;
+  }
   std::vector<int> hk3(3 * std::max(nbn, 1), 0);
   for (int b = 0; b < nbn; ++b) {
     hk3[3 * b] = nk[b].x;
     hk3[3 * b + 1] = nk[b].y;
     hk3[3 * b + 2] = nk[b].z;
   }
-  SCHK(hipMemcpyAsync(d_keys, hk3.data(), sizeof(int) * hk3.size(), hipMemcpyHostToDevice, s->st));
-  int* d_from;
-  SCHK(hipMalloc(&d_from, sizeof(int) * from.size()));
-  SCHK(hipMemcpyAsync(d_from, from.data(), sizeof(int) * from.size(), hipMemcpyHostToDevice, s->st));
+  SCHK(hipMemcpyAsync(s->d_keys, hk3.data(), sizeof(int) * hk3.size(), hipMemcpyHostToDevice, s->st));
+  SCHK(hipMemcpyAsync(s->d_from, from.data(), sizeof(int) * from.size(), hipMemcpyHostToDevice, s->st));
+  mac_sim::Cells& to = s->spare;
   if (nbn > 0) {
-    k_relayout_u8<<<nbn, 256, 0, s->st>>>(nbn, d_from, s->flags, flags, (uint8_t)0);
-    k_relayout_u8<<<nbn, 256, 0, s->st>>>(nbn, d_from, (const uint8_t*)s->layer, (uint8_t*)layer, (uint8_t)0xFF);
-    k_relayout_i16<<<nbn, 256, 0, s->st>>>(nbn, d_from, s->depth, depth, (int16_t)-1);
+    const int* d_from = s->d_from;
+    k_relayout_u8<<<nbn, 256, 0, s->st>>>(nbn, d_from, s->flags, to.flags, (uint8_t)0);
+    k_relayout_u8<<<nbn, 256, 0, s->st>>>(nbn, d_from, (const uint8_t*)s->layer, (uint8_t*)to.layer, (uint8_t)0xFF);
+    k_relayout_i16<<<nbn, 256, 0, s->st>>>(nbn, d_from, s->depth, to.depth, (int16_t)-1);
     for (int a = 0; a < 3; ++a)
-      for (int i = 0; i < 3; ++i) k_relayout_f64<<<nbn, 256, 0, s->st>>>(nbn, d_from, s->S[a][i], S[a][i]);
-    for (int i = 0; i < 3; ++i) k_relayout_f64<<<nbn, 256, 0, s->st>>>(nbn, d_from, s->A[i], A[i]);
-    k_relayout_f64<<<nbn, 256, 0, s->st>>>(nbn, d_from, s->P, P);
+      for (int i = 0; i < 3; ++i) k_relayout_f64<<<nbn, 256, 0, s->st>>>(nbn, d_from, s->S[a][i], to.S[a][i]);
+    for (int i = 0; i < 3; ++i) k_relayout_f64<<<nbn, 256, 0, s->st>>>(nbn, d_from, s->A[i], to.A[i]);
+    k_relayout_f64<<<nbn, 256, 0, s->st>>>(nbn, d_from, s->P, to.P);
   }
   SCHK(hipGetLastError());
-  SCHK(hipStreamSynchronize(s->st));
-  sim_free_bricks(s);
-  s->d_keys = d_keys;
-  s->flags = flags;
-  s->layer = layer;
-  s->depth = depth;
-  for (int a = 0; a < 3; ++a)
-    for (int i = 0; i < 3; ++i) s->S[a][i] = S[a][i];
-  for (int i = 0; i < 3; ++i) s->A[i] = A[i];
-  s->P = P;
-  s->Pprev = Pprev;
-  s->d_from = d_from;
+  {
+    mac_sim::Cells old = cur_cells(s);
+    set_cur_cells(s, to);
+    s->spare = old;
+    std::swap(s->cur_cap, s->spare_cap);
+  }
   s->keys = nk;
   s->nb = nbn;
   // hash (load factor <= 1/2) and neighbour table
   s->hcap = ceil_pow2(std::max(2 * nbn, 64));
-  SCHK(hipMalloc(&s->hk, sizeof(uint64_t) * s->hcap));
-  SCHK(hipMalloc(&s->hv, sizeof(int) * s->hcap));
-  SCHK(hipMalloc(&s->nbr, sizeof(int) * 27 * std::max(nbn, 1)));
-  SCHK(hipMalloc(&s->d_bflag, sizeof(int) * std::max(nbn, 1)));
-  SCHK(hipMalloc(&s->d_part, sizeof(double) * std::max(nbn, 1)));
+  if (s->hash_cap < s->hcap) {
+    hipFree(s->hk);
+    hipFree(s->hv);
+    SCHK(hipMalloc(&s->hk, sizeof(uint64_t) * s->hcap));
+    SCHK(hipMalloc(&s->hv, sizeof(int) * s->hcap));
+    s->hash_cap = s->hcap;
+  }
   k_hash_clear<<<(s->hcap + 255) / 256, 256, 0, s->st>>>(s->hcap, s->hk, s->hv);
   if (nbn > 0) {
     k_hash_insert<<<(nbn + 255) / 256, 256, 0, s->st>>>(nbn, s->d_keys, s->hk, s->hv, s->hcap);
@@ -487,6 +552,7 @@ void mac_sim_destroy(mac_sim* s) {
   hipFree(s->d_dest);
   hipFree(s->d_int);
   hipFree(s->d_any);
+  hipFree(s->d_front);
   hipFree(s->d_missing);
   if (s->proj) mac_destroy(s->proj);
   if (s->st) hipStreamDestroy(s->st);
@@ -704,37 +770,30 @@ namespace {
 int sim_extrapolate_fields(mac_sim* s, double* u, double* v, double* w) {
   if (s->nb == 0) return 0;
   const size_t cells = (size_t)s->nb * MAC_BC;
-  uint8_t* fr = nullptr;
-  int* dany = nullptr;
-  SCHK(hipMalloc(&fr, 2 * cells));
-  if (hipMalloc(&dany, sizeof(int)) != hipSuccess) {
-    hipFree(fr);
-    return mac_internal_set_error(MAC_E_HIP, "hipMalloc failed");
+  if (s->front_cap < 2 * cells) {
+    hipFree(s->d_front);
+    s->d_front = nullptr;
+    s->front_cap = 0;
+    SCHK(hipMalloc(&s->d_front, 2 * cells + cells / 4));
+    s->front_cap = 2 * cells + cells / 4;
   }
-  uint8_t* front = fr;
-  uint8_t* next = fr + cells;
-  int rc = 0;
+  uint8_t* front = s->d_front;
+  uint8_t* next = s->d_front + cells;
+  int* dany = s->d_any + 32;
   k_extrap_seed<<<(int)std::min<size_t>((cells + 255) / 256, 4096), 256, 0, s->st>>>(s->nb, s->flags, s->layer, front);
   // one frontier step per layer; layers are int8, so at most 127 steps
-  for (int lay = 1; lay < 128 && rc == 0; ++lay) {
+  for (int lay = 1; lay < 128; ++lay) {
     int any = 0;
-    if (hipMemsetAsync(next, 0, cells, s->st) != hipSuccess || hipMemsetAsync(dany, 0, sizeof(int), s->st) != hipSuccess) {
-      rc = mac_internal_set_error(MAC_E_HIP, "hipMemsetAsync failed");
-      break;
-    }
+    SCHK(hipMemsetAsync(next, 0, cells, s->st));
+    SCHK(hipMemsetAsync(dany, 0, sizeof(int), s->st));
     k_extrapolate<<<s->nb, 256, 0, s->st>>>(s->nb, s->nbr, s->flags, s->layer, u, v, w, front, next, dany, lay);
-    if (hipGetLastError() != hipSuccess || hipMemcpyAsync(&any, dany, sizeof(int), hipMemcpyDeviceToHost, s->st) != hipSuccess ||
-        hipStreamSynchronize(s->st) != hipSuccess) {
-      rc = mac_internal_set_error(MAC_E_HIP, "extrapolation launch failed");
-      break;
-    }
+    SCHK(hipGetLastError());
+    SCHK(hipMemcpyAsync(&any, dany, sizeof(int), hipMemcpyDeviceToHost, s->st));
+    SCHK(hipStreamSynchronize(s->st));
     if (!any) break;
     std::swap(front, next);
   }
-  hipStreamSynchronize(s->st);
-  hipFree(fr);
-  hipFree(dany);
-  return rc;
+  return 0;
 }
 
 // Algorithms_RungeKutta_all::Do (:35-156) for one step of size dt; the cell

@@ -73,3 +73,29 @@ def lateral_jet(nx=192, ny=96, nz=96, h=0.01, nozzle=6, inflow=None, seed=0):
                   box_hi=((nx - 0.5) * h, (ny - 0.5) * h, (nz - 0.5) * h))
     return {"keys": keys, "flags": flags, "u": u, "v": np.zeros(n), "w": np.zeros(n), "params": params,
             "inflow": float(inflow)}
+
+
+def fluid_block(dims=(96, 96, 192), offset=(80, 80, 32), box=256, h=0.01, ppc=8, vel=(1.0, 0.0, 0.0), seed=0):
+    """A large moving block of fluid for measuring the time step at scale: the
+    cells of a dims-sized block at `offset` inside a box^3 domain are Fluid,
+    each with `ppc` particles at seeded uniform positions inside the cell,
+    speed `vel` on their faces; particles leaving the box are deleted.  Only
+    the fluid's bricks exist at the start; the first cell-type update builds
+    the air layers."""
+    dims, offset = np.asarray(dims), np.asarray(offset)
+    lo_b, hi_b = offset // 8, (offset + dims - 1) // 8
+    bk = np.stack(np.meshgrid(*[np.arange(lo_b[i], hi_b[i] + 1) for i in range(3)], indexing="ij"), -1).reshape(-1, 3)
+    bk = bk[np.lexsort((bk[:, 0], bk[:, 1], bk[:, 2]))].astype(np.int32)
+    nb = len(bk)
+    t = np.arange(512)
+    g = bk[:, None, :] * 8 + np.stack([t & 7, (t >> 3) & 7, t >> 6], 1)[None, :, :]
+    inside = np.all((g >= offset) & (g < offset + dims), axis=-1).reshape(-1)
+    flags = np.where(inside, EXISTS | 1, 0).astype(np.uint8)
+    speed = [np.where(inside, float(vel[i]), 0.0) for i in range(3)]
+    cells = g.reshape(-1, 3)[inside]
+    rng = np.random.default_rng(seed)
+    pts = (np.repeat(cells, ppc, axis=0) + rng.uniform(-0.5, 0.5, size=(len(cells) * ppc, 3))) * h
+    params = dict(h=(h, h, h), gravity=(0.0, 0.0, -9.81), box_lo=(-0.5 * h,) * 3, box_hi=((box - 0.5) * h,) * 3,
+                  max_particles=int(len(pts) * 1.25) + (1 << 16), max_bricks=1 << 17)
+    return {"keys": bk, "flags": flags, "u": speed[0], "v": speed[1], "w": speed[2], "particles": pts, "params": params,
+            "fluid_cells": int(inside.sum())}

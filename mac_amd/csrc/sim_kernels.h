@@ -162,6 +162,22 @@ __global__ void k_particle_bricks(int np, const double* __restrict__ px, const d
 }
 
 // Flag bricks holding an existing inflow cell.
+// For every flagged brick, its missing 26-neighbourhood bricks (keys),
+// appended to out (duplicates possible); *n counts them, out holds at most
+// maxn.
+__global__ void k_missing_neighbours(int nb, const int* __restrict__ bflag, const int* __restrict__ nbr27,
+                                     const int* __restrict__ keys3, int* __restrict__ out, int* __restrict__ n, int maxn) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nb * 27) return;
+  const int b = i / 27, k = i % 27;
+  if (k == 13 || !bflag[b] || nbr27[i] >= 0) return;
+  const int slot = atomicAdd(n, 1);
+  if (slot >= maxn) return;
+  out[3 * slot] = keys3[3 * b] + k % 3 - 1;
+  out[3 * slot + 1] = keys3[3 * b + 1] + (k / 3) % 3 - 1;
+  out[3 * slot + 2] = keys3[3 * b + 2] + k / 9 - 1;
+}
+
 // Bricks holding a cell that the fill step can give layer 0: Inflow cells,
 // and Interior cells when `interior` is set.
 __global__ void k_inflow_bricks(int nb, const uint8_t* __restrict__ flags, int* __restrict__ bflag, int interior) {

@@ -82,7 +82,7 @@ struct mac_sim {
   int* d_dest = nullptr;  // per particle: its place after an erasing pass (-1: erased)
   int np = 0, capp = 0;
   // scratch
-  int* d_int = nullptr;  // [0] particle count, [1] missing count, [2] overflow, [3] kept
+  int* d_int = nullptr;  // [0] particle count, [1] missing count, [2] overflow, [3] kept / dropped, [4..5] compaction
   int* d_any = nullptr;  // per-round change flags of the depth search
   int* d_missing = nullptr;
   int* d_missing2 = nullptr;  // absent neighbour bricks of flagged bricks (keys, duplicates possible)
@@ -274,6 +274,26 @@ int sim_alloc_particles(mac_sim* s, int cap) {
                      &s->P2[2]})
     SCHK(hipMalloc(p, sizeof(double) * s->capp));
   SCHK(hipMalloc(&s->d_dest, sizeof(int) * s->capp));
+  return 0;
+}
+
+// Compacts slot 0 after a pass that marked its dropped particles with a NaN x
+// and counted them in d_int[3]: the kept ones fill the dropped ones' places
+// below the new count.
+int sim_compact_dropped(mac_sim* s) {
+  int dropped = 0;
+  SCHK(hipMemcpyAsync(&dropped, s->d_int + 3, sizeof(int), hipMemcpyDeviceToHost, s->st));
+  SCHK(hipStreamSynchronize(s->st));
+  if (dropped < 0 || dropped > s->np) return mac_internal_set_error(MAC_E_HIP, "dropped-particle count out of range");
+  if (dropped == 0) return 0;
+  const int n_new = s->np - dropped;
+  SCHK(hipMemsetAsync(s->d_int + 4, 0, 2 * sizeof(int), s->st));
+  if (n_new > 0) {
+    k_list_holes<<<(n_new + 255) / 256, 256, 0, s->st>>>(n_new, s->px, s->d_dest, s->d_int + 4);
+    k_fill_holes<<<(dropped + 255) / 256, 256, 0, s->st>>>(n_new, s->np, s->px, s->py, s->pz, s->d_dest, s->d_int + 5);
+    SCHK(hipGetLastError());
+  }
+  s->np = n_new;
   return 0;
 }
 
@@ -495,6 +515,7 @@ SimGrid sim_grid(mac_sim* s, int slot) {
   G.hk = s->hk;
   G.hv = s->hv;
   G.cap = s->hcap;
+  G.nbr27 = s->nbr;
   for (int i = 0; i < 3; ++i) {
     G.U[i] = s->S[slot][i];
     G.ih[i] = 1.0 / s->prm.h[i];
@@ -761,14 +782,10 @@ int mac_sim_move_particles(mac_sim* s, double dt) {
                                                             sim_grid(s, 0), dt, P.particle_rk == 4 ? 4 : 2, box ? 1 : 0, P.box_lo[0],
                                                             P.box_lo[1], P.box_lo[2], P.box_hi[0], P.box_hi[1], P.box_hi[2]);
   SCHK(hipGetLastError());
-  int kept = 0;
-  SCHK(hipMemcpyAsync(&kept, s->d_int + 3, sizeof(int), hipMemcpyDeviceToHost, s->st));
-  SCHK(hipStreamSynchronize(s->st));
   std::swap(s->px, s->qx);
   std::swap(s->py, s->qy);
   std::swap(s->pz, s->qz);
-  s->np = kept;
-  return 0;
+  return sim_compact_dropped(s);
 }
 
 int mac_sim_interpolate(mac_sim* s, int n, const double* pos, double* vel) {
@@ -876,17 +893,10 @@ int sim_box_delete(mac_sim* s) {
   const mac_sim_params& P = s->prm;
   if (s->np == 0 || (P.box_lo[0] == P.box_hi[0] && P.box_lo[1] == P.box_hi[1] && P.box_lo[2] == P.box_hi[2])) return 0;
   SCHK(hipMemsetAsync(s->d_int + 3, 0, sizeof(int), s->st));
-  k_box_delete<<<(s->np + 255) / 256, 256, 0, s->st>>>(s->np, s->px, s->py, s->pz, s->qx, s->qy, s->qz, s->d_int + 3, P.box_lo[0],
-                                                        P.box_lo[1], P.box_lo[2], P.box_hi[0], P.box_hi[1], P.box_hi[2]);
+  k_box_delete<<<(s->np + 255) / 256, 256, 0, s->st>>>(s->np, s->px, s->py, s->pz, s->d_int + 3, P.box_lo[0], P.box_lo[1],
+                                                        P.box_lo[2], P.box_hi[0], P.box_hi[1], P.box_hi[2]);
   SCHK(hipGetLastError());
-  int kept = 0;
-  SCHK(hipMemcpyAsync(&kept, s->d_int + 3, sizeof(int), hipMemcpyDeviceToHost, s->st));
-  SCHK(hipStreamSynchronize(s->st));
-  std::swap(s->px, s->qx);
-  std::swap(s->py, s->qy);
-  std::swap(s->pz, s->qz);
-  s->np = kept;
-  return 0;
+  return sim_compact_dropped(s);
 }
 }  // namespace
 

@@ -64,6 +64,38 @@ __device__ inline int wave_append(bool keep, int* counter) {
   return keep ? base + (int)__popcll(m & ((1ull << lane) - 1ull)) : -1;
 }
 
+// Adds the number of the wave's lanes with `flag` to *counter: one atomic in
+// each wave that has any, none elsewhere.
+__device__ inline void wave_count(bool flag, int* counter) {
+  const unsigned long long m = __ballot(flag);
+  if (m != 0ull && (int)__lane_id() == __ffsll((unsigned long long)m) - 1) atomicAdd(counter, (int)__popcll(m));
+}
+
+// In-place compaction of particle positions whose dropped entries hold a NaN
+// x (a kept particle lies inside the box, so its x is never NaN).  With d
+// dropped among np, the kept ones end in [0, np - d): k_list_holes lists the
+// dropped indices below np - d, k_fill_holes moves the kept ones at or above
+// it into them; the two counts are equal.  Particle order is not kept.
+__global__ void k_list_holes(int n_new, const double* __restrict__ x, int* __restrict__ holes, int* __restrict__ nholes) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n_new) return;
+  const int o = wave_append(__builtin_isnan(x[i]), nholes);
+  if (o >= 0) holes[o] = i;
+}
+
+__global__ void k_fill_holes(int n_new, int np, double* __restrict__ x, double* __restrict__ y, double* __restrict__ z,
+                             const int* __restrict__ holes, int* __restrict__ nfilled) {
+  const int i = n_new + blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= np) return;
+  const double xv = x[i];
+  const int o = wave_append(!__builtin_isnan(xv), nfilled);
+  if (o < 0) return;
+  const int j = holes[o];
+  x[j] = xv;
+  y[j] = y[i];
+  z[j] = z[i];
+}
+
 __device__ inline int dh_lookup(const uint64_t* __restrict__ hk, const int* __restrict__ hv, int cap, uint64_t k) {
   const uint32_t m = (uint32_t)cap - 1u;
   uint32_t i = (uint32_t)sim_mix(k) & m;
@@ -741,27 +773,62 @@ struct SimGrid {
   const uint64_t* hk;
   const int* hv;
   int cap;
+  const int* nbr27;  // 27-neighbour brick table (-1: absent)
   const double* U[3];
   double ih[3];
 };
 
-__device__ inline double speed_at(const SimGrid& G, int k, int kx, int ky, int kz) {
-  long long c = cell_of_key(G.hk, G.hv, G.cap, kx, ky, kz);
-  return c >= 0 ? G.U[k][c] : 0.0;
+// The brick holding a particle's nearest cell, kept between the
+// interpolations of one particle: the next position's brick is then a
+// neighbour-table read instead of a hash lookup.
+struct BrickHint {
+  int b = -1;  // brick index, -1 absent
+  int c[3] = {0, 0, 0};
+  bool valid = false;
+};
+
+__device__ inline int hinted_brick(const SimGrid& G, BrickHint& h, const int cb[3]) {
+  const int dx = cb[0] - h.c[0], dy = cb[1] - h.c[1], dz = cb[2] - h.c[2];
+  int b;
+  if (h.valid && h.b >= 0 && dx >= -1 && dx <= 1 && dy >= -1 && dy <= 1 && dz >= -1 && dz <= 1)
+    b = (dx | dy | dz) == 0 ? h.b : G.nbr27[h.b * 27 + (dx + 1) + 3 * (dy + 1) + 9 * (dz + 1)];
+  else
+    b = dh_lookup(G.hk, G.hv, G.cap, sim_pack(cb[0], cb[1], cb[2]));
+  h.b = b;
+  h.c[0] = cb[0], h.c[1] = cb[1], h.c[2] = cb[2];
+  h.valid = true;
+  return b;
+}
+
+// Face speed of component k at cell (kx, ky, kz), 0 where no brick holds the
+// cell.  The cell lies within one cell of brick cb, whose index is b0: its
+// brick comes from b0's neighbour table, or from the hash when b0 is absent.
+__device__ inline double speed_near(const SimGrid& G, int k, int b0, const int cb[3], int kx, int ky, int kz) {
+  int b;
+  if (b0 >= 0) {
+    const int sl = (sim_chunk(kx) - cb[0] + 1) + 3 * (sim_chunk(ky) - cb[1] + 1) + 9 * (sim_chunk(kz) - cb[2] + 1);
+    b = sl == 13 ? b0 : G.nbr27[b0 * 27 + sl];
+  } else {
+    b = dh_lookup(G.hk, G.hv, G.cap, sim_pack(sim_chunk(kx), sim_chunk(ky), sim_chunk(kz)));
+  }
+  return b >= 0 ? G.U[k][(size_t)b * MAC_BC + (kx & 7) + 8 * (ky & 7) + 64 * (kz & 7)] : 0.0;
 }
 
 // Policy_Speed_Interpolation_Linear_Symmetric::Get_Speed (:46-119) with the
 // recursive Get_Speed_Impl (:13-44) unrolled in the same nesting (x outer,
 // z inner) and operation order.
-__device__ inline void interp_speed(const SimGrid& G, const double pos[3], double out[3]) {
+__device__ inline void interp_speed(const SimGrid& G, const double pos[3], double out[3], BrickHint& hint) {
 #pragma clang fp contract(off)
   double d[3];
-  int key0[3];
+  int key0[3], cb[3];
   for (int i = 0; i < 3; ++i) {
     d[i] = pos[i] * G.ih[i];
     key0[i] = (int)lround(d[i]);
     d[i] -= key0[i];
+    cb[i] = sim_chunk(key0[i]);
   }
+  // every sample lies within one cell of key0, so inside cb's 27-neighbourhood
+  const int b0 = hinted_brick(G, hint, cb);
   for (int k = 0; k < 3; ++k) {
     int key[3];
     double d2[3];
@@ -788,14 +855,19 @@ __device__ inline void interp_speed(const SimGrid& G, const double pos[3], doubl
     for (int a = 0; a < 2; ++a) {
       double vy[2];
       for (int bb = 0; bb < 2; ++bb) {
-        double s0 = speed_at(G, k, key[0] + a, key[1] + bb, key[2]);
-        double s1 = speed_at(G, k, key[0] + a, key[1] + bb, key[2] + 1);
+        double s0 = speed_near(G, k, b0, cb, key[0] + a, key[1] + bb, key[2]);
+        double s1 = speed_near(G, k, b0, cb, key[0] + a, key[1] + bb, key[2] + 1);
         vy[bb] = w0[2] * s0 + w1[2] * s1;
       }
       vx[a] = w0[1] * vy[0] + w1[1] * vy[1];
     }
     out[k] = w0[0] * vx[0] + w1[0] * vx[1];
   }
+}
+
+__device__ inline void interp_speed(const SimGrid& G, const double pos[3], double out[3]) {
+  BrickHint h;
+  interp_speed(G, pos, out, h);
 }
 
 __global__ void k_interpolate(int n, const double* __restrict__ pos, double* __restrict__ vel, SimGrid G) {
@@ -810,11 +882,12 @@ __global__ void k_interpolate(int n, const double* __restrict__ pos, double* __r
 
 // Algorithms_Move_Particles::Do (:22-36): rk == 4 is
 // Policy_Advance_Ode_RungeKutta::AdvanceSolveODE (:15-27), rk == 2 the
-// midpoint rule x += dt v(x + dt v(x) / 2).  Particles leaving [lo, hi) are
-// dropped; the kept ones are appended (order is not significant).
+// midpoint rule x += dt v(x + dt v(x) / 2).  Each particle's new position is
+// written at its own index; one leaving [lo, hi) gets a NaN x there and is
+// counted in *ndropped, for k_list_holes / k_fill_holes to compact.
 __global__ void k_move_particles(int np, const double* __restrict__ px, const double* __restrict__ py,
                                  const double* __restrict__ pz, double* __restrict__ qx, double* __restrict__ qy,
-                                 double* __restrict__ qz, int* __restrict__ nkept, SimGrid G, double dt, int rk, int use_box,
+                                 double* __restrict__ qz, int* __restrict__ ndropped, SimGrid G, double dt, int rk, int use_box,
                                  double lo0, double lo1, double lo2, double hi0, double hi1, double hi2) {
 #pragma clang fp contract(off)
   int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -822,46 +895,46 @@ __global__ void k_move_particles(int np, const double* __restrict__ px, const do
   double pos[3] = {px[i], py[i], pz[i]};
   double res[3] = {pos[0], pos[1], pos[2]};
   double v[3], k[3], q[3];
+  BrickHint hint;
   if (rk == 4) {
-    interp_speed(G, pos, v);
+    interp_speed(G, pos, v, hint);
     for (int j = 0; j < 3; ++j) {
       k[j] = dt * v[j];
       res[j] += (1. / 6.) * k[j];
       q[j] = pos[j] + k[j] * 0.5;
     }
-    interp_speed(G, q, v);
+    interp_speed(G, q, v, hint);
     for (int j = 0; j < 3; ++j) {
       k[j] = dt * v[j];
       res[j] += (1. / 3.) * k[j];
       q[j] = pos[j] + k[j] * 0.5;
     }
-    interp_speed(G, q, v);
+    interp_speed(G, q, v, hint);
     for (int j = 0; j < 3; ++j) {
       k[j] = dt * v[j];
       res[j] += (1. / 3.) * k[j];
       q[j] = pos[j] + k[j];
     }
-    interp_speed(G, q, v);
+    interp_speed(G, q, v, hint);
     for (int j = 0; j < 3; ++j) {
       k[j] = dt * v[j];
       res[j] += (1. / 6.) * k[j];
     }
   } else {
-    interp_speed(G, pos, v);
+    interp_speed(G, pos, v, hint);
     for (int j = 0; j < 3; ++j) {
       k[j] = dt * v[j];
       q[j] = pos[j] + k[j] * 0.5;
     }
-    interp_speed(G, q, v);
+    interp_speed(G, q, v, hint);
     for (int j = 0; j < 3; ++j) res[j] = pos[j] + dt * v[j];
   }
   const bool keep =
       !use_box || (res[0] >= lo0 && res[0] < hi0 && res[1] >= lo1 && res[1] < hi1 && res[2] >= lo2 && res[2] < hi2);
-  const int o = wave_append(keep, nkept);
-  if (o < 0) return;
-  qx[o] = res[0];
-  qy[o] = res[1];
-  qz[o] = res[2];
+  wave_count(!keep, ndropped);
+  qx[i] = keep ? res[0] : __builtin_nan("");
+  qy[i] = res[1];
+  qz[i] = res[2];
 }
 
 // Particle part of the same stages (Policy_Move_Particle: the velocity is the
@@ -882,41 +955,40 @@ __global__ void k_rk4_particles(int np, int stage, double dt, double* __restrict
   double* P1[3] = {p1x, p1y, p1z};
   double* P2[3] = {p2x, p2y, p2z};
   double q[3], v[3];
+  BrickHint hint;
   if (stage == 1) {
     for (int k = 0; k < 3; ++k) q[k] = P0[k][i];
-    interp_speed(G, q, v);
+    interp_speed(G, q, v, hint);
     for (int k = 0; k < 3; ++k) {
       P2[k][i] = q[k] + dt * 0.5 * v[k];
       P1[k][i] = q[k] + dt * 1.0 / 6.0 * v[k];
     }
   } else if (stage == 4) {
     for (int k = 0; k < 3; ++k) q[k] = P2[k][i];
-    interp_speed(G, q, v);
+    interp_speed(G, q, v, hint);
     for (int k = 0; k < 3; ++k) P0[k][i] = P1[k][i] + dt * 1.0 / 6.0 * v[k];
   } else {
     for (int k = 0; k < 3; ++k) q[k] = P2[k][i];
-    interp_speed(G, q, v);
+    interp_speed(G, q, v, hint);
     for (int k = 0; k < 3; ++k) q[k] = P0[k][i] + (stage == 2 ? dt * 0.5 : dt) * v[k];
     for (int k = 0; k < 3; ++k) P2[k][i] = q[k];
-    interp_speed(G, q, v);
+    interp_speed(G, q, v, hint);
     for (int k = 0; k < 3; ++k) P1[k][i] = P1[k][i] + dt * 1.0 / 3.0 * v[k];
   }
 }
 
 // Particles outside [lo, hi) are deleted (slot 0 only; the other slots are
-// rewritten from slot 0 at the next step's first stage).
-__global__ void k_box_delete(int np, const double* __restrict__ px, const double* __restrict__ py, const double* __restrict__ pz,
-                             double* __restrict__ qx, double* __restrict__ qy, double* __restrict__ qz, int* __restrict__ nkept,
-                             double lo0, double lo1, double lo2, double hi0, double hi1, double hi2) {
+// rewritten from slot 0 at the next step's first stage): their x becomes NaN
+// in place and they are counted in *ndropped, for k_list_holes /
+// k_fill_holes to compact.
+__global__ void k_box_delete(int np, double* __restrict__ px, const double* __restrict__ py, const double* __restrict__ pz,
+                             int* __restrict__ ndropped, double lo0, double lo1, double lo2, double hi0, double hi1, double hi2) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= np) return;
   const double x = px[i], y = py[i], z = pz[i];
   const bool keep = x >= lo0 && x < hi0 && y >= lo1 && y < hi1 && z >= lo2 && z < hi2;
-  const int o = wave_append(keep, nkept);
-  if (o < 0) return;
-  qx[o] = x;
-  qy[o] = y;
-  qz[o] = z;
+  wave_count(!keep, ndropped);
+  if (!keep) px[i] = __builtin_nan("");
 }
 
 

@@ -316,3 +316,35 @@ def test_rk4all_depth_jet_vs_oracle():
         same_particles(dg["particles"], do["particles"], 1e-9)
         assert sg["max_div"] < 1e-7
     assert n_interior > 0
+
+
+@pytest.mark.parametrize("integrator,prk", [(0, 2), (2, 4)])
+def test_box_deletion_vs_oracle(integrator, prk):
+    """Particles leaving the box are deleted: a 16x8x8-cell block moving along
+    +x through the box's upper x bound, 8 shuffled particles per cell (so the
+    deleted ones are spread over the whole particle array), two steps on both
+    sides: the kept particles equal the oracle's, none is outside the box and
+    none is NaN."""
+    h, dt = 0.125, 0.0625
+    cells = np.argwhere(np.ones((16, 8, 8), bool)) + 4
+    rng = np.random.default_rng(11)
+    pts = np.repeat(cells, 8, axis=0) * h + rng.uniform(-0.49 * h, 0.49 * h, size=(8 * len(cells), 3))
+    pts = pts[rng.permutation(len(pts))]
+    keys = np.array(sorted({tuple(k) for k in (cells // 8).tolist()}, key=lambda k: (k[2], k[1], k[0])), np.int32)
+    n = len(keys) * 512
+    hi = (12.3 * h, 100.0, 100.0)
+    kw = dict(h=(h, h, h), gravity=(0, 0, 0), integrator=integrator, particle_rk=prk, particles_per_inflow_cell=0,
+              layer_max=2, cfl=1e9, dt_min=dt, dt_max=dt, box_lo=(-100.0, -100.0, -100.0), box_hi=hi)
+    pair = both(**kw)
+    for s in pair:
+        s.upload(keys, np.zeros(n, np.uint8), u=np.full(n, 1.0), v=np.zeros(n), w=np.zeros(n), particles=pts)
+    counts = []
+    for step in range(2):
+        for s in pair:
+            s.step()
+        pg, po = pair[0].download()["particles"], pair[1].download()["particles"]
+        assert not np.isnan(pg).any()
+        assert (pg[:, 0] < hi[0]).all()
+        same_particles(pg, po, 1e-12)
+        counts.append(len(pg))
+    assert len(pts) > counts[0] > counts[1] > 0

@@ -251,18 +251,60 @@ void flush_timing(mac_ctx* c) {
 
 // ---- hierarchy construction ----
 
-void build_nbr(Level& lv) {
+// Brick index by key (keys >= 0): a dense array over the keys' bounding box
+// when that box is at most 64 times the brick count, else a hash map.
+struct KeyIndex {
+  int ext[3] = {0, 0, 0};
+  bool dense_ok = false;
+  std::vector<int> dense;
   std::unordered_map<uint64_t, int> map;
-  map.reserve(lv.nb * 2);
-  for (int b = 0; b < lv.nb; ++b) map[pack_key(lv.keys[3 * b], lv.keys[3 * b + 1], lv.keys[3 * b + 2])] = b;
+  void init(const int* keys, int nb) {
+    ext[0] = ext[1] = ext[2] = 1;
+    for (int b = 0; b < nb; ++b)
+      for (int i = 0; i < 3; ++i) ext[i] = std::max(ext[i], keys[3 * b + i] + 1);
+    const double n = (double)ext[0] * ext[1] * ext[2];
+    dense_ok = n <= 64.0 * std::max(nb, 1024);
+    if (dense_ok) {
+      dense.assign((size_t)n, -1);
+    } else {
+      dense.clear();
+      map.clear();
+      map.reserve(2 * (size_t)nb);
+    }
+  }
+  // stores id at the key unless one is there; returns the id found there, or -1
+  int insert(int x, int y, int z, int id) {
+    if (dense_ok) {
+      int& v = dense[(size_t)x + (size_t)ext[0] * ((size_t)y + (size_t)ext[1] * z)];
+      const int was = v;
+      if (was < 0) v = id;
+      return was;
+    }
+    auto r = map.emplace(pack_key(x, y, z), id);
+    return r.second ? -1 : r.first->second;
+  }
+  int find(int x, int y, int z) const {
+    if (dense_ok) {
+      if (x < 0 || y < 0 || z < 0 || x >= ext[0] || y >= ext[1] || z >= ext[2]) return -1;
+      return dense[(size_t)x + (size_t)ext[0] * ((size_t)y + (size_t)ext[1] * z)];
+    }
+    auto it = map.find(pack_key(x, y, z));
+    return it == map.end() ? -1 : it->second;
+  }
+};
+
+void build_nbr(Level& lv) {
+  KeyIndex idx;
+  idx.init(lv.keys.data(), lv.nb);
+  for (int b = 0; b < lv.nb; ++b) idx.insert(lv.keys[3 * b], lv.keys[3 * b + 1], lv.keys[3 * b + 2], b);
   lv.h_nbr.assign((size_t)lv.nb * 27, -1);
-  for (int b = 0; b < lv.nb; ++b)
+  for (int b = 0; b < lv.nb; ++b) {
+    const int kx = lv.keys[3 * b], ky = lv.keys[3 * b + 1], kz = lv.keys[3 * b + 2];
+    int* nb = &lv.h_nbr[(size_t)b * 27];
     for (int oz = -1; oz <= 1; ++oz)
       for (int oy = -1; oy <= 1; ++oy)
-        for (int ox = -1; ox <= 1; ++ox) {
-          auto it = map.find(pack_key(lv.keys[3 * b] + ox, lv.keys[3 * b + 1] + oy, lv.keys[3 * b + 2] + oz));
-          lv.h_nbr[(size_t)b * 27 + (ox + 1) + 3 * (oy + 1) + 9 * (oz + 1)] = (it == map.end()) ? -1 : it->second;
-        }
+        for (int ox = -1; ox <= 1; ++ox) nb[(ox + 1) + 3 * (oy + 1) + 9 * (oz + 1)] = idx.find(kx + ox, ky + oy, kz + oz);
+  }
 }
 
 // Dense inverse of the coarsest operator (Cholesky; a tiny diagonal shift if
@@ -346,20 +388,13 @@ int build_grid(mac_ctx* c, int nbricks, const int32_t* keys3, const uint8_t* fla
   free_all(c, true);
   c->nb0 = nbricks;
   c->keys0.assign(keys3, keys3 + 3 * (size_t)nbricks);
-  // duplicate check + shift
+  // range check + shift
   int mn[3] = {1 << 30, 1 << 30, 1 << 30};
-  {
-    std::unordered_map<uint64_t, int> seen;
-    seen.reserve(nbricks * 2);
-    for (int b = 0; b < nbricks; ++b) {
-      for (int i = 0; i < 3; ++i) {
-        if (keys3[3 * b + i] < -(1 << 19) || keys3[3 * b + i] >= (1 << 19)) return fail(MAC_E_ARG, "chunk key out of range");
-        mn[i] = std::min(mn[i], (int)keys3[3 * b + i]);
-      }
-      if (!seen.emplace(pack_key(keys3[3 * b], keys3[3 * b + 1], keys3[3 * b + 2]), b).second)
-        return fail(MAC_E_ARG, "duplicate chunk key");
+  for (int b = 0; b < nbricks; ++b)
+    for (int i = 0; i < 3; ++i) {
+      if (keys3[3 * b + i] < -(1 << 19) || keys3[3 * b + i] >= (1 << 19)) return fail(MAC_E_ARG, "chunk key out of range");
+      mn[i] = std::min(mn[i], (int)keys3[3 * b + i]);
     }
-  }
   const mac_params& P = c->prm;
   double inv_h[3];
   for (int i = 0; i < 3; ++i) inv_h[i] = 1.0 / P.h[i];
@@ -373,29 +408,34 @@ int build_grid(mac_ctx* c, int nbricks, const int32_t* keys3, const uint8_t* fla
     for (int b = 0; b < nbricks; ++b)
       for (int i = 0; i < 3; ++i) l0.keys[3 * b + i] = keys3[3 * b + i] - mn[i];
     for (int i = 0; i < 3; ++i) l0.ih2[i] = inv_h[i] * inv_h[i];
+    KeyIndex seen;
+    seen.init(l0.keys.data(), nbricks);
+    for (int b = 0; b < nbricks; ++b)
+      if (seen.insert(l0.keys[3 * b], l0.keys[3 * b + 1], l0.keys[3 * b + 2], b) >= 0)
+        return fail(MAC_E_ARG, "duplicate chunk key");
   }
   // coarse levels until a single brick remains
   while (c->L.back().nb > 1) {
     Level& f = c->L.back();
     Level cl;
-    std::unordered_map<uint64_t, int> map;
-    map.reserve(f.nb);
+    KeyIndex map;
+    {
+      std::vector<int> ck(f.keys.size());
+      for (size_t i = 0; i < ck.size(); ++i) ck[i] = f.keys[i] >> 1;
+      map.init(ck.data(), f.nb);
+    }
     f.h_parent.assign(f.nb, -1);
     f.h_oct.assign(f.nb, 0);
+    cl.h_child.reserve((size_t)f.nb * 8);
     for (int b = 0; b < f.nb; ++b) {
       int kx = f.keys[3 * b] >> 1, ky = f.keys[3 * b + 1] >> 1, kz = f.keys[3 * b + 2] >> 1;
-      uint64_t k = pack_key(kx, ky, kz);
-      auto it = map.find(k);
-      int id;
-      if (it == map.end()) {
+      int id = map.insert(kx, ky, kz, cl.nb);
+      if (id < 0) {
         id = cl.nb++;
-        map.emplace(k, id);
         cl.keys.push_back(kx);
         cl.keys.push_back(ky);
         cl.keys.push_back(kz);
         cl.h_child.resize((size_t)cl.nb * 8, -1);
-      } else {
-        id = it->second;
       }
       int o = (f.keys[3 * b] & 1) + 2 * (f.keys[3 * b + 1] & 1) + 4 * (f.keys[3 * b + 2] & 1);
       cl.h_child[(size_t)id * 8 + o] = b;
@@ -431,15 +471,20 @@ int build_grid(mac_ctx* c, int nbricks, const int32_t* keys3, const uint8_t* fla
     k_code_c<<<lv.nb, KTHREADS, 0, c->st>>>(lv.nbr, lv.lab, lv.code);
   }
   HIPCHK(hipGetLastError());
-  // active bricks per level
+  // active bricks per level: every level counted, then one read back
+  std::vector<size_t> coff(c->L.size() + 1, 0);
+  for (size_t l = 0; l < c->L.size(); ++l) coff[l + 1] = coff[l] + (size_t)c->L[l].nb;
   int* dcnt;
-  CHK(dalloc(c, &dcnt, (size_t)nbricks));
+  CHK(dalloc(c, &dcnt, coff.back()));
+  for (size_t l = 0; l < c->L.size(); ++l)
+    k_brick_count<<<c->L[l].nb, KTHREADS, 0, c->st>>>(c->L[l].nb, c->L[l].code, dcnt + coff[l]);
+  HIPCHK(hipGetLastError());
+  std::vector<int> allcnt(coff.back());
+  HIPCHK(hipMemcpyAsync(allcnt.data(), dcnt, coff.back() * sizeof(int), hipMemcpyDeviceToHost, c->st));
+  HIPCHK(hipStreamSynchronize(c->st));
   for (size_t l = 0; l < c->L.size(); ++l) {
     Level& lv = c->L[l];
-    k_brick_count<<<lv.nb, KTHREADS, 0, c->st>>>(lv.nb, lv.code, dcnt);
-    std::vector<int> cnt(lv.nb);
-    HIPCHK(hipMemcpyAsync(cnt.data(), dcnt, lv.nb * sizeof(int), hipMemcpyDeviceToHost, c->st));
-    HIPCHK(hipStreamSynchronize(c->st));
+    const int* cnt = allcnt.data() + coff[l];
     std::vector<int> act;
     lv.ndom = 0;
     for (int b = 0; b < lv.nb; ++b)
@@ -549,8 +594,20 @@ int build_grid(mac_ctx* c, int nbricks, const int32_t* keys3, const uint8_t* fla
   CHK(dalloc(c, &c->d[1], n0));
   CHK(dalloc(c, &c->q, n0));
   CHK(dalloc(c, &c->tmpv, n0));
-  for (double* p : {c->u, c->v, c->w, c->x, c->f, c->r, c->d[0], c->d[1], c->q, c->tmpv})
-    HIPCHK(hipMemsetAsync(p, 0, n0 * sizeof(double), c->st));
+  // every vector starts at zero (zero outside the domain is assumed by the
+  // kernels; reused blocks hold old data): one launch clears them all
+  ZeroList zl;
+  zl.k = 0;
+  auto clear = [&](double* p, size_t n) -> int {
+    if (zl.k == ZeroList::CAP) {
+      k_zero_list<<<dim3(256, zl.k), KTHREADS, 0, c->st>>>(zl);
+      zl.k = 0;
+    }
+    zl.p[zl.k] = p;
+    zl.n[zl.k++] = n;
+    return 0;
+  };
+  for (double* p : {c->u, c->v, c->w, c->x, c->f, c->r, c->d[0], c->d[1], c->q, c->tmpv}) clear(p, n0);
   for (size_t l = 0; l < c->L.size(); ++l) {
     Level& lv = c->L[l];
     size_t n = (size_t)lv.nb * MAC_BC;
@@ -559,12 +616,14 @@ int build_grid(mac_ctx* c, int nbricks, const int32_t* keys3, const uint8_t* fla
     CHK(dalloc(c, &b, n));
     CHK(dalloc(c, &cc, n));
     CHK(dalloc(c, &dd, n));
-    for (double* p : {a, b, cc, dd}) HIPCHK(hipMemsetAsync(p, 0, n * sizeof(double), c->st));
+    for (double* p : {a, b, cc, dd}) clear(p, n);
     lv.r = a;
     lv.z = b;
     lv.zt = cc;
     lv.res = dd;
   }
+  if (zl.k > 0) k_zero_list<<<dim3(256, zl.k), KTHREADS, 0, c->st>>>(zl);
+  HIPCHK(hipGetLastError());
   CHK(dalloc(c, &c->partials, std::max(nbricks, 1024)));
   CHK(dalloc(c, &c->s, 1));
   HIPCHK(hipMemsetAsync(c->s, 0, sizeof(Scal), c->st));

@@ -297,6 +297,19 @@ __global__ void k_code_c(const int* __restrict__ nbr, const uint8_t* __restrict_
   }
 }
 
+// Clears up to CAP double arrays in one launch; blockIdx.y picks the array.
+struct ZeroList {
+  static constexpr int CAP = 48;
+  double* p[CAP];
+  size_t n[CAP];
+  int k;
+};
+__global__ void k_zero_list(ZeroList zl) {
+  double* p = zl.p[blockIdx.y];
+  const size_t n = zl.n[blockIdx.y];
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = 0.0;
+}
+
 __global__ void k_brick_count(int nb, const uint8_t* __restrict__ code, int* __restrict__ cnt) {
   int b = blockIdx.x;
   __shared__ int s;

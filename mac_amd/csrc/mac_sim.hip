@@ -85,8 +85,9 @@ struct mac_sim {
   int* d_int = nullptr;  // [0] particle count, [1] missing count, [2] overflow, [3] kept / dropped, [4..5] compaction
   int* d_any = nullptr;  // per-round change flags of the depth search
   int* d_missing = nullptr;
-  int* d_missing2 = nullptr;  // absent neighbour bricks of flagged bricks (keys, duplicates possible)
+  int* d_missing2 = nullptr;  // absent neighbour bricks of flagged bricks (keys, each once)
   int maxmissing2 = 0;
+  uint64_t* d_cset = nullptr;  // their de-duplication set (2 * maxmissing2 slots)
   int maxmissing = 0;
   int* d_bflag = nullptr;
   int* d_from = nullptr;
@@ -345,45 +346,43 @@ int sim_topology(mac_sim* s, bool full = true, int ks = 0) {
     k_particle_bricks<<<(s->np + 255) / 256, 256, 0, s->st>>>(s->np, *kp[0], *kp[1], *kp[2], ih, s->hk, s->hv, s->hcap, s->d_bflag,
                                                              s->d_missing, s->d_int + 1, s->maxmissing, floor_axis,
                                                              P.floor_value);
-  // the flagged bricks' absent neighbours, found on the device
-  if (s->nb > 0)
-    k_missing_neighbours<<<(s->nb * 27 + 255) / 256, 256, 0, s->st>>>(s->nb, s->d_bflag, s->nbr, s->d_keys, s->d_missing2,
-                                                                      s->d_int + 3, s->maxmissing2);
+  // the flagged bricks' absent neighbours, found and de-duplicated on the device
+  if (s->nb > 0) {
+    k_set_clear<<<256, 256, 0, s->st>>>(2 * s->maxmissing2, s->d_cset);
+    k_missing_neighbours<<<(s->nb * 27 + 255) / 256, 256, 0, s->st>>>(s->nb, s->d_bflag, s->nbr, s->d_keys, s->d_cset,
+                                                                      2 * s->maxmissing2, s->d_missing2, s->d_int + 3,
+                                                                      s->maxmissing2);
+  }
   SCHK(hipGetLastError());
   int cnt[4];
   SCHK(hipMemcpyAsync(cnt, s->d_int, sizeof(cnt), hipMemcpyDeviceToHost, s->st));
   SCHK(hipStreamSynchronize(s->st));
   if (cnt[1] > s->maxmissing) return mac_internal_set_error(MAC_E_ARG, "too many particles outside the grid in one step");
-  const int nmiss = cnt[1];
-  // beyond the list's capacity the flagged bricks are expanded on the host
-  const bool overflow = cnt[3] > s->maxmissing2;
-  const int nnb = overflow ? 0 : cnt[3];
-  std::vector<int> miss(3 * std::max(nmiss, 1)), nbm(3 * std::max(nnb, 1)), bflag;
-  if (nmiss > 0) SCHK(hipMemcpyAsync(miss.data(), s->d_missing, sizeof(int) * 3 * nmiss, hipMemcpyDeviceToHost, s->st));
-  if (nnb > 0) SCHK(hipMemcpyAsync(nbm.data(), s->d_missing2, sizeof(int) * 3 * nnb, hipMemcpyDeviceToHost, s->st));
-  if (overflow) {
-    bflag.resize(s->nb);
-    SCHK(hipMemcpyAsync(bflag.data(), s->d_bflag, sizeof(int) * s->nb, hipMemcpyDeviceToHost, s->st));
+  if (cnt[3] > s->maxmissing2) return mac_internal_set_error(MAC_E_ARG, "too many bricks to create in one step");
+  const int nmiss = cnt[1], nnb = cnt[3];
+  std::vector<Key3> add(nnb);
+  if (nnb > 0) {
+    std::vector<int> nbm(3 * nnb);
+    SCHK(hipMemcpyAsync(nbm.data(), s->d_missing2, sizeof(int) * 3 * nnb, hipMemcpyDeviceToHost, s->st));
+    SCHK(hipStreamSynchronize(s->st));
+    for (int i = 0; i < nnb; ++i) add[i] = {nbm[3 * i], nbm[3 * i + 1], nbm[3 * i + 2]};
   }
-  if (nmiss > 0 || nnb > 0 || overflow) SCHK(hipStreamSynchronize(s->st));
-  std::vector<Key3> add;
-  if (nmiss > 0 || nnb > 0 || overflow) {
+  if (nmiss > 0) {
+    // bricks of particles outside the grid, with their whole neighbourhood
+    std::vector<int> miss(3 * nmiss);
+    SCHK(hipMemcpyAsync(miss.data(), s->d_missing, sizeof(int) * 3 * nmiss, hipMemcpyDeviceToHost, s->st));
+    SCHK(hipStreamSynchronize(s->st));
     std::unordered_set<uint64_t> have;
-    have.reserve(2 * (s->keys.size() + nnb + 27 * (size_t)nmiss));
+    have.reserve(2 * (s->keys.size() + add.size() + 27 * (size_t)nmiss));
     for (const Key3& k : s->keys) have.insert(sim_pack(k.x, k.y, k.z));
-    auto hood = [&](int x, int y, int z) {
+    for (const Key3& k : add) have.insert(sim_pack(k.x, k.y, k.z));
+    for (int i = 0; i < nmiss; ++i)
       for (int oz = -1; oz <= 1; ++oz)
         for (int oy = -1; oy <= 1; ++oy)
-          for (int ox = -1; ox <= 1; ++ox)
-            if (have.insert(sim_pack(x + ox, y + oy, z + oz)).second) add.push_back({x + ox, y + oy, z + oz});
-    };
-    for (int i = 0; i < nnb; ++i)
-      if (have.insert(sim_pack(nbm[3 * i], nbm[3 * i + 1], nbm[3 * i + 2])).second)
-        add.push_back({nbm[3 * i], nbm[3 * i + 1], nbm[3 * i + 2]});
-    for (size_t b = 0; b < bflag.size(); ++b)
-      if (bflag[b]) hood(s->keys[b].x, s->keys[b].y, s->keys[b].z);
-    // bricks of particles outside the grid, with their whole neighbourhood
-    for (int i = 0; i < nmiss; ++i) hood(miss[3 * i], miss[3 * i + 1], miss[3 * i + 2]);
+          for (int ox = -1; ox <= 1; ++ox) {
+            const int x = miss[3 * i] + ox, y = miss[3 * i + 1] + oy, z = miss[3 * i + 2] + oz;
+            if (have.insert(sim_pack(x, y, z)).second) add.push_back({x, y, z});
+          }
   }
   if (!add.empty()) {
     std::sort(add.begin(), add.end());
@@ -578,6 +577,7 @@ int mac_sim_create(mac_sim** out, const mac_sim_params* p) {
   if (hipMalloc(&s->d_int, 8 * sizeof(int)) != hipSuccess || hipMalloc(&s->d_any, 64 * sizeof(int)) != hipSuccess ||
       hipMalloc(&s->d_missing, sizeof(int) * 3 * s->maxmissing) != hipSuccess ||
       hipMalloc(&s->d_missing2, sizeof(int) * 3 * s->maxmissing2) != hipSuccess ||
+      hipMalloc(&s->d_cset, sizeof(uint64_t) * 2 * s->maxmissing2) != hipSuccess ||
       sim_alloc_particles(s, p->max_particles) != 0) {
     mac_sim_destroy(s);
     return mac_internal_set_error(MAC_E_HIP, "allocation failed");
@@ -597,6 +597,7 @@ void mac_sim_destroy(mac_sim* s) {
   hipFree(s->d_front);
   hipFree(s->d_missing);
   hipFree(s->d_missing2);
+  hipFree(s->d_cset);
   if (s->proj) mac_destroy(s->proj);
   if (s->st) hipStreamDestroy(s->st);
   delete s;

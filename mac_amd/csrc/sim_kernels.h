@@ -193,21 +193,46 @@ __global__ void k_particle_bricks(int np, const double* __restrict__ px, const d
   }
 }
 
-// Flag bricks holding an existing inflow cell.
-// For every flagged brick, its missing 26-neighbourhood bricks (keys),
-// appended to out (duplicates possible); *n counts them, out holds at most
-// maxn.
+__global__ void k_set_clear(int cap, uint64_t* __restrict__ set) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < cap; i += gridDim.x * blockDim.x) set[i] = SIM_HEMPTY;
+}
+
+// The absent bricks of the flagged bricks' 26-neighbourhoods, each listed
+// once: a candidate key is inserted into the open-addressing set `set` (cap a
+// power of two, above twice maxn) and only its first inserter appends it.
+// *n counts the distinct keys; out holds at most maxn of them.
 __global__ void k_missing_neighbours(int nb, const int* __restrict__ bflag, const int* __restrict__ nbr27,
-                                     const int* __restrict__ keys3, int* __restrict__ out, int* __restrict__ n, int maxn) {
+                                     const int* __restrict__ keys3, uint64_t* __restrict__ set, int cap,
+                                     int* __restrict__ out, int* __restrict__ n, int maxn) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= nb * 27) return;
-  const int b = i / 27, k = i % 27;
-  if (k == 13 || !bflag[b] || nbr27[i] >= 0) return;
-  const int slot = atomicAdd(n, 1);
-  if (slot >= maxn) return;
-  out[3 * slot] = keys3[3 * b] + k % 3 - 1;
-  out[3 * slot + 1] = keys3[3 * b + 1] + (k / 3) % 3 - 1;
-  out[3 * slot + 2] = keys3[3 * b + 2] + k / 9 - 1;
+  bool fresh = false;
+  int kx = 0, ky = 0, kz = 0;
+  if (i < nb * 27) {
+    const int b = i / 27, k = i % 27;
+    if (k != 13 && bflag[b] && nbr27[i] < 0) {
+      kx = keys3[3 * b] + k % 3 - 1;
+      ky = keys3[3 * b + 1] + (k / 3) % 3 - 1;
+      kz = keys3[3 * b + 2] + k / 9 - 1;
+      const uint64_t key = sim_pack(kx, ky, kz);
+      const uint32_t m = (uint32_t)cap - 1u;
+      uint32_t h = (uint32_t)sim_mix(key) & m;
+      for (int t = 0; t < cap; ++t) {
+        const unsigned long long prev =
+            atomicCAS((unsigned long long*)&set[h], (unsigned long long)SIM_HEMPTY, (unsigned long long)key);
+        if (prev == SIM_HEMPTY) {
+          fresh = true;
+          break;
+        }
+        if (prev == key) break;
+        h = (h + 1u) & m;
+      }
+    }
+  }
+  const int slot = wave_append(fresh, n);
+  if (slot < 0 || slot >= maxn) return;
+  out[3 * slot] = kx;
+  out[3 * slot + 1] = ky;
+  out[3 * slot + 2] = kz;
 }
 
 // Bricks holding a cell that the fill step can give layer 0: Inflow cells,

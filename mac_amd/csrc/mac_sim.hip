@@ -79,7 +79,8 @@ struct mac_sim {
   // position slots 1 and 2 of the RK4 "all" integrator (Data_Particle m_pos[1..2])
   double* P1[3] = {nullptr, nullptr, nullptr};
   double* P2[3] = {nullptr, nullptr, nullptr};
-  int* d_dest = nullptr;  // per particle: its place after an erasing pass (-1: erased)
+  int* d_dest = nullptr;   // per particle: kept (1) or erased (0) by an erasing pass; dropped-particle holes
+  int* d_holes = nullptr;  // the erased places being refilled
   int np = 0, capp = 0;
   // scratch
   int* d_int = nullptr;  // [0] particle count, [1] missing count, [2] overflow, [3] kept / dropped, [4..5] compaction
@@ -275,6 +276,7 @@ int sim_alloc_particles(mac_sim* s, int cap) {
                      &s->P2[2]})
     SCHK(hipMalloc(p, sizeof(double) * s->capp));
   SCHK(hipMalloc(&s->d_dest, sizeof(int) * s->capp));
+  SCHK(hipMalloc(&s->d_holes, sizeof(int) * s->capp));
   return 0;
 }
 
@@ -298,25 +300,45 @@ int sim_compact_dropped(mac_sim* s) {
   return 0;
 }
 
-// Depth search of Algorithms_Update_CellType_Layer_Depth.h:96-133, one
-// level per launch.  Levels are launched four at a time before the host
-// reads their change flags: a level that changes nothing ends the search,
-// and the levels launched after it change nothing either.
+// Depth search of Algorithms_Update_CellType_Layer_Depth.h:96-133: relaxation
+// launches, two per read-back of their change flags, until one changes
+// nothing; then the reached cells' types.
 int sim_depth_search(mac_sim* s) {
   const mac_sim_params& P = s->prm;
-  const int batch = 4;
-  for (int d0 = 1;; d0 += batch) {
-    if (d0 + batch > 32767) return mac_internal_set_error(MAC_E_ARG, "fluid deeper than 32766 cells");
-    SCHK(hipMemsetAsync(s->d_any, 0, batch * sizeof(int), s->st));
-    for (int k = 0; k < batch; ++k)
-      k_depth_round<<<s->nb, 256, 0, s->st>>>(s->nb, s->nbr, s->flags, s->layer, s->depth, d0 + k, P.depth_interior,
-                                               s->d_any + k);
+  for (int round = 0;; ++round) {
+    if (round > 100000) return mac_internal_set_error(MAC_E_STATE, "depth search did not settle");
+    SCHK(hipMemsetAsync(s->d_any, 0, 3 * sizeof(int), s->st));
+    for (int k = 0; k < 2; ++k)
+      k_depth_relax<<<s->nb, 256, 0, s->st>>>(s->nb, s->nbr, s->flags, s->layer, s->depth, s->d_any + k, s->d_any + 2);
     SCHK(hipGetLastError());
-    int any[batch];
+    int any[3];
     SCHK(hipMemcpyAsync(any, s->d_any, sizeof(any), hipMemcpyDeviceToHost, s->st));
     SCHK(hipStreamSynchronize(s->st));
-    if (!any[batch - 1]) return 0;
+    if (any[2]) return mac_internal_set_error(MAC_E_ARG, "fluid deeper than 32766 cells");
+    if (!any[1]) break;
   }
+  k_depth_types<<<s->nb, 256, 0, s->st>>>(s->nb, s->flags, s->layer, s->depth, P.depth_interior);
+  SCHK(hipGetLastError());
+  return 0;
+}
+
+// Compacts the position slots in `sl` after k_mark_particles_keep marked the
+// kept particles in d_dest and counted the others in d_int[3].
+int sim_compact_marked(mac_sim* s, const PartSlots& sl) {
+  int dropped = 0;
+  SCHK(hipMemcpyAsync(&dropped, s->d_int + 3, sizeof(int), hipMemcpyDeviceToHost, s->st));
+  SCHK(hipStreamSynchronize(s->st));
+  if (dropped < 0 || dropped > s->np) return mac_internal_set_error(MAC_E_HIP, "erased-particle count out of range");
+  if (dropped == 0) return 0;
+  const int n_new = s->np - dropped;
+  SCHK(hipMemsetAsync(s->d_int + 4, 0, 2 * sizeof(int), s->st));
+  if (n_new > 0) {
+    k_list_holes_marked<<<(n_new + 255) / 256, 256, 0, s->st>>>(n_new, s->d_dest, s->d_holes, s->d_int + 4);
+    k_fill_holes_marked<<<(dropped + 255) / 256, 256, 0, s->st>>>(n_new, s->np, s->d_dest, s->d_holes, s->d_int + 5, sl);
+    SCHK(hipGetLastError());
+  }
+  s->np = n_new;
+  return 0;
 }
 
 // full: the step's cell-type chain (particles created, floor applied);
@@ -396,34 +418,23 @@ int sim_topology(mac_sim* s, bool full = true, int ks = 0) {
   // ones are appended to the second buffer, whose count starts the fill)
   const bool erase = dmode || floor_axis >= 0;
   if (s->np > 0 && erase) {
-    SCHK(hipMemsetAsync(s->d_int, 0, sizeof(int), s->st));
+    SCHK(hipMemsetAsync(s->d_int + 3, 0, sizeof(int), s->st));
     k_mark_particles_keep<<<(s->np + 255) / 256, 256, 0, s->st>>>(s->np, *kp[0], *kp[1], *kp[2], ih, s->hk, s->hv, s->hcap, s->flags,
                                                                  s->layer, dmode ? s->depth : nullptr, P.depth_remove,
-                                                                 floor_axis, P.floor_value, s->qx, s->qy, s->qz, s->d_int,
-                                                                 full ? nullptr : s->d_dest);
-    std::swap(*kp[0], s->qx);
-    std::swap(*kp[1], s->qy);
-    std::swap(*kp[2], s->qz);
-    if (!full) {
-      // the other two position slots follow the same erasures
-      double** other[2][3] = {{&s->px, &s->py, &s->pz}, {&s->P1[0], &s->P1[1], &s->P1[2]}};
-      if (ks == 0) {
-        other[0][0] = &s->P1[0], other[0][1] = &s->P1[1], other[0][2] = &s->P1[2];
-        other[1][0] = &s->P2[0], other[1][1] = &s->P2[1], other[1][2] = &s->P2[2];
-      }
-      for (auto& o : other) {
-        k_scatter_particles<<<(s->np + 255) / 256, 256, 0, s->st>>>(s->np, s->d_dest, *o[0], *o[1], *o[2], s->qx, s->qy, s->qz);
-        std::swap(*o[0], s->qx);
-        std::swap(*o[1], s->qy);
-        std::swap(*o[2], s->qz);
-      }
-    }
-  } else {
-    if (s->np > 0)
-      k_mark_particles<<<(s->np + 255) / 256, 256, 0, s->st>>>(s->np, *kp[0], *kp[1], *kp[2], ih, s->hk, s->hv, s->hcap, s->flags,
-                                                              s->layer);
-    SCHK(hipMemcpyAsync(s->d_int, &s->np, sizeof(int), hipMemcpyHostToDevice, s->st));
+                                                                 floor_axis, P.floor_value, s->d_dest, s->d_int + 3);
+    SCHK(hipGetLastError());
+    // inside a stage the other two position slots follow the same erasures
+    PartSlots sl;
+    double* slots[3][3] = {{s->px, s->py, s->pz}, {s->P1[0], s->P1[1], s->P1[2]}, {s->P2[0], s->P2[1], s->P2[2]}};
+    sl.n = full ? 1 : 3;
+    for (int k = 0; k < 9; ++k) sl.p[k] = full ? (k < 3 ? *kp[k] : nullptr) : slots[k / 3][k % 3];
+    SRC(sim_compact_marked(s, sl));
+  } else if (s->np > 0) {
+    k_mark_particles<<<(s->np + 255) / 256, 256, 0, s->st>>>(s->np, *kp[0], *kp[1], *kp[2], ih, s->hk, s->hv, s->hcap, s->flags,
+                                                            s->layer);
   }
+  // the particle count starts the fill's appends
+  SCHK(hipMemcpyAsync(s->d_int, &s->np, sizeof(int), hipMemcpyHostToDevice, s->st));
   // Algorithms_Create_Fluid_Particle(_Depth), or inside a stage
   // Algorithms_Create_Interior_Depth (no particles)
   if (s->nb > 0 && (P.particles_per_inflow_cell > 0 || P.fill_cube_n > 0 || dmode))
@@ -592,6 +603,7 @@ void mac_sim_destroy(mac_sim* s) {
   sim_free_bricks(s);
   for (double* p : {s->px, s->py, s->pz, s->qx, s->qy, s->qz, s->P1[0], s->P1[1], s->P1[2], s->P2[0], s->P2[1], s->P2[2]}) hipFree(p);
   hipFree(s->d_dest);
+  hipFree(s->d_holes);
   hipFree(s->d_int);
   hipFree(s->d_any);
   hipFree(s->d_front);

@@ -281,15 +281,14 @@ __global__ void k_reset_layer_depth(size_t ncell, const uint8_t* __restrict__ fl
 // A particle in an Interior cell gives the cell layer 0 and is erased when the
 // cell's depth exceeds depth_remove; any other particle's cell gets layer 0
 // and SetFluid.  depth == nullptr: no cell is treated as Interior (the
-// non-depth Algorithms_Layer_Initial_With_Particle with the floor).  The kept
-// particles are appended to q (*nkept counts them; order across waves is not
-// kept, particles being independent).
+// non-depth Algorithms_Layer_Initial_With_Particle with the floor).  keep[i]
+// records whether particle i stays; *ndropped counts the others, which
+// k_list_holes_marked / k_fill_holes_marked then compact away.
 __global__ void k_mark_particles_keep(int np, const double* __restrict__ px, const double* __restrict__ py,
                                       const double* __restrict__ pz, SimIH ih, const uint64_t* __restrict__ hk,
                                       const int* __restrict__ hv, int cap, uint8_t* __restrict__ flags, int8_t* __restrict__ layer,
                                       const int16_t* __restrict__ depth, int depth_remove, int floor_axis, double floor_value,
-                                      double* __restrict__ qx, double* __restrict__ qy, double* __restrict__ qz,
-                                      int* __restrict__ nkept, int* __restrict__ dest) {
+                                      int* __restrict__ keepmark, int* __restrict__ ndropped) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= np) return;
   const double x = px[i], y = py[i], z = pz[i];
@@ -312,27 +311,35 @@ __global__ void k_mark_particles_keep(int np, const double* __restrict__ px, con
       }
     }
   }
-  const int slot = wave_append(keep, nkept);
-  if (dest) dest[i] = slot;
-  if (slot >= 0) {
-    qx[slot] = x;
-    qy[slot] = y;
-    qz[slot] = z;
-  }
+  keepmark[i] = keep ? 1 : 0;
+  wave_count(!keep, ndropped);
 }
 
-// The other position slots of the particles k_mark_particles_keep kept, to
-// the places it gave them.
-__global__ void k_scatter_particles(int np, const int* __restrict__ dest, const double* __restrict__ sx,
-                                    const double* __restrict__ sy, const double* __restrict__ sz, double* __restrict__ qx,
-                                    double* __restrict__ qy, double* __restrict__ qz) {
+// In-place compaction by a keep mark, over up to three position slots moved
+// together: with d dropped among np, k_list_holes_marked lists the dropped
+// indices below np - d, k_fill_holes_marked moves the kept particles at or
+// above it into them (the two counts are equal).
+__global__ void k_list_holes_marked(int n_new, const int* __restrict__ keepmark, int* __restrict__ holes,
+                                    int* __restrict__ nholes) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n_new) return;
+  const int o = wave_append(keepmark[i] == 0, nholes);
+  if (o >= 0) holes[o] = i;
+}
+
+struct PartSlots {
+  double* p[9];  // x, y, z of each slot
+  int n;         // slots in use (1..3)
+};
+
+__global__ void k_fill_holes_marked(int n_new, int np, const int* __restrict__ keepmark, const int* __restrict__ holes,
+                                    int* __restrict__ nfilled, PartSlots sl) {
+  const int i = n_new + blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= np) return;
-  const int d = dest[i];
-  if (d < 0) return;
-  qx[d] = sx[i];
-  qy[d] = sy[i];
-  qz[d] = sz[i];
+  const int o = wave_append(keepmark[i] != 0, nfilled);
+  if (o < 0) return;
+  const int j = holes[o];
+  for (int k = 0; k < 3 * sl.n; ++k) sl.p[k][j] = sl.p[k][i];
 }
 
 // Start of Algorithms_Update_CellType_Layer_Depth::Do (:31-56): depth := empty;
@@ -366,30 +373,92 @@ __global__ void k_depth_surface(int nb, const int* __restrict__ nbr27, const uin
   }
 }
 
-// One level of the depth search (:96-133): a layer-0 cell of empty depth next
-// to a cell of depth d-1 gets depth d, and becomes Fluid (SetFluid) when
-// d < depth_interior, Interior otherwise.  *any is set when a cell changed.
-__global__ void k_depth_round(int nb, const int* __restrict__ nbr27, uint8_t* __restrict__ flags, const int8_t* __restrict__ layer,
-                              int16_t* __restrict__ depth, int d, int depth_interior, int* __restrict__ any) {
-  int b = blockIdx.x;
-  for (int t = threadIdx.x; t < MAC_BC; t += blockDim.x) {
-    size_t c = (size_t)b * MAC_BC + t;
-    const unsigned f = flags[c];
-    if (!(f & F_EXISTS) || layer[c] != 0 || depth[c] != -1) continue;
-    const int x = t & 7, y = (t >> 3) & 7, z = t >> 6;
-    const int dx[6] = {-1, 1, 0, 0, 0, 0}, dy[6] = {0, 0, -1, 1, 0, 0}, dz[6] = {0, 0, 0, 0, -1, 1};
-    bool hit = false;
-    for (int k = 0; k < 6 && !hit; ++k) {
+// The depth search of Algorithms_Update_CellType_Layer_Depth.h:96-133 as a
+// relaxation.  The reference's level-by-level search gives every layer-0 cell
+// of empty depth its distance, in face steps through layer-0 cells, from the
+// surface cells (depth 0); that distance is the unique fixed point of
+// depth(c) = 1 + min over face neighbours of depth >= 0, reached from any
+// upper bounds.  One block per brick holds the brick's depths and the face
+// planes of its six neighbours in LDS and relaxes in place (values only fall
+// and are always lengths of real paths) until nothing changes in the brick;
+// the neighbour planes are those of the launch's start.  *any is set when a
+// launch changed a depth; launches repeat until one changes nothing.
+// Depths are int16: a distance above 32766 sets *overflow.
+__global__ void __launch_bounds__(256) k_depth_relax(int nb, const int* __restrict__ nbr27, const uint8_t* __restrict__ flags,
+                                                     const int8_t* __restrict__ layer, int16_t* __restrict__ depth,
+                                                     int* __restrict__ any, int* __restrict__ overflow) {
+  __shared__ int16_t D[1000];  // [z+1][y+1][x+1], x, y, z in -1..8
+  const int b = blockIdx.x;
+  for (int i = threadIdx.x; i < 1000; i += blockDim.x) {
+    const int hx = i % 10 - 1, hy = (i / 10) % 10 - 1, hz = i / 100 - 1;
+    const int out = (hx < 0 || hx > 7) + (hy < 0 || hy > 7) + (hz < 0 || hz > 7);
+    int16_t v = -1;
+    if (out <= 1) {  // the brick and its face planes; edges and corners are never read
       int cell;
-      const int nbk = locate(nbr27, b, x + dx[k], y + dy[k], z + dz[k], cell);
-      if (nbk < 0) continue;
-      const size_t nc = (size_t)nbk * MAC_BC + cell;
-      hit = (flags[nc] & F_EXISTS) && depth[nc] == d - 1;
+      const int nbk = locate(nbr27, b, hx, hy, hz, cell);
+      if (nbk >= 0) v = depth[(size_t)nbk * MAC_BC + cell];
     }
-    if (!hit) continue;
-    depth[c] = (int16_t)d;
+    D[i] = v;
+  }
+  __syncthreads();
+  int li[2];
+  bool upd[2];
+  int16_t start[2];
+#pragma unroll
+  for (int k = 0; k < 2; ++k) {
+    const int t = threadIdx.x + 256 * k;
+    const size_t c = (size_t)b * MAC_BC + t;
+    li[k] = (t & 7) + 1 + 10 * (((t >> 3) & 7) + 1) + 100 * ((t >> 6) + 1);
+    start[k] = D[li[k]];
+    // surface cells (0) are fixed; every other existing layer-0 cell relaxes
+    upd[k] = (flags[c] & F_EXISTS) && layer[c] == 0 && start[k] != 0;
+  }
+  bool changed_any = false, big = false;
+  for (int it = 0; it < 4096; ++it) {
+    bool ch = false;
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+      if (!upd[k]) continue;
+      const int l = li[k];
+      int m = 1 << 30;
+      const int nbv[6] = {D[l - 1], D[l + 1], D[l - 10], D[l + 10], D[l - 100], D[l + 100]};
+#pragma unroll
+      for (int q = 0; q < 6; ++q)
+        if (nbv[q] >= 0) m = min(m, nbv[q] + 1);
+      if (m == (1 << 30)) continue;
+      const int cur = D[l];
+      if (cur < 0 || m < cur) {
+        if (m > 32766) {
+          big = true;
+          continue;
+        }
+        D[l] = (int16_t)m;
+        ch = true;
+      }
+    }
+    if (!__syncthreads_or(ch)) break;
+    changed_any = true;
+  }
+#pragma unroll
+  for (int k = 0; k < 2; ++k) {
+    const int t = threadIdx.x + 256 * k;
+    if (upd[k] && D[li[k]] != start[k]) depth[(size_t)b * MAC_BC + t] = D[li[k]];
+  }
+  if (big) *overflow = 1;
+  if (changed_any && threadIdx.x == 0) *any = 1;
+}
+
+// After the search: a cell it reached (layer 0, depth >= 1) becomes Fluid
+// (SetFluid) when its depth is below depth_interior, Interior otherwise.
+__global__ void k_depth_types(int nb, uint8_t* __restrict__ flags, const int8_t* __restrict__ layer,
+                              const int16_t* __restrict__ depth, int depth_interior) {
+  const int b = blockIdx.x;
+  for (int t = threadIdx.x; t < MAC_BC; t += blockDim.x) {
+    const size_t c = (size_t)b * MAC_BC + t;
+    const unsigned f = flags[c];
+    const int d = depth[c];
+    if (!(f & F_EXISTS) || layer[c] != 0 || d < 1) continue;
     flags[c] = (uint8_t)((f & ~7u) | (d < depth_interior ? mat_set_fluid(f & 7u) : 4u));
-    *any = 1;
   }
 }
 

@@ -297,3 +297,19 @@ def test_viscous_air_pressure_vs_oracle(name, prec):
         air = ((g["flags"] & 0x80) > 0) & ~dom_mask(g) & (ref["p"] != 0)
         assert air.any()
         assert np.allclose(p[air], ref["p"][air], rtol=1e-13, atol=0)
+
+
+@pytest.mark.parametrize("prec", [MAC_FP64, MAC_MIXED])
+def test_no_fluid_cells_leave_the_velocity(prec):
+    """A grid with no Fluid, Inflow or Interior cell has no unknowns: no
+    iteration, p = 0 and every face velocity unchanged, as in the oracle."""
+    g = W.random_grid(9, 2, 2, 2, fluid_p=0.0)
+    assert not dom_mask(g).any()
+    ref = O.project(g, rtol=1e-14)
+    pr = make(g, rtol=1e-12, precision=prec)
+    st = pr.Divergence_Projection()
+    assert st["iters"] == 0
+    assert np.all(pr.get_pressure() == 0.0) and np.all(ref["p"] == 0.0)
+    for a, b, k in zip(pr.get_velocity(), (ref["u"], ref["v"], ref["w"]), ("u", "v", "w")):
+        assert np.array_equal(a, g[k]) and np.array_equal(b, g[k])
+    pr.close()

@@ -348,3 +348,19 @@ def test_box_deletion_vs_oracle(integrator, prk):
         same_particles(pg, po, 1e-12)
         counts.append(len(pg))
     assert len(pts) > counts[0] > counts[1] > 0
+
+
+@pytest.mark.parametrize("integrator", [0, 2])
+def test_empty_state_steps_like_the_oracle(integrator):
+    """No bricks and no particles: a step changes nothing, and the time step
+    is the oracle's (the upper bound, nothing moves)."""
+    pair = both(h=(0.1, 0.1, 0.1), gravity=(0, 0, -9.81), particles_per_inflow_cell=0, integrator=integrator,
+                particle_rk=4 if integrator == 2 else 2)
+    for s in pair:
+        s.upload(np.zeros((0, 3), np.int32), np.zeros(0, np.uint8), u=np.zeros(0), v=np.zeros(0), w=np.zeros(0))
+    for _ in range(2):
+        sg, so = pair[0].step(), pair[1].step()
+        assert sg["dt"] == so["dt"]
+        dg, do = pair[0].download(), pair[1].download()
+        assert dg["keys"].shape == do["keys"].shape == (0, 3)
+        assert dg["particles"].shape == do["particles"].shape == (0, 3)

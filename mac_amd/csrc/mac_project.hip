@@ -92,6 +92,8 @@ struct mac_ctx {
   int solves_on_grid = 0;  // projections since the last build (graphs pay off from the third)
   const void* gz[2] = {nullptr, nullptr};  // the V-cycle output the captured pair starts from
   hipStream_t st = nullptr;
+  uint8_t* pin = nullptr;  // pinned staging for the hierarchy's host tables
+  size_t pin_cap = 0;
   int dev = 0;
   int nb0 = 0;
   std::vector<int> keys0;
@@ -193,6 +195,49 @@ template <typename T>
 int upload(mac_ctx* c, T** p, const std::vector<T>& h) {
   CHK(dalloc(c, p, h.size()));
   if (!h.empty()) HIPCHK(hipMemcpyAsync(*p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, c->st));
+  return 0;
+}
+
+// Host tables bound for the device, sent in one transfer: packed (256-byte
+// aligned) into the context's pinned staging buffer and copied into one
+// device block, whose parts the destinations then point at.  The staging
+// buffer is rewritten only after the stream has been synchronised since the
+// previous transfer.
+struct Uploads {
+  struct Item {
+    void** dst;
+    size_t off;
+  };
+  std::vector<Item> items;
+  std::vector<uint8_t> host;
+  template <typename T>
+  void add(T** dst, const std::vector<T>& v) {
+    const size_t off = host.size();
+    const size_t bytes = v.size() * sizeof(T);
+    host.resize(off + ((std::max<size_t>(bytes, 1) + 255) & ~(size_t)255), 0);
+    if (bytes) memcpy(host.data() + off, v.data(), bytes);
+    items.push_back({(void**)dst, off});
+  }
+};
+
+int flush_uploads(mac_ctx* c, Uploads& u) {
+  if (u.items.empty()) return 0;
+  const size_t n = u.host.size();
+  if (c->pin_cap < n) {
+    if (c->pin) hipHostFree(c->pin);
+    c->pin = nullptr;
+    c->pin_cap = 0;
+    const size_t cap = n + n / 2;
+    HIPCHK(hipHostMalloc((void**)&c->pin, cap, hipHostMallocDefault));
+    c->pin_cap = cap;
+  }
+  memcpy(c->pin, u.host.data(), n);
+  uint8_t* block;
+  CHK(dalloc(c, &block, n));
+  HIPCHK(hipMemcpyAsync(block, c->pin, n, hipMemcpyHostToDevice, c->st));
+  for (auto& it : u.items) *it.dst = block + it.off;
+  u.items.clear();
+  u.host.clear();
   return 0;
 }
 
@@ -448,17 +493,19 @@ int build_grid(mac_ctx* c, int nbricks, const int32_t* keys3, const uint8_t* fla
   for (auto& lv : c->L) build_nbr(lv);
 
   // device structures
+  Uploads up;
   for (size_t l = 0; l < c->L.size(); ++l) {
     Level& lv = c->L[l];
-    CHK(upload(c, &lv.nbr, lv.h_nbr));
-    if (!lv.h_child.empty()) CHK(upload(c, &lv.child, lv.h_child));
+    up.add(&lv.nbr, lv.h_nbr);
+    if (!lv.h_child.empty()) up.add(&lv.child, lv.h_child);
     if (!lv.h_parent.empty()) {
-      CHK(upload(c, &lv.parent, lv.h_parent));
-      CHK(upload(c, &lv.oct, lv.h_oct));
+      up.add(&lv.parent, lv.h_parent);
+      up.add(&lv.oct, lv.h_oct);
     }
     CHK(dalloc(c, &lv.code, (size_t)lv.nb * MAC_BC));
     CHK(dalloc(c, &lv.lab, (size_t)lv.nb * MAC_BC));
   }
+  CHK(flush_uploads(c, up));
   size_t n0 = (size_t)nbricks * MAC_BC;
   CHK(dalloc(c, &c->flags, n0));
   HIPCHK(hipMemcpyAsync(c->flags, flags, n0, hipMemcpyDefault, c->st));
@@ -480,7 +527,9 @@ int build_grid(mac_ctx* c, int nbricks, const int32_t* keys3, const uint8_t* fla
     k_brick_count<<<c->L[l].nb, KTHREADS, 0, c->st>>>(c->L[l].nb, c->L[l].code, dcnt + coff[l]);
   HIPCHK(hipGetLastError());
   std::vector<int> allcnt(coff.back());
+  std::vector<uint8_t> topcode(MAC_BC);  // the coarsest level's codes, for its exact solve
   HIPCHK(hipMemcpyAsync(allcnt.data(), dcnt, coff.back() * sizeof(int), hipMemcpyDeviceToHost, c->st));
+  HIPCHK(hipMemcpyAsync(topcode.data(), c->L.back().code, MAC_BC, hipMemcpyDeviceToHost, c->st));
   HIPCHK(hipStreamSynchronize(c->st));
   for (size_t l = 0; l < c->L.size(); ++l) {
     Level& lv = c->L[l];
@@ -494,7 +543,7 @@ int build_grid(mac_ctx* c, int nbricks, const int32_t* keys3, const uint8_t* fla
       }
     lv.nact = (int)act.size();
     lv.h_act = act;
-    CHK(upload(c, &lv.act, act));
+    up.add(&lv.act, act);
     if (l == 0) {
       // bricks whose faces the gradient update may touch: active, or with an
       // active -x/-y/-z neighbour
@@ -508,8 +557,8 @@ int build_grid(mac_ctx* c, int nbricks, const int32_t* keys3, const uint8_t* fla
         if (use) gl.push_back(b);
       }
       c->nglist = (int)gl.size();
-      CHK(upload(c, &c->glist, gl));
-      CHK(upload(c, &c->dkeys0, c->L[0].keys));
+      up.add(&c->glist, gl);
+      up.add(&c->dkeys0, c->L[0].keys);
       for (int i = 0; i < 3; ++i) c->ext0[i] = 0;
       for (int b = 0; b < c->nb0; ++b)
         for (int i = 0; i < 3; ++i) c->ext0[i] = std::max(c->ext0[i], c->L[0].keys[3 * b + i] + 1);
@@ -532,7 +581,7 @@ int build_grid(mac_ctx* c, int nbricks, const int32_t* keys3, const uint8_t* fla
       m[6] = nb[22];
       m[7] = 0;
     }
-    CHK(upload(c, &lv.meta, meta));
+    up.add(&lv.meta, meta);
     if (l + 1 < c->L.size()) {
       Level& cl = c->L[l + 1];
       std::vector<int> pm((size_t)std::max(lv.nact, 1) * PSTRIDE, -1);
@@ -548,7 +597,7 @@ int build_grid(mac_ctx* c, int nbricks, const int32_t* keys3, const uint8_t* fla
           m[2 + k] = cl.h_nbr[(size_t)P * 27 + (sx + 1) + 3 * (sy + 1) + 9 * (sz + 1)];
         }
       }
-      CHK(upload(c, &lv.pmeta, pm));
+      up.add(&lv.pmeta, pm);
       std::vector<int> rm((size_t)std::max(cl.nact, 1) * RSTRIDE, -1);
       for (int i = 0; i < cl.nact; ++i) {
         int C = cl.h_act[i];
@@ -561,24 +610,22 @@ int build_grid(mac_ctx* c, int nbricks, const int32_t* keys3, const uint8_t* fla
           m[4 + t] = cn >= 0 ? cl.h_child[(size_t)cn * 8 + (rx & 1) + 2 * (ry & 1) + 4 * (rz & 1)] : -1;
         }
       }
-      CHK(upload(c, &cl.rmeta, rm));
+      up.add(&cl.rmeta, rm);
     }
   }
   // coarsest exact solve
   {
     Level& lc = c->L.back();
-    std::vector<uint8_t> code(MAC_BC);
-    HIPCHK(hipMemcpyAsync(code.data(), lc.code, MAC_BC, hipMemcpyDeviceToHost, c->st));
-    HIPCHK(hipStreamSynchronize(c->st));
     std::vector<int> rowof, cellof;
     std::vector<double> inv;
-    dense_inverse(code, lc.ih2, rowof, cellof, inv);
+    dense_inverse(topcode, lc.ih2, rowof, cellof, inv);
     c->nc = (int)cellof.size();
     std::vector<float> inv32(inv.begin(), inv.end());
-    CHK(upload(c, &c->rowof, rowof));
-    CHK(upload(c, &c->cellof, cellof));
-    CHK(upload(c, &c->cinv64, inv));
-    CHK(upload(c, &c->cinv32, inv32));
+    up.add(&c->rowof, rowof);
+    up.add(&c->cellof, cellof);
+    up.add(&c->cinv64, inv);
+    up.add(&c->cinv32, inv32);
+    CHK(flush_uploads(c, up));
   }
   // vectors
   CHK(dalloc(c, &c->u, n0));
@@ -1094,6 +1141,7 @@ void mac_destroy(mac_ctx* c) {
   free_all(c);
   for (auto e : c->evpool) hipEventDestroy(e);
   if (c->hs) hipHostFree(c->hs);
+  if (c->pin) hipHostFree(c->pin);
   hipStreamDestroy(c->st);
   delete c;
 }

@@ -265,6 +265,38 @@ def jet_runs(args):
         s.close()
         log(f"fluid block: {wall:.1f} ms/step, {parts}, {npart} particles")
         res["block_256_14M_particles"] = r
+    # the same block under the reference jets' own step (test/Jet3.cpp): RK4
+    # "all", depth (3, 5), the particle floor, the 27-particle cube refill of
+    # shallow Interior cells, viscosity with the viscous air pressures
+    if args.big_rk4_steps > 0:
+        from mac_amd.lib import MAC_MIXED, Sim
+        b = J.fluid_block(seed=args.seed)
+        nu = 1.307e-6
+        kw = dict(b["params"], layer_max=2, integrator=2, seed=args.seed, particles_per_inflow_cell=0, fill_cube_n=1,
+                  depth_interior=3, depth_remove=5, floor_axis=2, floor_value=0.25, viscosity=nu)
+        s = Sim(**kw, proj={"rtol": 1e-8, "precision": MAC_MIXED, "viscosity": nu})
+        s.upload(b["keys"], b["flags"], u=b["u"], v=b["v"], w=b["w"], particles=b["particles"])
+        n0 = len(b["particles"])
+        del b
+        st = s.step()  # lays the air layers and the depth field, erases the deep particles
+        n1 = st["nparticles"]
+        parts = {k: 0.0 for k in ("ms_topology", "ms_forces", "ms_project", "ms_particles")}
+        iters, t0 = [], time.perf_counter()
+        for _ in range(args.big_rk4_steps):
+            st = s.step()
+            iters.append(sum(st["pcg_iters"]))
+            for k in parts:
+                parts[k] += st[k] / args.big_rk4_steps
+        wall = (time.perf_counter() - t0) * 1e3 / args.big_rk4_steps
+        r = {"workload": "fluid block 96x96x192 cells at 1 m/s in a 256^3 box (h=0.01), 8 particles per cell, the Jet3 "
+                         "step: RK4 all (5 cell-type updates, 4 acceleration projections per step), depth (3, 5), floor "
+                         "z=0.25, 27-particle cube refill, nu=1.307e-6, mixed projection rtol 1e-8",
+             "particles_start": n0, "particles_after_first_step": n1, "particles_end": st["nparticles"],
+             "fluid_cells": st["nfluid"], "bricks": st["nbricks"], "steps": args.big_rk4_steps,
+             "gpu_ms_per_step": wall, "gpu_ms_breakdown": parts, "pcg_iters_per_step": float(np.mean(iters))}
+        s.close()
+        log(f"fluid block, Jet3 step: {wall:.1f} ms/step, {parts}, {st['nparticles']} particles")
+        res["block_256_jet3_step"] = r
     return res
 
 
@@ -287,6 +319,7 @@ def main():
     ap.add_argument("--lateral-steps", type=int, default=500, help="time steps of the 192x96x96 lateral jet (0: skip)")
     ap.add_argument("--jet3-steps", type=int, default=100, help="RK4-all (Jet3) steps of the 192x96x96 lateral jet (0: skip)")
     ap.add_argument("--big-steps", type=int, default=10, help="steps of the 14M-particle fluid block (0: skip)")
+    ap.add_argument("--big-rk4-steps", type=int, default=5, help="Jet3-style steps of the fluid block (0: skip)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only: --gpus {args.gpus} is not supported (the projection runs on a "

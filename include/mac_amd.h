@@ -247,8 +247,12 @@ typedef struct {
   int nbricks;          /* bricks after the topology update                            */
   int nparticles;       /* particles after the move                                    */
   long long nfluid;     /* in-domain cells                                             */
-  int pcg_iters[2];     /* PCG iterations of the two projections                       */
+  int pcg_iters[2];     /* PCG iterations of the two projections (RK4 "all": of stages
+                           1-2 and of stages 3-4)                                       */
   double max_div;       /* max |div u| after the last projection                       */
+  /* host wall time per part (ms): cell-type updates (RK4 "all": the step's and
+     the four stages'), forces and velocity updates, projections, particle
+     moves (RK4 "all": the box deletion; the stages move the particles)       */
   double ms_topology, ms_forces, ms_project, ms_particles, ms_total;
 } mac_sim_stats;
 

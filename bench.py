@@ -139,41 +139,42 @@ def jet_runs(args):
     from mac_amd import jets as J
     res = {}
     # vertical
-    jet = J.vertical_jet(32, 32, 64, h=0.01, nozzle=4, inflow=1.0)
-    kw = dict(jet["params"], layer_max=2, integrator=0, particle_rk=2, seed=1)
-    r = {"workload": "vertical jet 32x32x64, h=0.01, nozzle 4x4 at 1.0 m/s, RK2, CFL 0.5, FP64 rtol 1e-12"}
-    s = _sim(jet, kw, {"rtol": 1e-12})
-    iters, parts, t0 = [], {k: 0.0 for k in ("ms_topology", "ms_forces", "ms_project", "ms_particles")}, time.perf_counter()
-    for _ in range(args.jet_steps):
-        st = s.step()
-        iters.append(sum(st["pcg_iters"]))
-        for k in parts:
-            parts[k] += st[k] / args.jet_steps
-    wall = time.perf_counter() - t0
-    r.update({"steps": args.jet_steps, "gpu_ms_per_step": wall * 1e3 / args.jet_steps, "final_bricks": st["nbricks"],
-              "final_particles": st["nparticles"], "final_fluid_cells": st["nfluid"],
-              "pcg_iters_per_step": float(np.mean(iters)), "max_div_after": st["max_div"], "gpu_ms_breakdown": parts})
-    s.close()
-    log(f"vertical jet: {r['gpu_ms_per_step']:.2f} ms/step over {args.jet_steps} steps, {r['final_particles']} particles")
-    if not args.no_cpu and args.jet_cpu_steps > 0:
-        from oracle import oracle as O
-        n = min(args.jet_cpu_steps, args.jet_steps)
+    if args.jet_steps > 0:
+        jet = J.vertical_jet(32, 32, 64, h=0.01, nozzle=4, inflow=1.0)
+        kw = dict(jet["params"], layer_max=2, integrator=0, particle_rk=2, seed=1)
+        r = {"workload": "vertical jet 32x32x64, h=0.01, nozzle 4x4 at 1.0 m/s, RK2, CFL 0.5, FP64 rtol 1e-12"}
         s = _sim(jet, kw, {"rtol": 1e-12})
-        t0 = time.perf_counter()
-        for _ in range(n):
-            s.step()
-        r["gpu_ms_per_step_first_cpu_steps"] = (time.perf_counter() - t0) * 1e3 / n
-        dg = s.download()
+        iters, parts, t0 = [], {k: 0.0 for k in ("ms_topology", "ms_forces", "ms_project", "ms_particles")}, time.perf_counter()
+        for _ in range(args.jet_steps):
+            st = s.step()
+            iters.append(sum(st["pcg_iters"]))
+            for k in parts:
+                parts[k] += st[k] / args.jet_steps
+        wall = time.perf_counter() - t0
+        r.update({"steps": args.jet_steps, "gpu_ms_per_step": wall * 1e3 / args.jet_steps, "final_bricks": st["nbricks"],
+                  "final_particles": st["nparticles"], "final_fluid_cells": st["nfluid"],
+                  "pcg_iters_per_step": float(np.mean(iters)), "max_div_after": st["max_div"], "gpu_ms_breakdown": parts})
         s.close()
-        o = O.Sim(**kw)
-        o.upload(jet["keys"], jet["flags"], u=jet["u"], v=jet["v"], w=jet["w"])
-        t0 = time.perf_counter()
-        for _ in range(n):
-            o.step()
-        r.update({"cpu_steps": n, "cpu_ms_per_step": (time.perf_counter() - t0) * 1e3 / n, "cpu_cores": 1})
-        r["vs_oracle"] = _compare(dg, o.download())
-        log(f"vertical jet oracle: {r['cpu_ms_per_step']:.1f} ms/step over {n} steps, {r['vs_oracle']}")
-    res["jet_vertical_32x32x64"] = r
+        log(f"vertical jet: {r['gpu_ms_per_step']:.2f} ms/step over {args.jet_steps} steps, {r['final_particles']} particles")
+        if not args.no_cpu and args.jet_cpu_steps > 0:
+            from oracle import oracle as O
+            n = min(args.jet_cpu_steps, args.jet_steps)
+            s = _sim(jet, kw, {"rtol": 1e-12})
+            t0 = time.perf_counter()
+            for _ in range(n):
+                s.step()
+            r["gpu_ms_per_step_first_cpu_steps"] = (time.perf_counter() - t0) * 1e3 / n
+            dg = s.download()
+            s.close()
+            o = O.Sim(**kw)
+            o.upload(jet["keys"], jet["flags"], u=jet["u"], v=jet["v"], w=jet["w"])
+            t0 = time.perf_counter()
+            for _ in range(n):
+                o.step()
+            r.update({"cpu_steps": n, "cpu_ms_per_step": (time.perf_counter() - t0) * 1e3 / n, "cpu_cores": 1})
+            r["vs_oracle"] = _compare(dg, o.download())
+            log(f"vertical jet oracle: {r['cpu_ms_per_step']:.1f} ms/step over {n} steps, {r['vs_oracle']}")
+        res["jet_vertical_32x32x64"] = r
     # lateral
     if args.lateral_steps > 0:
         from mac_amd.lib import MAC_FP64, MAC_MIXED
@@ -297,6 +298,42 @@ def jet_runs(args):
         s.close()
         log(f"fluid block, Jet3 step: {wall:.1f} ms/step, {parts}, {st['nparticles']} particles")
         res["block_256_jet3_step"] = r
+    # the large sparse jet: 512x256x256, a 12x12 nozzle at 3 m/s pulsed by 5%
+    # with a period of 40 steps, falling into a pool (about 20M fluid cells,
+    # 155M particles); the same steps with the mixed and the FP64 projection
+    if args.breakup_steps > 0:
+        from mac_amd.lib import MAC_FP64, MAC_MIXED, Sim
+        b = J.breakup_jet(seed=args.seed)
+        kw = dict(b["params"], layer_max=2, integrator=0, particle_rk=2, seed=args.seed, particles_per_inflow_cell=8)
+        r = {"workload": "breakup jet 512x256x256 (h=0.01): 12x12 nozzle at 3 m/s downward, inflow pulsed by 5% with a "
+                         "period of 40 steps (seeded phase), column falling into a 480x224x180-cell pool on a solid floor, "
+                         "8 particles per fluid cell, RK2, particle RK2, projection rtol 1e-8",
+             "fluid_cells_start": b["fluid_cells"], "particles_start": len(b["particles"]), "steps": args.breakup_steps}
+        final = {}
+        for name, prec in (("mixed", MAC_MIXED), ("fp64", MAC_FP64)):
+            s = Sim(**kw, proj={"rtol": 1e-8, "precision": prec})
+            s.upload(b["keys"], b["flags"], u=b["u"], v=b["v"], w=b["w"], particles=b["particles"])
+            st = s.step()  # lays the air layers and builds the first hierarchy
+            parts = {k: 0.0 for k in ("ms_topology", "ms_forces", "ms_project", "ms_particles")}
+            iters, t0 = [], time.perf_counter()
+            for _ in range(args.breakup_steps):
+                st = s.step()
+                iters.append(sum(st["pcg_iters"]))
+                for k in parts:
+                    parts[k] += st[k] / args.breakup_steps
+            wall = (time.perf_counter() - t0) * 1e3 / args.breakup_steps
+            r[name] = {"gpu_ms_per_step": wall, "gpu_ms_breakdown": parts, "pcg_iters_per_step": float(np.mean(iters)),
+                       "max_div_after": st["max_div"], "fluid_cells": st["nfluid"], "bricks": st["nbricks"],
+                       "particles": st["nparticles"]}
+            d = s.download()
+            final[name] = {k: d[k] for k in ("keys", "flags", "u", "v", "w", "p")}
+            del d
+            s.close()
+            log(f"breakup jet {name}: {wall:.1f} ms/step, {parts}, {st['nfluid']} fluid cells, {st['nparticles']} particles")
+        del b
+        r["mixed_vs_fp64"] = _compare(final["mixed"], final["fp64"])
+        log(f"breakup jet mixed vs fp64: {r['mixed_vs_fp64']}")
+        res["breakup_jet_512x256x256"] = r
     return res
 
 
@@ -320,6 +357,7 @@ def main():
     ap.add_argument("--jet3-steps", type=int, default=100, help="RK4-all (Jet3) steps of the 192x96x96 lateral jet (0: skip)")
     ap.add_argument("--big-steps", type=int, default=10, help="steps of the 14M-particle fluid block (0: skip)")
     ap.add_argument("--big-rk4-steps", type=int, default=5, help="Jet3-style steps of the fluid block (0: skip)")
+    ap.add_argument("--breakup-steps", type=int, default=3, help="steps of the 512x256x256 breakup jet per method (0: skip)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only: --gpus {args.gpus} is not supported (the projection runs on a "
@@ -465,7 +503,7 @@ def main():
         out["roofline_solve"] = {"algorithmic_bytes_per_solve": solve_bytes,
                                  "GBps": solve_bytes / (src["ms_per_solve"] * 1e-3) / 1e9,
                                  "frac": solve_bytes / (src["ms_per_solve"] * 1e-3) / 1e9 / HBM_PEAK_GBS}
-    if args.jet_steps > 0:
+    if max(args.jet_steps, args.lateral_steps, args.jet3_steps, args.big_steps, args.big_rk4_steps, args.breakup_steps) > 0:
         out.update(jet_runs(args))
     if not args.no_cpu:
         from oracle import oracle as O

@@ -239,6 +239,14 @@ typedef struct {
   int fill_cube_n;      /* > 0: new particles on the (2n+1)^3 lattice of
                            Policy_Add_Particle_Cube (Jet3: n = 1) instead of the
                            particles_per_inflow_cell jitter                           */
+  /* Inflow pulsation, the seeded counterpart of Algorithm_Init_Perturbation
+     (test/Jet.cpp:188-256): after the time step of every step, the const
+     faces of every existing Inflow cell get inflow_speed[i] * (1 + inflow_amp
+     * sin(2 pi n / inflow_period + phi)), n the step number (1 in the first
+     step), phi = 2 pi U(seed).  inflow_amp = 0: off.                           */
+  double inflow_speed[3];
+  double inflow_amp;
+  int inflow_period;
   mac_params proj;      /* projection solver settings                                  */
 } mac_sim_params;
 

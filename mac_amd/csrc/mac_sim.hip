@@ -565,6 +565,8 @@ int mac_sim_create(mac_sim** out, const mac_sim_params* p) {
   if (p->layer_max < 1 || p->layer_max > 7) return mac_internal_set_error(MAC_E_ARG, "layer_max must be in 1..7");
   if (p->integrator < 0 || p->integrator > 2) return mac_internal_set_error(MAC_E_ARG, "integrator must be 0, 1 or 2");
   if (p->fill_cube_n < 0 || p->fill_cube_n > 3) return mac_internal_set_error(MAC_E_ARG, "fill_cube_n must be in 0..3");
+  if (!std::isfinite(p->inflow_amp) || (p->inflow_amp != 0.0 && p->inflow_period < 1))
+    return mac_internal_set_error(MAC_E_ARG, "inflow_amp must be finite, with inflow_period >= 1 when it is not 0");
   if (p->depth_interior < 0 || p->depth_interior > 32767) return mac_internal_set_error(MAC_E_ARG, "depth_interior must be in 0..32767");
   if (p->floor_axis < -1 || p->floor_axis > 2) return mac_internal_set_error(MAC_E_ARG, "floor_axis must be -1, 0, 1 or 2");
   for (int i = 0; i < 3; ++i)
@@ -911,6 +913,18 @@ int sim_rk4all(mac_sim* s, double dt, mac_sim_stats* stats) {
   return 0;
 }
 
+// The inflow pulsation of step n = tick (mac_sim_params.inflow_amp).
+int sim_pulse_inflow(mac_sim* s) {
+  const mac_sim_params& P = s->prm;
+  if (P.inflow_amp == 0.0 || s->nb == 0) return 0;
+  const double phi = 2.0 * M_PI * sim_uniform(P.seed, 0x1f10, 0, 0);
+  const double f = 1.0 + P.inflow_amp * std::sin(2.0 * M_PI * (double)s->tick / (double)P.inflow_period + phi);
+  k_inflow_speed<<<s->nb, 256, 0, s->st>>>(s->nb, s->flags, s->S[0][0], s->S[0][1], s->S[0][2], P.inflow_speed[0] * f,
+                                            P.inflow_speed[1] * f, P.inflow_speed[2] * f);
+  SCHK(hipGetLastError());
+  return 0;
+}
+
 // Particles leaving [box_lo, box_hi) are deleted (slot 0).
 int sim_box_delete(mac_sim* s) {
   const mac_sim_params& P = s->prm;
@@ -957,6 +971,7 @@ int mac_sim_step(mac_sim* s, mac_sim_stats* stats) {
     // extrapolation of speed slot 0; the particles move inside the stages
     SRC(mac_sim_compute_dt(s, &dt));
     local.dt = dt;
+    SRC(sim_pulse_inflow(s));
     SRC(sim_extrapolate_fields(s, s->S[0][0], s->S[0][1], s->S[0][2]));
     SRC(sim_rk4all(s, dt, &local));
     t2 = std::chrono::steady_clock::now();
@@ -965,6 +980,7 @@ int mac_sim_step(mac_sim* s, mac_sim_stats* stats) {
   } else {
     SRC(mac_sim_compute_dt(s, &dt));
     local.dt = dt;
+    SRC(sim_pulse_inflow(s));
     SRC(mac_sim_advance(s, dt, &local));
     t2 = std::chrono::steady_clock::now();
     SRC(mac_sim_move_particles(s, dt));

@@ -235,6 +235,20 @@ __global__ void k_missing_neighbours(int nb, const int* __restrict__ bflag, cons
   out[3 * slot + 2] = kz;
 }
 
+// The pulsed inflow: the const faces of every existing Inflow cell get v[i].
+__global__ void k_inflow_speed(int nb, const uint8_t* __restrict__ flags, double* __restrict__ u, double* __restrict__ v,
+                               double* __restrict__ w, double v0, double v1, double v2) {
+  const int b = blockIdx.x;
+  for (int t = threadIdx.x; t < MAC_BC; t += blockDim.x) {
+    const size_t c = (size_t)b * MAC_BC + t;
+    const unsigned f = flags[c];
+    if (!(f & F_EXISTS) || (f & 7u) != 3u) continue;
+    if (f & 0x08u) u[c] = v0;
+    if (f & 0x10u) v[c] = v1;
+    if (f & 0x20u) w[c] = v2;
+  }
+}
+
 // Bricks holding a cell that the fill step can give layer 0: Inflow cells,
 // and Interior cells when `interior` is set.
 __global__ void k_inflow_bricks(int nb, const uint8_t* __restrict__ flags, int* __restrict__ bflag, int interior) {

@@ -308,6 +308,9 @@ class SimParams(ctypes.Structure):
         ("floor_axis", ctypes.c_int),
         ("floor_value", ctypes.c_double),
         ("fill_cube_n", ctypes.c_int),
+        ("inflow_speed", ctypes.c_double * 3),
+        ("inflow_amp", ctypes.c_double),
+        ("inflow_period", ctypes.c_int),
         ("proj", Params),
     ]
 
@@ -357,7 +360,7 @@ def sim_params(**kw):
     p = SimParams()
     load().mac_sim_default_params(ctypes.byref(p))
     for k, v in kw.items():
-        if k in ("h", "gravity", "box_lo", "box_hi"):
+        if k in ("h", "gravity", "box_lo", "box_hi", "inflow_speed"):
             if np.isscalar(v):
                 v = (v, v, v)
             arr = getattr(p, k)

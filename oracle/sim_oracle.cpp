@@ -84,6 +84,8 @@ struct Params {
   double floor_value;
   int cube_n;      // > 0: Policy_Add_Particle_Cube lattice instead of the jitter
   double proj_nu;  // > 0: the viscous air pressures of the projection
+  double inflow_speed[3], inflow_amp;  // the inflow pulsation (amp 0: off)
+  int inflow_period;
   uint64_t seed;
   double lo[3], hi[3];
 };
@@ -736,6 +738,24 @@ struct Sim {
   }
 
   // particles leaving [lo, hi) are deleted (slot 0)
+  // The seeded inflow pulsation (an addition of this build, standing where
+  // test/Jet.cpp:188-256 perturbs its inflow after the time step): the const
+  // faces of existing Inflow cells get inflow_speed * (1 + amp sin(2 pi n /
+  // period + phi)), n = tick, phi = 2 pi U(seed).
+  void pulse_inflow() {
+    if (P.inflow_amp == 0.0) return;
+    const double phi = 2.0 * M_PI * uni(P.seed, 0x1f10, 0, 0);
+    const double f = 1.0 + P.inflow_amp * std::sin(2.0 * M_PI * (double)tick / (double)P.inflow_period + phi);
+    for (auto& kv : grid) {
+      Chunk& ch = *kv.second;
+      for (int t = 0; t < 512; ++t) {
+        const unsigned fl = ch.flags[t];
+        if (!(fl & 0x80) || (fl & 7u) != 3u) continue;
+        for (int i = 0; i < 3; ++i)
+          if (fl & (0x08u << i)) ch.S[0][i][t] = P.inflow_speed[i] * f;
+      }
+    }
+  }
   void box_delete() {
     if (P.lo[0] == P.hi[0] && P.lo[1] == P.hi[1] && P.lo[2] == P.hi[2]) return;
     std::vector<std::array<double, 3>> out;
@@ -778,8 +798,8 @@ extern "C" {
 
 // prm: h[3], g[3], nu, cfl, dt_min, dt_max, lo[3], hi[3] (16 doubles);
 // iprm: layer_max, integrator, particle_rk, ppc, depth_interior, depth_remove,
-// floor_axis, fill_cube_n; prm[16] = floor_value, prm[17] = projection
-// viscosity; seed.
+// floor_axis, fill_cube_n, inflow_period; prm[16] = floor_value, prm[17] =
+// projection viscosity, prm[18..20] = inflow_speed, prm[21] = inflow_amp; seed.
 void* orc_sim_create(const double* prm, const int* iprm, unsigned long long seed) {
   Sim* s = new Sim();
   for (int i = 0; i < 3; ++i) {
@@ -803,6 +823,9 @@ void* orc_sim_create(const double* prm, const int* iprm, unsigned long long seed
   s->P.floor_value = prm[16];
   s->P.proj_nu = prm[17];
   s->P.cube_n = iprm[7];
+  s->P.inflow_period = iprm[8];
+  for (int i = 0; i < 3; ++i) s->P.inflow_speed[i] = prm[18 + i];
+  s->P.inflow_amp = prm[21];
   s->P.seed = seed;
   return s;
 }
@@ -888,6 +911,7 @@ double orc_sim_step(void* p, double* maxdiv) {
   Sim* s = (Sim*)p;
   s->topology();
   double dt = s->compute_dt();
+  s->pulse_inflow();
   if (s->P.integrator == 2) {
     // test/Jet3.cpp's init chain ends with the extrapolation of speed slot 0
     s->extrapolate(0);

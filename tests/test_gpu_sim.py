@@ -364,3 +364,47 @@ def test_empty_state_steps_like_the_oracle(integrator):
         dg, do = pair[0].download(), pair[1].download()
         assert dg["keys"].shape == do["keys"].shape == (0, 3)
         assert dg["particles"].shape == do["particles"].shape == (0, 3)
+
+
+@pytest.mark.parametrize("integrator,prk", [(0, 2), (2, 4)])
+def test_pulsed_inflow_jet_vs_oracle(integrator, prk):
+    """The vertical jet with the seeded inflow pulsation (5% at a period of 4
+    steps): the nozzle faces bit-exact, cells bit-exact, velocities and
+    particles within the loop's bars."""
+    jet = J.vertical_jet(16, 16, 24, h=0.01, nozzle=4, inflow=1.0)
+    kw = dict(jet["params"], layer_max=2, integrator=integrator, particle_rk=prk, seed=7, inflow_speed=(0.0, 0.0, -1.0),
+              inflow_amp=0.05, inflow_period=4)
+    pair = both(**kw)
+    upload_both(jet, pair)
+    for step in range(5):
+        sg, so = pair[0].step(), pair[1].step()
+        assert sg["dt"] == pytest.approx(so["dt"], rel=1e-9)
+        dg, do = pair[0].download(), pair[1].download()
+        same_cells(dg, do)
+        f = dg["flags"]
+        noz = ((f & 0x80) > 0) & ((f & 7) == W.INFLOW)
+        assert noz.sum() == 16 and np.array_equal(dg["w"][noz], do["w"][noz])
+        ex = (f & W.EXISTS) > 0
+        for k in ("u", "v", "w"):
+            ref = do[k][ex]
+            assert np.linalg.norm(dg[k][ex] - ref) <= 1e-8 * max(np.linalg.norm(ref), 1.0), (step, k)
+        same_particles(dg["particles"], do["particles"], 1e-9)
+
+
+def test_small_breakup_jet_vs_oracle():
+    """The large-jet layout (pulsed nozzle, falling column, pool on a solid
+    floor, open sides) at 32^3: three steps against the oracle."""
+    b = J.breakup_jet(nx=32, ny=32, nz=32, nozzle=4, pool_depth=6, margin=4, period=4)
+    kw = dict(b["params"], layer_max=2, integrator=0, particle_rk=2, seed=3, particles_per_inflow_cell=8)
+    pair = both(**kw)
+    upload_both(b, pair, particles=b["particles"])
+    for step in range(3):
+        sg, so = pair[0].step(), pair[1].step()
+        assert sg["dt"] == pytest.approx(so["dt"], rel=1e-9)
+        dg, do = pair[0].download(), pair[1].download()
+        same_cells(dg, do)
+        ex = (dg["flags"] & W.EXISTS) > 0
+        for k in ("u", "v", "w"):
+            ref = do[k][ex]
+            assert np.linalg.norm(dg[k][ex] - ref) <= 1e-8 * max(np.linalg.norm(ref), 1.0), (step, k)
+        same_particles(dg["particles"], do["particles"], 1e-9)

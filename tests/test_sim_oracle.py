@@ -431,3 +431,31 @@ def test_rk4all_free_fall_follows_the_reference_staging():
     assert np.all(d["u"][fluid] == 0.0) and np.all(d["v"][fluid] == 0.0)
     want = pts + np.array([0.0, 0.0, 2.0 / 3.0 * g * dt * dt])
     assert np.allclose(d["particles"], want, rtol=0, atol=1e-15)
+
+
+def test_inflow_pulsation_is_a_sinusoid_of_the_step():
+    """The pulsed inflow face speed v_n = v0 (1 + a sin(2 pi n / T + phi)):
+    with T = 8, v_n + v_{n+4} = 2 v0 (half a period apart), (v_n - v0)^2 +
+    (v_{n+2} - v0)^2 = (a v0)^2 (a quarter apart) and v_{n+8} = v_n, the same
+    on every Inflow cell's const face."""
+    jet = J.vertical_jet(16, 16, 24, h=0.01, nozzle=4, inflow=1.0)
+    v0, a, T = -1.5, 0.05, 8
+    kw = dict(jet["params"], layer_max=2, integrator=0, particle_rk=2, seed=11, particles_per_inflow_cell=0,
+              inflow_speed=(0.0, 0.0, v0), inflow_amp=a, inflow_period=T)
+    o = O.Sim(**kw)
+    o.upload(jet["keys"], jet["flags"], u=jet["u"], v=jet["v"], w=jet["w"])
+    vals = []
+    for _ in range(12):
+        o.step()
+        d = o.download()
+        f = d["flags"]
+        sel = ((f & 0x80) > 0) & ((f & 7) == W.INFLOW)
+        assert sel.sum() == 16
+        w = np.unique(d["w"][sel])
+        assert w.size == 1
+        vals.append(float(w[0]))
+    v = np.array(vals)
+    assert np.allclose(v[:8] + v[4:12], 2 * v0, rtol=0, atol=1e-14)
+    assert np.allclose((v[:10] - v0) ** 2 + (v[2:12] - v0) ** 2, (a * v0) ** 2, rtol=1e-10, atol=0)
+    assert np.allclose(v[8:12], v[0:4], rtol=0, atol=1e-14)
+    assert np.ptp(v) > 1.5 * a * abs(v0)

@@ -358,6 +358,7 @@ def main():
     ap.add_argument("--big-steps", type=int, default=10, help="steps of the 14M-particle fluid block (0: skip)")
     ap.add_argument("--big-rk4-steps", type=int, default=5, help="Jet3-style steps of the fluid block (0: skip)")
     ap.add_argument("--breakup-steps", type=int, default=3, help="steps of the 512x256x256 breakup jet per method (0: skip)")
+    ap.add_argument("--no-steps", action="store_true", help="the projection benchmark only: no time-step lines")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only: --gpus {args.gpus} is not supported (the projection runs on a "
@@ -503,6 +504,8 @@ def main():
         out["roofline_solve"] = {"algorithmic_bytes_per_solve": solve_bytes,
                                  "GBps": solve_bytes / (src["ms_per_solve"] * 1e-3) / 1e9,
                                  "frac": solve_bytes / (src["ms_per_solve"] * 1e-3) / 1e9 / HBM_PEAK_GBS}
+    if args.no_steps:
+        args.jet_steps = args.lateral_steps = args.jet3_steps = args.big_steps = args.big_rk4_steps = args.breakup_steps = 0
     if max(args.jet_steps, args.lateral_steps, args.jet3_steps, args.big_steps, args.big_rk4_steps, args.breakup_steps) > 0:
         out.update(jet_runs(args))
     if not args.no_cpu:

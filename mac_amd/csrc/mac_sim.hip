@@ -236,10 +236,20 @@ int sim_set_bricks(mac_sim* s, const std::vector<Key3>& nk) {
     k_relayout_u8<<<nbn, 256, 0, s->st>>>(nbn, d_from, s->flags, to.flags, (uint8_t)0);
     k_relayout_u8<<<nbn, 256, 0, s->st>>>(nbn, d_from, (const uint8_t*)s->layer, (uint8_t*)to.layer, (uint8_t)0xFF);
     k_relayout_i16<<<nbn, 256, 0, s->st>>>(nbn, d_from, s->depth, to.depth, (int16_t)-1);
+    RelayoutF64 arr;
+    int na = 0;
     for (int a = 0; a < 3; ++a)
-      for (int i = 0; i < 3; ++i) k_relayout_f64<<<nbn, 256, 0, s->st>>>(nbn, d_from, s->S[a][i], to.S[a][i]);
-    for (int i = 0; i < 3; ++i) k_relayout_f64<<<nbn, 256, 0, s->st>>>(nbn, d_from, s->A[i], to.A[i]);
-    k_relayout_f64<<<nbn, 256, 0, s->st>>>(nbn, d_from, s->P, to.P);
+      for (int i = 0; i < 3; ++i) {
+        arr.src[na] = s->S[a][i];
+        arr.dst[na++] = to.S[a][i];
+      }
+    for (int i = 0; i < 3; ++i) {
+      arr.src[na] = s->A[i];
+      arr.dst[na++] = to.A[i];
+    }
+    arr.src[na] = s->P;
+    arr.dst[na++] = to.P;
+    k_relayout_f64<<<dim3(nbn, na), 256, 0, s->st>>>(nbn, d_from, arr);
   }
   SCHK(hipGetLastError());
   {
@@ -300,22 +310,27 @@ int sim_compact_dropped(mac_sim* s) {
   return 0;
 }
 
-// Depth search of Algorithms_Update_CellType_Layer_Depth.h:96-133: relaxation
-// launches, two per read-back of their change flags, until one changes
-// nothing; then the reached cells' types.
+// Depth search of Algorithms_Update_CellType_Layer_Depth.h:96-133: batches
+// of four relaxation launches (a launch after one that changed nothing
+// returns at once), each batch's change flags read back once, until a
+// launch changes nothing; then the reached cells' types.
 int sim_depth_search(mac_sim* s) {
   const mac_sim_params& P = s->prm;
+  constexpr int batch = 4;
   for (int round = 0;; ++round) {
     if (round > 100000) return mac_internal_set_error(MAC_E_STATE, "depth search did not settle");
-    SCHK(hipMemsetAsync(s->d_any, 0, 3 * sizeof(int), s->st));
-    for (int k = 0; k < 2; ++k)
-      k_depth_relax<<<s->nb, 256, 0, s->st>>>(s->nb, s->nbr, s->flags, s->layer, s->depth, s->d_any + k, s->d_any + 2);
+    SCHK(hipMemsetAsync(s->d_any, 0, (batch + 1) * sizeof(int), s->st));
+    for (int k = 0; k < batch; ++k)
+      k_depth_relax<<<s->nb, 256, 0, s->st>>>(s->nb, s->nbr, s->flags, s->layer, s->depth, k ? s->d_any + k - 1 : nullptr,
+                                               s->d_any + k, s->d_any + batch);
     SCHK(hipGetLastError());
-    int any[3];
+    int any[batch + 1];
     SCHK(hipMemcpyAsync(any, s->d_any, sizeof(any), hipMemcpyDeviceToHost, s->st));
     SCHK(hipStreamSynchronize(s->st));
-    if (any[2]) return mac_internal_set_error(MAC_E_ARG, "fluid deeper than 32766 cells");
-    if (!any[1]) break;
+    if (any[batch]) return mac_internal_set_error(MAC_E_ARG, "fluid deeper than 32766 cells");
+    bool settled = false;
+    for (int k = 0; k < batch; ++k) settled |= !any[k];
+    if (settled) break;
   }
   k_depth_types<<<s->nb, 256, 0, s->st>>>(s->nb, s->flags, s->layer, s->depth, P.depth_interior);
   SCHK(hipGetLastError());

@@ -398,10 +398,13 @@ __global__ void k_depth_surface(int nb, const int* __restrict__ nbr27, const uin
 // the neighbour planes are those of the launch's start.  *any is set when a
 // launch changed a depth; launches repeat until one changes nothing.
 // Depths are int16: a distance above 32766 sets *overflow.
+// prev: the change flag of the launch before in the same batch (nullptr for
+// the first); when it is 0 the search has settled and the launch returns.
 __global__ void __launch_bounds__(256) k_depth_relax(int nb, const int* __restrict__ nbr27, const uint8_t* __restrict__ flags,
                                                      const int8_t* __restrict__ layer, int16_t* __restrict__ depth,
-                                                     int* __restrict__ any, int* __restrict__ overflow) {
+                                                     const int* __restrict__ prev, int* __restrict__ any, int* __restrict__ overflow) {
   __shared__ int16_t D[1000];  // [z+1][y+1][x+1], x, y, z in -1..8
+  if (prev && *prev == 0) return;
   const int b = blockIdx.x;
   for (int i = threadIdx.x; i < 1000; i += blockDim.x) {
     const int hx = i % 10 - 1, hy = (i / 10) % 10 - 1, hz = i / 100 - 1;
@@ -635,9 +638,17 @@ __global__ void k_relayout_i16(int nb, const int* __restrict__ from, const int16
   for (int t = threadIdx.x; t < MAC_BC; t += blockDim.x)
     dst[(size_t)j * MAC_BC + t] = o >= 0 ? src[(size_t)o * MAC_BC + t] : proto;
 }
-__global__ void k_relayout_f64(int nb, const int* __restrict__ from, const double* __restrict__ src, double* __restrict__ dst) {
-  int j = blockIdx.x;
-  int o = from[j];
+// Every FP64 per-cell array relaid in one launch: blockIdx.y picks the array.
+struct RelayoutF64 {
+  static constexpr int CAP = 16;
+  const double* src[CAP];
+  double* dst[CAP];
+};
+__global__ void k_relayout_f64(int nb, const int* __restrict__ from, RelayoutF64 arr) {
+  const int j = blockIdx.x;
+  const int o = from[j];
+  const double* __restrict__ src = arr.src[blockIdx.y];
+  double* __restrict__ dst = arr.dst[blockIdx.y];
   for (int t = threadIdx.x; t < MAC_BC; t += blockDim.x)
     dst[(size_t)j * MAC_BC + t] = o >= 0 ? src[(size_t)o * MAC_BC + t] : 0.0;
 }

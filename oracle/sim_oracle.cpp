@@ -738,8 +738,8 @@ struct Sim {
   }
 
   // particles leaving [lo, hi) are deleted (slot 0)
-  // The seeded inflow pulsation (an addition of this build, standing where
-  // test/Jet.cpp:188-256 perturbs its inflow after the time step): the const
+  // The seeded inflow pulsation (no counterpart in the reference; it stands
+  // where test/Jet.cpp:188-256 perturbs its inflow after the time step): the const
   // faces of existing Inflow cells get inflow_speed * (1 + amp sin(2 pi n /
   // period + phi)), n = tick, phi = 2 pi U(seed).
   void pulse_inflow() {

@@ -737,7 +737,6 @@ struct Sim {
     }
   }
 
-  // particles leaving [lo, hi) are deleted (slot 0)
   // The seeded inflow pulsation (no counterpart in the reference; it stands
   // where test/Jet.cpp:188-256 perturbs its inflow after the time step): the const
   // faces of existing Inflow cells get inflow_speed * (1 + amp sin(2 pi n /
@@ -756,6 +755,7 @@ struct Sim {
       }
     }
   }
+  // particles leaving [lo, hi) are deleted (slot 0)
   void box_delete() {
     if (P.lo[0] == P.hi[0] && P.lo[1] == P.hi[1] && P.lo[2] == P.hi[2]) return;
     std::vector<std::array<double, 3>> out;

@@ -110,6 +110,27 @@ inline int ilog2(int v) {
 // wave (4 when the height allows it).
 void launch_conv(mac_unet* u, const bf16* in, int in_cs, int in_co, int cin, Vol vol, const bf16* wt, int kpad, const float* bias,
                  bf16* out, int out_cs, int out_co, int cout, int relu) {
+  // the LDS-tiled kernel where its tile fits: H, W multiples of 16, Cin <= 32
+  if (vol.H % 16 == 0 && vol.W % 16 == 0 && (cin == 8 || cin == 16 || cin == 32)) {
+    const int blocks = vol.D * (vol.H / 16) * (vol.W / 16);
+    const int lc = ilog2(cin);
+#define CONVL(NB, C) \
+  k_conv3_lds<NB, C><<<blocks, 256, 0, u->st>>>(in, in_cs, in_co, lc, vol, wt, kpad, bias, out, out_cs, out_co, relu, 0)
+#define CONVC(NB)                          \
+  switch (cin) {                           \
+    case 8: CONVL(NB, 8); break;           \
+    case 16: CONVL(NB, 16); break;         \
+    default: CONVL(NB, 32); break;         \
+  }
+    switch (cout / 16) {
+      case 1: CONVC(1); break;
+      case 2: CONVC(2); break;
+      default: CONVC(4); break;
+    }
+#undef CONVC
+#undef CONVL
+    return;
+  }
   const int rows = (vol.H % 4 == 0) ? 4 : 1;
   const int waves = vol.D * (vol.H / rows) * (vol.W / 16);
   const int blocks = (waves + 3) / 4;

@@ -86,6 +86,86 @@ __global__ void __launch_bounds__(256) k_conv3(const bf16* __restrict__ in, int 
     }
 }
 
+// The same convolution with the input tile staged in LDS: a block of four
+// waves computes a 16 x 16 (x, y) tile of one z plane (wave w: rows 4w..4w+3),
+// and first loads the 18 x 18 x 3 input halo of its CIN channels once
+// (zero outside the volume), so each of the 27 taps reads the LDS copy
+// instead of re-reading global memory.  CIN = 1 << lcin in {8, 16, 32};
+// H and W multiples of 16.  Same arithmetic and K order as k_conv3.
+template <int NB, int CIN>
+__global__ void __launch_bounds__(256) k_conv3_lds(const bf16* __restrict__ in, int in_cs, int in_co, int lcin, Vol v,
+                                                   const bf16* __restrict__ wt, int Kpad, const float* __restrict__ bias,
+                                                   bf16* __restrict__ out, int out_cs, int out_co, int relu, int accum) {
+  constexpr int TX = 18, TY = 18, TZ = 3, C8 = CIN / 8;
+  __shared__ bf16x8 tile[TZ * TY * TX * C8];
+  const int nxt = v.W >> 4, nyt = v.H >> 4;
+  const int b = blockIdx.x;
+  const int x0 = (b % nxt) << 4, Y0 = ((b / nxt) % nyt) << 4, z = b / (nxt * nyt);
+  for (int e = threadIdx.x; e < TZ * TY * TX * C8; e += blockDim.x) {
+    const int c8 = e % C8, sidx = e / C8;
+    const int lx = sidx % TX, ly = (sidx / TX) % TY, lz = sidx / (TX * TY);
+    const int gx = x0 + lx - 1, gy = Y0 + ly - 1, gz = z + lz - 1;
+    bf16x8 val;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) val[j] = (bf16)0.0f;
+    if (gx >= 0 && gx < v.W && gy >= 0 && gy < v.H && gz >= 0 && gz < v.D)
+      val = *(const bf16x8*)(in + ((size_t)(gz * v.H + gy) * v.W + gx) * in_cs + in_co + 8 * c8);
+    tile[e] = val;
+  }
+  __syncthreads();
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int i = lane & 15, g = lane >> 4;
+  floatx4 acc[4][NB];
+#pragma unroll
+  for (int r = 0; r < 4; ++r)
+#pragma unroll
+    for (int nb = 0; nb < NB; ++nb) acc[r][nb] = floatx4{0.f, 0.f, 0.f, 0.f};
+  const int K = 27 * CIN;
+  const bf16* wrow = wt + (size_t)i * Kpad;
+  // weight fragments one chunk ahead, so their load overlaps the chunk's work
+  bf16x8 bnext[NB];
+#pragma unroll
+  for (int nb = 0; nb < NB; ++nb) bnext[nb] = *(const bf16x8*)(wrow + (size_t)16 * nb * Kpad + 8 * g);
+  for (int kc = 0; kc < Kpad; kc += 32) {
+    const int k = kc + 8 * g;
+    bf16x8 bfr[NB];
+#pragma unroll
+    for (int nb = 0; nb < NB; ++nb) bfr[nb] = bnext[nb];
+    if (kc + 32 < Kpad) {
+#pragma unroll
+      for (int nb = 0; nb < NB; ++nb) bnext[nb] = *(const bf16x8*)(wrow + (size_t)16 * nb * Kpad + k + 32);
+    }
+    const int tap = k >> lcin, c8 = (k & (CIN - 1)) >> 3;
+    const int dz = tap / 9, dy = (tap / 3) % 3, dx = tap % 3;  // offsets + 1 into the tile
+    const bool ok = k < K;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      bf16x8 a;
+#pragma unroll
+      for (int j = 0; j < 8; ++j) a[j] = (bf16)0.0f;
+      if (ok) a = tile[((dz * TY + 4 * w + r + dy) * TX + i + dx) * C8 + c8];
+#pragma unroll
+      for (int nb = 0; nb < NB; ++nb) acc[r][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bfr[nb], acc[r][nb], 0, 0, 0);
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < 4; ++r)
+#pragma unroll
+    for (int nb = 0; nb < NB; ++nb) {
+      const int co = 16 * nb + i;
+      const float bv = bias ? bias[co] : 0.f;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int row = 4 * g + j;
+        bf16* o = out + ((size_t)(z * v.H + Y0 + 4 * w + r) * v.W + x0 + row) * out_cs + out_co + co;
+        float val = acc[r][nb][j] + bv;
+        if (relu) val = val > 0.f ? val : 0.f;
+        if (accum) val += bf2f(*o);
+        *o = f2bf(val);
+      }
+    }
+}
+
 // Weight gradient of a 3x3x3 convolution, FP32:
 //   dW[co][tap*Cin + ci] += sum_v G[v][co] * X[v + off(tap)][ci]
 // Block = (tap, chunk of x-rows); thread = one 4x4 (co, ci) tile and a

@@ -17,34 +17,37 @@ from oracle import unet_oracle as U
 pytestmark = pytest.mark.gpu
 
 SHAPE = (8, 8, 64)
+# (D, H, W) with H a multiple of 16 on the first two levels: their
+# convolutions take the LDS-tiled kernel, the third level the direct one
+SHAPE_TILED = (4, 32, 64)
 
 
-def _inputs(seed):
+def _inputs(seed, shape=SHAPE):
     rng = np.random.default_rng(seed)
-    return rng.normal(size=(int(np.prod(SHAPE)), 5)).astype(np.float32), rng
+    return rng.normal(size=(int(np.prod(shape)), 5)).astype(np.float32), rng
 
 
-@pytest.mark.parametrize("seed", [0, 1])
-def test_forward_matches_oracle(seed):
-    net = UNet(*SHAPE, seed=seed)
+@pytest.mark.parametrize("seed,shape", [(0, SHAPE), (1, SHAPE), (2, SHAPE_TILED)])
+def test_forward_matches_oracle(seed, shape):
+    net = UNet(*shape, seed=seed)
     p = net.get_params().astype(np.float64)
-    x5, _ = _inputs(seed)
+    x5, _ = _inputs(seed, shape)
     out = net.forward(x5)
-    ref, _ = U.forward(p, x5, SHAPE, bf=True)
+    ref, _ = U.forward(p, x5, shape, bf=True)
     err = np.linalg.norm(out - ref) / np.linalg.norm(ref)
     assert err < 2e-2, err
 
 
-@pytest.mark.parametrize("seed", [0, 1])
-def test_backward_matches_oracle(seed):
-    net = UNet(*SHAPE, seed=seed + 10)
+@pytest.mark.parametrize("seed,shape", [(0, SHAPE), (1, SHAPE), (2, SHAPE_TILED)])
+def test_backward_matches_oracle(seed, shape):
+    net = UNet(*shape, seed=seed + 10)
     p = net.get_params().astype(np.float64)
-    x5, rng = _inputs(seed)
+    x5, rng = _inputs(seed, shape)
     net.forward(x5)
     gout = rng.normal(size=x5.shape[0]).astype(np.float32)
     g = net.backward(gout)
-    _, cache = U.forward(p, x5, SHAPE, bf=True)
-    ref = U.backward(p, cache, gout.astype(np.float64), SHAPE)
+    _, cache = U.forward(p, x5, shape, bf=True)
+    ref = U.backward(p, cache, gout.astype(np.float64), shape)
     lay, _ = U.layout()
     for k, (o, s) in lay.items():
         n = int(np.prod(s))

@@ -689,6 +689,60 @@ template <typename T>
 __global__ void __launch_bounds__(KTHREADS) k_restrict_sep(const int* __restrict__ rmeta, const uint8_t* __restrict__ codec,
                                                            const T* __restrict__ resf, T* __restrict__ rc, const Scal* s) {
   if (converged(rmeta, RSTRIDE, (int)gridDim.x, (int)blockIdx.x, s)) return;
+  if constexpr (sizeof(T) == 4) {
+    // FP32: both halves at once (the 18^3 fine region fits in LDS), so the
+    // global loads issue together and the three passes need three barriers
+    constexpr int NA = 18 * 18 * 18;  // [fz][fy][fx], fine x, y, z in -1..16
+    constexpr int NA1 = 18 * 18 * 8;  // [fz][fy][X]
+    __shared__ int fba[64];
+    __shared__ T FA[NA];
+    __shared__ T ta1[NA1];
+    T* ta2 = FA;  // [fz][Y][X], 18*64, after FA is consumed
+    const int* M = rmeta + (size_t)RSTRIDE * blockIdx.x;
+    const int C = M[0];
+    if (threadIdx.x < 64) fba[threadIdx.x] = M[4 + threadIdx.x];
+    __syncthreads();
+    T v[(NA + KTHREADS - 1) / KTHREADS];
+#pragma unroll
+    for (int k = 0; k < (NA + KTHREADS - 1) / KTHREADS; ++k) {
+      const int i = threadIdx.x + KTHREADS * k;
+      v[k] = T(0);
+      if (i < NA) {
+        const int gx = i % 18 - 1, gy = (i / 18) % 18 - 1, gz = i / 324 - 1;
+        const int f = fba[((gx + 8) >> 3) + 4 * ((gy + 8) >> 3) + 16 * ((gz + 8) >> 3)];
+        if (f >= 0) v[k] = resf[(size_t)f * MAC_BC + (gx & 7) + 8 * (gy & 7) + 64 * (gz & 7)];
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < (NA + KTHREADS - 1) / KTHREADS; ++k) {
+      const int i = threadIdx.x + KTHREADS * k;
+      if (i < NA) FA[i] = v[k];
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < NA1; i += blockDim.x) {
+      const int X = i & 7, fy = (i >> 3) % 18, fz = (i >> 3) / 18;
+      const T* row = &FA[(fz * 18 + fy) * 18 + 2 * X];
+      ta1[i] = T(0.25) * row[0] + T(0.75) * row[1] + T(0.75) * row[2] + T(0.25) * row[3];
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < 18 * 64; i += blockDim.x) {
+      const int X = i & 7, Y = (i >> 3) & 7, fz = i >> 6;
+      const T* col = &ta1[(fz * 18 + 2 * Y) * 8 + X];
+      ta2[i] = T(0.25) * col[0] + T(0.75) * col[8] + T(0.75) * col[16] + T(0.25) * col[24];
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < MAC_BC; t += blockDim.x) {
+      const int X = t & 7, Y = (t >> 3) & 7, Z = t >> 6;
+      const size_t c = (size_t)C * MAC_BC + t;
+      T acc = T(0);
+      if (codec[c] & CODE_DOM) {
+        const T* col = &ta2[(2 * Z) * 64 + Y * 8 + X];
+        acc = (T(0.25) * col[0] + T(0.75) * col[64] + T(0.75) * col[128] + T(0.25) * col[192]) * T(0.125);
+      }
+      rc[c] = acc;
+    }
+    return;
+  }
   constexpr int NF = 10 * 18 * 18;  // [fz][fy][fx], fine x,y in -1..16, 10 fine z per half
   constexpr int NT1 = 10 * 18 * 8;  // [fz][fy][X]
   __shared__ int fb[64];

@@ -603,6 +603,13 @@ __global__ void __launch_bounds__(KTHREADS) k_jacobi_block(int nact, const int* 
     TZ col0[8];
 #pragma unroll
     for (int z = 0; z < 8; ++z) col0[z] = col[z];
+    // FP32: the inverse diagonal once per cell instead of a division per sweep
+    TZ idg[8];
+#pragma unroll
+    for (int z = 0; z < 8; ++z) {
+      const TZ dg = diag_of<TZ>(cd[z], ihx2, ihy2, ihz2);
+      idg[z] = ((cd[z] & CODE_DOM) && dg > TZ(0)) ? TZ(1) / dg : TZ(0);
+    }
     for (int k = 0; k < w.m; ++k) {
       const TZ om = sizeof(TZ) == 4 ? (TZ)w.om[k] : (TZ)w.om64[k];
       TZ nw[8];
@@ -620,7 +627,10 @@ __global__ void __launch_bounds__(KTHREADS) k_jacobi_block(int nact, const int* 
         if (cd[z] & CODE_DOM) {
           TZ dg;
           const TZ lz = lap_cell<TZ>(cd[z], v, xm, xp, ym, yp, zm, zp, ihx2, ihy2, ihz2, dg);
-          out = dg > TZ(0) ? v + om * (rv[z] - lz) / dg : TZ(0);
+          if constexpr (sizeof(TZ) == 4)
+            out = idg[z] != TZ(0) ? v + om * (rv[z] - lz) * idg[z] : TZ(0);
+          else
+            out = dg > TZ(0) ? v + om * (rv[z] - lz) / dg : TZ(0);
         }
         nw[z] = out;
       }

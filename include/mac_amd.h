@@ -119,7 +119,8 @@ typedef struct {
 
 /* Defaults: h=1, rtol=1e-8, max_iter=500, FP64, nu=2, omega=0.9, check_every=4, stall_iters=0,
  * Chebyshev smoothing with a = 0.25 on level 0; below it nu_coarse=2 block-Jacobi
- * launches of coarse_inner=4 Chebyshev sweeps on [0.05, 2], damped by 0.8; the
+ * launches of coarse_inner=4 Chebyshev sweeps on [0.05, 2], damped by 0.8 (from
+ * level 3 on one undamped launch of nu_coarse * coarse_inner <= 8 sweeps); the
  * last two levels fused (coarse_tail=1); viscosity 0. */
 void mac_default_params(mac_params* prm);
 

@@ -87,6 +87,7 @@ struct mac_ctx {
   double om_pre[8], om_post[8];  // per-sweep Jacobi weights before / after the coarse correction, level 0
   double omc_pre[8], omc_post[8];  // the same on the coarse levels
   BlockW blockw;                   // inner sweeps of the coarse block smoother
+  BlockW blockw_deep;              // the single launch per side on the levels from BLOCK_DEEP on
   // two CG iterations captured as one graph, per V-cycle precision (0 FP64, 1 FP32)
   hipGraphExec_t gexec[2] = {nullptr, nullptr};
   int solves_on_grid = 0;  // projections since the last build (graphs pay off from the third)
@@ -725,11 +726,25 @@ void set_schedule(mac_ctx* c) {
     c->blockw.om64[k] = k < m ? pre[k] : 0.0;
     c->blockw.om[k] = (float)c->blockw.om64[k];
   }
+  // from BLOCK_DEEP on, the nu_coarse launches of m sweeps become one
+  // undamped launch of nu_coarse * m sweeps (at most 8)
+  const int md = std::min(8, nu_of(c->prm, 1) * m);
+  schedule(c->prm, md, c->prm.cheb_lower_coarse, pre, post);
+  c->blockw_deep.m = md;
+  c->blockw_deep.damp = 1.0;
+  for (int k = 0; k < 8; ++k) {
+    c->blockw_deep.om64[k] = k < md ? pre[k] : 0.0;
+    c->blockw_deep.om[k] = (float)c->blockw_deep.om64[k];
+  }
 }
 
 // ---- the V-cycle ----
 
 inline long long lvl_dom(const mac_ctx* c, int l) { return c->L[l].ndom; }
+
+// The first coarse level on which the block smoother runs as one launch per
+// side (those levels have few bricks, so a launch is mostly latency).
+#define BLOCK_DEEP 3
 
 // Damped-Jacobi sweep at level l (level 0 reads the CG residual in FP64).
 template <typename TZ>
@@ -787,10 +802,12 @@ int vcycle(mac_ctx* c, TZ** zout) {
     // smooth point-wise, fused or not)
     const bool block = l > 0 && l < nl - 2 && c->prm.coarse_inner > 1;
     if (block) {
-      for (int k = 0; k < nu; ++k) {
+      const bool deep = l >= BLOCK_DEEP && nu > 1;
+      for (int k = 0; k < (deep ? 1 : nu); ++k) {
         TimedLaunch tl(c, K_C_JACOBI, nd * ((k ? 3 : 2) * sz + 1));  // r, zin, zout, code
         k_jacobi_block<TZ, TZ><<<grid_of(lv.nact), KTHREADS, 0, c->st>>>(lv.nact, lv.meta, lv.code, (const TZ*)lv.r, k ? z : nullptr,
-                                                                          k ? zt : z, ih[0], ih[1], ih[2], c->blockw, c->s);
+                                                                          k ? zt : z, ih[0], ih[1], ih[2],
+                                                                          deep ? c->blockw_deep : c->blockw, c->s);
         if (k) std::swap(z, zt);
       }
     } else if (nu >= 2) {
@@ -883,14 +900,16 @@ int vcycle(mac_ctx* c, TZ** zout) {
       TimedLaunch tl(c, l == 0 ? K_PROLONG0 : K_C_PROLONG, nd * (2 * sz + 1) + (double)cl.ndom * sz);
       k_prolong_col<TZ><<<grid_of(lv.nact), KTHREADS, 0, c->st>>>(lv.nact, lv.pmeta, lv.code, zfin[l + 1], z, c->s);
     }
-    const int nu = nu_of(c->prm, l);
+    const bool blk = l > 0 && l < nl - 2 && c->prm.coarse_inner > 1;
+    const bool deep = blk && l >= BLOCK_DEEP && nu_of(c->prm, l) > 1;
+    const int nu = deep ? 1 : nu_of(c->prm, l);
     const double* opost = l ? c->omc_post : c->om_post;
     for (int k = 0; k < nu; ++k) {
-      if (l > 0 && l < nl - 2 && c->prm.coarse_inner > 1) {
+      if (blk) {
         const TZ ih[3] = {(TZ)lv.ih2[0], (TZ)lv.ih2[1], (TZ)lv.ih2[2]};
         TimedLaunch tl(c, K_C_JACOBI, nd * (3 * sz + 1));
         k_jacobi_block<TZ, TZ><<<grid_of(lv.nact), KTHREADS, 0, c->st>>>(lv.nact, lv.meta, lv.code, (const TZ*)lv.r, z, zt, ih[0],
-                                                                          ih[1], ih[2], c->blockw, c->s);
+                                                                          ih[1], ih[2], deep ? c->blockw_deep : c->blockw, c->s);
       } else {
         sweep<TZ>(c, l, z, zt, (l == 0 && k == nu - 1) ? 1 : 0, opost[k]);
       }

@@ -463,8 +463,16 @@ struct MG {
   int inner = 1;
   double omi[8] = {0};
   double damp = 1.0;
+  // from level 3 on (those levels have few bricks): one undamped launch of
+  // nuc * inner sweeps (at most 8) per side instead of nuc launches
+  int inner_d = 1;
+  double omi_d[8] = {0};
+  bool deep = false;  // the launch being run is a deep-level one
 
   void jacobi_block(DLevel& l, const std::vector<double>& r, const std::vector<double>* zin, std::vector<double>& zout) {
+    const int inner = deep ? inner_d : this->inner;
+    const double* omi = deep ? omi_d : this->omi;
+    const double damp = deep ? 1.0 : this->damp;
     const int bx = l.n[0] / 8, by = l.n[1] / 8, bz = l.n[2] / 8;
 #pragma omp parallel for schedule(static)
     for (long long B = 0; B < (long long)bx * by * bz; ++B) {
@@ -622,11 +630,13 @@ struct MG {
       const double* pre = l ? omc_pre : om_pre;
       const int n = l ? nuc : nu;
       if (l > 0 && l < nl - 2 && inner > 1) {
+        deep = l >= 3 && n > 1;
         jacobi_block(lv, lv.r, nullptr, lv.z);
-        for (int k = 1; k < n; ++k) {
+        for (int k = 1; k < (deep ? 1 : n); ++k) {
           jacobi_block(lv, lv.r, &lv.z, lv.zt);
           std::swap(lv.z, lv.zt);
         }
+        deep = false;
       } else {
         omega = pre[0];
         jacobi(lv, lv.r, nullptr, lv.z);
@@ -644,9 +654,11 @@ struct MG {
       DLevel& lv = L[l];
       prolong(lv, L[l + 1], lv.z);
       const double* post = l ? omc_post : om_post;
-      const int n = l ? nuc : nu;
+      const bool blk = l > 0 && l < nl - 2 && inner > 1;
+      deep = blk && l >= 3 && (l ? nuc : nu) > 1;
+      const int n = deep ? 1 : (l ? nuc : nu);
       for (int k = 0; k < n; ++k) {
-        if (l > 0 && l < nl - 2 && inner > 1) {
+        if (blk) {
           jacobi_block(lv, lv.r, &lv.z, lv.zt);
         } else {
           omega = post[k];
@@ -654,6 +666,7 @@ struct MG {
         }
         std::swap(lv.z, lv.zt);
       }
+      deep = false;
     }
   }
 };
@@ -700,6 +713,8 @@ int orc_mgpcg(int nb, const int32_t* keys, const uint8_t* flags, const double* h
   {
     double post[8];
     sched(mg.inner, cheb_lower_coarse, mg.omi, post);
+    mg.inner_d = std::min(8, mg.nuc * mg.inner);
+    sched(mg.inner_d, cheb_lower_coarse, mg.omi_d, post);
   }
   // level 0 dense over the brick bounding box
   {

@@ -150,6 +150,17 @@ __device__ __forceinline__ T lap_cell(unsigned cd, T vc, T xm, T xp, T ym, T yp,
   return d * vc - acc;
 }
 
+// a / d for a positive diagonal d: FP32 multiplies by the hardware reciprocal
+// (within 1 ulp; the same per-cell factor in every pre- and post-sweep, so the
+// smoother stays symmetric), FP64 divides.
+template <typename T>
+__device__ __forceinline__ T div_diag(T a, T d) {
+  if constexpr (sizeof(T) == 4)
+    return a * __builtin_amdgcn_rcpf(d);
+  else
+    return a / d;
+}
+
 template <typename T>
 __device__ __forceinline__ T diag_of(unsigned cd, T ihx2, T ihy2, T ihz2) {
   T d = T(0);
@@ -535,7 +546,7 @@ __global__ void __launch_bounds__(KTHREADS) k_jacobi(int nact, const int* __rest
                           TZ dg;
                           TZ lz = lap_cell<TZ>(cd[z], col[z], xm, xp, ym, yp, zm, zp, ihx2, ihy2, ihz2, dg);
                           TR rv = r[base + 64 * z];
-                          out = dg > TZ(0) ? col[z] + omega * ((TZ)rv - lz) / dg : TZ(0);
+                          out = dg > TZ(0) ? col[z] + div_diag<TZ>(omega * ((TZ)rv - lz), dg) : TZ(0);
                           if (want_dot) acc += (double)rv * (double)out;
                         }
                         zout[base + 64 * z] = out;
@@ -547,7 +558,7 @@ __global__ void __launch_bounds__(KTHREADS) k_jacobi(int nact, const int* __rest
         if (cd[z] & CODE_DOM) {
           TR rv = r[base + 64 * z];
           TZ dg = diag_of<TZ>(cd[z], ihx2, ihy2, ihz2);
-          out = dg > TZ(0) ? omega * (TZ)rv / dg : TZ(0);
+          out = dg > TZ(0) ? div_diag<TZ>(omega * (TZ)rv, dg) : TZ(0);
           if (want_dot) acc += (double)rv * (double)out;
         }
         zout[base + 64 * z] = out;
@@ -663,7 +674,7 @@ __global__ void __launch_bounds__(KTHREADS) k_jacobi2(int nact, const int* __res
       cd[z] = code[base + 64 * z];
       rv[z] = (TZ)r[base + 64 * z];
       TZ dg = diag_of<TZ>(cd[z], ihx2, ihy2, ihz2);
-      z1[z] = ((cd[z] & CODE_DOM) && dg > TZ(0)) ? omega * rv[z] / dg : TZ(0);
+      z1[z] = ((cd[z] & CODE_DOM) && dg > TZ(0)) ? div_diag<TZ>(omega * rv[z], dg) : TZ(0);
     }
     col_stencil<TZ>(
         M, lane, z1,
@@ -671,14 +682,14 @@ __global__ void __launch_bounds__(KTHREADS) k_jacobi2(int nact, const int* __res
           const size_t j = (size_t)nb * MAC_BC + cell;
           unsigned c2 = code[j];
           TZ d2 = diag_of<TZ>(c2, ihx2, ihy2, ihz2);
-          return ((c2 & CODE_DOM) && d2 > TZ(0)) ? omega * (TZ)r[j] / d2 : TZ(0);
+          return ((c2 & CODE_DOM) && d2 > TZ(0)) ? div_diag<TZ>(omega * (TZ)r[j], d2) : TZ(0);
         },
         [&](int z, TZ xm, TZ xp, TZ ym, TZ yp, TZ zm, TZ zp) {
           TZ out = TZ(0);
           if (cd[z] & CODE_DOM) {
             TZ dg;
             TZ lz = lap_cell<TZ>(cd[z], z1[z], xm, xp, ym, yp, zm, zp, ihx2, ihy2, ihz2, dg);
-            out = dg > TZ(0) ? z1[z] + omega2 * (rv[z] - lz) / dg : TZ(0);
+            out = dg > TZ(0) ? z1[z] + div_diag<TZ>(omega2 * (rv[z] - lz), dg) : TZ(0);
           }
           zout[base + 64 * z] = out;
         });

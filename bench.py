@@ -48,7 +48,7 @@ KERNEL_NAMES = {
 # The coarse classes are the launch-weighted mean over their levels (the
 # block smoother: four launches per iteration on levels 1 and 2, two on 3).
 MEASURED_TRAFFIC = {("jacobi_l0", 256, 0): 133.16e6, ("cg_pupdate_matvec", 256, 0): 249.74e6, ("cg_update", 256, 0): 253.74e6,
-                    ("jacobi2_l0", 256, 0): 142.82e6, ("residual_l0", 256, 0): 133.02e6, ("prolong_l0", 256, 0): 66.07e6,
+                    ("jacobi2_l0", 256, 0): 139.55e6, ("residual_l0", 256, 0): 133.02e6, ("prolong_l0", 256, 0): 66.07e6,
                     ("restrict_l1", 256, 0): 84.86e6, ("coarse_jacobi", 256, 0): 7.69e6, ("coarse_residual", 256, 0): 7.09e6,
                     ("coarse_prolong", 256, 0): 3.83e6, ("coarse_restrict", 256, 0): 5.90e6, ("coarse_tail", 256, 0): 0.07e6}
 

@@ -141,6 +141,15 @@ int mac_upload_bricks(mac_ctx* ctx, int nbricks, const int32_t* keys3, const uin
 /* Host <-> device copies of the cell fields, brick layout, nbricks*512 each. */
 int mac_set_velocity(mac_ctx* ctx, const double* u, const double* v, const double* w);
 int mac_get_velocity(mac_ctx* ctx, double* u, double* v, double* w);
+/* The real velocity the viscous air pressures are formed from
+ * (Policy_Dirichlet_Boundary.h:84-109 reads Speed_GetRef(-1,false), the
+ * speed slot, while Algorithms_Solve_Pressure_Force.h:16-24 has redirected
+ * Speed_GetRef() to the acceleration that is being projected).  Brick
+ * layout, nbricks*512 each, copied to the device; it holds for the current
+ * grid until the next mac_upload_bricks.  All three NULL: the air pressures
+ * are formed from the projected field again (the default).  Only read when
+ * mac_params.viscosity > 0. */
+int mac_set_real_velocity(mac_ctx* ctx, const double* u, const double* v, const double* w);
 int mac_get_pressure(mac_ctx* ctx, double* p);
 int mac_set_pressure(mac_ctx* ctx, const double* p);
 /* Device-side snapshot / restore of u,v,w (to re-run a projection on the same input). */

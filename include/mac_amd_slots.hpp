@@ -20,7 +20,9 @@
 //       CellType_GetRef().GetIsInDomain(), Speed_GetRef().Get/Set/Get_Const,
 //       Pressure_GetRef().Pressure_Set, m_h.GetRef_Inv().Get(i).  A cell type
 //       with GetIsSolid() marks solid cells (they only shape the coarse
-//       multigrid levels); without it every non-domain cell is air.
+//       multigrid levels) and one with GetIs0_Pres() marks _0_Pres cells
+//       (not air for the viscous air pressures); without them every
+//       non-domain cell is air.
 #ifndef MAC_AMD_SLOTS_HPP
 #define MAC_AMD_SLOTS_HPP
 
@@ -49,6 +51,19 @@ bool is_solid(C& c, std::true_type) { return c.GetIsSolid(); }
 template <class C>
 bool is_solid(C&, std::false_type) { return false; }
 
+// GetIs0_Pres (Data_CellType_Implemented_0_Pres.h:84): a _0_Pres cell is
+// neither in the domain nor air, so the viscous variant gives it no air
+// pressure (Policy_Dirichlet_Boundary.h:84-109 tests GetIsAir)
+template <class C, class = void>
+struct has_0_pres : std::false_type {};
+template <class C>
+struct has_0_pres<C, decltype((void)std::declval<C&>().GetIs0_Pres())> : std::true_type {};
+
+template <class C>
+bool is_0_pres(C& c, std::true_type) { return c.GetIs0_Pres(); }
+template <class C>
+bool is_0_pres(C&, std::false_type) { return false; }
+
 // Pressure_Set with the pressure object's own value type when it names one
 // (the reference wraps the value: type_pressure(res[i]), Algorithms_Solve_Pressure.h:189)
 template <class P, class = void>
@@ -64,6 +79,20 @@ template <class P>
 void pressure_set(P&& pr, double v, std::false_type) {
   pr.Pressure_Set(v);
 }
+
+// Speed_GetRef(-1, false): the speed slot itself even while
+// Data_Chunk_Speed::baccel_instead_speed redirects Speed_GetRef() to the
+// acceleration (Data_Chunk_Speed.h:44-55, Algorithms_Solve_Pressure_Force.h:16-24);
+// a cell type without that overload has only the one field.
+template <class D, class = void>
+struct has_real_speed : std::false_type {};
+template <class D>
+struct has_real_speed<D, decltype((void)std::declval<D&>().Speed_GetRef(-1, false))> : std::true_type {};
+
+template <class D>
+auto real_speed(D& d, std::true_type) -> decltype(d.Speed_GetRef(-1, false)) { return d.Speed_GetRef(-1, false); }
+template <class D>
+auto real_speed(D& d, std::false_type) -> decltype(d.Speed_GetRef()) { return d.Speed_GetRef(); }
 
 inline void report(int status, const char* message) {
   if (status != MAC_OK) std::cout << "mac_amd error or warning :" << status << " in " << message << ": " << mac_last_error() << std::endl;
@@ -117,13 +146,17 @@ class Algorithms_Solve_Pressure_HIP : public Policy {
   std::vector<int32_t> m_keys;            // 3 per brick
   std::vector<uint8_t> m_flags;           // 512 per brick
   std::vector<long long> m_cell_slot;     // grid iteration order -> brick slot (b * 512 + t)
-  std::vector<double> m_u, m_v, m_w, m_p;
+  std::vector<double> m_u, m_v, m_w, m_p, m_ur, m_vr, m_wr;
 
  public:
   mac_stats last_stats{};
 
   // viscosity > 0: the viscous variant, Algorithms_Solve_Pressure_Dirichlet
-  // (pass the grid's m_viscosity)
+  // (pass the grid's m_viscosity).  Its air pressures are formed from the real
+  // speed, Speed_GetRef(-1, false) (Policy_Dirichlet_Boundary.h:100-102), which
+  // Divergence_Projection uploads beside the projected field, so the adapter
+  // also works under Algorithms_Solve_Pressure_Force (test/Jet3.cpp:626,633),
+  // where the projected field is the acceleration.
   Algorithms_Solve_Pressure_HIP(DataType data, const Policy& pol, int precision = MAC_FP64, double rtol = 1e-8,
                                 double viscosity = 0.0)
       : Policy(pol), m_grid(data.m_data.GetGridData()) {
@@ -169,10 +202,13 @@ class Algorithms_Solve_Pressure_HIP : public Policy {
       m_cell_slot.push_back(slot);
       auto d = it.data();
       auto&& ct = d.CellType_GetRef();
+      typedef typename std::decay<decltype(ct)>::type type_ct;
       uint8_t mat = MAC_AIR;
       if (ct.GetIsInDomain()) mat = MAC_FLUID;
-      else if (mac_amd_detail::is_solid(ct, mac_amd_detail::has_solid<typename std::decay<decltype(ct)>::type>()))
+      else if (mac_amd_detail::is_solid(ct, mac_amd_detail::has_solid<type_ct>()))
         mat = MAC_SOLID;
+      else if (mac_amd_detail::is_0_pres(ct, mac_amd_detail::has_0_pres<type_ct>()))
+        mat = MAC_0_PRES;
       uint8_t f = (uint8_t)(MAC_FLAG_EXISTS | mat);
       auto&& sp = d.Speed_GetRef();
       if (sp.Get_Const(1)) f |= MAC_FLAG_CONST_X;
@@ -194,6 +230,12 @@ class Algorithms_Solve_Pressure_HIP : public Policy {
     m_v.assign(n, 0.0);
     m_w.assign(n, 0.0);
     m_p.assign(n, 0.0);
+    const bool viscous = m_prm.viscosity > 0.0;
+    if (viscous) {
+      m_ur.assign(n, 0.0);
+      m_vr.assign(n, 0.0);
+      m_wr.assign(n, 0.0);
+    }
     size_t ci = 0;
     for (iterator it = m_grid.begin(); it != m_grid.end(); ++it, ++ci) {
       auto d = it.data();
@@ -202,8 +244,16 @@ class Algorithms_Solve_Pressure_HIP : public Policy {
       m_u[s] = sp.Get(1);
       m_v[s] = sp.Get(2);
       m_w[s] = sp.Get(3);
+      if (viscous) {
+        auto&& rs = mac_amd_detail::real_speed(d, mac_amd_detail::has_real_speed<decltype(d)>());
+        m_ur[s] = rs.Get(1);
+        m_vr[s] = rs.Get(2);
+        m_wr[s] = rs.Get(3);
+      }
     }
     mac_amd_detail::report(mac_set_velocity(m_ctx, m_u.data(), m_v.data(), m_w.data()), " upload speed");
+    if (viscous)
+      mac_amd_detail::report(mac_set_real_velocity(m_ctx, m_ur.data(), m_vr.data(), m_wr.data()), " upload real speed");
     mac_amd_detail::report(mac_project(m_ctx, 0, &last_stats), " project");
     mac_amd_detail::report(mac_get_pressure(m_ctx, m_p.data()), " download pressure");
     mac_amd_detail::report(mac_get_velocity(m_ctx, m_u.data(), m_v.data(), m_w.data()), " download speed");

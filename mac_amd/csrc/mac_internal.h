@@ -18,6 +18,13 @@ MAC_HIDDEN int mac_internal_build(mac_ctx* c, int nbricks, const int32_t* keys3_
 // guess = 2 from the context's pressure as it stands (a learned guess).
 MAC_HIDDEN int mac_internal_project(mac_ctx* c, double* u, double* v, double* w, double* p, int guess, mac_stats* st);
 MAC_HIDDEN int mac_internal_set_error(int code, const char* msg);
+// The context's stream and device ordinal.
+MAC_HIDDEN hipStream_t mac_internal_stream(mac_ctx* c);
+MAC_HIDDEN int mac_internal_device(mac_ctx* c);
+// Device workspace of at least `bytes` for the CSR solve (slot A), owned by
+// the context: grown when a call needs more, freed by mac_destroy.  *pinned
+// receives a 256-byte pinned host block of the context for the solve's state.
+MAC_HIDDEN int mac_internal_csr_workspace(mac_ctx* c, size_t bytes, void** ws, void** pinned);
 // Device velocity the viscous air pressures are formed from (the real speed
 // of Speed_GetRef(-1, false) when an acceleration is projected); NULL: the
 // projected field itself.

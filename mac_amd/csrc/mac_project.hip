@@ -108,6 +108,13 @@ struct mac_ctx {
   // is formed from (nullptr: the field being projected)
   double* pD = nullptr;
   const double *ur = nullptr, *vr = nullptr, *wr = nullptr;
+  // device copy of a real velocity uploaded by mac_set_real_velocity (owned,
+  // part of the current grid's allocations)
+  double* ur_own[3] = {nullptr, nullptr, nullptr};
+  // slot A (mac_solve_csr): device workspace and pinned state block
+  void* csr_ws = nullptr;
+  size_t csr_ws_bytes = 0;
+  void* csr_pin = nullptr;
   int* dkeys0 = nullptr;  // level-0 shifted chunk keys on the device
   int ext0[3] = {0, 0, 0};
   // coarsest level exact solve
@@ -164,6 +171,8 @@ void free_all(mac_ctx* c, bool keep_for_reuse = false) {
     c->pool.clear();
   }
   c->pD = nullptr;
+  if (c->ur_own[0] && c->ur == c->ur_own[0]) c->ur = c->vr = c->wr = nullptr;
+  c->ur_own[0] = c->ur_own[1] = c->ur_own[2] = nullptr;
   c->L.clear();
   c->have_grid = false;
 }
@@ -1106,7 +1115,6 @@ int ensure(mac_ctx* c) {
 
 extern "C" {
 
-__attribute__((visibility("hidden"))) int mac_csr_set_error(const char* msg) { return fail(MAC_E_ARG, msg); }
 
 const char* mac_last_error(void) { return g_err.c_str(); }
 
@@ -1134,8 +1142,8 @@ int mac_create(mac_ctx** out, const mac_params* prm) {
   *out = nullptr;
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(MAC_E_NODEV, "no HIP device visible");
-  mac_ctx* c = new mac_ctx();
   if (prm && check_params(prm) != 0) return MAC_E_ARG;
+  mac_ctx* c = new mac_ctx();
   if (prm) c->prm = *prm;
   else mac_default_params(&c->prm);
   set_schedule(c);
@@ -1161,6 +1169,8 @@ void mac_destroy(mac_ctx* c) {
   for (auto e : c->evpool) hipEventDestroy(e);
   if (c->hs) hipHostFree(c->hs);
   if (c->pin) hipHostFree(c->pin);
+  if (c->csr_ws) hipFree(c->csr_ws);
+  if (c->csr_pin) hipHostFree(c->csr_pin);
   hipStreamDestroy(c->st);
   delete c;
 }
@@ -1193,6 +1203,26 @@ int mac_set_velocity(mac_ctx* c, const double* u, const double* v, const double*
   HIPCHK(hipMemcpyAsync(c->v, v, n, hipMemcpyHostToDevice, c->st));
   HIPCHK(hipMemcpyAsync(c->w, w, n, hipMemcpyHostToDevice, c->st));
   HIPCHK(hipStreamSynchronize(c->st));
+  return 0;
+}
+
+int mac_set_real_velocity(mac_ctx* c, const double* u, const double* v, const double* w) {
+  CHK(ensure(c));
+  if (!u && !v && !w) {
+    if (c->ur_own[0] && c->ur == c->ur_own[0]) c->ur = c->vr = c->wr = nullptr;
+    return 0;
+  }
+  if (!u || !v || !w) return fail(MAC_E_ARG, "mac_set_real_velocity: give all three components or none");
+  const size_t n = (size_t)c->nb0 * MAC_BC;
+  for (int i = 0; i < 3; ++i)
+    if (!c->ur_own[i]) CHK(dalloc(c, &c->ur_own[i], n));
+  const double* src[3] = {u, v, w};
+  for (int i = 0; i < 3; ++i)
+    HIPCHK(hipMemcpyAsync(c->ur_own[i], src[i], n * sizeof(double), hipMemcpyHostToDevice, c->st));
+  HIPCHK(hipStreamSynchronize(c->st));
+  c->ur = c->ur_own[0];
+  c->vr = c->ur_own[1];
+  c->wr = c->ur_own[2];
   return 0;
 }
 
@@ -1380,6 +1410,22 @@ int mac_max_divergence(mac_ctx* c, double* out) {
 }
 
 MAC_HIDDEN int mac_internal_set_error(int code, const char* msg) { return fail(code, msg); }
+MAC_HIDDEN hipStream_t mac_internal_stream(mac_ctx* c) { return c->st; }
+MAC_HIDDEN int mac_internal_device(mac_ctx* c) { return c->dev; }
+MAC_HIDDEN int mac_internal_csr_workspace(mac_ctx* c, size_t bytes, void** ws, void** pinned) {
+  if (!c->csr_pin) HIPCHK(hipHostMalloc(&c->csr_pin, 256, hipHostMallocDefault));
+  if (bytes > c->csr_ws_bytes) {
+    HIPCHK(hipStreamSynchronize(c->st));
+    if (c->csr_ws) hipFree(c->csr_ws);
+    c->csr_ws = nullptr;
+    c->csr_ws_bytes = 0;
+    HIPCHK(hipMalloc(&c->csr_ws, bytes));
+    c->csr_ws_bytes = bytes;
+  }
+  *ws = c->csr_ws;
+  *pinned = c->csr_pin;
+  return 0;
+}
 MAC_HIDDEN void mac_internal_set_real_velocity(mac_ctx* c, const double* u, const double* v, const double* w) {
   c->ur = u;
   c->vr = v;

@@ -79,6 +79,7 @@ SIGNATURES = {
     "mac_upload_bricks": (ctypes.c_int, [_P, ctypes.c_int, _P, _P, _P, _P, _P]),
     "mac_set_velocity": (ctypes.c_int, [_P, _P, _P, _P]),
     "mac_get_velocity": (ctypes.c_int, [_P, _P, _P, _P]),
+    "mac_set_real_velocity": (ctypes.c_int, [_P, _P, _P, _P]),
     "mac_get_pressure": (ctypes.c_int, [_P, _P]),
     "mac_set_pressure": (ctypes.c_int, [_P, _P]),
     "mac_save_velocity": (ctypes.c_int, [_P]),
@@ -186,6 +187,17 @@ class Projection:
             if x.size != self.nb * 512:
                 raise MacError("velocity arrays must hold 512 values per brick")
         _check(self.lib.mac_set_velocity(self.ctx, *[_ptr(x) for x in a]))
+
+    def set_real_velocity(self, u=None, v=None, w=None):
+        """Real speed for the viscous air pressures (None: the projected field)."""
+        if u is None:
+            _check(self.lib.mac_set_real_velocity(self.ctx, None, None, None))
+            return
+        a = [np.ascontiguousarray(x, dtype=np.float64).reshape(-1) for x in (u, v, w)]
+        for x in a:
+            if x.size != self.nb * 512:
+                raise MacError("velocity arrays must hold 512 values per brick")
+        _check(self.lib.mac_set_real_velocity(self.ctx, *[_ptr(x) for x in a]))
 
     def get_velocity(self):
         a = [np.empty(self.nb * 512, dtype=np.float64) for _ in range(3)]

@@ -7,8 +7,14 @@
 //   slots_test IN OUT
 // IN (little-endian): int32 nb; int32 keys[3nb]; u8 flags[512nb];
 //   f64 u,v,w[512nb]; f64 h[3]; int32 n; int32 off[n+1]; int32 nnz;
-//   int32 idx[nnz]; f64 val[nnz]; f64 b[n]
-// OUT: f64 p,u,v,w[512nb] (slot B, brick layout), f64 x[n] (slot A), int32 iters
+//   int32 idx[nnz]; f64 val[nnz]; f64 b[n]; f64 viscosity; int32 force;
+//   when force != 0: f64 ax,ay,az[512nb]
+// OUT: f64 p,u,v,w[512nb] (slot B, brick layout: the projected field, i.e.
+//   the acceleration when force != 0), f64 x[n] (slot A), int32 iters
+// force != 0 runs slot B the way Algorithms_Solve_Pressure_Force does
+// (src/Algorithms_Solve_Pressure_Force.h:16-24): Speed_GetRef() is
+// redirected to the acceleration around the three calls, while
+// Speed_GetRef(-1, false) still returns the speed (Data_Chunk_Speed.h:44-55).
 // Without a GPU both slots print the library's error and the program still
 // exits 0, as the reference's solver policy does.
 #include <cstdio>
@@ -42,6 +48,7 @@ struct CellType {
   int mat;
   bool GetIsInDomain() const { return mat == 1 || mat == 3 || mat == 4; }
   bool GetIsSolid() const { return mat == 2; }
+  bool GetIs0_Pres() const { return mat == 5; }
 };
 struct PressureValue {
   double v;
@@ -56,12 +63,17 @@ struct Cell {
   Key key;
   CellType ct;
   Speed sp;
+  Speed acc;  // the acceleration slot, same const bits
   Pressure pr;
 };
+bool accel_instead_speed = false;  // Data_Chunk_Speed::baccel_instead_speed
 struct Handle {  // what the reference's iterator data() returns: a handle by value
   Cell* c;
   CellType& CellType_GetRef() { return c->ct; }
-  Speed& Speed_GetRef() { return c->sp; }
+  Speed& Speed_GetRef(int i = -1, bool b = true) {
+    (void)i;
+    return (accel_instead_speed && b) ? c->acc : c->sp;
+  }
   Pressure& Pressure_GetRef() { return c->pr; }
 };
 struct Spacing {
@@ -122,6 +134,12 @@ int main(int argc, char** argv) {
   std::vector<int32_t> idx(nnz);
   std::vector<double> val(nnz), b(n), x(n, 0.0);
   if (!rd(in, idx.data(), idx.size()) || !rd(in, val.data(), val.size()) || !rd(in, b.data(), b.size())) return 2;
+  double viscosity = 0.0;
+  int32_t force = 0;
+  if (!rd(in, &viscosity, 1) || !rd(in, &force, 1)) return 2;
+  std::vector<double> ax(u.size(), 0.0), ay(u.size(), 0.0), az(u.size(), 0.0);
+  if (force && (!rd(in, ax.data(), ax.size()) || !rd(in, ay.data(), ay.size()) || !rd(in, az.data(), az.size())))
+    return 2;
 
   // the grid: every existing cell, in brick order (any order works)
   mock::Grid grid;
@@ -135,6 +153,7 @@ int main(int argc, char** argv) {
       c.key = {{8 * keys[3 * bk] + (t & 7), 8 * keys[3 * bk + 1] + ((t >> 3) & 7), 8 * keys[3 * bk + 2] + (t >> 6)}};
       c.ct.mat = f & MAC_FLAG_MAT_MASK;
       c.sp = {{u[s], v[s], w[s]}, {(f & MAC_FLAG_CONST_X) != 0, (f & MAC_FLAG_CONST_Y) != 0, (f & MAC_FLAG_CONST_Z) != 0}};
+      c.acc = {{ax[s], ay[s], az[s]}, {c.sp.c[0], c.sp.c[1], c.sp.c[2]}};
       grid.cells.push_back(c);
     }
   mock::DataRef data{mock::Data{&grid}};
@@ -142,10 +161,12 @@ int main(int argc, char** argv) {
   // slot B
   int32_t iters = -1;
   {
-    Algorithms_Solve_Pressure_HIP<mock::DataRef, mock::NoPolicy> alg(data, mock::NoPolicy(), MAC_FP64, 1e-12);
+    Algorithms_Solve_Pressure_HIP<mock::DataRef, mock::NoPolicy> alg(data, mock::NoPolicy(), MAC_FP64, 1e-12, viscosity);
+    mock::accel_instead_speed = force != 0;
     alg.Init_Iteration();
     alg.Divergence_Projection();
     alg.End_Iteration();
+    mock::accel_instead_speed = false;
     iters = alg.last_stats.iters;
   }
   // slot A on the reference-assembled system
@@ -165,10 +186,11 @@ int main(int argc, char** argv) {
     using mac_amd_detail::offset_of;
     const int bk = bidx[std::make_tuple(chunk_of(c.key.k[0]), chunk_of(c.key.k[1]), chunk_of(c.key.k[2]))];
     const size_t s = (size_t)bk * 512 + offset_of(c.key.k[0]) + 8 * offset_of(c.key.k[1]) + 64 * offset_of(c.key.k[2]);
+    const mock::Speed& out_field = force ? c.acc : c.sp;
     p[s] = c.pr.p;
-    uo[s] = c.sp.s[0];
-    vo[s] = c.sp.s[1];
-    wo[s] = c.sp.s[2];
+    uo[s] = out_field.s[0];
+    vo[s] = out_field.s[1];
+    wo[s] = out_field.s[2];
   }
   std::ofstream out(argv[2], std::ios::binary);
   for (auto* a : {&p, &uo, &vo, &wo}) out.write(reinterpret_cast<const char*>(a->data()), sizeof(double) * a->size());

@@ -17,7 +17,9 @@ def build():
                            "-L" + os.path.join(ROOT, "mac_amd"), "-lmac_amd", "-Wl,-rpath,$ORIGIN/../../mac_amd"])
 
 
-def write_input(path, g, n, off, idx, val, b):
+def write_input(path, g, n, off, idx, val, b, viscosity=0.0, accel=None):
+    """accel = (ax, ay, az): slot B runs under the Force wrapper and projects
+    them, the air pressures coming from g's u, v, w (the real speed)."""
     nb = g["keys"].shape[0]
     with open(path, "wb") as f:
         np.array([nb], np.int32).tofile(f)
@@ -33,6 +35,11 @@ def write_input(path, g, n, off, idx, val, b):
         np.ascontiguousarray(idx[:nnz], np.int32).tofile(f)
         np.ascontiguousarray(val[:nnz], np.float64).tofile(f)
         np.ascontiguousarray(b[:n], np.float64).tofile(f)
+        np.array([viscosity], np.float64).tofile(f)
+        np.array([0 if accel is None else 1], np.int32).tofile(f)
+        if accel is not None:
+            for a in accel:
+                np.ascontiguousarray(a, np.float64).tofile(f)
 
 
 def read_output(path, nb, n):

@@ -67,7 +67,9 @@ def test_dt_and_interpolation_bitexact(seed):
     g, layer = random_state(seed)
     pair = both(h=(0.1, 0.2, 0.15), cfl=0.5)
     upload_both(g, pair, layer)
-    assert pair[0].compute_dt() == pytest.approx(pair[1].compute_dt(), rel=1e-15)
+    # same field on both sides: the max is order-free and sqrt and the division
+    # are correctly rounded, so dt is bit-exact
+    assert pair[0].compute_dt() == pair[1].compute_dt()
     rng = np.random.default_rng(seed)
     lo = np.array([-1.0, -1.0, -1.0]) * np.array([0.1, 0.2, 0.15])
     hi = np.array([25.0, 25.0, 17.0]) * np.array([0.1, 0.2, 0.15])
@@ -111,6 +113,7 @@ def test_jet_steps_vs_oracle(integrator, prk):
     for step in range(6):
         sg = pair[0].step()
         so = pair[1].step()
+        # dt of a step reads the last step's projected speeds, which agree to 1e-8
         assert sg["dt"] == pytest.approx(so["dt"], rel=1e-9)
         dg, do = pair[0].download(), pair[1].download()
         same_cells(dg, do)
@@ -240,6 +243,7 @@ def test_depth_jet_steps_vs_oracle():
     for step in range(8):
         sg = pair[0].step()
         so = pair[1].step()
+        # dt of a step reads the last step's projected speeds, which agree to 1e-8
         assert sg["dt"] == pytest.approx(so["dt"], rel=1e-9)
         dg, do = pair[0].download(), pair[1].download()
         same_cells(dg, do)
@@ -304,6 +308,7 @@ def test_rk4all_depth_jet_vs_oracle():
     for step in range(4):
         sg = pair[0].step()
         so = pair[1].step()
+        # dt of a step reads the last step's projected speeds, which agree to 1e-8
         assert sg["dt"] == pytest.approx(so["dt"], rel=1e-9)
         dg, do = pair[0].download(), pair[1].download()
         same_cells(dg, do)
@@ -378,6 +383,7 @@ def test_pulsed_inflow_jet_vs_oracle(integrator, prk):
     upload_both(jet, pair)
     for step in range(5):
         sg, so = pair[0].step(), pair[1].step()
+        # dt of a step reads the last step's projected speeds, which agree to 1e-8
         assert sg["dt"] == pytest.approx(so["dt"], rel=1e-9)
         dg, do = pair[0].download(), pair[1].download()
         same_cells(dg, do)
@@ -400,6 +406,7 @@ def test_small_breakup_jet_vs_oracle():
     upload_both(b, pair, particles=b["particles"])
     for step in range(3):
         sg, so = pair[0].step(), pair[1].step()
+        # dt of a step reads the last step's projected speeds, which agree to 1e-8
         assert sg["dt"] == pytest.approx(so["dt"], rel=1e-9)
         dg, do = pair[0].download(), pair[1].download()
         same_cells(dg, do)

@@ -1,0 +1,293 @@
+"""The reference's own known-answer tests for the force, projection-piece and
+particle policies, restated on the CPU oracle, plus closed forms whose answer
+depends on the RK2 / RK2-TVD stage weights.
+
+The reference's old tests are 1-D and 2-D (`test/old/*.h`); here each runs on
+a 3-D embedding: the 1-D cells on the x axis, the 2-D cells in the z = 0
+plane (copied to z = -1 and z = 1 where a transverse stencil reads them),
+with every other face velocity 0.  Where the old API differs from the
+current one the test says how its number maps.
+"""
+import numpy as np
+import pytest
+
+from mac_amd import workloads as W
+from oracle import oracle as O
+
+
+def grid_of(cells, h=1.0):
+    """cells: dict (x, y, z) -> (material, u, v, w).  Returns a brick grid dict
+    (keys, flags, u, v, w, h, layer) plus an index function (x, y, z) -> slot."""
+    bkeys = sorted({(x >> 3, y >> 3, z >> 3) for (x, y, z) in cells}, key=lambda k: (k[2], k[1], k[0]))
+    index = {k: i for i, k in enumerate(bkeys)}
+    nb = len(bkeys)
+    flags = np.zeros(nb * 512, np.uint8)
+    layer = np.full(nb * 512, -1, np.int8)
+    u, v, w = (np.zeros(nb * 512) for _ in range(3))
+
+    def slot(x, y, z):
+        return index[(x >> 3, y >> 3, z >> 3)] * 512 + (x & 7) + 8 * (y & 7) + 64 * (z & 7)
+
+    for (x, y, z), (mat, a, b, c) in cells.items():
+        s = slot(x, y, z)
+        flags[s] = mat | W.EXISTS
+        layer[s] = 0 if mat == W.FLUID else 1
+        u[s], v[s], w[s] = a, b, c
+    g = {"keys": np.array(bkeys, np.int32).reshape(-1, 3), "flags": flags, "u": u, "v": v, "w": w,
+         "h": (h, h, h), "layer": layer}
+    return g, slot
+
+
+def forces_of(g):
+    s = O.Sim(h=g["h"], gravity=(0, 0, 0), viscosity=0.0, particles_per_inflow_cell=0)
+    s.upload(g["keys"], g["flags"], g["layer"], u=g["u"], v=g["v"], w=g["w"])
+    s.forces(0)
+    return s.download()
+
+
+def plane(cells2d, zs=(-1, 0, 1)):
+    """2-D cells {(x, y): (mat, u, v)} copied over the z planes zs (w = 0)."""
+    return {(x, y, z): (m, a, b, 0.0) for (x, y), (m, a, b) in cells2d.items() for z in zs}
+
+
+# ------------------------------------------------------------------ upwind --
+def test_upwind_kat_1d():
+    """test/old/TestPolicy_Upwind_1_Order.h:99,104: u(0) = 0, u(1) = 1, h = 1:
+    the convection speed of cell 1 is -1 (u > 0: backward difference); with
+    u(0) = -0.1 that of cell 0 is +0.11 (u < 0: forward difference)."""
+    g, slot = grid_of({(0, 0, 0): (W.FLUID, 0.0, 0, 0), (1, 0, 0): (W.FLUID, 1.0, 0, 0)})
+    assert forces_of(g)["ax"][slot(1, 0, 0)] == pytest.approx(-1.0, abs=1e-10)
+    g["u"][slot(0, 0, 0)] = -0.1
+    assert forces_of(g)["ax"][slot(0, 0, 0)] == pytest.approx(0.11, abs=1e-10)
+
+
+def _kat_2d_cells(centre_speed, probe, others):
+    c = {xy: (W.AIR, 0.0, 0.0) for xy in others}
+    c[probe] = (W.FLUID,) + centre_speed
+    return c
+
+
+def test_upwind_kat_2d():
+    """TestPolicy_Upwind_1_Order.h:214-215: cell (1,1) with speed (1,1), its
+    eight neighbours at rest: both components of its convection speed are
+    -1.25 (-u du/dx = -1 plus -0.25 du/dy, 0.25 being the mean of the four v
+    around the x face, Get_Speed_At_Bound)."""
+    others = [(0, 0), (1, 0), (0, 1), (1, 2), (2, 1), (0, 2), (2, 0)]
+    cells = _kat_2d_cells((1.0, 1.0), (1, 1), others)
+    g, slot = grid_of(plane(cells))
+    d = forces_of(g)
+    assert d["ax"][slot(1, 1, 0)] == pytest.approx(-1.25, abs=1e-10)
+    assert d["ay"][slot(1, 1, 0)] == pytest.approx(-1.25, abs=1e-10)
+
+
+# -------------------------------------------------------------- convection --
+def test_convection_kat_1d():
+    """test/old/TestAlgorithms_Convection.h:159: a fluid cell at rest between
+    cells moving at 1 keeps its speed (U = 0 multiplies the difference)."""
+    cells = {(x, 0, 0): (W.AIR, 1.0, 0, 0) for x in (-2, -1, 1, 2)}
+    cells[(0, 0, 0)] = (W.FLUID, 0.0, 0, 0)
+    g, slot = grid_of(cells)
+    assert forces_of(g)["ax"][slot(0, 0, 0)] == 0.0
+
+
+def test_convection_kat_2d():
+    """TestAlgorithms_Convection.h:418-419 (old API: the algorithm updated the
+    speed with dt = 0.1, test :96-121; the current one adds to the
+    acceleration): fluid cell (0,0) at (-1,-1) with (1,1) at (1,1) and every
+    other neighbour at rest ends at -0.875 per component, i.e. its convection
+    acceleration is (-0.875 - (-1)) / 0.1 = 1.25: forward differences for
+    U < 0, the mean of the four transverse speeds being -0.25.  The opposite
+    difference for a sign, or a mean over (c, c+e_j, c+e_i, c+e_i+e_j), gives
+    another number here or in the upwind KAT."""
+    rest = [(1, 0), (0, 1), (1, 2), (2, 1), (0, 2), (2, 0), (2, 2), (0, -1), (-1, 0), (-1, -1), (-1, 1), (1, -1),
+            (-1, 2), (2, -1), (0, -2), (1, -2), (2, -2), (-1, -2), (-2, -2)]
+    cells = {xy: (W.AIR, 0.0, 0.0) for xy in rest}
+    cells[(1, 1)] = (W.AIR, 1.0, 1.0)
+    cells[(0, 0)] = (W.FLUID, -1.0, -1.0)
+    g, slot = grid_of(plane(cells))
+    d = forces_of(g)
+    s = slot(0, 0, 0)
+    assert -1.0 + 0.1 * d["ax"][s] == pytest.approx(-0.875, abs=1e-10)
+    assert -1.0 + 0.1 * d["ay"][s] == pytest.approx(-0.875, abs=1e-10)
+
+
+# -------------------------------------------------------------- divergence --
+@pytest.mark.parametrize("h,expect", [(1.0, -1.0), (0.1, -10.0)])
+def test_divergence_kat_1d(h, expect):
+    """test/old/TestPolicy_Divergence.h:99,107,252,260: u(0) = 1, u(1) = 0 gives
+    div(0) = -1/h; with u(1) = 1 it is 0."""
+    g, slot = grid_of({(0, 0, 0): (W.FLUID, 1.0, 0, 0), (1, 0, 0): (W.AIR, 0.0, 0, 0),
+                       (0, 1, 0): (W.AIR, 0, 0, 0), (0, 0, 1): (W.AIR, 0, 0, 0)}, h)
+    assert O.divergence(g)[slot(0, 0, 0)] == pytest.approx(expect, abs=1e-10)
+    g["u"][slot(1, 0, 0)] = 1.0
+    assert O.divergence(g)[slot(0, 0, 0)] == pytest.approx(0.0, abs=1e-10)
+
+
+@pytest.mark.parametrize("h,expect", [(1.0, -2.0), (0.1, -20.0)])
+def test_divergence_kat_2d(h, expect):
+    """TestPolicy_Divergence.h:185,338: (0,0) moving at (1,1), (1,0) and (0,1)
+    at rest: div = -2/h."""
+    g, slot = grid_of({(0, 0, 0): (W.FLUID, 1.0, 1.0, 0), (1, 0, 0): (W.AIR, 0, 0, 0),
+                       (0, 1, 0): (W.AIR, 0, 0, 0), (0, 0, 1): (W.AIR, 0, 0, 0)}, h)
+    assert O.divergence(g)[slot(0, 0, 0)] == pytest.approx(expect, abs=1e-10)
+
+
+# ---------------------------------------------------------------- gradient --
+def test_gradient_kat_1d():
+    """test/old/TestPolicy_Gradiant.h:98,106: p(0) = 1, p(-1) = 0: the x
+    gradient on cell 0's face is 1; with p(-1) = 1 it is 0.  The gradient
+    update writes u -= grad p, so from u = 0 the face holds -grad p."""
+    g, slot = grid_of({(0, 0, 0): (W.FLUID, 0, 0, 0), (-1, 0, 0): (W.FLUID, 0, 0, 0)})
+    p = np.zeros(g["flags"].size)
+    p[slot(0, 0, 0)] = 1.0
+    u, _, _ = O.gradient_update(g, p, g["u"], g["v"], g["w"])
+    assert -u[slot(0, 0, 0)] == pytest.approx(1.0, abs=1e-10)
+    p[slot(-1, 0, 0)] = 1.0
+    u, _, _ = O.gradient_update(g, p, g["u"], g["v"], g["w"])
+    assert -u[slot(0, 0, 0)] == pytest.approx(0.0, abs=1e-10)
+
+
+def test_gradient_kat_2d():
+    """TestPolicy_Gradiant.h:181: p(0,0) = 1, p(-1,0) = p(0,-1) = 0: the x
+    gradient on (0,0)'s face is 1 (and by symmetry the y gradient)."""
+    g, slot = grid_of({(0, 0, 0): (W.FLUID, 0, 0, 0), (-1, 0, 0): (W.FLUID, 0, 0, 0),
+                       (0, -1, 0): (W.FLUID, 0, 0, 0)})
+    p = np.zeros(g["flags"].size)
+    p[slot(0, 0, 0)] = 1.0
+    u, v, _ = O.gradient_update(g, p, g["u"], g["v"], g["w"])
+    assert -u[slot(0, 0, 0)] == pytest.approx(1.0, abs=1e-10)
+    assert -v[slot(0, 0, 0)] == pytest.approx(1.0, abs=1e-10)
+
+
+# ------------------------------------------------------- particle motion --
+@pytest.mark.parametrize("prk", [2, 4])
+@pytest.mark.parametrize("h,dt", [(1.0, 0.1), (0.1, 0.01)])
+def test_move_particles_kat_1d(prk, h, dt):
+    """test/old/TestAlgorithms_Move_Particles.h:159,419: cells -1, 0, 1 moving
+    at u = 1, one particle at 0: after one move it is at dt (h = 1, dt = 0.1
+    and h = 0.1, dt = 0.01)."""
+    g, _ = grid_of({(x, 0, 0): (W.FLUID, 1.0, 0, 0) for x in (-1, 0, 1)}, h)
+    s = O.Sim(h=g["h"], gravity=(0, 0, 0), particle_rk=prk, particles_per_inflow_cell=0)
+    s.upload(g["keys"], g["flags"], g["layer"], u=g["u"], v=g["v"], w=g["w"], particles=np.zeros((1, 3)))
+    s.move_particles(dt)
+    pos = s.download()["particles"]
+    assert pos[0] == pytest.approx([dt, 0.0, 0.0], abs=1e-10)
+
+
+@pytest.mark.parametrize("prk", [2, 4])
+@pytest.mark.parametrize("h,dt", [(1.0, 0.1), (0.1, 0.01)])
+def test_move_particles_kat_2d(prk, h, dt):
+    """TestAlgorithms_Move_Particles.h:305-306,566-567: the seven cells around
+    the origin moving at (1, 1): the particle at the origin ends at (dt, dt)."""
+    xy = [(0, 0), (1, 0), (0, 1), (1, 1), (-1, -1), (-1, 0), (0, -1)]
+    g, _ = grid_of({(x, y, 0): (W.FLUID, 1.0, 1.0, 0.0) for x, y in xy}, h)
+    s = O.Sim(h=g["h"], gravity=(0, 0, 0), particle_rk=prk, particles_per_inflow_cell=0)
+    s.upload(g["keys"], g["flags"], g["layer"], u=g["u"], v=g["v"], w=g["w"], particles=np.zeros((1, 3)))
+    s.move_particles(dt)
+    pos = s.download()["particles"]
+    assert pos[0] == pytest.approx([dt, dt, 0.0], abs=1e-10)
+
+
+@pytest.mark.parametrize("prk", [2, 4])
+def test_particle_rk_weights_in_a_linear_field(prk):
+    """Policy_Advance_ODE_RungeKutta (RK4) and the RK2 midpoint mode on
+    dx/dt = diag(a) x, which the staggered trilinear interpolation reproduces
+    exactly: one step multiplies each coordinate by the method's Taylor
+    polynomial of z = a dt (1 + z + z^2/2 for RK2, up to z^4/24 for RK4).
+    Wrong stage weights change the z^2 (or z^3, z^4) coefficient."""
+    h = 0.1
+    a = np.array([0.9, -0.6, 0.4])
+    nb = 3
+    g = W.random_grid(1, nb, nb, nb, fluid_p=1.0, solid_p=0.0, const_p=0.0)
+    flags = np.full(g["flags"].size, W.FLUID | W.EXISTS, np.uint8)
+    keys = g["keys"]
+    t = np.arange(512)
+    X = [(keys[:, i:i + 1] * 8 + [(t & 7), ((t >> 3) & 7), (t >> 6)][i][None, :]).reshape(-1) * h for i in range(3)]
+    # component i sits on the low i face of its cell: position X - h/2 e_i
+    comps = [a[i] * (X[i] - 0.5 * h) for i in range(3)]
+    n = flags.size
+    s = O.Sim(h=(h, h, h), gravity=(0, 0, 0), particle_rk=prk, particles_per_inflow_cell=0)
+    rng = np.random.default_rng(4)
+    pos = rng.uniform(8 * h, 16 * h, size=(40, 3))
+    s.upload(keys, flags, np.zeros(n, np.int8), u=comps[0], v=comps[1], w=comps[2], particles=pos)
+    dt = 0.3
+    s.move_particles(dt)
+    out = s.download()["particles"]
+    z = a * dt
+    if prk == 2:
+        fac = 1 + z + z ** 2 / 2
+        wrong = 1 + z + z ** 2          # full-step midpoint
+    else:
+        fac = 1 + z + z ** 2 / 2 + z ** 3 / 6 + z ** 4 / 24
+        wrong = 1 + z + z ** 2 / 2 + z ** 3 / 6 + z ** 4 / 12   # last stage weighted 1/3
+    assert np.allclose(out, pos * fac, rtol=0, atol=1e-12)
+    assert np.abs(pos * wrong - pos * fac).max() > 1e-5
+
+
+# ------------------------------------------------ RK2 / RK2-TVD stage weights --
+def _lap7(a, h):
+    """7-point Laplacian of a dense (Z, Y, X) array (np.roll; the field is
+    compactly supported far from the array's edges)."""
+    out = -6.0 * a
+    for ax in range(3):
+        out = out + np.roll(a, 1, ax) + np.roll(a, -1, ax)
+    return out / (h * h)
+
+
+def _viscous_bump(nb=4, amp=1e-6, R=6.0):
+    """A fluid box (cells 2..8nb-3 per axis) in an air shell, holding the
+    discrete curl of a compactly supported stream function psi on the y-edges:
+    u = D_z psi, w = -D_x psi, v = 0.  Its discrete divergence is 0 in exact
+    arithmetic, so every projection leaves it unchanged, and at amplitude
+    1e-6 the (quadratic) convection term is 1e-7 of the viscous one."""
+    n = 8 * nb
+    c = n / 2.0
+    i = np.arange(n + 1, dtype=np.float64)
+
+    def bump(t):
+        r = (t - c) / R
+        return np.where(np.abs(r) < 1, (1 - r * r) ** 3, 0.0)
+
+    bx, by, bz = bump(i), bump(np.arange(n) + 0.5), bump(i)
+    psi = amp * bz[:, None, None] * by[None, :, None] * bx[None, None, :]   # (k, j, i) on y-edges
+    u = (psi[1:, :, :n] - psi[:-1, :, :n])          # (n, n, n): u(i,j,k) = psi(i,j,k+1) - psi(i,j,k)
+    w = -(psi[:n, :, 1:] - psi[:n, :, :-1])         # w(i,j,k) = -(psi(i+1,j,k) - psi(i,j,k))
+    mat = np.full((n, n, n), W.AIR, np.uint8)
+    mat[2:n - 2, 2:n - 2, 2:n - 2] = W.FLUID
+    keys = W.box_keys(nb, nb, nb)
+    flags = W.dense_to_bricks(mat | W.EXISTS, nb, nb, nb).reshape(-1)
+    layer = np.where((flags & 7) == W.FLUID, 0, 1).astype(np.int8)
+    z = np.zeros_like(u)
+    return keys, flags, layer, u, z, w
+
+
+@pytest.mark.parametrize("integrator", [0, 1])
+def test_rk2_stage_weights_closed_form(integrator):
+    """Algorithms_RungeKutta_RK2.h:41-59 (midpoint: u1 = u0 + dt/2 a(u0), then
+    u0 += dt a(u1)) and Algorithms_RungeKutta_RK2_TVD.h:40-61 (u1 = u0 + dt
+    a(u0), then u0 = u0/2 + u1/2 + dt/2 a(u1)), each followed by a projection.
+    With only the viscous term (a = nu L u, L the 7-point Laplacian) on a
+    divergence-free field both are u0 + dt nu L u0 + (dt nu)^2/2 L^2 u0.  The
+    plausible mistakes -- a full first stage in the midpoint rule, a missing
+    1/2 on the TVD stage's dt a(u1), a dropped u0/2 -- change the second-order
+    coefficient, which is ~3e-3 of the field here, 1e5 times the tolerance."""
+    nb, h, nu, dt = 4, 1.0, 1.0, 0.1
+    keys, flags, layer, u0, v0, w0 = _viscous_bump(nb)
+    to_b = lambda a: W.dense_to_bricks(a, nb, nb, nb).reshape(-1)
+    to_d = lambda a: W.bricks_to_dense(a.reshape(-1, 512), nb, nb, nb)
+    s = O.Sim(h=(h, h, h), gravity=(0, 0, 0), viscosity=nu, integrator=integrator, particles_per_inflow_cell=0)
+    s.upload(keys, flags, layer, u=to_b(u0), v=to_b(v0), w=to_b(w0))
+    s.advance(dt)
+    d = s.download()
+    c = nu * dt
+    for name, f0 in (("u", u0), ("w", w0)):
+        L1 = _lap7(f0, h)
+        L2 = _lap7(L1, h)
+        exact = f0 + c * L1 + 0.5 * c * c * L2
+        got = to_d(d[name])
+        scale = np.abs(f0).max()
+        assert np.abs(got - exact).max() <= 1e-8 * scale, name
+        # the second-order term is what the weights decide
+        assert np.abs(0.5 * c * c * L2).max() > 1e-3 * scale
+    # v only picks up the projections' rounding-level pressure gradients
+    assert np.abs(to_d(d["v"])).max() <= 1e-8 * np.abs(u0).max()

@@ -73,3 +73,37 @@ def test_slots_match_oracle(tmp_path, name):
         assert rel(out[k][exists], ref[k][exists]) < 1e-8, k
     xo, _ = O.cg_csr(n, off, idx, val, b, rtol=1e-14)
     assert rel(out["x"], xo) < 1e-9
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", sorted(GRIDS))
+def test_viscous_slot_b_under_force_wrapper(tmp_path, name):
+    """Jet3's composition (test/Jet3.cpp:626,633): the viscous slot B inside
+    Algorithms_Solve_Pressure_Force projects the acceleration while the air
+    pressures come from the real speed, Speed_GetRef(-1, false)
+    (Policy_Dirichlet_Boundary.h:100-102).  The stand-in grid holds an
+    acceleration that differs from its speed; the answer is the oracle's
+    viscous projection of the acceleration with the speed as the real field."""
+    g = GRIDS[name]()
+    ex = (g["flags"] & 0x80) > 0
+    for k in ("u", "v", "w"):
+        g[k] = np.where(ex, g[k], 0.0)
+    idxs = np.arange(g["flags"].size, dtype=np.uint64)
+    acc = [np.where(ex, W.normal(17, 40 + i, idxs), 0.0) for i in range(3)]
+    nu = 0.37
+    n, off, idx, val, row = O.assemble(g)
+    b = np.zeros(n)
+    fin, fout = str(tmp_path / "in.bin"), str(tmp_path / "out.bin")
+    S.write_input(fin, g, n, off, idx, val, b, viscosity=nu, accel=acc)
+    r = subprocess.run([_binary(), fin, fout], capture_output=True, text=True, timeout=120)
+    print(r.stdout, flush=True)
+    assert r.returncode == 0 and "error" not in r.stdout, r.stdout + r.stderr
+    out = S.read_output(fout, g["keys"].shape[0], n)
+    ga = dict(g, u=acc[0], v=acc[1], w=acc[2])
+    ref = O.project_dirichlet(ga, nu, rtol=1e-14, real=(g["u"], g["v"], g["w"]))
+    wrong = O.project_dirichlet(ga, nu, rtol=1e-14)  # air pressures from the acceleration
+    assert rel(out["p"][ex], ref["p"][ex]) < 1e-8
+    for k in ("u", "v", "w"):
+        assert rel(out[k][ex], ref[k][ex]) < 1e-8, k
+    # the two fields give different air pressures, so the check tells them apart
+    assert rel(wrong["p"][ex], ref["p"][ex]) > 1e-3

@@ -190,6 +190,22 @@ int mac_max_divergence(mac_ctx* ctx, double* out);
 int mac_solve_csr(mac_ctx* ctx, int n, const int* offset, const int* indice, const double* value,
                   const double* b, double* x, double rtol);
 
+/* The slot A solve without any allocation.  No counterpart in the reference:
+ * it is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * kernels and answer as mac_solve_csr, but every array is a caller-owned
+ * DEVICE array (d_offset[n+1], d_indice/d_value[nnz], d_b and d_x[n]; the CSR
+ * is checked on the device: off[0] = 0, off[n] = nnz, non-decreasing offsets,
+ * column indices in [0, n)), and the solve's work arrays live in a fixed
+ * static device array of the library (about 10.5 MB per GPU, held until the
+ * process ends).  n <= MAC_CSR_NOALLOC_MAX_N, larger n is rejected with
+ * MAC_E_ARG.  Calls that overlap on one GPU share that array and race: issue
+ * them from one thread, or serialise them.  Runs on the context's stream and
+ * returns when x is written; *iters (may be NULL) gets the CG iterations. */
+#define MAC_CSR_NOALLOC_MAX_N 262144
+int mac_solve_csr_noalloc(mac_ctx* ctx, int n, int nnz, const int* d_offset, const int* d_indice, const double* d_value,
+                          const double* d_b, double* d_x, double rtol, int* iters);
+
 /* Kernel timing for roofline figures: when enabled, mac_project records HIP
  * events around every launch of the named kernel classes on the context's
  * stream and accumulates their durations. */

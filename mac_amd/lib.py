@@ -90,6 +90,8 @@ SIGNATURES = {
     "mac_gradient_update": (ctypes.c_int, [_P]),
     "mac_max_divergence": (ctypes.c_int, [_P, _dp]),
     "mac_solve_csr": (ctypes.c_int, [_P, ctypes.c_int, _P, _P, _P, _P, _P, ctypes.c_double]),
+    "mac_solve_csr_noalloc": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int, _P, _P, _P, _P, _P, ctypes.c_double,
+                                             ctypes.POINTER(ctypes.c_int)]),
     "mac_enable_kernel_timing": (ctypes.c_int, [_P, ctypes.c_int]),
     "mac_kernel_times": (ctypes.c_int, [_P, ctypes.c_int, ctypes.POINTER(ctypes.c_char_p), _dp,
                                          ctypes.POINTER(ctypes.c_longlong), _dp]),

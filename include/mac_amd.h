@@ -288,6 +288,8 @@ typedef struct {
      the four stages'), forces and velocity updates, projections, particle
      moves (RK4 "all": the box deletion; the stages move the particles)       */
   double ms_topology, ms_forces, ms_project, ms_particles, ms_total;
+  double true_rel_resid; /* largest recomputed ||f - L p|| / ||f|| over the step's
+                            projections                                            */
 } mac_sim_stats;
 
 void mac_sim_default_params(mac_sim_params* p);

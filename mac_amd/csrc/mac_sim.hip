@@ -785,7 +785,10 @@ int mac_sim_advance(mac_sim* s, double dt, mac_sim_stats* stats) {
   SCHK(hipGetLastError());
   int rc = sim_project(s, 1, &st);
   if (rc) return rc;
-  if (stats) stats->pcg_iters[0] = st.iters;
+  if (stats) {
+    stats->pcg_iters[0] = st.iters;
+    stats->true_rel_resid = st.true_rel_resid;
+  }
   // stage 2: forces at u1, combine into u0, project u0
   SRC(sim_forces(s, 1));
   k_rk_combine<<<s->nb, 256, 0, s->st>>>(s->nb, s->flags, s->S[0][0], s->S[0][1], s->S[0][2], s->S[1][0], s->S[1][1], s->S[1][2],
@@ -796,6 +799,7 @@ int mac_sim_advance(mac_sim* s, double dt, mac_sim_stats* stats) {
   if (stats) {
     stats->pcg_iters[1] = st.iters;
     stats->max_div = st.max_div;
+    if (st.true_rel_resid > stats->true_rel_resid) stats->true_rel_resid = st.true_rel_resid;
     stats->nfluid = st.n_unknowns;
     stats->ms_project = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   }
@@ -877,7 +881,7 @@ int sim_extrapolate_fields(mac_sim* s, double* u, double* v, double* w) {
 int sim_rk4all(mac_sim* s, double dt, mac_sim_stats* stats) {
   mac_stats st;
   int iters[2] = {0, 0};
-  double maxdiv = 0.0;
+  double maxdiv = 0.0, worst_resid = 0.0;
   long long nfluid = 0;
   // Acceleration_GetRef().SetZero(true) on the cells that exist (before the
   // first stage and after each of the first three)
@@ -907,6 +911,7 @@ int sim_rk4all(mac_sim* s, double dt, mac_sim_stats* stats) {
     ms_proj += ms_since(tp);
     iters[stage > 2] += st.iters;
     maxdiv = st.max_div;
+    if (st.true_rel_resid > worst_resid) worst_resid = st.true_rel_resid;
     nfluid = st.n_unknowns;
     SRC(sim_extrapolate_fields(s, s->A[0], s->A[1], s->A[2]));
     k_rk4_combine<<<s->nb, 256, 0, s->st>>>(s->nb, s->flags, stage, dt, s->S[0][0], s->S[0][1], s->S[0][2], s->S[1][0], s->S[1][1],
@@ -921,6 +926,7 @@ int sim_rk4all(mac_sim* s, double dt, mac_sim_stats* stats) {
     stats->pcg_iters[0] = iters[0];
     stats->pcg_iters[1] = iters[1];
     stats->max_div = maxdiv;
+    stats->true_rel_resid = worst_resid;
     stats->nfluid = nfluid;
     stats->ms_topology += ms_topo;
     stats->ms_project += ms_proj;

@@ -342,6 +342,7 @@ class SimStats(ctypes.Structure):
         ("ms_project", ctypes.c_double),
         ("ms_particles", ctypes.c_double),
         ("ms_total", ctypes.c_double),
+        ("true_rel_resid", ctypes.c_double),
     ]
 
     def as_dict(self):

@@ -1,0 +1,159 @@
+"""The jet configurations at their stated workloads, on the device through the
+C-ABI, against the CPU oracle where the oracle finishes in seconds and
+through size-independent properties where it does not.
+
+  small vertical jet   32 x 32 x 64, h = 0.01, 4 x 4 nozzle at 1 m/s down,
+                       gravity 9.81, solid floor, open sides, 8 jittered
+                       particles per inflow cell, RK2, CFL 0.5, FP64: 100
+                       steps compared with the oracle after every step.
+  lateral jet          192 x 96 x 96, 6 x 6 nozzle on the x = 0 wall at
+                       mid-height, inflow U(0.5, 2) m/s from the seed,
+                       gravity -z, RK2-TVD, CFL 0.5: 500 steps, the FP64
+                       projection against the oracle every 50 steps, the
+                       mixed one against the FP64 one.
+  breakup jet          512 x 256 x 256, pulsed 12 x 12 nozzle at 3 m/s over
+                       a 19M-cell pool, 155M particles: mixed and FP64 steps
+                       from the same start, every projection's true residual,
+                       the divergence left, cells mixed vs FP64 bit-exact and
+                       the particle bookkeeping.
+
+Bars: cells (keys, flag bytes, layers) bit-exact; velocities within 1e-8
+relative of the oracle (projections to 1e-12 on the device, 1e-14 in the
+oracle), particle positions within 1e-9; mixed against FP64 cells
+bit-exact, velocities within 1e-6 relative."""
+import numpy as np
+import pytest
+from scipy.spatial import cKDTree
+
+from mac_amd import jets as J
+from mac_amd import workloads as W
+from mac_amd.lib import MAC_FP64, MAC_MIXED, Sim
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def same_cells(dg, do, where):
+    assert np.array_equal(dg["keys"], do["keys"]), where
+    assert np.array_equal(dg["flags"], do["flags"]), where
+    ex = (dg["flags"] & W.EXISTS) > 0
+    assert np.array_equal(dg["layer"][ex], do["layer"][ex]), where
+
+
+def rel_fields(dg, do, keys=("u", "v", "w")):
+    ex = (dg["flags"] & W.EXISTS) > 0
+    num = sum(np.linalg.norm(dg[k][ex] - do[k][ex]) ** 2 for k in keys) ** 0.5
+    den = sum(np.linalg.norm(do[k][ex]) ** 2 for k in keys) ** 0.5
+    return num / max(den, 1e-300)
+
+
+def same_particles(pg, po, tol, where):
+    assert pg.shape == po.shape, where
+    if pg.shape[0]:
+        d, _ = cKDTree(po).query(pg)
+        assert d.max() <= tol, (where, d.max())
+
+
+def test_vertical_jet_32x32x64_100_steps_vs_oracle():
+    jet = J.vertical_jet(32, 32, 64, h=0.01, nozzle=4, inflow=1.0)
+    kw = dict(jet["params"], layer_max=2, integrator=0, particle_rk=2, seed=1)
+    g = Sim(**kw, proj={"rtol": 1e-12, "precision": MAC_FP64})
+    o = O.Sim(**kw)
+    for s in (g, o):
+        s.upload(jet["keys"], jet["flags"], u=jet["u"], v=jet["v"], w=jet["w"])
+    for step in range(1, 101):
+        sg, so = g.step(), o.step()
+        # dt reads the last step's projected speeds, which agree to 1e-8
+        assert sg["dt"] == pytest.approx(so["dt"], rel=1e-9), step
+        dg, do = g.download(), o.download()
+        same_cells(dg, do, step)
+        assert rel_fields(dg, do) <= 1e-8, step
+        same_particles(dg["particles"], do["particles"], 1e-9, step)
+        assert sg["true_rel_resid"] <= 1e-12 * 1.0001, step
+    print(f"[c1] 100 steps: {dg['particles'].shape[0]} particles, {dg['keys'].shape[0]} bricks", flush=True)
+    g.close()
+
+
+def test_lateral_jet_192x96x96_500_steps():
+    jet = J.lateral_jet(192, 96, 96, h=0.01, nozzle=6, seed=0)
+    kw = dict(jet["params"], layer_max=2, integrator=1, particle_rk=2, seed=0)
+    fp64 = Sim(**kw, proj={"rtol": 1e-12, "precision": MAC_FP64})
+    mixed = Sim(**kw, proj={"rtol": 1e-8, "precision": MAC_MIXED})
+    o = O.Sim(**kw)
+    for s in (fp64, mixed, o):
+        s.upload(jet["keys"], jet["flags"], u=jet["u"], v=jet["v"], w=jet["w"])
+    for step in range(1, 501):
+        sf, sm, so = fp64.step(), mixed.step(), o.step()
+        assert sf["true_rel_resid"] <= 1e-12 * 1.0001 and sm["true_rel_resid"] <= 1e-8 * 1.0001, step
+        if step % 50:
+            continue
+        df, dm, do = fp64.download(), mixed.download(), o.download()
+        same_cells(df, do, step)
+        assert rel_fields(df, do) <= 1e-8, step
+        assert rel_fields(df, do, ("p",)) <= 1e-7, step
+        same_particles(df["particles"], do["particles"], 1e-9, step)
+        same_cells(dm, df, step)
+        assert rel_fields(dm, df) <= 1e-6, step
+        print(f"[c3] step {step}: {do['particles'].shape[0]} particles, {do['keys'].shape[0]} bricks, "
+              f"fp64 vs oracle u {rel_fields(df, do):.1e}, mixed vs fp64 u {rel_fields(dm, df):.1e}, "
+              f"max|div| mixed {sm['max_div']:.1e}", flush=True)
+    fp64.close()
+    mixed.close()
+
+
+def _particle_bookkeeping(d, params, dims):
+    """After a cell-type update: every particle lies in the box, and the cells
+    holding particles are exactly the Fluid cells plus particles that sit in
+    the fixed Inflow and Solid cells (SetFluid leaves those types alone)."""
+    nx, ny, nz = dims
+    lo, hi = np.array(params["box_lo"]), np.array(params["box_hi"])
+    pts = d["particles"]
+    assert np.all(pts >= lo) and np.all(pts < hi)
+    h = params["h"][0]
+    occ = np.zeros(nx * ny * nz, bool)
+    for i0 in range(0, pts.shape[0], 1 << 24):
+        x = pts[i0:i0 + (1 << 24)] * (1.0 / h)
+        k = (np.sign(x) * np.floor(np.abs(x) + 0.5)).astype(np.int64)  # lround
+        occ[k[:, 0] + nx * (k[:, 1] + ny * k[:, 2])] = True
+    keys = d["keys"].astype(np.int64)
+    t = np.arange(512)
+    cx = (keys[:, 0:1] * 8 + (t & 7)).reshape(-1)
+    cy = (keys[:, 1:2] * 8 + ((t >> 3) & 7)).reshape(-1)
+    cz = (keys[:, 2:3] * 8 + (t >> 6)).reshape(-1)
+    f = d["flags"]
+    ex = (f & W.EXISTS) > 0
+    mat = f & 7
+    inside = ex & (cx >= 0) & (cx < nx) & (cy >= 0) & (cy < ny) & (cz >= 0) & (cz < nz)
+    lin = np.where(inside, cx + nx * (cy + ny * cz), 0)
+    fluid = inside & (mat == W.FLUID)
+    # every Fluid cell holds a particle, and every particle is in a Fluid, Inflow or Solid cell
+    assert np.all(occ[lin[fluid]])
+    marked = np.zeros(nx * ny * nz, bool)
+    marked[lin[inside & np.isin(mat, [W.FLUID, W.INFLOW, W.SOLID])]] = True
+    assert not np.any(occ & ~marked)
+
+
+def test_breakup_jet_512x256x256_full_size():
+    b = J.breakup_jet(seed=0)
+    kw = dict(b["params"], layer_max=2, integrator=0, particle_rk=2, seed=0, particles_per_inflow_cell=8)
+    final = {}
+    for name, prec in (("mixed", MAC_MIXED), ("fp64", MAC_FP64)):
+        s = Sim(**kw, proj={"rtol": 1e-8, "precision": prec})
+        s.upload(b["keys"], b["flags"], u=b["u"], v=b["v"], w=b["w"], particles=b["particles"])
+        for step in range(1, 4):
+            st = s.step()
+            print(f"[c5] {name} step {step}: {st['pcg_iters']} iterations, true resid {st['true_rel_resid']:.2e}, "
+                  f"max|div| {st['max_div']:.2e}, {st['nfluid']} fluid cells, {st['nparticles']} particles", flush=True)
+            assert st["true_rel_resid"] <= 1e-8 * 1.0001
+            # the divergence left is the solver's residual: far below the
+            # inflow's scale, 3 m/s over h = 0.01 m
+            assert st["max_div"] <= 1e-5 * 3.0 / 0.01
+        s.topology()
+        d = s.download()
+        assert d["particles"].shape[0] == s.counts()[1]
+        _particle_bookkeeping(d, kw, (512, 256, 256))
+        final[name] = {k: d[k] for k in ("keys", "flags", "layer", "u", "v", "w", "p")}
+        del d
+        s.close()
+    same_cells(final["mixed"], final["fp64"], "mixed vs fp64")
+    assert rel_fields(final["mixed"], final["fp64"]) <= 1e-6

@@ -100,6 +100,12 @@ typedef struct {
                       with the neighbour bricks frozen (nu_coarse counts launches);
                       1: point Jacobi                                             */
   double coarse_damp; /* the block step's correction is scaled by this (0, 1]     */
+  int fuse_coarse;  /* 1 (default): each restriction into a block-smoothed level also
+                      takes that level's first smoothing step (from zero, brick-local),
+                      and each prolongation takes the first post-smoothing step of the
+                      level it lands on (the prolongated face planes of the neighbour
+                      bricks formed in registers; on level 0 in the FP32 V-cycle
+                      only); 0: separate launches                                 */
 } mac_params;
 
 typedef struct {

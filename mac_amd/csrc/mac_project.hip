@@ -52,12 +52,14 @@ inline uint64_t pack_key(int x, int y, int z) {
 // average launch time can be matched against a profiler's per-kernel figure.
 enum KClass {
   K_DIV, K_PMATVEC, K_UPDATE, K_JACOBI2_0, K_JACOBI_0, K_RESID0, K_PROLONG0, K_RESTRICT1,
-  K_C_JACOBI2, K_C_JACOBI, K_C_RESID, K_C_RESTRICT, K_C_PROLONG, K_CSOLVE, K_GRAD, K_MAXDIV, K_INIT, K_FIN, K_TAIL, K_N
+  K_C_JACOBI2, K_C_JACOBI, K_C_RESID, K_C_RESTRICT, K_C_PROLONG, K_CSOLVE, K_GRAD, K_MAXDIV, K_INIT, K_FIN, K_TAIL,
+  K_PROLONG_JACOBI0, K_RESTRICT_BLOCK1, K_C_RESTRICT_BLOCK, K_C_PROLONG_BLOCK, K_N
 };
 const char* kNames[K_N] = {"divergence",    "cg_pupdate_matvec", "cg_update",       "jacobi2_l0",     "jacobi_l0",
                            "residual_l0",   "prolong_l0",        "restrict_l1",     "coarse_jacobi2", "coarse_jacobi",
                            "coarse_residual", "coarse_restrict", "coarse_prolong",  "coarse_solve",   "gradient_update",
-                           "max_div",       "resid_init",        "finalize",        "coarse_tail"};
+                           "max_div",       "resid_init",        "finalize",        "coarse_tail",    "prolong_jacobi_l0",
+                           "restrict_block_l1", "coarse_restrict_block", "coarse_prolong_block"};
 
 inline int grid_of(int nact) {
   int g = (nact + WPB - 1) / WPB;
@@ -606,6 +608,13 @@ int build_grid(mac_ctx* c, int nbricks, const int32_t* keys3, const uint8_t* fla
           int sx = (k & 1) ? ex : 0, sy = (k & 2) ? ey : 0, sz = (k & 4) ? ez : 0;
           m[2 + k] = cl.h_nbr[(size_t)P * 27 + (sx + 1) + 3 * (sy + 1) + 9 * (sz + 1)];
         }
+        const int* nb = &lv.h_nbr[(size_t)B * 27];  // face neighbours, the order of meta
+        m[10] = nb[12];
+        m[11] = nb[14];
+        m[12] = nb[10];
+        m[13] = nb[16];
+        m[14] = nb[4];
+        m[15] = nb[22];
       }
       up.add(&lv.pmeta, pm);
       std::vector<int> rm((size_t)std::max(cl.nact, 1) * RSTRIDE, -1);
@@ -796,6 +805,10 @@ int vcycle(mac_ctx* c, TZ** zout) {
   const bool tail = c->prm.coarse_tail && bot == nl - 1 && nl >= 3 && c->L[nl - 1].nb == 1;
   const int tlev = tail ? nl - 2 : bot;
   std::vector<TZ*> zfin(nl, nullptr);
+  std::vector<char> first_done(nl, 0);
+  // the block smoother runs on the coarse levels above the last two (those
+  // smooth point-wise, fused or not)
+  auto is_block = [&](int l) { return l > 0 && l < nl - 2 && c->prm.coarse_inner > 1; };
   // down
   for (int l = 0; l < tlev; ++l) {
     Level& lv = c->L[l];
@@ -807,12 +820,11 @@ int vcycle(mac_ctx* c, TZ** zout) {
     TZ* res = (TZ*)lv.res;
     double nd = (double)lvl_dom(c, l);
     double sr = (l == 0) ? 8.0 : sz;
-    // the block smoother runs on the coarse levels above the last two (those
-    // smooth point-wise, fused or not)
-    const bool block = l > 0 && l < nl - 2 && c->prm.coarse_inner > 1;
+    const bool block = is_block(l);
     if (block) {
       const bool deep = l >= BLOCK_DEEP && nu > 1;
-      for (int k = 0; k < (deep ? 1 : nu); ++k) {
+      // the first step from zero ran inside the restriction into this level
+      for (int k = first_done[l] ? 1 : 0; k < (deep ? 1 : nu); ++k) {
         TimedLaunch tl(c, K_C_JACOBI, nd * ((k ? 3 : 2) * sz + 1));  // r, zin, zout, code
         k_jacobi_block<TZ, TZ><<<grid_of(lv.nact), KTHREADS, 0, c->st>>>(lv.nact, lv.meta, lv.code, (const TZ*)lv.r, k ? z : nullptr,
                                                                           k ? zt : z, ih[0], ih[1], ih[2],
@@ -846,8 +858,17 @@ int vcycle(mac_ctx* c, TZ** zout) {
     }
     Level& cl = c->L[l + 1];
     {
-      TimedLaunch tl(c, l == 0 ? K_RESTRICT1 : K_C_RESTRICT, nd * sz + (double)cl.ndom * sz);
-      k_restrict_sep<TZ><<<cl.nact, KTHREADS, 0, c->st>>>(cl.rmeta, cl.code, res, (TZ*)cl.r, c->s);
+      // into a block level visited below: the restriction also takes that
+      // level's first smoothing step from zero (brick-local)
+      const bool fuse = c->prm.fuse_coarse && l + 1 < tlev && is_block(l + 1);
+      const bool cdeep = l + 1 >= BLOCK_DEEP && nu_of(c->prm, l + 1) > 1;
+      const TZ ihc[3] = {(TZ)cl.ih2[0], (TZ)cl.ih2[1], (TZ)cl.ih2[2]};
+      const double cd = (double)cl.ndom;
+      TimedLaunch tl(c, fuse ? (l == 0 ? K_RESTRICT_BLOCK1 : K_C_RESTRICT_BLOCK) : (l == 0 ? K_RESTRICT1 : K_C_RESTRICT),
+                     nd * sz + cd * sz + (fuse ? cd * (sz + 1) : 0.0));
+      k_restrict_sep<TZ><<<cl.nact, KTHREADS, 0, c->st>>>(cl.rmeta, cl.code, res, (TZ*)cl.r, c->s, fuse ? (TZ*)cl.z : nullptr,
+                                                          ihc[0], ihc[1], ihc[2], cdeep ? c->blockw_deep : c->blockw);
+      first_done[l + 1] = fuse;
     }
     zfin[l] = z;  // pre-smoothed iterate, finished on the way up
   }
@@ -905,17 +926,36 @@ int vcycle(mac_ctx* c, TZ** zout) {
     TZ* z = zfin[l];
     TZ* zt = (z == (TZ*)lv.z) ? (TZ*)lv.zt : (TZ*)lv.z;
     double nd = (double)lvl_dom(c, l);
-    {
-      TimedLaunch tl(c, l == 0 ? K_PROLONG0 : K_C_PROLONG, nd * (2 * sz + 1) + (double)cl.ndom * sz);
-      k_prolong_col<TZ><<<grid_of(lv.nact), KTHREADS, 0, c->st>>>(lv.nact, lv.pmeta, lv.code, zfin[l + 1], z, c->s);
-    }
-    const bool blk = l > 0 && l < nl - 2 && c->prm.coarse_inner > 1;
+    const bool blk = is_block(l);
     const bool deep = blk && l >= BLOCK_DEEP && nu_of(c->prm, l) > 1;
     const int nu = deep ? 1 : nu_of(c->prm, l);
     const double* opost = l ? c->omc_post : c->om_post;
-    for (int k = 0; k < nu; ++k) {
+    const TZ ih[3] = {(TZ)lv.ih2[0], (TZ)lv.ih2[1], (TZ)lv.ih2[2]};
+    int k0 = 0;
+    // level 0 only in FP32: the FP64 fused sweep needs 200 registers and runs
+    // slower than the two launches
+    if (c->prm.fuse_coarse && (blk || (l == 0 && sizeof(TZ) == 4))) {
+      // prolongation fused with the first post-smoothing step: zt = S(z + P zc)
       if (blk) {
-        const TZ ih[3] = {(TZ)lv.ih2[0], (TZ)lv.ih2[1], (TZ)lv.ih2[2]};
+        TimedLaunch tl(c, K_C_PROLONG_BLOCK, nd * (3 * sz + 2) + (double)cl.ndom * sz);
+        k_prolong_block<TZ><<<grid_of(lv.nact), KTHREADS, 0, c->st>>>(lv.nact, lv.pmeta, lv.code, (const TZ*)lv.r, zfin[l + 1], z,
+                                                                       zt, ih[0], ih[1], ih[2],
+                                                                       deep ? c->blockw_deep : c->blockw, c->s);
+      } else {
+        const int want_dot = nu == 1 ? 1 : 0;
+        TimedLaunch tl(c, K_PROLONG_JACOBI0, nd * (8.0 + 2 * sz + 2) + (double)cl.ndom * sz);
+        k_prolong_jacobi<TZ><<<grid_of(lv.nact), KTHREADS, 0, c->st>>>(lv.nact, lv.pmeta, lv.code, c->r, zfin[l + 1], z, zt, ih[0],
+                                                                        ih[1], ih[2], (TZ)opost[0], want_dot, c->partials, c->s);
+        if (want_dot) finalize(c, grid_of(lv.nact), OP_RZ);
+      }
+      std::swap(z, zt);
+      k0 = 1;
+    } else {
+      TimedLaunch tl(c, l == 0 ? K_PROLONG0 : K_C_PROLONG, nd * (2 * sz + 1) + (double)cl.ndom * sz);
+      k_prolong_col<TZ><<<grid_of(lv.nact), KTHREADS, 0, c->st>>>(lv.nact, lv.pmeta, lv.code, zfin[l + 1], z, c->s);
+    }
+    for (int k = k0; k < nu; ++k) {
+      if (blk) {
         TimedLaunch tl(c, K_C_JACOBI, nd * (3 * sz + 1));
         k_jacobi_block<TZ, TZ><<<grid_of(lv.nact), KTHREADS, 0, c->st>>>(lv.nact, lv.meta, lv.code, (const TZ*)lv.r, z, zt, ih[0],
                                                                           ih[1], ih[2], deep ? c->blockw_deep : c->blockw, c->s);
@@ -1135,6 +1175,7 @@ void mac_default_params(mac_params* p) {
   p->viscosity = 0.0;
   p->coarse_inner = 4;
   p->coarse_damp = 0.8;
+  p->fuse_coarse = 1;
 }
 
 int mac_create(mac_ctx** out, const mac_params* prm) {

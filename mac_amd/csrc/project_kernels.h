@@ -581,6 +581,53 @@ struct BlockW {
   double om64[8];
   double damp;  // zout = zin + damp (block correction)
 };
+// The m sweeps of a block step on one brick held by a wave (lane = x + 8y,
+// the brick's z column in col): in-brick neighbours by cross-lane shuffles,
+// the frozen face values gx / gy (x / y edge lanes), below / above (z faces).
+// idg: the inverse diagonal (FP32) per cell, 0 off the domain.
+template <typename TZ>
+__device__ __forceinline__ void block_sweeps(int lane, const unsigned (&cd)[8], const TZ (&rv)[8], const TZ (&idg)[8],
+                                             TZ (&col)[8], const TZ (&gx)[8], const TZ (&gy)[8], TZ below, TZ above, TZ ihx2,
+                                             TZ ihy2, TZ ihz2, const BlockW& w) {
+  const int x = lane & 7, y = lane >> 3;
+  for (int k = 0; k < w.m; ++k) {
+    const TZ om = sizeof(TZ) == 4 ? (TZ)w.om[k] : (TZ)w.om64[k];
+    TZ nw[8];
+#pragma unroll
+    for (int z = 0; z < 8; ++z) {
+      const TZ v = col[z];
+      TZ xm = __shfl_up(v, 1), xp = __shfl_down(v, 1), ym = __shfl_up(v, 8), yp = __shfl_down(v, 8);
+      if (x == 0) xm = gx[z];
+      if (x == 7) xp = gx[z];
+      if (y == 0) ym = gy[z];
+      if (y == 7) yp = gy[z];
+      const TZ zm = (z > 0) ? col[z > 0 ? z - 1 : 0] : below;
+      const TZ zp = (z < 7) ? col[z < 7 ? z + 1 : 7] : above;
+      TZ out = TZ(0);
+      if (cd[z] & CODE_DOM) {
+        TZ dg;
+        const TZ lz = lap_cell<TZ>(cd[z], v, xm, xp, ym, yp, zm, zp, ihx2, ihy2, ihz2, dg);
+        if constexpr (sizeof(TZ) == 4)
+          out = idg[z] != TZ(0) ? v + om * (rv[z] - lz) * idg[z] : TZ(0);
+        else
+          out = dg > TZ(0) ? v + om * (rv[z] - lz) / dg : TZ(0);
+      }
+      nw[z] = out;
+    }
+#pragma unroll
+    for (int z = 0; z < 8; ++z) col[z] = nw[z];
+  }
+}
+
+template <typename TZ>
+__device__ __forceinline__ void inv_diag_col(const unsigned (&cd)[8], TZ ihx2, TZ ihy2, TZ ihz2, TZ (&idg)[8]) {
+#pragma unroll
+  for (int z = 0; z < 8; ++z) {
+    const TZ dg = diag_of<TZ>(cd[z], ihx2, ihy2, ihz2);
+    idg[z] = ((cd[z] & CODE_DOM) && dg > TZ(0)) ? TZ(1) / dg : TZ(0);
+  }
+}
+
 template <typename TR, typename TZ>
 __global__ void __launch_bounds__(KTHREADS) k_jacobi_block(int nact, const int* __restrict__ meta,
                                                            const uint8_t* __restrict__ code, const TR* __restrict__ r,
@@ -616,38 +663,8 @@ __global__ void __launch_bounds__(KTHREADS) k_jacobi_block(int nact, const int* 
     for (int z = 0; z < 8; ++z) col0[z] = col[z];
     // FP32: the inverse diagonal once per cell instead of a division per sweep
     TZ idg[8];
-#pragma unroll
-    for (int z = 0; z < 8; ++z) {
-      const TZ dg = diag_of<TZ>(cd[z], ihx2, ihy2, ihz2);
-      idg[z] = ((cd[z] & CODE_DOM) && dg > TZ(0)) ? TZ(1) / dg : TZ(0);
-    }
-    for (int k = 0; k < w.m; ++k) {
-      const TZ om = sizeof(TZ) == 4 ? (TZ)w.om[k] : (TZ)w.om64[k];
-      TZ nw[8];
-#pragma unroll
-      for (int z = 0; z < 8; ++z) {
-        const TZ v = col[z];
-        TZ xm = __shfl_up(v, 1), xp = __shfl_down(v, 1), ym = __shfl_up(v, 8), yp = __shfl_down(v, 8);
-        if (x == 0) xm = gx[z];
-        if (x == 7) xp = gx[z];
-        if (y == 0) ym = gy[z];
-        if (y == 7) yp = gy[z];
-        const TZ zm = (z > 0) ? col[z > 0 ? z - 1 : 0] : h.below;
-        const TZ zp = (z < 7) ? col[z < 7 ? z + 1 : 7] : h.above;
-        TZ out = TZ(0);
-        if (cd[z] & CODE_DOM) {
-          TZ dg;
-          const TZ lz = lap_cell<TZ>(cd[z], v, xm, xp, ym, yp, zm, zp, ihx2, ihy2, ihz2, dg);
-          if constexpr (sizeof(TZ) == 4)
-            out = idg[z] != TZ(0) ? v + om * (rv[z] - lz) * idg[z] : TZ(0);
-          else
-            out = dg > TZ(0) ? v + om * (rv[z] - lz) / dg : TZ(0);
-        }
-        nw[z] = out;
-      }
-#pragma unroll
-      for (int z = 0; z < 8; ++z) col[z] = nw[z];
-    }
+    inv_diag_col<TZ>(cd, ihx2, ihy2, ihz2, idg);
+    block_sweeps<TZ>(lane, cd, rv, idg, col, gx, gy, h.below, h.above, ihx2, ihy2, ihz2, w);
     const TZ dmp = (TZ)w.damp;
 #pragma unroll
     for (int z = 0; z < 8; ++z) zout[base + 64 * z] = col0[z] + dmp * (col[z] - col0[z]);
@@ -705,11 +722,38 @@ __global__ void __launch_bounds__(KTHREADS) k_jacobi2(int nact, const int* __res
 // 64 fine bricks covering fine cells -1..16 relative to the coarse brick's
 // first child, fb[bx + 4 by + 16 bz] with b* = 0..3 for fine brick offsets
 // -1..2 (-1 = none).
+// zsm != nullptr fuses the coarse level's first block-smoothing step from
+// zero (k_jacobi_block with zin == nullptr: the neighbour bricks are frozen
+// at 0, so the step needs nothing outside the brick): wave 0 takes the
+// restricted brick from LDS and writes zsm.
 #define RSTRIDE 68
 template <typename T>
+__device__ __forceinline__ void smooth_from_zero(int C, const uint8_t* __restrict__ codec, const T* rcl, T* __restrict__ zsm,
+                                                 T ihx2, T ihy2, T ihz2, const BlockW& w) {
+  const int lane = threadIdx.x;
+  const size_t base = (size_t)C * MAC_BC + lane;
+  unsigned cd[8];
+  T rv[8], col[8], gz[8], idg[8];
+#pragma unroll
+  for (int z = 0; z < 8; ++z) {
+    cd[z] = codec[base + 64 * z];
+    rv[z] = rcl[lane + 64 * z];
+    col[z] = T(0);
+    gz[z] = T(0);
+  }
+  inv_diag_col<T>(cd, ihx2, ihy2, ihz2, idg);
+  block_sweeps<T>(lane, cd, rv, idg, col, gz, gz, T(0), T(0), ihx2, ihy2, ihz2, w);
+  const T dmp = (T)w.damp;
+#pragma unroll
+  for (int z = 0; z < 8; ++z) zsm[base + 64 * z] = T(0) + dmp * (col[z] - T(0));
+}
+
+template <typename T>
 __global__ void __launch_bounds__(KTHREADS) k_restrict_sep(const int* __restrict__ rmeta, const uint8_t* __restrict__ codec,
-                                                           const T* __restrict__ resf, T* __restrict__ rc, const Scal* s) {
+                                                           const T* __restrict__ resf, T* __restrict__ rc, const Scal* s,
+                                                           T* __restrict__ zsm, T ihx2, T ihy2, T ihz2, BlockW w) {
   if (converged(rmeta, RSTRIDE, (int)gridDim.x, (int)blockIdx.x, s)) return;
+  __shared__ T rcl[MAC_BC];
   if constexpr (sizeof(T) == 4) {
     // FP32: both halves at once (the 18^3 fine region fits in LDS), so the
     // global loads issue together and the three passes need three barriers
@@ -761,6 +805,11 @@ __global__ void __launch_bounds__(KTHREADS) k_restrict_sep(const int* __restrict
         acc = (T(0.25) * col[0] + T(0.75) * col[64] + T(0.75) * col[128] + T(0.25) * col[192]) * T(0.125);
       }
       rc[c] = acc;
+      rcl[t] = acc;
+    }
+    if (zsm) {
+      __syncthreads();
+      if (threadIdx.x < 64) smooth_from_zero<T>(C, codec, rcl, zsm, ihx2, ihy2, ihz2, w);
     }
     return;
   }
@@ -814,7 +863,12 @@ __global__ void __launch_bounds__(KTHREADS) k_restrict_sep(const int* __restrict
         acc = (T(0.25) * col[0] + T(0.75) * col[64] + T(0.75) * col[128] + T(0.25) * col[192]) * T(0.125);
       }
       rc[c] = acc;
+      rcl[X + 8 * Y + 64 * (4 * half + zz)] = acc;
     }
+  }
+  if (zsm) {
+    __syncthreads();
+    if (threadIdx.x < 64) smooth_from_zero<T>(C, codec, rcl, zsm, ihx2, ihy2, ihz2, w);
   }
 }
 
@@ -827,7 +881,7 @@ __global__ void __launch_bounds__(KTHREADS) k_restrict_sep(const int* __restrict
 // pmeta per active fine brick (PSTRIDE ints): brick id, octant in its
 // parent, then the 8 coarse bricks bb[sx + 2 sy + 4 sz] (sx = 1: one step
 // from the parent toward the octant's side in x, etc.; -1 = none).
-#define PSTRIDE 12
+#define PSTRIDE 16
 template <typename T>
 __global__ void __launch_bounds__(KTHREADS) k_prolong_col(int nact, const int* __restrict__ pmeta,
                                                           const uint8_t* __restrict__ codef, const T* __restrict__ zc,
@@ -873,6 +927,184 @@ __global__ void __launch_bounds__(KTHREADS) k_prolong_col(int nact, const int* _
       zf[c] += T(0.75) * (T(0.75) * a00 + T(0.25) * a01) + T(0.25) * (T(0.75) * a10 + T(0.25) * a11);
     }
   }
+}
+
+// The prolongation of one fine brick, held by a wave: the four coarse
+// columns a lane's cells read (as in k_prolong_col) and the trilinear value
+// at the lane's own cells and at the cells just across the brick's faces.
+// A face cell of a neighbour fine brick lies between the same coarse cells
+// as the edge lane's own cell, with the two weights of the crossed direction
+// swapped, so the edge lanes form the prolongated halo themselves, with the
+// same operations in the same order as the neighbour forms its own cell.
+// pmeta rows (PSTRIDE ints): brick id, octant, the 8 coarse bricks of
+// k_prolong_col, then the 6 face-neighbour fine bricks (-x,+x,-y,+y,-z,+z).
+template <typename T>
+struct ProlongCols {
+  T c00[6], c10[6], c01[6], c11[6];
+  __device__ __forceinline__ void gather(const int* M, int lane, const T* __restrict__ zc) {
+    const int x = lane & 7, y = lane >> 3;
+    const int o = M[1];
+    const int ox = o & 1, oy = (o >> 1) & 1, oz = o >> 2;
+    const int bb0 = M[2], bb1 = M[3], bb2 = M[4], bb3 = M[5], bb4 = M[6], bb5 = M[7], bb6 = M[8], bb7 = M[9];
+    const int bx = 4 * ox, by = 4 * oy, bz = 4 * oz;
+    const int X0 = bx + (x >> 1), Y0 = by + (y >> 1);
+    const int X1 = X0 + ((x & 1) ? 1 : -1), Y1 = Y0 + ((y & 1) ? 1 : -1);
+    const bool sx = (X1 < 0 || X1 > 7), sy = (Y1 < 0 || Y1 > 7);
+    const int k00 = bb0, k10 = sx ? bb1 : bb0, k01 = sy ? bb2 : bb0, k11 = sx ? (sy ? bb3 : bb1) : (sy ? bb2 : bb0);
+    const int z00 = bb4, z10 = sx ? bb5 : bb4, z01 = sy ? bb6 : bb4, z11 = sx ? (sy ? bb7 : bb5) : (sy ? bb6 : bb4);
+    auto g = [&](int brk, int X, int Y, int Z) -> T {
+      return brk >= 0 ? zc[(size_t)brk * MAC_BC + (X & 7) + 8 * (Y & 7) + 64 * (Z & 7)] : T(0);
+    };
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+      const int Z = bz - 1 + k;
+      const bool out = (Z < 0 || Z > 7);
+      c00[k] = g(out ? z00 : k00, X0, Y0, Z);
+      c10[k] = g(out ? z10 : k10, X1, Y0, Z);
+      c01[k] = g(out ? z01 : k01, X0, Y1, Z);
+      c11[k] = g(out ? z11 : k11, X1, Y1, Z);
+    }
+  }
+  // z = -1 .. 8: coarse levels k0 (weight 3/4) and k1 (1/4)
+  __device__ __forceinline__ static void zlev(int z, int& k0, int& k1) {
+    if (z < 0) {
+      k0 = 0;
+      k1 = 1;
+    } else if (z > 7) {
+      k0 = 5;
+      k1 = 4;
+    } else {
+      k0 = (z >> 1) + 1;
+      k1 = (z & 1) ? k0 + 1 : k0 - 1;
+    }
+  }
+  // face: 0 own cell, 1 across an x face, 2 across a y face
+  __device__ __forceinline__ T at(int z, int face) const {
+    int k0, k1;
+    zlev(z, k0, k1);
+    const T a00 = T(0.75) * c00[k0] + T(0.25) * c00[k1];
+    const T a10 = T(0.75) * c10[k0] + T(0.25) * c10[k1];
+    const T a01 = T(0.75) * c01[k0] + T(0.25) * c01[k1];
+    const T a11 = T(0.75) * c11[k0] + T(0.25) * c11[k1];
+    if (face == 1) return T(0.75) * (T(0.75) * a10 + T(0.25) * a11) + T(0.25) * (T(0.75) * a00 + T(0.25) * a01);
+    if (face == 2) return T(0.75) * (T(0.75) * a01 + T(0.25) * a00) + T(0.25) * (T(0.75) * a11 + T(0.25) * a10);
+    return T(0.75) * (T(0.75) * a00 + T(0.25) * a01) + T(0.25) * (T(0.75) * a10 + T(0.25) * a11);
+  }
+};
+
+// The prolongated iterate zin = zfin + P zc of a fine brick: the wave's own
+// column and the frozen face values (edge lanes), with the face planes of
+// zfin and of the codes loaded lane-parallel from the neighbour bricks.
+template <typename T>
+struct ProlongedBrick {
+  unsigned cd[8];
+  T col[8], gx[8], gy[8], below, above;
+  __device__ __forceinline__ void load(const int* M, int lane, const uint8_t* __restrict__ code, const T* __restrict__ zfin,
+                                       const T* __restrict__ zc) {
+    const int x = lane & 7, y = lane >> 3;
+    const int B = M[0];
+    const int* F = M + 10;  // face neighbours, the order of meta
+    const size_t base = (size_t)B * MAC_BC + lane;
+    ProlongCols<T> pc;
+    pc.gather(M, lane, zc);
+#pragma unroll
+    for (int z = 0; z < 8; ++z) {
+      cd[z] = code[base + 64 * z];
+      const T v = zfin[base + 64 * z];
+      col[z] = (cd[z] & CODE_DOM) ? v + pc.at(z, 0) : v;
+    }
+    const Halo<T> hz = load_halo<T>(F - 1, lane, [&](int nb, int cell) { return zfin[(size_t)nb * MAC_BC + cell]; });
+    const Halo<unsigned> hc =
+        load_halo<unsigned>(F - 1, lane, [&](int nb, int cell) { return (unsigned)code[(size_t)nb * MAC_BC + cell]; });
+#pragma unroll
+    for (int z = 0; z < 8; ++z) {
+      const T vxm = __shfl(hz.xm, y + 8 * z), vxp = __shfl(hz.xp, y + 8 * z);
+      const T vym = __shfl(hz.ym, x + 8 * z), vyp = __shfl(hz.yp, x + 8 * z);
+      const unsigned cxm = __shfl(hc.xm, y + 8 * z), cxp = __shfl(hc.xp, y + 8 * z);
+      const unsigned cym = __shfl(hc.ym, x + 8 * z), cyp = __shfl(hc.yp, x + 8 * z);
+      const T vx = x == 0 ? vxm : vxp;
+      const unsigned cx = x == 0 ? cxm : cxp;
+      const T vy = y == 0 ? vym : vyp;
+      const unsigned cy = y == 0 ? cym : cyp;
+      gx[z] = (cx & CODE_DOM) ? vx + pc.at(z, 1) : vx;
+      gy[z] = (cy & CODE_DOM) ? vy + pc.at(z, 2) : vy;
+    }
+    below = (hc.below & CODE_DOM) ? hz.below + pc.at(-1, 0) : hz.below;
+    above = (hc.above & CODE_DOM) ? hz.above + pc.at(8, 0) : hz.above;
+  }
+};
+
+// Prolongation fused with the first post-smoothing block step on a coarse
+// level: zout = zin + damp (block correction of zin), zin = zfin + P zc.
+// Equal to k_prolong_col followed by k_jacobi_block(zin), without writing
+// and re-reading zin.
+template <typename T>
+__global__ void __launch_bounds__(KTHREADS) k_prolong_block(int nact, const int* __restrict__ pmeta,
+                                                            const uint8_t* __restrict__ code, const T* __restrict__ r,
+                                                            const T* __restrict__ zc, const T* __restrict__ zfin,
+                                                            T* __restrict__ zout, T ihx2, T ihy2, T ihz2, BlockW w, const Scal* s) {
+  if (converged(pmeta, PSTRIDE, nact, blockIdx.x * WPB + wave_in_block(), s)) return;
+  const int lane = lane_id();
+  for (int i_ = blockIdx.x * WPB + wave_in_block(); i_ < nact; i_ += gridDim.x * WPB) {
+    const int* M = pmeta + (size_t)PSTRIDE * __builtin_amdgcn_readfirstlane(i_);
+    const size_t base = (size_t)M[0] * MAC_BC + lane;
+    ProlongedBrick<T> pb;
+    pb.load(M, lane, code, zfin, zc);
+    T rv[8], col0[8], idg[8];
+#pragma unroll
+    for (int z = 0; z < 8; ++z) {
+      rv[z] = r[base + 64 * z];
+      col0[z] = pb.col[z];
+    }
+    inv_diag_col<T>(pb.cd, ihx2, ihy2, ihz2, idg);
+    block_sweeps<T>(lane, pb.cd, rv, idg, pb.col, pb.gx, pb.gy, pb.below, pb.above, ihx2, ihy2, ihz2, w);
+    const T dmp = (T)w.damp;
+#pragma unroll
+    for (int z = 0; z < 8; ++z) zout[base + 64 * z] = col0[z] + dmp * (pb.col[z] - col0[z]);
+  }
+}
+
+// Prolongation into level 0 fused with the first post-smoothing Jacobi sweep:
+// zout = zin + omega D^-1 (r - L zin), zin = zfin + P zc (k_prolong_col then
+// k_jacobi, without writing and re-reading zin).  want_dot: partial r . zout.
+template <typename TZ>
+__global__ void __launch_bounds__(KTHREADS) k_prolong_jacobi(int nact, const int* __restrict__ pmeta,
+                                                             const uint8_t* __restrict__ code, const double* __restrict__ r,
+                                                             const TZ* __restrict__ zc, const TZ* __restrict__ zfin,
+                                                             TZ* __restrict__ zout, TZ ihx2, TZ ihy2, TZ ihz2, TZ omega,
+                                                             int want_dot, double* __restrict__ partials, const Scal* s) {
+  if (converged(pmeta, PSTRIDE, nact, blockIdx.x * WPB + wave_in_block(), s)) return;
+  __shared__ double sh[16];
+  const int lane = lane_id();
+  const int x = lane & 7, y = lane >> 3;
+  double acc = 0.0;
+  for (int i_ = blockIdx.x * WPB + wave_in_block(); i_ < nact; i_ += gridDim.x * WPB) {
+    const int* M = pmeta + (size_t)PSTRIDE * __builtin_amdgcn_readfirstlane(i_);
+    const size_t base = (size_t)M[0] * MAC_BC + lane;
+    ProlongedBrick<TZ> pb;
+    pb.load(M, lane, code, zfin, zc);
+#pragma unroll
+    for (int z = 0; z < 8; ++z) {
+      const TZ v = pb.col[z];
+      TZ xm = __shfl_up(v, 1), xp = __shfl_down(v, 1), ym = __shfl_up(v, 8), yp = __shfl_down(v, 8);
+      if (x == 0) xm = pb.gx[z];
+      if (x == 7) xp = pb.gx[z];
+      if (y == 0) ym = pb.gy[z];
+      if (y == 7) yp = pb.gy[z];
+      const TZ zm = (z > 0) ? pb.col[z > 0 ? z - 1 : 0] : pb.below;
+      const TZ zp = (z < 7) ? pb.col[z < 7 ? z + 1 : 7] : pb.above;
+      TZ out = TZ(0);
+      if (pb.cd[z] & CODE_DOM) {
+        TZ dg;
+        const TZ lz = lap_cell<TZ>(pb.cd[z], v, xm, xp, ym, yp, zm, zp, ihx2, ihy2, ihz2, dg);
+        const double rv = r[base + 64 * z];
+        out = dg > TZ(0) ? v + div_diag<TZ>(omega * ((TZ)rv - lz), dg) : TZ(0);
+        if (want_dot) acc += rv * (double)out;
+      }
+      zout[base + 64 * z] = out;
+    }
+  }
+  if (want_dot) write_partial(acc, partials, sh, false);
 }
 
 // res = r - L z.

@@ -42,6 +42,7 @@ class Params(ctypes.Structure):
         ("viscosity", ctypes.c_double),
         ("coarse_inner", ctypes.c_int),
         ("coarse_damp", ctypes.c_double),
+        ("fuse_coarse", ctypes.c_int),
     ]
 
 

@@ -220,6 +220,11 @@ int mac_enable_kernel_timing(mac_ctx* ctx, int on);
 int mac_kernel_times(mac_ctx* ctx, int cap, const char** names, double* total_ms, long long* launches,
                      double* bytes);
 int mac_reset_kernel_times(mac_ctx* ctx);
+/* Mean duration, in microseconds, of an event pair around an empty launch on
+ * the context's stream (n launches): the part of each event-timed launch
+ * above that is not the kernel's own run time.  No counterpart in the
+ * reference; a measurement aid for the figures above. */
+int mac_timing_overhead(mac_ctx* ctx, int n, double* us);
 
 /* Synchronise the context's stream. */
 int mac_sync(mac_ctx* ctx);

@@ -254,6 +254,9 @@ int flush_uploads(mac_ctx* c, Uploads& u) {
 }
 
 // ---- kernel launch bookkeeping (optional event timing) ----
+// An empty launch: what an event pair measures around a kernel that does nothing.
+__global__ void k_empty() {}
+
 struct TimedLaunch {
   mac_ctx* c;
   int cls;
@@ -1551,6 +1554,28 @@ int mac_kernel_times(mac_ctx* c, int cap, const char** names, double* total_ms, 
     if (bytes) bytes[i] = c->kt_bytes[i];
   }
   return K_N;
+}
+
+int mac_timing_overhead(mac_ctx* c, int n, double* us) {
+  if (!c || n <= 0 || !us) return fail(MAC_E_ARG, "bad timing-overhead arguments");
+  std::vector<hipEvent_t> ev(2 * (size_t)n);
+  for (auto& e : ev) HIPCHK(hipEventCreate(&e));
+  for (int i = 0; i < n; ++i) {
+    HIPCHK(hipEventRecord(ev[2 * i], c->st));
+    k_empty<<<1, 64, 0, c->st>>>();
+    HIPCHK(hipEventRecord(ev[2 * i + 1], c->st));
+  }
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(c->st));
+  double tot = 0.0;
+  for (int i = 0; i < n; ++i) {
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, ev[2 * i], ev[2 * i + 1]));
+    tot += ms;
+  }
+  for (auto& e : ev) hipEventDestroy(e);
+  *us = tot * 1e3 / n;
+  return 0;
 }
 
 int mac_reset_kernel_times(mac_ctx* c) {

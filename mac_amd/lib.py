@@ -97,6 +97,7 @@ SIGNATURES = {
     "mac_kernel_times": (ctypes.c_int, [_P, ctypes.c_int, ctypes.POINTER(ctypes.c_char_p), _dp,
                                          ctypes.POINTER(ctypes.c_longlong), _dp]),
     "mac_reset_kernel_times": (ctypes.c_int, [_P]),
+    "mac_timing_overhead": (ctypes.c_int, [_P, ctypes.c_int, _dp]),
     "mac_sync": (ctypes.c_int, [_P]),
 }
 
@@ -257,6 +258,12 @@ class Projection:
 
     def reset_timing(self):
         _check(self.lib.mac_reset_kernel_times(self.ctx))
+
+    def timing_overhead_us(self, n=200):
+        """Mean event-pair duration around an empty launch (microseconds)."""
+        us = ctypes.c_double()
+        _check(self.lib.mac_timing_overhead(self.ctx, int(n), ctypes.byref(us)))
+        return us.value
 
     def kernel_times(self):
         cap = 32

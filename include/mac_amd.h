@@ -312,6 +312,14 @@ int mac_sim_upload(mac_sim* sim, int nbricks, const int32_t* keys3, const uint8_
                    const double* u, const double* v, const double* w, int nparticles, const double* pxyz);
 /* Bricks and particles currently held. */
 int mac_sim_counts(mac_sim* sim, int* nbricks, int* nparticles);
+/* Steps taken (the counter the seeded inflow jitter and the inflow pulsation
+ * read: the fill of step n draws from (seed, n), the pulsation of step n is
+ * sin(2 pi n / period + phi)).  A run resumed from a checkpoint sets it to the
+ * checkpoint's step so that it continues the same sequence; mac_sim_upload
+ * leaves it unchanged.  No counterpart in the reference, whose jitter is
+ * std::rand's global state. */
+int mac_sim_get_step(mac_sim* sim, unsigned long long* step);
+int mac_sim_set_step(mac_sim* sim, unsigned long long step);
 /* Download the state; bricks come in ascending (z, y, x) chunk-key order.  Any
  * pointer may be NULL.  a* = acceleration slot, p = pressure. */
 int mac_sim_download(mac_sim* sim, int32_t* keys3, uint8_t* flags, int8_t* layer, double* u, double* v, double* w,

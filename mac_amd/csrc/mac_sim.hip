@@ -705,6 +705,18 @@ int mac_sim_counts(mac_sim* s, int* nbricks, int* nparticles) {
   return 0;
 }
 
+int mac_sim_get_step(mac_sim* s, unsigned long long* step) {
+  if (!s || !step) return mac_internal_set_error(MAC_E_ARG, "null argument");
+  *step = s->tick;
+  return 0;
+}
+
+int mac_sim_set_step(mac_sim* s, unsigned long long step) {
+  if (!s) return mac_internal_set_error(MAC_E_ARG, "null sim");
+  s->tick = step;
+  return 0;
+}
+
 int mac_sim_download(mac_sim* s, int32_t* keys3, uint8_t* flags, int8_t* layer, double* u, double* v, double* w, double* ax,
                      double* ay, double* az, double* p, double* pxyz) {
   if (!s) return mac_internal_set_error(MAC_E_ARG, "null sim");

@@ -365,6 +365,8 @@ SIGNATURES.update({
     "mac_sim_destroy": (None, [_P]),
     "mac_sim_upload": (ctypes.c_int, [_P, ctypes.c_int, _P, _P, _P, _P, _P, _P, ctypes.c_int, _P]),
     "mac_sim_counts": (ctypes.c_int, [_P, _ip, _ip]),
+    "mac_sim_get_step": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_ulonglong)]),
+    "mac_sim_set_step": (ctypes.c_int, [_P, ctypes.c_ulonglong]),
     "mac_sim_download": (ctypes.c_int, [_P] + [_P] * 11),
     "mac_sim_topology": (ctypes.c_int, [_P]),
     "mac_sim_compute_dt": (ctypes.c_int, [_P, _dp]),
@@ -433,6 +435,14 @@ class Sim:
         nb, npart = ctypes.c_int(), ctypes.c_int()
         _check(self.lib.mac_sim_counts(self.ptr, ctypes.byref(nb), ctypes.byref(npart)))
         return nb.value, npart.value
+
+    def get_step(self):
+        n = ctypes.c_ulonglong()
+        _check(self.lib.mac_sim_get_step(self.ptr, ctypes.byref(n)))
+        return n.value
+
+    def set_step(self, n):
+        _check(self.lib.mac_sim_set_step(self.ptr, int(n)))
 
     def download(self):
         nb, npart = self.counts()

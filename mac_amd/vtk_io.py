@@ -4,12 +4,14 @@ Files follow the reference's output policies:
 
 * ``write_grid`` -- Policy_Output_Grid_Pressure_ID (OutputGridPressure,
   InputGridPressure): points at the low corner (key - 1/2) h of every
-  existing cell plus the +e_i corners of cells without a +e_i neighbour;
-  one VTK_VOXEL per existing cell whose 8 corners are points; cell arrays
-  Pressure, Fluid, Type (MaterialToInt), Solid, Interior, Key, Speed,
-  SpeedConst, SpeedAverage.  It is also the checkpoint: ``read_grid``
-  returns the bricks (keys, flag bytes, speeds, pressure) to upload.
-  ``Depth`` is added so that a resumed depth chain keeps its Interior depths.
+  existing cell plus the +e_i corners of cells without a +e_i neighbour (the
+  reference's), then the remaining corners of existing cells, so that every
+  existing cell is one VTK_VOXEL; cell arrays Pressure, Fluid, Type
+  (MaterialToInt), Solid, Interior, Key, Speed, SpeedConst, SpeedAverage.  It
+  is also the checkpoint: ``read_grid`` returns the bricks (keys, flag bytes,
+  layers, speeds, pressure, depth) and the step counter to resume from;
+  ``Flags``, ``Layer``, ``Depth`` and the field data ``Step`` and ``Time`` are
+  added for that.
 * ``write_speed`` -- Policy_Output_Grid_Speed_ID: one file per component,
   points at the face centres of the cells with a layer, VTK_VOXEL over faces
   whose 7 +e neighbours have a layer, point array speedx/speedy/speedz.
@@ -55,9 +57,14 @@ def _array(parent, name, a, ncomp=1):
     return e
 
 
-def _write_vtu(path, points, conn, offsets, types, cell_data=(), point_data=()):
+def _write_vtu(path, points, conn, offsets, types, cell_data=(), point_data=(), field_data=()):
     root = ET.Element("VTKFile", type="UnstructuredGrid", version="1.0", byte_order="LittleEndian", header_type="UInt64")
     grid = ET.SubElement(root, "UnstructuredGrid")
+    if field_data:
+        fd = ET.SubElement(grid, "FieldData")
+        for name, a in field_data:
+            e = _array(fd, name, np.atleast_1d(a))
+            e.set("NumberOfTuples", str(np.atleast_1d(a).size))
     piece = ET.SubElement(grid, "Piece", NumberOfPoints=str(len(points)), NumberOfCells=str(len(types)))
     pd = ET.SubElement(piece, "PointData")
     for name, a, nc in point_data:
@@ -96,7 +103,8 @@ def read_vtu(path):
     cells = arrays(piece.find("Cells"))
     return {"points": arrays(piece.find("Points"))["Points"], "connectivity": cells["connectivity"],
             "offsets": cells["offsets"], "types": cells["types"], "point_data": arrays(piece.find("PointData")),
-            "cell_data": arrays(piece.find("CellData"))}
+            "cell_data": arrays(piece.find("CellData")),
+            "field_data": arrays(ET.parse(path).getroot().find("UnstructuredGrid/FieldData"))}
 
 
 def _cells(state):
@@ -114,9 +122,14 @@ def _cells(state):
 _CORNERS = [(0, 0, 0), (1, 0, 0), (0, 1, 0), (1, 1, 0), (0, 0, 1), (1, 0, 1), (0, 1, 1), (1, 1, 1)]
 
 
-def write_grid(path, state, h, depth=None):
+def write_grid(path, state, h, depth=None, step=None, time=None):
     """Policy_Output_Grid_Pressure_ID::OutputGridPressure of a downloaded
-    state (mac_amd.lib.Sim.download / oracle.Sim.download)."""
+    state (mac_amd.lib.Sim.download / oracle.Sim.download).  The points are
+    the reference's (the low corner of every existing cell, then the +e_i
+    corners of cells without a +e_i neighbour) followed by every other corner
+    of an existing cell, so that every existing cell has its voxel and the
+    file is a complete checkpoint; Layer, Depth, Flags and, as field data, the
+    step counter and the time are added for the resume."""
     h = np.asarray(h, np.float64)
     gk, idx = _cells(state)
     pmap = {}
@@ -137,6 +150,9 @@ def write_grid(path, state, h, depth=None):
             n = tuple(n)
             if n not in exist:
                 add(n)
+    for k in gk.tolist():
+        for d in _CORNERS:
+            add(tuple(k[j] + d[j] for j in range(3)))
     conn, cell_idx, cell_keys = [], [], []
     for k, c in zip(gk.tolist(), idx.tolist()):
         corners = [tuple(k[j] + d[j] for j in range(3)) for d in _CORNERS]
@@ -166,24 +182,35 @@ def write_grid(path, state, h, depth=None):
                  ("Flags", flags.astype(np.uint8), 1)]
     if depth is not None:
         cell_data.append(("Depth", np.asarray(depth, np.int16)[ci], 1))
+    if "layer" in state:
+        cell_data.append(("Layer", np.asarray(state["layer"]).astype(np.int16)[ci], 1))
+    field = []
+    if step is not None:
+        field.append(("Step", np.int64(step)))
+    if time is not None:
+        field.append(("Time", np.float64(time)))
     n = len(cell_idx)
-    _write_vtu(path, np.asarray(pts).reshape(-1, 3), conn, 8 * np.arange(1, n + 1), np.full(n, VOXEL), cell_data)
+    _write_vtu(path, np.asarray(pts).reshape(-1, 3), conn, 8 * np.arange(1, n + 1), np.full(n, VOXEL), cell_data,
+               field_data=field)
     return n
 
 
 def read_grid(path):
     """InputGridPressure: the bricks of the cells a grid file holds, as
-    dict(keys, flags, u, v, w, p, depth) for Sim.upload / set_depth.  Cells
-    without all 8 corners in the file (none in a file of write_grid on a grid
-    whose cells all have +e neighbours) are not in it."""
-    d = read_vtu(path)["cell_data"]
+    dict(keys, flags, layer, u, v, w, p, depth, step, time) for Sim.upload /
+    set_depth / set_step (step and time None when the file has none)."""
+    vt = read_vtu(path)
+    d = vt["cell_data"]
     gk = d["Key"].astype(np.int64)
     bk = np.floor_divide(gk, 8)
     uniq = sorted({tuple(k) for k in bk.tolist()}, key=lambda k: (k[2], k[1], k[0]))
     bid = {k: i for i, k in enumerate(uniq)}
     nb = len(uniq)
     out = {"keys": np.asarray(uniq, np.int32).reshape(-1, 3), "flags": np.zeros(nb * 512, np.uint8),
-           "p": np.zeros(nb * 512), "depth": np.full(nb * 512, -1, np.int16)}
+           "p": np.zeros(nb * 512), "depth": np.full(nb * 512, -1, np.int16), "layer": np.full(nb * 512, -1, np.int8)}
+    fdat = vt["field_data"]
+    out["step"] = int(fdat["Step"][0]) if "Step" in fdat else None
+    out["time"] = float(fdat["Time"][0]) if "Time" in fdat else None
     for c in ("u", "v", "w"):
         out[c] = np.zeros(nb * 512)
     o = gk - bk * 8
@@ -194,6 +221,8 @@ def read_grid(path):
         out[c][flat] = d["Speed"][:, i]
     if "Depth" in d:
         out["depth"][flat] = d["Depth"]
+    if "Layer" in d:
+        out["layer"][flat] = d["Layer"].astype(np.int8)
     return out
 
 
@@ -261,7 +290,7 @@ def write_step(prefix, step, t, state, h, animation=None, depth=None):
     """Algorithms_Output's set for one output step: the grid file (checkpoint),
     the three speed files and the particle file, added to the animation."""
     files = [f"{prefix}press_{step}.vtu"]
-    write_grid(files[0], state, h, depth)
+    write_grid(files[0], state, h, depth, step=step, time=t)
     files += write_speed(f"{prefix}speed", step, state, h)
     files.append(write_particles(f"{prefix}topo_", step, state["particles"]))
     if animation is not None:

@@ -218,6 +218,9 @@ class Sim:
             lib.orc_sim_extrapolate.argtypes = [_P, ctypes.c_int]
             lib.orc_sim_get_depth.argtypes = [_P, _P]
             lib.orc_sim_set_depth.argtypes = [_P, _P]
+            lib.orc_sim_get_tick.restype = ctypes.c_ulonglong
+            lib.orc_sim_get_tick.argtypes = [_P]
+            lib.orc_sim_set_tick.argtypes = [_P, ctypes.c_ulonglong]
             lib.orc_sim_step.restype = ctypes.c_double
             lib.orc_sim_step.argtypes = [_P, _P]
             lib._sim_bound = True
@@ -259,6 +262,12 @@ class Sim:
         self.lib.orc_sim_download(self.ptr, *[_p(out[k]) for k in (
             "keys", "flags", "layer", "u", "v", "w", "ax", "ay", "az", "p", "particles")])
         return out
+
+    def get_step(self):
+        return int(self.lib.orc_sim_get_tick(self.ptr))
+
+    def set_step(self, n):
+        self.lib.orc_sim_set_tick(self.ptr, int(n))
 
     def get_depth(self):
         nb, _ = self.counts()

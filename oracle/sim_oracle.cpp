@@ -853,6 +853,9 @@ void orc_sim_upload(void* p, int nb, const int32_t* keys, const uint8_t* flags, 
   for (int k = 0; k < np; ++k) s->parts.push_back({pxyz[3 * k], pxyz[3 * k + 1], pxyz[3 * k + 2]});
 }
 
+unsigned long long orc_sim_get_tick(void* p) { return ((Sim*)p)->tick; }
+void orc_sim_set_tick(void* p, unsigned long long t) { ((Sim*)p)->tick = t; }
+
 void orc_sim_counts(void* p, int* nb, int* np) {
   Sim* s = (Sim*)p;
   *nb = (int)s->grid.size();

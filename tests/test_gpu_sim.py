@@ -415,3 +415,12 @@ def test_small_breakup_jet_vs_oracle():
             ref = do[k][ex]
             assert np.linalg.norm(dg[k][ex] - ref) <= 1e-8 * max(np.linalg.norm(ref), 1.0), (step, k)
         same_particles(dg["particles"], do["particles"], 1e-9)
+
+
+def test_resume_from_checkpoint_on_device(tmp_path):
+    """The device time step resumed from the output set (grid file with layers,
+    depth and the step counter, particle file) after 4 steps of the
+    depth-chain jet with a pulsed inflow equals the uninterrupted device run
+    4 steps later, bit for bit."""
+    from tests.test_vtk_io import resume_matches, same_state
+    same_state(*resume_matches(lambda kw: Sim(**kw, proj={"rtol": 1e-12}), tmp_path))

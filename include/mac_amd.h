@@ -188,11 +188,15 @@ int mac_gradient_update(mac_ctx* ctx);
 int mac_max_divergence(mac_ctx* ctx, double* out);
 
 /* Slot A of the reference (Policy_Solve_Linear_Umfpack.h:7-71): solve
- * A x = b for a symmetric CSR/CSC matrix (offset[n+1], indice/value[nnz]),
- * as assembled by Algorithms_Solve_Pressure::Init_Iteration.  n == 0 is a
- * no-op; n <= 3 is solved on the host by Gaussian elimination with partial
- * pivoting (the 3x3 systems of the extrapolation boundary condition); larger
- * systems by Jacobi-preconditioned CG in FP64 on the device to rtol. */
+ * A x = b for a matrix in compressed columns, UMFPACK's layout (offset[n+1]
+ * column pointers, indice[nnz] row indices, value[nnz]).  n == 0 is a no-op;
+ * n <= 3 is solved on the host by Gaussian elimination with partial
+ * pivoting (the non-symmetric 3x3 corner systems of the extrapolation
+ * boundary condition, Algorithm_Extrapolate_Boundary_Condition.h:557-576);
+ * larger systems must be symmetric (the pressure system of
+ * Algorithms_Solve_Pressure::Init_Iteration, where columns and rows
+ * coincide) and are solved by Jacobi-preconditioned CG in FP64 on the device
+ * to rtol. */
 int mac_solve_csr(mac_ctx* ctx, int n, const int* offset, const int* indice, const double* value,
                   const double* b, double* x, double rtol);
 

@@ -352,19 +352,20 @@ __global__ void k_csr_small(int n, int nnz, const int* __restrict__ off, const i
     st->status = -3;
     return;
   }
-  for (int i = 0; i < n; ++i) {
-    if (off[i + 1] < off[i]) {
+  // compressed columns, UMFPACK's layout: column j holds entries off[j]..off[j+1]
+  for (int j = 0; j < n; ++j) {
+    if (off[j + 1] < off[j]) {
       st->status = -3;
       return;
     }
-    for (int k = off[i]; k < off[i + 1]; ++k) {
+    for (int k = off[j]; k < off[j + 1]; ++k) {
       if (idx[k] < 0 || idx[k] >= n) {
         st->status = -3;
         return;
       }
-      A[i][idx[k]] += val[k];
+      A[idx[k]][j] += val[k];
     }
-    A[i][3] = b[i];
+    A[j][3] = b[j];
   }
   for (int c = 0; c < n; ++c) {
     int piv = c;
@@ -393,11 +394,13 @@ __global__ void k_csr_small(int n, int nnz, const int* __restrict__ off, const i
   st->status = 1;
 }
 
+// compressed columns (UMFPACK's layout: column j holds entries off[j]..off[j+1],
+// row indices idx), which matters for the non-symmetric 3x3 corner systems
 int small_solve(int n, const int* off, const int* idx, const double* val, const double* b, double* x) {
   double A[3][4] = {{0}};
-  for (int i = 0; i < n; ++i) {
-    for (int k = off[i]; k < off[i + 1]; ++k) A[i][idx[k]] += val[k];
-    A[i][3] = b[i];
+  for (int j = 0; j < n; ++j) {
+    for (int k = off[j]; k < off[j + 1]; ++k) A[idx[k]][j] += val[k];
+    A[j][3] = b[j];
   }
   for (int c = 0; c < n; ++c) {
     int piv = c;

@@ -1,7 +1,8 @@
 // csr_solve.hip -- slot A of the reference, the linear-solver policy
-// (Policy_Solve_Linear_Umfpack.h:7-71), on the GPU: solve A x = b for the
-// symmetric CSR (= CSC) matrix assembled by
-// Algorithms_Solve_Pressure::Init_Iteration (:44-157).  UMFPACK factorises;
+// (Policy_Solve_Linear_Umfpack.h:7-71), on the GPU: solve A x = b for a
+// matrix in UMFPACK's compressed columns; the large systems are the symmetric
+// pressure matrix assembled by Algorithms_Solve_Pressure::Init_Iteration
+// (:44-157), where columns and rows coincide.  UMFPACK factorises;
 // this slot runs Jacobi-preconditioned CG in FP64 on the device
 // (csr_kernels.h) to a tight relative residual, which at rtol <= 1e-12
 // agrees with the direct solve far inside the 1e-8 parity bound.
@@ -10,8 +11,9 @@
 // context (grown, never shrunk, freed by mac_destroy), so only a call larger
 // than every earlier one allocates; all work runs on the context's stream.
 // Systems of size <= 3 (the extrapolation boundary condition's corner
-// solves, Algorithm_Extrapolate_Boundary_Condition.h:549-576) are solved on
-// the host by Gaussian elimination with partial pivoting.
+// solves, Algorithm_Extrapolate_Boundary_Condition.h:549-576, which are not
+// symmetric) are solved on the host by Gaussian elimination with partial
+// pivoting, reading the arrays as compressed columns.
 #include "csr_kernels.h"
 
 extern "C" int mac_solve_csr(mac_ctx* ctx, int n, const int* offset, const int* indice, const double* value,

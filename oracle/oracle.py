@@ -193,6 +193,18 @@ def mgpcg(grid, rtol=1e-8, maxit=500, nu=2, omega=0.9, threads=0, smoother=1, ch
 
 
 # ---------------------------------------------------------------- time step --
+def corner_solve(h, signs, nb81):
+    """The three-axis corner system (SolveMatrix / CreateMatrix) on a 3x3x3
+    neighbourhood nb81[c, dz + 1, dy + 1, dx + 1] of face components."""
+    lib = load()
+    lib.orc_corner_solve.argtypes = [_P, ctypes.c_int, ctypes.c_int, ctypes.c_int, _P, _P]
+    hh = np.ascontiguousarray(h, np.float64)
+    nb = np.ascontiguousarray(nb81, np.float64).reshape(81)
+    res = np.zeros(3)
+    lib.orc_corner_solve(_p(hh), int(signs[0]), int(signs[1]), int(signs[2]), _p(nb), _p(res))
+    return res
+
+
 class Sim:
     """CPU restatement of the time-step loop (sim_oracle.cpp)."""
 
@@ -218,6 +230,7 @@ class Sim:
             lib.orc_sim_extrapolate.argtypes = [_P, ctypes.c_int]
             lib.orc_sim_get_depth.argtypes = [_P, _P]
             lib.orc_sim_set_depth.argtypes = [_P, _P]
+            lib.orc_sim_surface_bc.argtypes = [_P, ctypes.c_int, ctypes.c_int]
             lib.orc_sim_get_tick.restype = ctypes.c_ulonglong
             lib.orc_sim_get_tick.argtypes = [_P]
             lib.orc_sim_set_tick.argtypes = [_P, ctypes.c_ulonglong]
@@ -262,6 +275,11 @@ class Sim:
         self.lib.orc_sim_download(self.ptr, *[_p(out[k]) for k in (
             "keys", "flags", "layer", "u", "v", "w", "ax", "ay", "az", "p", "particles")])
         return out
+
+    def surface_bc(self, slot=0, dim=3):
+        """The free-surface velocity boundary condition on speed slot `slot`
+        (3 = the acceleration), in 3-D or (dim=2) on the x-y plane."""
+        self.lib.orc_sim_surface_bc(self.ptr, slot, dim)
 
     def get_step(self):
         return int(self.lib.orc_sim_get_tick(self.ptr))

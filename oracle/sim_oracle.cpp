@@ -90,6 +90,91 @@ struct Params {
   double lo[3], hi[3];
 };
 
+// Gaussian elimination with partial pivoting of a 3x3 system [A | b]; the
+// steps of the linear-solver slot's n <= 3 path (NaN on a zero pivot).
+void gauss3(double A[3][4], double x[3]) {
+  for (int c = 0; c < 3; ++c) {
+    int piv = c;
+    for (int i = c + 1; i < 3; ++i)
+      if (std::fabs(A[i][c]) > std::fabs(A[piv][c])) piv = i;
+    if (A[piv][c] == 0.0) {
+      x[0] = x[1] = x[2] = std::nan("");
+      return;
+    }
+    if (piv != c)
+      for (int j = 0; j < 4; ++j) std::swap(A[c][j], A[piv][j]);
+    for (int i = c + 1; i < 3; ++i) {
+      const double m = A[i][c] / A[c][c];
+      for (int j = c; j < 4; ++j) A[i][j] -= m * A[c][j];
+    }
+  }
+  for (int i = 2; i >= 0; --i) {
+    double s = A[i][3];
+    for (int j = i + 1; j < 3; ++j) s -= A[i][j] * x[j];
+    x[i] = s / A[i][i];
+  }
+}
+
+// SolveMatrix (:578-778): the three air faces of a cell with air on one side
+// of every axis (signs s1, s2, s3), from the 3x3 system of CreateMatrix
+// (:557-576) solved by Gaussian elimination with partial pivoting (the n = 3
+// path of the linear-solver slot); NaN if the right side is.  U(c, dx, dy, dz)
+// is face component c of the cell at offset (dx, dy, dz) from the corner cell.
+template <class UF>
+void corner_system(const double h[3], int s1, int s2, int s3, UF U, double res[3]) {
+  const double i1 = 1.0 / h[0], i2 = 1.0 / h[1], i3 = 1.0 / h[2];
+  // b[0]
+  double b0 = 0;
+  b0 += (s1 == 1) ? -2 * i1 * U(0, 0, 0, 0) : 2 * i1 * U(0, 1, 0, 0);
+  b0 += (s2 == 1) ? 2 * i2 * U(1, 0, 0, 0) : -2 * i2 * U(1, 0, 1, 0);
+  b0 += (s3 == 1) ? 0.5 * i1 * U(2, 0, 0, 0) : -0.5 * i1 * U(2, 0, 0, 1);
+  b0 += s3 * 0.5 * i1 * (-U(2, -s1, 0, 0) - U(2, -s1, 0, 1));
+  b0 += (s1 == 1) ? 0.5 * i3 * U(0, 0, 0, 0) : -0.5 * i3 * U(0, 1, 0, 0);
+  b0 += s1 * 0.5 * i3 * (-U(0, 0, 0, -s3) - U(0, 1, 0, -s3));
+  b0 += (s3 == 1) ? -0.5 * i2 * U(2, 0, 0, 0) : 0.5 * i2 * U(2, 0, 0, 1);
+  b0 += s3 * 0.5 * i2 * (U(2, 0, -s2, 0) + U(2, 0, -s2, 1));
+  b0 += (s2 == 1) ? -0.5 * i3 * U(1, 0, 0, 0) : 0.5 * i3 * U(1, 0, 1, 0);
+  b0 += s2 * 0.5 * i3 * (U(1, 0, 0, -s3) + U(1, 0, 1, -s3));
+  // b[1]
+  double b1 = 0;
+  b1 += (s3 == 1) ? -2 * i3 * U(2, 0, 0, 0) : 2 * i3 * U(2, 0, 0, -s3);
+  b1 += (s2 == 1) ? 2 * i2 * U(1, 0, 0, 0) : -2 * i2 * U(1, 0, 1, 0);
+  b1 += (s3 == 1) ? 0.5 * i1 * U(2, 0, 0, 0) : -0.5 * i1 * U(2, 0, 0, 1);
+  b1 += s3 * 0.5 * i1 * (-U(2, -s1, 0, 0) - U(2, -s1, 0, 1));
+  b1 += (s1 == 1) ? 0.5 * i3 * U(0, 0, 0, 0) : -0.5 * i3 * U(0, 1, 0, 0);
+  b1 += s1 * 0.5 * i3 * (-U(0, 0, 0, -s3) - U(0, 1, 0, -s3));
+  b1 += (s1 == 1) ? -0.5 * i2 * U(0, 0, 0, 0) : -0.5 * i2 * U(0, 1, 0, 0);
+  b1 += s1 * 0.5 * i2 * (U(0, 0, -s2, 0) + U(0, 1, -s2, 0));
+  b1 += (s2 == 1) ? -0.5 * i1 * U(1, 0, 0, 0) : 0.5 * i1 * U(1, 0, 1, 0);
+  b1 += s2 * 0.5 * i1 * (U(1, -s1, 0, 0) + U(1, -s1, 1, 0));
+  // b[2]
+  double b2 = 0;
+  b2 += (s1 == 1) ? i1 * U(0, 0, 0, 0) : -i1 * U(0, 1, 0, 0);
+  b2 += (s2 == 1) ? i2 * U(1, 0, 0, 0) : -i2 * U(1, 0, 1, 0);
+  b2 += (s3 == 1) ? i3 * U(2, 0, 0, 0) : -i3 * U(2, 0, 0, 1);
+  if (std::isnan(b0) || std::isnan(b1) || std::isnan(b2)) {
+    res[0] = res[1] = res[2] = std::nan("");
+    return;
+  }
+  // CreateMatrix: m_value[k] in compressed columns of 3 (UMFPACK's layout,
+  // umfpack_di_symbolic / umfpack_di_solve(UMFPACK_A, ...)): column k / 3,
+  // row k % 3, so the third row is the cell's divergence
+  double A[3][4];
+  A[0][0] = -s1 * 2 * i1 - s1 * 0.5 * i3;     // m_value[0]
+  A[0][1] = s2 * 2 * i2 + s2 * 0.5 * i3;      // m_value[3]
+  A[0][2] = -s3 * 0.5 * i1 + s2 * s3 * 0.5 * i2;  // m_value[6]
+  A[1][0] = -s1 * 0.5 * i3 + s1 * 0.5 * i2;   // m_value[1]
+  A[1][1] = s2 * 2 * i2 + s2 * 0.5 * i2;      // m_value[4]
+  A[1][2] = -s3 * 2 * i3 - s3 * 0.5 * i1;     // m_value[7]
+  A[2][0] = s1 * i1;                          // m_value[2]
+  A[2][1] = s2 * i2;                          // m_value[5]
+  A[2][2] = s3 * i3;                          // m_value[8]
+  A[0][3] = b0;
+  A[1][3] = b1;
+  A[2][3] = b2;
+  gauss3(A, res);
+}
+
 struct Sim {
   Params P;
   double ih[3];
@@ -354,6 +439,294 @@ struct Sim {
   // neighbours); cells with layer > lay reached that way form the next
   // frontier.  Non-forced Set: const faces keep their value.  A mean over no
   // face is 0/0, as in the reference.
+  // Algorithm_Extrapolate_Boundary_Condition (the free-surface velocity
+  // boundary condition, Algorithm_Extrapolate_Boundary_Condition.h:44-986) on
+  // field `slot`, dim = 3 (Algorithm_Extrapolate_Boundary_Condition_Dim<3>)
+  // or 2 (Dim<2>, the x-y plane; z faces untouched), in the defined form the
+  // device follows.  The reference visits cells in hash order and every rule
+  // writes faces other cells' rules read; here the rules run as phases, each
+  // phase reading what the phases before it wrote:
+  //   P1 apply_set_nan (:145-175) on every air cell;
+  //   P2 the zero-divergence face of every cell with one air neighbour
+  //      (Apply_Div_3d, :920-938, with the sum over both tangential
+  //      directions where the reference's `res =` keeps only the last); in
+  //      2-D Apply_Div_2d (:402-420) on the cells that have a plane partner;
+  //   P3 the tangential-stress faces of the plane pairs (Apply_3d_Plane
+  //      :905-918 / Apply_2d_Plane :428-448), every pair reading the state P2
+  //      left; a face two pairs write takes the pair with the larger
+  //      (normal axis, side);
+  //   P4 the 45-degree corners (Apply_3d_Plane_45 :940-957);
+  //   P5 the three-axis corners, a 3x3 solve each (SolveMatrix :578-778)
+  //      reading the state before any of them writes;
+  //   P6 the NaN marks of the remaining cases (Get_Is_Speed_In_Domain,
+  //      Policy_Is_In_Domain_Speed.h:15-49), and in 2-D of the one-air cells
+  //      without a plane partner;
+  //   P7 apply_nan_extrap (:182-245) on every boundary cell, its averages
+  //      read from the state before any cell writes.
+  // A neighbour handle that the reference dereferences without a validity
+  // check (undefined there) reads the chunk's memory here; an invalid cell
+  // is never air, fluid or a boundary.  In 3-D a one-air cell's face is not
+  // NaN-marked after P2: apply_nan_extrap would give it the same
+  // zero-divergence value back.  Every Set is non-forced (const faces keep
+  // their value).
+  void surface_bc(int slot, int dim) {
+    const double h[3] = {P.h[0], P.h[1], P.h[2]};
+    auto F = [&](int c, int x, int y, int z) { return speed(slot, c, x, y, z); };
+    auto S = [&](int c, int x, int y, int z, double v) {
+      Chunk* ch = chunk(x, y, z);
+      if (!ch) return;
+      const int o = off_of(x, y, z);
+      if (!(ch->flags[o] & 0x80)) return;
+      if (!(ch->flags[o] & (0x08u << c))) field(ch, slot, c)[o] = v;
+    };
+    auto mat = [&](int x, int y, int z) -> int { return chunk(x, y, z)->flags[off_of(x, y, z)] & 7; };
+    auto air = [&](int x, int y, int z) { return valid(x, y, z) && (mat(x, y, z) == 0 || mat(x, y, z) == 2); };
+    auto fl = [&](int x, int y, int z) { return valid(x, y, z) && is_fluid(mat(x, y, z)); };
+    struct Bnd {
+      int n = 0, dir[6], sign[6];
+    };
+    // IsBoundary (:88-119): air neighbours of a valid non-air cell, axis-major
+    auto bnd = [&](int x, int y, int z) {
+      Bnd b;
+      if (!valid(x, y, z) || air(x, y, z)) return b;
+      for (int i = 0; i < dim; ++i)
+        for (int s = -1; s <= 1; s += 2)
+          if (air(x + s * (i == 0), y + s * (i == 1), z + s * (i == 2))) {
+            b.dir[b.n] = i;
+            b.sign[b.n] = s;
+            ++b.n;
+          }
+      return b;
+    };
+    // Get_Is_Speed_In_Domain(N, dir): the face between N and N - e_dir
+    auto in_domain = [&](int x, int y, int z, int d) {
+      const int ex = (d == 0), ey = (d == 1), ez = (d == 2);
+      const int x2 = x - ex, y2 = y - ey, z2 = z - ez;
+      if (!valid(x2, y2, z2)) return fl(x, y, z);
+      const bool f1 = fl(x, y, z), f2 = fl(x2, y2, z2);
+      if (f1 && f2) return true;
+      if (!f1 && !f2) return false;
+      int x3, y3, z3;
+      if (f1) { x3 = x + ex; y3 = y + ey; z3 = z + ez; }
+      else { x3 = x2 - ex; y3 = y2 - ey; z3 = z2 - ez; }
+      if (!valid(x3, y3, z3)) return true;
+      return !fl(x3, y3, z3);
+    };
+    std::vector<K3> cells;
+    for_cells([&](Chunk*, int, int x, int y, int z) { cells.push_back({x, y, z}); });
+    const double NaN = std::nan("");
+    // partner of a one-air cell along axis t on side i: a valid non-air cell
+    // with exactly one air neighbour, on the same axis
+    auto partner = [&](int x, int y, int z, int d, int t, int i) {
+      const Bnd b = bnd(x + i * (t == 0), y + i * (t == 1), z + i * (t == 2));
+      return b.n == 1 && b.dir[0] == d;
+    };
+    auto has_partner2d = [&](int x, int y, int z, const Bnd& b) {
+      const int t = b.dir[0] == 0 ? 1 : 0;
+      return partner(x, y, z, b.dir[0], t, -1) || partner(x, y, z, b.dir[0], t, 1);
+    };
+    // P1
+    for (const K3& c : cells) {
+      if (!air(c.x, c.y, c.z)) continue;
+      for (int i = 0; i < dim; ++i)
+        for (int s = -1; s <= 1; s += 2) {
+          const int x2 = c.x + s * (i == 0), y2 = c.y + s * (i == 1), z2 = c.z + s * (i == 2);
+          if (valid(x2, y2, z2)) {
+            if (air(x2, y2, z2)) {
+              if (s == -1) S(i, c.x, c.y, c.z, NaN);
+              else S(i, x2, y2, z2, NaN);
+            }
+          } else if (s == -1) {
+            S(i, c.x, c.y, c.z, NaN);
+          }
+        }
+    }
+    // P2
+    for (const K3& c : cells) {
+      const Bnd b = bnd(c.x, c.y, c.z);
+      if (b.n != 1) continue;
+      if (dim == 2 && !has_partner2d(c.x, c.y, c.z, b)) continue;
+      const int d = b.dir[0], sg = b.sign[0];
+      const int ex = (d == 0), ey = (d == 1), ez = (d == 2);
+      double v = F(d, c.x, c.y, c.z);
+      int tx = c.x + ex, ty = c.y + ey, tz = c.z + ez;
+      if (sg == -1) {
+        v = F(d, c.x + ex, c.y + ey, c.z + ez);
+        tx = c.x; ty = c.y; tz = c.z;
+      }
+      double res;
+      if (dim == 2) {
+        const int t = d == 0 ? 1 : 0;
+        const double v2 = F(t, c.x, c.y, c.z), v3 = F(t, c.x + (t == 0), c.y + (t == 1), c.z);
+        res = v - sg * (1.0 / h[t]) * h[d] * (v3 - v2);
+      } else {
+        double sum = 0.0;
+        for (int t = 0; t < 3; ++t) {
+          if (t == d) continue;
+          const double v2 = F(t, c.x, c.y, c.z), v3 = F(t, c.x + (t == 0), c.y + (t == 1), c.z + (t == 2));
+          sum += (1.0 / h[t]) * (v3 - v2);
+        }
+        res = v - sg * sum * h[d];
+      }
+      S(d, tx, ty, tz, res);
+    }
+    // P3, gathered per target face: the face (component t) on the low t side
+    // of cell X, written by the pair (A, B) = (X - s e_d - e_t, X - s e_d) of
+    // one-air cells with air axis d where the pair's writing cell has side s.
+    // Every pair reads the state P2 left (a face one pair writes can be the
+    // air face another pair reads, on a staircase), so the values are
+    // gathered first and written after.
+    struct Pending {
+      K3 c;
+      int comp;
+      double v;
+    };
+    std::vector<Pending> p3;
+    for (const K3& X : cells) {
+      for (int t = 0; t < dim; ++t) {
+        bool have = false;
+        double val = 0.0;
+        for (int d = 0; d < dim; ++d) {
+          if (d == t) continue;
+          for (int s = -1; s <= 1; s += 2) {
+            const int ex = (d == 0), ey = (d == 1), ez = (d == 2);
+            const int bx = X.x - s * ex, by = X.y - s * ey, bz = X.z - s * ez;
+            const int ax = bx - (t == 0), ay = by - (t == 1), az = bz - (t == 2);
+            const Bnd bB = bnd(bx, by, bz), bA = bnd(ax, ay, az);
+            if (bB.n != 1 || bA.n != 1 || bB.dir[0] != d || bA.dir[0] != d) continue;
+            // the writing cell: B (partner on its -t side) or A (on its +t side)
+            if (bB.sign[0] != s && bA.sign[0] != s) continue;
+            if (dim == 2 && !(has_partner2d(bx, by, bz, bB) || has_partner2d(ax, ay, az, bA))) continue;
+            const int px = (s == 1) ? ex : 0, py = (s == 1) ? ey : 0, pz = (s == 1) ? ez : 0;
+            const double v1 = F(d, ax + px, ay + py, az + pz), v2 = F(d, bx + px, by + py, bz + pz);
+            const double v3 = F(t, bx, by, bz);
+            val = -s * (1.0 / h[t]) * h[d] * (v2 - v1) + v3;
+            have = true;
+          }
+        }
+        if (have) p3.push_back({X, t, val});
+      }
+    }
+    for (const Pending& q : p3) S(q.comp, q.c.x, q.c.y, q.c.z, q.v);
+    // P4
+    for (const K3& c : cells) {
+      const Bnd b = bnd(c.x, c.y, c.z);
+      if (b.n != 2 || b.dir[0] == b.dir[1]) continue;
+      for (int k = 0; k < 2; ++k) {
+        const int d = b.dir[k], sg = b.sign[k];
+        const int ex = (d == 0), ey = (d == 1), ez = (d == 2);
+        if (sg == -1) S(d, c.x, c.y, c.z, F(d, c.x + ex, c.y + ey, c.z + ez));
+        else S(d, c.x + ex, c.y + ey, c.z + ez, F(d, c.x, c.y, c.z));
+      }
+    }
+    // P5
+    if (dim == 3) {
+      struct Corner {
+        K3 c;
+        double r[3];
+        int s[3];
+      };
+      std::vector<Corner> todo;
+      for (const K3& c : cells) {
+        const Bnd b = bnd(c.x, c.y, c.z);
+        if (b.n != 3) continue;
+        int sa[3] = {0, 0, 0};
+        for (int i = 0; i < 3; ++i)
+          for (int k = 0; k < b.n; ++k)
+            if (b.dir[k] == i) {
+              sa[i] = b.sign[k];
+              break;
+            }
+        if (!sa[0] || !sa[1] || !sa[2]) continue;
+        Corner cr;
+        cr.c = c;
+        for (int i = 0; i < 3; ++i) cr.s[i] = sa[i];
+        corner_solve(slot, c.x, c.y, c.z, sa[0], sa[1], sa[2], cr.r);
+        todo.push_back(cr);
+      }
+      for (const Corner& cr : todo) {
+        const K3& c = cr.c;
+        for (int i = 0; i < 3; ++i) {
+          if (cr.s[i] == 1) S(i, c.x + (i == 0), c.y + (i == 1), c.z + (i == 2), cr.r[i]);
+          else S(i, c.x, c.y, c.z, cr.r[i]);
+        }
+      }
+    }
+    // P6
+    for (const K3& c : cells) {
+      const Bnd b = bnd(c.x, c.y, c.z);
+      bool mark = dim == 3 ? b.n >= 4 : b.n >= 3;
+      if (dim == 2 && b.n == 1 && !has_partner2d(c.x, c.y, c.z, b)) mark = true;
+      if (!mark) continue;
+      for (int k = 0; k < b.n; ++k) {
+        const int d = b.dir[k];
+        const int ex = (d == 0), ey = (d == 1), ez = (d == 2);
+        int x = c.x, y = c.y, z = c.z;
+        if (b.sign[k] == 1) { x += ex; y += ey; z += ez; }
+        if (!in_domain(x, y, z, d)) S(d, x, y, z, NaN);
+      }
+    }
+    // P7
+    {
+      std::map<K3, std::array<double, 3>> snap;
+      for (const K3& c : cells) snap[c] = {F(0, c.x, c.y, c.z), F(1, c.x, c.y, c.z), F(2, c.x, c.y, c.z)};
+      auto G = [&](int comp, int x, int y, int z, double& v) {
+        auto it = snap.find({x, y, z});
+        if (it == snap.end()) return false;
+        v = it->second[comp];
+        return true;
+      };
+      for (const K3& c : cells) {
+        if (bnd(c.x, c.y, c.z).n == 0) continue;
+        int nn = 0, tdir[6], tsgn[6];
+        for (int i = 0; i < dim; ++i) {
+          const int ex = (i == 0), ey = (i == 1), ez = (i == 2);
+          if (!valid(c.x + ex, c.y + ey, c.z + ez)) continue;
+          if (std::isnan(F(i, c.x, c.y, c.z))) { tdir[nn] = i; tsgn[nn] = 0; ++nn; }
+          if (std::isnan(F(i, c.x + ex, c.y + ey, c.z + ez))) { tdir[nn] = i; tsgn[nn] = 1; ++nn; }
+        }
+        if (nn == 0) continue;
+        for (int k = 0; k < nn; ++k) {
+          const int d = tdir[k];
+          int x = c.x, y = c.y, z = c.z;
+          if (tsgn[k] == 1) { x += (d == 0); y += (d == 1); z += (d == 2); }
+          double vt = 0.0;
+          int nb = 0;
+          for (int d2 = 0; d2 < dim; ++d2)
+            for (int s = -1; s <= 1; ++s) {
+              const int ss = s == 0 ? -1 : s;  // GetNeighbour(d2, 0) is the -1 neighbour
+              double a;
+              if (G(d, x + ss * (d2 == 0), y + ss * (d2 == 1), z + ss * (d2 == 2), a) && !std::isnan(a)) {
+                vt += a;
+                ++nb;
+              }
+            }
+          if (nb != 0) vt /= nb;
+          S(d, x, y, z, vt);
+        }
+        double div = 0.0;
+        for (int i = 0; i < dim; ++i) {
+          const double v1 = F(i, c.x, c.y, c.z), v2 = F(i, c.x + (i == 0), c.y + (i == 1), c.z + (i == 2));
+          div += (1.0 / h[i]) * (v2 - v1);
+        }
+        const double cor = div / nn;
+        for (int k = 0; k < nn; ++k) {
+          const int d = tdir[k];
+          int x = c.x, y = c.y, z = c.z;
+          if (tsgn[k] == 1) { x += (d == 0); y += (d == 1); z += (d == 2); }
+          S(d, x, y, z, F(d, x, y, z) - (2 * tsgn[k] - 1) * h[d] * cor);
+        }
+      }
+    }
+  }
+
+  // SolveMatrix (:578-778) for the corner cell (x, y, z): corner_system on
+  // its neighbourhood of slot `slot`, read through the grid.
+  void corner_solve(int slot, int x, int y, int z, int s1, int s2, int s3, double res[3]) {
+    auto U = [&](int c, int dx, int dy, int dz) { return speed(slot, c, x + dx, y + dy, z + dz); };
+    corner_system(P.h, s1, s2, s3, U, res);
+  }
+
   void extrapolate(int slot) {
     auto ok = [&](int x, int y, int z) { return valid(x, y, z) && layer_of(x, y, z) != -1; };
     auto U = [&](int comp, int x, int y, int z) { return field(chunk(x, y, z), slot, comp)[off_of(x, y, z)]; };
@@ -902,6 +1275,13 @@ void orc_sim_forces(void* p, int slot) { ((Sim*)p)->forces(slot); }
 void orc_sim_advance(void* p, double dt, double* maxdiv) { ((Sim*)p)->advance(dt, maxdiv); }
 void orc_sim_move(void* p, double dt) { ((Sim*)p)->move(dt); }
 void orc_sim_extrapolate(void* p, int slot) { ((Sim*)p)->extrapolate(slot); }
+void orc_sim_surface_bc(void* p, int slot, int dim) { ((Sim*)p)->surface_bc(slot, dim); }
+// The corner system alone on a 3x3x3 neighbourhood: nb81[c * 27 + (dx + 1) +
+// 3 (dy + 1) + 9 (dz + 1)] is face component c of the cell at (dx, dy, dz).
+void orc_corner_solve(const double* h, int s1, int s2, int s3, const double* nb81, double* res) {
+  auto U = [&](int c, int dx, int dy, int dz) { return nb81[c * 27 + (dx + 1) + 3 * (dy + 1) + 9 * (dz + 1)]; };
+  corner_system(h, s1, s2, s3, U, res);
+}
 void orc_sim_interpolate(void* p, int n, const double* pos, double* vel) {
   Sim* s = (Sim*)p;
   for (int k = 0; k < n; ++k) {

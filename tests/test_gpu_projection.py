@@ -208,6 +208,13 @@ def test_solve_csr_slot_a():
     x3 = ls.Solve_Linear(3, off3, idx3, val3, b3)
     A = np.array([[-2, 1, 0], [1, -3, 1], [0, 1, -2.0]])
     assert np.allclose(A @ x3, b3, rtol=0, atol=1e-14)
+    # a non-symmetric 3x3 (the corner systems' shape) in compressed columns,
+    # UMFPACK's layout: value[3c + r] is row r of column c
+    An = np.array([[2.0, -1.0, 0.5], [0.25, 3.0, -2.0], [1.0, 1.0, 1.0]])
+    offn = np.array([0, 3, 6, 9], dtype=np.int32)
+    idxn = np.array([0, 1, 2] * 3, dtype=np.int32)
+    xn = ls.Solve_Linear(3, offn, idxn, An.T.reshape(-1).copy(), b3)
+    assert np.allclose(xn, np.linalg.solve(An, b3), rtol=1e-14, atol=1e-14)
 
 
 def _box_system(nx, ny, nz, seed):

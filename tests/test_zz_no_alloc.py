@@ -105,6 +105,14 @@ def test_noalloc_small_system_and_limits():
     x3, _ = _noalloc(ls, 3, off3, idx3, val3, b3, 1e-13)
     ref = ls.Solve_Linear(3, off3, idx3, val3, b3)
     assert np.allclose(x3, ref, rtol=1e-14, atol=1e-15)
+    # non-symmetric, compressed columns (value[3c + r] is row r of column c)
+    An = np.array([[2.0, -1.0, 0.5], [0.25, 3.0, -2.0], [1.0, 1.0, 1.0]])
+    offn = np.array([0, 3, 6, 9], dtype=np.int32)
+    idxn = np.array([0, 1, 2] * 3, dtype=np.int32)
+    valn = An.T.reshape(-1).copy()
+    xn, _ = _noalloc(ls, 3, offn, idxn, valn, b3, 1e-13)
+    assert np.allclose(xn, ls.Solve_Linear(3, offn, idxn, valn, b3), rtol=1e-14, atol=1e-15)
+    assert np.allclose(xn, np.linalg.solve(An, b3), rtol=1e-14, atol=1e-14)
     # beyond the scratch: a clear error before any device work
     lib = load()
     rc = lib.mac_solve_csr_noalloc(ls.proj.ctx, (1 << 18) + 1, 0, 1, 1, 1, 1, 1, 1e-8, None)

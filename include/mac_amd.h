@@ -288,6 +288,14 @@ typedef struct {
   double inflow_speed[3];
   double inflow_amp;
   int inflow_period;
+  int surface_bc;       /* 1: every velocity extrapolation (mac_sim_extrapolate and,
+                           with integrator 2, those of the speeds and of the
+                           acceleration in every stage) first applies the
+                           free-surface velocity condition
+                           (Algorithm_Extrapolate_Boundary_Condition, chained before
+                           Algorithms_Extrapolate as test/Jet3.cpp:541-550,636-639
+                           does); 0: off.  Integrators 0 and 1 extrapolate nothing
+                           (TestJetDEau_Basic.cpp's chain).                          */
   mac_params proj;      /* projection solver settings                                  */
 } mac_sim_params;
 
@@ -370,6 +378,19 @@ int mac_sim_step(mac_sim* sim, mac_sim_stats* stats);
  * faces next to it that touch a cell of layer < L (non-forced: const faces
  * keep their value; a mean over no face is 0/0 = NaN, as in the reference). */
 int mac_sim_extrapolate(mac_sim* sim, int slot);
+/* Algorithm_Extrapolate_Boundary_Condition::Do
+ * (Algorithm_Extrapolate_Boundary_Condition.h:44-986) on speed slot `slot`
+ * (0..2) or, slot 3, the acceleration (Algorithms_Accel_To_Speed); dim 3, or
+ * 2 for Algorithm_Extrapolate_Boundary_Condition_Dim<2> on the x-y plane (z
+ * faces untouched).  The reference's rules run in a defined order of seven
+ * phases, each reading what the phases before it wrote (set NaN between air
+ * cells; the zero-divergence face of one-air cells; the zero-shear
+ * tangential faces of face-cell pairs; 45-degree corners; the 3x3 corner
+ * solves; NaN on air faces outside the domain; apply_nan_extrap), so the
+ * result does not depend on the order of the bricks.  Non-forced: const
+ * faces keep their value.  Faces between two non-air cells are read, never
+ * written; they must be finite. */
+int mac_sim_surface_bc(mac_sim* sim, int slot, int dim);
 
 /* ===================================================================== */
 /* Learned initial guess: 3-level 3-D U-Net (no counterpart in the        */

@@ -24,6 +24,7 @@
 
 #include "mac_internal.h"
 #include "sim_kernels.h"
+#include "surface_bc.h"
 
 namespace {
 
@@ -112,6 +113,12 @@ struct mac_sim {
   int keys_cap = 0, hash_cap = 0;
   uint8_t* d_front = nullptr;  // the extrapolation's two frontier masks
   size_t front_cap = 0;
+  // the free-surface condition's staging (3 values and 3 pending bytes per
+  // cell, the bytes zero between calls) and per-brick activity
+  double* bc_tmp = nullptr;
+  uint8_t* bc_pend = nullptr;
+  uint8_t* bc_brick = nullptr;  // [0, cap): summary, [cap, 2 cap): active
+  size_t bc_cap = 0;            // bricks
   // starting iterate of the projections: 0 zero, 1 last pressure, 2 learned
   int guess_mode = 0;
   mac_unet* net = nullptr;
@@ -582,6 +589,7 @@ int mac_sim_create(mac_sim** out, const mac_sim_params* p) {
   if (p->fill_cube_n < 0 || p->fill_cube_n > 3) return mac_internal_set_error(MAC_E_ARG, "fill_cube_n must be in 0..3");
   if (!std::isfinite(p->inflow_amp) || (p->inflow_amp != 0.0 && p->inflow_period < 1))
     return mac_internal_set_error(MAC_E_ARG, "inflow_amp must be finite, with inflow_period >= 1 when it is not 0");
+  if (p->surface_bc != 0 && p->surface_bc != 1) return mac_internal_set_error(MAC_E_ARG, "surface_bc must be 0 or 1");
   if (p->depth_interior < 0 || p->depth_interior > 32767) return mac_internal_set_error(MAC_E_ARG, "depth_interior must be in 0..32767");
   if (p->floor_axis < -1 || p->floor_axis > 2) return mac_internal_set_error(MAC_E_ARG, "floor_axis must be -1, 0, 1 or 2");
   for (int i = 0; i < 3; ++i)
@@ -624,6 +632,9 @@ void mac_sim_destroy(mac_sim* s) {
   hipFree(s->d_int);
   hipFree(s->d_any);
   hipFree(s->d_front);
+  hipFree(s->bc_tmp);
+  hipFree(s->bc_pend);
+  hipFree(s->bc_brick);
   hipFree(s->d_missing);
   hipFree(s->d_missing2);
   hipFree(s->d_cset);
@@ -851,9 +862,68 @@ int mac_sim_interpolate(mac_sim* s, int n, const double* pos, double* vel) {
 }
 
 namespace {
-// Algorithms_Extrapolate::Do on the face field (u, v, w) of the current bricks.
+// Algorithm_Extrapolate_Boundary_Condition::Do on the face field (u, v, w) of
+// the current bricks (surface_bc.h): seven phases in a fixed order.
+int sim_surface_bc(mac_sim* s, double* u, double* v, double* w, int dim) {
+  if (s->nb == 0) return 0;
+  const size_t nb = (size_t)s->nb;
+  if (s->bc_cap < nb) {
+    hipFree(s->bc_tmp);
+    hipFree(s->bc_pend);
+    hipFree(s->bc_brick);
+    s->bc_tmp = nullptr;
+    s->bc_pend = nullptr;
+    s->bc_brick = nullptr;
+    s->bc_cap = 0;
+    const size_t cap = nb + nb / 4 + 16;
+    SCHK(hipMalloc(&s->bc_tmp, 3 * cap * MAC_BC * sizeof(double)));
+    SCHK(hipMalloc(&s->bc_pend, 3 * cap * MAC_BC));
+    SCHK(hipMalloc(&s->bc_brick, 2 * cap));
+    SCHK(hipMemsetAsync(s->bc_pend, 0, 3 * cap * MAC_BC, s->st));
+    s->bc_cap = cap;
+  }
+  const size_t cells = s->bc_cap * MAC_BC;
+  BcArgs a;
+  a.nb = s->nb;
+  a.nbr = s->nbr;
+  a.flags = s->flags;
+  a.U[0] = u;
+  a.U[1] = v;
+  a.U[2] = w;
+  for (int i = 0; i < 3; ++i) {
+    a.tmp[i] = s->bc_tmp + i * cells;
+    a.pend[i] = s->bc_pend + i * cells;
+    a.h[i] = s->prm.h[i];
+    a.ih[i] = 1.0 / s->prm.h[i];
+  }
+  uint8_t* summ = s->bc_brick;
+  uint8_t* active = s->bc_brick + s->bc_cap;
+  a.active = active;
+  a.dim = dim;
+  const int g = s->nb;
+  k_bc_summary<<<g, 256, 0, s->st>>>(s->nb, s->flags, summ);
+  k_bc_active<<<(g + 255) / 256, 256, 0, s->st>>>(s->nb, s->nbr, summ, active);
+  k_bc_set_nan<<<g, 256, 0, s->st>>>(a);
+  k_bc_div<<<g, 256, 0, s->st>>>(a);
+  k_bc_plane<<<g, 256, 0, s->st>>>(a);
+  k_bc_apply<<<g, 256, 0, s->st>>>(a);
+  k_bc_corner45<<<g, 256, 0, s->st>>>(a);
+  if (dim == 3) {
+    k_bc_corner3<<<g, 256, 0, s->st>>>(a);
+    k_bc_apply<<<g, 256, 0, s->st>>>(a);
+  }
+  k_bc_mark<<<g, 256, 0, s->st>>>(a);
+  k_bc_nan_extrap<<<g, 256, 0, s->st>>>(a);
+  k_bc_apply<<<g, 256, 0, s->st>>>(a);
+  SCHK(hipGetLastError());
+  return 0;
+}
+
+// Algorithms_Extrapolate::Do on the face field (u, v, w) of the current
+// bricks, after the free-surface condition when surface_bc is set.
 int sim_extrapolate_fields(mac_sim* s, double* u, double* v, double* w) {
   if (s->nb == 0) return 0;
+  if (s->prm.surface_bc) SRC(sim_surface_bc(s, u, v, w, 3));
   const size_t cells = (size_t)s->nb * MAC_BC;
   if (s->front_cap < 2 * cells) {
     hipFree(s->d_front);
@@ -973,6 +1043,14 @@ int sim_box_delete(mac_sim* s) {
 int mac_sim_extrapolate(mac_sim* s, int slot) {
   if (!s || slot < 0 || slot > 2) return mac_internal_set_error(MAC_E_ARG, "bad arguments");
   return sim_extrapolate_fields(s, s->S[slot][0], s->S[slot][1], s->S[slot][2]);
+}
+
+int mac_sim_surface_bc(mac_sim* s, int slot, int dim) {
+  if (!s || slot < 0 || slot > 3 || (dim != 2 && dim != 3)) return mac_internal_set_error(MAC_E_ARG, "bad arguments");
+  double** f = slot == 3 ? s->A : s->S[slot];
+  SRC(sim_surface_bc(s, f[0], f[1], f[2], dim));
+  SCHK(hipStreamSynchronize(s->st));
+  return 0;
 }
 
 int mac_sim_set_guess(mac_sim* s, int mode, mac_unet* net, float out_scale) {

@@ -333,6 +333,7 @@ class SimParams(ctypes.Structure):
         ("inflow_speed", ctypes.c_double * 3),
         ("inflow_amp", ctypes.c_double),
         ("inflow_period", ctypes.c_int),
+        ("surface_bc", ctypes.c_int),
         ("proj", Params),
     ]
 
@@ -376,6 +377,7 @@ SIGNATURES.update({
     "mac_sim_interpolate": (ctypes.c_int, [_P, ctypes.c_int, _P, _P]),
     "mac_sim_step": (ctypes.c_int, [_P, ctypes.POINTER(SimStats)]),
     "mac_sim_extrapolate": (ctypes.c_int, [_P, ctypes.c_int]),
+    "mac_sim_surface_bc": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int]),
     "mac_sim_get_depth": (ctypes.c_int, [_P, _P]),
     "mac_sim_set_depth": (ctypes.c_int, [_P, _P]),
 })
@@ -500,8 +502,14 @@ class Sim:
         return st.as_dict()
 
     def extrapolate(self, slot=0):
-        """Algorithms_Extrapolate::Do on speed slot `slot`."""
+        """Algorithms_Extrapolate::Do on speed slot `slot` (with surface_bc set,
+        the free-surface velocity condition first)."""
         _check(self.lib.mac_sim_extrapolate(self.ptr, int(slot)))
+
+    def surface_bc(self, slot=0, dim=3):
+        """The free-surface velocity condition on speed slot `slot` (3 = the
+        acceleration), in 3-D or (dim=2) on the x-y plane."""
+        _check(self.lib.mac_sim_surface_bc(self.ptr, int(slot), int(dim)))
 
     def set_guess(self, mode, net=None, out_scale=1.0):
         """Projection starting iterate: GUESS_ZERO, GUESS_PREVIOUS or GUESS_LEARNED (with net)."""

@@ -211,7 +211,7 @@ class Sim:
     def __init__(self, h=(0.01, 0.01, 0.01), gravity=(0, 0, -9.81), viscosity=0.0, cfl=0.5, dt_min=0.0, dt_max=1e30,
                  layer_max=2, integrator=0, particle_rk=2, particles_per_inflow_cell=8, seed=0, box_lo=(0, 0, 0),
                  box_hi=(0, 0, 0), depth_interior=0, depth_remove=0, floor_axis=-1, floor_value=0.0, fill_cube_n=0,
-                 inflow_speed=(0.0, 0.0, 0.0), inflow_amp=0.0, inflow_period=1, proj=None, **_ignored):
+                 inflow_speed=(0.0, 0.0, 0.0), inflow_amp=0.0, inflow_period=1, surface_bc=0, proj=None, **_ignored):
         lib = load()
         if not hasattr(lib, "_sim_bound"):
             lib.orc_sim_create.restype = ctypes.c_void_p
@@ -242,7 +242,7 @@ class Sim:
                        + [floor_value, float((proj or {}).get("viscosity", 0.0))] + [float(x) for x in inflow_speed]
                        + [float(inflow_amp)], dtype=np.float64)
         iprm = np.array([layer_max, integrator, particle_rk, particles_per_inflow_cell, depth_interior, depth_remove,
-                         floor_axis, fill_cube_n, inflow_period], dtype=np.int32)
+                         floor_axis, fill_cube_n, inflow_period, surface_bc], dtype=np.int32)
         self.ptr = ctypes.c_void_p(lib.orc_sim_create(_p(prm), _p(iprm), seed))
 
     def __del__(self):

@@ -86,6 +86,7 @@ struct Params {
   double proj_nu;  // > 0: the viscous air pressures of the projection
   double inflow_speed[3], inflow_amp;  // the inflow pulsation (amp 0: off)
   int inflow_period;
+  int surface_bc;  // 1: the free-surface velocity condition before every extrapolation
   uint64_t seed;
   double lo[3], hi[3];
 };
@@ -727,7 +728,10 @@ struct Sim {
     corner_system(P.h, s1, s2, s3, U, res);
   }
 
+  // With surface_bc set, Algorithms<Algorithm_Extrapolate_Boundary_Condition,
+  // Algorithms_Extrapolate> as test/Jet3.cpp:541-543,636-639 chains them.
   void extrapolate(int slot) {
+    if (P.surface_bc) surface_bc(slot, 3);
     auto ok = [&](int x, int y, int z) { return valid(x, y, z) && layer_of(x, y, z) != -1; };
     auto U = [&](int comp, int x, int y, int z) { return field(chunk(x, y, z), slot, comp)[off_of(x, y, z)]; };
     auto set = [&](int x, int y, int z, int comp, double val) {
@@ -1171,7 +1175,7 @@ extern "C" {
 
 // prm: h[3], g[3], nu, cfl, dt_min, dt_max, lo[3], hi[3] (16 doubles);
 // iprm: layer_max, integrator, particle_rk, ppc, depth_interior, depth_remove,
-// floor_axis, fill_cube_n, inflow_period; prm[16] = floor_value, prm[17] =
+// floor_axis, fill_cube_n, inflow_period, surface_bc; prm[16] = floor_value, prm[17] =
 // projection viscosity, prm[18..20] = inflow_speed, prm[21] = inflow_amp; seed.
 void* orc_sim_create(const double* prm, const int* iprm, unsigned long long seed) {
   Sim* s = new Sim();
@@ -1197,6 +1201,7 @@ void* orc_sim_create(const double* prm, const int* iprm, unsigned long long seed
   s->P.proj_nu = prm[17];
   s->P.cube_n = iprm[7];
   s->P.inflow_period = iprm[8];
+  s->P.surface_bc = iprm[9];
   for (int i = 0; i < 3; ++i) s->P.inflow_speed[i] = prm[18 + i];
   s->P.inflow_amp = prm[21];
   s->P.seed = seed;

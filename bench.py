@@ -36,7 +36,7 @@ KERNEL_NAMES = {
     "jacobi2_l0": "k_jacobi2<double, {z}>", "jacobi_l0": "k_jacobi<double, {z}>", "residual_l0": "k_residual<double, {z}>",
     "prolong_l0": "k_prolong_col<{z}> (level 0)", "restrict_l1": "k_restrict_sep<{z}> (into level 1)",
     "coarse_jacobi2": "k_jacobi2<{z}, {z}> (levels 1-3)", "coarse_jacobi": "k_jacobi_block<{z}, {z}> (levels 1-3)",
-    "coarse_residual": "k_residual<{z}, {z}> (levels 1-3)", "coarse_restrict": "k_restrict_sep<{z}> (into levels 2-4)",
+    "coarse_residual": "k_residual<{z}, {z}> (levels 1-3)", "coarse_restrict": "k_restrict_sep<{z}> (coarse, unfused)",
     "coarse_prolong": "k_prolong_col<{z}> (levels 1-3)", "coarse_tail": "k_vcycle_tail<{z}> (levels 4-5)",
     "gradient_update": "k_gradient", "max_div": "k_maxdiv", "resid_init": "k_resid0",
     "prolong_jacobi_l0": "k_prolong_jacobi<{z}> (level 0)",
@@ -50,11 +50,17 @@ KERNEL_NAMES = {
 # seed 0, mixed method), over the dispatches that did work (a dispatch issued
 # after convergence returns at once): profiles/projection_256_mixed_hbm_traffic.md.
 # The coarse classes are the launch-weighted mean over their levels (the
-# block smoother: four launches per iteration on levels 1 and 2, two on 3).
-MEASURED_TRAFFIC = {("jacobi_l0", 256, 0): 133.16e6, ("cg_pupdate_matvec", 256, 0): 249.74e6, ("cg_update", 256, 0): 253.74e6,
-                    ("jacobi2_l0", 256, 0): 139.55e6, ("residual_l0", 256, 0): 133.02e6, ("prolong_l0", 256, 0): 66.07e6,
-                    ("restrict_l1", 256, 0): 84.86e6, ("coarse_jacobi", 256, 0): 7.69e6, ("coarse_residual", 256, 0): 7.09e6,
-                    ("coarse_prolong", 256, 0): 3.83e6, ("coarse_restrict", 256, 0): 5.90e6, ("coarse_tail", 256, 0): 0.07e6}
+# block smoother: two launches per iteration on each of levels 1 and 2; the
+# residual and the fused prolongation: one per level 1-3; the fused
+# restriction: one into each of levels 2 and 3).
+MEASURED_TRAFFIC = {("jacobi_l0", 256, 0): 133.17e6, ("cg_pupdate_matvec", 256, 0): 249.64e6, ("cg_update", 256, 0): 253.64e6,
+                    ("jacobi2_l0", 256, 0): 139.45e6, ("residual_l0", 256, 0): 133.01e6,
+                    ("prolong_jacobi_l0", 256, 0): 164.77e6, ("restrict_block_l1", 256, 0): 88.74e6,
+                    ("coarse_jacobi", 256, 0): 10.97e6, ("coarse_residual", 256, 0): 7.08e6,
+                    ("coarse_restrict_block", 256, 0): 9.10e6, ("coarse_prolong_block", 256, 0): 9.74e6,
+                    ("coarse_restrict", 256, 0): 0.44e6, ("coarse_tail", 256, 0): 0.07e6,
+                    ("divergence", 256, 0): 234.01e6, ("gradient_update", 256, 0): 388.00e6, ("max_div", 256, 0): 184.40e6,
+                    ("resid_init", 256, 0): 171.45e6}
 
 
 def log(*a):

@@ -418,6 +418,11 @@ int mac_unet_train_step(mac_unet* net, const float* in5, const float* target, co
 /* Event-timed forward convolutions: returns the time and FLOPs accumulated
  * since the previous call, then switches timing on or off. */
 int mac_unet_conv_timing(mac_unet* net, int on, double* ms, double* flop);
+/* Weight gradients of the 3x3x3 convolutions: mode 0 (default) on the BF16
+ * matrix cores (implicit GEMM over the voxels, FP32 accumulate; network
+ * volume at most 128 wide), mode 1 on FP32 FMA (the same sums in another
+ * order, kept to check the first). */
+int mac_unet_wgrad_mode(mac_unet* net, int mode);
 
 /* The network on a projection context.  The grid's bounding box (chunk keys
  * shifted to a minimum of 0 per axis) is mapped to the network volume, cell

@@ -75,6 +75,7 @@ struct mac_unet {
   hipEvent_t evx[2] = {nullptr, nullptr};  // hand-over to and from a projection stream
   double conv_ms = 0.0, conv_flop = 0.0;
   bool time_convs = false;
+  bool wgrad_fp32 = false;  // weight gradients on FP32 FMA instead of the matrix cores
 };
 
 namespace {
@@ -180,9 +181,37 @@ int conv_wgrad(mac_unet* u, int l, const bf16* g, int g_cs, int g_co, const bf16
   const int nchunk = (nrows + rpc - 1) / rpc;
   const size_t need = (size_t)nchunk * 27 * c.Cout * c.Cin;
   if (need > u->partial_n) return mac_internal_set_error(MAC_E_ARG, "weight-gradient scratch too small");
-  k_conv3_wgrad<<<27 * nchunk, 256, 0, u->st>>>(g, g_cs, g_co, c.Cout, x, x_cs, x_co, c.Cin, vol, rpc, u->partial);
+  if (u->wgrad_fp32) {
+    k_conv3_wgrad<<<27 * nchunk, 256, 0, u->st>>>(g, g_cs, g_co, c.Cout, x, x_cs, x_co, c.Cin, vol, rpc, u->partial);
+  } else {
+    if (vol.W > WG_MAXW || c.Cout % 16 || c.Cin % 8 || c.Cin > 64 || c.Cout + ((c.Cin + 15) & ~15) > 128)
+      return mac_internal_set_error(MAC_E_ARG, "weight gradient on the matrix cores: volume wider than 128 voxels");
+    k_conv3_wgrad_mfma<<<27 * nchunk, 256, 0, u->st>>>(g, g_cs, g_co, c.Cout, x, x_cs, x_co, c.Cin, vol, rpc, u->partial);
+  }
   k_wgrad_reduce<<<nblk(27 * c.Cout * c.Cin), 256, 0, u->st>>>(nchunk, c.Cout, c.Cin, u->partial, u->grads + c.w_off);
   k_bias_grad<<<c.Cout, 256, 0, u->st>>>(g, g_cs, g_co, (size_t)vol.D * vol.H * vol.W, u->grads + c.b_off);
+  UCHK(hipGetLastError());
+  return 0;
+}
+
+// weight and bias gradients of a transposed convolution (its output gradient
+// g: the first Cout channels of a g_cs-channel fine tensor; input x: the
+// coarse tensor, Cin channels)
+int up_wgrad(mac_unet* u, const Up& up, const bf16* g, int g_cs, Vol vc, const bf16* x) {
+  if (u->wgrad_fp32) {
+    k_up_wgrad<<<nblk(8 * up.Cout * up.Cin + up.Cout), 256, 0, u->st>>>(g, g_cs, 0, up.Cout, vc, x, up.Cin, u->grads + up.w_off,
+                                                                      u->grads + up.b_off);
+  } else {
+    if (vc.W > WG_MAXW || up.Cout % 16 || up.Cin % 16 || up.Cout + up.Cin > 128)
+      return mac_internal_set_error(MAC_E_ARG, "transposed-convolution weight gradient on the matrix cores: coarse volume wider than 128 voxels");
+    const int nrows = vc.D * vc.H;
+    const int rpc = std::max(1, nrows / 256);
+    const int nchunk = (nrows + rpc - 1) / rpc;
+    if ((size_t)nchunk * 8 * up.Cout * up.Cin > u->partial_n) return mac_internal_set_error(MAC_E_ARG, "weight-gradient scratch too small");
+    k_up_wgrad_mfma<<<8 * nchunk, 256, 0, u->st>>>(g, g_cs, 0, up.Cout, vc, x, up.Cin, rpc, u->partial);
+    k_up_wgrad_reduce<<<nblk(8 * up.Cout * up.Cin), 256, 0, u->st>>>(nchunk, up.Cout, up.Cin, u->partial, u->grads + up.w_off);
+    k_bias_grad<<<up.Cout, 256, 0, u->st>>>(g, g_cs, 0, (size_t)vc.D * vc.H * vc.W * 8, u->grads + up.b_off);
+  }
   UCHK(hipGetLastError());
   return 0;
 }
@@ -241,8 +270,7 @@ int backward(mac_unet* u) {
   conv_bwd_data(u, DEC1A, u->gD1a, 16, 0, v1, u->gC1, 32, 0);
   // up1 (U1 = C1[:, 0:16])
   relu_mask(u, n1, 16, u->gC1, 32, 0, u->C1, 32, 0);
-  k_up_wgrad<<<nblk(8 * 16 * 32 + 16), 256, 0, u->st>>>(u->gC1, 32, 0, 16, v2, u->D2, 32, u->grads + u->up1.w_off,
-                                                           u->grads + u->up1.b_off);
+  URC(up_wgrad(u, u->up1, u->gC1, 32, v2, u->D2));
   k_up_bwd_data<<<nblk(n2 * 32), 256, 0, u->st>>>(u->gC1, 32, 0, 16, v2, u->params + u->up1.w_off, 32, u->gD2);
   // dec2b, dec2a
   relu_mask(u, n2, 32, u->gD2, 32, 0, u->D2, 32, 0);
@@ -253,8 +281,7 @@ int backward(mac_unet* u) {
   conv_bwd_data(u, DEC2A, u->gD2a, 32, 0, v2, u->gC2, 64, 0);
   // up2 (U2 = C2[:, 0:32])
   relu_mask(u, n2, 32, u->gC2, 64, 0, u->C2, 64, 0);
-  k_up_wgrad<<<nblk(8 * 32 * 64 + 32), 256, 0, u->st>>>(u->gC2, 64, 0, 32, v3, u->E3, 64, u->grads + u->up2.w_off,
-                                                           u->grads + u->up2.b_off);
+  URC(up_wgrad(u, u->up2, u->gC2, 64, v3, u->E3));
   k_up_bwd_data<<<nblk(n3 * 64), 256, 0, u->st>>>(u->gC2, 64, 0, 32, v3, u->params + u->up2.w_off, 64, u->gE3);
   // enc3b, enc3a
   relu_mask(u, n3, 64, u->gE3, 64, 0, u->E3, 64, 0);
@@ -589,6 +616,12 @@ int mac_unet_train_step(mac_unet* u, const float* in5, const float* target, cons
 }
 
 // Forward-convolution timing (for the matrix-core utilisation figure).
+int mac_unet_wgrad_mode(mac_unet* u, int mode) {
+  if (!u || (mode != 0 && mode != 1)) return mac_internal_set_error(MAC_E_ARG, "bad arguments");
+  u->wgrad_fp32 = mode == 1;
+  return 0;
+}
+
 int mac_unet_conv_timing(mac_unet* u, int on, double* ms, double* flop) {
   if (!u) return mac_internal_set_error(MAC_E_ARG, "null argument");
   if (ms) *ms = u->conv_ms;

@@ -236,6 +236,166 @@ __global__ void __launch_bounds__(256) k_conv3_wgrad(const bf16* __restrict__ G,
   }
 }
 
+// Weight gradient of a 3x3x3 convolution on the BF16 matrix cores:
+//   dW_tap[co][ci] = sum_v G[v][co] X[v + off(tap)][ci]
+// one GEMM per tap with M = Cout, N = Cin, K = voxels, 32 x-voxels of a row
+// per v_mfma_f32_16x16x32_bf16.  Block = (tap, chunk of (z, y) rows).  Per
+// row the G row and the tap's shifted X row are staged in LDS channel-major
+// ([c][x], row stride S = Wp + 8 so that the 16 rows of a fragment start in
+// different banks; zero beyond the volume and in padded channels), so an
+// operand fragment -- one channel, 8 consecutive voxels -- is one 16-byte LDS
+// read: lane l supplies A row co0 + l % 16 and B column ci0 + l % 16 for the
+// voxels kc + 8 (l / 16) .. + 7, and holds C[co0 + 4 (l / 16) + j][ci0 + l % 16].
+// Wave w takes the 16 x 16 (co, ci) tiles w, w + 4, ...  Products of BF16
+// values are exact in FP32 and the sums run in a fixed order, so the result
+// is reproducible.  Each block writes its partial, summed in chunk order by
+// k_wgrad_reduce.  W <= WG_MAXW, Cout a multiple of 16, Cin a multiple of 8
+// and at most 64.
+constexpr int WG_MAXW = 128;
+constexpr int WG_S = WG_MAXW + 8;
+__global__ void __launch_bounds__(256) k_conv3_wgrad_mfma(const bf16* __restrict__ G, int g_cs, int g_co, int Cout,
+                                                          const bf16* __restrict__ X, int x_cs, int x_co, int Cin, Vol v,
+                                                          int rows_per_chunk, float* __restrict__ partial) {
+  __shared__ __align__(16) bf16 lds[128 * WG_S];  // [Cout][S] then [CinP][S]
+  const int tap = blockIdx.x % 27, chunk = blockIdx.x / 27;
+  const int dz = tap / 9 - 1, dy = (tap / 3) % 3 - 1, dx = tap % 3 - 1;
+  const int Wp = (v.W + 31) & ~31, S = Wp + 8;
+  const int nci = (Cin + 15) >> 4, CinP = nci << 4, nco = Cout >> 4;
+  const int ntiles = nco * nci;
+  bf16* gs = lds;
+  bf16* xs = lds + Cout * S;
+  for (int e = threadIdx.x; e < (Cout + CinP) * S; e += blockDim.x) lds[e] = (bf16)0.0f;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int i = lane & 15, g = lane >> 4;
+  floatx4 acc[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) acc[k] = floatx4{0.f, 0.f, 0.f, 0.f};
+  const int nrows = v.D * v.H;
+  const int r0 = chunk * rows_per_chunk, r1 = min(nrows, r0 + rows_per_chunk);
+  const int c8o = Cout >> 3, c8i = Cin >> 3;
+  for (int row = r0; row < r1; ++row) {
+    const int y = row % v.H, z = row / v.H;
+    const int yy = y + dy, zz = z + dz;
+    const bool rowok = (yy >= 0 && yy < v.H && zz >= 0 && zz < v.D);
+    __syncthreads();
+    for (int e = threadIdx.x; e < v.W * c8o; e += blockDim.x) {
+      const int x = e / c8o, c8 = e % c8o;
+      const bf16x8 val = *(const bf16x8*)(G + ((size_t)(z * v.H + y) * v.W + x) * g_cs + g_co + 8 * c8);
+#pragma unroll
+      for (int j = 0; j < 8; ++j) gs[(8 * c8 + j) * S + x] = val[j];
+    }
+    for (int e = threadIdx.x; e < v.W * c8i; e += blockDim.x) {
+      const int x = e / c8i, c8 = e % c8i;
+      const int xx = x + dx;
+      bf16x8 val;
+#pragma unroll
+      for (int j = 0; j < 8; ++j) val[j] = (bf16)0.0f;
+      if (rowok && xx >= 0 && xx < v.W) val = *(const bf16x8*)(X + ((size_t)(zz * v.H + yy) * v.W + xx) * x_cs + x_co + 8 * c8);
+#pragma unroll
+      for (int j = 0; j < 8; ++j) xs[(8 * c8 + j) * S + x] = val[j];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const int t = w + 4 * k;
+      if (t >= ntiles) break;
+      const int co0 = (t / nci) << 4, ci0 = (t % nci) << 4;
+      const bf16* ap = gs + (co0 + i) * S + 8 * g;
+      const bf16* bp = xs + (ci0 + i) * S + 8 * g;
+      for (int kc = 0; kc < Wp; kc += 32)
+        acc[k] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(*(const bf16x8*)(ap + kc), *(const bf16x8*)(bp + kc), acc[k], 0, 0, 0);
+    }
+  }
+  float* P = partial + ((size_t)chunk * 27 + tap) * Cout * Cin;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const int t = w + 4 * k;
+    if (t >= ntiles) break;
+    const int co0 = (t / nci) << 4, ci = ((t % nci) << 4) + i;
+    if (ci < Cin) {
+#pragma unroll
+      for (int j = 0; j < 4; ++j) P[(size_t)(co0 + 4 * g + j) * Cin + ci] = acc[k][j];
+    }
+  }
+}
+
+// Weight gradient of the 2x2x2 transposed convolution on the matrix cores:
+//   dW[o][co][ci] = sum over coarse voxels v of G[2v + o][co] X[v][ci]
+// one GEMM per octant o (M = Cout, N = Cin, K = coarse voxels, 32 x-voxels of
+// a coarse row per v_mfma_f32_16x16x32_bf16), staged and tiled as in
+// k_conv3_wgrad_mfma.  Block = (octant, chunk of coarse (z, y) rows); X is
+// channels-last with Cin channels.  Partials [chunk][o][co][ci], summed in
+// chunk order by k_up_wgrad_reduce.  vc.W <= WG_MAXW; Cout and Cin multiples
+// of 16 with Cout + Cin <= 128.
+__global__ void __launch_bounds__(256) k_up_wgrad_mfma(const bf16* __restrict__ G, int g_cs, int g_co, int Cout, Vol vc,
+                                                       const bf16* __restrict__ X, int Cin, int rows_per_chunk,
+                                                       float* __restrict__ partial) {
+  __shared__ __align__(16) bf16 lds[128 * WG_S];
+  const int o = blockIdx.x & 7, chunk = blockIdx.x >> 3;
+  const int ox = o & 1, oy = (o >> 1) & 1, oz = o >> 2;
+  const int Wp = (vc.W + 31) & ~31, S = Wp + 8;
+  const int nci = Cin >> 4, nco = Cout >> 4, ntiles = nco * nci;
+  const int Hf = 2 * vc.H, Wf = 2 * vc.W;
+  bf16* gs = lds;
+  bf16* xs = lds + Cout * S;
+  for (int e = threadIdx.x; e < (Cout + Cin) * S; e += blockDim.x) lds[e] = (bf16)0.0f;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int i = lane & 15, g = lane >> 4;
+  floatx4 acc[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) acc[k] = floatx4{0.f, 0.f, 0.f, 0.f};
+  const int nrows = vc.D * vc.H;
+  const int r0 = chunk * rows_per_chunk, r1 = min(nrows, r0 + rows_per_chunk);
+  const int c8o = Cout >> 3, c8i = Cin >> 3;
+  for (int row = r0; row < r1; ++row) {
+    const int y = row % vc.H, z = row / vc.H;
+    __syncthreads();
+    for (int e = threadIdx.x; e < vc.W * c8o; e += blockDim.x) {
+      const int x = e / c8o, c8 = e % c8o;
+      const bf16x8 val =
+          *(const bf16x8*)(G + ((size_t)((2 * z + oz) * Hf + 2 * y + oy) * Wf + 2 * x + ox) * g_cs + g_co + 8 * c8);
+#pragma unroll
+      for (int j = 0; j < 8; ++j) gs[(8 * c8 + j) * S + x] = val[j];
+    }
+    for (int e = threadIdx.x; e < vc.W * c8i; e += blockDim.x) {
+      const int x = e / c8i, c8 = e % c8i;
+      const bf16x8 val = *(const bf16x8*)(X + ((size_t)(z * vc.H + y) * vc.W + x) * Cin + 8 * c8);
+#pragma unroll
+      for (int j = 0; j < 8; ++j) xs[(8 * c8 + j) * S + x] = val[j];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const int t = w + 4 * k;
+      if (t >= ntiles) break;
+      const int co0 = (t / nci) << 4, ci0 = (t % nci) << 4;
+      const bf16* ap = gs + (co0 + i) * S + 8 * g;
+      const bf16* bp = xs + (ci0 + i) * S + 8 * g;
+      for (int kc = 0; kc < Wp; kc += 32)
+        acc[k] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(*(const bf16x8*)(ap + kc), *(const bf16x8*)(bp + kc), acc[k], 0, 0, 0);
+    }
+  }
+  float* P = partial + ((size_t)chunk * 8 + o) * Cout * Cin;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const int t = w + 4 * k;
+    if (t >= ntiles) break;
+    const int co0 = (t / nci) << 4, ci = ((t % nci) << 4) + i;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) P[(size_t)(co0 + 4 * g + j) * Cin + ci] = acc[k][j];
+  }
+}
+
+// dW[o][co][ci] = sum over chunks (in order) of partial[chunk][o][co][ci].
+__global__ void k_up_wgrad_reduce(int nchunk, int Cout, int Cin, const float* __restrict__ partial, float* __restrict__ dw) {
+  const int n = 8 * Cout * Cin;
+  for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x) {
+    float s = 0.f;
+    for (int c = 0; c < nchunk; ++c) s += partial[(size_t)c * n + e];
+    dw[e] = s;
+  }
+}
+
 // dW[co][tap*Cin+ci] = sum over chunks (in order) of partial[chunk][tap][co][ci].
 __global__ void k_wgrad_reduce(int nchunk, int Cout, int Cin, const float* __restrict__ partial, float* __restrict__ dw) {
   const int n = 27 * Cout * Cin;

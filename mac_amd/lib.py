@@ -528,6 +528,7 @@ SIGNATURES.update({
     "mac_unet_backward": (ctypes.c_int, [_P, _P, _P]),
     "mac_unet_train_step": (ctypes.c_int, [_P, _P, _P, _P, ctypes.c_float, ctypes.POINTER(ctypes.c_float)]),
     "mac_unet_conv_timing": (ctypes.c_int, [_P, ctypes.c_int, _dp, _dp]),
+    "mac_unet_wgrad_mode": (ctypes.c_int, [_P, ctypes.c_int]),
     "mac_unet_guess": (ctypes.c_int, [_P, _P, ctypes.c_int, ctypes.c_float]),
     "mac_unet_sample": (ctypes.c_int, [_P, _P, _P, ctypes.c_float, _P, _P, _P]),
     "mac_sim_set_guess": (ctypes.c_int, [_P, ctypes.c_int, _P, ctypes.c_float]),
@@ -608,6 +609,10 @@ class UNet:
         x5, t, m = self._pair_buffers()
         _check(self.lib.mac_sim_capture(sim.ptr, self.ptr, float(out_scale), _ptr(x5), _ptr(t), _ptr(m)))
         return x5, t, m
+
+    def wgrad_mode(self, mode):
+        """0: weight gradients on the matrix cores (default); 1: on FP32 FMA."""
+        _check(self.lib.mac_unet_wgrad_mode(self.ptr, int(mode)))
 
     def conv_timing(self, on):
         ms, fl = ctypes.c_double(), ctypes.c_double()

@@ -161,37 +161,44 @@ def forward(p, x5, shape, bf=False):
     return out.reshape(-1), c
 
 
-def backward(p, cache, gout, shape):
-    """FP64 gradients of sum(out * gout) w.r.t. the flat parameters."""
+def backward(p, cache, gout, shape, bf=False):
+    """FP64 gradients of sum(out * gout) w.r.t. the flat parameters.  With
+    bf=True every stored activation gradient is rounded to bfloat16 where the
+    device stores one (the head's output gradient, each data gradient of a
+    convolution or transposed convolution, each sum of a skip gradient and a
+    pooling gradient), and the convolutions' data gradients use the bfloat16
+    weights, as the device does; parameter gradients are FP64 sums."""
     P = unpack(p)
     D, H, W = shape
     c = cache
     G = {}
+    r = bf16 if bf else (lambda a: a)
+    wr = bf16 if bf else (lambda a: a)
     go = np.asarray(gout, np.float64).reshape(D, H, W)
     G["head.w"] = np.einsum("zyx,zyxc->c", go, c["d1"])
     G["head.b"] = np.array([go.sum()])
-    g = go[..., None] * P["head.w"]
+    g = r(go[..., None] * P["head.w"])
 
     def cvb(name, x, y, g, need_dx=True):
         g = g * (y > 0)
-        dx, dw, db = conv3_bwd(x, P[name + ".w"], g)
+        dx, dw, db = conv3_bwd(x, wr(P[name + ".w"]), g)
         G[name + ".w"], G[name + ".b"] = dw, db
-        return dx
+        return r(dx)
 
     g = cvb("dec1b", c["d1a"], c["d1"], g)
     gc1 = cvb("dec1a", c["c1"], c["d1a"], g)
     gu1 = gc1[..., :16] * (c["u1"] > 0)
     gd2, G["up1.w"], G["up1.b"] = up_bwd(c["d2"], P["up1.w"], gu1)
-    g = cvb("dec2b", c["d2a"], c["d2"], gd2)
+    g = cvb("dec2b", c["d2a"], c["d2"], r(gd2))
     gc2 = cvb("dec2a", c["c2"], c["d2a"], g)
     gu2 = gc2[..., :32] * (c["u2"] > 0)
     ge3, G["up2.w"], G["up2.b"] = up_bwd(c["e3"], P["up2.w"], gu2)
-    g = cvb("enc3b", c["e3a"], c["e3"], ge3)
+    g = cvb("enc3b", c["e3a"], c["e3"], r(ge3))
     gp2 = cvb("enc3a", c["p2"], c["e3a"], g)
-    ge2 = gc2[..., 32:] + pool_bwd(gp2)
+    ge2 = r(gc2[..., 32:] + pool_bwd(gp2))
     g = cvb("enc2b", c["e2a"], c["e2"], ge2)
     gp1 = cvb("enc2a", c["p1"], c["e2a"], g)
-    ge1 = gc1[..., 16:] + pool_bwd(gp1)
+    ge1 = r(gc1[..., 16:] + pool_bwd(gp1))
     g = cvb("enc1b", c["e1a"], c["e1"], ge1)
     cvb("enc1a", c["x0"], c["e1a"], g)
     return pack(G)

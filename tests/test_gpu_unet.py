@@ -1,13 +1,17 @@
 """GPU checks of the learned-guess U-Net (BF16 matrix-core convolutions, FP32
-accumulation) against the FP64 numpy oracle.
+accumulation) against the FP64 numpy oracle run with the device's bfloat16
+rounding of stored activations, activation gradients and convolution weights.
 
-Bars: the forward output within 2e-2 relative (L2) of the oracle run with the
-same bfloat16 rounding of stored activations and convolution weights; every
-parameter gradient within 1e-1 relative (L2) and cosine >= 0.995 of the
-oracle's FP64 gradient on the same stored activations (the device keeps
-activation gradients in bfloat16: about 2^-9 relative rounding per stored
-tensor, compounded over the ~20 tensors between the loss and the first
-layer)."""
+Bars: the forward output within 5e-3 relative (L2) of the oracle (measured
+4e-4 to 1.4e-3).  Every parameter gradient within 1e-2 relative (L2) and
+cosine >= 0.9999 of the oracle's on the same input (measured at most 1.5e-3)
+for two network inits; for a third init (seed 10 on the 8 x 8 x 64 volume)
+within 1e-1 and cosine >= 0.995 (measured 8.5e-2): that init leaves many
+activations near zero, whose ReLU masks flip between the device's FP32
+forward and the oracle's FP64 one -- the head-weight gradient, which reads
+only forward activations, already differs by 4e-3 there.  The weight
+gradients on the matrix cores against the FP32 FMA kernel on the same stored
+tensors: 1e-5."""
 import numpy as np
 import pytest
 
@@ -35,11 +39,13 @@ def test_forward_matches_oracle(seed, shape):
     out = net.forward(x5)
     ref, _ = U.forward(p, x5, shape, bf=True)
     err = np.linalg.norm(out - ref) / np.linalg.norm(ref)
-    assert err < 2e-2, err
+    print(f"[unet] forward {shape} seed {seed}: rel L2 {err:.2e}", flush=True)
+    assert err < 5e-3, err
 
 
-@pytest.mark.parametrize("seed,shape", [(0, SHAPE), (1, SHAPE), (2, SHAPE_TILED)])
-def test_backward_matches_oracle(seed, shape):
+@pytest.mark.parametrize("seed,shape,bar,cosbar", [(0, SHAPE, 1e-1, 0.995), (1, SHAPE, 1e-2, 0.9999),
+                                                   (2, SHAPE_TILED, 1e-2, 0.9999)])
+def test_backward_matches_oracle(seed, shape, bar, cosbar):
     net = UNet(*shape, seed=seed + 10)
     p = net.get_params().astype(np.float64)
     x5, rng = _inputs(seed, shape)
@@ -47,7 +53,7 @@ def test_backward_matches_oracle(seed, shape):
     gout = rng.normal(size=x5.shape[0]).astype(np.float32)
     g = net.backward(gout)
     _, cache = U.forward(p, x5, shape, bf=True)
-    ref = U.backward(p, cache, gout.astype(np.float64), shape)
+    ref = U.backward(p, cache, gout.astype(np.float64), shape, bf=True)
     lay, _ = U.layout()
     for k, (o, s) in lay.items():
         n = int(np.prod(s))
@@ -58,7 +64,8 @@ def test_backward_matches_oracle(seed, shape):
             continue
         rel = np.linalg.norm(a - b) / nb
         cos = a @ b / (np.linalg.norm(a) * nb)
-        assert rel < 1e-1 and cos > 0.995, (k, rel, cos)
+        print(f"[unet] grad {k} {shape} seed {seed}: rel L2 {rel:.2e} cos {cos:.6f}", flush=True)
+        assert rel < bar and cos > cosbar, (k, rel, cos)
 
 
 def test_training_reduces_loss():
@@ -71,3 +78,31 @@ def test_training_reduces_loss():
     for _ in range(60):
         l1 = net.train_step(x5, tgt, mask, 1e-3)
     assert l1 < 0.5 * l0, (l0, l1)
+
+
+@pytest.mark.parametrize("seed,shape", [(4, SHAPE), (5, SHAPE_TILED)])
+def test_wgrad_matrix_cores_match_fp32(seed, shape):
+    """The weight gradients on the matrix cores (mode 0) against the FP32 FMA
+    kernel (mode 1) on the same stored activations and output gradient: both
+    sum exact FP32 products of the same BF16 values, in different orders, so
+    every tensor agrees to 1e-5 relative; the data gradients, which neither
+    mode touches, agree exactly."""
+    net = UNet(*shape, seed=seed)
+    x5, rng = _inputs(seed, shape)
+    gout = rng.normal(size=x5.shape[0]).astype(np.float32)
+    net.forward(x5)
+    g_mfma = net.backward(gout).copy()
+    net.wgrad_mode(1)
+    net.forward(x5)
+    g_fma = net.backward(gout).copy()
+    net.wgrad_mode(0)
+    lay, _ = U.layout()
+    worst = 0.0
+    for k, (o, s) in lay.items():
+        n = int(np.prod(s))
+        a, b = g_mfma[o:o + n].astype(np.float64), g_fma[o:o + n].astype(np.float64)
+        nb = np.linalg.norm(b)
+        rel = np.linalg.norm(a - b) / nb if nb > 0 else np.linalg.norm(a)
+        worst = max(worst, rel)
+        assert rel < 1e-5, (k, rel)
+    print(f"[wgrad] matrix cores vs FP32 FMA: worst tensor {worst:.2e}", flush=True)

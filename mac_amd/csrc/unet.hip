@@ -220,14 +220,17 @@ void relu_mask(mac_unet* u, size_t nvox, int C, bf16* g, int g_cs, int g_co, con
   k_relu_mask<<<nblk(nvox * C), 256, 0, u->st>>>(nvox, C, g, g_cs, g_co, y, y_cs, y_co);
 }
 
-int forward(mac_unet* u) {
-  const Vol v1 = u->v1, v2 = u->v2, v3 = u->v3;
+// The forward pass over the volume v1 (the network's own, or a smaller one
+// stored contiguously in the same buffers: D and H multiples of 16, W of 64).
+int forward(mac_unet* u, Vol v1) {
+  const Vol v2 = {v1.D / 2, v1.H / 2, v1.W / 2}, v3 = {v1.D / 4, v1.H / 4, v1.W / 4};
+  const size_t n1 = (size_t)v1.D * v1.H * v1.W, n2 = n1 / 8, n3 = n1 / 64;
   conv_fwd(u, ENC1A, u->X0, 8, 0, v1, u->E1a, 16, 0, 1);
   conv_fwd(u, ENC1B, u->E1a, 16, 0, v1, u->C1, 32, 16, 1);
-  k_pool_fwd<<<nblk(u->n2 * 16), 256, 0, u->st>>>(u->C1, 32, 16, 16, v1, u->P1);
+  k_pool_fwd<<<nblk(n2 * 16), 256, 0, u->st>>>(u->C1, 32, 16, 16, v1, u->P1);
   conv_fwd(u, ENC2A, u->P1, 16, 0, v2, u->E2a, 32, 0, 1);
   conv_fwd(u, ENC2B, u->E2a, 32, 0, v2, u->C2, 64, 32, 1);
-  k_pool_fwd<<<nblk(u->n3 * 32), 256, 0, u->st>>>(u->C2, 64, 32, 32, v2, u->P2);
+  k_pool_fwd<<<nblk(n3 * 32), 256, 0, u->st>>>(u->C2, 64, 32, 32, v2, u->P2);
   conv_fwd(u, ENC3A, u->P2, 32, 0, v3, u->E3a, 64, 0, 1);
   conv_fwd(u, ENC3B, u->E3a, 64, 0, v3, u->E3, 64, 0, 1);
   k_up_fwd_mfma<2><<<(v3.D * v3.H * (v3.W / 16) + 3) / 4, 256, 0, u->st>>>(u->E3, 64, v3, u->up2.wb, u->params + u->up2.b_off, u->C2,
@@ -238,7 +241,7 @@ int forward(mac_unet* u) {
                                                                            32, 0);
   conv_fwd(u, DEC1A, u->C1, 32, 0, v1, u->D1a, 16, 0, 1);
   conv_fwd(u, DEC1B, u->D1a, 16, 0, v1, u->D1, 16, 0, 1);
-  k_head_fwd<<<nblk(u->n1), 256, 0, u->st>>>(u->D1, 16, u->n1, u->params + u->head_w, u->params + u->head_b, u->out);
+  k_head_fwd<<<nblk(n1), 256, 0, u->st>>>(u->D1, 16, n1, u->params + u->head_w, u->params + u->head_b, u->out);
   UCHK(hipGetLastError());
   return 0;
 }
@@ -316,12 +319,18 @@ int upload_input(mac_unet* u, const float* in5) {
   return 0;
 }
 
-// The grid of a projection context as the network's volume; fails when the
-// grid's bounding box does not fit.
-int guess_grid(mac_unet* u, mac_ctx* c, GuessGrid* g, mac_grid_view* v) {
+// The grid of a projection context as a volume of the network: its own
+// volume (whole = true, the layout of training pairs) or the grid's bounding
+// box rounded up to what the kernels take (D and H to 16, W to 64; inference
+// then costs what the grid spans, not the network's volume); fails when the
+// bounding box does not fit.
+int guess_grid(mac_unet* u, mac_ctx* c, GuessGrid* g, mac_grid_view* v, bool whole) {
   URC(mac_internal_view(c, v));
   if (8 * v->ext[0] > u->v1.W || 8 * v->ext[1] > u->v1.H || 8 * v->ext[2] > u->v1.D)
     return mac_internal_set_error(MAC_E_ARG, "grid bounding box larger than the network volume");
+  auto up = [](int n, int m, int cap) { return std::min(cap, (n + m - 1) / m * m); };
+  Vol a = u->v1;
+  if (!whole) a = {up(8 * v->ext[2], 16, u->v1.D), up(8 * v->ext[1], 16, u->v1.H), up(8 * v->ext[0], 64, u->v1.W)};
   g->nb = v->nb;
   g->keys = v->dkeys;
   g->flags = v->flags;
@@ -329,9 +338,9 @@ int guess_grid(mac_unet* u, mac_ctx* c, GuessGrid* g, mac_grid_view* v) {
   g->ff = v->ff;
   g->ndom = (double)v->ndom;
   g->ih2m = v->ih2m;
-  g->D = u->v1.D;
-  g->H = u->v1.H;
-  g->W = u->v1.W;
+  g->D = a.D;
+  g->H = a.H;
+  g->W = a.W;
   return 0;
 }
 
@@ -340,9 +349,10 @@ int guess_grid(mac_unet* u, mac_ctx* c, GuessGrid* g, mac_grid_view* v) {
 int guess_input(mac_unet* u, const GuessGrid& g, const mac_grid_view& v, const double* prev) {
   UCHK(hipEventRecord(u->evx[0], v.st));
   UCHK(hipStreamWaitEvent(u->st, u->evx[0], 0));
-  k_guess_fill<<<nblk(u->n1 * 5), 256, 0, u->st>>>(u->n1, u->in5);
+  const size_t n1 = (size_t)g.D * g.H * g.W;
+  k_guess_fill<<<nblk(n1 * 5), 256, 0, u->st>>>(n1, u->in5);
   k_guess_pack<<<nblk((size_t)g.nb * 512), 256, 0, u->st>>>(g, v.f, prev, u->in5);
-  k_pack_input<<<nblk(u->n1 * 8), 256, 0, u->st>>>(u->n1, u->in5, u->X0);
+  k_pack_input<<<nblk(n1 * 8), 256, 0, u->st>>>(n1, u->in5, u->X0);
   UCHK(hipGetLastError());
   return 0;
 }
@@ -356,10 +366,10 @@ MAC_HIDDEN int mac_internal_unet_guess(mac_unet* u, mac_ctx* c, const double* pr
   if (!(out_scale > 0.f)) return mac_internal_set_error(MAC_E_ARG, "out_scale must be positive");
   GuessGrid g;
   mac_grid_view v;
-  URC(guess_grid(u, c, &g, &v));
+  URC(guess_grid(u, c, &g, &v, false));
   URC(mac_internal_rhs(c));
   URC(guess_input(u, g, v, prev));
-  URC(forward(u));
+  URC(forward(u, Vol{g.D, g.H, g.W}));
   k_guess_unpack<<<nblk((size_t)g.nb * 512), 256, 0, u->st>>>(g, u->out, out_scale, v.x);
   UCHK(hipGetLastError());
   UCHK(hipEventRecord(u->evx[1], u->st));
@@ -382,7 +392,7 @@ MAC_HIDDEN int mac_internal_unet_sample(mac_unet* u, mac_ctx* c, const double* p
   if (!(out_scale > 0.f)) return mac_internal_set_error(MAC_E_ARG, "out_scale must be positive");
   GuessGrid g;
   mac_grid_view v;
-  URC(guess_grid(u, c, &g, &v));
+  URC(guess_grid(u, c, &g, &v, true));
   URC(guess_input(u, g, v, prev));
   UCHK(hipMemsetAsync(u->tgt, 0, sizeof(float) * u->n1, u->st));
   UCHK(hipMemsetAsync(u->msk, 0, sizeof(float) * u->n1, u->st));
@@ -574,7 +584,7 @@ int mac_unet_set_params(mac_unet* u, const float* in) {
 int mac_unet_forward(mac_unet* u, const float* in5, float* out) {
   if (!u || !in5) return mac_internal_set_error(MAC_E_ARG, "null argument");
   URC(upload_input(u, in5));
-  URC(forward(u));
+  URC(forward(u, u->v1));
   if (out) UCHK(hipMemcpyAsync(out, u->out, sizeof(float) * u->n1, hipMemcpyDeviceToHost, u->st));
   UCHK(hipStreamSynchronize(u->st));
   return 0;
@@ -597,7 +607,7 @@ int mac_unet_train_step(mac_unet* u, const float* in5, const float* target, cons
   double msum = 0;
   for (size_t i = 0; i < u->n1; ++i) msum += mask[i];
   if (msum <= 0) return 0;
-  URC(forward(u));
+  URC(forward(u, u->v1));
   const int nb = 256;
   k_mse<<<nb, 256, 0, u->st>>>(u->n1, u->out, u->tgt, u->msk, (float)(1.0 / msum), u->gout, u->hpart);
   std::vector<float> part(nb);

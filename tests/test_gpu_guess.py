@@ -89,6 +89,25 @@ def test_guess_writes_scaled_output_on_unknowns():
     assert np.allclose(x, expect, rtol=1e-6, atol=0)
 
 
+def test_guess_over_bounding_box_matches_network_of_that_size():
+    """A 32^3 grid under a 64 x 64 x 128 network: the guess runs over the
+    grid's bounding box rounded up to what the kernels take (32, 32, 64) and
+    equals, bit for bit, the guess of a network of exactly that volume with
+    the same weights."""
+    g = spheres(32, 3)
+    big = UNet(64, 64, 128, seed=5)
+    small = UNet(32, 32, 64, seed=5)
+    small.set_params(big.get_params())
+    xs = []
+    for net in (big, small):
+        pr = project(g)
+        net.guess(pr, use_prev=False, out_scale=1.5)
+        xs.append(pr.get_pressure())
+        pr.close()
+    assert np.abs(xs[1]).max() > 0
+    assert np.array_equal(xs[0], xs[1])
+
+
 @pytest.mark.parametrize("prec", [MAC_FP64, MAC_MIXED])
 def test_untrained_guess_still_converges(prec):
     g = spheres(32, 2)

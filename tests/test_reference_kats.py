@@ -111,6 +111,38 @@ def test_convection_kat_2d():
     assert -1.0 + 0.1 * d["ay"][s] == pytest.approx(-0.875, abs=1e-10)
 
 
+@pytest.mark.parametrize("signs", [(1, 1), (-1, 1), (1, -1), (-1, -1)])
+def test_convection_linear_shear_closed_form(signs):
+    """u = a y + u0, v = b x + v0 (w = 0) on a fluid block, h = (0.1, 0.13,
+    0.07): one-sided
+    differences are exact for a linear field, and the mean of the four
+    transverse speeds around a face is the transverse speed at the face
+    itself, so the convection acceleration is exactly -(b x_u + v0) a on a
+    u-face and -(a y_v + u0) b on a v-face, for either upwind direction.  Any
+    other set of four transverse faces, or the wrong one-sided difference at
+    the block's edge, gives another number."""
+    hx, hy = 0.1, 0.13
+    a, b = 0.7 * signs[0], -1.3 * signs[1]
+    u0, v0 = 0.2 * signs[1], -0.4 * signs[0]
+    cells = {}
+    for x in range(-1, 9):
+        for y in range(-1, 9):
+            for z in range(-1, 3):
+                # u-face of cell (x,y,z) at ((x - 1/2) h, y h), v-face at (x h, (y - 1/2) h)
+                cells[(x, y, z)] = (W.FLUID, a * y * hy + u0, b * x * hx + v0, 0.0)
+    g, slot = grid_of(cells)
+    g["h"] = (hx, hy, 0.07)
+    d = forces_of(g)
+    n = 0
+    for x in range(1, 7):
+        for y in range(1, 7):
+            s = slot(x, y, 0)
+            assert d["ax"][s] == pytest.approx(-(b * (x - 0.5) * hx + v0) * a, abs=1e-12), (x, y)
+            assert d["ay"][s] == pytest.approx(-(a * (y - 0.5) * hy + u0) * b, abs=1e-12), (x, y)
+            n += 1
+    assert n == 36
+
+
 # -------------------------------------------------------------- divergence --
 @pytest.mark.parametrize("h,expect", [(1.0, -1.0), (0.1, -10.0)])
 def test_divergence_kat_1d(h, expect):

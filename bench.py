@@ -217,11 +217,12 @@ def jet_runs(args):
         jet = J.lateral_jet(192, 96, 96, h=0.01, nozzle=6, seed=args.seed)
         nu = 1.307e-6
         kw = dict(jet["params"], layer_max=4, integrator=2, seed=args.seed, particles_per_inflow_cell=0, fill_cube_n=1,
-                  depth_interior=3, depth_remove=5, floor_axis=2, floor_value=0.0, viscosity=nu)
+                  depth_interior=3, depth_remove=5, floor_axis=2, floor_value=0.0, viscosity=nu, surface_bc=1)
         proj = {"rtol": 1e-8, "precision": MAC_FP64, "viscosity": nu}
         r = {"workload": f"lateral jet 192x96x96, h=0.01, nozzle 6x6 at {jet['inflow']:.3f} m/s, the Jet3 step: RK4 all "
                          "(4 cell-type updates and 4 acceleration projections per step), depth (3, 5), floor z=0, "
-                         f"27-particle cube fill, nu={nu}, FP64 rtol 1e-8"}
+                         f"27-particle cube fill, nu={nu}, the free-surface velocity condition before every "
+                         "extrapolation, FP64 rtol 1e-8"}
         s = _sim(jet, kw, proj)
         t0, iters = time.perf_counter(), []
         for _ in range(args.jet3_steps):
@@ -286,7 +287,7 @@ def jet_runs(args):
         b = J.fluid_block(seed=args.seed)
         nu = 1.307e-6
         kw = dict(b["params"], layer_max=2, integrator=2, seed=args.seed, particles_per_inflow_cell=0, fill_cube_n=1,
-                  depth_interior=3, depth_remove=5, floor_axis=2, floor_value=0.25, viscosity=nu)
+                  depth_interior=3, depth_remove=5, floor_axis=2, floor_value=0.25, viscosity=nu, surface_bc=1)
         s = Sim(**kw, proj={"rtol": 1e-8, "precision": MAC_MIXED, "viscosity": nu})
         s.upload(b["keys"], b["flags"], u=b["u"], v=b["v"], w=b["w"], particles=b["particles"])
         n0 = len(b["particles"])
@@ -303,7 +304,8 @@ def jet_runs(args):
         wall = (time.perf_counter() - t0) * 1e3 / args.big_rk4_steps
         r = {"workload": "fluid block 96x96x192 cells at 1 m/s in a 256^3 box (h=0.01), 8 particles per cell, the Jet3 "
                          "step: RK4 all (5 cell-type updates, 4 acceleration projections per step), depth (3, 5), floor "
-                         "z=0.25, 27-particle cube refill, nu=1.307e-6, mixed projection rtol 1e-8",
+                         "z=0.25, 27-particle cube refill, nu=1.307e-6, the free-surface velocity condition before "
+                         "every extrapolation, mixed projection rtol 1e-8",
              "particles_start": n0, "particles_after_first_step": n1, "particles_end": st["nparticles"],
              "fluid_cells": st["nfluid"], "bricks": st["nbricks"], "steps": args.big_rk4_steps,
              "gpu_ms_per_step": wall, "gpu_ms_breakdown": parts, "pcg_iters_per_step": float(np.mean(iters))}

@@ -117,6 +117,7 @@ struct mac_sim {
   // cell, the bytes zero between calls) and per-brick activity
   double* bc_tmp = nullptr;
   uint8_t* bc_pend = nullptr;
+  uint8_t* bc_cls = nullptr;   // per cell class bytes
   uint8_t* bc_brick = nullptr;  // [0, cap): summary, [cap, 2 cap): active
   size_t bc_cap = 0;            // bricks
   // starting iterate of the projections: 0 zero, 1 last pressure, 2 learned
@@ -635,6 +636,7 @@ void mac_sim_destroy(mac_sim* s) {
   hipFree(s->bc_tmp);
   hipFree(s->bc_pend);
   hipFree(s->bc_brick);
+  hipFree(s->bc_cls);
   hipFree(s->d_missing);
   hipFree(s->d_missing2);
   hipFree(s->d_cset);
@@ -871,13 +873,16 @@ int sim_surface_bc(mac_sim* s, double* u, double* v, double* w, int dim) {
     hipFree(s->bc_tmp);
     hipFree(s->bc_pend);
     hipFree(s->bc_brick);
+    hipFree(s->bc_cls);
     s->bc_tmp = nullptr;
     s->bc_pend = nullptr;
     s->bc_brick = nullptr;
+    s->bc_cls = nullptr;
     s->bc_cap = 0;
     const size_t cap = nb + nb / 4 + 16;
     SCHK(hipMalloc(&s->bc_tmp, 3 * cap * MAC_BC * sizeof(double)));
     SCHK(hipMalloc(&s->bc_pend, 3 * cap * MAC_BC));
+    SCHK(hipMalloc(&s->bc_cls, cap * MAC_BC));
     SCHK(hipMalloc(&s->bc_brick, 2 * cap));
     SCHK(hipMemsetAsync(s->bc_pend, 0, 3 * cap * MAC_BC, s->st));
     s->bc_cap = cap;
@@ -899,10 +904,12 @@ int sim_surface_bc(mac_sim* s, double* u, double* v, double* w, int dim) {
   uint8_t* summ = s->bc_brick;
   uint8_t* active = s->bc_brick + s->bc_cap;
   a.active = active;
+  a.cls = s->bc_cls;
   a.dim = dim;
   const int g = s->nb;
   k_bc_summary<<<g, 256, 0, s->st>>>(s->nb, s->flags, summ);
   k_bc_active<<<(g + 255) / 256, 256, 0, s->st>>>(s->nb, s->nbr, summ, active);
+  k_bc_classify<<<g, 256, 0, s->st>>>(a);
   k_bc_set_nan<<<g, 256, 0, s->st>>>(a);
   k_bc_div<<<g, 256, 0, s->st>>>(a);
   k_bc_plane<<<g, 256, 0, s->st>>>(a);

@@ -46,6 +46,8 @@ struct BcArgs {
   double* tmp[3];     // staged values
   uint8_t* pend[3];   // staged-value flags (zero between launches)
   const uint8_t* active;  // per brick: the brick or a neighbour holds both air and non-air cells
+  uint8_t* cls;           // per cell: bits 0-5 its air neighbours (-x,+x,-y,+y,-z,+z) if it is an
+                          // existing non-air cell, bit 6 existing non-air, bit 7 air (k_bc_classify)
   double h[3], ih[3];
   int dim;
 };
@@ -99,20 +101,32 @@ struct Cell {
     a.tmp[c][s] = v;
     a.pend[c][s] = 1;
   }
-  // IsBoundary (:88-119): the air neighbours of a valid non-air cell, axis-major
+  // IsBoundary (:88-119): the air neighbours of a valid non-air cell,
+  // axis-major, from the cell's class byte
   __device__ Bnd bnd(int dx, int dy, int dz) const {
     Bnd r;
     r.n = 0;
-    const unsigned f = flag(dx, dy, dz);
-    if (!f_valid(f) || f_air(f)) return r;
+    const long long s = slot(dx, dy, dz);
+    const unsigned m = s < 0 ? 0u : (a.cls[s] & 0x3fu);
+#pragma unroll
+    for (int k = 0; k < 6; ++k)
+      if (m & (1u << k)) {
+        r.dir[r.n] = k >> 1;
+        r.sign[r.n] = (k & 1) ? 1 : -1;
+        ++r.n;
+      }
+    return r;
+  }
+  // the same from the cell's own flags and its neighbours' (k_bc_classify)
+  __device__ unsigned classify() const {
+    const unsigned f = flag(0, 0, 0);
+    if (!f_valid(f)) return 0u;
+    if (f_air(f)) return 0x80u;
+    unsigned m = 0x40u;
     for (int i = 0; i < a.dim; ++i)
       for (int s = -1; s <= 1; s += 2)
-        if (f_air(flag(dx + s * (i == 0), dy + s * (i == 1), dz + s * (i == 2)))) {
-          r.dir[r.n] = i;
-          r.sign[r.n] = s;
-          ++r.n;
-        }
-    return r;
+        if (f_air(flag(s * (i == 0), s * (i == 1), s * (i == 2)))) m |= 1u << (2 * i + (s > 0));
+    return m;
   }
   // a one-air cell with air axis d at (dx, dy, dz) + i e_t
   __device__ bool partner(int dx, int dy, int dz, int d, int t, int i) const {
@@ -181,6 +195,22 @@ __global__ void k_bc_active(int nb, const int* __restrict__ nbr, const uint8_t* 
   active[b] = (m == 3u) ? 1 : 0;
 }
 
+// The class byte of every cell (BcArgs::cls); in an inactive brick no cell
+// has a neighbour of the other kind, so only its own kind is recorded.
+__global__ void k_bc_classify(BcArgs a) {
+  const bool act = a.active[blockIdx.x] != 0;
+  BC_FOR_CELLS {
+    const size_t e = (size_t)b * MAC_BC + t;
+    if (act) {
+      bc::Cell c(a, b, t);
+      a.cls[e] = (uint8_t)c.classify();
+    } else {
+      const unsigned f = a.flags[e];
+      a.cls[e] = (uint8_t)(!bc::f_valid(f) ? 0u : bc::f_air(f) ? 0x80u : 0x40u);
+    }
+  }
+}
+
 // P1
 __global__ void k_bc_set_nan(BcArgs a) {
   BC_FOR_CELLS {
@@ -241,7 +271,11 @@ __global__ void k_bc_plane(BcArgs a) {
   if (!a.active[blockIdx.x]) return;
   BC_FOR_CELLS {
     bc::Cell c(a, b, t);
+    // a pair writes X's face when its writing cell's air neighbour is X (X
+    // air) or X - e_t (then X's low t neighbour is air)
+    const unsigned cx = a.cls[(size_t)b * MAC_BC + t];
     for (int tt = 0; tt < a.dim; ++tt) {
+      if (!(cx & 0x80u) && !(cx & (1u << (2 * tt)))) continue;
       bool have = false;
       double val = 0.0;
       for (int d = 0; d < a.dim; ++d) {

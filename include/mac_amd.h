@@ -207,7 +207,7 @@ int mac_solve_csr(mac_ctx* ctx, int n, const int* offset, const int* indice, con
  * DEVICE array (d_offset[n+1], d_indice/d_value[nnz], d_b and d_x[n]; the CSR
  * is checked on the device: off[0] = 0, off[n] = nnz, non-decreasing offsets,
  * column indices in [0, n)), and the solve's work arrays live in a fixed
- * static device array of the library (about 10.5 MB per GPU, held until the
+ * static device array of the library (about 21.1 MB per GPU, held until the
  * process ends).  n <= MAC_CSR_NOALLOC_MAX_N, larger n is rejected with
  * MAC_E_ARG.  Calls that overlap on one GPU share that array and race: issue
  * them from one thread, or serialise them.  Runs on the context's stream and

@@ -7,10 +7,11 @@
 //                        iteration and the convergence test) in one launch;
 //                        the vectors stay in that CU's caches, and no launch
 //                        or host round trip separates the iterations.
-//   larger n             three grid-stride launches per iteration, scalars
-//                        reduced on the device in fixed block order; a batch
-//                        of CSR_BATCH iterations is queued before the host
-//                        reads the convergence flag once (launches after
+//   larger n             two grid-stride launches per iteration (the new
+//                        search direction is formed inside the product),
+//                        scalars reduced on the device in fixed block order;
+//                        a batch of CSR_BATCH iterations is queued before the
+//                        host reads the convergence flag once (launches after
 //                        convergence return at once).
 // The work arrays live wherever the caller puts them (carve()).
 #pragma once
@@ -29,30 +30,35 @@ constexpr int CSR_SINGLE_MAX = 32768;  // rows solved by the one-workgroup kerne
 constexpr int CSR_SINGLE_THREADS = 1024;
 constexpr int CSR_THREADS = 256;
 constexpr int CSR_GRID_MAX = 2048;
-constexpr int CSR_BATCH = 16;
+constexpr int CSR_BATCH = 16;       // iterations queued between host reads, at least
+constexpr int CSR_BATCH_MAX = 512;  // and at most
 
 // Device state of one solve.  status: 0 running, 1 converged, -1 breakdown
 // (p.(-A)p <= 0: the matrix is not negative definite), -2 max_iter reached,
 // -3 malformed CSR (device-side check), -4 singular small system.
 struct CsrState {
-  double rz, rr, bb, pq, tol2;
+  double rz, rr, bb, pq, tol2, beta;
+  double gam[2], alp[2];  // gamma and alpha of iterations of even and odd parity
   int iters, status, done, maxit;
   unsigned counter;
   unsigned pad[3];
 };
 static_assert(sizeof(CsrState) <= 256, "state block");
 
-// Device arrays of one solve.
+// Device arrays of one solve.  The one-workgroup kernel uses r, z, p, q and
+// dinv; the one-launch-per-iteration path uses r, s and w in two copies each
+// (rb, sb, wb: read one, write the other), p and dinv.
 struct CsrWork {
   CsrState* st;
-  double *partials, *r, *z, *p, *q, *dinv;
+  double *partials, *r, *z, *p, *q, *dinv, *r1, *s0, *s1, *w0, *w1;
 };
 
 constexpr size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
 
-// Bytes of the work arrays for n rows (state, partials, five vectors).
+// Bytes of the work arrays for n rows (state, two sets of three partials per
+// block, ten vectors).
 constexpr size_t work_bytes(size_t n) {
-  return 256 + align256(CSR_GRID_MAX * sizeof(double)) + 5 * align256(n * sizeof(double));
+  return 256 + align256(6 * CSR_GRID_MAX * sizeof(double)) + 10 * align256(n * sizeof(double));
 }
 
 CsrWork carve(void* base, size_t n) {
@@ -61,8 +67,8 @@ CsrWork carve(void* base, size_t n) {
   w.st = (CsrState*)b;
   b += 256;
   w.partials = (double*)b;
-  b += align256(CSR_GRID_MAX * sizeof(double));
-  double** v[5] = {&w.r, &w.z, &w.p, &w.q, &w.dinv};
+  b += align256(6 * CSR_GRID_MAX * sizeof(double));
+  double** v[10] = {&w.r, &w.z, &w.p, &w.q, &w.dinv, &w.r1, &w.s0, &w.s1, &w.w0, &w.w1};
   for (auto* pv : v) {
     *pv = (double*)b;
     b += align256(n * sizeof(double));
@@ -196,41 +202,9 @@ __global__ __launch_bounds__(CSR_SINGLE_THREADS) void k_csr_cg_single(int n, int
   }
 }
 
-// Fixed-order grid reduction of two values (partials[2 * block + 0/1]);
-// true in the last block to arrive, totals in t0, t1 (thread 0).
-__device__ __forceinline__ bool grid_reduce2(double a, double b, double* __restrict__ partials, unsigned* counter, double* t0,
-                                             double* t1, double* sh) {
-  __shared__ int am_last;
-  double sa = block_sum(a, sh);
-  double sb = block_sum(b, sh);
-  if (threadIdx.x == 0) {
-    __hip_atomic_store(&partials[2 * blockIdx.x], sa, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __hip_atomic_store(&partials[2 * blockIdx.x + 1], sb, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __threadfence();
-    unsigned prev = atomicAdd(counter, 1u);
-    am_last = (prev == gridDim.x - 1);
-  }
-  __syncthreads();
-  if (!am_last) return false;
-  __threadfence();
-  double x0 = 0.0, x1 = 0.0;
-  for (unsigned i = threadIdx.x; i < gridDim.x; i += blockDim.x) {
-    x0 += ld_coherent(&partials[2 * i]);
-    x1 += ld_coherent(&partials[2 * i + 1]);
-  }
-  x0 = block_sum(x0, sh);
-  x1 = block_sum(x1, sh);
-  if (threadIdx.x == 0) {
-    *t0 = x0;
-    *t1 = x1;
-    *counter = 0u;
-  }
-  return true;
-}
-
 __global__ void k_csr_reset(CsrState* st, int maxit) {
   if (threadIdx.x == 0) {
-    st->rz = st->rr = st->bb = st->pq = st->tol2 = 0.0;
+    st->rz = st->rr = st->bb = st->pq = st->tol2 = st->beta = 0.0;
     st->iters = 0;
     st->status = 0;
     st->done = 0;
@@ -248,95 +222,184 @@ __global__ void k_csr_check(int n, int nnz, const int* __restrict__ off, const i
   }
 }
 
-// x = 0, r = -b, z = p = D^-1 r; r.z and r.r
-__global__ void k_csr_init(int n, const int* __restrict__ off, const int* __restrict__ idx, const double* __restrict__ val,
-                           const double* __restrict__ b, double* __restrict__ x, CsrWork w, double rtol) {
-  __shared__ double sh[16];
+// The multi-workgroup path runs one launch per iteration: the
+// single-reduction form of preconditioned CG (Chronopoulos and Gear, 1989),
+// with u = D^-1 r and w = (-A) u carried as vectors, so that both inner
+// products of an iteration, gamma = r.u and delta = w.u, come from one
+// reduction.  Iteration k, with gamma_k, delta_k from the last launch:
+//   beta = gamma_k / gamma_{k-1}, alpha = gamma_k / (delta_k - beta gamma_k / alpha_{k-1})
+//   p = u + beta p, s = w + beta s, x += alpha p, r -= alpha s, u = D^-1 r, w = (-A) u
+// and the new r.u, w.u, r.r.  The product needs u at the neighbour rows; it
+// is formed at every gathered column from the neighbour's old r, w and s
+// (read from one copy while the new ones go to the other), so no launch
+// separates the vector updates from the product.  No grid-wide atomics:
+// each launch writes three partials per block with plain stores, and the
+// next launch -- after the launch boundary has made them visible -- sums them
+// in every block in the same fixed order, so every block takes the same
+// decisions.  Two sets of partials alternate by iteration parity (a block
+// that is done writes its set while others still read the other), and block
+// 0 keeps gamma and alpha in the state in two parity slots likewise.
+
+// Sums of a, b and c over the block, returned in every thread: the fixed
+// order of block_sum, with two barriers for the three.  sh holds 3 doubles
+// per wave.
+__device__ __forceinline__ void block_sum3_all(double& a, double& b, double& c, double* sh) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    a += __shfl_xor(a, o);
+    b += __shfl_xor(b, o);
+    c += __shfl_xor(c, o);
+  }
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  __syncthreads();  // earlier readers of sh are done
+  if (lane == 0) {
+    sh[3 * wv] = a;
+    sh[3 * wv + 1] = b;
+    sh[3 * wv + 2] = c;
+  }
+  __syncthreads();
+  a = b = c = 0.0;
+  for (int i = 0; i < nw; ++i) {
+    a += sh[3 * i];
+    b += sh[3 * i + 1];
+    c += sh[3 * i + 2];
+  }
+}
+
+// The three sums over the G blocks of a partial set (3 doubles per block),
+// in every block.
+__device__ __forceinline__ void sum_partials(const double* __restrict__ set, double& a, double& b, double& c, double* sh) {
+  a = b = c = 0.0;
+  for (int i = threadIdx.x; i < (int)gridDim.x; i += blockDim.x) {
+    a += set[3 * i];
+    b += set[3 * i + 1];
+    c += set[3 * i + 2];
+  }
+  block_sum3_all(a, b, c, sh);
+}
+
+// D^-1 of every row
+__global__ void k_csr_diag(int n, const int* __restrict__ off, const int* __restrict__ idx, const double* __restrict__ val,
+                           CsrWork w) {
   if (w.st->done) return;
-  double arz = 0.0, arr = 0.0;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) w.dinv[i] = inv_diag(off, idx, val, i);
+}
+
+// x = 0, r = -b, u = D^-1 r, w = (-A) u, p = s = 0; partials r.u, w.u, r.r
+// into set 0.
+__global__ __launch_bounds__(CSR_THREADS) void k_csr_cg_init(int n, const int* __restrict__ off, const int* __restrict__ idx, const double* __restrict__ val,
+                              const double* __restrict__ b, double* __restrict__ x, CsrWork w) {
+  __shared__ double sh[3 * (CSR_THREADS / 64)];
+  if (w.st->done) return;
+  double g = 0.0, d = 0.0, rr = 0.0;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    const double di = inv_diag(off, idx, val, i);
-    const double ri = -b[i];
-    const double zi = di * ri;
-    w.dinv[i] = di;
+    double acc = 0.0;
+    for (int k = off[i]; k < off[i + 1]; ++k) {
+      const int j = idx[k];
+      acc += val[k] * (w.dinv[j] * -b[j]);
+    }
+    const double ri = -b[i], ui = w.dinv[i] * ri, wi = -acc;
     x[i] = 0.0;
     w.r[i] = ri;
-    w.p[i] = zi;
-    arz += ri * zi;
-    arr += ri * ri;
+    w.w0[i] = wi;
+    w.s0[i] = 0.0;
+    w.p[i] = 0.0;
+    g += ri * ui;
+    d += wi * ui;
+    rr += ri * ri;
   }
-  double rz, rr;
-  if (grid_reduce2(arz, arr, w.partials, &w.st->counter, &rz, &rr, sh) && threadIdx.x == 0) {
-    w.st->rz = rz;
-    w.st->rr = rr;
-    w.st->bb = rr;
-    w.st->tol2 = rtol * rtol * rr;
-    if (rr == 0.0) {
-      w.st->status = 1;
-      w.st->done = 1;
-    }
+  block_sum3_all(g, d, rr, sh);
+  if (threadIdx.x == 0) {
+    w.partials[3 * blockIdx.x] = g;
+    w.partials[3 * blockIdx.x + 1] = d;
+    w.partials[3 * blockIdx.x + 2] = rr;
   }
 }
 
-// q = (-A) p, p.q
-__global__ void k_csr_spmv(int n, const int* __restrict__ off, const int* __restrict__ idx, const double* __restrict__ val,
-                           CsrWork w) {
-  __shared__ double sh[16];
-  if (w.st->done) return;
-  double apq = 0.0;
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    double s = 0.0;
-    for (int k = off[i]; k < off[i + 1]; ++k) s += val[k] * w.p[idx[k]];
-    w.q[i] = -s;
-    apq += w.p[i] * (-s);
+// Iteration k (partials of set k & 1 in, set (k + 1) & 1 out).  test_only:
+// the convergence test alone (it writes nothing unless the solve has ended).
+__global__ __launch_bounds__(CSR_THREADS) void k_csr_cg_iter(int n, const int* __restrict__ off, const int* __restrict__ idx, const double* __restrict__ val,
+                              double* __restrict__ x, CsrWork w, int k, double rtol, int test_only) {
+  __shared__ double sh[3 * (CSR_THREADS / 64)];
+  CsrState* st = w.st;
+  if (st->done) return;
+  const int G = gridDim.x;
+  const double* pin = w.partials + (size_t)(k & 1) * 3 * G;
+  double* pout = w.partials + (size_t)((k + 1) & 1) * 3 * G;
+  double gam, del, rr;
+  sum_partials(pin, gam, del, rr, sh);
+  const double tol2 = k == 0 ? rtol * rtol * rr : st->tol2;
+  int status = 0;
+  double beta = 0.0, alpha = 0.0;
+  if (rr <= tol2) {
+    status = 1;
+  } else if (k >= st->maxit) {
+    status = -2;
+  } else {
+    beta = k == 0 ? 0.0 : gam / st->gam[(k - 1) & 1];
+    const double den = k == 0 ? del : del - beta * gam / st->alp[(k - 1) & 1];
+    if (!(gam > 0.0) || !(den > 0.0)) status = -1;
+    else alpha = gam / den;
   }
-  double pq, unused;
-  if (grid_reduce2(apq, 0.0, w.partials, &w.st->counter, &pq, &unused, sh) && threadIdx.x == 0) {
-    w.st->pq = pq;
-    if (!(pq > 0.0)) {
-      w.st->status = -1;
-      w.st->done = 1;
-    }
-  }
-}
-
-// x += a p, r -= a q, z = D^-1 r; r.z, r.r; convergence test
-__global__ void k_csr_update(int n, double* __restrict__ x, CsrWork w) {
-  __shared__ double sh[16];
-  if (w.st->done) return;
-  const double alpha = w.st->rz / w.st->pq;
-  double arz = 0.0, arr = 0.0;
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    x[i] += alpha * w.p[i];
-    const double ri = w.r[i] - alpha * w.q[i];
-    w.r[i] = ri;
-    const double zi = w.dinv[i] * ri;
-    w.z[i] = zi;
-    arz += ri * zi;
-    arr += ri * ri;
-  }
-  double rz1, rr;
-  if (grid_reduce2(arz, arr, w.partials, &w.st->counter, &rz1, &rr, sh) && threadIdx.x == 0) {
-    CsrState* st = w.st;
-    st->iters += 1;
-    st->pq = rz1 / st->rz;  // beta for k_csr_pupdate
-    st->rz = rz1;
-    st->rr = rr;
-    if (rr <= st->tol2) {
-      st->status = 1;
-      st->done = 1;
-    } else if (st->iters >= st->maxit) {
-      st->status = -2;
+  if (status != 0) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      st->status = status;
+      st->iters = k;
+      st->rr = rr;
       st->done = 1;
     }
+    return;
   }
-}
-
-// p = z + beta p
-__global__ void k_csr_pupdate(int n, CsrWork w) {
-  if (w.st->done) return;
-  const double beta = w.st->pq;
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) w.p[i] = w.z[i] + beta * w.p[i];
+  if (test_only) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      st->rr = rr;
+      st->tol2 = tol2;
+    }
+    return;
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    st->gam[k & 1] = gam;
+    st->alp[k & 1] = alpha;
+    if (k == 0) {
+      st->tol2 = tol2;
+      st->bb = rr;
+    }
+  }
+  const double* r_in = (k & 1) ? w.r1 : w.r;
+  double* r_out = (k & 1) ? w.r : w.r1;
+  const double* s_in = (k & 1) ? w.s1 : w.s0;
+  double* s_out = (k & 1) ? w.s0 : w.s1;
+  const double* w_in = (k & 1) ? w.w1 : w.w0;
+  double* w_out = (k & 1) ? w.w0 : w.w1;
+  double g = 0.0, d = 0.0, r2 = 0.0;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const double di = w.dinv[i];
+    const double ui = di * r_in[i];
+    const double pi = ui + beta * w.p[i];
+    const double si = w_in[i] + beta * s_in[i];
+    w.p[i] = pi;
+    x[i] += alpha * pi;
+    const double rn = r_in[i] - alpha * si;
+    double acc = 0.0;
+    for (int kk = off[i]; kk < off[i + 1]; ++kk) {
+      const int j = idx[kk];
+      const double sj = w_in[j] + beta * s_in[j];
+      acc += val[kk] * (w.dinv[j] * (r_in[j] - alpha * sj));
+    }
+    const double un = di * rn, wn = -acc;
+    s_out[i] = si;
+    r_out[i] = rn;
+    w_out[i] = wn;
+    g += rn * un;
+    d += wn * un;
+    r2 += rn * rn;
+  }
+  block_sum3_all(g, d, r2, sh);
+  if (threadIdx.x == 0) {
+    pout[3 * blockIdx.x] = g;
+    pout[3 * blockIdx.x + 1] = d;
+    pout[3 * blockIdx.x + 2] = r2;
+  }
 }
 
 // Gaussian elimination with partial pivoting of an n <= 3 CSR system, one
@@ -453,18 +516,31 @@ int run_cg(hipStream_t st, int n, int nnz, const int* off, const int* idx, const
     k_csr_check<<<blocks, CSR_THREADS, 0, st>>>(n, nnz, off, idx, w.st);
     CSR_HIPRET(hipGetLastError());
   }
-  k_csr_init<<<blocks, CSR_THREADS, 0, st>>>(n, off, idx, val, b, x, w, rtol);
+  k_csr_diag<<<blocks, CSR_THREADS, 0, st>>>(n, off, idx, val, w);
+  k_csr_cg_init<<<blocks, CSR_THREADS, 0, st>>>(n, off, idx, val, b, x, w);
   CSR_HIPRET(hipGetLastError());
+  int it = 0, batch = CSR_BATCH, it_prev = 0;
+  double rr_prev = 0.0;
   for (;;) {
-    for (int k = 0; k < CSR_BATCH; ++k) {
-      k_csr_spmv<<<blocks, CSR_THREADS, 0, st>>>(n, off, idx, val, w);
-      k_csr_update<<<blocks, CSR_THREADS, 0, st>>>(n, x, w);
-      k_csr_pupdate<<<blocks, CSR_THREADS, 0, st>>>(n, w);
-    }
+    // a batch of iterations, then a launch that only tests the last one's
+    // residual so that the host sees it
+    for (int k = 0; k < batch; ++k, ++it) k_csr_cg_iter<<<blocks, CSR_THREADS, 0, st>>>(n, off, idx, val, x, w, it, rtol, 0);
+    k_csr_cg_iter<<<blocks, CSR_THREADS, 0, st>>>(n, off, idx, val, x, w, it, rtol, 1);
     CSR_HIPRET(hipGetLastError());
     CSR_HIPRET(hipMemcpyAsync(hst, w.st, sizeof(CsrState), hipMemcpyDeviceToHost, st));
     CSR_HIPRET(hipStreamSynchronize(st));
     if (hst->done) break;
+    // the next batch: about the iterations still needed at the mean decay
+    // rate of the last batch (a host read costs about as much as a few
+    // iterations; launches queued past convergence return at once)
+    batch = CSR_BATCH;
+    if (rr_prev > 0.0 && hst->rr > 0.0 && hst->rr < rr_prev && hst->tol2 > 0.0) {
+      const double rate = std::log(hst->rr / rr_prev) / (it - it_prev);  // < 0
+      const double need = std::log(hst->tol2 / hst->rr) / rate;
+      batch = std::max(CSR_BATCH, std::min(CSR_BATCH_MAX, (int)(0.9 * need)));
+    }
+    rr_prev = hst->rr;
+    it_prev = it;
   }
   return 0;
 }

@@ -1,9 +1,9 @@
 // csr_noalloc.hip -- mac_solve_csr_noalloc: the slot A solve (the same
 // Jacobi-PCG kernels as mac_solve_csr, csr_kernels.h) for callers that must
 // not allocate: caller-owned device arrays in and out, and the solve's work
-// arrays (state, reduction partials, r, z, p, q, D^-1) in `scratch`, a fixed
+// arrays (state, reduction partials, ten vectors) in `scratch`, a fixed
 // file-scope device array of work_bytes(MAC_CSR_NOALLOC_MAX_N) bytes (about
-// 10.5 MB), held by every GPU the library runs on until the process ends.
+// 21.1 MB), held by every GPU the library runs on until the process ends.
 //
 // Host code never passes `scratch` itself (in host code the name is a
 // host-side stand-in, not a device address): on the first call on a device,

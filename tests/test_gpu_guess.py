@@ -186,3 +186,27 @@ def test_trained_guess_gain_on_held_out_inputs():
         assert r["learned"]["iters"] < r["zero"]["iters"]
         if r["learned"]["same_topology"]:
             assert r["learned"]["p_rel_diff"] < 1e-6
+
+
+@pytest.mark.parametrize("n,steps", [(128, 40), (256, 20)])
+def test_learned_guess_at_evaluation_sizes(n, steps):
+    """Configuration 4's sizes: a held-out jet in an n^3 box for `steps`
+    steps with the zero, previous-pressure and learned starting iterates
+    (every projection to the same 1e-8; the learned start takes fewer
+    iterations than the zero start and the runs end in the same state within
+    1e-6), and the projection benchmark's input at n^3 (zero vs learned
+    start, same answer within 1e-6)."""
+    import os
+    from mac_amd.train_guess import eval_jet, eval_projection, load_net
+    w = os.path.join(os.path.dirname(__file__), "..", "mac_amd", "weights", "unet_guess.npz")
+    net, osc = load_net(w, (n + 16, n + 16, n + 64))
+    r = eval_jet(net, osc, 100, steps=steps, nx=n)
+    print(f"[c4] jet {n}^3 {steps} steps:", {k: r[k]["iters"] for k in ("zero", "previous", "learned")}, flush=True)
+    assert r["learned"]["iters"] < r["zero"]["iters"]
+    for k in ("previous", "learned"):
+        assert r[k]["same_topology"] and r[k]["p_rel_diff"] < 1e-6
+    g = W.sphere_box(n, 0)
+    p = eval_projection(g, net, osc, MAC_MIXED)
+    print(f"[c4] benchmark input {n}^3:", p, flush=True)
+    assert p["fallback"] == 0 and p["p_rel_diff"] < 1e-6 and p["guess_rel_err"] < 1.0
+    net.close()

@@ -249,10 +249,12 @@ def test_solve_csr_slot_a_paths(dims):
     xo, _ = O.cg_csr(n, off, idx, val, b, rtol=1e-14)
     ls = LinearSolver(rtol=1e-13)
     ls.Solve_Linear(n, off, idx, val, b)  # first call sizes the workspace
-    t0 = time.perf_counter()
-    xg = ls.Solve_Linear(n, off, idx, val, b)
-    ms = (time.perf_counter() - t0) * 1e3
-    print(f"slot A n={n}: {ms:.2f} ms per solve to 1e-13 (host arrays in and out)")
+    ts = []
+    for _ in range(9):
+        t0 = time.perf_counter()
+        xg = ls.Solve_Linear(n, off, idx, val, b)
+        ts.append((time.perf_counter() - t0) * 1e3)
+    print(f"slot A n={n}: {np.median(ts):.2f} ms per solve to 1e-13, median of 9 (host arrays in and out)")
     assert rel(xg, xo) < 1e-10
     n2, off2, idx2, val2, b2 = _box_system(16, 16, 8, seed=12)
     x2 = ls.Solve_Linear(n2, off2, idx2, val2, b2)

@@ -70,15 +70,16 @@ typedef struct {
   int precision;   /* enum mac_precision                                           */
   int nu;          /* damped-Jacobi sweeps before and after each coarse correction */
   double omega;    /* Jacobi damping                                               */
-  int check_every; /* host polls the device convergence flag every k iterations;
-                      from the third solve on the same grid (kernel timing off)
-                      the iterations run as captured HIP graphs of two instead
-                      and the host polls when the residual decay predicts
-                      convergence                                                 */
+  int check_every; /* iterations before the host's first poll of the device
+                      convergence flag; later polls come after as many
+                      iterations as the residual decay so far predicts are
+                      still needed.  From the third solve on the same grid
+                      (kernel timing off) each iteration is one captured HIP
+                      graph launch                                                */
   int stall_iters; /* guided solve (guess = 1): restart from zero if not converged
-                      after this many iterations (0: never; checked after every
-                      pair of iterations when they run as graphs); a guess whose
-                      first residual is no smaller than ||f|| is dropped at once  */
+                      after this many iterations (0: never; the host polls at
+                      that iteration); a guess whose first residual is no
+                      smaller than ||f|| is dropped at once                       */
   int smoother;    /* enum mac_smoother                                            */
   double cheb_lower; /* MAC_SMOOTH_CHEBYSHEV: lower end a of the eigenvalue range
                       [a, 2] of D^-1 A that the nu sweeps damp                    */

@@ -90,10 +90,12 @@ struct mac_ctx {
   double omc_pre[8], omc_post[8];  // the same on the coarse levels
   BlockW blockw;                   // inner sweeps of the coarse block smoother
   BlockW blockw_deep;              // the single launch per side on the levels from BLOCK_DEEP on
-  // two CG iterations captured as one graph, per V-cycle precision (0 FP64, 1 FP32)
-  hipGraphExec_t gexec[2] = {nullptr, nullptr};
+  // 1, 2 or 4 CG iterations captured as a graph, per V-cycle precision (0
+  // FP64, 1 FP32) and per parity of the search-direction buffers
+  hipGraphExec_t gexec[2][2][3] = {};
   int solves_on_grid = 0;  // projections since the last build (graphs pay off from the third)
-  const void* gz[2] = {nullptr, nullptr};  // the V-cycle output the captured pair starts from
+  const void* gz[2] = {nullptr, nullptr};  // the V-cycle output of the captured iterations
+  int last_iters[2] = {0, 0};              // iterations of the last converged solve, unguided / guided
   hipStream_t st = nullptr;
   uint8_t* pin = nullptr;  // pinned staging for the hierarchy's host tables
   size_t pin_cap = 0;
@@ -154,8 +156,11 @@ namespace {
 
 void drop_graphs(mac_ctx* c) {
   for (int i = 0; i < 2; ++i) {
-    if (c->gexec[i]) hipGraphExecDestroy(c->gexec[i]);
-    c->gexec[i] = nullptr;
+    for (int j = 0; j < 2; ++j)
+      for (int k = 0; k < 3; ++k) {
+        if (c->gexec[i][j][k]) hipGraphExecDestroy(c->gexec[i][j][k]);
+        c->gexec[i][j][k] = nullptr;
+      }
     c->gz[i] = nullptr;
   }
 }
@@ -283,9 +288,10 @@ struct TimedLaunch {
 
 // One-block reduction of the partials of the previous kernel plus the CG
 // scalar step (see k_finalize); returns the launch's sequence number.
-long long finalize(mac_ctx* c, int nparts, int op) {
+// to_host: the launch also writes the state to the pinned mirror (poll_scal).
+long long finalize(mac_ctx* c, int nparts, int op, bool to_host = false) {
   TimedLaunch tl(c, K_FIN, 0.0);
-  k_finalize<<<1, 1024, 0, c->st>>>(nparts, c->partials, c->s, op, c->prm.rtol * c->prm.rtol);
+  k_finalize<<<1, 1024, 0, c->st>>>(nparts, c->partials, c->s, op, c->prm.rtol * c->prm.rtol, to_host ? c->hs : nullptr);
   return tl.myseq;
 }
 
@@ -979,6 +985,83 @@ int read_scal(mac_ctx* c) {
   return 0;
 }
 
+// The state after a finalize launched with to_host: waiting is enough.
+int poll_scal(mac_ctx* c) {
+  HIPCHK(hipStreamSynchronize(c->st));
+  return 0;
+}
+
+// One CG iteration on the residual r: z = V(r) (the cycle's last sweep gives
+// r.z and beta), d_new = z + beta d_old, q = A d_new, alpha, x += alpha d_new,
+// r -= alpha q and the convergence test.  d[cur] holds d_old and d[cur ^ 1]
+// receives d_new.  *last gets the sequence number of the test's launch.
+template <typename TZ>
+int cg_iteration(mac_ctx* c, int cur, TZ** zout, long long* last) {
+  Level& l0 = c->L[0];
+  const double* ih = l0.ih2;
+  const double nd = (double)l0.ndom;
+  CHK(vcycle<TZ>(c, zout));
+  {
+    TimedLaunch tl(c, K_PMATVEC, nd * (25 + sizeof(TZ)));
+    k_pmatvec<TZ><<<grid_of(l0.nact), KTHREADS, 0, c->st>>>(l0.nact, l0.meta, l0.code, *zout, c->d[cur], c->d[cur ^ 1], c->q,
+                                                            ih[0], ih[1], ih[2], c->partials, c->s);
+  }
+  finalize(c, grid_of(l0.nact), OP_DQ);
+  {
+    TimedLaunch tl(c, K_UPDATE, nd * 49);
+    k_update<<<grid_of(l0.nact), KTHREADS, 0, c->st>>>(l0.nact, l0.meta, l0.code, c->x, c->r, c->d[cur ^ 1], c->q, c->partials,
+                                                       c->s);
+  }
+  *last = finalize(c, grid_of(l0.nact), OP_RR, true);
+  return 0;
+}
+
+// 2^k captured iterations starting at parity cur (captured on first use):
+// a launch boundary between graphs costs several microseconds, so a batch
+// runs as few graphs as its size allows.
+template <typename TZ>
+int iteration_graph(mac_ctx* c, int cur, int k, hipGraphExec_t* out) {
+  const int gi = std::is_same<TZ, float>::value ? 1 : 0;
+  hipGraphExec_t& ge = c->gexec[gi][cur][k];
+  if (!ge) {
+    hipGraph_t g = nullptr;
+    HIPCHK(hipStreamBeginCapture(c->st, hipStreamCaptureModeRelaxed));
+    int rc = 0;
+    for (int i = 0, p = cur; i < (1 << k) && !rc; ++i, p ^= 1) {
+      TZ* z = nullptr;
+      long long last;
+      rc = cg_iteration<TZ>(c, p, &z, &last);
+      if (!rc && c->gz[gi] && c->gz[gi] != (const void*)z) rc = fail(MAC_E_STATE, "V-cycle output buffer moved between cycles");
+      if (!rc) c->gz[gi] = (const void*)z;
+    }
+    if (rc) {
+      hipStreamEndCapture(c->st, &g);
+      if (g) hipGraphDestroy(g);
+      return rc;
+    }
+    HIPCHK(hipStreamEndCapture(c->st, &g));
+    hipError_t e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+    hipGraphDestroy(g);
+    if (e != hipSuccess) {
+      ge = nullptr;
+      return fail(MAC_E_HIP, (std::string("hipGraphInstantiate: ") + hipGetErrorString(e)).c_str());
+    }
+  }
+  *out = ge;
+  return 0;
+}
+
+// ||f - L x||^2 into the state's tmp (and the host mirror); r is kept.
+void true_residual(mac_ctx* c) {
+  Level& l0 = c->L[0];
+  {
+    TimedLaunch tl(c, K_INIT, (double)l0.ndom * 17);
+    k_resid0<<<grid_of(l0.nact), KTHREADS, 0, c->st>>>(l0.nact, l0.meta, l0.code, c->f, c->x, c->r, l0.ih2[0], l0.ih2[1],
+                                                       l0.ih2[2], 0, c->partials);
+  }
+  finalize(c, grid_of(l0.nact), OP_TRUE, true);
+}
+
 template <typename TZ>
 int pcg(mac_ctx* c, bool use_guess, mac_stats* st) {
   Level& l0 = c->L[0];
@@ -990,6 +1073,8 @@ int pcg(mac_ctx* c, bool use_guess, mac_stats* st) {
   int restarts = 0;
   bool fellback = false;
   int wasted = 0;  // iterations of a dropped guided start
+  int iter_now = 0;  // the device's iteration count as of the last poll
+  bool zero_x = !use_guess;  // this (re)start begins from x = 0
   // A guided solve (use_guess) falls back to the zero start when its first
   // residual is no better than the zero start's (or not finite), when it has
   // not converged after stall_iters iterations, or when it ends without
@@ -1000,131 +1085,113 @@ int pcg(mac_ctx* c, bool use_guess, mac_stats* st) {
     wasted = c->hs->iter;
     c->hs->iter = 0;
     c->hs->done = 0;
+    iter_now = 0;
+    zero_x = true;
     HIPCHK(hipMemcpyAsync(&c->s->iter, &c->hs->iter, 2 * sizeof(int), hipMemcpyHostToDevice, c->st));
     HIPCHK(hipMemsetAsync(c->x, 0, n0 * sizeof(double), c->st));
     return 0;
   };
   for (;;) {
-    HIPCHK(hipMemsetAsync(c->d[0], 0, n0 * sizeof(double), c->st));
-    int cur = 0;
+    int cur = 0;  // d[0] holds d_old of the first iteration, not read (beta = 0)
     // (re)start: r = f - L x
     {
       TimedLaunch tl(c, K_INIT, nd * 25);
       k_resid0<<<grid_of(l0.nact), KTHREADS, 0, c->st>>>(l0.nact, l0.meta, l0.code, c->f, c->x, c->r, ih[0], ih[1], ih[2], 1,
                                                          c->partials);
     }
-    finalize(c, grid_of(l0.nact), OP_RR0);
-    CHK(read_scal(c));
-    if (use_guess && !fellback && !(c->hs->rr <= c->hs->ff)) {
-      CHK(fall_back());
-      continue;
-    }
-    TZ* z = nullptr;
-    if (!c->hs->done) CHK(vcycle<TZ>(c, &z));
-    int iter_before = c->hs->iter;
-    // Without event timing the iterations run as a captured graph of two
-    // (the search directions alternate between d[0] and d[1]), and the host
-    // polls when the residual decay so far predicts convergence.
-    const int gi = std::is_same<TZ, float>::value ? 1 : 0;
-    // A grid that is rebuilt every step or two (the time-step loop) would
-    // pay a capture per solve: graphs start with the third solve on a grid.
-    const bool graphed = !c->timing && !c->hs->done && c->solves_on_grid >= 2;
-    if (graphed) {
-      if (!c->gexec[gi] || c->gz[gi] != (const void*)z) {
-        if (c->gexec[gi]) hipGraphExecDestroy(c->gexec[gi]);
-        c->gexec[gi] = nullptr;
-        hipGraph_t g = nullptr;
-        HIPCHK(hipStreamBeginCapture(c->st, hipStreamCaptureModeRelaxed));
-        TZ* zc = z;
-        int cc = 0;
-        for (int k = 0; k < 2; ++k) {
-          k_pmatvec<TZ><<<grid_of(l0.nact), KTHREADS, 0, c->st>>>(l0.nact, l0.meta, l0.code, zc, c->d[cc], c->d[cc ^ 1], c->q,
-                                                                  ih[0], ih[1], ih[2], c->partials, c->s);
-          finalize(c, grid_of(l0.nact), OP_DQ);
-          cc ^= 1;
-          k_update<<<grid_of(l0.nact), KTHREADS, 0, c->st>>>(l0.nact, l0.meta, l0.code, c->x, c->r, c->d[cc], c->q, c->partials, c->s);
-          finalize(c, grid_of(l0.nact), OP_RR);
-          int rc = vcycle<TZ>(c, &zc);
-          if (rc || zc != z) {
-            hipStreamEndCapture(c->st, &g);
-            if (g) hipGraphDestroy(g);
-            return rc ? rc : fail(MAC_E_STATE, "V-cycle output buffer moved between cycles");
-          }
-        }
-        HIPCHK(hipStreamEndCapture(c->st, &g));
-        hipError_t e = hipGraphInstantiate(&c->gexec[gi], g, nullptr, nullptr, 0);
-        hipGraphDestroy(g);
-        if (e != hipSuccess) {
-          c->gexec[gi] = nullptr;
-          return fail(MAC_E_HIP, (std::string("hipGraphInstantiate: ") + hipGetErrorString(e)).c_str());
-        }
-        c->gz[gi] = (const void*)z;
-      }
-      int pairs = 1;
-      double rr_prev = c->hs->rr;
-      while (!c->hs->done) {
-        for (int k = 0; k < pairs; ++k) HIPCHK(hipGraphLaunch(c->gexec[gi], c->st));
-        CHK(read_scal(c));
-        if (use_guess && !fellback && c->prm.stall_iters > 0 && !c->hs->done && c->hs->iter >= c->prm.stall_iters) break;
-        if (c->hs->done) break;
-        // iterations still needed at the mean decay rate so far; polling
-        // early costs one host round trip, polling late a pair of iterations
-        // whose kernels return at once
-        const int di = c->hs->iter - iter_before;
-        pairs = 1;
-        if (di > 0 && c->hs->rr > 0 && c->hs->rr < rr_prev && c->hs->tol2 > 0) {
-          const double rate = std::log(c->hs->rr / rr_prev) / di;  // < 0
-          const double need = std::log(c->hs->tol2 / c->hs->rr) / rate;
-          pairs = std::max(1, (int)std::floor(0.75 * need / 2.0));
-        }
-        if (use_guess && !fellback && c->prm.stall_iters > 0)  // stop at the first pair past stall_iters
-          pairs = std::min(pairs, std::max(1, (c->prm.stall_iters - c->hs->iter + 1) / 2));
-        rr_prev = c->hs->rr;
-        iter_before = c->hs->iter;
+    finalize(c, grid_of(l0.nact), OP_RR0, true);
+    const bool guided = use_guess && !fellback;
+    if (guided) {
+      CHK(poll_scal(c));
+      if (!(c->hs->rr <= c->hs->ff)) {
+        CHK(fall_back());
+        continue;
       }
     }
-    while (!graphed && !c->hs->done) {
-      // A batch of `check` iterations; once the update kernel of one of them
-      // sets `done`, every later kernel of the batch returns at once.
-      std::vector<long long> useq;
-      for (int k = 0; k < check; ++k) {
-        {
-          TimedLaunch tl(c, K_PMATVEC, nd * (25 + sizeof(TZ)));
-          k_pmatvec<TZ><<<grid_of(l0.nact), KTHREADS, 0, c->st>>>(l0.nact, l0.meta, l0.code, z, c->d[cur], c->d[cur ^ 1], c->q,
-                                                                  ih[0], ih[1], ih[2], c->partials, c->s);
-        }
-        finalize(c, grid_of(l0.nact), OP_DQ);
-        cur ^= 1;
-        {
-          TimedLaunch tl(c, K_UPDATE, nd * 49);
-          k_update<<<grid_of(l0.nact), KTHREADS, 0, c->st>>>(l0.nact, l0.meta, l0.code, c->x, c->r, c->d[cur], c->q, c->partials, c->s);
-        }
-        useq.push_back(finalize(c, grid_of(l0.nact), OP_RR));
-        CHK(vcycle<TZ>(c, &z));
+    // Iterations run in batches between host polls of the convergence flag:
+    // `check` before the first poll, then as many as the residual decay so
+    // far predicts are still needed, so that the last batch ends at the
+    // converging iteration (no iteration launched past it, no cycle after
+    // it).  From the third solve on a grid (kernel timing off) each
+    // iteration is one captured graph launch.
+    const bool graphed = !c->timing && c->solves_on_grid >= 2;
+    // the first batch: as many iterations as the last solve of this kind
+    // took to converge (repeated solves converge alike), else check_every
+    int batch = (restarts == 0 && c->last_iters[guided]) ? c->last_iters[guided] : check;
+    bool expect = restarts == 0 && c->last_iters[guided] > 0;  // the batch is predicted to converge
+    bool have_true = false;
+    // the decay rate is measured from this (re)start: its ||r||^2 is known
+    // for a guided start (read above) and for a zero start (= ||f||^2, read
+    // with the first poll); after a restart from a nonzero x, from the first poll
+    const int base_it = iter_now;
+    double base_rr = guided ? c->hs->rr : -1.0;
+    for (;;) {
+      if (guided && c->prm.stall_iters > 0 && batch > c->prm.stall_iters - iter_now) {
+        batch = std::max(1, c->prm.stall_iters - iter_now);
+        expect = false;
       }
+      std::vector<long long> ends;
+      const long long first = c->seq;
+      for (int k = 0; k < batch;) {
+        if (graphed) {
+          const int lg = batch - k >= 4 ? 2 : batch - k >= 2 ? 1 : 0;
+          hipGraphExec_t ge;
+          CHK(iteration_graph<TZ>(c, cur, lg, &ge));
+          HIPCHK(hipGraphLaunch(ge, c->st));
+          k += 1 << lg;
+          if (lg == 0) cur ^= 1;
+        } else {
+          TZ* z = nullptr;
+          long long last;
+          CHK(cg_iteration<TZ>(c, cur, &z, &last));
+          ends.push_back(last);
+          ++k;
+          cur ^= 1;
+        }
+      }
+      // a batch predicted to converge takes the true residual of its result
+      // along (the residual kernel runs whatever the flag says), so that a
+      // converged batch needs no second round trip
+      const long long tr_first = c->seq;
+      if (expect) true_residual(c);
+      const long long tr_last = c->seq - 1;
       HIPCHK(hipGetLastError());
-      CHK(read_scal(c));
+      CHK(poll_scal(c));
+      const int ran = c->hs->iter - iter_now;
+      iter_now = c->hs->iter;
       if (c->hs->done) {
-        int eff = c->hs->iter - iter_before;
-        long long last = eff > 0 ? useq[eff - 1] : useq[0];
-        c->noop.push_back({last + 1, c->seq - 1});
+        // launches after the converging iteration's test exited at once
+        if (!graphed && ran < (int)ends.size()) c->noop.push_back({ran > 0 ? ends[ran - 1] + 1 : first, tr_first - 1});
+        have_true = expect;
+        break;
       }
-      iter_before = c->hs->iter;
-      if (use_guess && !fellback && c->prm.stall_iters > 0 && !c->hs->done && c->hs->iter >= c->prm.stall_iters) break;
+      if (expect) c->noop.push_back({tr_first, tr_last});  // not used: timed as nothing
+      if (guided && c->prm.stall_iters > 0 && iter_now >= c->prm.stall_iters) break;
+      if (base_rr < 0.0 && zero_x) base_rr = c->hs->ff;
+      batch = check;
+      if (base_rr > 0.0 && iter_now > base_it && c->hs->rr > 0.0 && c->hs->rr < base_rr && c->hs->tol2 > 0.0) {
+        // iterations still needed at the mean decay rate so far
+        const double rate = std::log(c->hs->rr / base_rr) / (iter_now - base_it);  // < 0
+        const double need = std::log(c->hs->tol2 / c->hs->rr) / rate;
+        // the mean rate since the start overestimates the later rate of an
+        // MG-PCG (its first iterations gain the most), so need is rarely too
+        // high; a quarter iteration of slack still avoids a batch too many
+        batch = std::max(1, (int)std::ceil(need - 0.25));
+        expect = true;
+      } else {
+        expect = false;
+        if (base_rr < 0.0) base_rr = c->hs->rr;
+      }
     }
     if (use_guess && !fellback && c->hs->done != 1) {
       CHK(fall_back());
       continue;
     }
     int done = c->hs->done;
-    // true residual
-    {
-      TimedLaunch tl(c, K_INIT, nd * 17);
-      k_resid0<<<grid_of(l0.nact), KTHREADS, 0, c->st>>>(l0.nact, l0.meta, l0.code, c->f, c->x, c->r, ih[0], ih[1], ih[2], 0,
-                                                         c->partials);
+    if (!have_true) {
+      true_residual(c);
+      CHK(poll_scal(c));
     }
-    finalize(c, grid_of(l0.nact), OP_TRUE);
-    CHK(read_scal(c));
     double ff = c->hs->ff;
     double tr = ff > 0 ? std::sqrt(c->hs->tmp / ff) : 0.0;
     if (st) {
@@ -1135,11 +1202,13 @@ int pcg(mac_ctx* c, bool use_guess, mac_stats* st) {
     if (done == 1 && tr > c->prm.rtol * 1.0000001 && restarts < 3 && c->hs->iter < c->prm.max_iter) {
       // recursive residual drifted from the true one: restart from x
       ++restarts;
+      zero_x = false;
       c->hs->done = 0;
       HIPCHK(hipMemcpyAsync(&c->s->done, &c->hs->done, sizeof(int), hipMemcpyHostToDevice, c->st));
       continue;
     }
     if (st) st->restarts = restarts;
+    if (done == 1 && restarts == 0) c->last_iters[use_guess && !fellback] = c->hs->iter;
     if (done == 2) return fail(MAC_E_NOCONV, "PCG reached max_iter before rtol");
     if (done == 3) return fail(MAC_E_NOCONV, "PCG breakdown (d.q <= 0)");
     return 0;

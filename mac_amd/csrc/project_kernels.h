@@ -16,6 +16,7 @@
 // CG iteration never waits on the host.
 #pragma once
 #include "brick.h"
+#include <type_traits>
 
 struct Scal {
   double ff;      // ||f||^2
@@ -179,9 +180,14 @@ __device__ __forceinline__ void write_partial(double v, double* __restrict__ par
 }
 
 // One block: total of the partials in block order, then the scalar step.
-__global__ void k_finalize(int n, const double* __restrict__ partials, Scal* s, int op, double rtol2) {
+// mirror (pinned host memory, or null): receives a copy of the state, so a
+// host that waits for this launch reads the state without a copy of its own.
+__global__ void k_finalize(int n, const double* __restrict__ partials, Scal* s, int op, double rtol2, Scal* mirror) {
   __shared__ double sh[16];
-  if ((op == OP_DQ || op == OP_RR || op == OP_RZ) && s->done) return;
+  if ((op == OP_DQ || op == OP_RR || op == OP_RZ) && s->done) {
+    if (mirror && threadIdx.x == 0) *mirror = *s;
+    return;
+  }
   const bool mx = (op == OP_MAX);
   double acc = 0.0;
   for (int i = threadIdx.x; i < n; i += blockDim.x) acc = mx ? fmax(acc, partials[i]) : acc + partials[i];
@@ -225,6 +231,7 @@ __global__ void k_finalize(int n, const double* __restrict__ partials, Scal* s, 
       s->maxdiv = t;
       break;
   }
+  if (mirror) *mirror = *s;
 }
 
 // ---------------------------------------------------------------- setup ----
@@ -464,29 +471,38 @@ __global__ void __launch_bounds__(KTHREADS) k_pmatvec(int nact, const int* __res
   const double beta = s->beta;
   const int lane = lane_id();
   double acc = 0.0;
-  FOR_EACH_BRICK(meta, nact, M) {
-    const int b = M[0];
-    const size_t base = (size_t)b * MAC_BC + lane;
-    double col[8];
-    unsigned cd[8];
+  // beta = 0 (the first iteration of a start): d = z, and d_old, whatever it
+  // holds, is not read
+  auto sweep = [&](auto first) {
+    FOR_EACH_BRICK(meta, nact, M) {
+      const int b = M[0];
+      const size_t base = (size_t)b * MAC_BC + lane;
+      auto dval = [&](size_t c) {
+        if constexpr (decltype(first)::value) return (double)zv[c];
+        else return (double)zv[c] + beta * dold[c];
+      };
+      double col[8];
+      unsigned cd[8];
 #pragma unroll
-    for (int z = 0; z < 8; ++z) {
-      col[z] = (double)zv[base + 64 * z] + beta * dold[base + 64 * z];
-      cd[z] = code[base + 64 * z];
+      for (int z = 0; z < 8; ++z) {
+        col[z] = dval(base + 64 * z);
+        cd[z] = code[base + 64 * z];
+      }
+      col_stencil<double>(
+          M, lane, col, [&](int nb, int cell) { return dval((size_t)nb * MAC_BC + cell); },
+          [&](int z, double xm, double xp, double ym, double yp, double zm, double zp) {
+            if (cd[z] & CODE_DOM) {
+              double dg;
+              double qv = lap_cell<double>(cd[z], col[z], xm, xp, ym, yp, zm, zp, ihx2, ihy2, ihz2, dg);
+              dnew[base + 64 * z] = col[z];
+              q[base + 64 * z] = qv;
+              acc += col[z] * qv;
+            }
+          });
     }
-    col_stencil<double>(
-        M, lane, col,
-        [&](int nb, int cell) { return (double)zv[(size_t)nb * MAC_BC + cell] + beta * dold[(size_t)nb * MAC_BC + cell]; },
-        [&](int z, double xm, double xp, double ym, double yp, double zm, double zp) {
-          if (cd[z] & CODE_DOM) {
-            double dg;
-            double qv = lap_cell<double>(cd[z], col[z], xm, xp, ym, yp, zm, zp, ihx2, ihy2, ihz2, dg);
-            dnew[base + 64 * z] = col[z];
-            q[base + 64 * z] = qv;
-            acc += col[z] * qv;
-          }
-        });
-  }
+  };
+  if (beta == 0.0) sweep(std::true_type{});
+  else sweep(std::false_type{});
   write_partial(acc, partials, sh, false);
 }
 

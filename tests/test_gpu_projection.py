@@ -291,8 +291,8 @@ def test_guided_solve_falls_back_after_stall_iters():
     pr.set_pressure(0.5 * p0)  # a fair guess that still needs many iterations
     st1 = pr.Divergence_Projection(guess=True)
     assert st1["fallback_used"] == 1
-    # the stall check runs after each captured pair of iterations
-    assert st0["iters"] + 3 <= st1["iters"] <= st0["iters"] + 4, (st0["iters"], st1["iters"])
+    # the host polls at the stall iteration: exactly 3 iterations are dropped
+    assert st1["iters"] == st0["iters"] + 3, (st0["iters"], st1["iters"])
     assert st1["true_rel_resid"] <= 1e-10 * 1.0001
     assert rel(pr.get_pressure(), p0) < 1e-8
 

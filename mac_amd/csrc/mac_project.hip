@@ -856,6 +856,7 @@ int vcycle(mac_ctx* c, TZ** zout) {
       sweep<TZ>(c, l, z, zt, 0, opre[k]);
       std::swap(z, zt);
     }
+    Level& cl = c->L[l + 1];
     {
       TimedLaunch tl(c, l == 0 ? K_RESID0 : K_C_RESID, nd * (sr + 2 * sz + 1));
       if (l == 0)
@@ -865,7 +866,6 @@ int vcycle(mac_ctx* c, TZ** zout) {
         k_residual<TZ, TZ><<<grid_of(lv.nact), KTHREADS, 0, c->st>>>(lv.nact, lv.meta, lv.code, (const TZ*)lv.r, z, res, ih[0],
                                                                      ih[1], ih[2], c->s);
     }
-    Level& cl = c->L[l + 1];
     {
       // into a block level visited below: the restriction also takes that
       // level's first smoothing step from zero (brick-local)
@@ -873,10 +873,14 @@ int vcycle(mac_ctx* c, TZ** zout) {
       const bool cdeep = l + 1 >= BLOCK_DEEP && nu_of(c->prm, l + 1) > 1;
       const TZ ihc[3] = {(TZ)cl.ih2[0], (TZ)cl.ih2[1], (TZ)cl.ih2[2]};
       const double cd = (double)cl.ndom;
-      TimedLaunch tl(c, fuse ? (l == 0 ? K_RESTRICT_BLOCK1 : K_C_RESTRICT_BLOCK) : (l == 0 ? K_RESTRICT1 : K_C_RESTRICT),
-                     nd * sz + cd * sz + (fuse ? cd * (sz + 1) : 0.0));
-      k_restrict_sep<TZ><<<cl.nact, KTHREADS, 0, c->st>>>(cl.rmeta, cl.code, res, (TZ*)cl.r, c->s, fuse ? (TZ*)cl.z : nullptr,
-                                                          ihc[0], ihc[1], ihc[2], cdeep ? c->blockw_deep : c->blockw);
+      TZ* zsm = fuse ? (TZ*)cl.z : nullptr;
+      const BlockW& bw = cdeep ? c->blockw_deep : c->blockw;
+      {
+        TimedLaunch tl(c, fuse ? (l == 0 ? K_RESTRICT_BLOCK1 : K_C_RESTRICT_BLOCK) : (l == 0 ? K_RESTRICT1 : K_C_RESTRICT),
+                       nd * sz + cd * sz + (fuse ? cd * (sz + 1) : 0.0));
+        k_restrict_sep<TZ><<<cl.nact, KTHREADS, 0, c->st>>>(cl.rmeta, cl.code, res, (TZ*)cl.r, c->s, zsm, ihc[0], ihc[1], ihc[2],
+                                                            bw);
+      }
       first_done[l + 1] = fuse;
     }
     zfin[l] = z;  // pre-smoothed iterate, finished on the way up
@@ -943,7 +947,10 @@ int vcycle(mac_ctx* c, TZ** zout) {
     int k0 = 0;
     // level 0 only in FP32: the FP64 fused sweep needs 200 registers and runs
     // slower than the two launches
-    if (c->prm.fuse_coarse && (blk || (l == 0 && sizeof(TZ) == 4))) {
+#ifndef MAC_FUSE_L0
+#define MAC_FUSE_L0 1
+#endif
+    if (c->prm.fuse_coarse && (blk || (MAC_FUSE_L0 && l == 0 && sizeof(TZ) == 4))) {
       // prolongation fused with the first post-smoothing step: zt = S(z + P zc)
       if (blk) {
         TimedLaunch tl(c, K_C_PROLONG_BLOCK, nd * (3 * sz + 2) + (double)cl.ndom * sz);

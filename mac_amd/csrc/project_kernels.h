@@ -5,9 +5,10 @@
 // z-neighbours are register reads, the x/y neighbours inside the brick are
 // cross-lane shuffles (lane +-1, +-8), and only the lanes on a brick face
 // load halo values from the neighbour brick.  Every load of a brick's own
-// cells is a fully coalesced 64-lane row.  Waves walk the active-brick list
-// with a grid stride (WPB waves per 256-thread block, at most GRID_CAP
-// blocks), so a launch fills all 256 CUs and per-block set-up is amortised.
+// cells is a fully coalesced 64-lane row.  A launch has one wave per active
+// brick (WPB waves per 256-thread block), and the dispatcher balances the
+// blocks over the 256 CUs; the loops keep a grid stride for launches capped
+// at GRID_CAP blocks.
 //
 // All vectors are zero outside the domain, so the 7-point operator needs only
 // the cell's own code byte (brick.h).  Reductions write one partial per block;
@@ -37,12 +38,34 @@ struct Scal {
 
 #define KTHREADS 256
 #define WPB 4          // waves per block
-#define GRID_CAP 2048  // blocks per launch at most (8 per CU)
+#ifndef GRID_CAP
+#define GRID_CAP (1 << 30)  // no cap: one wave per brick, the dispatcher balances the CUs
+#endif
 
 enum FinOp { OP_FF = 0, OP_RR0 = 1, OP_TRUE = 2, OP_DQ = 3, OP_RR = 4, OP_RZ = 5, OP_MAX = 6 };
 
 __device__ __forceinline__ int lane_id() { return threadIdx.x & 63; }
 __device__ __forceinline__ int wave_in_block() { return threadIdx.x >> 6; }
+
+// Block order by XCD.  The dispatcher hands workgroup b of a launch to XCD
+// b % 8 (round robin over the eight XCDs, each with its own L2).  The kernels
+// walk their lists in the order of xcd_block(), which renumbers the blocks so
+// that XCD x runs one contiguous range of logical blocks: bricks that are
+// close in the (z,y,x)-sorted lists, and so mostly neighbours in space, are
+// then worked on by one XCD, and the face planes a wave reads from its
+// neighbour bricks are mostly in that XCD's L2 already.  A bijection for any
+// grid size; MAC_XCD_ORDER = 0 keeps the hardware order.
+#ifndef MAC_XCD_ORDER
+#define MAC_XCD_ORDER 1
+#endif
+#define NXCD 8
+__device__ __forceinline__ int xcd_block() {
+  if (!MAC_XCD_ORDER) return blockIdx.x;
+  const int G = gridDim.x, b = blockIdx.x;
+  const int x = b % NXCD, q = G / NXCD, rem = G % NXCD;
+  return x * q + min(x, rem) + b / NXCD;
+}
+__device__ __forceinline__ int first_wave() { return xcd_block() * WPB + wave_in_block(); }
 
 // Wave-uniform loop over the active bricks of a level.  meta holds, per
 // active brick, MSTRIDE ints: the brick id and its 6 face neighbours
@@ -50,7 +73,7 @@ __device__ __forceinline__ int wave_in_block() { return threadIdx.x >> 6; }
 // its brick with one scalar load and no dependent table lookups.
 #define MSTRIDE 8
 #define FOR_EACH_BRICK(meta, n, M)                                                            \
-  for (int i_ = blockIdx.x * WPB + wave_in_block(); i_ < (n); i_ += gridDim.x * WPB)         \
+  for (int i_ = first_wave(); i_ < (n); i_ += gridDim.x * WPB)                               \
     if (const int* M = (meta) + (size_t)MSTRIDE * __builtin_amdgcn_readfirstlane(i_); true)
 
 // The CG convergence flag read in the same round trip as the wave's first
@@ -466,7 +489,7 @@ __global__ void __launch_bounds__(KTHREADS) k_pmatvec(int nact, const int* __res
                                                       const double* __restrict__ dold, double* __restrict__ dnew,
                                                       double* __restrict__ q, double ihx2, double ihy2, double ihz2,
                                                       double* __restrict__ partials, const Scal* s) {
-  if (converged(meta, MSTRIDE, nact, blockIdx.x * WPB + wave_in_block(), s)) return;
+  if (converged(meta, MSTRIDE, nact, first_wave(), s)) return;
   __shared__ double sh[16];
   const double beta = s->beta;
   const int lane = lane_id();
@@ -511,7 +534,7 @@ __global__ void __launch_bounds__(KTHREADS) k_update(int nact, const int* __rest
                                                      double* __restrict__ x, double* __restrict__ r,
                                                      const double* __restrict__ d, const double* __restrict__ q,
                                                      double* __restrict__ partials, const Scal* s) {
-  if (converged(meta, MSTRIDE, nact, blockIdx.x * WPB + wave_in_block(), s)) return;
+  if (converged(meta, MSTRIDE, nact, first_wave(), s)) return;
   __shared__ double sh[16];
   const double alpha = s->alpha;
   const int lane = lane_id();
@@ -541,7 +564,7 @@ __global__ void __launch_bounds__(KTHREADS) k_jacobi(int nact, const int* __rest
                                                      const TZ* __restrict__ zin, TZ* __restrict__ zout, TZ ihx2, TZ ihy2,
                                                      TZ ihz2, TZ omega, int want_dot, double* __restrict__ partials,
                                                      const Scal* s) {
-  if (converged(meta, MSTRIDE, nact, blockIdx.x * WPB + wave_in_block(), s)) return;
+  if (converged(meta, MSTRIDE, nact, first_wave(), s)) return;
   __shared__ double sh[16];
   const int lane = lane_id();
   double acc = 0.0;
@@ -649,7 +672,7 @@ __global__ void __launch_bounds__(KTHREADS) k_jacobi_block(int nact, const int* 
                                                            const uint8_t* __restrict__ code, const TR* __restrict__ r,
                                                            const TZ* __restrict__ zin, TZ* __restrict__ zout, TZ ihx2, TZ ihy2,
                                                            TZ ihz2, BlockW w, const Scal* s) {
-  if (converged(meta, MSTRIDE, nact, blockIdx.x * WPB + wave_in_block(), s)) return;
+  if (converged(meta, MSTRIDE, nact, first_wave(), s)) return;
   const int lane = lane_id();
   FOR_EACH_BRICK(meta, nact, M) {
     const int b = M[0];
@@ -695,7 +718,7 @@ __global__ void __launch_bounds__(KTHREADS) k_jacobi2(int nact, const int* __res
                                                       const uint8_t* __restrict__ code, const TR* __restrict__ r,
                                                       TZ* __restrict__ zout, TZ ihx2, TZ ihy2, TZ ihz2, TZ omega,
                                                       TZ omega2, const Scal* s) {
-  if (converged(meta, MSTRIDE, nact, blockIdx.x * WPB + wave_in_block(), s)) return;
+  if (converged(meta, MSTRIDE, nact, first_wave(), s)) return;
   const int lane = lane_id();
   FOR_EACH_BRICK(meta, nact, M) {
     const int b = M[0];
@@ -768,7 +791,7 @@ template <typename T>
 __global__ void __launch_bounds__(KTHREADS) k_restrict_sep(const int* __restrict__ rmeta, const uint8_t* __restrict__ codec,
                                                            const T* __restrict__ resf, T* __restrict__ rc, const Scal* s,
                                                            T* __restrict__ zsm, T ihx2, T ihy2, T ihz2, BlockW w) {
-  if (converged(rmeta, RSTRIDE, (int)gridDim.x, (int)blockIdx.x, s)) return;
+  if (converged(rmeta, RSTRIDE, (int)gridDim.x, xcd_block(), s)) return;
   __shared__ T rcl[MAC_BC];
   if constexpr (sizeof(T) == 4) {
     // FP32: both halves at once (the 18^3 fine region fits in LDS), so the
@@ -779,7 +802,7 @@ __global__ void __launch_bounds__(KTHREADS) k_restrict_sep(const int* __restrict
     __shared__ T FA[NA];
     __shared__ T ta1[NA1];
     T* ta2 = FA;  // [fz][Y][X], 18*64, after FA is consumed
-    const int* M = rmeta + (size_t)RSTRIDE * blockIdx.x;
+    const int* M = rmeta + (size_t)RSTRIDE * xcd_block();
     const int C = M[0];
     if (threadIdx.x < 64) fba[threadIdx.x] = M[4 + threadIdx.x];
     __syncthreads();
@@ -835,7 +858,7 @@ __global__ void __launch_bounds__(KTHREADS) k_restrict_sep(const int* __restrict
   __shared__ T F[NF];
   __shared__ T t1[NT1];
   T* t2 = F;  // [fz][Y][X], 10*64, after F is consumed
-  const int* M = rmeta + (size_t)RSTRIDE * blockIdx.x;
+  const int* M = rmeta + (size_t)RSTRIDE * xcd_block();
   const int C = M[0];
   if (threadIdx.x < 64) fb[threadIdx.x] = M[4 + threadIdx.x];
   for (int half = 0; half < 2; ++half) {
@@ -902,10 +925,10 @@ template <typename T>
 __global__ void __launch_bounds__(KTHREADS) k_prolong_col(int nact, const int* __restrict__ pmeta,
                                                           const uint8_t* __restrict__ codef, const T* __restrict__ zc,
                                                           T* __restrict__ zf, const Scal* s) {
-  if (converged(pmeta, PSTRIDE, nact, blockIdx.x * WPB + wave_in_block(), s)) return;
+  if (converged(pmeta, PSTRIDE, nact, first_wave(), s)) return;
   const int lane = lane_id();
   const int x = lane & 7, y = lane >> 3;
-  for (int i_ = blockIdx.x * WPB + wave_in_block(); i_ < nact; i_ += gridDim.x * WPB) {
+  for (int i_ = first_wave(); i_ < nact; i_ += gridDim.x * WPB) {
     const int* M = pmeta + (size_t)PSTRIDE * __builtin_amdgcn_readfirstlane(i_);
     const int B = M[0], o = M[1];
     const int ox = o & 1, oy = (o >> 1) & 1, oz = o >> 2;
@@ -1059,9 +1082,9 @@ __global__ void __launch_bounds__(KTHREADS) k_prolong_block(int nact, const int*
                                                             const uint8_t* __restrict__ code, const T* __restrict__ r,
                                                             const T* __restrict__ zc, const T* __restrict__ zfin,
                                                             T* __restrict__ zout, T ihx2, T ihy2, T ihz2, BlockW w, const Scal* s) {
-  if (converged(pmeta, PSTRIDE, nact, blockIdx.x * WPB + wave_in_block(), s)) return;
+  if (converged(pmeta, PSTRIDE, nact, first_wave(), s)) return;
   const int lane = lane_id();
-  for (int i_ = blockIdx.x * WPB + wave_in_block(); i_ < nact; i_ += gridDim.x * WPB) {
+  for (int i_ = first_wave(); i_ < nact; i_ += gridDim.x * WPB) {
     const int* M = pmeta + (size_t)PSTRIDE * __builtin_amdgcn_readfirstlane(i_);
     const size_t base = (size_t)M[0] * MAC_BC + lane;
     ProlongedBrick<T> pb;
@@ -1084,17 +1107,21 @@ __global__ void __launch_bounds__(KTHREADS) k_prolong_block(int nact, const int*
 // zout = zin + omega D^-1 (r - L zin), zin = zfin + P zc (k_prolong_col then
 // k_jacobi, without writing and re-reading zin).  want_dot: partial r . zout.
 template <typename TZ>
-__global__ void __launch_bounds__(KTHREADS) k_prolong_jacobi(int nact, const int* __restrict__ pmeta,
+#ifndef MAC_PJ_WAVES
+#define MAC_PJ_WAVES 4
+#endif
+__global__ void __launch_bounds__(KTHREADS) __attribute__((amdgpu_waves_per_eu(MAC_PJ_WAVES ? MAC_PJ_WAVES : 1)))
+k_prolong_jacobi(int nact, const int* __restrict__ pmeta,
                                                              const uint8_t* __restrict__ code, const double* __restrict__ r,
                                                              const TZ* __restrict__ zc, const TZ* __restrict__ zfin,
                                                              TZ* __restrict__ zout, TZ ihx2, TZ ihy2, TZ ihz2, TZ omega,
                                                              int want_dot, double* __restrict__ partials, const Scal* s) {
-  if (converged(pmeta, PSTRIDE, nact, blockIdx.x * WPB + wave_in_block(), s)) return;
+  if (converged(pmeta, PSTRIDE, nact, first_wave(), s)) return;
   __shared__ double sh[16];
   const int lane = lane_id();
   const int x = lane & 7, y = lane >> 3;
   double acc = 0.0;
-  for (int i_ = blockIdx.x * WPB + wave_in_block(); i_ < nact; i_ += gridDim.x * WPB) {
+  for (int i_ = first_wave(); i_ < nact; i_ += gridDim.x * WPB) {
     const int* M = pmeta + (size_t)PSTRIDE * __builtin_amdgcn_readfirstlane(i_);
     const size_t base = (size_t)M[0] * MAC_BC + lane;
     ProlongedBrick<TZ> pb;
@@ -1129,7 +1156,7 @@ __global__ void __launch_bounds__(KTHREADS) k_residual(int nact, const int* __re
                                                        const uint8_t* __restrict__ code, const TR* __restrict__ r,
                                                        const TZ* __restrict__ zv, TZ* __restrict__ res, TZ ihx2, TZ ihy2,
                                                        TZ ihz2, const Scal* s) {
-  if (converged(meta, MSTRIDE, nact, blockIdx.x * WPB + wave_in_block(), s)) return;
+  if (converged(meta, MSTRIDE, nact, first_wave(), s)) return;
   const int lane = lane_id();
   FOR_EACH_BRICK(meta, nact, M) {
     const int b = M[0];

@@ -667,6 +667,117 @@ __device__ __forceinline__ void inv_diag_col(const unsigned (&cd)[8], TZ ihx2, T
   }
 }
 
+// The block step with one brick per 256-thread block (the slab form, for the
+// coarse levels, whose few bricks leave most of the chip idle when a wave
+// carries a whole brick): wave w holds z-planes 2w and 2w+1 of the brick, the
+// same lane = x + 8y columns, so a sweep's serial chain is two cells instead
+// of eight.  Each sweep publishes the wave's two planes in LDS (double
+// buffered, one barrier per sweep) and takes the planes 2w-1 and 2w+2 of the
+// neighbouring waves from there; the frozen face values come in as for
+// block_sweeps (below for wave 0, above for wave 3).  Every cell update is
+// the expression of block_sweeps with the same operands, so both forms give
+// identical bits.
+template <typename T>
+struct SlabLDS {
+  T pl[2][WPB][2][64];  // [buffer][wave][low/high plane][lane]
+};
+template <typename TZ>
+__device__ __forceinline__ void slab_sweeps(int wv, int lane, const unsigned (&cd)[2], const TZ (&rv)[2], const TZ (&idg)[2],
+                                            TZ (&col)[2], const TZ (&gx)[2], const TZ (&gy)[2], TZ below, TZ above, TZ ihx2,
+                                            TZ ihy2, TZ ihz2, const BlockW& w, SlabLDS<TZ>& L) {
+  const int x = lane & 7, y = lane >> 3;
+  for (int k = 0; k < w.m; ++k) {
+    const TZ om = sizeof(TZ) == 4 ? (TZ)w.om[k] : (TZ)w.om64[k];
+    auto& P = L.pl[k & 1];
+    P[wv][0][lane] = col[0];
+    P[wv][1][lane] = col[1];
+    __syncthreads();
+    const TZ lo = wv > 0 ? P[wv - 1][1][lane] : below;
+    const TZ hi = wv < WPB - 1 ? P[wv + 1][0][lane] : above;
+    TZ nw[2];
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      const TZ v = col[j];
+      TZ xm = __shfl_up(v, 1), xp = __shfl_down(v, 1), ym = __shfl_up(v, 8), yp = __shfl_down(v, 8);
+      if (x == 0) xm = gx[j];
+      if (x == 7) xp = gx[j];
+      if (y == 0) ym = gy[j];
+      if (y == 7) yp = gy[j];
+      const TZ zm = j ? col[0] : lo;
+      const TZ zp = j ? hi : col[1];
+      TZ out = TZ(0);
+      if (cd[j] & CODE_DOM) {
+        TZ dg;
+        const TZ lz = lap_cell<TZ>(cd[j], v, xm, xp, ym, yp, zm, zp, ihx2, ihy2, ihz2, dg);
+        if constexpr (sizeof(TZ) == 4)
+          out = idg[j] != TZ(0) ? v + om * (rv[j] - lz) * idg[j] : TZ(0);
+        else
+          out = dg > TZ(0) ? v + om * (rv[j] - lz) / dg : TZ(0);
+      }
+      nw[j] = out;
+    }
+    col[0] = nw[0];
+    col[1] = nw[1];
+  }
+}
+
+template <typename TZ>
+__device__ __forceinline__ void inv_diag_slab(const unsigned (&cd)[2], TZ ihx2, TZ ihy2, TZ ihz2, TZ (&idg)[2]) {
+#pragma unroll
+  for (int j = 0; j < 2; ++j) {
+    const TZ dg = diag_of<TZ>(cd[j], ihx2, ihy2, ihz2);
+    idg[j] = ((cd[j] & CODE_DOM) && dg > TZ(0)) ? TZ(1) / dg : TZ(0);
+  }
+}
+
+// k_jacobi_block in the slab form: one block per active brick.
+template <typename TR, typename TZ>
+__global__ void __launch_bounds__(KTHREADS) k_jacobi_block_slab(int nact, const int* __restrict__ meta,
+                                                                const uint8_t* __restrict__ code, const TR* __restrict__ r,
+                                                                const TZ* __restrict__ zin, TZ* __restrict__ zout, TZ ihx2,
+                                                                TZ ihy2, TZ ihz2, BlockW w, const Scal* s) {
+  const int i = xcd_block();
+  if (converged(meta, MSTRIDE, nact, i, s)) return;
+  __shared__ SlabLDS<TZ> L;
+  const int lane = lane_id(), wv = wave_in_block();
+  const int x = lane & 7, y = lane >> 3;
+  const int* M = meta + (size_t)MSTRIDE * i;
+  const size_t base = (size_t)M[0] * MAC_BC + lane + 128 * wv;
+  unsigned cd[2];
+  TZ col[2], rv[2];
+#pragma unroll
+  for (int j = 0; j < 2; ++j) {
+    cd[j] = code[base + 64 * j];
+    rv[j] = (TZ)r[base + 64 * j];
+    col[j] = zin ? zin[base + 64 * j] : TZ(0);
+  }
+  TZ gx[2] = {TZ(0), TZ(0)}, gy[2] = {TZ(0), TZ(0)}, below = TZ(0), above = TZ(0);
+  if (zin) {
+    auto ld = [&](int nb, int cell) { return zin[(size_t)nb * MAC_BC + cell]; };
+    const TZ hxm = M[1] >= 0 ? ld(M[1], 7 + 8 * x + 64 * y) : TZ(0);  // lane = y' + 8 z'
+    const TZ hxp = M[2] >= 0 ? ld(M[2], 8 * x + 64 * y) : TZ(0);
+    const TZ hym = M[3] >= 0 ? ld(M[3], x + 56 + 64 * y) : TZ(0);  // lane = x' + 8 z'
+    const TZ hyp = M[4] >= 0 ? ld(M[4], x + 64 * y) : TZ(0);
+    if (wv == 0 && M[5] >= 0) below = ld(M[5], lane + 448);
+    if (wv == WPB - 1 && M[6] >= 0) above = ld(M[6], lane);
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      const int z = 2 * wv + j;
+      const TZ gxm = __shfl(hxm, y + 8 * z), gxp = __shfl(hxp, y + 8 * z);
+      const TZ gym = __shfl(hym, x + 8 * z), gyp = __shfl(hyp, x + 8 * z);
+      gx[j] = x == 0 ? gxm : gxp;
+      gy[j] = y == 0 ? gym : gyp;
+    }
+  }
+  const TZ col0[2] = {col[0], col[1]};
+  TZ idg[2];
+  inv_diag_slab<TZ>(cd, ihx2, ihy2, ihz2, idg);
+  slab_sweeps<TZ>(wv, lane, cd, rv, idg, col, gx, gy, below, above, ihx2, ihy2, ihz2, w, L);
+  const TZ dmp = (TZ)w.damp;
+#pragma unroll
+  for (int j = 0; j < 2; ++j) zout[base + 64 * j] = col0[j] + dmp * (col[j] - col0[j]);
+}
+
 template <typename TR, typename TZ>
 __global__ void __launch_bounds__(KTHREADS) k_jacobi_block(int nact, const int* __restrict__ meta,
                                                            const uint8_t* __restrict__ code, const TR* __restrict__ r,
@@ -766,25 +877,28 @@ __global__ void __launch_bounds__(KTHREADS) k_jacobi2(int nact, const int* __res
 // at 0, so the step needs nothing outside the brick): wave 0 takes the
 // restricted brick from LDS and writes zsm.
 #define RSTRIDE 68
+// The coarse level's first block step from zero on the restricted brick in
+// LDS (rcl), in the slab form, run by all four waves of the block.
 template <typename T>
-__device__ __forceinline__ void smooth_from_zero(int C, const uint8_t* __restrict__ codec, const T* rcl, T* __restrict__ zsm,
-                                                 T ihx2, T ihy2, T ihz2, const BlockW& w) {
-  const int lane = threadIdx.x;
-  const size_t base = (size_t)C * MAC_BC + lane;
-  unsigned cd[8];
-  T rv[8], col[8], gz[8], idg[8];
+__device__ __forceinline__ void smooth_from_zero_slab(int C, const uint8_t* __restrict__ codec, const T* rcl,
+                                                      T* __restrict__ zsm, T ihx2, T ihy2, T ihz2, const BlockW& w,
+                                                      SlabLDS<T>& L) {
+  const int lane = lane_id(), wv = wave_in_block();
+  const size_t base = (size_t)C * MAC_BC + lane + 128 * wv;
+  unsigned cd[2];
+  T rv[2], col[2], gz[2], idg[2];
 #pragma unroll
-  for (int z = 0; z < 8; ++z) {
-    cd[z] = codec[base + 64 * z];
-    rv[z] = rcl[lane + 64 * z];
-    col[z] = T(0);
-    gz[z] = T(0);
+  for (int j = 0; j < 2; ++j) {
+    cd[j] = codec[base + 64 * j];
+    rv[j] = rcl[lane + 128 * wv + 64 * j];
+    col[j] = T(0);
+    gz[j] = T(0);
   }
-  inv_diag_col<T>(cd, ihx2, ihy2, ihz2, idg);
-  block_sweeps<T>(lane, cd, rv, idg, col, gz, gz, T(0), T(0), ihx2, ihy2, ihz2, w);
+  inv_diag_slab<T>(cd, ihx2, ihy2, ihz2, idg);
+  slab_sweeps<T>(wv, lane, cd, rv, idg, col, gz, gz, T(0), T(0), ihx2, ihy2, ihz2, w, L);
   const T dmp = (T)w.damp;
 #pragma unroll
-  for (int z = 0; z < 8; ++z) zsm[base + 64 * z] = T(0) + dmp * (col[z] - T(0));
+  for (int j = 0; j < 2; ++j) zsm[base + 64 * j] = T(0) + dmp * (col[j] - T(0));
 }
 
 template <typename T>
@@ -847,8 +961,9 @@ __global__ void __launch_bounds__(KTHREADS) k_restrict_sep(const int* __restrict
       rcl[t] = acc;
     }
     if (zsm) {
+      __shared__ SlabLDS<T> L;
       __syncthreads();
-      if (threadIdx.x < 64) smooth_from_zero<T>(C, codec, rcl, zsm, ihx2, ihy2, ihz2, w);
+      smooth_from_zero_slab<T>(C, codec, rcl, zsm, ihx2, ihy2, ihz2, w, L);
     }
     return;
   }
@@ -906,8 +1021,9 @@ __global__ void __launch_bounds__(KTHREADS) k_restrict_sep(const int* __restrict
     }
   }
   if (zsm) {
+    __shared__ SlabLDS<T> L;
     __syncthreads();
-    if (threadIdx.x < 64) smooth_from_zero<T>(C, codec, rcl, zsm, ihx2, ihy2, ihz2, w);
+    smooth_from_zero_slab<T>(C, codec, rcl, zsm, ihx2, ihy2, ihz2, w, L);
   }
 }
 
@@ -1101,6 +1217,71 @@ __global__ void __launch_bounds__(KTHREADS) k_prolong_block(int nact, const int*
 #pragma unroll
     for (int z = 0; z < 8; ++z) zout[base + 64 * z] = col0[z] + dmp * (pb.col[z] - col0[z]);
   }
+}
+
+// k_prolong_block in the slab form: one block per active fine brick, wave w
+// on z-planes 2w, 2w+1 (the prolongated values formed as in ProlongedBrick).
+template <typename T>
+__global__ void __launch_bounds__(KTHREADS) k_prolong_block_slab(int nact, const int* __restrict__ pmeta,
+                                                                 const uint8_t* __restrict__ code, const T* __restrict__ r,
+                                                                 const T* __restrict__ zc, const T* __restrict__ zfin,
+                                                                 T* __restrict__ zout, T ihx2, T ihy2, T ihz2, BlockW w,
+                                                                 const Scal* s) {
+  const int i = xcd_block();
+  if (converged(pmeta, PSTRIDE, nact, i, s)) return;
+  __shared__ SlabLDS<T> L;
+  const int lane = lane_id(), wv = wave_in_block();
+  const int x = lane & 7, y = lane >> 3;
+  const int* M = pmeta + (size_t)PSTRIDE * i;
+  const int* F = M + 10;  // face neighbours, the order of meta
+  const size_t base = (size_t)M[0] * MAC_BC + lane + 128 * wv;
+  ProlongCols<T> pc;
+  pc.gather(M, lane, zc);
+  unsigned cd[2];
+  T col[2], rv[2];
+#pragma unroll
+  for (int j = 0; j < 2; ++j) {
+    cd[j] = code[base + 64 * j];
+    rv[j] = r[base + 64 * j];
+    const T v = zfin[base + 64 * j];
+    col[j] = (cd[j] & CODE_DOM) ? v + pc.at(2 * wv + j, 0) : v;
+  }
+  auto ldz = [&](int nb, int cell) { return nb >= 0 ? zfin[(size_t)nb * MAC_BC + cell] : T(0); };
+  auto ldc = [&](int nb, int cell) { return nb >= 0 ? (unsigned)code[(size_t)nb * MAC_BC + cell] : 0u; };
+  const T zxm = ldz(F[0], 7 + 8 * x + 64 * y), zxp = ldz(F[1], 8 * x + 64 * y);
+  const T zym = ldz(F[2], x + 56 + 64 * y), zyp = ldz(F[3], x + 64 * y);
+  const unsigned cxm = ldc(F[0], 7 + 8 * x + 64 * y), cxp = ldc(F[1], 8 * x + 64 * y);
+  const unsigned cym = ldc(F[2], x + 56 + 64 * y), cyp = ldc(F[3], x + 64 * y);
+  T gx[2], gy[2], below = T(0), above = T(0);
+#pragma unroll
+  for (int j = 0; j < 2; ++j) {
+    const int z = 2 * wv + j;
+    const T vxm = __shfl(zxm, y + 8 * z), vxp = __shfl(zxp, y + 8 * z);
+    const T vym = __shfl(zym, x + 8 * z), vyp = __shfl(zyp, x + 8 * z);
+    const unsigned gcxm = __shfl(cxm, y + 8 * z), gcxp = __shfl(cxp, y + 8 * z);
+    const unsigned gcym = __shfl(cym, x + 8 * z), gcyp = __shfl(cyp, x + 8 * z);
+    const T vx = x == 0 ? vxm : vxp;
+    const unsigned cx = x == 0 ? gcxm : gcxp;
+    const T vy = y == 0 ? vym : vyp;
+    const unsigned cy = y == 0 ? gcym : gcyp;
+    gx[j] = (cx & CODE_DOM) ? vx + pc.at(z, 1) : vx;
+    gy[j] = (cy & CODE_DOM) ? vy + pc.at(z, 2) : vy;
+  }
+  if (wv == 0) {
+    const T zb = ldz(F[4], lane + 448);
+    below = (ldc(F[4], lane + 448) & CODE_DOM) ? zb + pc.at(-1, 0) : zb;
+  }
+  if (wv == WPB - 1) {
+    const T za = ldz(F[5], lane);
+    above = (ldc(F[5], lane) & CODE_DOM) ? za + pc.at(8, 0) : za;
+  }
+  const T col0[2] = {col[0], col[1]};
+  T idg[2];
+  inv_diag_slab<T>(cd, ihx2, ihy2, ihz2, idg);
+  slab_sweeps<T>(wv, lane, cd, rv, idg, col, gx, gy, below, above, ihx2, ihy2, ihz2, w, L);
+  const T dmp = (T)w.damp;
+#pragma unroll
+  for (int j = 0; j < 2; ++j) zout[base + 64 * j] = col0[j] + dmp * (col[j] - col0[j]);
 }
 
 // Prolongation into level 0 fused with the first post-smoothing Jacobi sweep:

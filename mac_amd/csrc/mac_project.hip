@@ -1094,7 +1094,7 @@ void true_residual(mac_ctx* c) {
 }
 
 template <typename TZ>
-int pcg(mac_ctx* c, bool use_guess, mac_stats* st) {
+int pcg(mac_ctx* c, bool use_guess, bool r_ready, mac_stats* st) {
   Level& l0 = c->L[0];
   const double* ih = l0.ih2;
   size_t n0 = (size_t)c->nb0 * MAC_BC;
@@ -1124,13 +1124,18 @@ int pcg(mac_ctx* c, bool use_guess, mac_stats* st) {
   };
   for (;;) {
     int cur = 0;  // d[0] holds d_old of the first iteration, not read (beta = 0)
-    // (re)start: r = f - L x
-    {
-      TimedLaunch tl(c, K_INIT, nd * 25);
-      k_resid0<<<grid_of(l0.nact), KTHREADS, 0, c->st>>>(l0.nact, l0.meta, l0.code, c->f, c->x, c->r, ih[0], ih[1], ih[2], 1,
-                                                         c->partials);
+    // (re)start: r = f - L x (the first zero start has r = f and ||r||^2 from
+    // the divergence pass)
+    if (r_ready) {
+      r_ready = false;
+    } else {
+      {
+        TimedLaunch tl(c, K_INIT, nd * 25);
+        k_resid0<<<grid_of(l0.nact), KTHREADS, 0, c->st>>>(l0.nact, l0.meta, l0.code, c->f, c->x, c->r, ih[0], ih[1], ih[2], 1,
+                                                           c->partials);
+      }
+      finalize(c, grid_of(l0.nact), OP_RR0, true);
     }
-    finalize(c, grid_of(l0.nact), OP_RR0, true);
     const bool guided = use_guess && !fellback;
     if (guided) {
       CHK(poll_scal(c));
@@ -1435,13 +1440,16 @@ int mac_project(mac_ctx* c, int guess, mac_stats* st) {
   HIPCHK(hipMemcpyAsync(c->s, &init, sizeof(Scal), hipMemcpyHostToDevice, c->st));
   c->seq = 0;
   const double nd = (double)l0.ndom;
-  if (l0.nact > 0) {
-    TimedLaunch tl(c, K_DIV, nd * 33);
-    k_div<<<grid_of(l0.nact), KTHREADS, 0, c->st>>>(l0.nact, l0.meta, l0.code, c->u, c->v, c->w, inv_h[0], inv_h[1], inv_h[2],
-                                                    c->f, nullptr, c->partials);
-  }
-  if (l0.nact > 0) finalize(c, grid_of(l0.nact), OP_FF);
   const bool visc = c->prm.viscosity > 0;
+  // a zero start without the viscous right-hand side term takes r = f from
+  // the divergence pass (no residual pass of its own)
+  const bool r_ready = !visc && !guess;
+  if (l0.nact > 0) {
+    TimedLaunch tl(c, K_DIV, nd * (r_ready ? 41 : 33));
+    k_div<<<grid_of(l0.nact), KTHREADS, 0, c->st>>>(l0.nact, l0.meta, l0.code, c->u, c->v, c->w, inv_h[0], inv_h[1], inv_h[2],
+                                                    c->f, nullptr, r_ready ? c->r : nullptr, c->partials);
+  }
+  if (l0.nact > 0) finalize(c, grid_of(l0.nact), r_ready ? OP_FF_R0 : OP_FF);
   if (visc) {
     if (!c->pD) CHK(dalloc(c, &c->pD, (size_t)c->nb0 * MAC_BC));
     k_air_pressure<<<c->nb0, KTHREADS, 0, c->st>>>(l0.nbr, c->flags, c->ur ? c->ur : c->u, c->vr ? c->vr : c->v,
@@ -1458,8 +1466,8 @@ int mac_project(mac_ctx* c, int guess, mac_stats* st) {
   auto ts0 = std::chrono::steady_clock::now();
   int rc = 0;
   if (l0.nact > 0) {
-    if (c->prm.precision == MAC_MIXED) rc = pcg<float>(c, guess != 0, st);
-    else rc = pcg<double>(c, guess != 0, st);
+    if (c->prm.precision == MAC_MIXED) rc = pcg<float>(c, guess != 0, r_ready, st);
+    else rc = pcg<double>(c, guess != 0, r_ready, st);
     c->solves_on_grid += 1;
   } else {
     HIPCHK(hipMemsetAsync(c->x, 0, (size_t)c->nb0 * MAC_BC * sizeof(double), c->st));
@@ -1469,8 +1477,8 @@ int mac_project(mac_ctx* c, int guess, mac_stats* st) {
   if (visc) k_dirichlet_store<<<c->nb0, KTHREADS, 0, c->st>>>(c->nb0, l0.code, c->pD, c->x);
   if (c->nglist > 0) {
     TimedLaunch tl(c, K_GRAD, nd * 57);
-    k_gradient<<<c->nglist, KTHREADS, 0, c->st>>>(c->glist, l0.nbr, c->flags, l0.code, c->x, c->u, c->v, c->w, inv_h[0], inv_h[1],
-                                                  inv_h[2]);
+    k_gradient<<<grid_of(c->nglist), KTHREADS, 0, c->st>>>(c->nglist, c->glist, l0.nbr, c->flags, l0.code, c->x, c->u, c->v,
+                                                            c->w, inv_h[0], inv_h[1], inv_h[2]);
   }
   if (l0.nact > 0) {
     TimedLaunch tl(c, K_MAXDIV, nd * 25);
@@ -1500,7 +1508,7 @@ int mac_divergence(mac_ctx* c, double* div_out) {
   HIPCHK(hipMemsetAsync(c->s, 0, sizeof(Scal), c->st));
   if (l0.nact > 0)
     k_div<<<grid_of(l0.nact), KTHREADS, 0, c->st>>>(l0.nact, l0.meta, l0.code, c->u, c->v, c->w, inv_h[0], inv_h[1], inv_h[2],
-                                                    c->f, c->tmpv, c->partials);
+                                                    c->f, c->tmpv, nullptr, c->partials);
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpyAsync(div_out, c->tmpv, n0 * sizeof(double), hipMemcpyDeviceToHost, c->st));
   HIPCHK(hipStreamSynchronize(c->st));
@@ -1528,8 +1536,8 @@ int mac_gradient_update(mac_ctx* c) {
   double inv_h[3];
   for (int i = 0; i < 3; ++i) inv_h[i] = 1.0 / c->prm.h[i];
   if (c->nglist > 0)
-    k_gradient<<<c->nglist, KTHREADS, 0, c->st>>>(c->glist, l0.nbr, c->flags, l0.code, c->x, c->u, c->v, c->w, inv_h[0], inv_h[1],
-                                                  inv_h[2]);
+    k_gradient<<<grid_of(c->nglist), KTHREADS, 0, c->st>>>(c->nglist, c->glist, l0.nbr, c->flags, l0.code, c->x, c->u, c->v,
+                                                            c->w, inv_h[0], inv_h[1], inv_h[2]);
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(c->st));
   return 0;
@@ -1605,7 +1613,7 @@ MAC_HIDDEN int mac_internal_rhs(mac_ctx* c) {
   HIPCHK(hipMemsetAsync(c->s, 0, sizeof(Scal), c->st));
   if (l0.nact > 0) {
     k_div<<<grid_of(l0.nact), KTHREADS, 0, c->st>>>(l0.nact, l0.meta, l0.code, c->u, c->v, c->w, inv_h[0], inv_h[1], inv_h[2],
-                                                    c->f, nullptr, c->partials);
+                                                    c->f, nullptr, nullptr, c->partials);
     finalize(c, grid_of(l0.nact), OP_FF);
   }
   HIPCHK(hipGetLastError());

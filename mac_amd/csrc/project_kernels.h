@@ -42,7 +42,7 @@ struct Scal {
 #define GRID_CAP (1 << 30)  // no cap: one wave per brick, the dispatcher balances the CUs
 #endif
 
-enum FinOp { OP_FF = 0, OP_RR0 = 1, OP_TRUE = 2, OP_DQ = 3, OP_RR = 4, OP_RZ = 5, OP_MAX = 6 };
+enum FinOp { OP_FF = 0, OP_RR0 = 1, OP_TRUE = 2, OP_DQ = 3, OP_RR = 4, OP_RZ = 5, OP_MAX = 6, OP_FF_R0 = 7 };
 
 __device__ __forceinline__ int lane_id() { return threadIdx.x & 63; }
 __device__ __forceinline__ int wave_in_block() { return threadIdx.x >> 6; }
@@ -221,6 +221,15 @@ __global__ void k_finalize(int n, const double* __restrict__ partials, Scal* s, 
       s->ff = t;
       s->tol2 = rtol2 * t;
       s->done = (t == 0.0) ? 1 : 0;
+      break;
+    case OP_FF_R0:
+      // ||f||^2 of a divergence pass that also wrote r = f (a zero start):
+      // the OP_FF and OP_RR0 steps with ||r||^2 = ||f||^2
+      s->ff = t;
+      s->tol2 = rtol2 * t;
+      s->rr = t;
+      s->rz_valid = 0;
+      s->done = (t == 0.0 || t <= s->tol2) ? 1 : 0;
       break;
     case OP_RR0:
       s->rr = t;
@@ -410,7 +419,7 @@ __global__ void __launch_bounds__(KTHREADS) k_div(int nact, const int* __restric
                                                   const uint8_t* __restrict__ code, const double* __restrict__ u,
                                                   const double* __restrict__ v, const double* __restrict__ w, double ihx,
                                                   double ihy, double ihz, double* __restrict__ f, double* __restrict__ div_out,
-                                                  double* __restrict__ partials) {
+                                                  double* __restrict__ r0, double* __restrict__ partials) {
   __shared__ double sh[16];
   const int lane = lane_id();
   double acc = 0.0;
@@ -424,6 +433,7 @@ __global__ void __launch_bounds__(KTHREADS) k_div(int nact, const int* __restric
       double dv = dom ? dc.d[z] : 0.0;
       if (div_out) div_out[base + 64 * z] = dv;
       f[base + 64 * z] = -dv;
+      if (r0) r0[base + 64 * z] = -dv;  // the residual of a zero start
       acc += dv * dv;
     }
   }
@@ -1219,6 +1229,52 @@ __global__ void __launch_bounds__(KTHREADS) k_prolong_block(int nact, const int*
   }
 }
 
+// The coarse columns of ProlongCols that wave w of a slab-form block reads:
+// the 3 coarse z-levels bz-1+w .. bz+1+w (local levels 0..2) cover its planes
+// 2w, 2w+1 and the halo plane below (w = 0) or above (w = 3).  at() takes
+// its two levels as constants, so the columns stay in registers, and forms
+// the value with the expressions of ProlongCols::at (identical bits).
+template <typename T>
+struct SlabCols {
+  T c00[3], c10[3], c01[3], c11[3];
+  __device__ __forceinline__ void gather(const int* M, int lane, int wv, const T* __restrict__ zc) {
+    const int x = lane & 7, y = lane >> 3;
+    const int o = M[1];
+    const int ox = o & 1, oy = (o >> 1) & 1, oz = o >> 2;
+    const int bb0 = M[2], bb1 = M[3], bb2 = M[4], bb3 = M[5], bb4 = M[6], bb5 = M[7], bb6 = M[8], bb7 = M[9];
+    const int bx = 4 * ox, by = 4 * oy, bz = 4 * oz;
+    const int X0 = bx + (x >> 1), Y0 = by + (y >> 1);
+    const int X1 = X0 + ((x & 1) ? 1 : -1), Y1 = Y0 + ((y & 1) ? 1 : -1);
+    const bool sx = (X1 < 0 || X1 > 7), sy = (Y1 < 0 || Y1 > 7);
+    const int k00 = bb0, k10 = sx ? bb1 : bb0, k01 = sy ? bb2 : bb0, k11 = sx ? (sy ? bb3 : bb1) : (sy ? bb2 : bb0);
+    const int z00 = bb4, z10 = sx ? bb5 : bb4, z01 = sy ? bb6 : bb4, z11 = sx ? (sy ? bb7 : bb5) : (sy ? bb6 : bb4);
+    auto g = [&](int brk, int X, int Y, int Z) -> T {
+      return brk >= 0 ? zc[(size_t)brk * MAC_BC + (X & 7) + 8 * (Y & 7) + 64 * (Z & 7)] : T(0);
+    };
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      const int Z = bz - 1 + wv + k;
+      const bool out = (Z < 0 || Z > 7);
+      c00[k] = g(out ? z00 : k00, X0, Y0, Z);
+      c10[k] = g(out ? z10 : k10, X1, Y0, Z);
+      c01[k] = g(out ? z01 : k01, X0, Y1, Z);
+      c11[k] = g(out ? z11 : k11, X1, Y1, Z);
+    }
+  }
+  template <int K0, int K1>
+  __device__ __forceinline__ T at(int face) const {
+    const T a00 = T(0.75) * c00[K0] + T(0.25) * c00[K1];
+    const T a10 = T(0.75) * c10[K0] + T(0.25) * c10[K1];
+    const T a01 = T(0.75) * c01[K0] + T(0.25) * c01[K1];
+    const T a11 = T(0.75) * c11[K0] + T(0.25) * c11[K1];
+    if (face == 1) return T(0.75) * (T(0.75) * a10 + T(0.25) * a11) + T(0.25) * (T(0.75) * a00 + T(0.25) * a01);
+    if (face == 2) return T(0.75) * (T(0.75) * a01 + T(0.25) * a00) + T(0.25) * (T(0.75) * a11 + T(0.25) * a10);
+    return T(0.75) * (T(0.75) * a00 + T(0.25) * a01) + T(0.25) * (T(0.75) * a10 + T(0.25) * a11);
+  }
+  // plane 2w + j (j = 0, 1)
+  __device__ __forceinline__ T at_plane(int j, int face) const { return j ? at<1, 2>(face) : at<1, 0>(face); }
+};
+
 // k_prolong_block in the slab form: one block per active fine brick, wave w
 // on z-planes 2w, 2w+1 (the prolongated values formed as in ProlongedBrick).
 template <typename T>
@@ -1235,8 +1291,8 @@ __global__ void __launch_bounds__(KTHREADS) k_prolong_block_slab(int nact, const
   const int* M = pmeta + (size_t)PSTRIDE * i;
   const int* F = M + 10;  // face neighbours, the order of meta
   const size_t base = (size_t)M[0] * MAC_BC + lane + 128 * wv;
-  ProlongCols<T> pc;
-  pc.gather(M, lane, zc);
+  SlabCols<T> pc;
+  pc.gather(M, lane, wv, zc);
   unsigned cd[2];
   T col[2], rv[2];
 #pragma unroll
@@ -1244,7 +1300,7 @@ __global__ void __launch_bounds__(KTHREADS) k_prolong_block_slab(int nact, const
     cd[j] = code[base + 64 * j];
     rv[j] = r[base + 64 * j];
     const T v = zfin[base + 64 * j];
-    col[j] = (cd[j] & CODE_DOM) ? v + pc.at(2 * wv + j, 0) : v;
+    col[j] = (cd[j] & CODE_DOM) ? v + pc.at_plane(j, 0) : v;
   }
   auto ldz = [&](int nb, int cell) { return nb >= 0 ? zfin[(size_t)nb * MAC_BC + cell] : T(0); };
   auto ldc = [&](int nb, int cell) { return nb >= 0 ? (unsigned)code[(size_t)nb * MAC_BC + cell] : 0u; };
@@ -1264,16 +1320,16 @@ __global__ void __launch_bounds__(KTHREADS) k_prolong_block_slab(int nact, const
     const unsigned cx = x == 0 ? gcxm : gcxp;
     const T vy = y == 0 ? vym : vyp;
     const unsigned cy = y == 0 ? gcym : gcyp;
-    gx[j] = (cx & CODE_DOM) ? vx + pc.at(z, 1) : vx;
-    gy[j] = (cy & CODE_DOM) ? vy + pc.at(z, 2) : vy;
+    gx[j] = (cx & CODE_DOM) ? vx + pc.at_plane(j, 1) : vx;
+    gy[j] = (cy & CODE_DOM) ? vy + pc.at_plane(j, 2) : vy;
   }
   if (wv == 0) {
     const T zb = ldz(F[4], lane + 448);
-    below = (ldc(F[4], lane + 448) & CODE_DOM) ? zb + pc.at(-1, 0) : zb;
+    below = (ldc(F[4], lane + 448) & CODE_DOM) ? zb + pc.template at<0, 1>(0) : zb;
   }
   if (wv == WPB - 1) {
     const T za = ldz(F[5], lane);
-    above = (ldc(F[5], lane) & CODE_DOM) ? za + pc.at(8, 0) : za;
+    above = (ldc(F[5], lane) & CODE_DOM) ? za + pc.template at<2, 1>(0) : za;
   }
   const T col0[2] = {col[0], col[1]};
   T idg[2];
@@ -1541,40 +1597,94 @@ __global__ void __launch_bounds__(TAIL_THREADS) k_vcycle_tail(TailArgs a, const 
 // cell or whose -e_i neighbour is in the domain, unless the face is const
 // (Algorithms_Solve_Pressure.h:198-207; Policy_Pressure_If_Correction.h:33-49;
 // Policy_Gradiant.h:32-37; Data_Grid_Speed.tpp:20-26).  Same operation order.
-__global__ void k_gradient(const int* __restrict__ list, const int* __restrict__ nbr, const uint8_t* __restrict__ flags,
-                           const uint8_t* __restrict__ code, const double* __restrict__ p, double* __restrict__ u,
-                           double* __restrict__ v, double* __restrict__ w, double ihx, double ihy, double ihz) {
+// One wave per listed brick in the column layout: the -x and -y neighbours
+// of a cell are cross-lane shuffles (the brick's -x / -y face planes of p and
+// of the codes loaded lane-parallel from the neighbour bricks), the -z
+// neighbour the column's previous cell (or the -z brick's top plane).
+__global__ void __launch_bounds__(KTHREADS) k_gradient(int n, const int* __restrict__ list, const int* __restrict__ nbr,
+                                                       const uint8_t* __restrict__ flags, const uint8_t* __restrict__ code,
+                                                       const double* __restrict__ p, double* __restrict__ u,
+                                                       double* __restrict__ v, double* __restrict__ w, double ihx, double ihy,
+                                                       double ihz) {
 #pragma clang fp contract(off)
-  int b = list[blockIdx.x];
-  for (int t = threadIdx.x; t < MAC_BC; t += blockDim.x) {
-    int x = t & 7, y = (t >> 3) & 7, z = t >> 6;
-    size_t c = (size_t)b * MAC_BC + t;
-    unsigned fl = flags[c];
-    if (!(fl & 0x80u)) continue;
-    bool dom = code[c] & CODE_DOM;
-    double pc = p[c];
-    {
-      unsigned nd = ld_rel(code, nbr, b, x - 1, y, z);
-      if ((dom || (nd & CODE_DOM)) && !(fl & 0x08u)) {
-        double g = pc * ihx;
-        g -= ld_rel(p, nbr, b, x - 1, y, z) * ihx;
-        u[c] = u[c] - g;
-      }
+  const int lane = lane_id();
+  const int x = lane & 7, y = lane >> 3;
+  for (int i_ = first_wave(); i_ < n; i_ += gridDim.x * WPB) {
+    const int b = list[__builtin_amdgcn_readfirstlane(i_)];
+    const int* N = nbr + (size_t)b * 27;
+    const int bxm = N[12], bym = N[10], bzm = N[4];
+    const size_t base = (size_t)b * MAC_BC + lane;
+    unsigned fl[8], cd[8];
+    double pc[8];
+#pragma unroll
+    for (int z = 0; z < 8; ++z) {
+      fl[z] = flags[base + 64 * z];
+      cd[z] = code[base + 64 * z];
+      pc[z] = p[base + 64 * z];
     }
-    {
-      unsigned nd = ld_rel(code, nbr, b, x, y - 1, z);
-      if ((dom || (nd & CODE_DOM)) && !(fl & 0x10u)) {
-        double g = pc * ihy;
-        g -= ld_rel(p, nbr, b, x, y - 1, z) * ihy;
-        v[c] = v[c] - g;
+    // face planes of the low-side neighbour bricks (x faces indexed (y, z), y faces (x, z))
+    const double hpx = bxm >= 0 ? p[(size_t)bxm * MAC_BC + 7 + 8 * x + 64 * y] : 0.0;
+    const unsigned hcx = bxm >= 0 ? code[(size_t)bxm * MAC_BC + 7 + 8 * x + 64 * y] : 0u;
+    const double hpy = bym >= 0 ? p[(size_t)bym * MAC_BC + x + 56 + 64 * y] : 0.0;
+    const unsigned hcy = bym >= 0 ? code[(size_t)bym * MAC_BC + x + 56 + 64 * y] : 0u;
+    const double hpz = bzm >= 0 ? p[(size_t)bzm * MAC_BC + lane + 448] : 0.0;
+    const unsigned hcz = bzm >= 0 ? code[(size_t)bzm * MAC_BC + lane + 448] : 0u;
+    // per half column: the updates each cell takes and their gradients,
+    // then the half's loads of u, v, w in flight at once, then the stores
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      double gx[4], gy[4], gz[4];
+      unsigned todo[4];
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const int z = 4 * h + k;
+        double pxm = __shfl_up(pc[z], 1), pym = __shfl_up(pc[z], 8);
+        unsigned cxm = __shfl_up(cd[z], 1), cym = __shfl_up(cd[z], 8);
+        const double gpx = __shfl(hpx, y + 8 * z), gpy = __shfl(hpy, x + 8 * z);
+        const unsigned gcx = __shfl(hcx, y + 8 * z), gcy = __shfl(hcy, x + 8 * z);
+        if (x == 0) {
+          pxm = gpx;
+          cxm = gcx;
+        }
+        if (y == 0) {
+          pym = gpy;
+          cym = gcy;
+        }
+        const double pzm = z > 0 ? pc[z > 0 ? z - 1 : 0] : hpz;
+        const unsigned czm = z > 0 ? cd[z > 0 ? z - 1 : 0] : hcz;
+        const unsigned f = fl[z];
+        const bool dom = cd[z] & CODE_DOM;
+        unsigned t = 0;
+        if (f & 0x80u) {
+          if ((dom || (cxm & CODE_DOM)) && !(f & 0x08u)) t |= 1u;
+          if ((dom || (cym & CODE_DOM)) && !(f & 0x10u)) t |= 2u;
+          if ((dom || (czm & CODE_DOM)) && !(f & 0x20u)) t |= 4u;
+        }
+        todo[k] = t;
+        double g = pc[z] * ihx;
+        g -= pxm * ihx;
+        gx[k] = g;
+        g = pc[z] * ihy;
+        g -= pym * ihy;
+        gy[k] = g;
+        g = pc[z] * ihz;
+        g -= pzm * ihz;
+        gz[k] = g;
       }
-    }
-    {
-      unsigned nd = ld_rel(code, nbr, b, x, y, z - 1);
-      if ((dom || (nd & CODE_DOM)) && !(fl & 0x20u)) {
-        double g = pc * ihz;
-        g -= ld_rel(p, nbr, b, x, y, z - 1) * ihz;
-        w[c] = w[c] - g;
+      double uu[4], vv[4], ww[4];
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const size_t c = base + 64 * (4 * h + k);
+        uu[k] = (todo[k] & 1u) ? u[c] : 0.0;
+        vv[k] = (todo[k] & 2u) ? v[c] : 0.0;
+        ww[k] = (todo[k] & 4u) ? w[c] : 0.0;
+      }
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const size_t c = base + 64 * (4 * h + k);
+        if (todo[k] & 1u) u[c] = uu[k] - gx[k];
+        if (todo[k] & 2u) v[c] = vv[k] - gy[k];
+        if (todo[k] & 4u) w[c] = ww[k] - gz[k];
       }
     }
   }

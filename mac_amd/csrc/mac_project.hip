@@ -818,16 +818,6 @@ int vcycle(mac_ctx* c, TZ** zout) {
   // the block smoother runs on the coarse levels above the last two (those
   // smooth point-wise, fused or not)
   auto is_block = [&](int l) { return l > 0 && l < nl - 2 && c->prm.coarse_inner > 1; };
-  // the block steps of a level with few bricks (at most four per CU) run in
-  // the slab form (one 256-thread block per brick): a wave per brick would
-  // leave most CUs idle and carry an eight-cell serial chain per sweep.  On
-  // a level with more bricks the wave form is faster (256^3 benchmark input:
-  // level 1, 1,815 bricks, slab form 3.81-3.86 ms per mixed solve
-  // against 3.69-3.70 ms in the wave form)
-#ifndef MAC_SLAB_MAX
-#define MAC_SLAB_MAX 1024
-#endif
-  auto slab = [&](const Level& lv) { return lv.nact <= MAC_SLAB_MAX; };
   // down
   for (int l = 0; l < tlev; ++l) {
     Level& lv = c->L[l];
@@ -845,14 +835,9 @@ int vcycle(mac_ctx* c, TZ** zout) {
       // the first step from zero ran inside the restriction into this level
       for (int k = first_done[l] ? 1 : 0; k < (deep ? 1 : nu); ++k) {
         TimedLaunch tl(c, K_C_JACOBI, nd * ((k ? 3 : 2) * sz + 1));  // r, zin, zout, code
-        if (slab(lv))
-          k_jacobi_block_slab<TZ, TZ><<<lv.nact, KTHREADS, 0, c->st>>>(lv.nact, lv.meta, lv.code, (const TZ*)lv.r, k ? z : nullptr,
-                                                                        k ? zt : z, ih[0], ih[1], ih[2],
-                                                                        deep ? c->blockw_deep : c->blockw, c->s);
-        else
-          k_jacobi_block<TZ, TZ><<<grid_of(lv.nact), KTHREADS, 0, c->st>>>(lv.nact, lv.meta, lv.code, (const TZ*)lv.r,
-                                                                            k ? z : nullptr, k ? zt : z, ih[0], ih[1], ih[2],
-                                                                            deep ? c->blockw_deep : c->blockw, c->s);
+        k_jacobi_block<TZ, TZ><<<lv.nact, KTHREADS, 0, c->st>>>(lv.nact, lv.meta, lv.code, (const TZ*)lv.r, k ? z : nullptr,
+                                                                 k ? zt : z, ih[0], ih[1], ih[2],
+                                                                 deep ? c->blockw_deep : c->blockw, c->s);
         if (k) std::swap(z, zt);
       }
     } else if (nu >= 2) {
@@ -969,13 +954,8 @@ int vcycle(mac_ctx* c, TZ** zout) {
       // prolongation fused with the first post-smoothing step: zt = S(z + P zc)
       if (blk) {
         TimedLaunch tl(c, K_C_PROLONG_BLOCK, nd * (3 * sz + 2) + (double)cl.ndom * sz);
-        if (slab(lv))
-          k_prolong_block_slab<TZ><<<lv.nact, KTHREADS, 0, c->st>>>(lv.nact, lv.pmeta, lv.code, (const TZ*)lv.r, zfin[l + 1], z, zt,
-                                                                     ih[0], ih[1], ih[2], deep ? c->blockw_deep : c->blockw, c->s);
-        else
-          k_prolong_block<TZ><<<grid_of(lv.nact), KTHREADS, 0, c->st>>>(lv.nact, lv.pmeta, lv.code, (const TZ*)lv.r, zfin[l + 1],
-                                                                         z, zt, ih[0], ih[1], ih[2],
-                                                                         deep ? c->blockw_deep : c->blockw, c->s);
+        k_prolong_block<TZ><<<lv.nact, KTHREADS, 0, c->st>>>(lv.nact, lv.pmeta, lv.code, (const TZ*)lv.r, zfin[l + 1], z, zt,
+                                                              ih[0], ih[1], ih[2], deep ? c->blockw_deep : c->blockw, c->s);
       } else {
         const int want_dot = nu == 1 ? 1 : 0;
         TimedLaunch tl(c, K_PROLONG_JACOBI0, nd * (8.0 + 2 * sz + 2) + (double)cl.ndom * sz);
@@ -992,13 +972,8 @@ int vcycle(mac_ctx* c, TZ** zout) {
     for (int k = k0; k < nu; ++k) {
       if (blk) {
         TimedLaunch tl(c, K_C_JACOBI, nd * (3 * sz + 1));
-        if (slab(lv))
-          k_jacobi_block_slab<TZ, TZ><<<lv.nact, KTHREADS, 0, c->st>>>(lv.nact, lv.meta, lv.code, (const TZ*)lv.r, z, zt, ih[0],
-                                                                        ih[1], ih[2], deep ? c->blockw_deep : c->blockw, c->s);
-        else
-          k_jacobi_block<TZ, TZ><<<grid_of(lv.nact), KTHREADS, 0, c->st>>>(lv.nact, lv.meta, lv.code, (const TZ*)lv.r, z, zt,
-                                                                            ih[0], ih[1], ih[2], deep ? c->blockw_deep : c->blockw,
-                                                                            c->s);
+        k_jacobi_block<TZ, TZ><<<lv.nact, KTHREADS, 0, c->st>>>(lv.nact, lv.meta, lv.code, (const TZ*)lv.r, z, zt, ih[0], ih[1],
+                                                                 ih[2], deep ? c->blockw_deep : c->blockw, c->s);
       } else {
         sweep<TZ>(c, l, z, zt, (l == 0 && k == nu - 1) ? 1 : 0, opost[k]);
       }

@@ -621,72 +621,23 @@ __global__ void __launch_bounds__(KTHREADS) k_jacobi(int nact, const int* __rest
 // sweeps (weights om[0..m-1]) among themselves with the neighbour bricks'
 // face planes frozen at zin, i.e. zout = zin + q(D^-1 A_BB) D^-1 (r - A zin)
 // on each brick (q the sweeps' polynomial) scaled by w.damp, a symmetric
-// block smoother.  The
-// sweeps after the first run in registers and cross-lane shuffles.
-// zin == nullptr: from zero.
+// block smoother.  zin == nullptr: from zero.
 struct BlockW {
   int m;
   float om[8];
   double om64[8];
   double damp;  // zout = zin + damp (block correction)
 };
-// The m sweeps of a block step on one brick held by a wave (lane = x + 8y,
-// the brick's z column in col): in-brick neighbours by cross-lane shuffles,
-// the frozen face values gx / gy (x / y edge lanes), below / above (z faces).
-// idg: the inverse diagonal (FP32) per cell, 0 off the domain.
-template <typename TZ>
-__device__ __forceinline__ void block_sweeps(int lane, const unsigned (&cd)[8], const TZ (&rv)[8], const TZ (&idg)[8],
-                                             TZ (&col)[8], const TZ (&gx)[8], const TZ (&gy)[8], TZ below, TZ above, TZ ihx2,
-                                             TZ ihy2, TZ ihz2, const BlockW& w) {
-  const int x = lane & 7, y = lane >> 3;
-  for (int k = 0; k < w.m; ++k) {
-    const TZ om = sizeof(TZ) == 4 ? (TZ)w.om[k] : (TZ)w.om64[k];
-    TZ nw[8];
-#pragma unroll
-    for (int z = 0; z < 8; ++z) {
-      const TZ v = col[z];
-      TZ xm = __shfl_up(v, 1), xp = __shfl_down(v, 1), ym = __shfl_up(v, 8), yp = __shfl_down(v, 8);
-      if (x == 0) xm = gx[z];
-      if (x == 7) xp = gx[z];
-      if (y == 0) ym = gy[z];
-      if (y == 7) yp = gy[z];
-      const TZ zm = (z > 0) ? col[z > 0 ? z - 1 : 0] : below;
-      const TZ zp = (z < 7) ? col[z < 7 ? z + 1 : 7] : above;
-      TZ out = TZ(0);
-      if (cd[z] & CODE_DOM) {
-        TZ dg;
-        const TZ lz = lap_cell<TZ>(cd[z], v, xm, xp, ym, yp, zm, zp, ihx2, ihy2, ihz2, dg);
-        if constexpr (sizeof(TZ) == 4)
-          out = idg[z] != TZ(0) ? v + om * (rv[z] - lz) * idg[z] : TZ(0);
-        else
-          out = dg > TZ(0) ? v + om * (rv[z] - lz) / dg : TZ(0);
-      }
-      nw[z] = out;
-    }
-#pragma unroll
-    for (int z = 0; z < 8; ++z) col[z] = nw[z];
-  }
-}
-
-template <typename TZ>
-__device__ __forceinline__ void inv_diag_col(const unsigned (&cd)[8], TZ ihx2, TZ ihy2, TZ ihz2, TZ (&idg)[8]) {
-#pragma unroll
-  for (int z = 0; z < 8; ++z) {
-    const TZ dg = diag_of<TZ>(cd[z], ihx2, ihy2, ihz2);
-    idg[z] = ((cd[z] & CODE_DOM) && dg > TZ(0)) ? TZ(1) / dg : TZ(0);
-  }
-}
-
-// The block step with one brick per 256-thread block (the slab form, for the
-// coarse levels, whose few bricks leave most of the chip idle when a wave
-// carries a whole brick): wave w holds z-planes 2w and 2w+1 of the brick, the
-// same lane = x + 8y columns, so a sweep's serial chain is two cells instead
-// of eight.  Each sweep publishes the wave's two planes in LDS (double
-// buffered, one barrier per sweep) and takes the planes 2w-1 and 2w+2 of the
-// neighbouring waves from there; the frozen face values come in as for
-// block_sweeps (below for wave 0, above for wave 3).  Every cell update is
-// the expression of block_sweeps with the same operands, so both forms give
-// identical bits.
+// The block step runs with one brick per 256-thread block (the slab form):
+// wave w holds z-planes 2w and 2w+1 of the brick in the lane = x + 8y
+// columns, so a sweep's serial chain is two cells, and the coarse levels,
+// whose few bricks would leave most of the chip idle with a wave per brick,
+// get four waves per brick.  Each sweep publishes the wave's two planes in
+// LDS (double buffered, one barrier per sweep) and takes the planes 2w-1 and
+// 2w+2 of the neighbouring waves from there; in-brick x/y neighbours are
+// cross-lane shuffles, and the frozen face values of the neighbour bricks
+// come in as gx / gy (x / y edge lanes), below (wave 0) and above (wave 3).
+// idg: the inverse diagonal per cell (FP32; FP64 divides), 0 off the domain.
 template <typename T>
 struct SlabLDS {
   T pl[2][WPB][2][64];  // [buffer][wave][low/high plane][lane]
@@ -740,9 +691,9 @@ __device__ __forceinline__ void inv_diag_slab(const unsigned (&cd)[2], TZ ihx2, 
   }
 }
 
-// k_jacobi_block in the slab form: one block per active brick.
+// The block step, one block per active brick.
 template <typename TR, typename TZ>
-__global__ void __launch_bounds__(KTHREADS) k_jacobi_block_slab(int nact, const int* __restrict__ meta,
+__global__ void __launch_bounds__(KTHREADS) k_jacobi_block(int nact, const int* __restrict__ meta,
                                                                 const uint8_t* __restrict__ code, const TR* __restrict__ r,
                                                                 const TZ* __restrict__ zin, TZ* __restrict__ zout, TZ ihx2,
                                                                 TZ ihy2, TZ ihz2, BlockW w, const Scal* s) {
@@ -786,49 +737,6 @@ __global__ void __launch_bounds__(KTHREADS) k_jacobi_block_slab(int nact, const 
   const TZ dmp = (TZ)w.damp;
 #pragma unroll
   for (int j = 0; j < 2; ++j) zout[base + 64 * j] = col0[j] + dmp * (col[j] - col0[j]);
-}
-
-template <typename TR, typename TZ>
-__global__ void __launch_bounds__(KTHREADS) k_jacobi_block(int nact, const int* __restrict__ meta,
-                                                           const uint8_t* __restrict__ code, const TR* __restrict__ r,
-                                                           const TZ* __restrict__ zin, TZ* __restrict__ zout, TZ ihx2, TZ ihy2,
-                                                           TZ ihz2, BlockW w, const Scal* s) {
-  if (converged(meta, MSTRIDE, nact, first_wave(), s)) return;
-  const int lane = lane_id();
-  FOR_EACH_BRICK(meta, nact, M) {
-    const int b = M[0];
-    const size_t base = (size_t)b * MAC_BC + lane;
-    unsigned cd[8];
-    TZ col[8], rv[8];
-#pragma unroll
-    for (int z = 0; z < 8; ++z) {
-      cd[z] = code[base + 64 * z];
-      rv[z] = (TZ)r[base + 64 * z];
-      col[z] = zin ? zin[base + 64 * z] : TZ(0);
-    }
-    Halo<TZ> h{TZ(0), TZ(0), TZ(0), TZ(0), TZ(0), TZ(0)};
-    if (zin) h = load_halo<TZ>(M, lane, [&](int nb, int cell) { return zin[(size_t)nb * MAC_BC + cell]; });
-    // the frozen face values each edge lane reads, redistributed once
-    const int x = lane & 7, y = lane >> 3;
-    TZ gx[8], gy[8];
-#pragma unroll
-    for (int z = 0; z < 8; ++z) {
-      const TZ gxm = __shfl(h.xm, y + 8 * z), gxp = __shfl(h.xp, y + 8 * z);
-      const TZ gym = __shfl(h.ym, x + 8 * z), gyp = __shfl(h.yp, x + 8 * z);
-      gx[z] = x == 0 ? gxm : gxp;  // only the edge lanes use them
-      gy[z] = y == 0 ? gym : gyp;
-    }
-    TZ col0[8];
-#pragma unroll
-    for (int z = 0; z < 8; ++z) col0[z] = col[z];
-    // FP32: the inverse diagonal once per cell instead of a division per sweep
-    TZ idg[8];
-    inv_diag_col<TZ>(cd, ihx2, ihy2, ihz2, idg);
-    block_sweeps<TZ>(lane, cd, rv, idg, col, gx, gy, h.below, h.above, ihx2, ihy2, ihz2, w);
-    const TZ dmp = (TZ)w.damp;
-#pragma unroll
-    for (int z = 0; z < 8; ++z) zout[base + 64 * z] = col0[z] + dmp * (col[z] - col0[z]);
-  }
 }
 
 // Two damped-Jacobi sweeps from a zero start in one pass:
@@ -890,7 +798,7 @@ __global__ void __launch_bounds__(KTHREADS) k_jacobi2(int nact, const int* __res
 // The coarse level's first block step from zero on the restricted brick in
 // LDS (rcl), in the slab form, run by all four waves of the block.
 template <typename T>
-__device__ __forceinline__ void smooth_from_zero_slab(int C, const uint8_t* __restrict__ codec, const T* rcl,
+__device__ __forceinline__ void smooth_from_zero(int C, const uint8_t* __restrict__ codec, const T* rcl,
                                                       T* __restrict__ zsm, T ihx2, T ihy2, T ihz2, const BlockW& w,
                                                       SlabLDS<T>& L) {
   const int lane = lane_id(), wv = wave_in_block();
@@ -973,7 +881,7 @@ __global__ void __launch_bounds__(KTHREADS) k_restrict_sep(const int* __restrict
     if (zsm) {
       __shared__ SlabLDS<T> L;
       __syncthreads();
-      smooth_from_zero_slab<T>(C, codec, rcl, zsm, ihx2, ihy2, ihz2, w, L);
+      smooth_from_zero<T>(C, codec, rcl, zsm, ihx2, ihy2, ihz2, w, L);
     }
     return;
   }
@@ -1033,7 +941,7 @@ __global__ void __launch_bounds__(KTHREADS) k_restrict_sep(const int* __restrict
   if (zsm) {
     __shared__ SlabLDS<T> L;
     __syncthreads();
-    smooth_from_zero_slab<T>(C, codec, rcl, zsm, ihx2, ihy2, ihz2, w, L);
+    smooth_from_zero<T>(C, codec, rcl, zsm, ihx2, ihy2, ihz2, w, L);
   }
 }
 
@@ -1199,37 +1107,7 @@ struct ProlongedBrick {
   }
 };
 
-// Prolongation fused with the first post-smoothing block step on a coarse
-// level: zout = zin + damp (block correction of zin), zin = zfin + P zc.
-// Equal to k_prolong_col followed by k_jacobi_block(zin), without writing
-// and re-reading zin.
-template <typename T>
-__global__ void __launch_bounds__(KTHREADS) k_prolong_block(int nact, const int* __restrict__ pmeta,
-                                                            const uint8_t* __restrict__ code, const T* __restrict__ r,
-                                                            const T* __restrict__ zc, const T* __restrict__ zfin,
-                                                            T* __restrict__ zout, T ihx2, T ihy2, T ihz2, BlockW w, const Scal* s) {
-  if (converged(pmeta, PSTRIDE, nact, first_wave(), s)) return;
-  const int lane = lane_id();
-  for (int i_ = first_wave(); i_ < nact; i_ += gridDim.x * WPB) {
-    const int* M = pmeta + (size_t)PSTRIDE * __builtin_amdgcn_readfirstlane(i_);
-    const size_t base = (size_t)M[0] * MAC_BC + lane;
-    ProlongedBrick<T> pb;
-    pb.load(M, lane, code, zfin, zc);
-    T rv[8], col0[8], idg[8];
-#pragma unroll
-    for (int z = 0; z < 8; ++z) {
-      rv[z] = r[base + 64 * z];
-      col0[z] = pb.col[z];
-    }
-    inv_diag_col<T>(pb.cd, ihx2, ihy2, ihz2, idg);
-    block_sweeps<T>(lane, pb.cd, rv, idg, pb.col, pb.gx, pb.gy, pb.below, pb.above, ihx2, ihy2, ihz2, w);
-    const T dmp = (T)w.damp;
-#pragma unroll
-    for (int z = 0; z < 8; ++z) zout[base + 64 * z] = col0[z] + dmp * (pb.col[z] - col0[z]);
-  }
-}
-
-// The coarse columns of ProlongCols that wave w of a slab-form block reads:
+// The coarse columns of ProlongCols that wave w of a k_prolong_block block reads:
 // the 3 coarse z-levels bz-1+w .. bz+1+w (local levels 0..2) cover its planes
 // 2w, 2w+1 and the halo plane below (w = 0) or above (w = 3).  at() takes
 // its two levels as constants, so the columns stay in registers, and forms
@@ -1275,10 +1153,14 @@ struct SlabCols {
   __device__ __forceinline__ T at_plane(int j, int face) const { return j ? at<1, 2>(face) : at<1, 0>(face); }
 };
 
-// k_prolong_block in the slab form: one block per active fine brick, wave w
-// on z-planes 2w, 2w+1 (the prolongated values formed as in ProlongedBrick).
+// Prolongation fused with the first post-smoothing block step on a coarse
+// level: zout = zin + damp (block correction of zin), zin = zfin + P zc,
+// equal to k_prolong_col followed by k_jacobi_block(zin) without writing and
+// re-reading zin.  One block per active fine brick, wave w on z-planes 2w,
+// 2w+1; the prolongated values (own cells and frozen halo) are formed as in
+// ProlongedBrick.
 template <typename T>
-__global__ void __launch_bounds__(KTHREADS) k_prolong_block_slab(int nact, const int* __restrict__ pmeta,
+__global__ void __launch_bounds__(KTHREADS) k_prolong_block(int nact, const int* __restrict__ pmeta,
                                                                  const uint8_t* __restrict__ code, const T* __restrict__ r,
                                                                  const T* __restrict__ zc, const T* __restrict__ zfin,
                                                                  T* __restrict__ zout, T ihx2, T ihy2, T ihz2, BlockW w,

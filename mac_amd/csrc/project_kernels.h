@@ -838,27 +838,70 @@ __global__ void __launch_bounds__(KTHREADS) k_restrict_sep(const int* __restrict
     const int C = M[0];
     if (threadIdx.x < 64) fba[threadIdx.x] = M[4 + threadIdx.x];
     __syncthreads();
-    T v[(NA + KTHREADS - 1) / KTHREADS];
+    // Each thread loads whole fine z-columns (fz = -1..16) of the region: the
+    // 16x16 interior columns one per thread, the 68 columns of the ring
+    // around them one each for threads 0..67; along a column only the brick
+    // slot (at fz = 0 and 16) and the cell offset (+64) change.
+    const int t = threadIdx.x;
+    const int ix = t & 15, iy = t >> 4;
+    int rx, ry;  // the ring column of threads 0..67
+    if (t < 18) {
+      rx = t - 1;
+      ry = -1;
+    } else if (t < 36) {
+      rx = t - 19;
+      ry = 16;
+    } else if (t < 52) {
+      rx = -1;
+      ry = t - 36;
+    } else {
+      rx = 16;
+      ry = t - 52;
+    }
+    const bool ring = t < 68;
+    auto col_base = [&](int gx, int gy, int& slot, int& cell) {
+      slot = ((gx + 8) >> 3) + 4 * ((gy + 8) >> 3);
+      cell = (gx & 7) + 8 * (gy & 7);
+    };
+    int si, ci, sr, cr;
+    col_base(ix, iy, si, ci);
+    col_base(rx, ry, sr, cr);
+    T vi[18], vr[18];
 #pragma unroll
-    for (int k = 0; k < (NA + KTHREADS - 1) / KTHREADS; ++k) {
-      const int i = threadIdx.x + KTHREADS * k;
-      v[k] = T(0);
-      if (i < NA) {
-        const int gx = i % 18 - 1, gy = (i / 18) % 18 - 1, gz = i / 324 - 1;
-        const int f = fba[((gx + 8) >> 3) + 4 * ((gy + 8) >> 3) + 16 * ((gz + 8) >> 3)];
-        if (f >= 0) v[k] = resf[(size_t)f * MAC_BC + (gx & 7) + 8 * (gy & 7) + 64 * (gz & 7)];
+    for (int k = 0; k < 18; ++k) {
+      const int gz = k - 1, zs = 16 * ((gz + 8) >> 3), zc = 64 * (gz & 7);
+      const int fi = fba[si + zs];
+      vi[k] = fi >= 0 ? resf[(size_t)fi * MAC_BC + ci + zc] : T(0);
+      vr[k] = T(0);
+      if (ring) {
+        const int fr = fba[sr + zs];
+        if (fr >= 0) vr[k] = resf[(size_t)fr * MAC_BC + cr + zc];
       }
     }
 #pragma unroll
-    for (int k = 0; k < (NA + KTHREADS - 1) / KTHREADS; ++k) {
-      const int i = threadIdx.x + KTHREADS * k;
-      if (i < NA) FA[i] = v[k];
+    for (int k = 0; k < 18; ++k) {
+      FA[k * 324 + (iy + 1) * 18 + ix + 1] = vi[k];
+      if (ring) FA[k * 324 + (ry + 1) * 18 + rx + 1] = vr[k];
     }
     __syncthreads();
-    for (int i = threadIdx.x; i < NA1; i += blockDim.x) {
-      const int X = i & 7, fy = (i >> 3) % 18, fz = (i >> 3) / 18;
-      const T* row = &FA[(fz * 18 + fy) * 18 + 2 * X];
-      ta1[i] = T(0.25) * row[0] + T(0.75) * row[1] + T(0.75) * row[2] + T(0.25) * row[3];
+    // x pass: thread t takes X = t & 7 on the rows (fy, fz) = r, r + 32, ...
+    {
+      const int X = t & 7;
+      int fy = t >> 3, fz = 0;  // r = t >> 3 < 32: fy = r % 18, fz = r / 18
+      if (fy >= 18) {
+        fy -= 18;
+        fz = 1;
+      }
+      for (int r = t >> 3; r < 324; r += 32) {
+        const T* row = &FA[(fz * 18 + fy) * 18 + 2 * X];
+        ta1[r * 8 + X] = T(0.25) * row[0] + T(0.75) * row[1] + T(0.75) * row[2] + T(0.25) * row[3];
+        fy += 14;  // r += 32 = one fz and 14 fy
+        fz += 1;
+        if (fy >= 18) {
+          fy -= 18;
+          fz += 1;
+        }
+      }
     }
     __syncthreads();
     for (int i = threadIdx.x; i < 18 * 64; i += blockDim.x) {

@@ -32,7 +32,7 @@ WEIGHTS = os.path.join(ROOT, "mac_amd", "weights", "unet_guess.npz")
 
 # timing class -> device kernel (the name a kernel-trace profile lists), per precision
 KERNEL_NAMES = {
-    "divergence": "k_div", "cg_pupdate_matvec": "k_pmatvec<{z}>", "cg_update": "k_update",
+    "divergence": "k_div", "cg_pupdate_matvec": "k_pmatvec<{z}, {z}>", "cg_update": "k_update<{z}>",
     "jacobi2_l0": "k_jacobi2<double, {z}>", "jacobi_l0": "k_jacobi<double, {z}>", "residual_l0": "k_residual<double, {z}>",
     "prolong_l0": "k_prolong_col<{z}> (level 0)", "restrict_l1": "k_restrict_sep<{z}> (into level 1)",
     "coarse_jacobi2": "k_jacobi2<{z}, {z}> (levels 1-3)", "coarse_jacobi": "k_jacobi_block<{z}, {z}> (levels 1-3)",

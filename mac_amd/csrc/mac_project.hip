@@ -1007,16 +1007,21 @@ int cg_iteration(mac_ctx* c, int cur, TZ** zout, long long* last) {
   const double* ih = l0.ih2;
   const double nd = (double)l0.ndom;
   CHK(vcycle<TZ>(c, zout));
+  // the search direction is stored in the cycle's precision (FP32 in the
+  // mixed method; the d buffers hold n0 doubles, so either type fits)
+  using TD = TZ;
+  const double sd = sizeof(TD);
   {
-    TimedLaunch tl(c, K_PMATVEC, nd * (25 + sizeof(TZ)));
-    k_pmatvec<TZ><<<grid_of(l0.nact), KTHREADS, 0, c->st>>>(l0.nact, l0.meta, l0.code, *zout, c->d[cur], c->d[cur ^ 1], c->q,
-                                                            ih[0], ih[1], ih[2], c->partials, c->s);
+    TimedLaunch tl(c, K_PMATVEC, nd * (9 + sizeof(TZ) + 2 * sd));  // z, d_old, code; d_new, q
+    k_pmatvec<TZ, TD><<<grid_of(l0.nact), KTHREADS, 0, c->st>>>(l0.nact, l0.meta, l0.code, *zout, (const TD*)c->d[cur],
+                                                                (TD*)c->d[cur ^ 1], c->q, ih[0], ih[1], ih[2], c->partials,
+                                                                c->s);
   }
   finalize(c, grid_of(l0.nact), OP_DQ);
   {
-    TimedLaunch tl(c, K_UPDATE, nd * 49);
-    k_update<<<grid_of(l0.nact), KTHREADS, 0, c->st>>>(l0.nact, l0.meta, l0.code, c->x, c->r, c->d[cur ^ 1], c->q, c->partials,
-                                                       c->s);
+    TimedLaunch tl(c, K_UPDATE, nd * (41 + sd));  // code, x, r, d, q; x, r
+    k_update<TD><<<grid_of(l0.nact), KTHREADS, 0, c->st>>>(l0.nact, l0.meta, l0.code, c->x, c->r, (const TD*)c->d[cur ^ 1], c->q,
+                                                           c->partials, c->s);
   }
   *last = finalize(c, grid_of(l0.nact), OP_RR, true);
   return 0;

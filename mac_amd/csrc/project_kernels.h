@@ -492,11 +492,14 @@ __global__ void __launch_bounds__(KTHREADS) k_resid0(int nact, const int* __rest
 }
 
 // d_new = z + beta d_old (formed on the fly at the neighbours too), q = L d_new,
-// partial d_new . q.
-template <typename TZ>
+// partial d_new . q.  TD: the storage type of the search direction (FP32 in
+// the mixed method): d_new is formed in FP64 and rounded to TD, and q, the
+// dot product and the later x update all use the rounded value, so x and the
+// recursive residual r stay consistent.
+template <typename TZ, typename TD>
 __global__ void __launch_bounds__(KTHREADS) k_pmatvec(int nact, const int* __restrict__ meta,
                                                       const uint8_t* __restrict__ code, const TZ* __restrict__ zv,
-                                                      const double* __restrict__ dold, double* __restrict__ dnew,
+                                                      const TD* __restrict__ dold, TD* __restrict__ dnew,
                                                       double* __restrict__ q, double ihx2, double ihy2, double ihz2,
                                                       double* __restrict__ partials, const Scal* s) {
   if (converged(meta, MSTRIDE, nact, first_wave(), s)) return;
@@ -510,9 +513,9 @@ __global__ void __launch_bounds__(KTHREADS) k_pmatvec(int nact, const int* __res
     FOR_EACH_BRICK(meta, nact, M) {
       const int b = M[0];
       const size_t base = (size_t)b * MAC_BC + lane;
-      auto dval = [&](size_t c) {
-        if constexpr (decltype(first)::value) return (double)zv[c];
-        else return (double)zv[c] + beta * dold[c];
+      auto dval = [&](size_t c) -> double {
+        if constexpr (decltype(first)::value) return (double)(TD)zv[c];
+        else return (double)(TD)((double)zv[c] + beta * (double)dold[c]);
       };
       double col[8];
       unsigned cd[8];
@@ -527,7 +530,7 @@ __global__ void __launch_bounds__(KTHREADS) k_pmatvec(int nact, const int* __res
             if (cd[z] & CODE_DOM) {
               double dg;
               double qv = lap_cell<double>(cd[z], col[z], xm, xp, ym, yp, zm, zp, ihx2, ihy2, ihz2, dg);
-              dnew[base + 64 * z] = col[z];
+              dnew[base + 64 * z] = (TD)col[z];
               q[base + 64 * z] = qv;
               acc += col[z] * qv;
             }
@@ -540,9 +543,10 @@ __global__ void __launch_bounds__(KTHREADS) k_pmatvec(int nact, const int* __res
 }
 
 // x += alpha d, r -= alpha q, partial ||r||^2.
+template <typename TD>
 __global__ void __launch_bounds__(KTHREADS) k_update(int nact, const int* __restrict__ meta, const uint8_t* __restrict__ code,
                                                      double* __restrict__ x, double* __restrict__ r,
-                                                     const double* __restrict__ d, const double* __restrict__ q,
+                                                     const TD* __restrict__ d, const double* __restrict__ q,
                                                      double* __restrict__ partials, const Scal* s) {
   if (converged(meta, MSTRIDE, nact, first_wave(), s)) return;
   __shared__ double sh[16];
@@ -556,7 +560,7 @@ __global__ void __launch_bounds__(KTHREADS) k_update(int nact, const int* __rest
     for (int z = 0; z < 8; ++z) {
       const size_t c = base + 64 * z;
       if (code[c] & CODE_DOM) {
-        x[c] += alpha * d[c];
+        x[c] += alpha * (double)d[c];
         double rv = r[c] - alpha * q[c];
         r[c] = rv;
         acc += rv * rv;

@@ -50,6 +50,13 @@ struct BcArgs {
                           // existing non-air cell, bit 6 existing non-air, bit 7 air (k_bc_classify)
   double h[3], ih[3];
   int dim;
+  // per-axis members chosen by a select, not an index: an index into a
+  // kernel argument's array would copy the argument to scratch memory
+  __device__ __forceinline__ double* Uc(int c) const { return c == 0 ? U[0] : (c == 1 ? U[1] : U[2]); }
+  __device__ __forceinline__ double* tmpc(int c) const { return c == 0 ? tmp[0] : (c == 1 ? tmp[1] : tmp[2]); }
+  __device__ __forceinline__ uint8_t* pendc(int c) const { return c == 0 ? pend[0] : (c == 1 ? pend[1] : pend[2]); }
+  __device__ __forceinline__ double hc(int c) const { return c == 0 ? h[0] : (c == 1 ? h[1] : h[2]); }
+  __device__ __forceinline__ double ihc(int c) const { return c == 0 ? ih[0] : (c == 1 ? ih[1] : ih[2]); }
 };
 
 namespace bc {
@@ -61,9 +68,19 @@ __device__ __forceinline__ bool f_fluid(unsigned f) {
   return (f & 0x80u) && (m == 1u || m == 3u || m == 4u || m == 5u);
 }
 
+// The air neighbours of a cell as a bit set (bit 2 axis + (side > 0)); the
+// k-th of them in axis-major order found by clearing the lower set bits, so
+// no array is indexed by a run-time value (that would live in scratch memory).
 struct Bnd {
+  unsigned m;
   int n;
-  int dir[6], sign[6];
+  __device__ __forceinline__ int bit(int k) const {
+    unsigned v = m;
+    for (int i = 0; i < k; ++i) v &= v - 1u;
+    return __builtin_ctz(v);
+  }
+  __device__ __forceinline__ int dir(int k) const { return bit(k) >> 1; }
+  __device__ __forceinline__ int sign(int k) const { return (bit(k) & 1) ? 1 : -1; }
 };
 
 // A cell of brick b at local (x, y, z) and lookups at offsets from it.
@@ -82,7 +99,7 @@ struct Cell {
   }
   __device__ double F(int c, int dx, int dy, int dz) const {
     const long long s = slot(dx, dy, dz);
-    return s < 0 ? 0.0 : a.U[c][s];
+    return s < 0 ? 0.0 : a.Uc(c)[s];
   }
   // the face may be written: the cell exists and the face is not const
   __device__ long long target(int c, int dx, int dy, int dz) const {
@@ -93,28 +110,21 @@ struct Cell {
   }
   __device__ void set(int c, int dx, int dy, int dz, double v) const {
     const long long s = target(c, dx, dy, dz);
-    if (s >= 0) a.U[c][s] = v;
+    if (s >= 0) a.Uc(c)[s] = v;
   }
   __device__ void stage(int c, int dx, int dy, int dz, double v) const {
     const long long s = slot(dx, dy, dz);
     if (s < 0) return;
-    a.tmp[c][s] = v;
-    a.pend[c][s] = 1;
+    a.tmpc(c)[s] = v;
+    a.pendc(c)[s] = 1;
   }
   // IsBoundary (:88-119): the air neighbours of a valid non-air cell,
   // axis-major, from the cell's class byte
   __device__ Bnd bnd(int dx, int dy, int dz) const {
     Bnd r;
-    r.n = 0;
     const long long s = slot(dx, dy, dz);
-    const unsigned m = s < 0 ? 0u : (a.cls[s] & 0x3fu);
-#pragma unroll
-    for (int k = 0; k < 6; ++k)
-      if (m & (1u << k)) {
-        r.dir[r.n] = k >> 1;
-        r.sign[r.n] = (k & 1) ? 1 : -1;
-        ++r.n;
-      }
+    r.m = s < 0 ? 0u : (a.cls[s] & 0x3fu);
+    r.n = __builtin_popcount(r.m);
     return r;
   }
   // the same from the cell's own flags and its neighbours' (k_bc_classify)
@@ -131,11 +141,11 @@ struct Cell {
   // a one-air cell with air axis d at (dx, dy, dz) + i e_t
   __device__ bool partner(int dx, int dy, int dz, int d, int t, int i) const {
     const Bnd p = bnd(dx + i * (t == 0), dy + i * (t == 1), dz + i * (t == 2));
-    return p.n == 1 && p.dir[0] == d;
+    return p.n == 1 && p.dir(0) == d;
   }
   __device__ bool has_partner2d(int dx, int dy, int dz, const Bnd& r) const {
-    const int t = r.dir[0] == 0 ? 1 : 0;
-    return partner(dx, dy, dz, r.dir[0], t, -1) || partner(dx, dy, dz, r.dir[0], t, 1);
+    const int t = r.dir(0) == 0 ? 1 : 0;
+    return partner(dx, dy, dz, r.dir(0), t, -1) || partner(dx, dy, dz, r.dir(0), t, 1);
   }
   // Get_Is_Speed_In_Domain(N, d) for N at (dx, dy, dz): the face between N and N - e_d
   __device__ bool in_domain(int dx, int dy, int dz, int d) const {
@@ -232,7 +242,7 @@ __global__ void k_bc_div(BcArgs a) {
     const bc::Bnd r = c.bnd(0, 0, 0);
     if (r.n != 1) continue;
     if (a.dim == 2 && !c.has_partner2d(0, 0, 0, r)) continue;
-    const int d = r.dir[0], sg = r.sign[0];
+    const int d = r.dir(0), sg = r.sign(0);
     const int ex = (d == 0), ey = (d == 1), ez = (d == 2);
     double v;
     int tx, ty, tz;
@@ -249,15 +259,15 @@ __global__ void k_bc_div(BcArgs a) {
     if (a.dim == 2) {
       const int tt = d == 0 ? 1 : 0;
       const double v2 = c.F(tt, 0, 0, 0), v3 = c.F(tt, tt == 0, tt == 1, 0);
-      res = v - sg * a.ih[tt] * a.h[d] * (v3 - v2);
+      res = v - sg * a.ihc(tt) * a.hc(d) * (v3 - v2);
     } else {
       double sum = 0.0;
       for (int tt = 0; tt < 3; ++tt) {
         if (tt == d) continue;
         const double v2 = c.F(tt, 0, 0, 0), v3 = c.F(tt, tt == 0, tt == 1, tt == 2);
-        sum += a.ih[tt] * (v3 - v2);
+        sum += a.ihc(tt) * (v3 - v2);
       }
-      res = v - sg * sum * a.h[d];
+      res = v - sg * sum * a.hc(d);
     }
     c.set(d, tx, ty, tz, res);
   }
@@ -285,15 +295,15 @@ __global__ void k_bc_plane(BcArgs a) {
           const int bx = -s * ex, by = -s * ey, bz = -s * ez;
           const int ax = bx - (tt == 0), ay = by - (tt == 1), az = bz - (tt == 2);
           const bc::Bnd rB = c.bnd(bx, by, bz);
-          if (rB.n != 1 || rB.dir[0] != d) continue;
+          if (rB.n != 1 || rB.dir(0) != d) continue;
           const bc::Bnd rA = c.bnd(ax, ay, az);
-          if (rA.n != 1 || rA.dir[0] != d) continue;
-          if (rB.sign[0] != s && rA.sign[0] != s) continue;
+          if (rA.n != 1 || rA.dir(0) != d) continue;
+          if (rB.sign(0) != s && rA.sign(0) != s) continue;
           if (a.dim == 2 && !(c.has_partner2d(bx, by, bz, rB) || c.has_partner2d(ax, ay, az, rA))) continue;
           const int px = (s == 1) ? ex : 0, py = (s == 1) ? ey : 0, pz = (s == 1) ? ez : 0;
           const double v1 = c.F(d, ax + px, ay + py, az + pz), v2 = c.F(d, bx + px, by + py, bz + pz);
           const double v3 = c.F(tt, bx, by, bz);
-          val = -s * a.ih[tt] * a.h[d] * (v2 - v1) + v3;
+          val = -s * a.ihc(tt) * a.hc(d) * (v2 - v1) + v3;
           have = true;
         }
       }
@@ -309,9 +319,9 @@ __global__ void k_bc_apply(BcArgs a) {
     const size_t s = (size_t)b * MAC_BC + t;
     const unsigned f = a.flags[s];
     for (int i = 0; i < 3; ++i) {
-      if (!a.pend[i][s]) continue;
-      a.pend[i][s] = 0;
-      if ((f & 0x80u) && !(f & (0x08u << i))) a.U[i][s] = a.tmp[i][s];
+      if (!a.pendc(i)[s]) continue;
+      a.pendc(i)[s] = 0;
+      if ((f & 0x80u) && !(f & (0x08u << i))) a.Uc(i)[s] = a.tmpc(i)[s];
     }
   }
 }
@@ -322,11 +332,11 @@ __global__ void k_bc_corner45(BcArgs a) {
   BC_FOR_CELLS {
     bc::Cell c(a, b, t);
     const bc::Bnd r = c.bnd(0, 0, 0);
-    if (r.n != 2 || r.dir[0] == r.dir[1]) continue;
+    if (r.n != 2 || r.dir(0) == r.dir(1)) continue;
     for (int k = 0; k < 2; ++k) {
-      const int d = r.dir[k];
+      const int d = r.dir(k);
       const int ex = (d == 0), ey = (d == 1), ez = (d == 2);
-      if (r.sign[k] == -1)
+      if (r.sign(k) == -1)
         c.set(d, 0, 0, 0, c.F(d, ex, ey, ez));
       else
         c.set(d, ex, ey, ez, c.F(d, 0, 0, 0));
@@ -337,6 +347,7 @@ __global__ void k_bc_corner45(BcArgs a) {
 // 3x3 Gaussian elimination with partial pivoting on [A | b]; NaN on a zero pivot.
 __device__ inline void bc_gauss3(double A[3][4], double x[3]) {
 #pragma clang fp contract(off)
+#pragma unroll
   for (int col = 0; col < 3; ++col) {
     int piv = col;
     for (int i = col + 1; i < 3; ++i)
@@ -345,12 +356,16 @@ __device__ inline void bc_gauss3(double A[3][4], double x[3]) {
       x[0] = x[1] = x[2] = NAN;
       return;
     }
-    if (piv != col)
-      for (int j = 0; j < 4; ++j) {
-        const double tmp = A[col][j];
-        A[col][j] = A[piv][j];
-        A[piv][j] = tmp;
-      }
+    // the swap with the pivot row, row by row (no run-time row index)
+#pragma unroll
+    for (int i = col + 1; i < 3; ++i)
+      if (i == piv)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          const double tmp = A[col][j];
+          A[col][j] = A[i][j];
+          A[i][j] = tmp;
+        }
     for (int i = col + 1; i < 3; ++i) {
       const double m = A[i][col] / A[col][col];
       for (int j = col; j < 4; ++j) A[i][j] -= m * A[col][j];
@@ -373,16 +388,12 @@ __global__ void k_bc_corner3(BcArgs a) {
     bc::Cell c(a, b, t);
     const bc::Bnd r = c.bnd(0, 0, 0);
     if (r.n != 3) continue;
-    int sv[3] = {0, 0, 0};
-    for (int i = 0; i < 3; ++i)
-      for (int k = 0; k < 3; ++k)
-        if (r.dir[k] == i) {
-          sv[i] = r.sign[k];
-          break;
-        }
-    const int s1 = sv[0], s2 = sv[1], s3 = sv[2];
+    // the side of each axis's air neighbour (the low one first, as in the
+    // axis-major list; 0 without one)
+    auto side = [&](int i) { return (r.m & (1u << (2 * i))) ? -1 : ((r.m & (2u << (2 * i))) ? 1 : 0); };
+    const int s1 = side(0), s2 = side(1), s3 = side(2);
     if (!s1 || !s2 || !s3) continue;
-    const double i1 = a.ih[0], i2 = a.ih[1], i3 = a.ih[2];
+    const double i1 = a.ihc(0), i2 = a.ihc(1), i3 = a.ihc(2);
     auto U = [&](int comp, int dx, int dy, int dz) { return c.F(comp, dx, dy, dz); };
     double b0 = 0;
     b0 += (s1 == 1) ? -2 * i1 * U(0, 0, 0, 0) : 2 * i1 * U(0, 1, 0, 0);
@@ -429,8 +440,9 @@ __global__ void k_bc_corner3(BcArgs a) {
       A[2][3] = b2;
       bc_gauss3(A, res);
     }
+#pragma unroll
     for (int i = 0; i < 3; ++i) {
-      if (sv[i] == 1)
+      if (side(i) == 1)
         c.stage(i, i == 0, i == 1, i == 2, res[i]);
       else
         c.stage(i, 0, 0, 0, res[i]);
@@ -448,8 +460,8 @@ __global__ void k_bc_mark(BcArgs a) {
     if (a.dim == 2 && r.n == 1 && !c.has_partner2d(0, 0, 0, r)) mark = true;
     if (!mark) continue;
     for (int k = 0; k < r.n; ++k) {
-      const int d = r.dir[k];
-      const int ox = r.sign[k] == 1 ? (d == 0) : 0, oy = r.sign[k] == 1 ? (d == 1) : 0, oz = r.sign[k] == 1 ? (d == 2) : 0;
+      const int d = r.dir(k);
+      const int ox = r.sign(k) == 1 ? (d == 0) : 0, oy = r.sign(k) == 1 ? (d == 1) : 0, oz = r.sign(k) == 1 ? (d == 2) : 0;
       if (!c.in_domain(ox, oy, oz, d)) c.set(d, ox, oy, oz, NAN);
     }
   }
@@ -506,12 +518,12 @@ __global__ void k_bc_nan_extrap(BcArgs a) {
       if (wr[k]) cur[d][tsgn[k]] = vt;
     }
     double div = 0.0;
-    for (int i = 0; i < a.dim; ++i) div += a.ih[i] * (cur[i][1] - cur[i][0]);
+    for (int i = 0; i < a.dim; ++i) div += a.ihc(i) * (cur[i][1] - cur[i][0]);
     const double cor = div / nn;
     for (int k = 0; k < nn; ++k) {
       const int d = tdir[k];
       const int ox = tsgn[k] ? (d == 0) : 0, oy = tsgn[k] ? (d == 1) : 0, oz = tsgn[k] ? (d == 2) : 0;
-      const double nv = cur[d][tsgn[k]] - (2 * tsgn[k] - 1) * a.h[d] * cor;
+      const double nv = cur[d][tsgn[k]] - (2 * tsgn[k] - 1) * a.hc(d) * cor;
       if (wr[k]) {
         cur[d][tsgn[k]] = nv;
         c.stage(d, ox, oy, oz, nv);

@@ -79,6 +79,16 @@ struct Level {
   long long ndom = 0;
   uint8_t *code = nullptr, *lab = nullptr;
   double ih2[3] = {1, 1, 1};
+  // coarse levels: the operator's couplings and diagonal (NCF arrays of
+  // nb * MAC_BC values), FP64 and FP32
+  double* cf64 = nullptr;
+  float* cf32 = nullptr;
+  size_t ncell() const { return (size_t)nb * MAC_BC; }
+  template <typename T>
+  const T* cf() const {
+    if constexpr (sizeof(T) == 4) return (const T*)cf32;
+    else return (const T*)cf64;
+  }
   void *r = nullptr, *z = nullptr, *zt = nullptr, *res = nullptr;
 };
 
@@ -375,8 +385,10 @@ void build_nbr(Level& lv) {
 
 // Dense inverse of the coarsest operator (Cholesky; a tiny diagonal shift if
 // a component has no Dirichlet contact, which keeps the preconditioner SPD).
-void dense_inverse(const std::vector<uint8_t>& code, const double ih2[3], std::vector<int>& rowof, std::vector<int>& cellof,
-                   std::vector<double>& inv) {
+// cf (coarse levels): the operator's couplings and diagonal (NCF arrays of
+// MAC_BC values); nullptr on level 0, whose operator is the codes and ih2.
+void dense_inverse(const std::vector<uint8_t>& code, const double ih2[3], const double* cf, std::vector<int>& rowof,
+                   std::vector<int>& cellof, std::vector<double>& inv) {
   rowof.assign(MAC_BC, -1);
   cellof.clear();
   for (int t = 0; t < MAC_BC; ++t)
@@ -392,10 +404,11 @@ void dense_inverse(const std::vector<uint8_t>& code, const double ih2[3], std::v
   for (int i = 0; i < n; ++i) {
     int t = cellof[i];
     int x = t & 7, y = (t >> 3) & 7, z = t >> 6;
+    if (cf) A[(size_t)i * n + i] = cf[(size_t)6 * MAC_BC + t];
     for (int k = 0; k < 6; ++k) {
-      if (!(code[t] & (1u << k))) continue;
-      double w = ih2[k >> 1];
-      A[(size_t)i * n + i] += w;
+      if (!cf && !(code[t] & (1u << k))) continue;
+      double w = cf ? cf[(size_t)k * MAC_BC + t] : ih2[k >> 1];
+      if (!cf) A[(size_t)i * n + i] += w;
       int xx = x + dx[k], yy = y + dy[k], zz = z + dz[k];
       if (xx < 0 || yy < 0 || zz < 0 || xx > 7 || yy > 7 || zz > 7) continue;
       int j = rowof[xx + 8 * yy + 64 * zz];
@@ -538,6 +551,15 @@ int build_grid(mac_ctx* c, int nbricks, const int32_t* keys3, const uint8_t* fla
     k_coarsen_lab<<<lv.nb, KTHREADS, 0, c->st>>>(lv.child, c->L[l - 1].lab, lv.lab);
     k_code_c<<<lv.nb, KTHREADS, 0, c->st>>>(lv.nbr, lv.lab, lv.code);
   }
+  // the coarse operators, level by level from the one above
+  for (size_t l = 1; l < c->L.size(); ++l) {
+    Level& lv = c->L[l];
+    const Level& f = c->L[l - 1];
+    CHK(dalloc(c, &lv.cf64, NCF * lv.ncell()));
+    CHK(dalloc(c, &lv.cf32, NCF * lv.ncell()));
+    k_coarse_coef<<<lv.nb, MAC_BC, 0, c->st>>>(f.code, f.nbr, l > 1 ? f.cf64 : nullptr, f.ncell(), f.ih2[0], f.ih2[1], f.ih2[2],
+                                                lv.child, lv.code, lv.nbr, lv.cf64, lv.cf32, lv.ncell());
+  }
   HIPCHK(hipGetLastError());
   // active bricks per level: every level counted, then one read back
   std::vector<size_t> coff(c->L.size() + 1, 0);
@@ -548,9 +570,17 @@ int build_grid(mac_ctx* c, int nbricks, const int32_t* keys3, const uint8_t* fla
     k_brick_count<<<c->L[l].nb, KTHREADS, 0, c->st>>>(c->L[l].nb, c->L[l].code, dcnt + coff[l]);
   HIPCHK(hipGetLastError());
   std::vector<int> allcnt(coff.back());
-  std::vector<uint8_t> topcode(MAC_BC);  // the coarsest level's codes, for its exact solve
+  std::vector<uint8_t> topcode(MAC_BC);  // the coarsest level's codes and operator, for its exact solve
+  std::vector<double> topcf;
   HIPCHK(hipMemcpyAsync(allcnt.data(), dcnt, coff.back() * sizeof(int), hipMemcpyDeviceToHost, c->st));
   HIPCHK(hipMemcpyAsync(topcode.data(), c->L.back().code, MAC_BC, hipMemcpyDeviceToHost, c->st));
+  if (c->L.size() > 1) {
+    const Level& lt = c->L.back();  // a single brick
+    topcf.resize((size_t)NCF * MAC_BC);
+    for (int q = 0; q < NCF; ++q)
+      HIPCHK(hipMemcpyAsync(topcf.data() + (size_t)q * MAC_BC, lt.cf64 + (size_t)q * lt.ncell(), MAC_BC * sizeof(double),
+                            hipMemcpyDeviceToHost, c->st));
+  }
   HIPCHK(hipStreamSynchronize(c->st));
   for (size_t l = 0; l < c->L.size(); ++l) {
     Level& lv = c->L[l];
@@ -646,7 +676,7 @@ int build_grid(mac_ctx* c, int nbricks, const int32_t* keys3, const uint8_t* fla
     Level& lc = c->L.back();
     std::vector<int> rowof, cellof;
     std::vector<double> inv;
-    dense_inverse(topcode, lc.ih2, rowof, cellof, inv);
+    dense_inverse(topcode, lc.ih2, topcf.empty() ? nullptr : topcf.data(), rowof, cellof, inv);
     c->nc = (int)cellof.size();
     std::vector<float> inv32(inv.begin(), inv.end());
     up.add(&c->rowof, rowof);
@@ -782,14 +812,16 @@ void sweep(mac_ctx* c, int l, const TZ* zin, TZ* zout, int want_dot, double omeg
   const double sz = sizeof(TZ), nd = (double)lv.ndom;
   const double sr = (l == 0) ? 8.0 : sz;
   {
-  TimedLaunch tl(c, l == 0 ? K_JACOBI_0 : K_C_JACOBI, nd * (sr + (zin ? 2 : 1) * sz + 1));
+  // the coarse levels read NCF coefficient arrays instead of the code byte
+  TimedLaunch tl(c, l == 0 ? K_JACOBI_0 : K_C_JACOBI, nd * (sr + (zin ? 2 : 1) * sz + (l == 0 ? 1 : 1 + NCF * sz)));
   if (l == 0)
     k_jacobi<double, TZ><<<grid_of(lv.nact), KTHREADS, 0, c->st>>>(lv.nact, lv.meta, lv.code, c->r, zin, zout, ih[0], ih[1],
                                                                    ih[2], om, want_dot, c->partials, c->s);
   else
-    k_jacobi<TZ, TZ><<<grid_of(lv.nact), KTHREADS, 0, c->st>>>(lv.nact, lv.meta, lv.code, (const TZ*)lv.r, zin, zout, ih[0],
-                                                               ih[1], ih[2], om, want_dot, c->partials, c->s);
+    k_jacobi_coef<TZ><<<lv.nact, KTHREADS, 0, c->st>>>(lv.nact, lv.meta, lv.code, (const TZ*)lv.r, zin, zout, lv.cf<TZ>(),
+                                                       lv.ncell(), om, c->s);
   }
+  // coarse sweeps never take the dot product (the bottom level is level 0 then)
   if (want_dot) finalize(c, grid_of(lv.nact), OP_RZ);
 }
 
@@ -834,21 +866,23 @@ int vcycle(mac_ctx* c, TZ** zout) {
       const bool deep = l >= BLOCK_DEEP && nu > 1;
       // the first step from zero ran inside the restriction into this level
       for (int k = first_done[l] ? 1 : 0; k < (deep ? 1 : nu); ++k) {
-        TimedLaunch tl(c, K_C_JACOBI, nd * ((k ? 3 : 2) * sz + 1));  // r, zin, zout, code
+        TimedLaunch tl(c, K_C_JACOBI, nd * ((k ? 3 : 2) * sz + 1 + NCF * sz));  // r, zin, zout, code, operator
         k_jacobi_block<TZ, TZ><<<lv.nact, KTHREADS, 0, c->st>>>(lv.nact, lv.meta, lv.code, (const TZ*)lv.r, k ? z : nullptr,
-                                                                 k ? zt : z, ih[0], ih[1], ih[2],
+                                                                 k ? zt : z, lv.cf<TZ>(), lv.ncell(),
                                                                  deep ? c->blockw_deep : c->blockw, c->s);
         if (k) std::swap(z, zt);
       }
     } else if (nu >= 2) {
       // the first two sweeps from zero in one pass
-      TimedLaunch tl(c, l == 0 ? K_JACOBI2_0 : K_C_JACOBI2, nd * (sr + sz + 1));
-      if (l == 0)
+      if (l == 0) {
+        TimedLaunch tl(c, K_JACOBI2_0, nd * (sr + sz + 1));
         k_jacobi2<double, TZ><<<grid_of(lv.nact), KTHREADS, 0, c->st>>>(lv.nact, lv.meta, lv.code, c->r, z, ih[0], ih[1],
                                                                          ih[2], (TZ)opre[0], (TZ)opre[1], c->s);
-      else
-        k_jacobi2<TZ, TZ><<<grid_of(lv.nact), KTHREADS, 0, c->st>>>(lv.nact, lv.meta, lv.code, (const TZ*)lv.r, z, ih[0],
-                                                                     ih[1], ih[2], (TZ)opre[0], (TZ)opre[1], c->s);
+      } else {
+        // a coarse level: the two sweeps as two launches (zt from zero, then z)
+        sweep<TZ>(c, l, nullptr, zt, 0, opre[0]);
+        sweep<TZ>(c, l, zt, z, 0, opre[1]);
+      }
     } else {
       sweep<TZ>(c, l, nullptr, z, 0, opre[0]);
     }
@@ -863,22 +897,21 @@ int vcycle(mac_ctx* c, TZ** zout) {
         k_residual<double, TZ><<<grid_of(lv.nact), KTHREADS, 0, c->st>>>(lv.nact, lv.meta, lv.code, c->r, z, res, ih[0], ih[1],
                                                                          ih[2], c->s);
       else
-        k_residual<TZ, TZ><<<grid_of(lv.nact), KTHREADS, 0, c->st>>>(lv.nact, lv.meta, lv.code, (const TZ*)lv.r, z, res, ih[0],
-                                                                     ih[1], ih[2], c->s);
+        k_residual_coef<TZ><<<lv.nact, KTHREADS, 0, c->st>>>(lv.nact, lv.meta, lv.code, (const TZ*)lv.r, z, res, lv.cf<TZ>(),
+                                                             lv.ncell(), c->s);
     }
     {
       // into a block level visited below: the restriction also takes that
       // level's first smoothing step from zero (brick-local)
       const bool fuse = c->prm.fuse_coarse && l + 1 < tlev && is_block(l + 1);
       const bool cdeep = l + 1 >= BLOCK_DEEP && nu_of(c->prm, l + 1) > 1;
-      const TZ ihc[3] = {(TZ)cl.ih2[0], (TZ)cl.ih2[1], (TZ)cl.ih2[2]};
       const double cd = (double)cl.ndom;
       TZ* zsm = fuse ? (TZ*)cl.z : nullptr;
       const BlockW& bw = cdeep ? c->blockw_deep : c->blockw;
       {
         TimedLaunch tl(c, fuse ? (l == 0 ? K_RESTRICT_BLOCK1 : K_C_RESTRICT_BLOCK) : (l == 0 ? K_RESTRICT1 : K_C_RESTRICT),
-                       nd * sz + cd * sz + (fuse ? cd * (sz + 1) : 0.0));
-        k_restrict_sep<TZ><<<cl.nact, KTHREADS, 0, c->st>>>(cl.rmeta, cl.code, res, (TZ*)cl.r, c->s, zsm, ihc[0], ihc[1], ihc[2],
+                       nd * sz + cd * sz + (fuse ? cd * (sz + 1 + NCF * sz) : 0.0));
+        k_restrict_sep<TZ><<<cl.nact, KTHREADS, 0, c->st>>>(cl.rmeta, cl.code, res, (TZ*)cl.r, c->s, zsm, cl.cf<TZ>(), cl.ncell(),
                                                             bw);
       }
       first_done[l + 1] = fuse;
@@ -896,12 +929,11 @@ int vcycle(mac_ctx* c, TZ** zout) {
       a.om_pre[k] = c->omc_pre[k];
       a.om_post[k] = c->omc_post[k];
     }
-    for (int i = 0; i < 3; ++i) a.ih2[i] = lt.ih2[i];
     const TZ* cinv = std::is_same<TZ, float>::value ? (const TZ*)c->cinv32 : (const TZ*)c->cinv64;
     {
       TimedLaunch tl(c, K_TAIL, 0.0);
-      k_vcycle_tail<TZ><<<1, TAIL_THREADS, 0, c->st>>>(a, lt.code, (const TZ*)lt.r, (TZ*)lt.z, top.code, c->rowof, c->cellof,
-                                                       cinv, c->s);
+      k_vcycle_tail<TZ><<<1, TAIL_THREADS, 0, c->st>>>(a, lt.code, lt.cf<TZ>(), lt.ncell(), (const TZ*)lt.r, (TZ*)lt.z, top.code,
+                                                       c->rowof, c->cellof, cinv, c->s);
     }
     zfin[tlev] = (TZ*)lt.z;
   } else {
@@ -953,9 +985,9 @@ int vcycle(mac_ctx* c, TZ** zout) {
     if (c->prm.fuse_coarse && (blk || (MAC_FUSE_L0 && l == 0 && sizeof(TZ) == 4))) {
       // prolongation fused with the first post-smoothing step: zt = S(z + P zc)
       if (blk) {
-        TimedLaunch tl(c, K_C_PROLONG_BLOCK, nd * (3 * sz + 2) + (double)cl.ndom * sz);
+        TimedLaunch tl(c, K_C_PROLONG_BLOCK, nd * (3 * sz + 2 + NCF * sz) + (double)cl.ndom * sz);
         k_prolong_block<TZ><<<lv.nact, KTHREADS, 0, c->st>>>(lv.nact, lv.pmeta, lv.code, (const TZ*)lv.r, zfin[l + 1], z, zt,
-                                                              ih[0], ih[1], ih[2], deep ? c->blockw_deep : c->blockw, c->s);
+                                                              lv.cf<TZ>(), lv.ncell(), deep ? c->blockw_deep : c->blockw, c->s);
       } else {
         const int want_dot = nu == 1 ? 1 : 0;
         TimedLaunch tl(c, K_PROLONG_JACOBI0, nd * (8.0 + 2 * sz + 2) + (double)cl.ndom * sz);
@@ -971,9 +1003,9 @@ int vcycle(mac_ctx* c, TZ** zout) {
     }
     for (int k = k0; k < nu; ++k) {
       if (blk) {
-        TimedLaunch tl(c, K_C_JACOBI, nd * (3 * sz + 1));
-        k_jacobi_block<TZ, TZ><<<lv.nact, KTHREADS, 0, c->st>>>(lv.nact, lv.meta, lv.code, (const TZ*)lv.r, z, zt, ih[0], ih[1],
-                                                                 ih[2], deep ? c->blockw_deep : c->blockw, c->s);
+        TimedLaunch tl(c, K_C_JACOBI, nd * (3 * sz + 1 + NCF * sz));
+        k_jacobi_block<TZ, TZ><<<lv.nact, KTHREADS, 0, c->st>>>(lv.nact, lv.meta, lv.code, (const TZ*)lv.r, z, zt, lv.cf<TZ>(),
+                                                                 lv.ncell(), deep ? c->blockw_deep : c->blockw, c->s);
       } else {
         sweep<TZ>(c, l, z, zt, (l == 0 && k == nu - 1) ? 1 : 0, opost[k]);
       }

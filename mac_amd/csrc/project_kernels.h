@@ -621,6 +621,110 @@ __global__ void __launch_bounds__(KTHREADS) k_jacobi(int nact, const int* __rest
   if (want_dot) write_partial(acc, partials, sh, false);
 }
 
+// The coarse-level operator (levels >= 1).  Level l+1's operator is the
+// Galerkin operator P0^T A_l P0 / 16 of the piecewise-constant aggregation P0
+// (a coarse cell's value on its 8 children), which equals the rediscretised
+// 7-point operator away from boundaries: the coupling to a face neighbour is
+// the sum of the fine couplings across the coarse face / 16 (0 toward a cell
+// that is not an unknown: its share stays in the diagonal), the diagonal
+// (the children's diagonals - the couplings between children, both ways) /
+// 16.  A Neumann (const) face or a Dirichlet cell inside a coarse cell so
+// enters the coarse operator through the fine couplings it removes, and the
+// coarse grids see the boundary where the fine grid has it; with labels alone
+// a floor face inside the coarse cells made the multigrid take 24 and 34
+// iterations on deep pools 256 and 512 cells wide.  Stored per level as NCF
+// arrays of one value per cell (couplings -x,+x,-y,+y,-z,+z, then the
+// diagonal), in FP64 and FP32.
+#define NCF 7
+template <typename T>
+struct Coef {
+  T c[6];
+  T dg;
+};
+template <typename T>
+__device__ __forceinline__ Coef<T> load_coef(const T* __restrict__ cf, size_t ncell, size_t i) {
+  Coef<T> k;
+#pragma unroll
+  for (int q = 0; q < 6; ++q) k.c[q] = cf[(size_t)q * ncell + i];
+  k.dg = cf[(size_t)6 * ncell + i];
+  return k;
+}
+// L v at a coarse cell: dg v - sum of couplings x neighbours
+template <typename T>
+__device__ __forceinline__ T lap_coef(const Coef<T>& k, T vc, T xm, T xp, T ym, T yp, T zm, T zp) {
+  T acc = k.c[0] * xm;
+  acc += k.c[1] * xp;
+  acc += k.c[2] * ym;
+  acc += k.c[3] * yp;
+  acc += k.c[4] * zm;
+  acc += k.c[5] * zp;
+  return k.dg * vc - acc;
+}
+
+// One coarse brick per block, one coarse cell per thread: level l+1's
+// coefficients from level l's (cff, or for l = 0 the stencil codes and the
+// squared inverse spacings).  childc: the 8 level-l bricks of each level-(l+1)
+// brick (octant order, -1 = none); nbrf / nbrc: the 27-neighbour tables.
+__global__ void k_coarse_coef(const uint8_t* __restrict__ codef, const int* __restrict__ nbrf, const double* __restrict__ cff,
+                              size_t ncf, double ihx2, double ihy2, double ihz2, const int* __restrict__ childc,
+                              const uint8_t* __restrict__ codec, const int* __restrict__ nbrc, double* __restrict__ cfc,
+                              float* __restrict__ cfc32, size_t ncc) {
+  const int C = blockIdx.x;
+  const double ih2[3] = {ihx2, ihy2, ihz2};
+  for (int t = threadIdx.x; t < MAC_BC; t += blockDim.x) {
+    const size_t ci = (size_t)C * MAC_BC + t;
+    double cs[6] = {0, 0, 0, 0, 0, 0}, d = 0.0;
+    const int X = t & 7, Y = (t >> 3) & 7, Z = t >> 6;
+    if (codec[ci] & CODE_DOM) {
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        const int a = q & 1, b = (q >> 1) & 1, e = q >> 2;
+        const int fx = 2 * X + a, fy = 2 * Y + b, fz = 2 * Z + e;  // 0..15 over the 8 children
+        const int fb = childc[(size_t)C * 8 + (fx >> 3) + 2 * (fy >> 3) + 4 * (fz >> 3)];
+        if (fb < 0) continue;
+        const int lx = fx & 7, ly = fy & 7, lz = fz & 7;
+        const size_t fi = (size_t)fb * MAC_BC + lx + 8 * ly + 64 * lz;
+        const unsigned cdf = codef[fi];
+        if (!(cdf & CODE_DOM)) continue;
+        d += cff ? cff[(size_t)6 * ncf + fi] : diag_of<double>(cdf, ihx2, ihy2, ihz2);
+        const bool leaves[6] = {a == 0, a == 1, b == 0, b == 1, e == 0, e == 1};
+#pragma unroll
+        for (int k = 0; k < 6; ++k) {
+          double w;
+          if (cff) {
+            w = cff[(size_t)k * ncf + fi];
+          } else {
+            w = 0.0;
+            if (cdf & (1u << k)) {
+              const int sx = (k == 0) ? -1 : (k == 1) ? 1 : 0, sy = (k == 2) ? -1 : (k == 3) ? 1 : 0,
+                        sz = (k == 4) ? -1 : (k == 5) ? 1 : 0;
+              const unsigned cg = ld_rel(codef, nbrf, fb, lx + sx, ly + sy, lz + sz);
+              if (cg & CODE_DOM) w = ih2[k >> 1];
+            }
+          }
+          if (leaves[k]) cs[k] += w;
+          else d -= w;
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < 6; ++k) {
+        const int sx = (k == 0) ? -1 : (k == 1) ? 1 : 0, sy = (k == 2) ? -1 : (k == 3) ? 1 : 0,
+                  sz = (k == 4) ? -1 : (k == 5) ? 1 : 0;
+        const unsigned cn = ld_rel(codec, nbrc, C, X + sx, Y + sy, Z + sz);
+        cs[k] = (cn & CODE_DOM) ? cs[k] / 16.0 : 0.0;
+      }
+      d /= 16.0;
+    }
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+      cfc[(size_t)k * ncc + ci] = cs[k];
+      cfc32[(size_t)k * ncc + ci] = (float)cs[k];
+    }
+    cfc[(size_t)6 * ncc + ci] = d;
+    cfc32[(size_t)6 * ncc + ci] = (float)d;
+  }
+}
+
 // Block-Jacobi smoothing step: the brick's unknowns take m damped-Jacobi
 // sweeps (weights om[0..m-1]) among themselves with the neighbour bricks'
 // face planes frozen at zin, i.e. zout = zin + q(D^-1 A_BB) D^-1 (r - A zin)
@@ -647,9 +751,9 @@ struct SlabLDS {
   T pl[2][WPB][2][64];  // [buffer][wave][low/high plane][lane]
 };
 template <typename TZ>
-__device__ __forceinline__ void slab_sweeps(int wv, int lane, const unsigned (&cd)[2], const TZ (&rv)[2], const TZ (&idg)[2],
-                                            TZ (&col)[2], const TZ (&gx)[2], const TZ (&gy)[2], TZ below, TZ above, TZ ihx2,
-                                            TZ ihy2, TZ ihz2, const BlockW& w, SlabLDS<TZ>& L) {
+__device__ __forceinline__ void slab_sweeps(int wv, int lane, const unsigned (&cd)[2], const Coef<TZ> (&K)[2],
+                                            const TZ (&rv)[2], const TZ (&idg)[2], TZ (&col)[2], const TZ (&gx)[2],
+                                            const TZ (&gy)[2], TZ below, TZ above, const BlockW& w, SlabLDS<TZ>& L) {
   const int x = lane & 7, y = lane >> 3;
   for (int k = 0; k < w.m; ++k) {
     const TZ om = sizeof(TZ) == 4 ? (TZ)w.om[k] : (TZ)w.om64[k];
@@ -672,12 +776,11 @@ __device__ __forceinline__ void slab_sweeps(int wv, int lane, const unsigned (&c
       const TZ zp = j ? hi : col[1];
       TZ out = TZ(0);
       if (cd[j] & CODE_DOM) {
-        TZ dg;
-        const TZ lz = lap_cell<TZ>(cd[j], v, xm, xp, ym, yp, zm, zp, ihx2, ihy2, ihz2, dg);
+        const TZ lz = lap_coef<TZ>(K[j], v, xm, xp, ym, yp, zm, zp);
         if constexpr (sizeof(TZ) == 4)
           out = idg[j] != TZ(0) ? v + om * (rv[j] - lz) * idg[j] : TZ(0);
         else
-          out = dg > TZ(0) ? v + om * (rv[j] - lz) / dg : TZ(0);
+          out = K[j].dg > TZ(0) ? v + om * (rv[j] - lz) / K[j].dg : TZ(0);
       }
       nw[j] = out;
     }
@@ -687,20 +790,18 @@ __device__ __forceinline__ void slab_sweeps(int wv, int lane, const unsigned (&c
 }
 
 template <typename TZ>
-__device__ __forceinline__ void inv_diag_slab(const unsigned (&cd)[2], TZ ihx2, TZ ihy2, TZ ihz2, TZ (&idg)[2]) {
+__device__ __forceinline__ void inv_diag_slab(const unsigned (&cd)[2], const Coef<TZ> (&K)[2], TZ (&idg)[2]) {
 #pragma unroll
-  for (int j = 0; j < 2; ++j) {
-    const TZ dg = diag_of<TZ>(cd[j], ihx2, ihy2, ihz2);
-    idg[j] = ((cd[j] & CODE_DOM) && dg > TZ(0)) ? TZ(1) / dg : TZ(0);
-  }
+  for (int j = 0; j < 2; ++j) idg[j] = ((cd[j] & CODE_DOM) && K[j].dg > TZ(0)) ? TZ(1) / K[j].dg : TZ(0);
 }
 
 // The block step, one block per active brick.
 template <typename TR, typename TZ>
 __global__ void __launch_bounds__(KTHREADS) k_jacobi_block(int nact, const int* __restrict__ meta,
                                                                 const uint8_t* __restrict__ code, const TR* __restrict__ r,
-                                                                const TZ* __restrict__ zin, TZ* __restrict__ zout, TZ ihx2,
-                                                                TZ ihy2, TZ ihz2, BlockW w, const Scal* s) {
+                                                                const TZ* __restrict__ zin, TZ* __restrict__ zout,
+                                                                const TZ* __restrict__ cf, size_t ncell, BlockW w,
+                                                                const Scal* s) {
   const int i = xcd_block();
   if (converged(meta, MSTRIDE, nact, i, s)) return;
   __shared__ SlabLDS<TZ> L;
@@ -735,12 +836,115 @@ __global__ void __launch_bounds__(KTHREADS) k_jacobi_block(int nact, const int* 
     }
   }
   const TZ col0[2] = {col[0], col[1]};
+  const Coef<TZ> K[2] = {load_coef<TZ>(cf, ncell, base), load_coef<TZ>(cf, ncell, base + 64)};
   TZ idg[2];
-  inv_diag_slab<TZ>(cd, ihx2, ihy2, ihz2, idg);
-  slab_sweeps<TZ>(wv, lane, cd, rv, idg, col, gx, gy, below, above, ihx2, ihy2, ihz2, w, L);
+  inv_diag_slab<TZ>(cd, K, idg);
+  slab_sweeps<TZ>(wv, lane, cd, K, rv, idg, col, gx, gy, below, above, w, L);
   const TZ dmp = (TZ)w.damp;
 #pragma unroll
   for (int j = 0; j < 2; ++j) zout[base + 64 * j] = col0[j] + dmp * (col[j] - col0[j]);
+}
+
+// A coarse brick in the slab form without sweeps: wave w's planes 2w, 2w+1
+// of zv with their six neighbours (the planes 2w-1 and 2w+2 read from
+// memory, the x / y face planes of the neighbour bricks lane-parallel).
+template <typename TZ>
+struct SlabCells {
+  unsigned cd[2];
+  TZ v[2], xm[2], xp[2], ym[2], yp[2], zm[2], zp[2];
+  __device__ __forceinline__ void load(const int* M, int lane, int wv, const uint8_t* __restrict__ code,
+                                       const TZ* __restrict__ zv) {
+    const int x = lane & 7, y = lane >> 3;
+    const int b = M[0];
+    const size_t base = (size_t)b * MAC_BC + lane + 128 * wv;
+    auto ld = [&](int nb, int cell) { return nb >= 0 ? zv[(size_t)nb * MAC_BC + cell] : TZ(0); };
+    const TZ hxm = ld(M[1], 7 + 8 * x + 64 * y), hxp = ld(M[2], 8 * x + 64 * y);  // lane = y' + 8 z'
+    const TZ hym = ld(M[3], x + 56 + 64 * y), hyp = ld(M[4], x + 64 * y);         // lane = x' + 8 z'
+    const TZ below = wv > 0 ? zv[base - 64] : ld(M[5], lane + 448);
+    const TZ above = wv < WPB - 1 ? zv[base + 128] : ld(M[6], lane);
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      cd[j] = code[base + 64 * j];
+      v[j] = zv[base + 64 * j];
+    }
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      const int z = 2 * wv + j;
+      TZ a = __shfl_up(v[j], 1), bq = __shfl_down(v[j], 1), c = __shfl_up(v[j], 8), d = __shfl_down(v[j], 8);
+      const TZ gxm = __shfl(hxm, y + 8 * z), gxp = __shfl(hxp, y + 8 * z);
+      const TZ gym = __shfl(hym, x + 8 * z), gyp = __shfl(hyp, x + 8 * z);
+      xm[j] = x == 0 ? gxm : a;
+      xp[j] = x == 7 ? gxp : bq;
+      ym[j] = y == 0 ? gym : c;
+      yp[j] = y == 7 ? gyp : d;
+    }
+    zm[0] = below;
+    zp[0] = v[1];
+    zm[1] = v[0];
+    zp[1] = above;
+  }
+};
+
+// res = r - L z on a coarse level, one block per active brick.
+template <typename TZ>
+__global__ void __launch_bounds__(KTHREADS) k_residual_coef(int nact, const int* __restrict__ meta,
+                                                            const uint8_t* __restrict__ code, const TZ* __restrict__ r,
+                                                            const TZ* __restrict__ zv, TZ* __restrict__ res,
+                                                            const TZ* __restrict__ cf, size_t ncell, const Scal* s) {
+  const int i = xcd_block();
+  if (converged(meta, MSTRIDE, nact, i, s)) return;
+  const int lane = lane_id(), wv = wave_in_block();
+  const int* M = meta + (size_t)MSTRIDE * i;
+  const size_t base = (size_t)M[0] * MAC_BC + lane + 128 * wv;
+  SlabCells<TZ> sc;
+  sc.load(M, lane, wv, code, zv);
+#pragma unroll
+  for (int j = 0; j < 2; ++j) {
+    TZ out = TZ(0);
+    if (sc.cd[j] & CODE_DOM) {
+      const Coef<TZ> K = load_coef<TZ>(cf, ncell, base + 64 * j);
+      out = r[base + 64 * j] - lap_coef<TZ>(K, sc.v[j], sc.xm[j], sc.xp[j], sc.ym[j], sc.yp[j], sc.zm[j], sc.zp[j]);
+    }
+    res[base + 64 * j] = out;
+  }
+}
+
+// One damped-Jacobi sweep on a coarse level: zout = zin + omega D^-1 (r - L zin);
+// zin == nullptr: from zero (zout = omega D^-1 r).  One block per active brick.
+template <typename TZ>
+__global__ void __launch_bounds__(KTHREADS) k_jacobi_coef(int nact, const int* __restrict__ meta,
+                                                          const uint8_t* __restrict__ code, const TZ* __restrict__ r,
+                                                          const TZ* __restrict__ zin, TZ* __restrict__ zout,
+                                                          const TZ* __restrict__ cf, size_t ncell, TZ omega, const Scal* s) {
+  const int i = xcd_block();
+  if (converged(meta, MSTRIDE, nact, i, s)) return;
+  const int lane = lane_id(), wv = wave_in_block();
+  const int* M = meta + (size_t)MSTRIDE * i;
+  const size_t base = (size_t)M[0] * MAC_BC + lane + 128 * wv;
+  SlabCells<TZ> sc;
+  if (zin) {
+    sc.load(M, lane, wv, code, zin);
+  } else {
+#pragma unroll
+    for (int j = 0; j < 2; ++j) sc.cd[j] = code[base + 64 * j];
+  }
+#pragma unroll
+  for (int j = 0; j < 2; ++j) {
+    TZ out = TZ(0);
+    if (sc.cd[j] & CODE_DOM) {
+      const Coef<TZ> K = load_coef<TZ>(cf, ncell, base + 64 * j);
+      const TZ rv = r[base + 64 * j];
+      if (K.dg > TZ(0)) {
+        if (zin)
+          out = sc.v[j] + div_diag<TZ>(omega * (rv - lap_coef<TZ>(K, sc.v[j], sc.xm[j], sc.xp[j], sc.ym[j], sc.yp[j], sc.zm[j],
+                                                                      sc.zp[j])),
+                                       K.dg);
+        else
+          out = div_diag<TZ>(omega * rv, K.dg);
+      }
+    }
+    zout[base + 64 * j] = out;
+  }
 }
 
 // Two damped-Jacobi sweeps from a zero start in one pass:
@@ -803,8 +1007,8 @@ __global__ void __launch_bounds__(KTHREADS) k_jacobi2(int nact, const int* __res
 // LDS (rcl), in the slab form, run by all four waves of the block.
 template <typename T>
 __device__ __forceinline__ void smooth_from_zero(int C, const uint8_t* __restrict__ codec, const T* rcl,
-                                                      T* __restrict__ zsm, T ihx2, T ihy2, T ihz2, const BlockW& w,
-                                                      SlabLDS<T>& L) {
+                                                      T* __restrict__ zsm, const T* __restrict__ cf, size_t ncell,
+                                                      const BlockW& w, SlabLDS<T>& L) {
   const int lane = lane_id(), wv = wave_in_block();
   const size_t base = (size_t)C * MAC_BC + lane + 128 * wv;
   unsigned cd[2];
@@ -816,8 +1020,9 @@ __device__ __forceinline__ void smooth_from_zero(int C, const uint8_t* __restric
     col[j] = T(0);
     gz[j] = T(0);
   }
-  inv_diag_slab<T>(cd, ihx2, ihy2, ihz2, idg);
-  slab_sweeps<T>(wv, lane, cd, rv, idg, col, gz, gz, T(0), T(0), ihx2, ihy2, ihz2, w, L);
+  const Coef<T> K[2] = {load_coef<T>(cf, ncell, base), load_coef<T>(cf, ncell, base + 64)};
+  inv_diag_slab<T>(cd, K, idg);
+  slab_sweeps<T>(wv, lane, cd, K, rv, idg, col, gz, gz, T(0), T(0), w, L);
   const T dmp = (T)w.damp;
 #pragma unroll
   for (int j = 0; j < 2; ++j) zsm[base + 64 * j] = T(0) + dmp * (col[j] - T(0));
@@ -826,7 +1031,7 @@ __device__ __forceinline__ void smooth_from_zero(int C, const uint8_t* __restric
 template <typename T>
 __global__ void __launch_bounds__(KTHREADS) k_restrict_sep(const int* __restrict__ rmeta, const uint8_t* __restrict__ codec,
                                                            const T* __restrict__ resf, T* __restrict__ rc, const Scal* s,
-                                                           T* __restrict__ zsm, T ihx2, T ihy2, T ihz2, BlockW w) {
+                                                           T* __restrict__ zsm, const T* __restrict__ cfc, size_t ncc, BlockW w) {
   if (converged(rmeta, RSTRIDE, (int)gridDim.x, xcd_block(), s)) return;
   __shared__ T rcl[MAC_BC];
   if constexpr (sizeof(T) == 4) {
@@ -928,7 +1133,7 @@ __global__ void __launch_bounds__(KTHREADS) k_restrict_sep(const int* __restrict
     if (zsm) {
       __shared__ SlabLDS<T> L;
       __syncthreads();
-      smooth_from_zero<T>(C, codec, rcl, zsm, ihx2, ihy2, ihz2, w, L);
+      smooth_from_zero<T>(C, codec, rcl, zsm, cfc, ncc, w, L);
     }
     return;
   }
@@ -988,7 +1193,7 @@ __global__ void __launch_bounds__(KTHREADS) k_restrict_sep(const int* __restrict
   if (zsm) {
     __shared__ SlabLDS<T> L;
     __syncthreads();
-    smooth_from_zero<T>(C, codec, rcl, zsm, ihx2, ihy2, ihz2, w, L);
+    smooth_from_zero<T>(C, codec, rcl, zsm, cfc, ncc, w, L);
   }
 }
 
@@ -1210,8 +1415,8 @@ template <typename T>
 __global__ void __launch_bounds__(KTHREADS) k_prolong_block(int nact, const int* __restrict__ pmeta,
                                                                  const uint8_t* __restrict__ code, const T* __restrict__ r,
                                                                  const T* __restrict__ zc, const T* __restrict__ zfin,
-                                                                 T* __restrict__ zout, T ihx2, T ihy2, T ihz2, BlockW w,
-                                                                 const Scal* s) {
+                                                                 T* __restrict__ zout, const T* __restrict__ cf, size_t ncell,
+                                                                 BlockW w, const Scal* s) {
   const int i = xcd_block();
   if (converged(pmeta, PSTRIDE, nact, i, s)) return;
   __shared__ SlabLDS<T> L;
@@ -1261,9 +1466,10 @@ __global__ void __launch_bounds__(KTHREADS) k_prolong_block(int nact, const int*
     above = (ldc(F[5], lane) & CODE_DOM) ? za + pc.template at<2, 1>(0) : za;
   }
   const T col0[2] = {col[0], col[1]};
+  const Coef<T> K[2] = {load_coef<T>(cf, ncell, base), load_coef<T>(cf, ncell, base + 64)};
   T idg[2];
-  inv_diag_slab<T>(cd, ihx2, ihy2, ihz2, idg);
-  slab_sweeps<T>(wv, lane, cd, rv, idg, col, gx, gy, below, above, ihx2, ihy2, ihz2, w, L);
+  inv_diag_slab<T>(cd, K, idg);
+  slab_sweeps<T>(wv, lane, cd, K, rv, idg, col, gx, gy, below, above, w, L);
   const T dmp = (T)w.damp;
 #pragma unroll
   for (int j = 0; j < 2; ++j) zout[base + 64 * j] = col0[j] + dmp * (col[j] - col0[j]);
@@ -1376,7 +1582,8 @@ __global__ void k_coarse_solve(int nc, const int* __restrict__ rowof, const int*
 // The last two levels of the V-cycle in one launch of one 1024-thread block.
 // The level above the top brick holds at most its 8 children, a 16^3 box of
 // cells; the box lives in LDS (z, its sweep partner, the restriction rows),
-// r and the codes in registers (4 cells per thread), so every sweep is one
+// r, the codes and the operator's coefficients in registers (4 cells per
+// thread), so every sweep is one
 // pass over LDS between two barriers instead of a launch with its chain of
 // dependent global loads.  The steps and their expressions are those of
 // k_jacobi2 / k_jacobi (pre-sweeps), k_residual, k_restrict_sep,
@@ -1387,11 +1594,11 @@ struct TailArgs {
   int nu;                 // sweeps before and after the correction
   int nc;                 // unknowns of the top brick
   double om_pre[8], om_post[8];
-  double ih2[3];          // level t
 };
 #define TAIL_THREADS 1024
 template <typename TZ>
 __global__ void __launch_bounds__(TAIL_THREADS) k_vcycle_tail(TailArgs a, const uint8_t* __restrict__ code_t,
+                                                              const TZ* __restrict__ cf_t, size_t nct,
                                                               const TZ* __restrict__ r_t, TZ* __restrict__ z_t,
                                                               const uint8_t* __restrict__ code_c, const int* __restrict__ rowof,
                                                               const int* __restrict__ cellof, const TZ* __restrict__ cinv,
@@ -1401,10 +1608,10 @@ __global__ void __launch_bounds__(TAIL_THREADS) k_vcycle_tail(TailArgs a, const 
   __shared__ TZ t1[2048];
   __shared__ TZ rc[MAC_BC], zc[MAC_BC];
   const int tid = threadIdx.x;
-  const TZ ihx2 = (TZ)a.ih2[0], ihy2 = (TZ)a.ih2[1], ihz2 = (TZ)a.ih2[2];
   long long g[4];
   unsigned cd[4];
   TZ rv[4];
+  Coef<TZ> K[4];
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
     const int i = tid + TAIL_THREADS * k;
@@ -1413,6 +1620,7 @@ __global__ void __launch_bounds__(TAIL_THREADS) k_vcycle_tail(TailArgs a, const 
     g[k] = b >= 0 ? (long long)b * MAC_BC + (X & 7) + 8 * (Y & 7) + 64 * (Z & 7) : -1;
     cd[k] = g[k] >= 0 ? code_t[g[k]] : 0u;
     rv[k] = g[k] >= 0 ? r_t[g[k]] : TZ(0);
+    if (cd[k] & CODE_DOM) K[k] = load_coef<TZ>(cf_t, nct, (size_t)g[k]);
   }
   auto at = [](const TZ* v, int X, int Y, int Z) -> TZ {
     return ((unsigned)X < 16u && (unsigned)Y < 16u && (unsigned)Z < 16u) ? v[X + 16 * Y + 256 * Z] : TZ(0);
@@ -1425,13 +1633,12 @@ __global__ void __launch_bounds__(TAIL_THREADS) k_vcycle_tail(TailArgs a, const 
       const int X = i & 15, Y = (i >> 4) & 15, Z = i >> 8;
       TZ out = TZ(0);
       if (cd[k] & CODE_DOM) {
+        const TZ dg = K[k].dg;
         if (zin) {
-          TZ dg;
-          const TZ lz = lap_cell<TZ>(cd[k], zin[i], at(zin, X - 1, Y, Z), at(zin, X + 1, Y, Z), at(zin, X, Y - 1, Z),
-                                     at(zin, X, Y + 1, Z), at(zin, X, Y, Z - 1), at(zin, X, Y, Z + 1), ihx2, ihy2, ihz2, dg);
+          const TZ lz = lap_coef<TZ>(K[k], zin[i], at(zin, X - 1, Y, Z), at(zin, X + 1, Y, Z), at(zin, X, Y - 1, Z),
+                                     at(zin, X, Y + 1, Z), at(zin, X, Y, Z - 1), at(zin, X, Y, Z + 1));
           out = dg > TZ(0) ? zin[i] + om * (rv[k] - lz) / dg : TZ(0);
         } else {
-          const TZ dg = diag_of<TZ>(cd[k], ihx2, ihy2, ihz2);
           out = dg > TZ(0) ? om * rv[k] / dg : TZ(0);
         }
       }
@@ -1453,11 +1660,9 @@ __global__ void __launch_bounds__(TAIL_THREADS) k_vcycle_tail(TailArgs a, const 
     const int i = tid + TAIL_THREADS * k;
     const int X = i & 15, Y = (i >> 4) & 15, Z = i >> 8;
     TZ out = TZ(0);
-    if (cd[k] & CODE_DOM) {
-      TZ dg;
-      out = rv[k] - lap_cell<TZ>(cd[k], z[i], at(z, X - 1, Y, Z), at(z, X + 1, Y, Z), at(z, X, Y - 1, Z), at(z, X, Y + 1, Z),
-                                 at(z, X, Y, Z - 1), at(z, X, Y, Z + 1), ihx2, ihy2, ihz2, dg);
-    }
+    if (cd[k] & CODE_DOM)
+      out = rv[k] - lap_coef<TZ>(K[k], z[i], at(z, X - 1, Y, Z), at(z, X + 1, Y, Z), at(z, X, Y - 1, Z), at(z, X, Y + 1, Z),
+                                 at(z, X, Y, Z - 1), at(z, X, Y, Z + 1));
     res[i] = out;
   }
   __syncthreads();

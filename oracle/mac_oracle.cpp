@@ -442,6 +442,10 @@ struct DLevel {
   double ih2[3];
   std::vector<uint8_t> code, lab;
   std::vector<double> r, z, zt, res;
+  // coarse levels: the operator as face couplings cf[k] (-x,+x,-y,+y,-z,+z;
+  // 0 toward a cell that is not an unknown) and the diagonal dg
+  bool coef = false;
+  std::vector<double> cf[6], dg;
   long long size() const { return (long long)n[0] * n[1] * n[2]; }
   long long id(int x, int y, int z) const { return x + (long long)n[0] * (y + (long long)n[1] * z); }
 };
@@ -487,11 +491,11 @@ struct MG {
           const unsigned cd = l.code[c];
           double out = 0;
           if (cd & 0x40) {
-            double acc = 0, d = 0;
+            double acc = 0, d = l.coef ? l.dg[c] : 0.0;
             for (int q = 0; q < 6; ++q) {
-              if (!(cd & (1u << q))) continue;
-              const double w = l.ih2[q >> 1];
-              d += w;
+              if (!(cd & (1u << q)) && !l.coef) continue;
+              const double w = l.coef ? l.cf[q][c] : l.ih2[q >> 1];
+              if (!l.coef) d += w;
               const int xx = x + DX[q], yy = y + DY[q], zz = z + DZ[q];
               double v;
               if (xx >= 0 && yy >= 0 && zz >= 0 && xx < 8 && yy < 8 && zz < 8) {
@@ -517,10 +521,11 @@ struct MG {
     long long c = l.id(x, y, z);
     unsigned cd = l.code[c];
     double acc = 0, d = 0;
+    if (l.coef) d = l.dg[c];
     for (int k = 0; k < 6; ++k) {
-      if (!(cd & (1u << k))) continue;
-      double w = l.ih2[k >> 1];
-      d += w;
+      if (!(cd & (1u << k)) && !l.coef) continue;
+      double w = l.coef ? l.cf[k][c] : l.ih2[k >> 1];
+      if (!l.coef) d += w;
       int xx = x + DX[k], yy = y + DY[k], zz = z + DZ[k];
       if (xx < 0 || yy < 0 || zz < 0 || xx >= l.n[0] || yy >= l.n[1] || zz >= l.n[2]) continue;
       acc += w * v[l.id(xx, yy, zz)];
@@ -543,8 +548,10 @@ struct MG {
               out = dg > 0 ? (*zin)[c] + omega * (r[c] - lz) / dg : 0;
             } else {
               double dg = 0;
-              for (int k = 0; k < 6; ++k)
-                if (l.code[c] & (1u << k)) dg += l.ih2[k >> 1];
+              if (l.coef) dg = l.dg[c];
+              else
+                for (int k = 0; k < 6; ++k)
+                  if (l.code[c] & (1u << k)) dg += l.ih2[k >> 1];
               out = dg > 0 ? omega * r[c] / dg : 0;
             }
           }
@@ -817,6 +824,65 @@ int orc_mgpcg(int nb, const int32_t* keys, const uint8_t* flags, const double* h
           }
           cl.code[C] = cd;
         }
+    {
+      // The coarse operator: the Galerkin operator P0^T A P0 / 16 of the
+      // piecewise-constant aggregation P0 (a coarse cell's value on its 8
+      // children), which equals the rediscretised 7-point operator away from
+      // boundaries.  Coupling to a face neighbour = the sum of the fine
+      // couplings across the coarse face / 16 (0 when the neighbour is not
+      // a coarse unknown: its share stays in the diagonal); diagonal = (the
+      // children's diagonals - the couplings between children, both ways) /
+      // 16.  A Neumann face or a Dirichlet cell inside a coarse cell enters
+      // through the fine couplings it removes, so the coarse grids see the
+      // boundary where the fine grid has it.
+      cl.coef = true;
+      for (int k = 0; k < 6; ++k) cl.cf[k].assign(cl.size(), 0.0);
+      cl.dg.assign(cl.size(), 0.0);
+      auto fcoup = [&](int x, int y, int z, int k) -> double {
+        const long long c = f.id(x, y, z);
+        if (f.coef) return f.cf[k][c];
+        if (!(f.code[c] & (1u << k))) return 0.0;
+        const int xx = x + DX[k], yy = y + DY[k], zz = z + DZ[k];
+        if (xx < 0 || yy < 0 || zz < 0 || xx >= f.n[0] || yy >= f.n[1] || zz >= f.n[2]) return 0.0;
+        return (f.code[f.id(xx, yy, zz)] & 0x40) ? f.ih2[k >> 1] : 0.0;
+      };
+      auto fdiag = [&](int x, int y, int z) -> double {
+        const long long c = f.id(x, y, z);
+        if (f.coef) return f.dg[c];
+        double d = 0;
+        for (int k = 0; k < 6; ++k)
+          if (f.code[c] & (1u << k)) d += f.ih2[k >> 1];
+        return d;
+      };
+#pragma omp parallel for collapse(2) schedule(static)
+      for (int Z = 0; Z < cl.n[2]; ++Z)
+        for (int Y = 0; Y < cl.n[1]; ++Y)
+          for (int X = 0; X < cl.n[0]; ++X) {
+            const long long C = cl.id(X, Y, Z);
+            if (!(cl.code[C] & 0x40)) continue;
+            double cs[6] = {0, 0, 0, 0, 0, 0}, d = 0;
+            for (int q = 0; q < 8; ++q) {
+              const int a = q & 1, b = (q >> 1) & 1, e = q >> 2;
+              const int x = 2 * X + a, y = 2 * Y + b, z = 2 * Z + e;
+              if (x >= f.n[0] || y >= f.n[1] || z >= f.n[2]) continue;
+              if (!(f.code[f.id(x, y, z)] & 0x40)) continue;
+              d += fdiag(x, y, z);
+              const int side[6] = {a == 0, a == 1, b == 0, b == 1, e == 0, e == 1};  // face k leaves the block
+              for (int k = 0; k < 6; ++k) {
+                const double w = fcoup(x, y, z, k);
+                if (side[k]) cs[k] += w;
+                else d -= w;
+              }
+            }
+            cl.dg[C] = d / 16.0;
+            for (int k = 0; k < 6; ++k) {
+              const int XX = X + DX[k], YY = Y + DY[k], ZZ = Z + DZ[k];
+              const bool nb_unknown = XX >= 0 && YY >= 0 && ZZ >= 0 && XX < cl.n[0] && YY < cl.n[1] && ZZ < cl.n[2] &&
+                                      (cl.code[cl.id(XX, YY, ZZ)] & 0x40);
+              cl.cf[k][C] = nb_unknown ? cs[k] / 16.0 : 0.0;
+            }
+          }
+    }
   }
   // coarsest dense inverse
   {
@@ -832,10 +898,11 @@ int orc_mgpcg(int nb, const int32_t* keys, const uint8_t* flags, const double* h
     for (int i = 0; i < n; ++i) {
       long long c = mg.cellof[i];
       int x = (int)(c % lc.n[0]), y = (int)((c / lc.n[0]) % lc.n[1]), z = (int)(c / ((long long)lc.n[0] * lc.n[1]));
+      if (lc.coef) A[(size_t)i * n + i] = lc.dg[c];
       for (int k = 0; k < 6; ++k) {
-        if (!(lc.code[c] & (1u << k))) continue;
-        double wgt = lc.ih2[k >> 1];
-        A[(size_t)i * n + i] += wgt;
+        if (!(lc.code[c] & (1u << k)) && !lc.coef) continue;
+        double wgt = lc.coef ? lc.cf[k][c] : lc.ih2[k >> 1];
+        if (!lc.coef) A[(size_t)i * n + i] += wgt;
         int xx = x + DX[k], yy = y + DY[k], zz = z + DZ[k];
         if (xx < 0 || yy < 0 || zz < 0 || xx >= lc.n[0] || yy >= lc.n[1] || zz >= lc.n[2]) continue;
         int j = rowof[lc.id(xx, yy, zz)];

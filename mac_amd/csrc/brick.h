@@ -19,7 +19,10 @@
 //              neighbour value is 0 for every cell outside the domain, which is
 //              how air and missing cells act as the p=0 Dirichlet ghost).
 //   bit 6:     the cell is an unknown (in the domain).
+//   bit 7:     (coarse levels) the cell's Galerkin coefficients are the
+//              rediscretised ones that bits 0..5 and the spacings give.
 #define CODE_DOM 0x40u
+#define CODE_STD 0x80u
 
 // Labels used to coarsen the cell types from one level to the next.
 #define LAB_N 0  // Neumann-like: solid, missing

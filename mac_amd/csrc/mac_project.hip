@@ -89,6 +89,10 @@ struct Level {
     if constexpr (sizeof(T) == 4) return (const T*)cf32;
     else return (const T*)cf64;
   }
+  template <typename T>
+  Ih<T> ihc() const {
+    return {(T)ih2[0], (T)ih2[1], (T)ih2[2]};
+  }
   void *r = nullptr, *z = nullptr, *zt = nullptr, *res = nullptr;
 };
 
@@ -299,6 +303,13 @@ struct TimedLaunch {
 // One-block reduction of the partials of the previous kernel plus the CG
 // scalar step (see k_finalize); returns the launch's sequence number.
 // to_host: the launch also writes the state to the pinned mirror (poll_scal).
+// x = 0 on every cell of the level-0 bricks
+void clear_x(mac_ctx* c) {
+  const size_t n16 = (size_t)c->nb0 * MAC_BC * sizeof(double) / 16;
+  const unsigned blocks = (unsigned)std::min<size_t>((n16 + 511) / 512, 65535);
+  k_zero16<<<blocks, 256, 0, c->st>>>((uint4*)c->x, n16);
+}
+
 long long finalize(mac_ctx* c, int nparts, int op, bool to_host = false) {
   TimedLaunch tl(c, K_FIN, 0.0);
   k_finalize<<<1, 1024, 0, c->st>>>(nparts, c->partials, c->s, op, c->prm.rtol * c->prm.rtol, to_host ? c->hs : nullptr);
@@ -812,14 +823,15 @@ void sweep(mac_ctx* c, int l, const TZ* zin, TZ* zout, int want_dot, double omeg
   const double sz = sizeof(TZ), nd = (double)lv.ndom;
   const double sr = (l == 0) ? 8.0 : sz;
   {
-  // the coarse levels read NCF coefficient arrays instead of the code byte
-  TimedLaunch tl(c, l == 0 ? K_JACOBI_0 : K_C_JACOBI, nd * (sr + (zin ? 2 : 1) * sz + (l == 0 ? 1 : 1 + NCF * sz)));
+  // (the coarse levels' coefficient arrays, read near boundaries only, are
+  // not counted)
+  TimedLaunch tl(c, l == 0 ? K_JACOBI_0 : K_C_JACOBI, nd * (sr + (zin ? 2 : 1) * sz + 1));
   if (l == 0)
     k_jacobi<double, TZ><<<grid_of(lv.nact), KTHREADS, 0, c->st>>>(lv.nact, lv.meta, lv.code, c->r, zin, zout, ih[0], ih[1],
                                                                    ih[2], om, want_dot, c->partials, c->s);
   else
     k_jacobi_coef<TZ><<<lv.nact, KTHREADS, 0, c->st>>>(lv.nact, lv.meta, lv.code, (const TZ*)lv.r, zin, zout, lv.cf<TZ>(),
-                                                       lv.ncell(), om, c->s);
+                                                       lv.ncell(), lv.ihc<TZ>(), om, c->s);
   }
   // coarse sweeps never take the dot product (the bottom level is level 0 then)
   if (want_dot) finalize(c, grid_of(lv.nact), OP_RZ);
@@ -866,9 +878,9 @@ int vcycle(mac_ctx* c, TZ** zout) {
       const bool deep = l >= BLOCK_DEEP && nu > 1;
       // the first step from zero ran inside the restriction into this level
       for (int k = first_done[l] ? 1 : 0; k < (deep ? 1 : nu); ++k) {
-        TimedLaunch tl(c, K_C_JACOBI, nd * ((k ? 3 : 2) * sz + 1 + NCF * sz));  // r, zin, zout, code, operator
+        TimedLaunch tl(c, K_C_JACOBI, nd * ((k ? 3 : 2) * sz + 1));  // r, zin, zout, code
         k_jacobi_block<TZ, TZ><<<lv.nact, KTHREADS, 0, c->st>>>(lv.nact, lv.meta, lv.code, (const TZ*)lv.r, k ? z : nullptr,
-                                                                 k ? zt : z, lv.cf<TZ>(), lv.ncell(),
+                                                                 k ? zt : z, lv.cf<TZ>(), lv.ncell(), lv.ihc<TZ>(),
                                                                  deep ? c->blockw_deep : c->blockw, c->s);
         if (k) std::swap(z, zt);
       }
@@ -898,7 +910,7 @@ int vcycle(mac_ctx* c, TZ** zout) {
                                                                          ih[2], c->s);
       else
         k_residual_coef<TZ><<<lv.nact, KTHREADS, 0, c->st>>>(lv.nact, lv.meta, lv.code, (const TZ*)lv.r, z, res, lv.cf<TZ>(),
-                                                             lv.ncell(), c->s);
+                                                             lv.ncell(), lv.ihc<TZ>(), c->s);
     }
     {
       // into a block level visited below: the restriction also takes that
@@ -910,9 +922,9 @@ int vcycle(mac_ctx* c, TZ** zout) {
       const BlockW& bw = cdeep ? c->blockw_deep : c->blockw;
       {
         TimedLaunch tl(c, fuse ? (l == 0 ? K_RESTRICT_BLOCK1 : K_C_RESTRICT_BLOCK) : (l == 0 ? K_RESTRICT1 : K_C_RESTRICT),
-                       nd * sz + cd * sz + (fuse ? cd * (sz + 1 + NCF * sz) : 0.0));
+                       nd * sz + cd * sz + (fuse ? cd * (sz + 1) : 0.0));
         k_restrict_sep<TZ><<<cl.nact, KTHREADS, 0, c->st>>>(cl.rmeta, cl.code, res, (TZ*)cl.r, c->s, zsm, cl.cf<TZ>(), cl.ncell(),
-                                                            bw);
+                                                            cl.ihc<TZ>(), bw);
       }
       first_done[l + 1] = fuse;
     }
@@ -925,6 +937,7 @@ int vcycle(mac_ctx* c, TZ** zout) {
     for (int o = 0; o < 8; ++o) a.child[o] = top.h_child[o];
     a.nu = nu_of(c->prm, tlev);
     a.nc = c->nc;
+    for (int k = 0; k < 3; ++k) a.ih2[k] = lt.ih2[k];
     for (int k = 0; k < 8; ++k) {
       a.om_pre[k] = c->omc_pre[k];
       a.om_post[k] = c->omc_post[k];
@@ -985,9 +998,10 @@ int vcycle(mac_ctx* c, TZ** zout) {
     if (c->prm.fuse_coarse && (blk || (MAC_FUSE_L0 && l == 0 && sizeof(TZ) == 4))) {
       // prolongation fused with the first post-smoothing step: zt = S(z + P zc)
       if (blk) {
-        TimedLaunch tl(c, K_C_PROLONG_BLOCK, nd * (3 * sz + 2 + NCF * sz) + (double)cl.ndom * sz);
+        TimedLaunch tl(c, K_C_PROLONG_BLOCK, nd * (3 * sz + 2) + (double)cl.ndom * sz);
         k_prolong_block<TZ><<<lv.nact, KTHREADS, 0, c->st>>>(lv.nact, lv.pmeta, lv.code, (const TZ*)lv.r, zfin[l + 1], z, zt,
-                                                              lv.cf<TZ>(), lv.ncell(), deep ? c->blockw_deep : c->blockw, c->s);
+                                                              lv.cf<TZ>(), lv.ncell(), lv.ihc<TZ>(),
+                                                              deep ? c->blockw_deep : c->blockw, c->s);
       } else {
         const int want_dot = nu == 1 ? 1 : 0;
         TimedLaunch tl(c, K_PROLONG_JACOBI0, nd * (8.0 + 2 * sz + 2) + (double)cl.ndom * sz);
@@ -1003,9 +1017,10 @@ int vcycle(mac_ctx* c, TZ** zout) {
     }
     for (int k = k0; k < nu; ++k) {
       if (blk) {
-        TimedLaunch tl(c, K_C_JACOBI, nd * (3 * sz + 1 + NCF * sz));
+        TimedLaunch tl(c, K_C_JACOBI, nd * (3 * sz + 1));
         k_jacobi_block<TZ, TZ><<<lv.nact, KTHREADS, 0, c->st>>>(lv.nact, lv.meta, lv.code, (const TZ*)lv.r, z, zt, lv.cf<TZ>(),
-                                                                 lv.ncell(), deep ? c->blockw_deep : c->blockw, c->s);
+                                                                 lv.ncell(), lv.ihc<TZ>(), deep ? c->blockw_deep : c->blockw,
+                                                                 c->s);
       } else {
         sweep<TZ>(c, l, z, zt, (l == 0 && k == nu - 1) ? 1 : 0, opost[k]);
       }
@@ -1110,7 +1125,7 @@ int pcg(mac_ctx* c, bool use_guess, bool r_ready, mac_stats* st) {
   Level& l0 = c->L[0];
   const double* ih = l0.ih2;
   size_t n0 = (size_t)c->nb0 * MAC_BC;
-  if (!use_guess) HIPCHK(hipMemsetAsync(c->x, 0, n0 * sizeof(double), c->st));
+  if (!use_guess) clear_x(c);
   const int check = std::max(1, c->prm.check_every);
   const double nd = (double)l0.ndom;
   int restarts = 0;
@@ -1131,7 +1146,7 @@ int pcg(mac_ctx* c, bool use_guess, bool r_ready, mac_stats* st) {
     iter_now = 0;
     zero_x = true;
     HIPCHK(hipMemcpyAsync(&c->s->iter, &c->hs->iter, 2 * sizeof(int), hipMemcpyHostToDevice, c->st));
-    HIPCHK(hipMemsetAsync(c->x, 0, n0 * sizeof(double), c->st));
+    clear_x(c);
     return 0;
   };
   for (;;) {
@@ -1482,7 +1497,7 @@ int mac_project(mac_ctx* c, int guess, mac_stats* st) {
     else rc = pcg<double>(c, guess != 0, r_ready, st);
     c->solves_on_grid += 1;
   } else {
-    HIPCHK(hipMemsetAsync(c->x, 0, (size_t)c->nb0 * MAC_BC * sizeof(double), c->st));
+    clear_x(c);
   }
   auto ts1 = std::chrono::steady_clock::now();
   if (rc != 0 && rc != MAC_E_NOCONV) return rc;

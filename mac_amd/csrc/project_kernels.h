@@ -205,65 +205,72 @@ __device__ __forceinline__ void write_partial(double v, double* __restrict__ par
 // One block: total of the partials in block order, then the scalar step.
 // mirror (pinned host memory, or null): receives a copy of the state, so a
 // host that waits for this launch reads the state without a copy of its own.
-__global__ void k_finalize(int n, const double* __restrict__ partials, Scal* s, int op, double rtol2, Scal* mirror) {
+// Thread 0 loads the whole state while every thread loads its partials, so
+// the launch makes one round trip to memory (a converged solve's skipped
+// steps sum the stale partials and drop the total).
+__global__ void __launch_bounds__(1024) k_finalize(int n, const double* __restrict__ partials, Scal* s, int op, double rtol2,
+                                                   Scal* mirror) {
   __shared__ double sh[16];
-  if ((op == OP_DQ || op == OP_RR || op == OP_RZ) && s->done) {
-    if (mirror && threadIdx.x == 0) *mirror = *s;
-    return;
-  }
   const bool mx = (op == OP_MAX);
+  Scal sv;
+  if (threadIdx.x == 0) sv = *s;
   double acc = 0.0;
   for (int i = threadIdx.x; i < n; i += blockDim.x) acc = mx ? fmax(acc, partials[i]) : acc + partials[i];
   double t = mx ? block_max(acc, sh) : block_sum(acc, sh);
   if (threadIdx.x != 0) return;
+  if ((op == OP_DQ || op == OP_RR || op == OP_RZ) && sv.done) {
+    if (mirror) *mirror = sv;
+    return;
+  }
   switch (op) {
     case OP_FF:
-      s->ff = t;
-      s->tol2 = rtol2 * t;
-      s->done = (t == 0.0) ? 1 : 0;
+      sv.ff = t;
+      sv.tol2 = rtol2 * t;
+      sv.done = (t == 0.0) ? 1 : 0;
       break;
     case OP_FF_R0:
       // ||f||^2 of a divergence pass that also wrote r = f (a zero start):
       // the OP_FF and OP_RR0 steps with ||r||^2 = ||f||^2
-      s->ff = t;
-      s->tol2 = rtol2 * t;
-      s->rr = t;
-      s->rz_valid = 0;
-      s->done = (t == 0.0 || t <= s->tol2) ? 1 : 0;
+      sv.ff = t;
+      sv.tol2 = rtol2 * t;
+      sv.rr = t;
+      sv.rz_valid = 0;
+      sv.done = (t == 0.0 || t <= sv.tol2) ? 1 : 0;
       break;
     case OP_RR0:
-      s->rr = t;
-      s->rz_valid = 0;
-      s->done = (t <= s->tol2) ? 1 : 0;
+      sv.rr = t;
+      sv.rz_valid = 0;
+      sv.done = (t <= sv.tol2) ? 1 : 0;
       break;
     case OP_TRUE:
-      s->tmp = t;
+      sv.tmp = t;
       break;
     case OP_DQ:
-      s->dq = t;
+      sv.dq = t;
       if (t > 0.0) {
-        s->alpha = s->rz / t;
+        sv.alpha = sv.rz / t;
       } else {
-        s->alpha = 0.0;
-        s->done = 3;
+        sv.alpha = 0.0;
+        sv.done = 3;
       }
       break;
     case OP_RR:
-      s->rr = t;
-      s->iter += 1;
-      if (t <= s->tol2) s->done = 1;
-      else if (s->iter >= s->max_iter) s->done = 2;
+      sv.rr = t;
+      sv.iter += 1;
+      if (t <= sv.tol2) sv.done = 1;
+      else if (sv.iter >= sv.max_iter) sv.done = 2;
       break;
     case OP_RZ:
-      s->beta = s->rz_valid ? t / s->rz : 0.0;
-      s->rz = t;
-      s->rz_valid = 1;
+      sv.beta = sv.rz_valid ? t / sv.rz : 0.0;
+      sv.rz = t;
+      sv.rz_valid = 1;
       break;
     case OP_MAX:
-      s->maxdiv = t;
+      sv.maxdiv = t;
       break;
   }
-  if (mirror) *mirror = *s;
+  *s = sv;
+  if (mirror) *mirror = sv;
 }
 
 // ---------------------------------------------------------------- setup ----
@@ -358,6 +365,13 @@ __global__ void k_zero_list(ZeroList zl) {
   double* p = zl.p[blockIdx.y];
   const size_t n = zl.n[blockIdx.y];
   for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = 0.0;
+}
+
+// Clears n 16-byte words: a plain fill kernel, about three times the rate of
+// the runtime's fill on the pressure vector, and a kernel node when captured.
+__global__ void __launch_bounds__(256) k_zero16(uint4* __restrict__ p, size_t n) {
+  const uint4 z = make_uint4(0u, 0u, 0u, 0u);
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = z;
 }
 
 __global__ void k_brick_count(int nb, const uint8_t* __restrict__ code, int* __restrict__ cnt) {
@@ -661,13 +675,38 @@ __device__ __forceinline__ T lap_coef(const Coef<T>& k, T vc, T xm, T xp, T ym, 
   return k.dg * vc - acc;
 }
 
+// A coarse cell whose coefficients are the rediscretised 7-point ones (the
+// code byte's CODE_STD bit: couplings inv_h^2 on the faces of the code, the
+// code's diagonal) takes them from the code and the level's squared inverse
+// spacings instead of reading the NCF arrays; most coarse cells are such
+// cells, so the arrays are read near boundaries only.
+template <typename T>
+struct Ih {
+  T x, y, z;
+};
+template <typename T>
+__device__ __forceinline__ Coef<T> coef_at(unsigned cd, const T* __restrict__ cf, size_t ncell, size_t i, Ih<T> ih) {
+  if (cd & CODE_STD) {
+    Coef<T> k;
+    k.c[0] = (cd & 1u) ? ih.x : T(0);
+    k.c[1] = (cd & 2u) ? ih.x : T(0);
+    k.c[2] = (cd & 4u) ? ih.y : T(0);
+    k.c[3] = (cd & 8u) ? ih.y : T(0);
+    k.c[4] = (cd & 16u) ? ih.z : T(0);
+    k.c[5] = (cd & 32u) ? ih.z : T(0);
+    k.dg = diag_of<T>(cd, ih.x, ih.y, ih.z);
+    return k;
+  }
+  return load_coef<T>(cf, ncell, i);
+}
+
 // One coarse brick per block, one coarse cell per thread: level l+1's
 // coefficients from level l's (cff, or for l = 0 the stencil codes and the
 // squared inverse spacings).  childc: the 8 level-l bricks of each level-(l+1)
 // brick (octant order, -1 = none); nbrf / nbrc: the 27-neighbour tables.
 __global__ void k_coarse_coef(const uint8_t* __restrict__ codef, const int* __restrict__ nbrf, const double* __restrict__ cff,
                               size_t ncf, double ihx2, double ihy2, double ihz2, const int* __restrict__ childc,
-                              const uint8_t* __restrict__ codec, const int* __restrict__ nbrc, double* __restrict__ cfc,
+                              uint8_t* __restrict__ codec, const int* __restrict__ nbrc, double* __restrict__ cfc,
                               float* __restrict__ cfc32, size_t ncc) {
   const int C = blockIdx.x;
   const double ih2[3] = {ihx2, ihy2, ihz2};
@@ -706,14 +745,34 @@ __global__ void k_coarse_coef(const uint8_t* __restrict__ codef, const int* __re
           else d -= w;
         }
       }
+      // the coarse level's own spacings; a cell within 1e-12 of the
+      // rediscretised operator (on every face toward an unknown) takes it
+      // exactly and is marked CODE_STD
+      const unsigned cc = codec[ci];
+      const double ic[3] = {0.25 * ihx2, 0.25 * ihy2, 0.25 * ihz2};
+      bool stdc = true;
 #pragma unroll
       for (int k = 0; k < 6; ++k) {
         const int sx = (k == 0) ? -1 : (k == 1) ? 1 : 0, sy = (k == 2) ? -1 : (k == 3) ? 1 : 0,
                   sz = (k == 4) ? -1 : (k == 5) ? 1 : 0;
         const unsigned cn = ld_rel(codec, nbrc, C, X + sx, Y + sy, Z + sz);
         cs[k] = (cn & CODE_DOM) ? cs[k] / 16.0 : 0.0;
+        const double want = ((cn & CODE_DOM) && (cc & (1u << k))) ? ic[k >> 1] : 0.0;
+        stdc = stdc && fabs(cs[k] - want) <= 1e-12 * ic[k >> 1];
       }
       d /= 16.0;
+      const double dstd = diag_of<double>(cc, ic[0], ic[1], ic[2]);
+      stdc = stdc && fabs(d - dstd) <= 1e-12 * dstd;
+      if (stdc) {
+#pragma unroll
+        for (int k = 0; k < 6; ++k) cs[k] = cs[k] != 0.0 ? ic[k >> 1] : 0.0;
+        d = dstd;
+      }
+      // codec is read by the neighbouring threads' ld_rel above only for
+      // CODE_DOM, which the mark leaves alone
+      if (stdc) codec[ci] = (uint8_t)(cc | CODE_STD);
+    } else {
+      codec[ci] = (uint8_t)(codec[ci] | CODE_STD);  // all-zero coefficients, as the code gives
     }
 #pragma unroll
     for (int k = 0; k < 6; ++k) {
@@ -800,8 +859,8 @@ template <typename TR, typename TZ>
 __global__ void __launch_bounds__(KTHREADS) k_jacobi_block(int nact, const int* __restrict__ meta,
                                                                 const uint8_t* __restrict__ code, const TR* __restrict__ r,
                                                                 const TZ* __restrict__ zin, TZ* __restrict__ zout,
-                                                                const TZ* __restrict__ cf, size_t ncell, BlockW w,
-                                                                const Scal* s) {
+                                                                const TZ* __restrict__ cf, size_t ncell, Ih<TZ> ih,
+                                                                BlockW w, const Scal* s) {
   const int i = xcd_block();
   if (converged(meta, MSTRIDE, nact, i, s)) return;
   __shared__ SlabLDS<TZ> L;
@@ -836,7 +895,7 @@ __global__ void __launch_bounds__(KTHREADS) k_jacobi_block(int nact, const int* 
     }
   }
   const TZ col0[2] = {col[0], col[1]};
-  const Coef<TZ> K[2] = {load_coef<TZ>(cf, ncell, base), load_coef<TZ>(cf, ncell, base + 64)};
+  const Coef<TZ> K[2] = {coef_at<TZ>(cd[0], cf, ncell, base, ih), coef_at<TZ>(cd[1], cf, ncell, base + 64, ih)};
   TZ idg[2];
   inv_diag_slab<TZ>(cd, K, idg);
   slab_sweeps<TZ>(wv, lane, cd, K, rv, idg, col, gx, gy, below, above, w, L);
@@ -890,7 +949,8 @@ template <typename TZ>
 __global__ void __launch_bounds__(KTHREADS) k_residual_coef(int nact, const int* __restrict__ meta,
                                                             const uint8_t* __restrict__ code, const TZ* __restrict__ r,
                                                             const TZ* __restrict__ zv, TZ* __restrict__ res,
-                                                            const TZ* __restrict__ cf, size_t ncell, const Scal* s) {
+                                                            const TZ* __restrict__ cf, size_t ncell, Ih<TZ> ih,
+                                                            const Scal* s) {
   const int i = xcd_block();
   if (converged(meta, MSTRIDE, nact, i, s)) return;
   const int lane = lane_id(), wv = wave_in_block();
@@ -902,7 +962,7 @@ __global__ void __launch_bounds__(KTHREADS) k_residual_coef(int nact, const int*
   for (int j = 0; j < 2; ++j) {
     TZ out = TZ(0);
     if (sc.cd[j] & CODE_DOM) {
-      const Coef<TZ> K = load_coef<TZ>(cf, ncell, base + 64 * j);
+      const Coef<TZ> K = coef_at<TZ>(sc.cd[j], cf, ncell, base + 64 * j, ih);
       out = r[base + 64 * j] - lap_coef<TZ>(K, sc.v[j], sc.xm[j], sc.xp[j], sc.ym[j], sc.yp[j], sc.zm[j], sc.zp[j]);
     }
     res[base + 64 * j] = out;
@@ -915,7 +975,8 @@ template <typename TZ>
 __global__ void __launch_bounds__(KTHREADS) k_jacobi_coef(int nact, const int* __restrict__ meta,
                                                           const uint8_t* __restrict__ code, const TZ* __restrict__ r,
                                                           const TZ* __restrict__ zin, TZ* __restrict__ zout,
-                                                          const TZ* __restrict__ cf, size_t ncell, TZ omega, const Scal* s) {
+                                                          const TZ* __restrict__ cf, size_t ncell, Ih<TZ> ih, TZ omega,
+                                                          const Scal* s) {
   const int i = xcd_block();
   if (converged(meta, MSTRIDE, nact, i, s)) return;
   const int lane = lane_id(), wv = wave_in_block();
@@ -932,7 +993,7 @@ __global__ void __launch_bounds__(KTHREADS) k_jacobi_coef(int nact, const int* _
   for (int j = 0; j < 2; ++j) {
     TZ out = TZ(0);
     if (sc.cd[j] & CODE_DOM) {
-      const Coef<TZ> K = load_coef<TZ>(cf, ncell, base + 64 * j);
+      const Coef<TZ> K = coef_at<TZ>(sc.cd[j], cf, ncell, base + 64 * j, ih);
       const TZ rv = r[base + 64 * j];
       if (K.dg > TZ(0)) {
         if (zin)
@@ -1008,7 +1069,7 @@ __global__ void __launch_bounds__(KTHREADS) k_jacobi2(int nact, const int* __res
 template <typename T>
 __device__ __forceinline__ void smooth_from_zero(int C, const uint8_t* __restrict__ codec, const T* rcl,
                                                       T* __restrict__ zsm, const T* __restrict__ cf, size_t ncell,
-                                                      const BlockW& w, SlabLDS<T>& L) {
+                                                      Ih<T> ih, const BlockW& w, SlabLDS<T>& L) {
   const int lane = lane_id(), wv = wave_in_block();
   const size_t base = (size_t)C * MAC_BC + lane + 128 * wv;
   unsigned cd[2];
@@ -1020,7 +1081,7 @@ __device__ __forceinline__ void smooth_from_zero(int C, const uint8_t* __restric
     col[j] = T(0);
     gz[j] = T(0);
   }
-  const Coef<T> K[2] = {load_coef<T>(cf, ncell, base), load_coef<T>(cf, ncell, base + 64)};
+  const Coef<T> K[2] = {coef_at<T>(cd[0], cf, ncell, base, ih), coef_at<T>(cd[1], cf, ncell, base + 64, ih)};
   inv_diag_slab<T>(cd, K, idg);
   slab_sweeps<T>(wv, lane, cd, K, rv, idg, col, gz, gz, T(0), T(0), w, L);
   const T dmp = (T)w.damp;
@@ -1031,7 +1092,8 @@ __device__ __forceinline__ void smooth_from_zero(int C, const uint8_t* __restric
 template <typename T>
 __global__ void __launch_bounds__(KTHREADS) k_restrict_sep(const int* __restrict__ rmeta, const uint8_t* __restrict__ codec,
                                                            const T* __restrict__ resf, T* __restrict__ rc, const Scal* s,
-                                                           T* __restrict__ zsm, const T* __restrict__ cfc, size_t ncc, BlockW w) {
+                                                           T* __restrict__ zsm, const T* __restrict__ cfc, size_t ncc, Ih<T> ih,
+                                                           BlockW w) {
   if (converged(rmeta, RSTRIDE, (int)gridDim.x, xcd_block(), s)) return;
   __shared__ T rcl[MAC_BC];
   if constexpr (sizeof(T) == 4) {
@@ -1133,7 +1195,7 @@ __global__ void __launch_bounds__(KTHREADS) k_restrict_sep(const int* __restrict
     if (zsm) {
       __shared__ SlabLDS<T> L;
       __syncthreads();
-      smooth_from_zero<T>(C, codec, rcl, zsm, cfc, ncc, w, L);
+      smooth_from_zero<T>(C, codec, rcl, zsm, cfc, ncc, ih, w, L);
     }
     return;
   }
@@ -1193,7 +1255,7 @@ __global__ void __launch_bounds__(KTHREADS) k_restrict_sep(const int* __restrict
   if (zsm) {
     __shared__ SlabLDS<T> L;
     __syncthreads();
-    smooth_from_zero<T>(C, codec, rcl, zsm, cfc, ncc, w, L);
+    smooth_from_zero<T>(C, codec, rcl, zsm, cfc, ncc, ih, w, L);
   }
 }
 
@@ -1416,7 +1478,7 @@ __global__ void __launch_bounds__(KTHREADS) k_prolong_block(int nact, const int*
                                                                  const uint8_t* __restrict__ code, const T* __restrict__ r,
                                                                  const T* __restrict__ zc, const T* __restrict__ zfin,
                                                                  T* __restrict__ zout, const T* __restrict__ cf, size_t ncell,
-                                                                 BlockW w, const Scal* s) {
+                                                                 Ih<T> ih, BlockW w, const Scal* s) {
   const int i = xcd_block();
   if (converged(pmeta, PSTRIDE, nact, i, s)) return;
   __shared__ SlabLDS<T> L;
@@ -1466,7 +1528,7 @@ __global__ void __launch_bounds__(KTHREADS) k_prolong_block(int nact, const int*
     above = (ldc(F[5], lane) & CODE_DOM) ? za + pc.template at<2, 1>(0) : za;
   }
   const T col0[2] = {col[0], col[1]};
-  const Coef<T> K[2] = {load_coef<T>(cf, ncell, base), load_coef<T>(cf, ncell, base + 64)};
+  const Coef<T> K[2] = {coef_at<T>(cd[0], cf, ncell, base, ih), coef_at<T>(cd[1], cf, ncell, base + 64, ih)};
   T idg[2];
   inv_diag_slab<T>(cd, K, idg);
   slab_sweeps<T>(wv, lane, cd, K, rv, idg, col, gx, gy, below, above, w, L);
@@ -1593,6 +1655,7 @@ struct TailArgs {
   int child[8];           // level-t bricks of the box, octant order (-1 = none)
   int nu;                 // sweeps before and after the correction
   int nc;                 // unknowns of the top brick
+  double ih2[3];          // level t's squared inverse spacings (CODE_STD cells)
   double om_pre[8], om_post[8];
 };
 #define TAIL_THREADS 1024
@@ -1620,7 +1683,7 @@ __global__ void __launch_bounds__(TAIL_THREADS) k_vcycle_tail(TailArgs a, const 
     g[k] = b >= 0 ? (long long)b * MAC_BC + (X & 7) + 8 * (Y & 7) + 64 * (Z & 7) : -1;
     cd[k] = g[k] >= 0 ? code_t[g[k]] : 0u;
     rv[k] = g[k] >= 0 ? r_t[g[k]] : TZ(0);
-    if (cd[k] & CODE_DOM) K[k] = load_coef<TZ>(cf_t, nct, (size_t)g[k]);
+    if (cd[k] & CODE_DOM) K[k] = coef_at<TZ>(cd[k], cf_t, nct, (size_t)g[k], Ih<TZ>{(TZ)a.ih2[0], (TZ)a.ih2[1], (TZ)a.ih2[2]});
   }
   auto at = [](const TZ* v, int X, int Y, int Z) -> TZ {
     return ((unsigned)X < 16u && (unsigned)Y < 16u && (unsigned)Z < 16u) ? v[X + 16 * Y + 256 * Z] : TZ(0);

@@ -875,11 +875,24 @@ int orc_mgpcg(int nb, const int32_t* keys, const uint8_t* flags, const double* h
               }
             }
             cl.dg[C] = d / 16.0;
+            // a cell within 1e-12 of the rediscretised operator (on every
+            // face toward an unknown) takes it exactly, as the device does
+            const unsigned cc = cl.code[C];
+            bool std_cell = true;
+            double dstd = 0.0;
             for (int k = 0; k < 6; ++k) {
               const int XX = X + DX[k], YY = Y + DY[k], ZZ = Z + DZ[k];
               const bool nb_unknown = XX >= 0 && YY >= 0 && ZZ >= 0 && XX < cl.n[0] && YY < cl.n[1] && ZZ < cl.n[2] &&
                                       (cl.code[cl.id(XX, YY, ZZ)] & 0x40);
               cl.cf[k][C] = nb_unknown ? cs[k] / 16.0 : 0.0;
+              const double want = (nb_unknown && (cc & (1u << k))) ? cl.ih2[k >> 1] : 0.0;
+              std_cell = std_cell && std::fabs(cl.cf[k][C] - want) <= 1e-12 * cl.ih2[k >> 1];
+              if (cc & (1u << k)) dstd += cl.ih2[k >> 1];
+            }
+            std_cell = std_cell && std::fabs(cl.dg[C] - dstd) <= 1e-12 * dstd;
+            if (std_cell) {
+              for (int k = 0; k < 6; ++k) cl.cf[k][C] = cl.cf[k][C] != 0.0 ? cl.ih2[k >> 1] : 0.0;
+              cl.dg[C] = dstd;
             }
           }
     }

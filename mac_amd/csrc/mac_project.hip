@@ -1005,9 +1005,19 @@ int vcycle(mac_ctx* c, TZ** zout) {
       } else {
         const int want_dot = nu == 1 ? 1 : 0;
         TimedLaunch tl(c, K_PROLONG_JACOBI0, nd * (8.0 + 2 * sz + 2) + (double)cl.ndom * sz);
-        k_prolong_jacobi<TZ><<<grid_of(lv.nact), KTHREADS, 0, c->st>>>(lv.nact, lv.pmeta, lv.code, c->r, zfin[l + 1], z, zt, ih[0],
-                                                                        ih[1], ih[2], (TZ)opost[0], want_dot, c->partials, c->s);
-        if (want_dot) finalize(c, grid_of(lv.nact), OP_RZ);
+#ifndef MAC_PJ_SLAB
+#define MAC_PJ_SLAB 1
+#endif
+        if (MAC_PJ_SLAB) {
+          k_prolong_jacobi_slab<TZ><<<lv.nact, KTHREADS, 0, c->st>>>(lv.nact, lv.pmeta, lv.code, c->r, zfin[l + 1], z, zt, ih[0],
+                                                                      ih[1], ih[2], (TZ)opost[0], want_dot, c->partials, c->s);
+          if (want_dot) finalize(c, lv.nact, OP_RZ);
+        } else {
+          k_prolong_jacobi<TZ><<<grid_of(lv.nact), KTHREADS, 0, c->st>>>(lv.nact, lv.pmeta, lv.code, c->r, zfin[l + 1], z, zt,
+                                                                          ih[0], ih[1], ih[2], (TZ)opost[0], want_dot, c->partials,
+                                                                          c->s);
+          if (want_dot) finalize(c, grid_of(lv.nact), OP_RZ);
+        }
       }
       std::swap(z, zt);
       k0 = 1;

@@ -37,6 +37,7 @@ struct Scal {
 };
 
 #define KTHREADS 256
+
 #define WPB 4          // waves per block
 #ifndef GRID_CAP
 #define GRID_CAP (1 << 30)  // no cap: one wave per brick, the dispatcher balances the CUs
@@ -603,6 +604,11 @@ __global__ void __launch_bounds__(KTHREADS) k_jacobi(int nact, const int* __rest
 #pragma unroll
     for (int z = 0; z < 8; ++z) cd[z] = code[base + 64 * z];
     if (zin) {
+      // r loaded with the codes and the column, ahead of the sweep (its
+      // loads inside the per-cell branches would wait one by one)
+      TR rr[8];
+#pragma unroll
+      for (int z = 0; z < 8; ++z) rr[z] = r[base + 64 * z];
       TZ col[8];
 #pragma unroll
       for (int z = 0; z < 8; ++z) col[z] = zin[base + 64 * z];
@@ -612,7 +618,7 @@ __global__ void __launch_bounds__(KTHREADS) k_jacobi(int nact, const int* __rest
                         if (cd[z] & CODE_DOM) {
                           TZ dg;
                           TZ lz = lap_cell<TZ>(cd[z], col[z], xm, xp, ym, yp, zm, zp, ihx2, ihy2, ihz2, dg);
-                          TR rv = r[base + 64 * z];
+                          TR rv = rr[z];
                           out = dg > TZ(0) ? col[z] + div_diag<TZ>(omega * ((TZ)rv - lz), dg) : TZ(0);
                           if (want_dot) acc += (double)rv * (double)out;
                         }
@@ -1580,6 +1586,102 @@ k_prolong_jacobi(int nact, const int* __restrict__ pmeta,
       }
       zout[base + 64 * z] = out;
     }
+  }
+  if (want_dot) write_partial(acc, partials, sh, false);
+}
+
+// k_prolong_jacobi in the slab form: one 256-thread block per active fine
+// brick, wave w on z-planes 2w, 2w+1 (a quarter of the column form's
+// registers, four times the waves).  The prolongated values of the wave's
+// planes and of the cells across the x / y faces are formed as in
+// k_prolong_block (the face planes of zfin and of the codes loaded only on
+// the 16 lanes that hold the wave's planes); the planes 2w-1 and 2w+2 come
+// from the neighbouring waves through LDS (one barrier).  Same expressions
+// as k_prolong_jacobi, so zout is bitwise the same; want_dot: one partial
+// r . zout per block.
+template <typename TZ>
+__global__ void __launch_bounds__(KTHREADS) k_prolong_jacobi_slab(int nact, const int* __restrict__ pmeta,
+                                                                  const uint8_t* __restrict__ code, const double* __restrict__ r,
+                                                                  const TZ* __restrict__ zc, const TZ* __restrict__ zfin,
+                                                                  TZ* __restrict__ zout, TZ ihx2, TZ ihy2, TZ ihz2, TZ omega,
+                                                                  int want_dot, double* __restrict__ partials, const Scal* s) {
+  const int i = xcd_block();
+  if (converged(pmeta, PSTRIDE, nact, i, s)) return;
+  __shared__ TZ pl[WPB][2][64];
+  __shared__ double sh[16];
+  const int lane = lane_id(), wv = wave_in_block();
+  const int x = lane & 7, y = lane >> 3;
+  const int* M = pmeta + (size_t)PSTRIDE * i;
+  const int* F = M + 10;  // face neighbours, the order of meta
+  const size_t base = (size_t)M[0] * MAC_BC + lane + 128 * wv;
+  SlabCols<TZ> pc;
+  pc.gather(M, lane, wv, zc);
+  unsigned cd[2];
+  TZ col[2];
+  double rv[2];
+#pragma unroll
+  for (int j = 0; j < 2; ++j) {
+    cd[j] = code[base + 64 * j];
+    rv[j] = r[base + 64 * j];
+    const TZ v = zfin[base + 64 * j];
+    col[j] = (cd[j] & CODE_DOM) ? v + pc.at_plane(j, 0) : v;
+  }
+  // face planes: lane = y' + 8 z' (x faces) or x' + 8 z' (y faces); the
+  // wave's planes z' = 2w, 2w+1 are lanes 16w .. 16w+15
+  const bool mine = (lane >> 4) == wv;
+  auto ldz = [&](int nb, int cell) { return (mine && nb >= 0) ? zfin[(size_t)nb * MAC_BC + cell] : TZ(0); };
+  auto ldc = [&](int nb, int cell) { return (mine && nb >= 0) ? (unsigned)code[(size_t)nb * MAC_BC + cell] : 0u; };
+  const TZ zxm = ldz(F[0], 7 + 8 * x + 64 * y), zxp = ldz(F[1], 8 * x + 64 * y);
+  const TZ zym = ldz(F[2], x + 56 + 64 * y), zyp = ldz(F[3], x + 64 * y);
+  const unsigned cxm = ldc(F[0], 7 + 8 * x + 64 * y), cxp = ldc(F[1], 8 * x + 64 * y);
+  const unsigned cym = ldc(F[2], x + 56 + 64 * y), cyp = ldc(F[3], x + 64 * y);
+  TZ gx[2], gy[2], below = TZ(0), above = TZ(0);
+#pragma unroll
+  for (int j = 0; j < 2; ++j) {
+    const int z = 2 * wv + j;
+    const TZ vxm = __shfl(zxm, y + 8 * z), vxp = __shfl(zxp, y + 8 * z);
+    const TZ vym = __shfl(zym, x + 8 * z), vyp = __shfl(zyp, x + 8 * z);
+    const unsigned gcxm = __shfl(cxm, y + 8 * z), gcxp = __shfl(cxp, y + 8 * z);
+    const unsigned gcym = __shfl(cym, x + 8 * z), gcyp = __shfl(cyp, x + 8 * z);
+    const TZ vx = x == 0 ? vxm : vxp;
+    const unsigned cx = x == 0 ? gcxm : gcxp;
+    const TZ vy = y == 0 ? vym : vyp;
+    const unsigned cy = y == 0 ? gcym : gcyp;
+    gx[j] = (cx & CODE_DOM) ? vx + pc.at_plane(j, 1) : vx;
+    gy[j] = (cy & CODE_DOM) ? vy + pc.at_plane(j, 2) : vy;
+  }
+  if (wv == 0 && F[4] >= 0) {
+    const TZ zb = zfin[(size_t)F[4] * MAC_BC + lane + 448];
+    below = (code[(size_t)F[4] * MAC_BC + lane + 448] & CODE_DOM) ? zb + pc.template at<0, 1>(0) : zb;
+  }
+  if (wv == WPB - 1 && F[5] >= 0) {
+    const TZ za = zfin[(size_t)F[5] * MAC_BC + lane];
+    above = (code[(size_t)F[5] * MAC_BC + lane] & CODE_DOM) ? za + pc.template at<2, 1>(0) : za;
+  }
+  pl[wv][0][lane] = col[0];
+  pl[wv][1][lane] = col[1];
+  __syncthreads();
+  const TZ zlo = wv > 0 ? pl[wv - 1][1][lane] : below;
+  const TZ zhi = wv < WPB - 1 ? pl[wv + 1][0][lane] : above;
+  double acc = 0.0;
+#pragma unroll
+  for (int j = 0; j < 2; ++j) {
+    const TZ v = col[j];
+    TZ xm = __shfl_up(v, 1), xp = __shfl_down(v, 1), ym = __shfl_up(v, 8), yp = __shfl_down(v, 8);
+    if (x == 0) xm = gx[j];
+    if (x == 7) xp = gx[j];
+    if (y == 0) ym = gy[j];
+    if (y == 7) yp = gy[j];
+    const TZ zm = j ? col[0] : zlo;
+    const TZ zp = j ? zhi : col[1];
+    TZ out = TZ(0);
+    if (cd[j] & CODE_DOM) {
+      TZ dg;
+      const TZ lz = lap_cell<TZ>(cd[j], v, xm, xp, ym, yp, zm, zp, ihx2, ihy2, ihz2, dg);
+      out = dg > TZ(0) ? v + div_diag<TZ>(omega * ((TZ)rv[j] - lz), dg) : TZ(0);
+      if (want_dot) acc += rv[j] * (double)out;
+    }
+    zout[base + 64 * j] = out;
   }
   if (want_dot) write_partial(acc, partials, sh, false);
 }

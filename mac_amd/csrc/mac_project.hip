@@ -990,12 +990,15 @@ int vcycle(mac_ctx* c, TZ** zout) {
     const double* opost = l ? c->omc_post : c->om_post;
     const TZ ih[3] = {(TZ)lv.ih2[0], (TZ)lv.ih2[1], (TZ)lv.ih2[2]};
     int k0 = 0;
-    // level 0 only in FP32: the FP64 fused sweep needs 200 registers and runs
-    // slower than the two launches
+    // level 0: the slab form in both precisions (the FP64 column form needs
+    // 200 registers and ran slower than the two launches)
 #ifndef MAC_FUSE_L0
 #define MAC_FUSE_L0 1
 #endif
-    if (c->prm.fuse_coarse && (blk || (MAC_FUSE_L0 && l == 0 && sizeof(TZ) == 4))) {
+#ifndef MAC_PJ_SLAB
+#define MAC_PJ_SLAB 1
+#endif
+    if (c->prm.fuse_coarse && (blk || (MAC_FUSE_L0 && l == 0 && (sizeof(TZ) == 4 || MAC_PJ_SLAB)))) {
       // prolongation fused with the first post-smoothing step: zt = S(z + P zc)
       if (blk) {
         TimedLaunch tl(c, K_C_PROLONG_BLOCK, nd * (3 * sz + 2) + (double)cl.ndom * sz);
@@ -1005,9 +1008,6 @@ int vcycle(mac_ctx* c, TZ** zout) {
       } else {
         const int want_dot = nu == 1 ? 1 : 0;
         TimedLaunch tl(c, K_PROLONG_JACOBI0, nd * (8.0 + 2 * sz + 2) + (double)cl.ndom * sz);
-#ifndef MAC_PJ_SLAB
-#define MAC_PJ_SLAB 1
-#endif
         if (MAC_PJ_SLAB) {
           k_prolong_jacobi_slab<TZ><<<lv.nact, KTHREADS, 0, c->st>>>(lv.nact, lv.pmeta, lv.code, c->r, zfin[l + 1], z, zt, ih[0],
                                                                       ih[1], ih[2], (TZ)opost[0], want_dot, c->partials, c->s);

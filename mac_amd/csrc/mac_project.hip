@@ -6,8 +6,10 @@
 // child and parent tables: a few tens of thousands of entries) and drives the
 // kernels of project_kernels.h on the context's stream.
 #include <hip/hip_runtime.h>
+#include <hip/hip_ext.h>
 
 #include <algorithm>
+#include <tuple>
 #include <type_traits>
 #include <chrono>
 #include <cmath>
@@ -282,7 +284,9 @@ struct TimedLaunch {
   double bytes;
   hipEvent_t a = nullptr, b = nullptr;
   long long myseq;
-  TimedLaunch(mac_ctx* c_, int cls_, double bytes_) : c(c_), cls(cls_), bytes(bytes_), myseq(c_->seq++) {
+  bool ext;  // the launch records a and b itself (launch() below)
+  TimedLaunch(mac_ctx* c_, int cls_, double bytes_, bool ext_ = true)
+      : c(c_), cls(cls_), bytes(bytes_), myseq(c_->seq++), ext(ext_) {
     if (!c->timing) return;
     while (c->evused + 2 > c->evpool.size()) {
       hipEvent_t e;
@@ -291,14 +295,33 @@ struct TimedLaunch {
     }
     a = c->evpool[c->evused++];
     b = c->evpool[c->evused++];
-    hipEventRecord(a, c->st);
+    if (!ext) hipEventRecord(a, c->st);
   }
   ~TimedLaunch() {
     if (!c->timing) return;
-    hipEventRecord(b, c->st);
+    if (!ext) hipEventRecord(b, c->st);
     c->pend.push_back({cls, a, b, bytes, myseq});
   }
 };
+
+// Launches kern on the context's stream.  In timing mode the dispatch itself
+// records the TimedLaunch's events (hipExtLaunchKernel), so they bracket the
+// kernel's own execution, as a kernel-trace profile's timestamps do, without
+// the cost of two event records around the launch.
+template <typename... KArgs, typename... Args>
+void launch(const TimedLaunch& tl, void (*kern)(KArgs...), dim3 grid, dim3 block, Args... args) {
+  static_assert(sizeof...(KArgs) == sizeof...(Args), "kernel argument count");
+  std::tuple<KArgs...> t{static_cast<KArgs>(args)...};
+  void* p[sizeof...(KArgs)];
+  std::apply(
+      [&](auto&... e) {
+        int i = 0;
+        ((p[i++] = (void*)&e), ...);
+      },
+      t);
+  if (tl.a) (void)hipExtLaunchKernel((const void*)kern, grid, block, p, 0, tl.c->st, tl.a, tl.b, 0);
+  else (void)hipLaunchKernel((const void*)kern, grid, block, p, 0, tl.c->st);
+}
 
 // One-block reduction of the partials of the previous kernel plus the CG
 // scalar step (see k_finalize); returns the launch's sequence number.
@@ -311,7 +334,7 @@ void clear_x(mac_ctx* c) {
 }
 
 long long finalize(mac_ctx* c, int nparts, int op, bool to_host = false) {
-  TimedLaunch tl(c, K_FIN, 0.0);
+  TimedLaunch tl(c, K_FIN, 0.0, false);
   k_finalize<<<1, 1024, 0, c->st>>>(nparts, c->partials, c->s, op, c->prm.rtol * c->prm.rtol, to_host ? c->hs : nullptr);
   return tl.myseq;
 }
@@ -827,10 +850,10 @@ void sweep(mac_ctx* c, int l, const TZ* zin, TZ* zout, int want_dot, double omeg
   // not counted)
   TimedLaunch tl(c, l == 0 ? K_JACOBI_0 : K_C_JACOBI, nd * (sr + (zin ? 2 : 1) * sz + 1));
   if (l == 0)
-    k_jacobi<double, TZ><<<grid_of(lv.nact), KTHREADS, 0, c->st>>>(lv.nact, lv.meta, lv.code, c->r, zin, zout, ih[0], ih[1],
+    launch(tl, k_jacobi<double, TZ>, grid_of(lv.nact), KTHREADS, lv.nact, lv.meta, lv.code, c->r, zin, zout, ih[0], ih[1],
                                                                    ih[2], om, want_dot, c->partials, c->s);
   else
-    k_jacobi_coef<TZ><<<lv.nact, KTHREADS, 0, c->st>>>(lv.nact, lv.meta, lv.code, (const TZ*)lv.r, zin, zout, lv.cf<TZ>(),
+    launch(tl, k_jacobi_coef<TZ>, lv.nact, KTHREADS, lv.nact, lv.meta, lv.code, (const TZ*)lv.r, zin, zout, lv.cf<TZ>(),
                                                        lv.ncell(), lv.ihc<TZ>(), om, c->s);
   }
   // coarse sweeps never take the dot product (the bottom level is level 0 then)
@@ -879,7 +902,7 @@ int vcycle(mac_ctx* c, TZ** zout) {
       // the first step from zero ran inside the restriction into this level
       for (int k = first_done[l] ? 1 : 0; k < (deep ? 1 : nu); ++k) {
         TimedLaunch tl(c, K_C_JACOBI, nd * ((k ? 3 : 2) * sz + 1));  // r, zin, zout, code
-        k_jacobi_block<TZ, TZ><<<lv.nact, KTHREADS, 0, c->st>>>(lv.nact, lv.meta, lv.code, (const TZ*)lv.r, k ? z : nullptr,
+        launch(tl, k_jacobi_block<TZ, TZ>, lv.nact, KTHREADS, lv.nact, lv.meta, lv.code, (const TZ*)lv.r, k ? z : nullptr,
                                                                  k ? zt : z, lv.cf<TZ>(), lv.ncell(), lv.ihc<TZ>(),
                                                                  deep ? c->blockw_deep : c->blockw, c->s);
         if (k) std::swap(z, zt);
@@ -888,7 +911,7 @@ int vcycle(mac_ctx* c, TZ** zout) {
       // the first two sweeps from zero in one pass
       if (l == 0) {
         TimedLaunch tl(c, K_JACOBI2_0, nd * (sr + sz + 1));
-        k_jacobi2<double, TZ><<<grid_of(lv.nact), KTHREADS, 0, c->st>>>(lv.nact, lv.meta, lv.code, c->r, z, ih[0], ih[1],
+        launch(tl, k_jacobi2<double, TZ>, grid_of(lv.nact), KTHREADS, lv.nact, lv.meta, lv.code, c->r, z, ih[0], ih[1],
                                                                          ih[2], (TZ)opre[0], (TZ)opre[1], c->s);
       } else {
         // a coarse level: the two sweeps as two launches (zt from zero, then z)
@@ -906,10 +929,10 @@ int vcycle(mac_ctx* c, TZ** zout) {
     {
       TimedLaunch tl(c, l == 0 ? K_RESID0 : K_C_RESID, nd * (sr + 2 * sz + 1));
       if (l == 0)
-        k_residual<double, TZ><<<grid_of(lv.nact), KTHREADS, 0, c->st>>>(lv.nact, lv.meta, lv.code, c->r, z, res, ih[0], ih[1],
+        launch(tl, k_residual<double, TZ>, grid_of(lv.nact), KTHREADS, lv.nact, lv.meta, lv.code, c->r, z, res, ih[0], ih[1],
                                                                          ih[2], c->s);
       else
-        k_residual_coef<TZ><<<lv.nact, KTHREADS, 0, c->st>>>(lv.nact, lv.meta, lv.code, (const TZ*)lv.r, z, res, lv.cf<TZ>(),
+        launch(tl, k_residual_coef<TZ>, lv.nact, KTHREADS, lv.nact, lv.meta, lv.code, (const TZ*)lv.r, z, res, lv.cf<TZ>(),
                                                              lv.ncell(), lv.ihc<TZ>(), c->s);
     }
     {
@@ -923,7 +946,7 @@ int vcycle(mac_ctx* c, TZ** zout) {
       {
         TimedLaunch tl(c, fuse ? (l == 0 ? K_RESTRICT_BLOCK1 : K_C_RESTRICT_BLOCK) : (l == 0 ? K_RESTRICT1 : K_C_RESTRICT),
                        nd * sz + cd * sz + (fuse ? cd * (sz + 1) : 0.0));
-        k_restrict_sep<TZ><<<cl.nact, KTHREADS, 0, c->st>>>(cl.rmeta, cl.code, res, (TZ*)cl.r, c->s, zsm, cl.cf<TZ>(), cl.ncell(),
+        launch(tl, k_restrict_sep<TZ>, cl.nact, KTHREADS, cl.rmeta, cl.code, res, (TZ*)cl.r, c->s, zsm, cl.cf<TZ>(), cl.ncell(),
                                                             cl.ihc<TZ>(), bw);
       }
       first_done[l + 1] = fuse;
@@ -945,7 +968,7 @@ int vcycle(mac_ctx* c, TZ** zout) {
     const TZ* cinv = std::is_same<TZ, float>::value ? (const TZ*)c->cinv32 : (const TZ*)c->cinv64;
     {
       TimedLaunch tl(c, K_TAIL, 0.0);
-      k_vcycle_tail<TZ><<<1, TAIL_THREADS, 0, c->st>>>(a, lt.code, lt.cf<TZ>(), lt.ncell(), (const TZ*)lt.r, (TZ*)lt.z, top.code,
+      launch(tl, k_vcycle_tail<TZ>, 1, TAIL_THREADS, a, lt.code, lt.cf<TZ>(), lt.ncell(), (const TZ*)lt.r, (TZ*)lt.z, top.code,
                                                        c->rowof, c->cellof, cinv, c->s);
     }
     zfin[tlev] = (TZ*)lt.z;
@@ -954,7 +977,7 @@ int vcycle(mac_ctx* c, TZ** zout) {
     const int want_dot = (bot == 0) ? 1 : 0;
     if (bot == nl - 1) {
       const TZ* cinv = std::is_same<TZ, float>::value ? (const TZ*)c->cinv32 : (const TZ*)c->cinv64;
-      TimedLaunch* tl = new TimedLaunch(c, K_CSOLVE, 0.0);
+      TimedLaunch* tl = new TimedLaunch(c, K_CSOLVE, 0.0, false);
       if (bot == 0)
         k_coarse_solve<double, TZ><<<(MAC_BC + 63) / 64, 64, 0, c->st>>>(c->nc, c->rowof, c->cellof, cinv, c->r, (TZ*)lb.z,
                                                                           want_dot, c->partials, c->s);
@@ -1002,18 +1025,18 @@ int vcycle(mac_ctx* c, TZ** zout) {
       // prolongation fused with the first post-smoothing step: zt = S(z + P zc)
       if (blk) {
         TimedLaunch tl(c, K_C_PROLONG_BLOCK, nd * (3 * sz + 2) + (double)cl.ndom * sz);
-        k_prolong_block<TZ><<<lv.nact, KTHREADS, 0, c->st>>>(lv.nact, lv.pmeta, lv.code, (const TZ*)lv.r, zfin[l + 1], z, zt,
+        launch(tl, k_prolong_block<TZ>, lv.nact, KTHREADS, lv.nact, lv.pmeta, lv.code, (const TZ*)lv.r, zfin[l + 1], z, zt,
                                                               lv.cf<TZ>(), lv.ncell(), lv.ihc<TZ>(),
                                                               deep ? c->blockw_deep : c->blockw, c->s);
       } else {
         const int want_dot = nu == 1 ? 1 : 0;
         TimedLaunch tl(c, K_PROLONG_JACOBI0, nd * (8.0 + 2 * sz + 2) + (double)cl.ndom * sz);
         if (MAC_PJ_SLAB) {
-          k_prolong_jacobi_slab<TZ><<<lv.nact, KTHREADS, 0, c->st>>>(lv.nact, lv.pmeta, lv.code, c->r, zfin[l + 1], z, zt, ih[0],
+          launch(tl, k_prolong_jacobi_slab<TZ>, lv.nact, KTHREADS, lv.nact, lv.pmeta, lv.code, c->r, zfin[l + 1], z, zt, ih[0],
                                                                       ih[1], ih[2], (TZ)opost[0], want_dot, c->partials, c->s);
           if (want_dot) finalize(c, lv.nact, OP_RZ);
         } else {
-          k_prolong_jacobi<TZ><<<grid_of(lv.nact), KTHREADS, 0, c->st>>>(lv.nact, lv.pmeta, lv.code, c->r, zfin[l + 1], z, zt,
+          launch(tl, k_prolong_jacobi<TZ>, grid_of(lv.nact), KTHREADS, lv.nact, lv.pmeta, lv.code, c->r, zfin[l + 1], z, zt,
                                                                           ih[0], ih[1], ih[2], (TZ)opost[0], want_dot, c->partials,
                                                                           c->s);
           if (want_dot) finalize(c, grid_of(lv.nact), OP_RZ);
@@ -1023,12 +1046,12 @@ int vcycle(mac_ctx* c, TZ** zout) {
       k0 = 1;
     } else {
       TimedLaunch tl(c, l == 0 ? K_PROLONG0 : K_C_PROLONG, nd * (2 * sz + 1) + (double)cl.ndom * sz);
-      k_prolong_col<TZ><<<grid_of(lv.nact), KTHREADS, 0, c->st>>>(lv.nact, lv.pmeta, lv.code, zfin[l + 1], z, c->s);
+      launch(tl, k_prolong_col<TZ>, grid_of(lv.nact), KTHREADS, lv.nact, lv.pmeta, lv.code, zfin[l + 1], z, c->s);
     }
     for (int k = k0; k < nu; ++k) {
       if (blk) {
         TimedLaunch tl(c, K_C_JACOBI, nd * (3 * sz + 1));
-        k_jacobi_block<TZ, TZ><<<lv.nact, KTHREADS, 0, c->st>>>(lv.nact, lv.meta, lv.code, (const TZ*)lv.r, z, zt, lv.cf<TZ>(),
+        launch(tl, k_jacobi_block<TZ, TZ>, lv.nact, KTHREADS, lv.nact, lv.meta, lv.code, (const TZ*)lv.r, z, zt, lv.cf<TZ>(),
                                                                  lv.ncell(), lv.ihc<TZ>(), deep ? c->blockw_deep : c->blockw,
                                                                  c->s);
       } else {
@@ -1070,14 +1093,14 @@ int cg_iteration(mac_ctx* c, int cur, TZ** zout, long long* last) {
   const double sd = sizeof(TD);
   {
     TimedLaunch tl(c, K_PMATVEC, nd * (9 + sizeof(TZ) + 2 * sd));  // z, d_old, code; d_new, q
-    k_pmatvec<TZ, TD><<<grid_of(l0.nact), KTHREADS, 0, c->st>>>(l0.nact, l0.meta, l0.code, *zout, (const TD*)c->d[cur],
+    launch(tl, k_pmatvec<TZ, TD>, grid_of(l0.nact), KTHREADS, l0.nact, l0.meta, l0.code, *zout, (const TD*)c->d[cur],
                                                                 (TD*)c->d[cur ^ 1], c->q, ih[0], ih[1], ih[2], c->partials,
                                                                 c->s);
   }
   finalize(c, grid_of(l0.nact), OP_DQ);
   {
     TimedLaunch tl(c, K_UPDATE, nd * (41 + sd));  // code, x, r, d, q; x, r
-    k_update<TD><<<grid_of(l0.nact), KTHREADS, 0, c->st>>>(l0.nact, l0.meta, l0.code, c->x, c->r, (const TD*)c->d[cur ^ 1], c->q,
+    launch(tl, k_update<TD>, grid_of(l0.nact), KTHREADS, l0.nact, l0.meta, l0.code, c->x, c->r, (const TD*)c->d[cur ^ 1], c->q,
                                                            c->partials, c->s);
   }
   *last = finalize(c, grid_of(l0.nact), OP_RR, true);
@@ -1124,7 +1147,7 @@ void true_residual(mac_ctx* c) {
   Level& l0 = c->L[0];
   {
     TimedLaunch tl(c, K_INIT, (double)l0.ndom * 17);
-    k_resid0<<<grid_of(l0.nact), KTHREADS, 0, c->st>>>(l0.nact, l0.meta, l0.code, c->f, c->x, c->r, l0.ih2[0], l0.ih2[1],
+    launch(tl, k_resid0, grid_of(l0.nact), KTHREADS, l0.nact, l0.meta, l0.code, c->f, c->x, c->r, l0.ih2[0], l0.ih2[1],
                                                        l0.ih2[2], 0, c->partials);
   }
   finalize(c, grid_of(l0.nact), OP_TRUE, true);
@@ -1168,7 +1191,7 @@ int pcg(mac_ctx* c, bool use_guess, bool r_ready, mac_stats* st) {
     } else {
       {
         TimedLaunch tl(c, K_INIT, nd * 25);
-        k_resid0<<<grid_of(l0.nact), KTHREADS, 0, c->st>>>(l0.nact, l0.meta, l0.code, c->f, c->x, c->r, ih[0], ih[1], ih[2], 1,
+        launch(tl, k_resid0, grid_of(l0.nact), KTHREADS, l0.nact, l0.meta, l0.code, c->f, c->x, c->r, ih[0], ih[1], ih[2], 1,
                                                            c->partials);
       }
       finalize(c, grid_of(l0.nact), OP_RR0, true);
@@ -1483,7 +1506,7 @@ int mac_project(mac_ctx* c, int guess, mac_stats* st) {
   const bool r_ready = !visc && !guess;
   if (l0.nact > 0) {
     TimedLaunch tl(c, K_DIV, nd * (r_ready ? 41 : 33));
-    k_div<<<grid_of(l0.nact), KTHREADS, 0, c->st>>>(l0.nact, l0.meta, l0.code, c->u, c->v, c->w, inv_h[0], inv_h[1], inv_h[2],
+    launch(tl, k_div, grid_of(l0.nact), KTHREADS, l0.nact, l0.meta, l0.code, c->u, c->v, c->w, inv_h[0], inv_h[1], inv_h[2],
                                                     c->f, nullptr, r_ready ? c->r : nullptr, c->partials);
   }
   if (l0.nact > 0) finalize(c, grid_of(l0.nact), r_ready ? OP_FF_R0 : OP_FF);
@@ -1514,12 +1537,12 @@ int mac_project(mac_ctx* c, int guess, mac_stats* st) {
   if (visc) k_dirichlet_store<<<c->nb0, KTHREADS, 0, c->st>>>(c->nb0, l0.code, c->pD, c->x);
   if (c->nglist > 0) {
     TimedLaunch tl(c, K_GRAD, nd * 57);
-    k_gradient<<<grid_of(c->nglist), KTHREADS, 0, c->st>>>(c->nglist, c->glist, l0.nbr, c->flags, l0.code, c->x, c->u, c->v,
+    launch(tl, k_gradient, grid_of(c->nglist), KTHREADS, c->nglist, c->glist, l0.nbr, c->flags, l0.code, c->x, c->u, c->v,
                                                             c->w, inv_h[0], inv_h[1], inv_h[2]);
   }
   if (l0.nact > 0) {
     TimedLaunch tl(c, K_MAXDIV, nd * 25);
-    k_maxdiv<<<grid_of(l0.nact), KTHREADS, 0, c->st>>>(l0.nact, l0.meta, l0.code, c->u, c->v, c->w, inv_h[0], inv_h[1], inv_h[2],
+    launch(tl, k_maxdiv, grid_of(l0.nact), KTHREADS, l0.nact, l0.meta, l0.code, c->u, c->v, c->w, inv_h[0], inv_h[1], inv_h[2],
                                                        c->partials);
   }
   if (l0.nact > 0) finalize(c, grid_of(l0.nact), OP_MAX);

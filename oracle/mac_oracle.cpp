@@ -687,6 +687,24 @@ extern "C" {
 // Velocity is not updated (use orc_gradient_update).  out[0] = iterations,
 // out[1] = solve ms (excluding set-up), out[2] = set-up ms, out[3] = true
 // relative residual, out[4] = levels.  Returns 0, or -1 on non-convergence.
+// Test hook: the next orc_mgpcg call copies one level of its hierarchy out
+// after the set-up: dims (n[3], dense over the level's brick bounding box),
+// and, when n0 n1 n2 <= cap, the stencil codes and the coarse operator's
+// NCF = 7 arrays (couplings -x,+x,-y,+y,-z,+z, then the diagonal; zeros on
+// level 0), each of n0 n1 n2 values in x-fastest order.
+static int g_dump_level = -1;
+static int* g_dump_dims = nullptr;
+static uint8_t* g_dump_code = nullptr;
+static double* g_dump_cf = nullptr;
+static long long g_dump_cap = 0;
+void orc_mg_dump_request(int level, int* dims, uint8_t* code, double* cf, long long cap) {
+  g_dump_level = level;
+  g_dump_dims = dims;
+  g_dump_code = code;
+  g_dump_cf = cf;
+  g_dump_cap = cap;
+}
+
 int orc_mgpcg(int nb, const int32_t* keys, const uint8_t* flags, const double* h, const double* u, const double* v,
               const double* w, double* p, double rtol, int maxit, int nu, double omega, int smoother, double cheb_lower,
               int nu_coarse, double cheb_lower_coarse, int coarse_inner, double coarse_damp, int nthreads, double* out) {
@@ -896,6 +914,22 @@ int orc_mgpcg(int nb, const int32_t* keys, const uint8_t* flags, const double* h
             }
           }
     }
+  }
+  if (g_dump_level >= 0) {
+    if (g_dump_level < nl) {
+      const DLevel& dl = mg.L[g_dump_level];
+      for (int i = 0; i < 3; ++i) g_dump_dims[i] = dl.n[i];
+      const long long sz = dl.size();
+      if (sz <= g_dump_cap) {
+        std::copy(dl.code.begin(), dl.code.end(), g_dump_code);
+        for (int k = 0; k < 7; ++k)
+          for (long long c = 0; c < sz; ++c)
+            g_dump_cf[k * sz + c] = dl.coef ? (k < 6 ? dl.cf[k][c] : dl.dg[c]) : 0.0;
+      }
+    } else {
+      g_dump_dims[0] = g_dump_dims[1] = g_dump_dims[2] = 0;
+    }
+    g_dump_level = -1;
   }
   // coarsest dense inverse
   {

@@ -192,6 +192,28 @@ def mgpcg(grid, rtol=1e-8, maxit=500, nu=2, omega=0.9, threads=0, smoother=1, ch
             "levels": int(out[4]), "ok": rc == 0}
 
 
+def mg_level(grid, level):
+    """One level of the MG-PCG hierarchy of orc_mgpcg (a test hook): dims
+    (n0, n1, n2), the stencil codes as [z, y, x] and the coarse operator's 7
+    arrays (couplings -x,+x,-y,+y,-z,+z, diagonal) as [k, z, y, x]; level 0
+    has no coefficient arrays (zeros).  dims is (0, 0, 0) past the top."""
+    lib = load()
+    lib.orc_mg_dump_request.restype = None
+    lib.orc_mg_dump_request.argtypes = [ctypes.c_int, _P, _P, _P, ctypes.c_longlong]
+    dims = np.zeros(3, np.int32)
+    lib.orc_mg_dump_request(int(level), _p(dims), None, None, 0)
+    mgpcg(grid, rtol=1e-2, maxit=1)
+    n = int(np.prod(dims.astype(np.int64)))
+    if n == 0:
+        return tuple(dims), None, None
+    code = np.zeros(n, np.uint8)
+    cf = np.zeros(7 * n)
+    lib.orc_mg_dump_request(int(level), _p(dims), _p(code), _p(cf), n)
+    mgpcg(grid, rtol=1e-2, maxit=1)
+    shape = (int(dims[2]), int(dims[1]), int(dims[0]))
+    return tuple(int(d) for d in dims), code.reshape(shape), cf.reshape((7,) + shape)
+
+
 # ---------------------------------------------------------------- time step --
 def corner_solve(h, signs, nb81):
     """The three-axis corner system (SolveMatrix / CreateMatrix) on a 3x3x3

@@ -485,19 +485,20 @@ __global__ void __launch_bounds__(KTHREADS) k_resid0(int nact, const int* __rest
   FOR_EACH_BRICK(meta, nact, M) {
     const int b = M[0];
     const size_t base = (size_t)b * MAC_BC + lane;
-    double col[8];
+    double col[8], fv[8];
     unsigned cd[8];
 #pragma unroll
     for (int z = 0; z < 8; ++z) {
       col[z] = xv[base + 64 * z];
       cd[z] = code[base + 64 * z];
+      fv[z] = f[base + 64 * z];  // ahead of the per-cell branches
     }
     col_stencil<double>(M, lane, col, [&](int nb, int cell) { return xv[(size_t)nb * MAC_BC + cell]; },
                         [&](int z, double xm, double xp, double ym, double yp, double zm, double zp) {
                           if (cd[z] & CODE_DOM) {
                             double dg;
                             double lx = lap_cell<double>(cd[z], col[z], xm, xp, ym, yp, zm, zp, ihx2, ihy2, ihz2, dg);
-                            double rv = f[base + 64 * z] - lx;
+                            double rv = fv[z] - lx;
                             if (write_r) r[base + 64 * z] = rv;
                             acc += rv * rv;
                           }
@@ -571,12 +572,25 @@ __global__ void __launch_bounds__(KTHREADS) k_update(int nact, const int* __rest
   FOR_EACH_BRICK(meta, nact, M) {
     const int b = M[0];
     const size_t base = (size_t)b * MAC_BC + lane;
+    // the column's loads issued together, ahead of the per-cell branches
+    unsigned cd[8];
+    double xv[8], rv0[8], qv[8];
+    TD dv[8];
 #pragma unroll
     for (int z = 0; z < 8; ++z) {
       const size_t c = base + 64 * z;
-      if (code[c] & CODE_DOM) {
-        x[c] += alpha * (double)d[c];
-        double rv = r[c] - alpha * q[c];
+      cd[z] = code[c];
+      xv[z] = x[c];
+      rv0[z] = r[c];
+      qv[z] = q[c];
+      dv[z] = d[c];
+    }
+#pragma unroll
+    for (int z = 0; z < 8; ++z) {
+      const size_t c = base + 64 * z;
+      if (cd[z] & CODE_DOM) {
+        x[c] = xv[z] + alpha * (double)dv[z];
+        double rv = rv0[z] - alpha * qv[z];
         r[c] = rv;
         acc += rv * rv;
       }
@@ -1699,17 +1713,19 @@ __global__ void __launch_bounds__(KTHREADS) k_residual(int nact, const int* __re
     const size_t base = (size_t)b * MAC_BC + lane;
     unsigned cd[8];
     TZ col[8];
+    TR rr[8];
 #pragma unroll
     for (int z = 0; z < 8; ++z) {
       cd[z] = code[base + 64 * z];
       col[z] = zv[base + 64 * z];
+      rr[z] = r[base + 64 * z];
     }
     col_stencil<TZ>(M, lane, col, [&](int nb, int cell) { return zv[(size_t)nb * MAC_BC + cell]; },
                     [&](int z, TZ xm, TZ xp, TZ ym, TZ yp, TZ zm, TZ zp) {
                       TZ out = TZ(0);
                       if (cd[z] & CODE_DOM) {
                         TZ dg;
-                        out = (TZ)r[base + 64 * z] - lap_cell<TZ>(cd[z], col[z], xm, xp, ym, yp, zm, zp, ihx2, ihy2, ihz2, dg);
+                        out = (TZ)rr[z] - lap_cell<TZ>(cd[z], col[z], xm, xp, ym, yp, zm, zp, ihx2, ihy2, ihz2, dg);
                       }
                       res[base + 64 * z] = out;
                     });

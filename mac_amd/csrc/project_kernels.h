@@ -976,6 +976,7 @@ __global__ void __launch_bounds__(KTHREADS) k_residual_coef(int nact, const int*
   const int lane = lane_id(), wv = wave_in_block();
   const int* M = meta + (size_t)MSTRIDE * i;
   const size_t base = (size_t)M[0] * MAC_BC + lane + 128 * wv;
+  const TZ rv[2] = {r[base], r[base + 64]};
   SlabCells<TZ> sc;
   sc.load(M, lane, wv, code, zv);
 #pragma unroll
@@ -983,7 +984,7 @@ __global__ void __launch_bounds__(KTHREADS) k_residual_coef(int nact, const int*
     TZ out = TZ(0);
     if (sc.cd[j] & CODE_DOM) {
       const Coef<TZ> K = coef_at<TZ>(sc.cd[j], cf, ncell, base + 64 * j, ih);
-      out = r[base + 64 * j] - lap_coef<TZ>(K, sc.v[j], sc.xm[j], sc.xp[j], sc.ym[j], sc.yp[j], sc.zm[j], sc.zp[j]);
+      out = rv[j] - lap_coef<TZ>(K, sc.v[j], sc.xm[j], sc.xp[j], sc.ym[j], sc.yp[j], sc.zm[j], sc.zp[j]);
     }
     res[base + 64 * j] = out;
   }
@@ -1054,9 +1055,10 @@ __global__ void __launch_bounds__(KTHREADS) k_jacobi2(int nact, const int* __res
         M, lane, z1,
         [&](int nb, int cell) {
           const size_t j = (size_t)nb * MAC_BC + cell;
-          unsigned c2 = code[j];
+          const unsigned c2 = code[j];
+          const TZ rj = (TZ)r[j];  // loaded with the code, not behind its test
           TZ d2 = diag_of<TZ>(c2, ihx2, ihy2, ihz2);
-          return ((c2 & CODE_DOM) && d2 > TZ(0)) ? div_diag<TZ>(omega * (TZ)r[j], d2) : TZ(0);
+          return ((c2 & CODE_DOM) && d2 > TZ(0)) ? div_diag<TZ>(omega * rj, d2) : TZ(0);
         },
         [&](int z, TZ xm, TZ xp, TZ ym, TZ yp, TZ zm, TZ zp) {
           TZ out = TZ(0);

@@ -53,14 +53,14 @@ KERNEL_NAMES = {
 # block smoother: two launches per iteration on each of levels 1 and 2; the
 # residual and the fused prolongation: one per level 1-3; the fused
 # restriction: one into each of levels 2 and 3).
-MEASURED_TRAFFIC = {("jacobi_l0", 256, 0): 115.57e6, ("cg_pupdate_matvec", 256, 0): 131.50e6, ("cg_update", 256, 0): 232.84e6,
-                    ("jacobi2_l0", 256, 0): 91.90e6, ("residual_l0", 256, 0): 107.60e6,
+MEASURED_TRAFFIC = {("jacobi_l0", 256, 0): 115.57e6, ("cg_pupdate_matvec", 256, 0): 131.50e6, ("cg_update", 256, 0): 255.33e6,
+                    ("jacobi2_l0", 256, 0): 91.90e6, ("residual_l0", 256, 0): 115.45e6,
                     ("prolong_jacobi_l0", 256, 0): 125.56e6, ("restrict_block_l1", 256, 0): 51.18e6,
-                    ("coarse_jacobi", 256, 0): 10.35e6, ("coarse_residual", 256, 0): 6.75e6,
+                    ("coarse_jacobi", 256, 0): 10.35e6, ("coarse_residual", 256, 0): 7.11e6,
                     ("coarse_restrict_block", 256, 0): 5.83e6, ("coarse_prolong_block", 256, 0): 7.92e6,
                     ("coarse_restrict", 256, 0): 0.38e6, ("coarse_tail", 256, 0): 0.10e6,
-                    ("divergence", 256, 0): 267.27e6, ("gradient_update", 256, 0): 342.74e6, ("max_div", 256, 0): 166.45e6,
-                    ("resid_init", 256, 0): 115.21e6}
+                    ("divergence", 256, 0): 267.27e6, ("gradient_update", 256, 0): 342.73e6, ("max_div", 256, 0): 166.44e6,
+                    ("resid_init", 256, 0): 123.54e6}
 
 
 def log(*a):

@@ -149,6 +149,37 @@ def small_spheres(n, seed):
     return W.sphere_box(n, seed, nspheres=6, rmin=3.0, rmax=10.0, slab=6)
 
 
+def deep_pool():
+    """The breakup jet's layout at 128x64x64: a pool 45 cells deep on a const
+    floor, the falling column above it, N(0,1) face velocities on the fluid
+    cells (seeded)."""
+    from mac_amd import jets as J
+    b = J.breakup_jet(nx=128, ny=64, nz=64, pool_depth=45, margin=4, nozzle=8, ppc=1, seed=0)
+    g = {"keys": b["keys"], "flags": b["flags"], "h": (0.01, 0.01, 0.01)}
+    fluid = dom_mask(g)
+    rng = np.random.default_rng(0)
+    for k in "uvw":
+        a = np.zeros(g["flags"].size)
+        a[fluid] = rng.standard_normal(int(fluid.sum()))
+        g[k] = a
+    return g
+
+
+@pytest.mark.parametrize("precision", [MAC_FP64, MAC_MIXED])
+def test_deep_pool_multigrid_size_independent(precision):
+    """On the deep pool the device MG-PCG takes the oracle's iterations, 10
+    with the Galerkin coarse operator (coarse operators rediscretised from the
+    coarsened labels take 29 there), and reaches the tolerance."""
+    g = deep_pool()
+    cpu = O.mgpcg(g, rtol=1e-8)
+    pr = make(g, rtol=1e-8, precision=precision)
+    st = pr.Divergence_Projection()
+    assert st["true_rel_resid"] <= 1e-8 * 1.0001
+    assert abs(st["iters"] - cpu["iters"]) <= 1, (st["iters"], cpu["iters"])
+    assert st["iters"] <= 12, st["iters"]
+    assert rel(pr.get_pressure(), cpu["p"]) < 1e-6
+
+
 @pytest.mark.parametrize("seed", [0, 1])
 def test_sphere_box_48_fp64_vs_cpu_mg(seed):
     g = small_spheres(48, seed)
